@@ -1,0 +1,1762 @@
+// render.hip — wavefront path tracer for gfx950 and the C ABI (include/rt3d.h).
+//
+// One camera sample = one path slot for the lifetime of a chunk.  A chunk is
+// K whole sample passes over the image (path p = s_local * W*H + pixel), so
+// the per-pixel film sum is taken in sample order exactly like the
+// reference's `cumulativeColor += sample` (Renderer.cs:668-672).  Each bounce
+// runs as its own stage:
+//   k_raygen   camera rays, Sobol dims 0-1 (+ lens)      Renderer.cs:630-653
+//   k_trace    closest hit of the active queue          Renderer.cs:1609-1644
+//   k_shade    medium free flight, emission with MIS, NEE light samples
+//              (shadow rays queued), BSDF sample, Russian roulette, next
+//              ray; survivors are compacted into the next queue with a
+//              wave-aggregated atomic                    Renderer.cs:1008-1383, 1700-1782
+//   k_shadow   transparent shadow walk of the queued shadow rays
+//                                                        ShadowRay.cs:59-158
+//   k_finalize per-pixel clamp + film accumulation       Renderer.cs:668-721
+//
+// The recursion's radiance sum L = Le0 + Ld0 + clampInd(att0 · L1) is
+// carried forward with two accumulators per path (primary-vertex terms and
+// the post-primary "indirect" sum that the depth-aware clamp acts on,
+// Renderer.cs:1222-1223), a throughput for roulette/dead-path decisions
+// (the reference's pathThroughput, which excludes Beer-Lambert) and a
+// separate radiance weight (which includes it).
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rt_scene.h"
+#include "shade.h"
+
+namespace rt {
+
+// ═════════════════════════════════════════════════════════════════════════
+// Device-side scene view and render parameters
+// ═════════════════════════════════════════════════════════════════════════
+struct DevScene {
+    const GNode* nodes;
+    const GEntry* entries;
+    const float4* aux;
+    const GMat* mats;
+    const GLight* lights;
+    const float* lut_e;
+    const float* light_cdf;
+    const float* light_pdf;
+    uint32_t top_start, top_count;
+    int num_lights;
+    int medium;
+    f3 cam_o, cam_llc, cam_h, cam_v, cam_u, cam_vv;
+    float lens_radius;
+    f3 sig_t, sig_s;
+};
+
+struct RParams {
+    int W, H, spp, max_depth, npix;
+    float max_rad, ind_max_rad;
+    int power_mis, picked;      // picked: one light per NEE event (power / uniform)
+    int rr_min_b;
+    float rr_min_s;
+    uint32_t seed_xor;
+    int aovs;
+    int maxs;                   // shadow slots per path
+    int base_sample;            // first sample index of the chunk
+    int chunk_samples;
+    int n_paths;
+};
+
+struct PathBufs {
+    float4 *o, *d, *thr, *wt, *w0, *a0, *ai, *ab;
+    float* ior;                 // 8 × N (slot k of path p at k*N + p)
+    int32_t* hit_e;
+    float4* hit_tuv;
+    float4 *av0, *av1, *av2, *av3, *av4;   // AOV probe
+    float4 *sh_o, *sh_d, *sh_c, *sh_r;     // shadow slot records (N × maxs)
+    int32_t *q_a, *q_b, *q_sh;
+    int32_t* counters;          // [0] next queue, [1] shadow queue, [2] stack overflow
+    int n;
+};
+
+struct FilmBufs {
+    float *sum, *sa, *sb, *alba, *albb, *nrma, *nrmb, *depa, *depb;
+    int32_t *dca, *dcb;
+    float *out_beauty, *out_a, *out_b, *out_disp, *out_alba, *out_albb, *out_nrma, *out_nrmb, *out_depa, *out_depb;
+};
+
+// path flag bits (stored as the int bits of d.w)
+enum : int32_t {
+    kPfPrevDelta = 1,
+    kPfCatShift = 1,           // 2 bits: 0 camera, 1 diffuse, 2 glossy, 3 transmission
+    kPfTgtInd = 8,             // contributions go to the indirect accumulator
+    kPfPendShift = 8,          // 8 bits: pending shadow slots
+    kPfPendInd = 1 << 16,      // pending shadows belong to the indirect accumulator
+};
+// AOV probe flag bits (av0.w)
+enum : int32_t { kAovPending = 1, kAovHaveDepth = 2 };
+
+constexpr int kBlock = 64;
+
+__device__ __forceinline__ int pf_cat(int32_t f) { return (f >> kPfCatShift) & 3; }
+__device__ __forceinline__ int pf_pend(int32_t f) { return (f >> kPfPendShift) & 0xff; }
+
+// XCD-aware block order: gfx950 dispatches consecutive workgroups round-robin
+// over its 8 XCDs; renumber so each XCD walks one contiguous range of the
+// queue (neighbouring rays share BVH nodes in that XCD's L2).
+__device__ __forceinline__ uint32_t xcd_block() {
+    uint32_t b = blockIdx.x, nb = gridDim.x;
+    uint32_t q = nb / 8u, r = nb % 8u, x = b % 8u;
+    return x * q + min(x, r) + b / 8u;
+}
+
+// Wave-aggregated queue allocation: returns the first slot for this lane's
+// `n` items (one atomic per wave).
+__device__ __forceinline__ int wave_alloc(int32_t* counter, int n) {
+    int lane = __lane_id();
+    int incl = n;
+    for (int off = 1; off < 64; off <<= 1) {
+        int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    int total = __shfl(incl, 63, 64);
+    int base = 0;
+    if (lane == 63 && total > 0) base = atomicAdd(counter, total);
+    base = __shfl(base, 63, 64);
+    return base + incl - n;
+}
+
+// ── surface reconstruction from an entry (primitive Hit bodies) ─────────
+__device__ __forceinline__ Surf make_surf(const DevScene& S, int32_t eidx, f3 o, f3 d, float t, float u, float v) {
+    GEntry e = S.entries[eidx];
+    int32_t tag = tag_of(e.a);
+    int kind = tag & 0xff;
+    Surf s;
+    s.t = t;
+    s.rel_eta = 0.0f;
+    s.p = o + t * d;
+    s.mat = __float_as_int(e.c.w);
+    s.prim = __float_as_int(e.b.w);
+    if (kind == kTri) {
+        f3 e1 = f3of(e.b), e2 = f3of(e.c);
+        s.n = face(d, f3of(e.d), s.front);
+        s.tan = length_sq(e1) > 0.0f ? normalize(e1) : mk(1.0f, 0.0f, 0.0f);
+        s.bitan = length_sq(e2) > 0.0f ? normalize(e2) : mk(0.0f, 0.0f, 1.0f);
+    } else if (kind == kSphere) {
+        f3 c = f3of(e.a);
+        float r = e.b.x;
+        f3 outward = (s.p - c) / r;
+        s.n = face(d, outward, s.front);
+        f3 td = cross(mk(0.0f, 1.0f, 0.0f), outward);
+        if (length_sq(td) < 1e-4f) td = mk(1.0f, 0.0f, 0.0f);
+        s.tan = normalize(td);
+        s.bitan = normalize(cross(outward, s.tan));
+    } else if (kind == kQuad) {
+        s.n = face(d, f3of(e.b), s.front);
+        int ai = __float_as_int(e.d.y);
+        s.tan = f3of(S.aux[4 * ai + 2]);
+        s.bitan = f3of(S.aux[4 * ai + 3]);
+    } else {
+        f3 n = f3of(e.b);
+        s.n = face(d, n, s.front);
+        f3 ua = fabsf(n.y) < 0.99f ? normalize(cross(n, mk(0.0f, 1.0f, 0.0f))) : mk(1.0f, 0.0f, 0.0f);
+        s.tan = ua;
+        s.bitan = cross(n, ua);
+    }
+    (void)u;
+    (void)v;
+    return s;
+}
+
+// ── lights (SphereLight/AreaLight/DirectionalLight/GeometryLight) ───────
+struct LSample {
+    bool skip;      // in shadow / zero contribution / early return
+    bool delta;
+    f3 color;       // unoccluded light colour of the estimator
+    f3 dir, so;     // direction to light, shadow-ray origin
+    float tmax;     // shadow-ray tMax
+    float dist;     // distance reported to the medium transmittance
+};
+
+__device__ __forceinline__ void grid_xi(int si, int sq, Rng& rng, float& xi1, float& xi2) {
+    float inv = 1.0f / (float)sq;
+    int su = si % sq, sv = si / sq;
+    xi1 = ((float)su + rng.next()) * inv;
+    xi2 = ((float)sv + rng.next()) * inv;
+}
+
+// Sphere.SolidAnglePdf (Sphere.cs:206-229) / SphereLight.PdfSolidAngle
+__device__ __forceinline__ float sphere_cone_pdf(f3 c, float r, f3 hp, f3 wi) {
+    f3 to_c = c - hp;
+    float d2 = length_sq(to_c);
+    float r2 = r * r;
+    if (d2 <= r2) return 1.0f / (4.0f * kPi);
+    float ctm = sqrtf(fmaxf(0.0f, 1.0f - r2 / d2));
+    float wl = length(wi);
+    if (wl < kEps) return 0.0f;
+    f3 w = to_c / sqrtf(d2);
+    float ct = dot(wi, w) / wl;
+    if (ct < ctm) return 0.0f;
+    float sa = 2.0f * kPi * (1.0f - ctm);
+    return sa > kEps ? 1.0f / sa : 0.0f;
+}
+
+// ILight.PdfSolidAngle for every supported light
+__device__ float light_pdf(const GLight& L, f3 hp, f3 wi) {
+    int kind = L.l0.x;
+    if (kind == RT_LIGHT_SPHERE) return sphere_cone_pdf(f3of(L.l2), L.l2.w, hp, wi);
+    if (kind == RT_LIGHT_AREA) {
+        f3 n = f3of(L.l5), corner = f3of(L.l2), u = f3of(L.l3), v = f3of(L.l4);
+        float denom = dot(n, wi);
+        if (fabsf(denom) < 1e-7f) return 0.0f;
+        float t = dot(corner - hp, n) / denom;
+        if (t <= kEps) return 0.0f;
+        f3 lc = (hp + t * wi) - corner;
+        float uu = dot(u, u), vv = dot(v, v), uv = dot(u, v);
+        float det = uu * vv - uv * uv;
+        if (det < 1e-12f) return 0.0f;
+        float lu = dot(lc, u), lv = dot(lc, v);
+        float a = (lu * vv - lv * uv) / det, b = (lv * uu - lu * uv) / det;
+        if (a < 0.0f || a > 1.0f || b < 0.0f || b > 1.0f) return 0.0f;
+        float cl = fabsf(denom);
+        if (cl < 1e-6f) return 0.0f;
+        return (t * t) / (L.l2.w * cl);
+    }
+    if (kind == RT_LIGHT_DIRECTIONAL) {
+        if (L.l2.w <= 0.0f) return 0.0f;
+        float solid = L.l3.y;
+        if (solid <= 0.0f) return 0.0f;
+        float wl = length(wi);
+        if (wl < kEps) return 0.0f;
+        float ct = dot(wi / wl, -f3of(L.l2));
+        if (ct < L.l3.x) return 0.0f;
+        return 1.0f / solid;
+    }
+    // geometry light
+    int sub = L.l0.w;
+    if (sub == kGeoSphere) return sphere_cone_pdf(f3of(L.l2), L.l2.w, hp, wi);
+    float t, u, v;
+    bool hit;
+    f3 n = f3of(L.l5);
+    if (sub == kGeoTri) hit = hit_triangle(hp, wi, f3of(L.l2), f3of(L.l3), f3of(L.l4), kEps, kInfinity, t, u, v);
+    else {
+        float denom = dot(n, wi);
+        hit = false;
+        if (fabsf(denom) >= 1e-8f) {
+            f3 q = f3of(L.l2), U = f3of(L.l3), V = f3of(L.l4);
+            float D = dot(n, q);
+            t = (D - dot(n, hp)) / denom;
+            if (!(t < kEps || t > kInfinity)) {
+                f3 nn = cross(U, V);
+                f3 w = nn / dot(nn, nn);
+                f3 pv = (hp + t * wi) - q;
+                float a = dot(w, cross(pv, V)), b = dot(w, cross(U, pv));
+                hit = !(a < 0.0f || a > 1.0f || b < 0.0f || b > 1.0f);
+            }
+        }
+    }
+    if (!hit) return 0.0f;
+    float cl = fabsf(dot(-wi, n));
+    if (cl < 1e-6f) return 0.0f;
+    float area = L.l5.w;
+    if (area < 1e-12f) return 0.0f;
+    return (t * t) / (area * cl);
+}
+
+// IlluminateAndTestStratified without the shadow test (the shadow ray is
+// queued; its transmittance multiplies the colour later).
+__device__ LSample light_sample(const GLight& L, f3 hp, f3 sn, int si, Rng& rng) {
+    LSample o;
+    o.skip = true;
+    o.delta = false;
+    o.color = splat(0.0f);
+    o.dir = mk(0.0f, 1.0f, 0.0f);
+    o.so = hp;
+    o.tmax = 0.0f;
+    o.dist = 0.0f;
+    int kind = L.l0.x, ss = L.l0.y, sq = L.l0.z;
+    if (kind == RT_LIGHT_SPHERE) {
+        f3 c = f3of(L.l2);
+        float r = L.l2.w;
+        f3 to_c = c - hp;
+        float d2 = length_sq(to_c);
+        if (d2 < kEps * kEps) return o;
+        float dist = sqrtf(d2);
+        f3 w = to_c / dist;
+        float s2 = r * r / d2;
+        float ctm = s2 >= 1.0f ? -1.0f : sqrtf(1.0f - s2);
+        float xi1, xi2;
+        grid_xi(si, sq, rng, xi1, xi2);
+        float ct = 1.0f - xi1 * (1.0f - ctm);
+        float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+        float phi = 2.0f * kPi * xi2;
+        f3 dw = local_to_world(mk(st * cosf(phi), st * sinf(phi), ct), w);
+        f3 oc = hp - c;
+        float b = dot(oc, dw);
+        float cc = length_sq(oc) - r * r;
+        float disc = b * b - cc;
+        if (disc < 0.0f) return o;
+        float sd = sqrtf(disc);
+        float t = -b - sd;
+        if (t < kEps) t = -b + sd;
+        if (t < kEps) return o;
+        f3 sp = hp + t * dw;
+        f3 ln = (sp - c) / r;
+        float cl = fmaxf(0.0f, dot(-dw, ln));
+        if (cl <= 0.0f) return o;
+        o.so = offset_origin(hp, sn);
+        o.tmax = length(sp - o.so) - kEps;
+        float sa = 2.0f * kPi * (1.0f - ctm);
+        o.color = f3of(L.l1) * (L.l1.w * sa / (float)ss);
+        o.dir = dw;
+        o.dist = t;
+        o.skip = false;
+        return o;
+    }
+    if (kind == RT_LIGHT_AREA) {
+        float ru, rv;
+        grid_xi(si, sq, rng, ru, rv);
+        f3 sp = (f3of(L.l2) + ru * f3of(L.l3)) + rv * f3of(L.l4);
+        f3 tl = sp - hp;
+        float d2 = length_sq(tl);
+        if (d2 < kEps * kEps) return o;
+        float dist = sqrtf(d2);
+        f3 dl = tl / dist;
+        float cl = fmaxf(0.0f, dot(-dl, f3of(L.l5)));
+        if (cl <= 0.0f) return o;
+        o.so = offset_origin(hp, sn);
+        o.tmax = length(sp - o.so) - kEps;
+        float ad2 = d2;
+        float soft = L.l3.w;
+        if (soft > 0.0f) ad2 = fmaxf(ad2, soft * soft);
+        o.color = f3of(L.l1) * (L.l1.w * L.l2.w * cl / (ad2 * (float)ss));
+        o.dir = dl;
+        o.dist = dist;
+        o.skip = false;
+        return o;
+    }
+    if (kind == RT_LIGHT_DIRECTIONAL) {
+        f3 dir = f3of(L.l2);
+        f3 tl;
+        if (L.l2.w > 0.0f) {
+            float xi1, xi2;
+            grid_xi(si, sq, rng, xi1, xi2);
+            float ct = 1.0f - xi1 * (1.0f - L.l3.x);
+            float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+            float phi = 2.0f * kPi * xi2;
+            tl = local_to_world(mk(st * cosf(phi), st * sinf(phi), ct), -dir);
+        } else {
+            tl = -dir;
+            o.delta = true;
+        }
+        o.so = offset_origin(hp, sn);
+        o.tmax = kInfinity;
+        o.color = f3of(L.l1) * (L.l1.w / (float)ss);
+        o.dir = tl;
+        o.dist = kInfinity;
+        o.skip = false;
+        return o;
+    }
+    // GeometryLight (GeometryLight.cs:133-279)
+    int sub = L.l0.w;
+    f3 em = f3of(L.l1);
+    if (sub == kGeoSphere) {
+        float xi1, xi2;
+        grid_xi(si, sq, rng, xi1, xi2);
+        f3 c = f3of(L.l2);
+        float r = L.l2.w;
+        f3 to_c = c - hp;
+        float d2c = length_sq(to_c);
+        float r2 = r * r;
+        f3 sp, ln;
+        float pdf;
+        if (d2c <= r2) {
+            f3 pu = random_unit_vector(rng);
+            sp = c + pu * r;
+            ln = pu;
+            pdf = 1.0f / (4.0f * kPi);
+        } else {
+            float dd = sqrtf(d2c);
+            f3 w = to_c / dd;
+            float ctm = sqrtf(fmaxf(0.0f, 1.0f - r2 / d2c));
+            float ct = 1.0f - xi1 * (1.0f - ctm);
+            float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+            float phi = 2.0f * kPi * xi2;
+            f3 dw = local_to_world(mk(st * cosf(phi), st * sinf(phi), ct), w);
+            f3 oc = hp - c;
+            float b = dot(oc, dw);
+            float cc = length_sq(oc) - r2;
+            float disc = b * b - cc;
+            float t;
+            if (disc <= 0.0f) t = -b;
+            else {
+                float sd = sqrtf(disc);
+                t = -b - sd;
+                if (t < kEps) t = -b + sd;
+            }
+            sp = hp + t * dw;
+            ln = normalize(sp - c);
+            float sa = 2.0f * kPi * (1.0f - ctm);
+            pdf = sa > kEps ? 1.0f / sa : 0.0f;
+        }
+        if (pdf <= 0.0f) return o;
+        f3 tl = sp - hp;
+        float d2 = length_sq(tl);
+        if (d2 < kEps * kEps) return o;
+        float dist = sqrtf(d2);
+        f3 dl = tl / dist;
+        float cl = fmaxf(0.0f, dot(-dl, ln));
+        if (cl <= 0.0f) return o;
+        o.so = offset_origin(hp, sn);
+        o.tmax = length(sp - o.so) - kEps;
+        o.color = em * (1.0f / (pdf * (float)ss));
+        o.dir = dl;
+        o.dist = dist;
+        o.skip = false;
+        return o;
+    }
+    f3 sp, ln = f3of(L.l5);
+    float area = L.l5.w;
+    {
+        float inv = 1.0f / (float)sq;
+        int su = si % sq, sv = si / sq;
+        float a = ((float)su + rng.next()) * inv;
+        float b = ((float)sv + rng.next()) * inv;
+        if (sub == kGeoTri) {
+            float r1 = sqrtf(a);
+            float uu = 1.0f - r1, vv = b * r1;
+            sp = f3of(L.l2) + uu * f3of(L.l3) + vv * f3of(L.l4);
+        } else {
+            sp = (f3of(L.l2) + a * f3of(L.l3)) + b * f3of(L.l4);
+        }
+    }
+    f3 tl = sp - hp;
+    float d2 = length_sq(tl);
+    if (d2 < kEps * kEps) return o;
+    float dist = sqrtf(d2);
+    f3 dl = tl / dist;
+    float cl = fmaxf(0.0f, dot(-dl, ln));
+    if (cl <= 0.0f) return o;
+    o.so = offset_origin(hp, sn);
+    o.tmax = length(sp - o.so) - kEps;
+    float ad2 = d2;
+    float soft = L.l3.w;
+    if (soft > 0.0f) ad2 = fmaxf(ad2, soft * soft);
+    o.color = em * (area * cl / (ad2 * (float)ss));
+    o.dir = dl;
+    o.dist = dist;
+    o.skip = false;
+    return o;
+}
+
+__device__ __forceinline__ float mis_w(float p, float q, int power) {
+    if (power) {
+        float p2 = p * p, q2 = q * q;
+        return p2 / (p2 + q2 + 1e-30f);
+    }
+    return p / (p + q + 1e-30f);
+}
+__device__ __forceinline__ f3 sanitize(f3 c) {
+    if (isnan(c.x) || isinf(c.x)) c.x = 0.0f;
+    if (isnan(c.y) || isinf(c.y)) c.y = 0.0f;
+    if (isnan(c.z) || isinf(c.z)) c.z = 0.0f;
+    return mk(fmaxf(c.x, 0.0f), fmaxf(c.y, 0.0f), fmaxf(c.z, 0.0f));
+}
+__device__ __forceinline__ f3 clamp_lum(f3 c, float m, bool require_pos) {
+    c = sanitize(c);
+    float l = luminance(c);
+    if (l > m && (!require_pos || l > 0.0f)) c *= m / l;
+    return c;
+}
+__device__ __forceinline__ f3 clamp01(f3 v) {
+    return mk(clampf(v.x, 0.0f, 1.0f), clampf(v.y, 0.0f, 1.0f), clampf(v.z, 0.0f, 1.0f));
+}
+__device__ __forceinline__ f3 medium_tr(const DevScene& S, float tmax) {
+    float d = fminf(fmaxf(tmax, 0.0f), 1e30f);
+    return mk(expf(-S.sig_t.x * d), expf(-S.sig_t.y * d), expf(-S.sig_t.z * d));
+}
+
+// ═════════════════════════════════════════════════════════════════════════
+// Kernels
+// ═════════════════════════════════════════════════════════════════════════
+__device__ __forceinline__ void path_sample(const RParams& R, int p, int& i, int& j, int& s) {
+    int pix = p % R.npix;
+    s = R.base_sample + p / R.npix;
+    i = pix % R.W;
+    j = pix / R.W;
+}
+
+// Camera ray (Renderer.cs:630-653, Camera.cs:127-153)
+__device__ __forceinline__ void camera_ray(const DevScene& S, const RParams& R, int i, int j, Rng& rng, f3& o, f3& d) {
+    float ju = rng.next();
+    float jv = rng.next();
+    float u = ((float)i + ju) / (float)R.W;
+    float v = ((float)(R.H - j - 1) + jv) / (float)R.H;
+    f3 target = (S.cam_llc + u * S.cam_h) + v * S.cam_v;
+    if (S.lens_radius == 0.0f) {
+        o = S.cam_o;
+        d = target - S.cam_o;
+        return;
+    }
+    float a = 2.0f * rng.next() - 1.0f;
+    float b = 2.0f * rng.next() - 1.0f;
+    f3 rd = splat(0.0f);
+    if (!(a == 0.0f && b == 0.0f)) {
+        float r, phi;
+        if (a * a > b * b) { r = a; phi = (kPi / 4.0f) * (b / a); }
+        else { r = b; phi = (kPi / 2.0f) - (kPi / 4.0f) * (a / b); }
+        rd = mk(r * cosf(phi), r * sinf(phi), 0.0f);
+    }
+    rd = S.lens_radius * rd;
+    f3 off = S.cam_u * rd.x + S.cam_vv * rd.y;
+    o = S.cam_o + off;
+    d = target - o;
+}
+
+__global__ void __launch_bounds__(kBlock) k_raygen(DevScene S, RParams R, PathBufs P) {
+    int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= R.n_paths) return;
+    int i, j, s;
+    path_sample(R, p, i, j, s);
+    Rng rng{(uint32_t)s, pixel_seed(i, j, R.seed_xor), 0u};
+    f3 o, d;
+    camera_ray(S, R, i, j, rng, o, d);
+    P.o[p] = make_float4(o.x, o.y, o.z, 0.0f);
+    P.d[p] = make_float4(d.x, d.y, d.z, __int_as_float(kPfPrevDelta));
+    P.thr[p] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(R.max_depth));
+    P.wt[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(rng.dim));
+    P.w0[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    P.a0[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.ai[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.ab[p] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
+    if (R.aovs) {
+        P.av0[p] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(kAovPending));
+        P.av1[p] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
+        P.av2[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.av3[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.av4[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    P.q_a[p] = p;
+}
+
+template <int SD>
+__global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P, const int32_t* __restrict__ q, int count) {
+    __shared__ uint2 stk[SD * kBlock];
+    int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= count) return;
+    int p = q[i];
+    float4 o4 = P.o[p], d4 = P.d[p];
+    f3 o = f3of(o4), d = f3of(d4);
+    bool camera = pf_cat(__float_as_int(d4.w)) == 0;
+    HitOut h;
+    bool moved = false;
+    for (int skip = 0;;) {
+        h = trace_closest<SD, kBlock>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps, kInfinity,
+                                      &stk[threadIdx.x], (uint32_t*)&P.counters[2]);
+        if (h.entry < 0 || !camera) break;
+        int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
+        if (!(fl & kEntryCamInvisible)) break;
+        o = offset_origin(o + h.t * d, d);
+        moved = true;
+        if (++skip >= 8) break;
+    }
+    if (moved) P.o[p] = make_float4(o.x, o.y, o.z, o4.w);
+    P.hit_e[p] = h.entry;
+    P.hit_tuv[p] = make_float4(h.t, h.u, h.v, 0.0f);
+}
+
+__device__ __forceinline__ void add_target(PathBufs& P, int p, bool ind, f3 c) {
+    float4* acc = ind ? P.ai : P.a0;
+    float4 a = acc[p];
+    acc[p] = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w);
+}
+
+// Adds the transmittance-weighted shadow contributions of the path's last
+// vertex to its accumulator, in light/sample order.
+__device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, int p, int32_t flags) {
+    int pend = pf_pend(flags);
+    if (pend == 0) return flags;
+    f3 s = splat(0.0f);
+    for (int k = 0; k < pend; k++) s += f3of(P.sh_r[(int64_t)p * R.maxs + k]);
+    add_target(P, p, (flags & kPfPendInd) != 0, s);
+    return flags & ~(0xff << kPfPendShift) & ~kPfPendInd;
+}
+
+__global__ void __launch_bounds__(kBlock) k_shade(DevScene S, RParams R, PathBufs P, const int32_t* __restrict__ q,
+                                                  int count) {
+    int i = xcd_block() * kBlock + threadIdx.x;
+    bool active = i < count;
+    int p = active ? q[i] : 0;
+    int n_shadow = 0;
+    bool cont = false;
+    // per-lane shadow records are written straight into the path's slots
+    if (active) {
+        float4 o4 = P.o[p], d4 = P.d[p], thr4 = P.thr[p], wt4 = P.wt[p];
+        int32_t flags = __float_as_int(d4.w);
+        flags = fold_shadows(P, R, p, flags);
+        int depth = __float_as_int(thr4.w);
+        int pi, pj, ps;
+        path_sample(R, p, pi, pj, ps);
+        Rng rng{(uint32_t)ps, pixel_seed(pi, pj, R.seed_xor), __float_as_uint(wt4.w)};
+        f3 o = f3of(o4), d = f3of(d4);
+        float prev_pdf = o4.w;
+        bool prev_delta = (flags & kPfPrevDelta) != 0;
+        int cat = pf_cat(flags);
+        bool tgt = (flags & kPfTgtInd) != 0;
+        f3 thr = f3of(thr4), wt = f3of(wt4);
+        float4 ab4 = P.ab[p];
+        f3 sigma_abs = f3of(ab4);
+        int ior_n = __float_as_int(ab4.w);
+        int32_t he = P.hit_e[p];
+        float4 htuv = P.hit_tuv[p];
+        bool hit = he >= 0;
+        bool primary = depth == R.max_depth;
+        int used = R.max_depth - depth;
+
+        // ── medium free flight (HomogeneousMedium.Sample) ──────────────
+        f3 mb = splat(1.0f);
+        bool scat = false;
+        float tmed = 0.0f;
+        if (S.medium) {
+            float tmax = hit ? htuv.x : 1e30f;
+            int ch = (int)(rng.next() * 3.0f);
+            if (ch > 2) ch = 2;
+            float st = ch == 0 ? S.sig_t.x : (ch == 1 ? S.sig_t.y : S.sig_t.z);
+            if (st <= 0.0f) tmed = tmax;
+            else {
+                float dist = -logf(1.0f - rng.next()) / st;
+                scat = dist < tmax;
+                tmed = scat ? dist : tmax;
+                f3 tr = medium_tr(S, tmed);
+                f3 dens = scat ? S.sig_t * tr : tr;
+                float pdf = (dens.x + dens.y + dens.z) / 3.0f;
+                if (pdf < 1e-20f) pdf = 1.0f;
+                mb = scat ? (tr * S.sig_s / pdf) : (tr / pdf);
+            }
+        }
+        // ── Beer-Lambert on the segment (ApplyBeerLambert) ──────────────
+        f3 bl = splat(1.0f);
+        if (sigma_abs.x > 0.0f || sigma_abs.y > 0.0f || sigma_abs.z > 0.0f) {
+            if (!scat && !hit) bl = splat(0.0f);
+            else {
+                float tseg = scat ? tmed : htuv.x;
+                if (tseg > 0.0f)
+                    bl = mk(sigma_abs.x > 0.0f ? expf(-sigma_abs.x * tseg) : 1.0f,
+                            sigma_abs.y > 0.0f ? expf(-sigma_abs.y * tseg) : 1.0f,
+                            sigma_abs.z > 0.0f ? expf(-sigma_abs.z * tseg) : 1.0f);
+            }
+        }
+        f3 seg = bl * mb;
+        if (primary) P.w0[p] = make_float4(seg.x, seg.y, seg.z, 0.0f);
+        else wt = wt * seg;
+
+        int aovf = 0;
+        float4 av0, av1, av2;
+        if (R.aovs) {
+            av0 = P.av0[p];
+            av1 = P.av1[p];
+            av2 = P.av2[p];
+            aovf = __float_as_int(av0.w);
+            if (hit && cat == 0 && (aovf & kAovPending) && !(aovf & kAovHaveDepth)) {
+                av1.w = htuv.x * length(d);
+                aovf |= kAovHaveDepth;
+            }
+        }
+        auto aov_commit = [&](f3 alb, f3 nrm) {
+            f3 aw = f3of(av0);
+            f3 a = aw * alb;
+            P.av3[p] = make_float4(a.x, a.y, a.z, 0.0f);
+            P.av4[p] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+            aovf &= ~kAovPending;
+        };
+
+        const int64_t slot0 = (int64_t)p * R.maxs;
+        f3 next_o = o, next_d = d;
+        float next_pdf = 0.0f;
+        bool next_delta = false;
+        int next_cat = 0;
+        f3 thr_next = thr, wt_next = wt;
+        bool next_tgt = tgt;
+
+        if (scat) {
+            // ── medium scattering event (Renderer.cs:1703-1770) ─────────
+            f3 ps3 = o + d * tmed;
+            if (R.aovs && (aovf & kAovPending)) {
+                if (!(aovf & kAovHaveDepth)) { av1.w = tmed * length(d); aovf |= kAovHaveDepth; }
+                aov_commit(splat(1.0f), splat(0.0f));
+            }
+            f3 seg_w = primary ? splat(1.0f) : wt;
+            int first_l = 0, last_l = S.num_lights;
+            float ppick = -1.0f;
+            if (R.picked && S.num_lights > 0) {
+                float xi = rng.next();
+                int lo = 0, hi = S.num_lights - 1;
+                if (S.num_lights > 1)
+                    while (lo < hi) {
+                        int mid = (lo + hi) / 2;
+                        if (S.light_cdf[mid] < xi) lo = mid + 1;
+                        else hi = mid;
+                    }
+                first_l = lo;
+                last_l = lo + 1;
+                ppick = S.light_pdf[lo];
+            }
+            for (int li = first_l; li < last_l; li++) {
+                GLight L = S.lights[li];
+                for (int si = 0; si < L.l0.y; si++) {
+                    LSample ls = light_sample(L, ps3, splat(0.0f), si, rng);
+                    if (ls.skip) continue;
+                    f3 tr = medium_tr(S, ls.dist);
+                    float wn = 1.0f;
+                    if (!ls.delta) {
+                        float pl = light_pdf(L, ps3, ls.dir);
+                        float pn = ppick > 0.0f ? ppick * pl : pl;
+                        if (pn + kInv4Pi > 0.0f) wn = mis_w(pn, kInv4Pi, R.power_mis);
+                    }
+                    f3 c = wn * ls.color * kInv4Pi * tr;
+                    if (ppick > 0.0f) c = c / ppick;
+                    c = seg_w * c;
+                    if (is_zero(c)) continue;
+                    int64_t sl = slot0 + n_shadow;
+                    P.sh_o[sl] = make_float4(ls.so.x, ls.so.y, ls.so.z, ls.tmax);
+                    P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
+                    P.sh_c[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+                    n_shadow++;
+                }
+            }
+            f3 mt = thr * mb;
+            float iscale = 1.0f;
+            bool kill = false;
+            if (used >= R.rr_min_b) {
+                float sp = fmax3(mt);
+                sp = fmaxf(sp, R.rr_min_s);
+                sp = fminf(sp, 0.95f);
+                if (rng.next() > sp) kill = true;
+                else iscale = 1.0f / sp;
+            }
+            if (!kill) {
+                float u1 = rng.next(), u2 = rng.next();
+                float z = 1.0f - 2.0f * u1;
+                float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+                float phi = 2.0f * kPi * u2;
+                f3 wi = mk(rr * cosf(phi), rr * sinf(phi), z);
+                float pw = kInv4Pi / kInv4Pi;
+                thr_next = mt * (pw * iscale);
+                wt_next = (primary ? splat(1.0f) : wt) * (pw * iscale);
+                next_o = ps3;
+                next_d = wi;
+                next_pdf = kInv4Pi;
+                next_delta = false;
+                next_cat = 0;
+                cont = true;
+            }
+        } else if (!hit) {
+            // ── miss: flat black sky (SampleSky) ───────────────────────
+            if (R.aovs && (aovf & kAovPending)) aov_commit(clamp01(splat(0.0f)), splat(0.0f));
+        } else {
+            // ── surface (ShadeSurface, Renderer.cs:1008-1228) ──────────
+            Surf s = make_surf(S, he, o, d, htuv.x, htuv.y, htuv.z);
+            const GMat g = S.mats[s.mat];
+            int kind = g.m0.x;
+            int mflags = g.m0.y;
+            f3 thr_s = thr * mb;
+            f3 vw = primary ? splat(1.0f) : wt;   // weight of this vertex's own terms
+            if (R.aovs && (aovf & kAovPending)) {
+                P.av2[p] = make_float4(s.n.x, s.n.y, s.n.z, 0.0f);
+                f3 ta = clamp01(f3of(g.m1));
+                av1 = make_float4(ta.x, ta.y, ta.z, av1.w);
+            }
+            // emission with MIS (WeightEmission, Renderer.cs:1549-1565)
+            if (kind == RT_MAT_EMISSIVE && s.front) {
+                f3 raw = f3of(g.m1);
+                if (raw.x > 0.0f || raw.y > 0.0f || raw.z > 0.0f) {
+                    f3 em = raw;
+                    int lix = g.m0.z;
+                    if (!prev_delta && lix >= 0) {
+                        float pl = light_pdf(S.lights[lix], o, d);
+                        if (prev_pdf + pl <= 1e-20f) em = splat(0.0f);
+                        else em = raw * mis_w(prev_pdf, pl, R.power_mis);
+                    }
+                    add_target(P, p, tgt, sanitize(vw * em));
+                }
+            }
+            f3 view = normalize(-d);
+            if (mflags & kMatDielectricIor) {
+                float top = ior_n > 0 ? P.ior[(int64_t)(ior_n - 1) * P.n + p] : 1.0f;
+                float enc = ior_n >= 2 ? P.ior[(int64_t)(ior_n - 2) * P.n + p] : 1.0f;
+                float mi = g.m3.w;
+                if (mi > 0.0f) s.rel_eta = s.front ? (top / mi) : (mi / enc);
+            }
+            DisneyP DP;
+            if (kind == RT_MAT_DISNEY) DP = load_disney(g);
+            // ── NEE (ComputeDirectLighting, Renderer.cs:1894-2036) ──────
+            if (kind != RT_MAT_EMISSIVE) {
+                float view_term = shadow_terminator(s.n, s.n, view);
+                bool has_brdf = kind == RT_MAT_LAMBERTIAN || kind == RT_MAT_DISNEY;
+                int first_l = 0, last_l = S.num_lights;
+                float ppick = -1.0f;
+                if (R.picked && S.num_lights > 0) {
+                    float xi = rng.next();
+                    int lo = 0, hi = S.num_lights - 1;
+                    if (S.num_lights > 1)
+                        while (lo < hi) {
+                            int mid = (lo + hi) / 2;
+                            if (S.light_cdf[mid] < xi) lo = mid + 1;
+                            else hi = mid;
+                        }
+                    first_l = lo;
+                    last_l = lo + 1;
+                    ppick = S.light_pdf[lo];
+                }
+                for (int li = first_l; li < last_l; li++) {
+                    GLight L = S.lights[li];
+                    for (int si = 0; si < L.l0.y; si++) {
+                        LSample ls = light_sample(L, s.p, s.n, si, rng);
+                        if (ls.skip || !has_brdf) continue;
+                        f3 brdf;
+                        if (kind == RT_MAT_LAMBERTIAN) {
+                            float ndl = fmaxf(dot(s.n, ls.dir), 0.0f);
+                            brdf = ndl <= 0.0f ? splat(0.0f) : f3of(g.m1) * (ndl / kPi);
+                        } else {
+                            brdf = disney_eval_direct(DP, S.lut_e, ls.dir, view, s);
+                        }
+                        brdf *= view_term * shadow_terminator(s.n, s.n, ls.dir);
+                        if (is_zero(brdf)) continue;
+                        f3 tr = S.medium ? medium_tr(S, ls.dist) : splat(1.0f);
+                        float wn = 1.0f;
+                        if (!ls.delta) {
+                            float pb;
+                            if (kind == RT_MAT_LAMBERTIAN) {
+                                float c = dot(s.n, ls.dir);
+                                pb = c > 0.0f ? c * (1.0f / kPi) : 0.0f;
+                            } else pb = disney_pdf(DP, view, ls.dir, s);
+                            if (pb > 0.0f) {
+                                float pl = light_pdf(L, s.p, ls.dir);
+                                float pn = ppick > 0.0f ? ppick * pl : pl;
+                                wn = (pn + pb > 0.0f) ? mis_w(pn, pb, R.power_mis) : 1.0f;
+                            }
+                        }
+                        f3 c = wn * ls.color * brdf * tr;
+                        if (ppick > 0.0f) c = c / ppick;
+                        c = vw * c;
+                        if (is_zero(c)) continue;
+                        int64_t sl = slot0 + n_shadow;
+                        P.sh_o[sl] = make_float4(ls.so.x, ls.so.y, ls.so.z, ls.tmax);
+                        P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
+                        P.sh_c[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+                        n_shadow++;
+                    }
+                }
+            }
+            // ── indirect bounce ─────────────────────────────────────────
+            bool have = false, legacy = false, is_delta = false;
+            f3 att = splat(0.0f), wo = splat(0.0f), origin = s.p;
+            float spdf = 0.0f;
+            BsdfS bs;
+            bs.has_next_abs = false;
+            bs.transition = 0;
+            if (kind == RT_MAT_LAMBERTIAN) {
+                f3 alb = f3of(g.m1);
+                f3 dir = s.n + random_unit_vector(rng);
+                if (near_zero(dir)) dir = s.n;
+                f3 w = normalize(dir);
+                float ndw = dot(s.n, w);
+                if (ndw > 0.0f) {
+                    bs.wo = w;
+                    bs.f = alb * (1.0f / kPi);
+                    bs.pdf = ndw * (1.0f / kPi);
+                    bs.delta = false;
+                    have = true;
+                } else {
+                    f3 d2 = s.n + random_unit_vector(rng);
+                    if (near_zero(d2)) d2 = s.n;
+                    wo = d2;
+                    att = alb;
+                    legacy = true;
+                }
+            } else if (kind == RT_MAT_DISNEY) {
+                if (disney_sample(DP, S.lut_e, view, s, rng, bs)) have = true;
+                else {
+                    int lobe;
+                    if (disney_scatter(DP, S.lut_e, s, d, rng, att, wo, lobe)) legacy = true;
+                }
+            } else if (kind == RT_MAT_DIELECTRIC) {
+                // Dielectric.Scatter (Dielectric.cs:75-105)
+                float ior = g.m3.w;
+                float ri = s.rel_eta > 0.0f ? s.rel_eta : (s.front ? (1.0f / ior) : ior);
+                f3 ud = normalize(d);
+                float ct = fminf(dot(-ud, s.n), 1.0f);
+                float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+                bool cannot = ri * st > 1.0f;
+                f3 dir;
+                if (cannot || schlick(ct, ri) > rng.next()) dir = reflect(ud, s.n);
+                else dir = refract(ud, s.n, ri);
+                f3 on = dot(dir, s.n) >= 0.0f ? s.n : -s.n;
+                origin = offset_origin(s.p, on);
+                wo = dir;
+                att = f3of(g.m1);
+                legacy = true;
+                is_delta = true;
+            }
+            if (have) {
+                // ShadeSampleBounce (Renderer.cs:1235-1383)
+                bool ok = true;
+                if (bs.delta) att = bs.f;
+                else {
+                    float a = fabsf(dot(s.n, bs.wo));
+                    if (a <= 0.0f || bs.pdf <= 0.0f) ok = false;
+                    else att = bs.f * a / bs.pdf;
+                }
+                if (ok) {
+                    if (R.aovs && (aovf & kAovPending)) {
+                        if (bs.delta) {
+                            f3 aw = f3of(av0) * clamp01(bs.f);
+                            av0 = make_float4(aw.x, aw.y, aw.z, av0.w);
+                        } else aov_commit(f3of(av1), s.n);
+                    }
+                    f3 nt = thr_s * att;
+                    bool killed = false;
+                    if (used >= R.rr_min_b) {
+                        float sp = fmax3(nt);
+                        sp = fmaxf(sp, R.rr_min_s);
+                        sp = fminf(sp, 0.95f);
+                        if (rng.next() > sp) killed = true;
+                        else {
+                            float inv = 1.0f / sp;
+                            att *= inv;
+                            nt *= inv;
+                        }
+                    }
+                    if (!killed) {
+                        f3 od = dot(bs.wo, s.n) >= 0.0f ? s.n : -s.n;
+                        next_o = offset_origin(s.p, od);
+                        next_d = bs.wo;
+                        next_pdf = bs.delta ? 0.0f : bs.pdf;
+                        next_delta = bs.delta;
+                        next_cat = dot(s.n, bs.wo) < 0.0f ? 3 : (bs.delta ? 2 : 1);
+                        if (bs.has_next_abs) sigma_abs = bs.next_abs;
+                        if (bs.transition == 1) {
+                            float v = (mflags & kMatDielectricIor) ? g.m3.w
+                                                                   : (ior_n > 0 ? P.ior[(int64_t)(ior_n - 1) * P.n + p] : 1.0f);
+                            if (ior_n >= 8) {
+                                for (int k = 0; k < 7; k++) P.ior[(int64_t)k * P.n + p] = P.ior[(int64_t)(k + 1) * P.n + p];
+                                P.ior[(int64_t)7 * P.n + p] = v;
+                            } else {
+                                P.ior[(int64_t)ior_n * P.n + p] = v;
+                                ior_n++;
+                            }
+                        } else if (bs.transition == 2 && ior_n > 0) ior_n--;
+                        thr_next = nt;
+                        wt_next = primary ? att : wt * att;
+                        if (primary) next_tgt = true;
+                        cont = true;
+                    }
+                }
+            } else if (legacy) {
+                // legacy Scatter path (Renderer.cs:1127-1224)
+                if (R.aovs && (aovf & kAovPending)) {
+                    if (is_delta) {
+                        f3 aw = f3of(av0) * clamp01(att);
+                        av0 = make_float4(aw.x, aw.y, aw.z, av0.w);
+                    } else aov_commit(f3of(av1), s.n);
+                }
+                f3 nt = thr_s * att;
+                bool killed = false;
+                if (used >= R.rr_min_b) {
+                    float sp = fmax3(nt);
+                    sp = fmaxf(sp, R.rr_min_s);
+                    sp = fminf(sp, 0.95f);
+                    if (rng.next() > sp) killed = true;
+                    else {
+                        float inv = 1.0f / sp;
+                        att *= inv;
+                        nt *= inv;
+                    }
+                }
+                if (!killed) {
+                    next_o = origin;
+                    next_d = wo;
+                    next_pdf = 0.0f;
+                    next_delta = is_delta;
+                    next_cat = dot(s.n, wo) < 0.0f ? 3 : (is_delta ? 2 : 1);
+                    if ((mflags & kMatDielectricIor) && dot(wo, s.n) < 0.0f) {
+                        if (s.front) {
+                            float v = g.m3.w;
+                            if (ior_n >= 8) {
+                                for (int k = 0; k < 7; k++) P.ior[(int64_t)k * P.n + p] = P.ior[(int64_t)(k + 1) * P.n + p];
+                                P.ior[(int64_t)7 * P.n + p] = v;
+                            } else {
+                                P.ior[(int64_t)ior_n * P.n + p] = v;
+                                ior_n++;
+                            }
+                        } else if (ior_n > 0) ior_n--;
+                    }
+                    thr_next = nt;
+                    wt_next = primary ? att : wt * att;
+                    if (primary) next_tgt = true;
+                    cont = true;
+                }
+            }
+        }
+        // pending shadows of this vertex belong to this vertex's target
+        flags = flags & ~(0xff << kPfPendShift) & ~kPfPendInd;
+        flags |= (n_shadow << kPfPendShift) | (tgt ? kPfPendInd : 0);
+        // TraceRay entry checks of the next segment (Renderer.cs:1589-1603)
+        if (cont) {
+            if (depth - 1 <= 0 || fmax3(thr_next) < 1e-4f) cont = false;
+        }
+        if (cont) {
+            flags &= ~(kPfPrevDelta | (3 << kPfCatShift) | kPfTgtInd);
+            flags |= (next_delta ? kPfPrevDelta : 0) | (next_cat << kPfCatShift) | (next_tgt ? kPfTgtInd : 0);
+            P.o[p] = make_float4(next_o.x, next_o.y, next_o.z, next_pdf);
+            P.d[p] = make_float4(next_d.x, next_d.y, next_d.z, __int_as_float(flags));
+            P.thr[p] = make_float4(thr_next.x, thr_next.y, thr_next.z, __int_as_float(depth - 1));
+            P.wt[p] = make_float4(wt_next.x, wt_next.y, wt_next.z, __uint_as_float(rng.dim));
+            P.ab[p] = make_float4(sigma_abs.x, sigma_abs.y, sigma_abs.z, __int_as_float(ior_n));
+        } else {
+            P.d[p] = make_float4(d.x, d.y, d.z, __int_as_float(flags));
+        }
+        if (R.aovs) {
+            P.av0[p] = make_float4(av0.x, av0.y, av0.z, __int_as_float(aovf));
+            P.av1[p] = av1;
+        }
+    }
+    // compaction: next queue and shadow queue (wave-aggregated)
+    int nb = wave_alloc(&P.counters[0], cont ? 1 : 0);
+    if (cont) P.q_b[nb] = p;
+    int sb = wave_alloc(&P.counters[1], n_shadow);
+    for (int k = 0; k < n_shadow; k++) P.q_sh[sb + k] = (int32_t)((int64_t)p * R.maxs + k);
+}
+
+// Transparent shadow walk (ShadowRay.Transmittance, ShadowRay.cs:59-158)
+template <int SD>
+__global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, const int32_t* __restrict__ q, int count) {
+    __shared__ uint2 stk[SD * kBlock];
+    int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= count) return;
+    int64_t sl = q[i];
+    float4 o4 = P.sh_o[sl], d4 = P.sh_d[sl];
+    f3 org = f3of(o4), dir = f3of(d4);
+    float rem = o4.w;
+    f3 thr = splat(1.0f), sigma = splat(0.0f), entry = splat(0.0f);
+    bool in_med = false;
+    bool done = false;
+    for (int it = 0; it < 8 && !done; it++) {
+        HitOut h = trace_closest<SD, kBlock>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org, dir, kEps, rem,
+                                             &stk[threadIdx.x], (uint32_t*)&P.counters[2]);
+        if (h.entry < 0) { done = true; break; }
+        Surf s = make_surf(S, h.entry, org, dir, h.t, h.u, h.v);
+        const GMat g = S.mats[s.mat];
+        if (in_med) {
+            float dd = length(s.p - entry);
+            thr = mk(thr.x * expf(-sigma.x * dd), thr.y * expf(-sigma.y * dd), thr.z * expf(-sigma.z * dd));
+            in_med = false;
+            if (near_zero(thr)) { thr = splat(0.0f); done = true; break; }
+        }
+        f3 t = mat_shadow_tr(g, dir, s.n);
+        if (near_zero(t)) { thr = splat(0.0f); done = true; break; }
+        thr *= t;
+        if (near_zero(thr)) { thr = splat(0.0f); done = true; break; }
+        if (s.front) {
+            f3 sg = mat_shadow_abs(g);
+            if (!near_zero(sg)) { in_med = true; entry = s.p; sigma = sg; }
+        }
+        f3 on = dot(dir, s.n) >= 0.0f ? s.n : -s.n;
+        org = offset_origin(s.p, on);
+        rem -= h.t + 2.0f * kEps;
+        if (rem <= kEps) { done = true; break; }
+    }
+    if (!done || near_zero(thr)) thr = splat(0.0f);
+    f3 c = f3of(P.sh_c[sl]) * thr;
+    P.sh_r[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+}
+
+// Per-pixel film accumulation in sample order (Renderer.cs:668-672,
+// CaptureAccumulator.Add, Renderer.cs:829-856)
+__global__ void __launch_bounds__(kBlock) k_finalize(RParams R, PathBufs P, FilmBufs F) {
+    int pix = blockIdx.x * kBlock + threadIdx.x;
+    if (pix >= R.npix) return;
+    const int np = R.npix;
+    f3 sum = mk(F.sum[pix], F.sum[np + pix], F.sum[2 * np + pix]);
+    f3 sa = mk(F.sa[pix], F.sa[np + pix], F.sa[2 * np + pix]);
+    f3 sb = mk(F.sb[pix], F.sb[np + pix], F.sb[2 * np + pix]);
+    for (int sl = 0; sl < R.chunk_samples; sl++) {
+        int s = R.base_sample + sl;
+        if (s >= R.spp) break;
+        int p = sl * np + pix;
+        int32_t flags = __float_as_int(P.d[p].w);
+        fold_shadows(P, R, p, flags);
+        f3 a0 = f3of(P.a0[p]), ai = f3of(P.ai[p]), w0 = f3of(P.w0[p]);
+        f3 L = w0 * (sanitize(a0) + clamp_lum(ai, R.ind_max_rad, true));
+        L = clamp_lum(L, R.max_rad, false);
+        sum += L;
+        bool isA = (s & 1) == 0;
+        if (isA) sa += L;
+        else sb += L;
+        if (R.aovs) {
+            float4 av0 = P.av0[p], av1 = P.av1[p];
+            int aovf = __float_as_int(av0.w);
+            f3 alb, nrm;
+            if (aovf & kAovPending) {
+                alb = f3of(av0) * f3of(av1);
+                nrm = f3of(P.av2[p]);
+            } else {
+                alb = f3of(P.av3[p]);
+                nrm = f3of(P.av4[p]);
+            }
+            float* A = isA ? F.alba : F.albb;
+            float* N = isA ? F.nrma : F.nrmb;
+            A[pix] += alb.x; A[np + pix] += alb.y; A[2 * np + pix] += alb.z;
+            N[pix] += nrm.x; N[np + pix] += nrm.y; N[2 * np + pix] += nrm.z;
+            if (aovf & kAovHaveDepth) {
+                if (isA) { F.depa[pix] += av1.w; F.dca[pix]++; }
+                else { F.depb[pix] += av1.w; F.dcb[pix]++; }
+            }
+        }
+    }
+    F.sum[pix] = sum.x; F.sum[np + pix] = sum.y; F.sum[2 * np + pix] = sum.z;
+    F.sa[pix] = sa.x; F.sa[np + pix] = sa.y; F.sa[2 * np + pix] = sa.z;
+    F.sb[pix] = sb.x; F.sb[np + pix] = sb.y; F.sb[2 * np + pix] = sb.z;
+}
+
+// Means, halves and the display transform (CaptureAccumulator.Store,
+// Renderer.cs:862-883; AcesToneMap, Renderer.cs:893-907)
+__global__ void k_resolve(RParams R, FilmBufs F, float exposure) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= R.npix) return;
+    const int np = R.npix;
+    const int nA = (R.spp + 1) / 2, nB = R.spp / 2;
+    for (int c = 0; c < 3; c++) {
+        int k = c * np + pix;
+        float lin = F.sum[k] / (float)R.spp;
+        F.out_beauty[k] = lin;
+        float mA = F.sa[k] / (float)nA;
+        F.out_a[k] = mA;
+        F.out_b[k] = nB > 0 ? F.sb[k] / (float)nB : mA;
+        float x = fmaxf(lin * exposure, 0.0f);
+        float a = x * (2.51f * x + 0.03f);
+        float b = x * (2.43f * x + 0.59f) + 0.14f;
+        F.out_disp[k] = powf(clampf(a / b, 0.0f, 1.0f), 1.0f / 2.2f);
+        if (R.aovs) {
+            float aa = F.alba[k] / (float)nA, na = F.nrma[k] / (float)nA;
+            F.out_alba[k] = aa;
+            F.out_albb[k] = nB > 0 ? F.albb[k] / (float)nB : aa;
+            F.out_nrma[k] = na;
+            F.out_nrmb[k] = nB > 0 ? F.nrmb[k] / (float)nB : na;
+        }
+    }
+    if (R.aovs) {
+        float dA = F.dca[pix] > 0 ? F.depa[pix] / (float)F.dca[pix] : -1.0f;
+        float dB = nB > 0 ? (F.dcb[pix] > 0 ? F.depb[pix] / (float)F.dcb[pix] : -1.0f) : dA;
+        F.out_depa[pix] = dA;
+        F.out_depb[pix] = dB;
+    }
+}
+
+// rt_trace_primary / rt_trace_rays
+template <int SD>
+__global__ void __launch_bounds__(kBlock) k_trace_primary(DevScene S, RParams R, int n, int32_t* prim, float* to,
+                                                          float* uo, float* vo) {
+    __shared__ uint2 stk[SD * kBlock];
+    int p = xcd_block() * kBlock + threadIdx.x;
+    if (p >= n) return;
+    int i, j, s;
+    path_sample(R, p, i, j, s);
+    Rng rng{(uint32_t)s, pixel_seed(i, j, R.seed_xor), 0u};
+    f3 o, d;
+    camera_ray(S, R, i, j, rng, o, d);
+    HitOut h;
+    for (int skip = 0;;) {
+        h = trace_closest<SD, kBlock>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps, kInfinity,
+                                      &stk[threadIdx.x], (uint32_t*)(prim + n));
+        if (h.entry < 0) break;
+        int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
+        if (!(fl & kEntryCamInvisible)) break;
+        o = offset_origin(o + h.t * d, d);
+        if (++skip >= 8) break;
+    }
+    prim[p] = h.entry < 0 ? -1 : __float_as_int(S.entries[h.entry].b.w);
+    to[p] = h.entry < 0 ? 0.0f : h.t;
+    uo[p] = h.entry < 0 ? 0.0f : h.u;
+    vo[p] = h.entry < 0 ? 0.0f : h.v;
+}
+
+template <int SD>
+__global__ void __launch_bounds__(kBlock) k_trace_rays(DevScene S, const float* __restrict__ rays, int n, int32_t* prim,
+                                                       float* to, float* uo, float* vo) {
+    __shared__ uint2 stk[SD * kBlock];
+    int p = xcd_block() * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const float* r = rays + 6 * (int64_t)p;
+    HitOut h = trace_closest<SD, kBlock>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, mk(r[0], r[1], r[2]),
+                                         mk(r[3], r[4], r[5]), kEps, kInfinity, &stk[threadIdx.x],
+                                         (uint32_t*)(prim + n));
+    prim[p] = h.entry < 0 ? -1 : __float_as_int(S.entries[h.entry].b.w);
+    to[p] = h.entry < 0 ? 0.0f : h.t;
+    uo[p] = h.entry < 0 ? 0.0f : h.u;
+    vo[p] = h.entry < 0 ? 0.0f : h.v;
+}
+
+}  // namespace rt
+
+// ═════════════════════════════════════════════════════════════════════════
+// Host side: context, scene upload, render loop, C ABI
+// ═════════════════════════════════════════════════════════════════════════
+using namespace rt;
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // cached path/film buffers (grow-only)
+    void* path_mem = nullptr;
+    size_t path_bytes = 0;
+    void* film_mem = nullptr;
+    size_t film_bytes = 0;
+    hipEvent_t ev[8] = {};
+};
+
+struct rt_scene {
+    rt_ctx* ctx = nullptr;
+    HostScene hs;
+    GNode* d_nodes = nullptr;
+    GEntry* d_entries = nullptr;
+    float4* d_aux = nullptr;
+    GMat* d_mats = nullptr;
+    GLight* d_lights = nullptr;
+    float* d_lut = nullptr;
+    float* d_ldist = nullptr;   // cdf[n], pdf[n]
+    int stack_depth = 0;
+    int64_t device_bytes = 0;
+};
+
+#define HIP_TRY(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            return e_ == hipErrorOutOfMemory ? RT_ERR_OUT_OF_MEMORY : RT_ERR_DEVICE; \
+        }                                                                          \
+    } while (0)
+
+static int fail(rt_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+template <typename T>
+static int upload(rt_ctx* ctx, T** dst, const T* src, size_t n, int64_t& bytes) {
+    size_t b = std::max<size_t>(n, 1) * sizeof(T);
+    HIP_TRY(hipMalloc((void**)dst, b));
+    if (n) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    bytes += (int64_t)b;
+    return RT_OK;
+}
+
+static DevScene dev_view(const rt_scene* sc) {
+    const HostScene& h = sc->hs;
+    DevScene S;
+    S.nodes = sc->d_nodes;
+    S.entries = sc->d_entries;
+    S.aux = sc->d_aux;
+    S.mats = sc->d_mats;
+    S.lights = sc->d_lights;
+    S.lut_e = sc->d_lut;
+    S.light_cdf = sc->d_ldist;
+    S.light_pdf = sc->d_ldist + std::max<size_t>(h.lights.size(), 1);
+    S.top_start = h.top_start;
+    S.top_count = h.top_count;
+    S.num_lights = (int)h.lights.size();
+    S.medium = h.medium ? 1 : 0;
+    S.cam_o = h.cam_origin;
+    S.cam_llc = h.cam_llc;
+    S.cam_h = h.cam_horiz;
+    S.cam_v = h.cam_vert;
+    S.cam_u = h.cam_u;
+    S.cam_vv = h.cam_v;
+    S.lens_radius = h.lens_radius;
+    S.sig_t = h.sig_t;
+    S.sig_s = h.sig_s;
+    return S;
+}
+
+// Launches a traversal kernel with the LDS stack depth the scene needs.
+#define DISPATCH_SD(sd, KERNEL, grid, ...)                                                       \
+    do {                                                                                         \
+        if ((sd) <= 24) hipLaunchKernelGGL(KERNEL<24>, grid, dim3(kBlock), 0, __VA_ARGS__);      \
+        else if ((sd) <= 32) hipLaunchKernelGGL(KERNEL<32>, grid, dim3(kBlock), 0, __VA_ARGS__); \
+        else if ((sd) <= 48) hipLaunchKernelGGL(KERNEL<48>, grid, dim3(kBlock), 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(KERNEL<64>, grid, dim3(kBlock), 0, __VA_ARGS__);                 \
+    } while (0)
+
+static int check_settings(rt_ctx* ctx, const rt_settings* s) {
+    if (!s) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "settings is null");
+    if (s->width <= 0 || s->height <= 0 || s->samples_per_pixel <= 0 || s->max_depth < 0)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "width, height and samples_per_pixel must be positive, max_depth >= 0");
+    if ((int64_t)s->width * s->height > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
+    if (s->sampler != RT_SAMPLER_SOBOL) return fail(ctx, RT_ERR_UNSUPPORTED, "only the Sobol sampler is implemented");
+    if (s->caustics) return fail(ctx, RT_ERR_UNSUPPORTED, "photon caustics are not implemented");
+    if (s->light_sampling < RT_LIGHTS_ALL || s->light_sampling > RT_LIGHTS_UNIFORM)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bad light_sampling");
+    return RT_OK;
+}
+
+extern "C" {
+
+int rt_get_version(void) { return RT3D_ABI_VERSION; }
+
+void rt_material_defaults(rt_material* m, int32_t kind) {
+    if (!m) return;
+    std::memset(m, 0, sizeof(*m));
+    m->kind = kind;
+    m->color[0] = m->color[1] = m->color[2] = 0.8f;
+    m->intensity = 1.0f;
+    m->roughness = 0.5f;
+    m->specular = 0.5f;
+    m->sheen_tint = 0.5f;
+    m->sheen_roughness = 0.3f;
+    m->clearcoat_gloss = 1.0f;
+    m->ior = 1.5f;
+    m->coat_ior = 1.5f;
+    m->thin_film_ior = 1.5f;
+    m->transmission_color[0] = m->transmission_color[1] = m->transmission_color[2] = 1.0f;
+}
+
+void rt_settings_defaults(rt_settings* s) {
+    if (!s) return;
+    std::memset(s, 0, sizeof(*s));
+    s->width = 960;
+    s->height = 540;
+    s->samples_per_pixel = 16;
+    s->max_depth = 8;
+    s->sampler = RT_SAMPLER_SOBOL;
+    s->mis = RT_MIS_BALANCE;
+    s->light_sampling = RT_LIGHTS_ALL;
+    s->max_sample_radiance = 10.0f;
+    s->indirect_clamp_factor = 0.25f;
+    s->exposure_ev = 0.0f;
+    s->rr_min_bounces = -1;
+    s->rr_min_survival = -1.0f;
+}
+
+int rt_create(int32_t device, rt_ctx** out) {
+    if (!out) return RT_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return RT_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return RT_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return RT_ERR_DEVICE;
+    rt_ctx* ctx = new rt_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return RT_ERR_DEVICE;
+    }
+    for (auto& e : ctx->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return RT_ERR_DEVICE;
+        }
+    }
+    *out = ctx;
+    return RT_OK;
+}
+
+void rt_destroy(rt_ctx* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->path_mem) hipFree(ctx->path_mem);
+    if (ctx->film_mem) hipFree(ctx->film_mem);
+    for (auto& e : ctx->ev)
+        if (e) hipEventDestroy(e);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* rt_last_error(const rt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rt_scene_create(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
+    if (!ctx || !out) return RT_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    hipSetDevice(ctx->device);
+    rt_scene* sc = new rt_scene();
+    sc->ctx = ctx;
+    std::string err;
+    int rc = build_host_scene(desc, sc->hs, err);
+    if (rc != RT_OK) {
+        delete sc;
+        return fail(ctx, rc, err);
+    }
+    HostScene& h = sc->hs;
+    // Stack bound: deepest world path + a leaf continuation + the deepest
+    // nested mesh path (build_host_scene records the maximum over all BVHs).
+    sc->stack_depth = h.world_depth + (h.mesh_depth >= 0 ? 1 + h.mesh_depth : 0) + 1;
+    if (sc->stack_depth > 64) {
+        delete sc;
+        return fail(ctx, RT_ERR_UNSUPPORTED, "BVH too deep for the 64-entry traversal stack");
+    }
+    int64_t& b = sc->device_bytes;
+    auto cleanup = [&](int code) {
+        rt_scene_destroy(sc);
+        return code;
+    };
+    if ((rc = upload(ctx, &sc->d_nodes, h.nodes.data(), h.nodes.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_entries, h.entries.data(), h.entries.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_aux, h.quad_aux.data(), h.quad_aux.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_mats, h.mats.data(), h.mats.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_lights, h.lights.data(), h.lights.size(), b))) return cleanup(rc);
+    std::vector<float> lut(h.lut_e, h.lut_e + 32 * 32);
+    if ((rc = upload(ctx, &sc->d_lut, lut.data(), lut.size(), b))) return cleanup(rc);
+    std::vector<float> ld(2 * std::max<size_t>(h.lights.size(), 1), 0.0f);
+    if ((rc = upload(ctx, &sc->d_ldist, ld.data(), ld.size(), b))) return cleanup(rc);
+    *out = sc;
+    return RT_OK;
+}
+
+void rt_scene_destroy(rt_scene* sc) {
+    if (!sc) return;
+    if (sc->ctx) hipSetDevice(sc->ctx->device);
+    hipFree(sc->d_nodes);
+    hipFree(sc->d_entries);
+    hipFree(sc->d_aux);
+    hipFree(sc->d_mats);
+    hipFree(sc->d_lights);
+    hipFree(sc->d_lut);
+    hipFree(sc->d_ldist);
+    delete sc;
+}
+
+int rt_scene_get_info(const rt_scene* sc, rt_scene_info* info) {
+    if (!sc || !info) return RT_ERR_INVALID_ARGUMENT;
+    std::memset(info, 0, sizeof(*info));
+    const HostScene& h = sc->hs;
+    info->num_primitives = h.num_prims;
+    info->num_objects = h.num_objects;
+    info->num_nodes = h.num_internal;
+    info->num_leaves = h.num_leaves;
+    info->max_depth = h.max_depth;
+    info->has_bvh = h.has_bvh;
+    info->rr_min_bounces = h.rr_min_bounces;
+    info->rr_min_survival = h.rr_min_survival;
+    info->scene_radius = h.scene_radius;
+    info->mean_irradiance = h.mean_irradiance;
+    info->device_bytes = sc->device_bytes;
+    return RT_OK;
+}
+
+int rt_trace_primary(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, int32_t sample_begin, int32_t sample_count,
+                     int32_t* prim_id, float* t, float* u, float* v) {
+    if (!ctx || !sc) return RT_ERR_INVALID_ARGUMENT;
+    int rc = check_settings(ctx, s);
+    if (rc) return rc;
+    if (sample_begin < 0 || sample_count <= 0) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bad sample range");
+    hipSetDevice(ctx->device);
+    int64_t npix = (int64_t)s->width * s->height;
+    int64_t n = npix * sample_count;
+    if (n > (1ll << 31) - 2) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many rays");
+    RParams R{};
+    R.W = s->width;
+    R.H = s->height;
+    R.npix = (int)npix;
+    R.base_sample = sample_begin;
+    R.seed_xor = s->seed_xor;
+    DevScene S = dev_view(sc);
+    char* mem = nullptr;
+    size_t bytes = (size_t)n * 16 + 64;
+    HIP_TRY(hipMalloc((void**)&mem, bytes));
+    int32_t* d_prim = (int32_t*)mem;
+    float* d_t = (float*)(mem + 4 * (size_t)n + 16);
+    float* d_u = d_t + n;
+    float* d_v = d_u + n;
+    hipMemsetAsync(mem, 0, bytes, ctx->stream);
+    dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+    DISPATCH_SD(sc->stack_depth, k_trace_primary, grid, ctx->stream, S, R, (int)n, d_prim, d_t, d_u, d_v);
+    hipError_t e = hipGetLastError();
+    int32_t ovf = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&ovf, d_prim + n, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && prim_id) e = hipMemcpyAsync(prim_id, d_prim, 4 * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && t) e = hipMemcpyAsync(t, d_t, 4 * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && u) e = hipMemcpyAsync(u, d_u, 4 * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && v) e = hipMemcpyAsync(v, d_v, 4 * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(mem);
+    if (e != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string("trace_primary: ") + hipGetErrorString(e));
+    if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+    return RT_OK;
+}
+
+int rt_trace_rays(rt_ctx* ctx, const rt_scene* sc, const float* rays, int64_t num_rays, int32_t* prim_id, float* t,
+                  float* u, float* v) {
+    if (!ctx || !sc || (num_rays > 0 && !rays) || num_rays < 0) return RT_ERR_INVALID_ARGUMENT;
+    if (num_rays == 0) return RT_OK;
+    if (num_rays > (1ll << 31) - 2) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many rays");
+    hipSetDevice(ctx->device);
+    int n = (int)num_rays;
+    DevScene S = dev_view(sc);
+    char* mem = nullptr;
+    size_t bytes = (size_t)n * (24 + 16) + 64;
+    HIP_TRY(hipMalloc((void**)&mem, bytes));
+    float* d_rays = (float*)mem;
+    int32_t* d_prim = (int32_t*)(mem + (size_t)n * 24);
+    float* d_t = (float*)(d_prim + n + 4);
+    float* d_u = d_t + n;
+    float* d_v = d_u + n;
+    hipError_t e = hipMemcpyAsync(d_rays, rays, (size_t)n * 24, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_prim, 0, (size_t)(n + 4) * 4, ctx->stream);
+    if (e == hipSuccess) {
+        dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+        DISPATCH_SD(sc->stack_depth, k_trace_rays, grid, ctx->stream, S, d_rays, n, d_prim, d_t, d_u, d_v);
+        e = hipGetLastError();
+    }
+    int32_t ovf = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&ovf, d_prim + n, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && prim_id) e = hipMemcpyAsync(prim_id, d_prim, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && t) e = hipMemcpyAsync(t, d_t, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && u) e = hipMemcpyAsync(u, d_u, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && v) e = hipMemcpyAsync(v, d_v, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(mem);
+    if (e != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string("trace_rays: ") + hipGetErrorString(e));
+    if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+    return RT_OK;
+}
+
+int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_film* film, rt_render_stats* stats) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (!ctx || !sc) return RT_ERR_INVALID_ARGUMENT;
+    int rc = check_settings(ctx, s);
+    if (rc) return rc;
+    hipSetDevice(ctx->device);
+    const HostScene& h = sc->hs;
+    const int W = s->width, H = s->height, spp = s->samples_per_pixel;
+    const int npix = W * H;
+    // wavefront size: whole sample passes over the image
+    int64_t target = s->paths_in_flight > 0 ? s->paths_in_flight : (1 << 21);
+    int K = (int)std::max<int64_t>(1, std::min<int64_t>(spp, target / npix));
+    const int64_t N = (int64_t)K * npix;
+    if (N > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "wavefront too large");
+    const int maxs = std::max(1, s->light_sampling != RT_LIGHTS_ALL ? h.max_light_samples : h.max_shadow_per_vertex);
+    const bool aovs = s->capture_aovs != 0;
+
+    // ── path buffers (grow-only cache in the context) ──────────────────────
+    size_t per_path = 8 * 16 + 8 * 4 + 4 + 16 + (aovs ? 5 * 16 : 0) + 4 * 16 * (size_t)maxs + 3 * 4 + 4 * (size_t)maxs;
+    size_t need = (size_t)N * per_path + 32 * 256 + 16384;
+    if (need > ctx->path_bytes) {
+        if (ctx->path_mem) hipFree(ctx->path_mem);
+        ctx->path_mem = nullptr;
+        ctx->path_bytes = 0;
+        HIP_TRY(hipMalloc(&ctx->path_mem, need));
+        ctx->path_bytes = need;
+    }
+    char* m = (char*)ctx->path_mem;
+    auto take = [&](size_t bytes) {
+        char* r = m;
+        m += (bytes + 255) & ~(size_t)255;
+        return (void*)r;
+    };
+    PathBufs P{};
+    P.n = (int)N;
+    P.counters = (int32_t*)take(64);
+    P.o = (float4*)take(16 * N);
+    P.d = (float4*)take(16 * N);
+    P.thr = (float4*)take(16 * N);
+    P.wt = (float4*)take(16 * N);
+    P.w0 = (float4*)take(16 * N);
+    P.a0 = (float4*)take(16 * N);
+    P.ai = (float4*)take(16 * N);
+    P.ab = (float4*)take(16 * N);
+    P.ior = (float*)take(4 * 8 * N);
+    P.hit_e = (int32_t*)take(4 * N);
+    P.hit_tuv = (float4*)take(16 * N);
+    if (aovs) {
+        P.av0 = (float4*)take(16 * N);
+        P.av1 = (float4*)take(16 * N);
+        P.av2 = (float4*)take(16 * N);
+        P.av3 = (float4*)take(16 * N);
+        P.av4 = (float4*)take(16 * N);
+    }
+    P.sh_o = (float4*)take(16 * N * maxs);
+    P.sh_d = (float4*)take(16 * N * maxs);
+    P.sh_c = (float4*)take(16 * N * maxs);
+    P.sh_r = (float4*)take(16 * N * maxs);
+    P.q_a = (int32_t*)take(4 * N);
+    P.q_b = (int32_t*)take(4 * N);
+    P.q_sh = (int32_t*)take(4 * N * maxs);
+    if ((size_t)(m - (char*)ctx->path_mem) > ctx->path_bytes) return fail(ctx, RT_ERR_INTERNAL, "path buffer layout");
+
+    // ── film accumulators ───────────────────────────────────────────────────
+    // 25 accumulator + 26 output floats per pixel, 21 regions padded to 256 B
+    size_t fneed = (size_t)npix * 4 * 51 + 32 * 256;
+    if (fneed > ctx->film_bytes) {
+        if (ctx->film_mem) hipFree(ctx->film_mem);
+        ctx->film_mem = nullptr;
+        ctx->film_bytes = 0;
+        HIP_TRY(hipMalloc(&ctx->film_mem, fneed));
+        ctx->film_bytes = fneed;
+    }
+    char* fm = (char*)ctx->film_mem;
+    auto ftake = [&](size_t bytes) {
+        char* r = fm;
+        fm += (bytes + 255) & ~(size_t)255;
+        return (void*)r;
+    };
+    FilmBufs F{};
+    F.sum = (float*)ftake(12 * (size_t)npix);
+    F.sa = (float*)ftake(12 * (size_t)npix);
+    F.sb = (float*)ftake(12 * (size_t)npix);
+    F.alba = (float*)ftake(12 * (size_t)npix);
+    F.albb = (float*)ftake(12 * (size_t)npix);
+    F.nrma = (float*)ftake(12 * (size_t)npix);
+    F.nrmb = (float*)ftake(12 * (size_t)npix);
+    F.depa = (float*)ftake(4 * (size_t)npix);
+    F.depb = (float*)ftake(4 * (size_t)npix);
+    F.dca = (int32_t*)ftake(4 * (size_t)npix);
+    F.dcb = (int32_t*)ftake(4 * (size_t)npix);
+    size_t acc_bytes = (size_t)(fm - (char*)ctx->film_mem);
+    F.out_beauty = (float*)ftake(12 * (size_t)npix);
+    F.out_a = (float*)ftake(12 * (size_t)npix);
+    F.out_b = (float*)ftake(12 * (size_t)npix);
+    F.out_disp = (float*)ftake(12 * (size_t)npix);
+    F.out_alba = (float*)ftake(12 * (size_t)npix);
+    F.out_albb = (float*)ftake(12 * (size_t)npix);
+    F.out_nrma = (float*)ftake(12 * (size_t)npix);
+    F.out_nrmb = (float*)ftake(12 * (size_t)npix);
+    F.out_depa = (float*)ftake(4 * (size_t)npix);
+    F.out_depb = (float*)ftake(4 * (size_t)npix);
+    if ((size_t)(fm - (char*)ctx->film_mem) > ctx->film_bytes) return fail(ctx, RT_ERR_INTERNAL, "film buffer layout");
+    HIP_TRY(hipMemsetAsync(ctx->film_mem, 0, acc_bytes, ctx->stream));
+
+    // ── scene view, light distribution ──────────────────────────────────────
+    rt_scene* msc = const_cast<rt_scene*>(sc);
+    int nl = (int)h.lights.size();
+    bool picked = s->light_sampling != RT_LIGHTS_ALL && nl > 0;
+    if (picked) {
+        // LightDistribution (LightDistribution.cs:41-99)
+        std::vector<float> cdf(nl), pdf(nl), pw(nl, 0.0f);
+        float sum = 0.0f;
+        if (s->light_sampling == RT_LIGHTS_POWER)
+            for (int i = 0; i < nl; i++) { pw[i] = std::max(0.0f, h.light_power[i]); sum += pw[i]; }
+        float cum = 0.0f;
+        if (sum > 0.0f) {
+            float inv = 1.0f / sum;
+            for (int i = 0; i < nl; i++) { pdf[i] = pw[i] * inv; cum += pdf[i]; cdf[i] = cum; }
+        } else {
+            float un = 1.0f / (float)nl;
+            for (int i = 0; i < nl; i++) { pdf[i] = un; cum += un; cdf[i] = cum; }
+        }
+        cdf[nl - 1] = 1.0f;
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpy(msc->d_ldist, cdf.data(), 4 * nl, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(msc->d_ldist + nl, pdf.data(), 4 * nl, hipMemcpyHostToDevice));
+    }
+    DevScene S = dev_view(sc);
+    RParams R{};
+    R.W = W;
+    R.H = H;
+    R.spp = spp;
+    R.max_depth = s->max_depth;
+    R.npix = npix;
+    R.max_rad = s->max_sample_radiance;
+    R.ind_max_rad = s->max_sample_radiance * std::max(0.0f, s->indirect_clamp_factor);
+    R.power_mis = s->mis == RT_MIS_POWER;
+    R.picked = picked ? 1 : 0;
+    R.rr_min_b = s->rr_min_bounces >= 0 ? s->rr_min_bounces : h.rr_min_bounces;
+    R.rr_min_s = s->rr_min_survival >= 0.0f ? s->rr_min_survival : h.rr_min_survival;
+    R.seed_xor = s->seed_xor;
+    R.aovs = aovs ? 1 : 0;
+    R.maxs = maxs;
+    R.chunk_samples = K;
+
+    double t_trace = 0, t_shade = 0, t_shadow = 0;
+    int64_t n_closest = 0, n_shadow_rays = 0, launches = 0;
+    hipEvent_t* ev = ctx->ev;
+    HIP_TRY(hipEventRecord(ev[6], ctx->stream));
+    for (int base = 0; base < spp; base += K) {
+        int live_samples = std::min(K, spp - base);
+        R.base_sample = base;
+        R.n_paths = live_samples * npix;
+        HIP_TRY(hipMemsetAsync(P.counters, 0, 64, ctx->stream));
+        hipLaunchKernelGGL(k_raygen, dim3((R.n_paths + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, S, R, P);
+        int count = s->max_depth > 0 ? R.n_paths : 0;
+        int32_t* qa = P.q_a;
+        int32_t* qb = P.q_b;
+        for (int bounce = 0; bounce < s->max_depth && count > 0; bounce++) {
+            HIP_TRY(hipMemsetAsync(P.counters, 0, 8, ctx->stream));
+            dim3 grid((count + kBlock - 1) / kBlock);
+            HIP_TRY(hipEventRecord(ev[0], ctx->stream));
+            DISPATCH_SD(sc->stack_depth, k_trace, grid, ctx->stream, S, P, qa, count);
+            HIP_TRY(hipEventRecord(ev[1], ctx->stream));
+            PathBufs Pb = P;
+            Pb.q_b = qb;
+            hipLaunchKernelGGL(k_shade, grid, dim3(kBlock), 0, ctx->stream, S, R, Pb, qa, count);
+            HIP_TRY(hipEventRecord(ev[2], ctx->stream));
+            int32_t cnt[3];
+            HIP_TRY(hipMemcpyAsync(cnt, P.counters, 12, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            float ms = 0;
+            hipEventElapsedTime(&ms, ev[0], ev[1]);
+            t_trace += ms;
+            hipEventElapsedTime(&ms, ev[1], ev[2]);
+            t_shade += ms;
+            n_closest += count;
+            launches++;
+            if (cnt[2]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+            if (cnt[1] > 0) {
+                HIP_TRY(hipEventRecord(ev[3], ctx->stream));
+                DISPATCH_SD(sc->stack_depth, k_shadow, dim3((cnt[1] + kBlock - 1) / kBlock), ctx->stream, S, P, P.q_sh,
+                            cnt[1]);
+                HIP_TRY(hipEventRecord(ev[4], ctx->stream));
+                HIP_TRY(hipEventSynchronize(ev[4]));
+                hipEventElapsedTime(&ms, ev[3], ev[4]);
+                t_shadow += ms;
+                n_shadow_rays += cnt[1];
+            }
+            std::swap(qa, qb);
+            count = cnt[0];
+        }
+        hipLaunchKernelGGL(k_finalize, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, R, P, F);
+        HIP_TRY(hipGetLastError());
+    }
+    float exposure = powf(2.0f, s->exposure_ev);
+    hipLaunchKernelGGL(k_resolve, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, R, F, exposure);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev[7], ctx->stream));
+    HIP_TRY(hipEventSynchronize(ev[7]));
+    float dev_ms = 0;
+    hipEventElapsedTime(&dev_ms, ev[6], ev[7]);
+    int32_t ovf = 0;
+    HIP_TRY(hipMemcpy(&ovf, P.counters + 2, 4, hipMemcpyDeviceToHost));
+    if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+    // ── film D2H ────────────────────────────────────────────────────────────
+    if (film) {
+        size_t c3 = 12 * (size_t)npix, c1 = 4 * (size_t)npix;
+        struct { void* dst; const void* src; size_t n; } copies[] = {
+            {film->beauty, F.out_beauty, c3}, {film->beauty_a, F.out_a, c3}, {film->beauty_b, F.out_b, c3},
+            {film->display, F.out_disp, c3},
+            {aovs ? film->albedo_a : nullptr, F.out_alba, c3}, {aovs ? film->albedo_b : nullptr, F.out_albb, c3},
+            {aovs ? film->normal_a : nullptr, F.out_nrma, c3}, {aovs ? film->normal_b : nullptr, F.out_nrmb, c3},
+            {aovs ? film->depth_a : nullptr, F.out_depa, c1}, {aovs ? film->depth_b : nullptr, F.out_depb, c1},
+        };
+        for (auto& c : copies)
+            if (c.dst) HIP_TRY(hipMemcpyAsync(c.dst, c.src, c.n, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (film->sample_count)
+            for (int k = 0; k < npix; k++) film->sample_count[k] = spp;
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        stats->device_ms = dev_ms;
+        stats->trace_ms = t_trace;
+        stats->shade_ms = t_shade;
+        stats->shadow_ms = t_shadow;
+        stats->camera_samples = (int64_t)npix * spp;
+        stats->rays_closest = n_closest;
+        stats->rays_shadow = n_shadow_rays;
+        stats->trace_launches = launches;
+    }
+    return RT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// rt_scene.h — host-side scene representation and the builder interface.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../include/rt3d.h"
+#include "rt_internal.h"
+
+namespace rt {
+
+struct Box {
+    f3 mn, mx;
+};
+
+// Host copy of everything rt_scene_create uploads.
+struct HostScene {
+    std::vector<GNode> nodes;
+    std::vector<GEntry> entries;
+    std::vector<float4> quad_aux;      // 4 per quad: (U, area) (V, 0) (tangent, 0) (bitangent, 0)
+    std::vector<GMat> mats;
+    std::vector<GLight> lights;
+    std::vector<int32_t> light_prim;   // geometry light → primitive id (diagnostics)
+    uint32_t top_start = 0, top_count = 0;   // top-level HittableList entries
+    bool has_bvh = false;
+    int max_depth = 0;                 // deepest root-to-leaf path over all BVHs
+    int world_depth = 0;               // deepest leaf of the world BVH
+    int mesh_depth = -1;               // deepest leaf over nested mesh BVHs (-1: none)
+    std::vector<float> light_power;    // ILight.ApproximatePower per light
+    std::vector<uint32_t> mesh_refs;   // root reference of each mesh object, object order
+    uint32_t world_root = 0;           // root reference of the world BVH (has_bvh)
+    Box world_box;
+    int64_t num_internal = 0, num_leaves = 0, num_objects = 0, num_prims = 0;
+    float lut_e[32 * 32];
+    float lut_eavg[32];
+    // camera basis (Camera.cs:26-51)
+    f3 cam_origin, cam_llc, cam_horiz, cam_vert, cam_u, cam_v;
+    float lens_radius = 0.0f;
+    f3 sky;
+    bool medium = false;
+    f3 sig_a, sig_s, sig_t;
+    // scene analysis (Renderer.cs:361-378)
+    Box bounds;
+    float total_flux = 0, scene_radius = 0, mean_irradiance = 0;
+    int rr_min_bounces = 3;
+    float rr_min_survival = 0.1f;
+    int max_shadow_per_vertex = 0;     // Σ_l shadow_samples
+    int max_light_samples = 0;         // max_l shadow_samples
+};
+
+// Builds the host scene; returns RT_OK or an error with `err` set.
+int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err);
+
+// Kulla–Conty energy table (EnergyCompensationLut.cs:34-130)
+void build_energy_lut(float* E, float* Eavg);
+float lut_sample_eavg(const float* Eavg, float alpha);
+
+}  // namespace rt

@@ -1,0 +1,778 @@
+// scene_build.hip — host side of rt_scene_create: world assembly, binned-SAH
+// BVH, flattening to the 64-B node/entry layout, material/light tables,
+// energy LUT and the Renderer constructor's scene analysis.
+//
+// The BVH follows BvhNode.cs:91-363 step for step (16 bins per axis,
+// traversal cost 0.5, strict '<' keeps the first best split, in-place swap
+// partition `idx <= bestBin`, median fallback, fat leaves of <= 4): the tree
+// must be the reference's tree, because the bit-exact primary-hit gate
+// depends on the visit order it induces.  Large spans are built on worker
+// threads like the reference's Parallel.Invoke (BvhNode.cs:132-142); the
+// ranges are disjoint so the result does not depend on scheduling.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <future>
+#include <string>
+#include <memory>
+#include <thread>
+
+#include "rt_scene.h"
+
+namespace rt {
+
+static Box box_empty() { return Box{splat(FLT_MAX), splat(-FLT_MAX)}; }
+static Box box_union(const Box& a, const Box& b) { return Box{vmin(a.mn, b.mn), vmax(a.mx, b.mx)}; }
+static float box_area(const Box& b) {
+    f3 d = b.mx - b.mn;
+    if (d.x < 0.0f || d.y < 0.0f || d.z < 0.0f) return 0.0f;
+    return 2.0f * (d.x * d.y + d.x * d.z + d.y * d.z);
+}
+static f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+static int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
+static float ibits(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
+
+// Primitive bounding boxes (Triangle.cs:144-149, Sphere.cs:271-275,
+// Quad.cs:389-396, InfinitePlane.cs:69-73)
+static Box prim_box(const rt_primitive& p) {
+    const float* v = p.v;
+    switch (p.kind) {
+        case RT_PRIM_TRIANGLE: {
+            f3 a = ld3(v), b = ld3(v + 3), c = ld3(v + 6);
+            return Box{vmin(vmin(a, b), c) - splat(kEps), vmax(vmax(a, b), c) + splat(kEps)};
+        }
+        case RT_PRIM_SPHERE: {
+            f3 c = ld3(v);
+            f3 r = splat(v[3]);
+            return Box{c - r, c + r};
+        }
+        case RT_PRIM_QUAD: {
+            f3 q = ld3(v), u = ld3(v + 3), w = ld3(v + 6);
+            f3 bmin = q, bmax = (q + u) + w;
+            return Box{vmin(bmin, vmin(q + u, q + w)) - splat(0.0001f), vmax(bmax, vmax(q + u, q + w)) + splat(0.0001f)};
+        }
+        default:
+            return Box{splat(-1e6f), splat(1e6f)};
+    }
+}
+
+// ── binned SAH build ─────────────────────────────────────────────────────
+struct BNode {
+    Box box;
+    int axis = 0;
+    std::unique_ptr<BNode> l, r;
+    int start = 0, end = 0;   // leaf item range
+    bool leaf() const { return !l; }
+};
+
+static bool try_sah(std::vector<int32_t>& items, std::vector<Box>& bx, int start, int end, int& best_axis, int& mid) {
+    constexpr int kBins = 16;
+    int span = end - start;
+    f3 cmin = splat(FLT_MAX), cmax = splat(-FLT_MAX);
+    for (int i = start; i < end; i++) {
+        f3 c = (bx[i].mn + bx[i].mx) * 0.5f;
+        cmin = vmin(cmin, c);
+        cmax = vmax(cmax, c);
+    }
+    f3 ext = cmax - cmin;
+    float max_ext = mx(ext.x, mx(ext.y, ext.z));
+    if (max_ext <= 0.0f) { best_axis = -1; mid = start; return false; }
+    best_axis = -1;
+    int best_bin = -1;
+    float best_cost = FLT_MAX;
+    int cnt[kBins], pc[kBins], sc[kBins];
+    Box bb[kBins], pb[kBins], sb[kBins];
+    for (int axis = 0; axis < 3; axis++) {
+        float ea = get(ext, axis);
+        if (ea <= 0.0f) continue;
+        float cm = get(cmin, axis);
+        float scale = (float)kBins / ea;
+        for (int i = 0; i < kBins; i++) { cnt[i] = 0; bb[i] = box_empty(); }
+        for (int i = start; i < end; i++) {
+            float c = (get(bx[i].mn, axis) + get(bx[i].mx, axis)) * 0.5f;
+            int idx = (int)((c - cm) * scale);
+            if (idx < 0) idx = 0;
+            else if (idx >= kBins) idx = kBins - 1;
+            cnt[idx]++;
+            bb[idx] = box_union(bb[idx], bx[i]);
+        }
+        int cum = 0;
+        Box cb = box_empty();
+        for (int i = 0; i < kBins; i++) { cum += cnt[i]; cb = box_union(cb, bb[i]); pc[i] = cum; pb[i] = cb; }
+        cum = 0;
+        cb = box_empty();
+        for (int i = kBins - 1; i >= 0; i--) { cum += cnt[i]; cb = box_union(cb, bb[i]); sc[i] = cum; sb[i] = cb; }
+        float parea = box_area(pb[kBins - 1]);
+        if (parea <= 0.0f) continue;
+        for (int i = 0; i < kBins - 1; i++) {
+            int nl = pc[i], nr = sc[i + 1];
+            if (nl == 0 || nr == 0) continue;
+            float cost = 0.5f + ((float)nl * box_area(pb[i]) + (float)nr * box_area(sb[i + 1])) / parea;
+            if (cost < best_cost) { best_cost = cost; best_axis = axis; best_bin = i; }
+        }
+    }
+    if (best_axis < 0) { mid = start; return false; }
+    float amin = get(cmin, best_axis);
+    float ascale = (float)kBins / get(ext, best_axis);
+    int wp = start;
+    for (int i = start; i < end; i++) {
+        float c = (get(bx[i].mn, best_axis) + get(bx[i].mx, best_axis)) * 0.5f;
+        int idx = (int)((c - amin) * ascale);
+        if (idx < 0) idx = 0;
+        else if (idx >= kBins) idx = kBins - 1;
+        if (idx <= best_bin) {
+            std::swap(items[i], items[wp]);
+            std::swap(bx[i], bx[wp]);
+            wp++;
+        }
+    }
+    mid = wp;
+    if (mid == start || mid == end) mid = start + span / 2;
+    return true;
+}
+
+static std::unique_ptr<BNode> build_node(std::vector<int32_t>& items, std::vector<Box>& bx, int start, int end, int par_depth) {
+    auto n = std::make_unique<BNode>();
+    int span = end - start;
+    if (span <= 4) {
+        Box b = bx[start];
+        for (int i = 1; i < span; i++) b = box_union(b, bx[start + i]);
+        n->box = b;
+        n->start = start;
+        n->end = end;
+        return n;
+    }
+    int ba, mid;
+    if (!try_sah(items, bx, start, end, ba, mid)) { ba = 0; mid = start + span / 2; }
+    n->axis = ba;
+    if (span > 8192 && par_depth < 4) {
+        auto fl = std::async(std::launch::async, [&, start, mid]() { return build_node(items, bx, start, mid, par_depth + 1); });
+        n->r = build_node(items, bx, mid, end, par_depth + 1);
+        n->l = fl.get();
+    } else {
+        n->l = build_node(items, bx, start, mid, par_depth);
+        n->r = build_node(items, bx, mid, end, par_depth);
+    }
+    n->box = box_union(n->l->box, n->r->box);
+    return n;
+}
+
+struct Flattener {
+    HostScene& hs;
+    const rt_scene_desc* d;
+    std::vector<int32_t> quad_aux_of;          // primitive → quad aux index
+    // object → entry template (for mesh objects: a BVH entry)
+    std::vector<GEntry> obj_entry;
+    int deepest = 0;                            // deepest leaf seen by the current flatten
+
+    GEntry prim_entry(int32_t id) const {
+        const rt_primitive& p = d->primitives[id];
+        const float* v = p.v;
+        GEntry e{};
+        int32_t flags = 0;
+        if (p.flags & RT_PRIMF_BACKFACE_CULLED) flags |= kEntryCulled;
+        if (p.flags & RT_PRIMF_CAMERA_INVISIBLE) flags |= kEntryCamInvisible;
+        int32_t tag = p.kind | (flags << 8);
+        switch (p.kind) {
+            case RT_PRIM_TRIANGLE: {
+                f3 v0 = ld3(v), v1 = ld3(v + 3), v2 = ld3(v + 6);
+                f3 e1 = v1 - v0, e2 = v2 - v0;
+                f3 cr = cross(e1, e2);
+                float cl = length(cr);
+                f3 n = cl > 0.0f ? cr / cl : mk(0, 1, 0);
+                e.a = make_float4(v0.x, v0.y, v0.z, ibits(tag));
+                e.b = make_float4(e1.x, e1.y, e1.z, ibits(id));
+                e.c = make_float4(e2.x, e2.y, e2.z, ibits(p.material));
+                e.d = make_float4(n.x, n.y, n.z, 0.5f * cl);
+                break;
+            }
+            case RT_PRIM_SPHERE:
+                e.a = make_float4(v[0], v[1], v[2], ibits(tag));
+                e.b = make_float4(v[3], 0.0f, 0.0f, ibits(id));
+                e.c = make_float4(0.0f, 0.0f, 0.0f, ibits(p.material));
+                break;
+            case RT_PRIM_QUAD: {
+                f3 q = ld3(v), u = ld3(v + 3), w = ld3(v + 6);
+                f3 nn = cross(u, w);
+                f3 n = normalize(nn);
+                float dd = dot(n, q);
+                f3 ww = nn / dot(nn, nn);
+                e.a = make_float4(q.x, q.y, q.z, ibits(tag));
+                e.b = make_float4(n.x, n.y, n.z, ibits(id));
+                e.c = make_float4(ww.x, ww.y, ww.z, ibits(p.material));
+                e.d = make_float4(dd, ibits(quad_aux_of[id]), 0.0f, 0.0f);
+                break;
+            }
+            default: {
+                f3 n = normalize(mk(v[3], v[4], v[5]));
+                e.a = make_float4(v[0], v[1], v[2], ibits(tag));
+                e.b = make_float4(n.x, n.y, n.z, ibits(id));
+                e.c = make_float4(0.0f, 0.0f, 0.0f, ibits(p.material));
+                break;
+            }
+        }
+        return e;
+    }
+
+    // Flattens a built tree whose items are object ids (or primitive ids when
+    // `prim_items`), returns the root reference.
+    uint32_t flatten(const BNode* n, const std::vector<int32_t>& items, bool prim_items, int depth) {
+        hs.max_depth = std::max(hs.max_depth, depth);
+        deepest = std::max(deepest, depth);
+        if (n->leaf()) {
+            hs.num_leaves++;
+            uint32_t start = (uint32_t)hs.entries.size();
+            for (int i = n->start; i < n->end; i++)
+                hs.entries.push_back(prim_items ? prim_entry(items[i]) : obj_entry[items[i]]);
+            return leaf_ref(start, (uint32_t)(n->end - n->start));
+        }
+        hs.num_internal++;
+        uint32_t idx = (uint32_t)hs.nodes.size();
+        hs.nodes.push_back(GNode{});
+        uint32_t rl = flatten(n->l.get(), items, prim_items, depth + 1);
+        uint32_t rr = flatten(n->r.get(), items, prim_items, depth + 1);
+        const Box& a = n->l->box;
+        const Box& b = n->r->box;
+        GNode& g = hs.nodes[idx];
+        g.a = make_float4(a.mn.x, a.mn.y, a.mn.z, a.mx.x);
+        g.b = make_float4(a.mx.y, a.mx.z, b.mn.x, b.mn.y);
+        g.c = make_float4(b.mn.z, b.mx.x, b.mx.y, b.mx.z);
+        g.d = make_int4((int32_t)rl, (int32_t)rr, n->axis, 0);
+        return idx;
+    }
+};
+
+static GEntry bvh_entry(const Box& b, uint32_t ref, int32_t flags) {
+    GEntry e{};
+    e.a = make_float4(b.mn.x, b.mn.y, b.mn.z, ibits(kBvh | (flags << 8)));
+    e.b = make_float4(b.mx.x, b.mx.y, b.mx.z, ibits((int32_t)ref));
+    return e;
+}
+
+// ── energy LUT (EnergyCompensationLut.cs:34-130) ─────────────────────────
+static f3 vndf_sample(f3 v, float ax, float ay, float u1, float u2) {
+    f3 vh = normalize(mk(ax * v.x, ay * v.y, v.z));
+    float l2 = vh.x * vh.x + vh.y * vh.y;
+    f3 t1 = l2 > 0.0f ? mk(-vh.y, vh.x, 0.0f) / sqrtf(l2) : mk(1, 0, 0);
+    f3 t2 = cross(vh, t1);
+    float r = sqrtf(u1);
+    float phi = 2.0f * kPi * u2;
+    float a = r * cosf(phi);
+    float b = r * sinf(phi);
+    float s = 0.5f * (1.0f + vh.z);
+    b = (1.0f - s) * sqrtf(mx(0.0f, 1.0f - a * a)) + s * b;
+    f3 nh = a * t1 + b * t2 + sqrtf(mx(0.0f, 1.0f - a * a - b * b)) * vh;
+    return normalize(mk(ax * nh.x, ay * nh.y, mx(0.0f, nh.z)));
+}
+static float g1_aniso(f3 w, float ax, float ay) {
+    float c2 = w.z * w.z;
+    float lam;
+    if (c2 < 1e-14f) lam = 1e10f;
+    else {
+        float num = (ax * w.x) * (ax * w.x) + (ay * w.y) * (ay * w.y);
+        lam = 0.5f * (-1.0f + sqrtf(1.0f + num / c2));
+    }
+    return 1.0f / (1.0f + lam);
+}
+
+void build_energy_lut(float* E, float* Eavg) {
+    auto mu_v = [](int i) { return ((float)i + 0.5f) / 32.0f; };
+    for (int ai = 0; ai < 32; ai++) {
+        float alpha = mx(((float)ai + 0.5f) / 32.0f, 0.001f);
+        for (int mi = 0; mi < 32; mi++) {
+            float mu = mu_v(mi);
+            uint32_t state = ((uint32_t)mi * 0x9E3779B9u) ^ ((uint32_t)ai * 0xB5297A4Du) ^ 0xC2B2AE3Du;
+            if (state == 0) state = 1;
+            float st = sqrtf(mx(0.0f, 1.0f - mu * mu));
+            f3 v = mk(st, 0.0f, mu);
+            double sum = 0.0;
+            int hits = 0;
+            for (int s = 0; s < 2048; s++) {
+                state ^= state << 13; state ^= state >> 17; state ^= state << 5;
+                float u1 = (float)(state & 0x00FFFFFFu) / (float)0x01000000u;
+                state ^= state << 13; state ^= state >> 17; state ^= state << 5;
+                float u2 = (float)(state & 0x00FFFFFFu) / (float)0x01000000u;
+                f3 h = vndf_sample(v, alpha, alpha, u1, u2);
+                float vdh = dot(v, h);
+                f3 l = 2.0f * vdh * h - v;
+                if (l.z <= 0.0f) continue;
+                sum += (double)g1_aniso(l, alpha, alpha);
+                hits++;
+            }
+            double e = hits == 0 ? 0.0 : sum / 2048.0;
+            E[mi * 32 + ai] = (float)(e < 0.0 ? 0.0 : (e > 1.0 ? 1.0 : e));
+        }
+        float avg = 0.0f;
+        for (int mi = 0; mi < 32; mi++) avg += 2.0f * E[mi * 32 + ai] * mu_v(mi);
+        Eavg[ai] = clampf(avg / 32.0f, 0.0f, 1.0f);
+    }
+}
+
+float lut_sample_eavg(const float* Eavg, float alpha) {
+    alpha = clampf(alpha, 0.001f, 1.0f);
+    float fal = alpha * 32.0f - 0.5f;
+    int a0 = std::min(std::max((int)floorf(fal), 0), 31);
+    int a1 = std::min(a0 + 1, 31);
+    float ta = clampf(fal - (float)a0, 0.0f, 1.0f);
+    return Eavg[a0] + (Eavg[a1] - Eavg[a0]) * ta;
+}
+
+// ── materials (DisneyBsdf.cs:213-293, 1016-1049, 1097-1139, 1943-1959) ──
+static GMat make_gmat(const rt_material& m, const float* Eavg) {
+    GMat g{};
+    f3 c = ld3(m.color);
+    int32_t flags = 0;
+    g.m0 = make_int4(m.kind, 0, -1, 0);
+    switch (m.kind) {
+        case RT_MAT_LAMBERTIAN:
+            g.m1 = make_float4(c.x, c.y, c.z, 0.0f);
+            break;
+        case RT_MAT_EMISSIVE: {
+            float inten = mx(m.intensity, 0.0f);
+            f3 e = c * inten;
+            g.m1 = make_float4(e.x, e.y, e.z, 0.0f);
+            if (m.flags & RT_MATF_LIGHT_PROXY) flags |= kMatEmissiveProxy;
+            break;
+        }
+        case RT_MAT_DIELECTRIC:
+            g.m1 = make_float4(c.x, c.y, c.z, 0.0f);
+            g.m3 = make_float4(0.0f, 0.0f, 0.0f, m.ior);
+            flags |= kMatSpecTransmissive | kMatDielectricIor;
+            break;
+        default: {
+            float metallic = clampf(m.metallic, 0, 1), roughness = clampf(m.roughness, 0, 1);
+            float specular = clampf(m.specular, 0, 2), spec_tint = clampf(m.specular_tint, 0, 1);
+            float sheen = clampf(m.sheen, 0, 1), sheen_tint = clampf(m.sheen_tint, 0, 1);
+            float sheen_rough = clampf(m.sheen_roughness, 0.04f, 1);
+            float clearcoat = clampf(m.clearcoat, 0, 1), cc_gloss = clampf(m.clearcoat_gloss, 0, 1);
+            float spec_trans = clampf(m.spec_trans, 0, 1), ior = mx(m.ior, 1.0001f);
+            float aniso = clampf(m.anisotropic, 0, 1);
+            float rot = m.anisotropic_rotation - floorf(m.anisotropic_rotation);
+            float diff_trans = clampf(m.diff_trans, 0, 1);
+            float alpha = mx(roughness * roughness, 0.001f);
+            float aspect = sqrtf(1.0f - 0.9f * aniso);
+            float ax = mx(alpha / aspect, 0.001f), ay = mx(alpha * aspect, 0.001f);
+            float cr = (m.flags & RT_MATF_COAT_ROUGHNESS) ? m.coat_roughness : -1.0f;
+            float cc_alpha = cr >= 0.0f ? mx(cr * cr, 0.001f) : lerpf(0.1f, 0.001f, cc_gloss);
+            float ec = mx(m.coat_ior, 1.0001f);
+            float r0 = (ec - 1.0f) / (ec + 1.0f);
+            float cc_f0 = r0 * r0;
+            // ComputeF0
+            float r = (ior - 1.0f) / (ior + 1.0f);
+            float scaled = r * r * 2.0f * specular;
+            float lum = luminance(c);
+            f3 tint = lum > 0.0f ? c / lum : splat(1.0f);
+            f3 diel = lerp3(splat(scaled), scaled * tint, spec_tint);
+            f3 f0 = lerp3(diel, c, metallic);
+            // ComputeLobeWeights
+            float dall = (1.0f - metallic) * (1.0f - spec_trans);
+            float dw = dall * (1.0f - diff_trans), dtw = dall * diff_trans;
+            float sf0;
+            if (metallic > 0.5f) sf0 = luminance(c);
+            else sf0 = r * r * 2.0f * specular;
+            float sfloor = 0.1f * (1.0f - spec_trans * 0.9f);
+            float sw = mx(sfloor, lerpf(sf0, 1.0f, metallic));
+            float tw = (1.0f - metallic) * spec_trans;
+            float cw = clearcoat * 0.04f;
+            float shw = sheen * 0.25f * dall;
+            float alpha_iso = sqrtf(ax * ay);
+            float eavg = lut_sample_eavg(Eavg, alpha_iso);
+            float msw = sw * (1.0f - eavg);
+            float sum = dw + sw + tw + cw + shw + msw + dtw;
+            float tot = sum < 1e-6f ? 1.0f : sum;
+            // ResolveTransmission
+            f3 ttint, sigma = splat(0.0f);
+            bool has_tc = (m.flags & RT_MATF_TRANSMISSION_COLOR) != 0;
+            bool thin = (m.flags & RT_MATF_THIN_WALLED) != 0;
+            if (!has_tc) ttint = mk(sqrtf(mx(c.x, 0.0f)), sqrtf(mx(c.y, 0.0f)), sqrtf(mx(c.z, 0.0f)));
+            else {
+                f3 tc = ld3(m.transmission_color);
+                float td = mx(m.transmission_depth, 0.0f);
+                if (td <= 0.0f) ttint = tc;
+                else {
+                    f3 cc = mk(mx(tc.x, 1e-6f), mx(tc.y, 1e-6f), mx(tc.z, 1e-6f));
+                    float inv = 1.0f / td;
+                    sigma = mk(-logf(cc.x) * inv, -logf(cc.y) * inv, -logf(cc.z) * inv);
+                    ttint = splat(1.0f);
+                }
+            }
+            if (thin) flags |= kMatThinWalled;
+            if (has_tc) flags |= kMatHasTransColor;
+            // IsSpecularTransmissive / TryGetDielectricIor read the raw knobs
+            if (!thin && m.spec_trans >= 0.5f && m.roughness <= 0.05f) flags |= kMatSpecTransmissive;
+            if (!thin && m.spec_trans > 0.0f) flags |= kMatDielectricIor;
+            g.m1 = make_float4(c.x, c.y, c.z, metallic);
+            g.m2 = make_float4(f0.x, f0.y, f0.z, roughness);
+            g.m3 = make_float4(alpha, ax, ay, ior);
+            g.m4 = make_float4(dw / tot, sw / tot, tw / tot, shw / tot);
+            g.m5 = make_float4(msw / tot, dtw / tot, cw / tot, spec_trans);
+            g.m6 = make_float4(sheen, sheen_tint, sheen_rough, diff_trans);
+            g.m7 = make_float4(clearcoat, cc_alpha, cc_f0, rot);
+            g.m8 = make_float4(ttint.x, ttint.y, ttint.z, alpha_iso);
+            g.m9 = make_float4(sigma.x, sigma.y, sigma.z, eavg);
+            // raw spec_trans for ShadowTransmittance (DisneyBsdf.cs:570-584)
+            g.m0.w = fbits(m.spec_trans);
+            break;
+        }
+    }
+    g.m0.y = flags;
+    return g;
+}
+
+int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
+    if (!d) { err = "desc is null"; return RT_ERR_INVALID_ARGUMENT; }
+    if (d->num_primitives < 0 || d->num_materials < 0 || d->num_lights < 0 ||
+        (d->num_primitives > 0 && !d->primitives) || (d->num_materials > 0 && !d->materials) ||
+        (d->num_lights > 0 && !d->lights)) {
+        err = "invalid table pointer or count";
+        return RT_ERR_INVALID_ARGUMENT;
+    }
+    if (d->sky_color[0] != 0.0f || d->sky_color[1] != 0.0f || d->sky_color[2] != 0.0f) {
+        // A non-black flat sky becomes an EnvironmentLight in the reference
+        // (FlatSky.HasImportanceSampling, SceneLoader.cs:528-531).
+        err = "non-black sky (environment light) is not supported";
+        return RT_ERR_UNSUPPORTED;
+    }
+    const int64_t np = d->num_primitives;
+    if (np >= (int64_t)kLeafStartMask) { err = "too many primitives"; return RT_ERR_UNSUPPORTED; }
+    hs = HostScene();
+    hs.num_prims = np;
+    build_energy_lut(hs.lut_e, hs.lut_eavg);
+
+    // materials
+    hs.mats.resize(d->num_materials);
+    for (int64_t i = 0; i < d->num_materials; i++) {
+        int k = d->materials[i].kind;
+        if (k < RT_MAT_LAMBERTIAN || k > RT_MAT_DISNEY) { err = "bad material kind"; return RT_ERR_INVALID_ARGUMENT; }
+        if (k == RT_MAT_DISNEY && d->materials[i].thin_film_thickness > 0.0f) {
+            err = "thin-film Disney materials are not supported";
+            return RT_ERR_UNSUPPORTED;
+        }
+        hs.mats[i] = make_gmat(d->materials[i], hs.lut_eavg);
+    }
+
+    // objects (SceneLoader object list; a mesh is one object)
+    Flattener F{hs, d, {}, {}};
+    F.quad_aux_of.assign(np, -1);
+    std::vector<Box> pbox(np);
+    std::vector<int32_t> obj_first, obj_count;   // primitive range per object
+    std::vector<int32_t> obj_mesh;
+    int last_mesh = -1;
+    for (int64_t i = 0; i < np; i++) {
+        const rt_primitive& p = d->primitives[i];
+        if (p.kind < RT_PRIM_TRIANGLE || p.kind > RT_PRIM_PLANE) { err = "bad primitive kind"; return RT_ERR_INVALID_ARGUMENT; }
+        if (p.material < 0 || p.material >= d->num_materials) { err = "bad material index"; return RT_ERR_INVALID_ARGUMENT; }
+        if (p.mesh >= 0 && p.kind != RT_PRIM_TRIANGLE) { err = "mesh faces must be triangles"; return RT_ERR_INVALID_ARGUMENT; }
+        pbox[i] = prim_box(p);
+        if (p.kind == RT_PRIM_QUAD) {
+            f3 u = ld3(p.v + 3), w = ld3(p.v + 6);
+            F.quad_aux_of[i] = (int32_t)(hs.quad_aux.size() / 4);
+            f3 tn = normalize(u), bn = normalize(w);
+            hs.quad_aux.push_back(make_float4(u.x, u.y, u.z, length(cross(u, w))));
+            hs.quad_aux.push_back(make_float4(w.x, w.y, w.z, 0.0f));
+            hs.quad_aux.push_back(make_float4(tn.x, tn.y, tn.z, 0.0f));
+            hs.quad_aux.push_back(make_float4(bn.x, bn.y, bn.z, 0.0f));
+        }
+        if (p.mesh >= 0 && p.mesh == last_mesh) {
+            obj_count.back()++;
+        } else {
+            obj_first.push_back((int32_t)i);
+            obj_count.push_back(1);
+            obj_mesh.push_back(p.mesh);
+            last_mesh = p.mesh;
+        }
+    }
+    const int64_t nobj = (int64_t)obj_first.size();
+    hs.num_objects = nobj;
+    std::vector<Box> obox(nobj);
+    std::vector<bool> is_plane(nobj, false);
+    F.obj_entry.resize(nobj);
+    for (int64_t o = 0; o < nobj; o++) {
+        int32_t f = obj_first[o];
+        if (obj_mesh[o] < 0) {
+            obox[o] = pbox[f];
+            is_plane[o] = d->primitives[f].kind == RT_PRIM_PLANE;
+            F.obj_entry[o] = F.prim_entry(f);
+            continue;
+        }
+        // Mesh (Mesh.cs:75-116): nested BVH over its faces when > 4, else a list
+        int32_t cnt = obj_count[o];
+        std::vector<int32_t> items(cnt);
+        std::vector<Box> bx(cnt);
+        for (int k = 0; k < cnt; k++) { items[k] = f + k; bx[k] = pbox[f + k]; }
+        if (cnt > 4) {
+            auto root = build_node(items, bx, 0, cnt, 0);
+            F.deepest = 0;
+            uint32_t ref = F.flatten(root.get(), items, true, 0);
+            hs.mesh_depth = std::max(hs.mesh_depth, F.deepest);
+            hs.mesh_refs.push_back(ref);
+            obox[o] = root->box;
+            F.obj_entry[o] = bvh_entry(root->box, ref, 0);
+        } else {
+            uint32_t start = (uint32_t)hs.entries.size();
+            hs.mesh_depth = std::max(hs.mesh_depth, 0);
+            hs.mesh_refs.push_back(leaf_ref(start, (uint32_t)cnt));
+            Box b = bx[0];
+            for (int k = 0; k < cnt; k++) { hs.entries.push_back(F.prim_entry(items[k])); if (k) b = box_union(b, bx[k]); }
+            obox[o] = b;
+            F.obj_entry[o] = bvh_entry(b, leaf_ref(start, (uint32_t)cnt), kEntryNoBox);
+        }
+    }
+
+    // World (SceneLoader.cs:460-487)
+    std::vector<GEntry> top;
+    std::vector<Box> top_boxes;
+    if (nobj > 4) {
+        std::vector<int32_t> fin;
+        std::vector<Box> fbx;
+        std::vector<int32_t> inf;
+        for (int64_t o = 0; o < nobj; o++) {
+            if (is_plane[o]) inf.push_back((int32_t)o);
+            else { fin.push_back((int32_t)o); fbx.push_back(obox[o]); }
+        }
+        if (!fin.empty()) {
+            auto root = build_node(fin, fbx, 0, (int)fin.size(), 0);
+            F.deepest = 0;
+            uint32_t ref = F.flatten(root.get(), fin, false, 0);
+            hs.world_depth = F.deepest;
+            hs.world_root = ref;
+            hs.world_box = root->box;
+            top.push_back(bvh_entry(root->box, ref, 0));
+            top_boxes.push_back(root->box);
+            hs.has_bvh = true;
+        }
+        for (int32_t o : inf) { top.push_back(F.obj_entry[o]); top_boxes.push_back(obox[o]); }
+    } else {
+        for (int64_t o = 0; o < nobj; o++) { top.push_back(F.obj_entry[o]); top_boxes.push_back(obox[o]); }
+    }
+    hs.top_start = (uint32_t)hs.entries.size();
+    hs.top_count = (uint32_t)top.size();
+    for (auto& e : top) hs.entries.push_back(e);
+    if (hs.entries.size() >= kLeafStartMask) { err = "scene too large"; return RT_ERR_UNSUPPORTED; }
+
+    // Camera basis (Camera.cs:26-51)
+    {
+        const rt_camera& c = d->camera;
+        f3 lf = ld3(c.look_from), la = ld3(c.look_at), vup = ld3(c.vup);
+        float theta = c.vfov_deg * kPi / 180.0f;
+        float h = tanf(theta / 2.0f);
+        float vh = 2.0f * h, vw = c.aspect * vh;
+        f3 w = normalize(lf - la);
+        if (length_sq(cross(w, vup)) < 1e-8f) vup = fabsf(w.x) < 0.9f ? mk(1, 0, 0) : mk(0, 0, 1);
+        f3 u = normalize(cross(vup, w));
+        f3 v = cross(w, u);
+        hs.cam_origin = lf;
+        hs.cam_horiz = c.focus_dist * vw * u;
+        hs.cam_vert = c.focus_dist * vh * v;
+        hs.cam_llc = lf - hs.cam_horiz / 2.0f - hs.cam_vert / 2.0f - c.focus_dist * w;
+        hs.cam_u = u;
+        hs.cam_v = v;
+        hs.lens_radius = c.aperture / 2.0f;
+    }
+    hs.sky = ld3(d->sky_color);
+    if (d->global_medium.enabled) {
+        hs.medium = true;
+        hs.sig_a = vmax(ld3(d->global_medium.sigma_a), splat(0.0f));
+        hs.sig_s = vmax(ld3(d->global_medium.sigma_s), splat(0.0f));
+        hs.sig_t = hs.sig_a + hs.sig_s;
+    }
+
+    // Finite scene bounds (Renderer.cs:476-502)
+    {
+        Box acc = box_empty();
+        bool any = false;
+        for (const Box& b : top_boxes) {
+            f3 dd = b.mx - b.mn;
+            if (dd.x < 1e5f && dd.y < 1e5f && dd.z < 1e5f) { acc = any ? box_union(acc, b) : b; any = true; }
+        }
+        if (!any) {
+            Box raw = box_empty();
+            for (size_t k = 0; k < top_boxes.size(); k++) raw = k ? box_union(raw, top_boxes[k]) : top_boxes[k];
+            acc = Box{vmax(raw.mn, splat(-1e3f)), vmin(raw.mx, splat(1e3f))};
+        }
+        hs.bounds = acc;
+    }
+
+    // Lights
+    float flux = 0.0f;
+    hs.lights.resize(d->num_lights);
+    hs.light_prim.assign(d->num_lights, -1);
+    hs.light_power.assign(d->num_lights, 0.0f);
+    for (int64_t i = 0; i < d->num_lights; i++) {
+        const rt_light& l = d->lights[i];
+        GLight& g = hs.lights[i];
+        g = GLight{};
+        f3 col = ld3(l.color);
+        float power = 0.0f;
+        int ss = 1;
+        switch (l.kind) {
+            case RT_LIGHT_SPHERE: {
+                float r = mx(l.radius, kEps);
+                ss = std::max(1, l.shadow_samples);
+                g.l1 = make_float4(col.x, col.y, col.z, l.intensity);
+                g.l2 = make_float4(l.position[0], l.position[1], l.position[2], r);
+                power = 4.0f * kPi * luminance(col) * l.intensity;
+                break;
+            }
+            case RT_LIGHT_AREA: {
+                f3 u = ld3(l.u), v = ld3(l.v);
+                f3 cr = cross(u, v);
+                float area = length(cr);
+                f3 n = cr / area;
+                ss = std::max(1, l.shadow_samples);
+                g.l1 = make_float4(col.x, col.y, col.z, l.intensity);
+                g.l2 = make_float4(l.position[0], l.position[1], l.position[2], area);
+                g.l3 = make_float4(u.x, u.y, u.z, mx(0.0f, l.soft_radius));
+                g.l4 = make_float4(v.x, v.y, v.z, 0.0f);
+                g.l5 = make_float4(n.x, n.y, n.z, area);
+                power = kPi * area * luminance(col) * l.intensity;
+                break;
+            }
+            case RT_LIGHT_DIRECTIONAL: {
+                f3 dir = normalize(ld3(l.position));
+                float ang = mx(0.0f, l.radius);
+                bool disc = ang > 0.0f;
+                float rad = ang * kPi / 180.0f;
+                float cosa = disc ? cosf(rad) : 1.0f;
+                float solid = disc ? 2.0f * kPi * (1.0f - cosa) : 0.0f;
+                ss = l.shadow_samples > 0 ? l.shadow_samples : (disc ? 4 : 1);
+                g.l1 = make_float4(col.x, col.y, col.z, l.intensity);
+                g.l2 = make_float4(dir.x, dir.y, dir.z, ang);
+                g.l3 = make_float4(cosa, solid, 0.0f, 0.0f);
+                f3 e = hs.bounds.mx - hs.bounds.mn;
+                float rr = 0.5f * length(e);
+                power = luminance(col) * l.intensity * (kPi * rr * rr);
+                break;
+            }
+            case RT_LIGHT_GEOMETRY: {
+                if (l.primitive < 0 || l.primitive >= np) { err = "geometry light primitive out of range"; return RT_ERR_INVALID_ARGUMENT; }
+                const rt_primitive& p = d->primitives[l.primitive];
+                const rt_material& pm = d->materials[p.material];
+                if (pm.kind != RT_MAT_EMISSIVE || p.kind == RT_PRIM_PLANE || p.mesh >= 0) {
+                    err = "geometry light needs a standalone emissive triangle, quad or sphere";
+                    return RT_ERR_UNSUPPORTED;
+                }
+                ss = std::max(1, l.shadow_samples);
+                f3 em = ld3(pm.color) * mx(pm.intensity, 0.0f);
+                g.l1 = make_float4(em.x, em.y, em.z, 0.0f);
+                float area;
+                const float* v = p.v;
+                if (p.kind == RT_PRIM_TRIANGLE) {
+                    f3 v0 = ld3(v), e1 = ld3(v + 3) - v0, e2 = ld3(v + 6) - v0;
+                    f3 cr = cross(e1, e2);
+                    float cl = length(cr);
+                    f3 n = cl > 0.0f ? cr / cl : mk(0, 1, 0);
+                    area = 0.5f * cl;
+                    g.l0.w = kGeoTri;
+                    g.l2 = make_float4(v0.x, v0.y, v0.z, area);
+                    g.l3 = make_float4(e1.x, e1.y, e1.z, mx(0.0f, l.soft_radius));
+                    g.l4 = make_float4(e2.x, e2.y, e2.z, 0.0f);
+                    g.l5 = make_float4(n.x, n.y, n.z, area);
+                } else if (p.kind == RT_PRIM_QUAD) {
+                    f3 q = ld3(v), u = ld3(v + 3), w = ld3(v + 6);
+                    f3 nn = cross(u, w);
+                    f3 n = normalize(nn);
+                    area = length(nn);
+                    g.l0.w = kGeoQuad;
+                    g.l2 = make_float4(q.x, q.y, q.z, area);
+                    g.l3 = make_float4(u.x, u.y, u.z, mx(0.0f, l.soft_radius));
+                    g.l4 = make_float4(w.x, w.y, w.z, 0.0f);
+                    g.l5 = make_float4(n.x, n.y, n.z, area);
+                } else {
+                    area = 4.0f * kPi * v[3] * v[3];
+                    g.l0.w = kGeoSphere;
+                    g.l2 = make_float4(v[0], v[1], v[2], v[3]);
+                    g.l5 = make_float4(0, 0, 0, area);
+                }
+                power = kPi * area * luminance(em);
+                hs.light_prim[i] = l.primitive;
+                break;
+            }
+            default:
+                err = "bad light kind";
+                return RT_ERR_INVALID_ARGUMENT;
+        }
+        int sq = (int)ceilf(sqrtf((float)ss));
+        g.l0.x = l.kind;
+        g.l0.y = ss;
+        g.l0.z = sq;
+        flux += power;
+        hs.light_power[i] = power;
+        hs.max_shadow_per_vertex += ss;
+        hs.max_light_samples = std::max(hs.max_light_samples, ss);
+    }
+    // emitter → light map (Renderer.cs:409-426): geometry lights (last write
+    // wins), then light proxies
+    for (int64_t i = 0; i < d->num_lights; i++)
+        if (d->lights[i].kind == RT_LIGHT_GEOMETRY) hs.mats[d->primitives[d->lights[i].primitive].material].m0.z = (int32_t)i;
+    for (int64_t i = 0; i < d->num_lights; i++) {
+        const rt_light& l = d->lights[i];
+        if ((l.kind == RT_LIGHT_SPHERE || l.kind == RT_LIGHT_AREA) && l.material >= 0 && l.material < d->num_materials)
+            hs.mats[l.material].m0.z = (int32_t)i;
+    }
+
+    // Russian-roulette classifier (Renderer.cs:361-378)
+    f3 ext = hs.bounds.mx - hs.bounds.mn;
+    float R = mx(0.5f * length(ext), 1e-3f);
+    hs.total_flux = flux;
+    hs.scene_radius = R;
+    hs.mean_irradiance = flux / (4.0f * kPi * R * R);
+    bool ind = hs.mean_irradiance < 0.5f;
+    hs.rr_min_bounces = ind ? 6 : 3;
+    hs.rr_min_survival = ind ? 0.40f : 0.10f;
+    return RT_OK;
+}
+
+}  // namespace rt
+
+// ── diagnostics: serialise the flattened world BVH ───────────────────────
+// Pre-order: internal node → (0, axis, 6 box floats as bits); leaf → (1,
+// count, 6 box floats, entry ids...) with an entry id = primitive id, or
+// -(mesh ordinal + 2) for a nested mesh.  Same format as the CPU oracle's
+// dump of the reference tree.
+extern "C" int64_t rt_debug_bvh_dump(const rt_scene_desc* desc, int32_t* out, int64_t cap) {
+    using namespace rt;
+    HostScene hs;
+    std::string err;
+    if (build_host_scene(desc, hs, err) != RT_OK) return -1;
+    if (!hs.has_bvh) return 0;
+    std::vector<int32_t> v;
+    auto box = [&](const Box& b) {
+        float f[6] = {b.mn.x, b.mn.y, b.mn.z, b.mx.x, b.mx.y, b.mx.z};
+        for (float x : f) v.push_back(fbits(x));
+    };
+    std::function<void(uint32_t, const Box&)> rec = [&](uint32_t ref, const Box& b) {
+        if (ref_is_leaf(ref)) {
+            uint32_t s = leaf_start(ref), c = leaf_count(ref);
+            v.push_back(1);
+            v.push_back((int32_t)c);
+            box(b);
+            for (uint32_t k = 0; k < c; k++) {
+                const GEntry& e = hs.entries[s + k];
+                int32_t kind = fbits(e.a.w) & 0xff;
+                if (kind != kBvh) v.push_back(fbits(e.b.w));
+                else {
+                    uint32_t r = (uint32_t)fbits(e.b.w);
+                    int32_t id = -1;
+                    for (size_t m = 0; m < hs.mesh_refs.size(); m++)
+                        if (hs.mesh_refs[m] == r) id = -(int32_t)m - 2;
+                    v.push_back(id);
+                }
+            }
+            return;
+        }
+        const GNode& n = hs.nodes[ref];
+        v.push_back(0);
+        v.push_back(n.d.z);
+        box(b);
+        Box b0{mk(n.a.x, n.a.y, n.a.z), mk(n.a.w, n.b.x, n.b.y)};
+        Box b1{mk(n.b.z, n.b.w, n.c.x), mk(n.c.y, n.c.z, n.c.w)};
+        rec((uint32_t)n.d.x, b0);
+        rec((uint32_t)n.d.y, b1);
+    };
+    rec(hs.world_root, hs.world_box);
+    int64_t cnt = (int64_t)v.size();
+    if (out && cap >= cnt) std::memcpy(out, v.data(), (size_t)cnt * 4);
+    return cnt;
+}

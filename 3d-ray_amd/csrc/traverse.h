@@ -1,0 +1,233 @@
+// traverse.h — closest-hit traversal of the flattened BVH on gfx950.
+//
+// Visit order and every comparison reproduce the reference's recursive
+// IHittable.Hit chain (HittableList.cs:19-38 → BvhNode.cs:157-206 →
+// Mesh.cs:110-116 → primitive Hit), so the closest hit — including the
+// "later primitive wins on equal t" tie rule — is bit-identical:
+//   * the top-level list and every leaf are scanned in order with the
+//     running closest t as tMax (accept t == tMax);
+//   * at an internal node both child boxes are slab-tested against the
+//     current closest; the near child (dir[axis] >= 0 → left) is entered
+//     first and the far child is pushed with its entry distance tEnter;
+//   * a popped far child is visited iff tEnter <= closest — exactly the
+//     reference's far-box test `tExit >= tEnter` evaluated after the near
+//     subtree, because its tExit term already passed at push time;
+//   * a nested BVH (a Mesh, or the world BVH root) inside a leaf is entered
+//     after a box test, with the rest of the leaf pushed as a continuation.
+//
+// The per-lane stack lives in LDS (the wavefront's traversal stack), entries
+// (ref, tEnter) interleaved by lane so a wave's pushes/pops are conflict-free.
+#pragma once
+#include "rt_internal.h"
+
+namespace rt {
+
+struct Tri {
+    f3 v0, e1, e2;
+};
+
+RT_HD f3 f3of(float4 v) { return mk(v.x, v.y, v.z); }
+RT_HD int32_t tag_of(float4 a) { return __float_as_int(a.w); }
+
+// Triangle.Hit (Triangle.cs:49-89)
+__device__ __forceinline__ bool hit_triangle(f3 o, f3 d, f3 v0, f3 e1, f3 e2, float tmin, float tmax, float& t, float& u,
+                                             float& v) {
+    f3 h = cross(d, e2);
+    float a = dot(e1, h);
+    if (fabsf(a) < 1e-7f) return false;
+    float f = 1.0f / a;
+    f3 s = o - v0;
+    u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return false;
+    f3 q = cross(s, e1);
+    v = f * dot(d, q);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = f * dot(e2, q);
+    if (t < tmin || t > tmax) return false;
+    return true;
+}
+
+// Sphere.Hit (Sphere.cs:20-40)
+__device__ __forceinline__ bool hit_sphere(f3 o, f3 d, f3 c, float r, float tmin, float tmax, float& t) {
+    f3 oc = o - c;
+    float a = length_sq(d);
+    float hb = dot(oc, d);
+    float cc = length_sq(oc) - r * r;
+    float disc = hb * hb - a * cc;
+    if (disc < 0.0f) return false;
+    float sd = sqrtf(disc);
+    float root = (-hb - sd) / a;
+    if (root < tmin || root > tmax) {
+        root = (-hb + sd) / a;
+        if (root < tmin || root > tmax) return false;
+    }
+    t = root;
+    return true;
+}
+
+// Quad.Hit (Quad.cs:318-362) [+ BackFaceCulledHittable when culled]
+__device__ __forceinline__ bool hit_quad(f3 o, f3 d, const GEntry& e, const float4* aux, bool culled, float tmin,
+                                         float tmax, float& t, float& u, float& v) {
+    f3 n = f3of(e.b);
+    float denom = dot(n, d);
+    if (fabsf(denom) < 1e-8f) return false;
+    float D = e.d.x;
+    t = (D - dot(n, o)) / denom;
+    if (t < tmin || t > tmax) return false;
+    f3 q = f3of(e.a), w = f3of(e.c);
+    int ai = __float_as_int(e.d.y);
+    f3 U = f3of(aux[4 * ai]), V = f3of(aux[4 * ai + 1]);
+    f3 ip = o + t * d;
+    f3 pv = ip - q;
+    u = dot(w, cross(pv, V));
+    v = dot(w, cross(U, pv));
+    if (u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f) return false;
+    if (culled && !(dot(d, n) < 0.0f)) return false;
+    return true;
+}
+
+// InfinitePlane.Hit (InfinitePlane.cs:28-40)
+__device__ __forceinline__ bool hit_plane(f3 o, f3 d, f3 p, f3 n, float tmin, float tmax, float& t) {
+    float denom = dot(n, d);
+    if (fabsf(denom) < 1e-6f) return false;
+    t = dot(p - o, n) / denom;
+    if (t < tmin || t > tmax) return false;
+    return true;
+}
+
+// AABB.Hit (AABB.cs:28-52).  min/max of the two slab distances use the
+// hardware maxNum/minNum; a NaN anywhere in t0/t1 (origin on a slab plane of
+// an axis-parallel ray) makes the reference's NaN-propagating reductions fail
+// the `tExit >= tEnter` test, which the explicit NaN check reproduces.
+__device__ __forceinline__ bool slab(f3 mn_, f3 mx_, f3 o, f3 inv, float tmin, float tmax, float& te) {
+    float t0x = (mn_.x - o.x) * inv.x, t0y = (mn_.y - o.y) * inv.y, t0z = (mn_.z - o.z) * inv.z;
+    float t1x = (mx_.x - o.x) * inv.x, t1y = (mx_.y - o.y) * inv.y, t1z = (mx_.z - o.z) * inv.z;
+    float tsx = fminf(t0x, t1x), tsy = fminf(t0y, t1y), tsz = fminf(t0z, t1z);
+    float tlx = fmaxf(t0x, t1x), tly = fmaxf(t0y, t1y), tlz = fmaxf(t0z, t1z);
+    te = fmaxf(tmin, fmaxf(tsx, fmaxf(tsy, tsz)));
+    float tx = fminf(tmax, fminf(tlx, fminf(tly, tlz)));
+    bool nan = (t0x != t0x) | (t0y != t0y) | (t0z != t0z) | (t1x != t1x) | (t1y != t1y) | (t1z != t1z);
+    bool empty = (mn_.x > mx_.x) | (mn_.y > mx_.y) | (mn_.z > mx_.z);
+    return (tx >= te) & !nan & !empty;
+}
+
+struct HitOut {
+    int32_t entry;   // entry index, -1 = miss
+    float t, u, v;
+};
+
+// Tests one primitive entry; updates `h` and `closest` on a hit.
+__device__ __forceinline__ void test_prim(const GEntry& e, uint32_t idx, int32_t kind, int32_t flags, f3 o, f3 d,
+                                          float tmin, const float4* aux, float& closest, HitOut& h) {
+    float t, u = 0.0f, v = 0.0f;
+    bool hit;
+    if (kind == kTri) hit = hit_triangle(o, d, f3of(e.a), f3of(e.b), f3of(e.c), tmin, closest, t, u, v);
+    else if (kind == kSphere) hit = hit_sphere(o, d, f3of(e.a), e.b.x, tmin, closest, t);
+    else if (kind == kQuad) hit = hit_quad(o, d, e, aux, (flags & kEntryCulled) != 0, tmin, closest, t, u, v);
+    else hit = hit_plane(o, d, f3of(e.a), f3of(e.b), tmin, closest, t);
+    if (hit) {
+        closest = t;
+        h.entry = (int32_t)idx;
+        h.t = t;
+        h.u = u;
+        h.v = v;
+    }
+}
+
+// Stack layout: stk[k * kStride + lane] (uint2: ref, tEnter bits).
+template <int SD, int STRIDE>
+__device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes, const GEntry* __restrict__ entries,
+                                                const float4* __restrict__ aux, uint32_t top_start, uint32_t top_count,
+                                                f3 o, f3 d, float tmin, float tmax, uint2* stk,
+                                                uint32_t* overflow) {
+    HitOut h;
+    h.entry = -1;
+    h.t = 0.0f;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float closest = tmax;
+    for (uint32_t ti = 0; ti < top_count; ti++) {
+        uint32_t eidx = top_start + ti;
+        GEntry e = entries[eidx];
+        int32_t tag = tag_of(e.a);
+        int32_t kind = tag & 0xff, flags = tag >> 8;
+        if (kind != kBvh) {
+            test_prim(e, eidx, kind, flags, o, d, tmin, aux, closest, h);
+            continue;
+        }
+        float te;
+        if (!(flags & kEntryNoBox) && !slab(f3of(e.a), f3of(e.b), o, inv, tmin, closest, te)) continue;
+        uint32_t cur = (uint32_t)__float_as_int(e.b.w);
+        int sp = 0;
+        for (;;) {
+            if (ref_is_leaf(cur)) {
+                uint32_t start = leaf_start(cur), cnt = leaf_count(cur);
+                bool descended = false;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    uint32_t idx = start + k;
+                    GEntry le = entries[idx];
+                    int32_t ltag = tag_of(le.a);
+                    int32_t lk = ltag & 0xff, lf = ltag >> 8;
+                    if (lk != kBvh) {
+                        test_prim(le, idx, lk, lf, o, d, tmin, aux, closest, h);
+                        continue;
+                    }
+                    float lte;
+                    if (!(lf & kEntryNoBox) && !slab(f3of(le.a), f3of(le.b), o, inv, tmin, closest, lte)) continue;
+                    if (k + 1 < cnt) {
+                        if (sp >= SD) { *overflow = 1u; return h; }
+                        stk[sp * STRIDE] = make_uint2(leaf_ref(idx + 1, cnt - k - 1), __float_as_uint(-FLT_MAX));
+                        sp++;
+                    }
+                    cur = (uint32_t)__float_as_int(le.b.w);
+                    descended = true;
+                    break;
+                }
+                if (descended) continue;
+            } else {
+                const GNode nd = nodes[cur];
+                f3 c0mn = mk(nd.a.x, nd.a.y, nd.a.z), c0mx = mk(nd.a.w, nd.b.x, nd.b.y);
+                f3 c1mn = mk(nd.b.z, nd.b.w, nd.c.x), c1mx = mk(nd.c.y, nd.c.z, nd.c.w);
+                float te0, te1;
+                bool h0 = slab(c0mn, c0mx, o, inv, tmin, closest, te0);
+                bool h1 = slab(c1mn, c1mx, o, inv, tmin, closest, te1);
+                int axis = nd.d.z;
+                float da = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+                bool left_first = da >= 0.0f;
+                uint32_t rn = (uint32_t)(left_first ? nd.d.x : nd.d.y);
+                uint32_t rf = (uint32_t)(left_first ? nd.d.y : nd.d.x);
+                bool hn = left_first ? h0 : h1, hf = left_first ? h1 : h0;
+                float tf = left_first ? te1 : te0;
+                if (hn) {
+                    if (hf) {
+                        if (sp >= SD) { *overflow = 1u; return h; }
+                        stk[sp * STRIDE] = make_uint2(rf, __float_as_uint(tf));
+                        sp++;
+                    }
+                    cur = rn;
+                    continue;
+                }
+                if (hf) {
+                    cur = rf;
+                    continue;
+                }
+            }
+            // pop
+            bool found = false;
+            while (sp > 0) {
+                sp--;
+                uint2 s = stk[sp * STRIDE];
+                if (__uint_as_float(s.y) <= closest) {
+                    cur = s.x;
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) break;
+        }
+    }
+    return h;
+}
+
+}  // namespace rt

@@ -1,0 +1,223 @@
+"""ctypes binding of lib3dray_amd.so, shaped like the reference's render
+entry point:
+
+  reference (C#)                                    here
+  new Renderer(world, camera, lights, sky, spp,     Renderer(scene, samples_per_pixel, max_depth,
+               maxDepth, globalMedium,                        max_sample_radiance=..., ...)
+               maxSampleRadiance, verbose,
+               misHeuristic, lightSamplingStrategy,
+               indirectClampFactor, ..., exposureEv)
+      -- src/RayTracer/Rendering/Renderer.cs:300-319
+  renderer.Render(width, height, options)           renderer.render(width, height, capture_aovs=...)
+      -- Renderer.cs:522-733, returns RenderResult      returns RenderResult(pixels, buffers)
+         {Pixels, Buffers} (RenderBuffers.cs)
+
+The world/camera/lights/sky/medium travel together as a scenes.SceneDesc
+(the flattened SceneLoader output).  There is no CPU fallback: a missing
+library or device raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib3dray_amd.so")
+_lib = None
+
+
+class RtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rt error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Loads lib3dray_amd.so (fails loudly when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() (or python 3d-ray_amd/build.py)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    fp, ip = C.POINTER(C.c_float), C.POINTER(C.c_int32)
+    L.rt_get_version.restype = C.c_int
+    L.rt_material_defaults.argtypes = [C.POINTER(abi.rt_material), C.c_int32]
+    L.rt_material_defaults.restype = None
+    L.rt_settings_defaults.argtypes = [C.POINTER(abi.rt_settings)]
+    L.rt_settings_defaults.restype = None
+    L.rt_create.argtypes = [C.c_int32, C.POINTER(vp)]
+    L.rt_destroy.argtypes = [vp]
+    L.rt_destroy.restype = None
+    L.rt_last_error.argtypes = [vp]
+    L.rt_last_error.restype = C.c_char_p
+    L.rt_scene_create.argtypes = [vp, C.POINTER(abi.rt_scene_desc), C.POINTER(vp)]
+    L.rt_scene_destroy.argtypes = [vp]
+    L.rt_scene_destroy.restype = None
+    L.rt_scene_get_info.argtypes = [vp, C.POINTER(abi.rt_scene_info)]
+    L.rt_render.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.POINTER(abi.rt_film), C.POINTER(abi.rt_render_stats)]
+    L.rt_trace_primary.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.c_int32, C.c_int32, ip, fp, fp, fp]
+    L.rt_trace_rays.argtypes = [vp, vp, fp, C.c_int64, ip, fp, fp, fp]
+    L.rt_debug_bvh_dump.argtypes = [C.POINTER(abi.rt_scene_desc), ip, C.c_int64]
+    L.rt_debug_bvh_dump.restype = C.c_int64
+    _lib = L
+    return L
+
+
+EXPORTED_SYMBOLS = ("rt_get_version", "rt_material_defaults", "rt_settings_defaults", "rt_create", "rt_destroy",
+                    "rt_last_error", "rt_scene_create", "rt_scene_destroy", "rt_scene_get_info", "rt_render",
+                    "rt_trace_primary", "rt_trace_rays", "rt_debug_bvh_dump")
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+
+
+def bvh_dump(desc):
+    """Host-side build of `desc`; the flattened world BVH in pre-order
+    (rt_debug_bvh_dump).  Needs no GPU."""
+    L = load()
+    n = L.rt_debug_bvh_dump(desc.ptr, None, 0)
+    if n < 0:
+        raise RtError(int(n), "scene build failed")
+    out = np.empty(n, np.int32)
+    L.rt_debug_bvh_dump(desc.ptr, _i(out), n)
+    return out
+
+
+class Context:
+    """One HIP device + stream (rt_create)."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = C.c_void_p()
+        rc = L.rt_create(device, C.byref(h))
+        if rc != 0:
+            raise RtError(rc, "rt_create failed (no HIP device?)")
+        self.h = h
+
+    def check(self, rc):
+        if rc != 0:
+            raise RtError(rc, load().rt_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().rt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Scene:
+    """Device-resident scene (rt_scene_create): BVH, tables, LUT in HBM."""
+
+    def __init__(self, ctx, desc):
+        self.ctx = ctx
+        self.desc = desc
+        h = C.c_void_p()
+        ctx.check(load().rt_scene_create(ctx.h, desc.ptr, C.byref(h)))
+        self.h = h
+
+    def info(self):
+        i = abi.rt_scene_info()
+        self.ctx.check(load().rt_scene_get_info(self.h, C.byref(i)))
+        return i
+
+    def trace_primary(self, settings, sample_begin=0, sample_count=1):
+        n = settings.width * settings.height * sample_count
+        prim = np.empty(n, np.int32)
+        t, u, v = (np.empty(n, np.float32) for _ in range(3))
+        self.ctx.check(load().rt_trace_primary(self.ctx.h, self.h, C.byref(settings), sample_begin, sample_count,
+                                               _i(prim), _f(t), _f(u), _f(v)))
+        return prim, t, u, v
+
+    def trace_rays(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = len(rays)
+        prim = np.empty(n, np.int32)
+        t, u, v = (np.empty(n, np.float32) for _ in range(3))
+        self.ctx.check(load().rt_trace_rays(self.ctx.h, self.h, _f(rays), n, _i(prim), _f(t), _f(u), _f(v)))
+        return prim, t, u, v
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().rt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class RenderBuffers:
+    """Linear-HDR film (RenderBuffers.cs:52-169), planes shaped (C, H, W)."""
+
+    def __init__(self, planes, spp):
+        self.__dict__.update(planes)
+        self.samples_a = (spp + 1) // 2
+        self.samples_b = spp // 2
+
+
+class RenderResult:
+    def __init__(self, pixels, buffers, stats):
+        self.pixels = pixels      # tone-mapped display (H, W, 3)
+        self.buffers = buffers
+        self.stats = stats
+
+
+_MIS = {"balance": abi.RT_MIS_BALANCE, "power": abi.RT_MIS_POWER}
+_LIGHTS = {"all": abi.RT_LIGHTS_ALL, "power": abi.RT_LIGHTS_POWER, "uniform": abi.RT_LIGHTS_UNIFORM}
+
+
+class Renderer:
+    """Mirror of Renderer(...) + Render(width, height, options)."""
+
+    def __init__(self, scene, samples_per_pixel, max_depth, max_sample_radiance=10.0, mis_heuristic="balance",
+                 light_sampling_strategy="all", indirect_clamp_factor=0.25, exposure_ev=0.0, rr_min_bounces=-1,
+                 rr_min_survival=-1.0, seed_xor=0, paths_in_flight=0):
+        self.scene = scene
+        s = abi.rt_settings()
+        load().rt_settings_defaults(C.byref(s))
+        s.samples_per_pixel = samples_per_pixel
+        s.max_depth = max_depth
+        s.max_sample_radiance = max_sample_radiance
+        s.mis = _MIS[mis_heuristic]
+        s.light_sampling = _LIGHTS[light_sampling_strategy]
+        s.indirect_clamp_factor = indirect_clamp_factor
+        s.exposure_ev = exposure_ev
+        s.rr_min_bounces = rr_min_bounces
+        s.rr_min_survival = rr_min_survival
+        s.seed_xor = seed_xor
+        s.paths_in_flight = paths_in_flight
+        self.settings = s
+
+    def render(self, width, height, capture_aovs=False):
+        s = self.settings
+        s.width, s.height, s.capture_aovs = width, height, 1 if capture_aovs else 0
+        npx = width * height
+        planes = {k: np.zeros(3 * npx, np.float32) for k in ("beauty", "beauty_a", "beauty_b", "display")}
+        if capture_aovs:
+            for k in ("albedo_a", "albedo_b", "normal_a", "normal_b"):
+                planes[k] = np.zeros(3 * npx, np.float32)
+            planes["depth_a"] = np.zeros(npx, np.float32)
+            planes["depth_b"] = np.zeros(npx, np.float32)
+        count = np.zeros(npx, np.int32)
+        film = abi.rt_film()
+        for k, a in planes.items():
+            setattr(film, k, _f(a))
+        film.sample_count = _i(count)
+        stats = abi.rt_render_stats()
+        ctx = self.scene.ctx
+        ctx.check(load().rt_render(ctx.h, self.scene.h, C.byref(s), C.byref(film), C.byref(stats)))
+        shaped = {k: (a.reshape(3, height, width) if a.size == 3 * npx else a.reshape(height, width))
+                  for k, a in planes.items()}
+        shaped["sample_count"] = count.reshape(height, width)
+        pixels = np.transpose(shaped["display"], (1, 2, 0))
+        return RenderResult(pixels, RenderBuffers(shaped, s.samples_per_pixel), stats)

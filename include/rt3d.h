@@ -1,0 +1,337 @@
+/*
+ * rt3d.h — C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * This is the drop-in boundary for the render entry point of the C# path
+ * tracer fiorenzobrioni/3d-ray.  The reference exposes no FFI; its render
+ * entry point is the pair
+ *
+ *   new Renderer(world, camera, lights, sky, samplesPerPixel, maxDepth,
+ *                globalMedium, maxSampleRadiance, verbose, misHeuristic,
+ *                lightSamplingStrategy, indirectClampFactor, ...)
+ *       -- src/RayTracer/Rendering/Renderer.cs:300-319
+ *   RenderResult Render(int width, int height, RenderCaptureOptions options)
+ *       -- src/RayTracer/Rendering/Renderer.cs:522-733
+ *
+ * and the world it receives is assembled by SceneLoader.Load
+ * (src/RayTracer/Scene/SceneLoader.cs:122-553).  A host (the C# P/Invoke shim,
+ * the Python ctypes harness, or C++) flattens that object graph into the POD
+ * tables below, in the SceneLoader's object order, and calls this library.
+ * The library builds the binned-SAH BVH on the host (restating
+ * src/RayTracer/Acceleration/BvhNode.cs:91-363), uploads everything to HBM
+ * once per scene, and renders with hand-written HIP kernels.
+ *
+ * Conventions
+ *   - every entry point returns an rt_status (0 = OK, < 0 = error); on error
+ *     rt_last_error(ctx) holds a message until the next call on that context;
+ *   - the caller owns all host memory; tables are copied during the call and
+ *     never retained; the library owns device memory inside rt_scene/rt_ctx;
+ *   - calls on one rt_ctx must be serialised by the caller; distinct contexts
+ *     are independent; there is no process-global render state (the
+ *     reference's Sampler.SetKind and ShadowRay.BlockSpecularTransmission
+ *     statics are per-call settings here);
+ *   - all structs are plain C with fixed sizes (checked by static asserts in
+ *     the implementation) so C# [StructLayout(Sequential)] and ctypes map 1:1.
+ */
+#ifndef RT3D_H
+#define RT3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT3D_ABI_VERSION 1
+
+typedef enum rt_status {
+    RT_OK = 0,
+    RT_ERR_INVALID_ARGUMENT = -1,
+    RT_ERR_NO_DEVICE = -2,
+    RT_ERR_DEVICE = -3,
+    RT_ERR_OUT_OF_MEMORY = -4,
+    RT_ERR_UNSUPPORTED = -5,
+    RT_ERR_INTERNAL = -6
+} rt_status;
+
+/* ── Primitives ───────────────────────────────────────────────────────────
+ * One record per intersectable primitive, in SceneLoader object order
+ * (ground, YAML entities, then light proxies appended by CreateLight,
+ * SceneLoader.cs:282-457).  The index of a record is its primitive ID — the
+ * ID the bit-exact primary-hit parity gate compares.
+ *
+ *   TRIANGLE  v[0..8]  = v0, v1, v2           (Triangle.cs:35-89)
+ *   SPHERE    v[0..3]  = centre, radius        (Sphere.cs:13-75)
+ *   QUAD      v[0..8]  = Q, U, V               (Quad.cs:302-362)
+ *   PLANE     v[0..5]  = point, normal         (InfinitePlane.cs:28)
+ *
+ * mesh = -1 for a standalone entity.  mesh = k >= 0 marks the faces of one
+ * Mesh object (Mesh.cs:75-116, a nested BVH); its faces must be contiguous
+ * and in face order, and the mesh takes the object-list position of its
+ * first face.  Only TRIANGLE faces may belong to a mesh.
+ */
+typedef enum rt_prim_kind {
+    RT_PRIM_TRIANGLE = 0,
+    RT_PRIM_SPHERE = 1,
+    RT_PRIM_QUAD = 2,
+    RT_PRIM_PLANE = 3
+} rt_prim_kind;
+
+typedef enum rt_prim_flags {
+    RT_PRIMF_BACKFACE_CULLED = 1,   /* BackFaceCulledHittable (area-light proxy) */
+    RT_PRIMF_CAMERA_INVISIBLE = 2   /* CameraInvisibleHittable */
+} rt_prim_flags;
+
+typedef struct rt_primitive {
+    int32_t kind;       /* rt_prim_kind */
+    int32_t material;   /* index into rt_scene_desc.materials */
+    int32_t flags;      /* rt_prim_flags */
+    int32_t mesh;       /* -1 or mesh object index */
+    float v[12];
+} rt_primitive;         /* 64 bytes */
+
+/* ── Materials ─────────────────────────────────────────────────────────────
+ *   LAMBERTIAN  color = albedo                         (Lambertian.cs:37-96)
+ *   EMISSIVE    color, intensity                       (Emissive.cs:56-104)
+ *   DIELECTRIC  color = albedo, ior                    (Dielectric.cs:46-105)
+ *   DISNEY      color = baseColor + every Disney knob  (DisneyBsdf.cs:213-293)
+ * rt_material_defaults() fills the DisneyBsdf constructor defaults.
+ */
+typedef enum rt_material_kind {
+    RT_MAT_LAMBERTIAN = 0,
+    RT_MAT_EMISSIVE = 1,
+    RT_MAT_DIELECTRIC = 2,
+    RT_MAT_DISNEY = 3
+} rt_material_kind;
+
+typedef enum rt_material_flags {
+    RT_MATF_LIGHT_PROXY = 1,          /* Emissive.IsLightProxy */
+    RT_MATF_THIN_WALLED = 2,          /* DisneyBsdf.ThinWalled */
+    RT_MATF_TRANSMISSION_COLOR = 4,   /* DisneyBsdf.TransmissionColor != null */
+    RT_MATF_COAT_ROUGHNESS = 8        /* DisneyBsdf.CoatRoughness != null */
+} rt_material_flags;
+
+typedef struct rt_material {
+    int32_t kind;
+    int32_t flags;
+    float color[3];
+    float intensity;
+    float metallic, roughness, specular, specular_tint;
+    float sheen, sheen_tint, sheen_roughness, clearcoat;
+    float clearcoat_gloss, spec_trans, ior, anisotropic;
+    float anisotropic_rotation, diff_trans, coat_ior, coat_roughness;
+    float thin_film_thickness, thin_film_ior, transmission_depth;
+    float transmission_color[3];
+    int32_t reserved[4];
+} rt_material;          /* 128 bytes */
+
+/* ── Lights (final ILight list order of SceneLoader.Load) ─────────────────
+ *   SPHERE       position = centre, radius, color, intensity, shadow_samples,
+ *                primitive/material = visible proxy  (SphereLight.cs:112-271)
+ *   AREA         position = corner, u, v, color, intensity, shadow_samples,
+ *                soft_radius, primitive/material = proxy (AreaLight.cs:97-226)
+ *   DIRECTIONAL  position = direction, color, intensity, radius = angular
+ *                radius in degrees, shadow_samples (0 = reference default)
+ *                                                   (DirectionalLight.cs:102-181)
+ *   GEOMETRY     primitive = emissive TRIANGLE/QUAD/SPHERE, material = its
+ *                Emissive, shadow_samples, soft_radius (GeometryLight.cs:133-279)
+ */
+typedef enum rt_light_kind {
+    RT_LIGHT_SPHERE = 0,
+    RT_LIGHT_AREA = 1,
+    RT_LIGHT_DIRECTIONAL = 2,
+    RT_LIGHT_GEOMETRY = 3
+} rt_light_kind;
+
+typedef struct rt_light {
+    int32_t kind;
+    int32_t shadow_samples;
+    int32_t primitive;
+    int32_t material;
+    float color[3];
+    float intensity;
+    float position[3];
+    float radius;
+    float u[3];
+    float soft_radius;
+    float v[3];
+    float reserved;
+} rt_light;             /* 80 bytes */
+
+/* Look-at thin-lens camera (Camera.cs:26-106).  aspect = width/height as the
+ * SceneLoader computes it (SceneLoader.cs:491). */
+typedef struct rt_camera {
+    float look_from[3];
+    float look_at[3];
+    float vup[3];
+    float vfov_deg;
+    float aspect;
+    float aperture;
+    float focus_dist;
+    float reserved;
+} rt_camera;            /* 56 bytes */
+
+/* Global homogeneous medium with isotropic phase (HomogeneousMedium.cs:36-76). */
+typedef struct rt_medium {
+    int32_t enabled;
+    float sigma_a[3];
+    float sigma_s[3];
+    float reserved;
+} rt_medium;            /* 32 bytes */
+
+typedef struct rt_scene_desc {
+    const rt_primitive* primitives;
+    int64_t num_primitives;
+    const rt_material* materials;
+    int64_t num_materials;
+    const rt_light* lights;
+    int64_t num_lights;
+    rt_camera camera;
+    float sky_color[3];   /* flat sky (FlatSky.cs); must be black (see rt_scene_create) */
+    int32_t reserved0;
+    rt_medium global_medium;
+} rt_scene_desc;
+
+/* ── Render settings (Renderer ctor arguments + Render(width, height)) ───── */
+typedef enum rt_sampler_kind { RT_SAMPLER_SOBOL = 0, RT_SAMPLER_PRNG = 1 } rt_sampler_kind;
+typedef enum rt_mis_kind { RT_MIS_BALANCE = 0, RT_MIS_POWER = 1 } rt_mis_kind;
+typedef enum rt_light_sampling { RT_LIGHTS_ALL = 0, RT_LIGHTS_POWER = 1, RT_LIGHTS_UNIFORM = 2 } rt_light_sampling;
+
+typedef struct rt_settings {
+    int32_t width;
+    int32_t height;
+    int32_t samples_per_pixel;
+    int32_t max_depth;
+    int32_t sampler;                /* rt_sampler_kind (only SOBOL is implemented) */
+    int32_t mis;                    /* rt_mis_kind */
+    int32_t light_sampling;         /* rt_light_sampling */
+    int32_t capture_aovs;           /* RenderCaptureOptions.CaptureAovs */
+    float max_sample_radiance;      /* --clamp, reference default 10 */
+    float indirect_clamp_factor;    /* --indirect-clamp-factor, default 0.25 */
+    float exposure_ev;              /* --exposure, default 0 */
+    int32_t rr_min_bounces;         /* < 0: scene classifier (Renderer.cs:361-378) */
+    float rr_min_survival;          /* < 0: scene classifier */
+    uint32_t seed_xor;              /* 0 = the reference's Sobol streams; any other
+                                       value XORs every pixel seed (independent
+                                       streams for statistical tests) */
+    int32_t caustics;               /* reserved: photon caustics (not yet built) */
+    int32_t caustic_photons;
+    float caustic_gather_radius;
+    int32_t paths_in_flight;        /* wavefront size; 0 = library default */
+    int32_t reserved[6];
+} rt_settings;          /* 96 bytes */
+
+/* ── Film (RenderBuffers.cs:52-169; FrameBuffer planar layout
+ *    Data[c*W*H + y*W + x], FrameBuffer.cs:20-41).  Caller-allocated host
+ *    arrays; any NULL plane is not written.  Colour planes hold 3*W*H floats,
+ *    depth planes W*H floats, sample_count W*H ints.  display is the
+ *    tone-mapped RenderResult.Pixels (exposure, ACES, gamma 2.2). */
+typedef struct rt_film {
+    float* beauty;
+    float* beauty_a;
+    float* beauty_b;
+    float* albedo_a;
+    float* albedo_b;
+    float* normal_a;
+    float* normal_b;
+    float* depth_a;
+    float* depth_b;
+    int32_t* sample_count;
+    float* display;
+} rt_film;
+
+/* Per-render counters and timings filled by rt_render (optional). */
+typedef struct rt_render_stats {
+    double wall_ms;              /* whole rt_render call */
+    double device_ms;            /* HIP-event time of all render kernels */
+    double trace_ms;             /* HIP-event time of the closest-hit kernels */
+    double shadow_ms;            /* HIP-event time of the shadow kernels */
+    double shade_ms;             /* HIP-event time of the shading kernels */
+    int64_t camera_samples;      /* width*height*spp */
+    int64_t rays_closest;        /* primary + extension segments traced */
+    int64_t rays_shadow;         /* shadow segments (each transparent-shadow
+                                    continuation counts once) */
+    int64_t trace_launches;      /* closest-hit kernel launches */
+    int64_t reserved[7];
+} rt_render_stats;
+
+typedef struct rt_scene_info {
+    int64_t num_primitives;
+    int64_t num_objects;         /* top-level objects (meshes count once) */
+    int64_t num_nodes;           /* internal BVH nodes, all levels */
+    int64_t num_leaves;
+    int32_t max_depth;           /* deepest root-to-leaf path */
+    int32_t has_bvh;             /* 0 when the world is a flat list (<= 4 objects) */
+    int32_t rr_min_bounces;      /* resolved by the scene classifier */
+    float rr_min_survival;
+    float scene_radius;
+    float mean_irradiance;
+    int64_t device_bytes;        /* scene bytes resident in HBM */
+    int64_t reserved[4];
+} rt_scene_info;
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_scene rt_scene;
+
+/* Library ABI version (RT3D_ABI_VERSION). */
+int rt_get_version(void);
+
+/* Fills the DisneyBsdf constructor defaults (DisneyBsdf.cs:167-212) and the
+ * reference's per-kind defaults into *m for the given kind. */
+void rt_material_defaults(rt_material* m, int32_t kind);
+
+/* Fills the reference's CLI/ctor defaults (Renderer.cs:300-319,
+ * Program.cs Appendix-A canonical flags). */
+void rt_settings_defaults(rt_settings* s);
+
+/* Creates a context on HIP device `device` (one HIP stream, default priority).
+ * Replaces: the implicit process-wide state of Renderer.Render's Parallel.For. */
+int rt_create(int32_t device, rt_ctx** out);
+void rt_destroy(rt_ctx* ctx);
+const char* rt_last_error(const rt_ctx* ctx);
+
+/* Builds the world (SceneLoader.cs:460-487: BVH over finite objects when there
+ * are more than 4, infinite planes kept beside it), the binned-SAH BVH
+ * (BvhNode.cs:91-363), the light tables and the Kulla–Conty energy LUT
+ * (EnergyCompensationLut.cs:34-171), and uploads them.
+ * Replaces: SceneLoader.Load's world assembly + the Renderer constructor's
+ * scene analysis (Renderer.cs:321-441). */
+int rt_scene_create(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out);
+void rt_scene_destroy(rt_scene* scene);
+int rt_scene_get_info(const rt_scene* scene, rt_scene_info* info);
+
+/* Renders settings->samples_per_pixel samples per pixel into the film.
+ * Replaces: Renderer.Render(width, height, options) (Renderer.cs:522-733). */
+int rt_render(rt_ctx* ctx, const rt_scene* scene, const rt_settings* settings,
+              const rt_film* film, rt_render_stats* stats);
+
+/* Parity hook: for samples [sample_begin, sample_begin+sample_count) of every
+ * pixel, the camera ray of Renderer.cs:630-653 (+ the camera-visibility skip
+ * of TraceRay, Renderer.cs:1609-1644) is traced and its closest hit written as
+ * prim_id (-1 = miss), t, u, v.  Output index = s_local*W*H + y*W + x.
+ * NULL outputs are skipped.  No reference counterpart (HitRecord carries no
+ * primitive index); this is the bit-exact primary-hit gate. */
+int rt_trace_primary(rt_ctx* ctx, const rt_scene* scene, const rt_settings* settings,
+                     int32_t sample_begin, int32_t sample_count,
+                     int32_t* prim_id, float* t, float* u, float* v);
+
+/* Closest hit of arbitrary rays (origin xyz, direction xyz per ray, 6 floats)
+ * against the scene, tMin = 1e-4, tMax = float.MaxValue, exactly as
+ * world.Hit(ray, MathUtils.Epsilon, MathUtils.Infinity) (Renderer.cs:1629).
+ * Replaces: IHittable.Hit on the assembled world (HittableList.cs:19-38). */
+int rt_trace_rays(rt_ctx* ctx, const rt_scene* scene, const float* rays, int64_t num_rays,
+                  int32_t* prim_id, float* t, float* u, float* v);
+
+/* Diagnostics (host only, no device needed): builds the scene on the host and
+ * serialises the flattened world BVH in pre-order — internal node: (0, axis,
+ * 6 box floats as int bits); leaf: (1, count, 6 box floats, entry ids) with
+ * an entry id = primitive id or -(mesh ordinal + 2).  Returns the number of
+ * int32 values (written only when cap is large enough), 0 when the world has
+ * no BVH, < 0 on error.  No reference counterpart: it lets a test compare the
+ * tree with the reference's BvhNode tree (BvhNode.cs:91-363) node by node. */
+int64_t rt_debug_bvh_dump(const rt_scene_desc* desc, int32_t* out, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RT3D_H */

@@ -1,0 +1,2816 @@
+// oracle.cpp — CPU restatement of the reference's render hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// it.  It follows the reference C# source function by function (file:line
+// cited at each piece) with the reference's recursive control flow, so that
+// a difference between it and the wavefront GPU design shows up as a test
+// failure rather than being shared by both.
+//
+// Parity status per function (what pins it) is listed in DESIGN.md; the
+// functions whose bits depend on .NET's Vector3 reduction order or MathF
+// transcendentals are statistically pinned only.
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+#include <algorithm>
+#include <functional>
+#include <string>
+#include <cmath>
+
+#include "../include/rt3d.h"
+#include "orc_core.h"
+
+namespace orc {
+
+// ═════════════════════════════════════════════════════════════════════════
+// Sampler context (Sampler.cs:75-118).  Thread-local like the reference's
+// [ThreadStatic] Ctx.  "fixed" mode feeds a caller-supplied list of uniforms
+// (unit tests of BSDF/light sampling).
+// ═════════════════════════════════════════════════════════════════════════
+struct SamplerCtx {
+    bool active = false;
+    uint32_t seed = 0, index = 0, dim = 0;
+    const float* fixed = nullptr;
+    int nfixed = 0, fixed_pos = 0;
+    uint64_t rng = 0x853c49e6748fea9bull;
+};
+static thread_local SamplerCtx t_smp;
+
+static float rng_next(uint64_t& s) {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    uint32_t x = (uint32_t)(((s >> 18u) ^ s) >> 27u);
+    uint32_t rot = (uint32_t)(s >> 59u);
+    x = (x >> rot) | (x << ((32u - rot) & 31u));
+    return (float)(x >> 8) * (1.0f / 16777216.0f);
+}
+
+// Sampler.Sample1D / MathUtils.RandomFloat (Sampler.cs:106-118)
+static float random_float() {
+    SamplerCtx& c = t_smp;
+    if (c.fixed) {
+        if (c.fixed_pos < c.nfixed) return c.fixed[c.fixed_pos++];
+        c.fixed_pos++;
+        return 0.5f;
+    }
+    if (!c.active) return rng_next(c.rng);
+    uint32_t d = c.dim++;
+    return sobol_sample(c.index, d, c.seed);
+}
+
+// MathUtils.RandomUnitVector (MathUtils.cs:93-101)
+static V3 random_unit_vector() {
+    float u = random_float();
+    float v = random_float();
+    float z = 1.0f - 2.0f * u;
+    float r = std::sqrt(std::fmax(0.0f, 1.0f - z * z));
+    float phi = 2.0f * kPi * v;
+    return V3(r * std::cos(phi), r * std::sin(phi), z);
+}
+// MathUtils.RandomInUnitDisk (MathUtils.cs:137-155)
+static V3 random_in_unit_disk() {
+    float a = 2.0f * random_float() - 1.0f;
+    float b = 2.0f * random_float() - 1.0f;
+    if (a == 0.0f && b == 0.0f) return V3(0.0f);
+    float r, phi;
+    if (a * a > b * b) { r = a; phi = (kPi / 4.0f) * (b / a); }
+    else { r = b; phi = (kPi / 2.0f) - (kPi / 4.0f) * (a / b); }
+    return V3(r * std::cos(phi), r * std::sin(phi), 0.0f);
+}
+
+// Traversal instrumentation (node box tests / primitive tests), used to
+// count algorithmic bytes on the reference tree.
+static thread_local int64_t t_node_tests = 0;
+static thread_local int64_t t_prim_tests = 0;
+
+struct Material;
+
+// HitRecord.cs:9-163 (the fields the path reads)
+struct HitRecord {
+    V3 point, normal, shading_ng, tangent, bitangent;
+    float t = 0, u = 0, v = 0;
+    bool front_face = false;
+    const Material* mat = nullptr;
+    float relative_eta = 0.0f;
+    int prim = -1;
+    bool camera_invisible = false;
+    void set_face_normal(const Ray& r, V3 outward) {
+        front_face = dot(r.d, outward) < 0.0f;
+        normal = front_face ? outward : -outward;
+    }
+};
+
+// ═════════════════════════════════════════════════════════════════════════
+// Geometry
+// ═════════════════════════════════════════════════════════════════════════
+struct Hittable {
+    virtual ~Hittable() {}
+    virtual bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const = 0;
+    virtual AABB box() const = 0;
+    virtual bool is_plane() const { return false; }
+};
+
+// Triangle.cs:35-89,144-149
+struct Triangle : Hittable {
+    V3 v0, v1, v2, e1, e2, n, tan, bitan;
+    float area;
+    const Material* mat;
+    int prim;
+    Triangle(V3 a, V3 b, V3 c, const Material* m, int id) : v0(a), v1(b), v2(c), mat(m), prim(id) {
+        e1 = v1 - v0;
+        e2 = v2 - v0;
+        V3 cr = cross(e1, e2);
+        float cl = length(cr);
+        n = cl > 0.0f ? cr / cl : V3(0, 1, 0);
+        area = 0.5f * cl;
+        tan = length_sq(e1) > 0.0f ? normalize(e1) : V3(1, 0, 0);
+        bitan = length_sq(e2) > 0.0f ? normalize(e2) : V3(0, 0, 1);
+    }
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override {
+        t_prim_tests++;
+        V3 h = cross(r.d, e2);
+        float a = dot(e1, h);
+        if (std::fabs(a) < 1e-7f) return false;
+        float f = 1.0f / a;
+        V3 s = r.o - v0;
+        float u = f * dot(s, h);
+        if (u < 0.0f || u > 1.0f) return false;
+        V3 q = cross(s, e1);
+        float v = f * dot(r.d, q);
+        if (v < 0.0f || u + v > 1.0f) return false;
+        float t = f * dot(e2, q);
+        if (t < tmin || t > tmax) return false;
+        rec.t = t;
+        rec.point = r.at(t);
+        rec.set_face_normal(r, n);
+        rec.u = u;
+        rec.v = v;
+        rec.tangent = tan;
+        rec.bitangent = bitan;
+        rec.mat = mat;
+        rec.prim = prim;
+        rec.camera_invisible = false;
+        return true;
+    }
+    AABB box() const override {
+        V3 mn = vmin(vmin(v0, v1), v2) - V3(kEpsilon);
+        V3 mx = vmax(vmax(v0, v1), v2) + V3(kEpsilon);
+        return AABB(mn, mx);
+    }
+    // Triangle.SampleStratified/SampleAt (Triangle.cs:120-140)
+    void sample_stratified(int idx, int sq, V3& p, V3& nrm, float& ar) const {
+        float inv = 1.0f / (float)sq;
+        int su = idx % sq, sv = idx / sq;
+        float xi1 = ((float)su + random_float()) * inv;
+        float xi2 = ((float)sv + random_float()) * inv;
+        float r1 = std::sqrt(xi1);
+        float u = 1.0f - r1;
+        float v = xi2 * r1;
+        p = v0 + u * e1 + v * e2;
+        nrm = n;
+        ar = area;
+    }
+};
+
+static void sphere_uv(V3 p, float& u, float& v) {
+    float theta = std::acos(-p.y);
+    float phi = std::atan2(-p.z, p.x) + kPi;
+    u = phi / (2.0f * kPi);
+    v = theta / kPi;
+}
+
+// Frisvad ONB + local→world (Sphere.cs:237-256, SphereLight.cs:284-305)
+static V3 local_to_world(V3 local, V3 w) {
+    V3 u, v;
+    if (w.z < -0.999f) { u = V3(0, -1, 0); v = V3(-1, 0, 0); }
+    else {
+        float a = 1.0f / (1.0f + w.z);
+        float b = -w.x * w.y * a;
+        u = V3(1.0f - w.x * w.x * a, b, -w.x);
+        v = V3(b, 1.0f - w.y * w.y * a, -w.y);
+    }
+    V3 res = local.x * u + local.y * v + local.z * w;
+    float l2 = length_sq(res);
+    return l2 > 1e-8f ? res / std::sqrt(l2) : w;
+}
+
+// Sphere.cs:20-75,139-229,271-275
+struct Sphere : Hittable {
+    V3 c;
+    float r;
+    const Material* mat;
+    int prim;
+    Sphere(V3 center, float radius, const Material* m, int id) : c(center), r(radius), mat(m), prim(id) {}
+    bool hit(const Ray& ray, float tmin, float tmax, HitRecord& rec) const override {
+        t_prim_tests++;
+        V3 oc = ray.o - c;
+        float a = length_sq(ray.d);
+        float hb = dot(oc, ray.d);
+        float cc = length_sq(oc) - r * r;
+        float disc = hb * hb - a * cc;
+        if (disc < 0.0f) return false;
+        float sd = std::sqrt(disc);
+        float root = (-hb - sd) / a;
+        if (root < tmin || root > tmax) {
+            root = (-hb + sd) / a;
+            if (root < tmin || root > tmax) return false;
+        }
+        rec.t = root;
+        rec.point = ray.at(root);
+        V3 outward = (rec.point - c) / r;
+        rec.set_face_normal(ray, outward);
+        sphere_uv(outward, rec.u, rec.v);
+        V3 td = cross(V3(0, 1, 0), outward);
+        if (length_sq(td) < 1e-4f) td = V3(1, 0, 0);
+        rec.tangent = normalize(td);
+        rec.bitangent = normalize(cross(outward, rec.tangent));
+        rec.mat = mat;
+        rec.prim = prim;
+        rec.camera_invisible = false;
+        return true;
+    }
+    AABB box() const override { return AABB(c - V3(r), c + V3(r)); }
+    // Sphere.SampleSolidAngleCore (Sphere.cs:139-203)
+    void sample_solid_angle(V3 from, float xi1, float xi2, V3& p, V3& nrm, float& pdf) const {
+        V3 to_c = c - from;
+        float d2 = length_sq(to_c);
+        float r2 = r * r;
+        if (d2 <= r2) {
+            V3 pu = random_unit_vector();
+            p = c + pu * r;
+            nrm = pu;
+            pdf = 1.0f / (4.0f * kPi);
+            return;
+        }
+        float dist = std::sqrt(d2);
+        V3 w = to_c / dist;
+        float s2 = r2 / d2;
+        float ctm = std::sqrt(std::fmax(0.0f, 1.0f - s2));
+        float ct = 1.0f - xi1 * (1.0f - ctm);
+        float st = std::sqrt(std::fmax(0.0f, 1.0f - ct * ct));
+        float phi = 2.0f * kPi * xi2;
+        V3 local(st * std::cos(phi), st * std::sin(phi), ct);
+        V3 dw = local_to_world(local, w);
+        V3 oc = from - c;
+        float b = dot(oc, dw);
+        float cc = length_sq(oc) - r2;
+        float disc = b * b - cc;
+        float t;
+        if (disc <= 0.0f) t = -b;
+        else {
+            float sd = std::sqrt(disc);
+            t = -b - sd;
+            if (t < kEpsilon) t = -b + sd;
+        }
+        p = from + t * dw;
+        nrm = normalize(p - c);
+        float sa = 2.0f * kPi * (1.0f - ctm);
+        pdf = sa > kEpsilon ? 1.0f / sa : 0.0f;
+    }
+    // Sphere.SolidAnglePdf (Sphere.cs:206-229)
+    float solid_angle_pdf(V3 from, V3 wi) const {
+        V3 to_c = c - from;
+        float d2 = length_sq(to_c);
+        float r2 = r * r;
+        if (d2 <= r2) return 1.0f / (4.0f * kPi);
+        float s2 = r2 / d2;
+        float ctm = std::sqrt(std::fmax(0.0f, 1.0f - s2));
+        float dist = std::sqrt(d2);
+        V3 w = to_c / dist;
+        float wl = length(wi);
+        if (wl < kEpsilon) return 0.0f;
+        float ct = dot(wi, w) / wl;
+        if (ct < ctm) return 0.0f;
+        float sa = 2.0f * kPi * (1.0f - ctm);
+        return sa > kEpsilon ? 1.0f / sa : 0.0f;
+    }
+};
+
+// Quad.cs:302-396 (+ BackFaceCulledHittable.cs:42-50 via `culled`)
+struct Quad : Hittable {
+    V3 q, u, v, n, w, tan, bitan;
+    float d, area;
+    const Material* mat;
+    int prim;
+    bool culled;
+    Quad(V3 Q, V3 U, V3 V, const Material* m, int id, bool cull) : q(Q), u(U), v(V), mat(m), prim(id), culled(cull) {
+        V3 nn = cross(u, v);
+        n = normalize(nn);
+        d = dot(n, q);
+        w = nn / dot(nn, nn);
+        tan = normalize(u);
+        bitan = normalize(v);
+        area = length(nn);
+    }
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override {
+        t_prim_tests++;
+        float denom = dot(n, r.d);
+        if (std::fabs(denom) < 1e-8f) return false;
+        float t = (d - dot(n, r.o)) / denom;
+        if (t < tmin || t > tmax) return false;
+        V3 ip = r.at(t);
+        V3 pv = ip - q;
+        float alpha = dot(w, cross(pv, v));
+        float beta = dot(w, cross(u, pv));
+        if (alpha < 0.0f || alpha > 1.0f || beta < 0.0f || beta > 1.0f) return false;
+        bool front = dot(r.d, n) < 0.0f;
+        if (culled && !front) return false;
+        rec.u = alpha;
+        rec.v = beta;
+        rec.t = t;
+        rec.point = ip;
+        rec.set_face_normal(r, n);
+        rec.tangent = tan;
+        rec.bitangent = bitan;
+        rec.mat = mat;
+        rec.prim = prim;
+        rec.camera_invisible = false;
+        return true;
+    }
+    AABB box() const override {
+        V3 bmin = q, bmax = (q + u) + v;
+        V3 mn = vmin(bmin, vmin(q + u, q + v)) - V3(0.0001f);
+        V3 mx = vmax(bmax, vmax(q + u, q + v)) + V3(0.0001f);
+        return AABB(mn, mx);
+    }
+    void sample_stratified(int idx, int sq, V3& p, V3& nrm, float& ar) const {
+        float inv = 1.0f / (float)sq;
+        int su = idx % sq, sv = idx / sq;
+        float a = ((float)su + random_float()) * inv;
+        float b = ((float)sv + random_float()) * inv;
+        p = (q + a * u) + b * v;
+        nrm = n;
+        ar = area;
+    }
+};
+
+// InfinitePlane.cs:18-80
+struct Plane : Hittable {
+    V3 p, n, ua, va;
+    const Material* mat;
+    int prim;
+    Plane(V3 point, V3 normal, const Material* m, int id) : p(point), mat(m), prim(id) {
+        n = normalize(normal);
+        ua = std::fabs(n.y) < 0.99f ? normalize(cross(n, V3(0, 1, 0))) : V3(1, 0, 0);
+        va = cross(n, ua);
+    }
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override {
+        t_prim_tests++;
+        float denom = dot(n, r.d);
+        if (std::fabs(denom) < 1e-6f) return false;
+        float t = dot(p - r.o, n) / denom;
+        if (t < tmin || t > tmax) return false;
+        rec.t = t;
+        rec.point = r.at(t);
+        rec.set_face_normal(r, n);
+        V3 lp = rec.point - p;
+        rec.u = dot(lp, ua);
+        rec.v = dot(lp, va);
+        rec.u -= std::floor(rec.u);
+        rec.v -= std::floor(rec.v);
+        rec.tangent = ua;
+        rec.bitangent = va;
+        rec.mat = mat;
+        rec.prim = prim;
+        rec.camera_invisible = false;
+        return true;
+    }
+    AABB box() const override { return AABB(V3(-1e6f), V3(1e6f)); }
+    bool is_plane() const override { return true; }
+};
+
+// CameraInvisibleHittable: stamps the camera bit of the visibility mask.
+struct CameraInvisible : Hittable {
+    Hittable* inner;
+    explicit CameraInvisible(Hittable* h) : inner(h) {}
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override {
+        if (!inner->hit(r, tmin, tmax, rec)) return false;
+        rec.camera_invisible = true;
+        return true;
+    }
+    AABB box() const override { return inner->box(); }
+    bool is_plane() const override { return inner->is_plane(); }
+};
+
+// HittableList.cs:19-38
+struct HittableList : Hittable {
+    std::vector<Hittable*> objs;
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override {
+        HitRecord tmp;
+        bool any = false;
+        float closest = tmax;
+        for (Hittable* o : objs) {
+            if (o->hit(r, tmin, closest, tmp)) {
+                any = true;
+                closest = tmp.t;
+                rec = tmp;
+            }
+        }
+        return any;
+    }
+    AABB box() const override {
+        if (objs.empty()) return AABB::empty();
+        AABB b = objs[0]->box();
+        for (size_t i = 1; i < objs.size(); i++) b = surround(b, objs[i]->box());
+        return b;
+    }
+};
+
+// BvhNode.cs:45-371
+struct BvhNode : Hittable {
+    BvhNode* left = nullptr;
+    BvhNode* right = nullptr;
+    std::vector<Hittable*> prims;
+    AABB bbox;
+    int axis = 0;
+
+    static constexpr int kMaxLeaf = 4, kBins = 16;
+    static constexpr float kTrav = 0.5f;
+
+    static bool try_sah(std::vector<Hittable*>& o, std::vector<AABB>& bx, int start, int end, int& best_axis, int& mid) {
+        int span = end - start;
+        V3 cmin(FLT_MAX), cmax(-FLT_MAX);
+        for (int i = start; i < end; i++) {
+            V3 c = (bx[i].mn + bx[i].mx) * 0.5f;
+            cmin = vmin(cmin, c);
+            cmax = vmax(cmax, c);
+        }
+        V3 ext = cmax - cmin;
+        float max_ext = fmax_p(ext.x, fmax_p(ext.y, ext.z));
+        if (max_ext <= 0.0f) { best_axis = -1; mid = start; return false; }
+        best_axis = -1;
+        int best_bin = -1;
+        float best_cost = FLT_MAX;
+        int cnt[kBins], pc[kBins], sc[kBins];
+        AABB bb[kBins], pb[kBins], sb[kBins];
+        for (int axis = 0; axis < 3; axis++) {
+            float ea = ext[axis];
+            if (ea <= 0.0f) continue;
+            float cm = cmin[axis];
+            float scale = (float)kBins / ea;
+            for (int i = 0; i < kBins; i++) { cnt[i] = 0; bb[i] = AABB::empty(); }
+            for (int i = start; i < end; i++) {
+                float c = ((bx[i].mn + bx[i].mx) * 0.5f)[axis];
+                int idx = (int)((c - cm) * scale);
+                if (idx < 0) idx = 0;
+                else if (idx >= kBins) idx = kBins - 1;
+                cnt[idx]++;
+                bb[idx] = surround(bb[idx], bx[i]);
+            }
+            int cum = 0;
+            AABB cb = AABB::empty();
+            for (int i = 0; i < kBins; i++) { cum += cnt[i]; cb = surround(cb, bb[i]); pc[i] = cum; pb[i] = cb; }
+            cum = 0;
+            cb = AABB::empty();
+            for (int i = kBins - 1; i >= 0; i--) { cum += cnt[i]; cb = surround(cb, bb[i]); sc[i] = cum; sb[i] = cb; }
+            float parea = pb[kBins - 1].area();
+            if (parea <= 0.0f) continue;
+            for (int i = 0; i < kBins - 1; i++) {
+                int nl = pc[i], nr = sc[i + 1];
+                if (nl == 0 || nr == 0) continue;
+                float cost = kTrav + ((float)nl * pb[i].area() + (float)nr * sb[i + 1].area()) / parea;
+                if (cost < best_cost) { best_cost = cost; best_axis = axis; best_bin = i; }
+            }
+        }
+        if (best_axis < 0) { mid = start; return false; }
+        float amin = cmin[best_axis];
+        float ascale = (float)kBins / ext[best_axis];
+        int wp = start;
+        for (int i = start; i < end; i++) {
+            float c = ((bx[i].mn + bx[i].mx) * 0.5f)[best_axis];
+            int idx = (int)((c - amin) * ascale);
+            if (idx < 0) idx = 0;
+            else if (idx >= kBins) idx = kBins - 1;
+            if (idx <= best_bin) {
+                std::swap(o[i], o[wp]);
+                std::swap(bx[i], bx[wp]);
+                wp++;
+            }
+        }
+        mid = wp;
+        if (mid == start || mid == end) mid = start + span / 2;
+        return true;
+    }
+
+    BvhNode(std::vector<Hittable*>& o, std::vector<AABB>& bx, int start, int end) {
+        int span = end - start;
+        if (span <= kMaxLeaf) {
+            AABB b = bx[start];
+            prims.push_back(o[start]);
+            for (int i = 1; i < span; i++) { prims.push_back(o[start + i]); b = surround(b, bx[start + i]); }
+            bbox = b;
+            axis = 0;
+            return;
+        }
+        int ba, mid;
+        if (!try_sah(o, bx, start, end, ba, mid)) { ba = 0; mid = start + span / 2; }
+        axis = ba;
+        left = new BvhNode(o, bx, start, mid);
+        right = new BvhNode(o, bx, mid, end);
+        bbox = surround(left->bbox, right->bbox);
+    }
+    ~BvhNode() override { delete left; delete right; }
+
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override {
+        t_node_tests++;
+        if (!bbox.hit(r, tmin, tmax)) return false;
+        if (!prims.empty()) {
+            bool any = false;
+            float closest = tmax;
+            for (Hittable* p : prims) {
+                if (p->hit(r, tmin, closest, rec)) { any = true; closest = rec.t; }
+            }
+            return any;
+        }
+        float da = r.d[axis];
+        const BvhNode* nr = da >= 0.0f ? left : right;
+        const BvhNode* fr = da >= 0.0f ? right : left;
+        bool hn = nr->hit(r, tmin, tmax, rec);
+        bool hf = fr->hit(r, tmin, hn ? rec.t : tmax, rec);
+        return hn || hf;
+    }
+    AABB box() const override { return bbox; }
+
+    void stats(int depth, int64_t& nodes, int64_t& leaves, int& maxd) const {
+        if (!prims.empty()) { leaves++; maxd = std::max(maxd, depth); return; }
+        nodes++;
+        left->stats(depth + 1, nodes, leaves, maxd);
+        right->stats(depth + 1, nodes, leaves, maxd);
+    }
+};
+
+static Hittable* build_bvh(std::vector<Hittable*> objs) {
+    std::vector<AABB> bx(objs.size());
+    for (size_t i = 0; i < objs.size(); i++) bx[i] = objs[i]->box();
+    return new BvhNode(objs, bx, 0, (int)objs.size());
+}
+
+// Mesh.cs:75-116 — nested BVH (or list when <= 4 faces)
+struct Mesh : Hittable {
+    std::vector<Hittable*> tris;
+    Hittable* accel = nullptr;
+    void finish() {
+        if (tris.size() > 4) accel = build_bvh(tris);
+        else {
+            auto* l = new HittableList();
+            l->objs = tris;
+            accel = l;
+        }
+    }
+    ~Mesh() override { delete accel; }
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override { return accel->hit(r, tmin, tmax, rec); }
+    AABB box() const override { return accel->box(); }
+};
+
+// ═════════════════════════════════════════════════════════════════════════
+// Energy-compensation LUT (EnergyCompensationLut.cs:34-171) + microfacet
+// helpers (Microfacet.cs).
+// ═════════════════════════════════════════════════════════════════════════
+static void build_tangent_frame(V3 n, V3& t, V3& b) {
+    if (n.z < -0.999f) { t = V3(0, -1, 0); b = V3(-1, 0, 0); }
+    else {
+        float a = 1.0f / (1.0f + n.z);
+        float bb = -n.x * n.y * a;
+        t = V3(1.0f - n.x * n.x * a, bb, -n.x);
+        b = V3(bb, 1.0f - n.y * n.y * a, -n.y);
+    }
+}
+static float d_ggx_aniso(V3 h, float ax, float ay) {
+    float hx = h.x / ax, hy = h.y / ay;
+    float d = hx * hx + hy * hy + h.z * h.z;
+    return 1.0f / std::fmax(kPi * ax * ay * d * d, 1e-20f);
+}
+static float lambda_ggx_aniso(V3 w, float ax, float ay) {
+    float c2 = w.z * w.z;
+    if (c2 < 1e-14f) return 1e10f;
+    float num = (ax * w.x) * (ax * w.x) + (ay * w.y) * (ay * w.y);
+    return 0.5f * (-1.0f + std::sqrt(1.0f + num / c2));
+}
+static float g1_ggx_aniso(V3 w, float ax, float ay) { return 1.0f / (1.0f + lambda_ggx_aniso(w, ax, ay)); }
+static float smith_g1(float ndx, float alpha) {
+    float a2 = alpha * alpha;
+    float n2 = ndx * ndx;
+    float den = ndx + std::sqrt(a2 + (1.0f - a2) * n2);
+    return 2.0f * ndx / std::fmax(den, 1e-7f);
+}
+static V3 sample_ggx_vndf(V3 v, float ax, float ay, float u1, float u2) {
+    V3 vh = normalize(V3(ax * v.x, ay * v.y, v.z));
+    float l2 = vh.x * vh.x + vh.y * vh.y;
+    V3 t1 = l2 > 0.0f ? V3(-vh.y, vh.x, 0.0f) / std::sqrt(l2) : V3(1, 0, 0);
+    V3 t2 = cross(vh, t1);
+    float r = std::sqrt(u1);
+    float phi = 2.0f * kPi * u2;
+    float a = r * std::cos(phi);
+    float b = r * std::sin(phi);
+    float s = 0.5f * (1.0f + vh.z);
+    b = (1.0f - s) * std::sqrt(std::fmax(0.0f, 1.0f - a * a)) + s * b;
+    V3 nh = a * t1 + b * t2 + std::sqrt(std::fmax(0.0f, 1.0f - a * a - b * b)) * vh;
+    return normalize(V3(ax * nh.x, ay * nh.y, std::fmax(0.0f, nh.z)));
+}
+
+struct EnergyLut {
+    float E[32 * 32];
+    float Eavg[32];
+    static float mu_value(int i) { return ((float)i + 0.5f) / 32.0f; }
+    static float alpha_value(int i) { return std::fmax(((float)i + 0.5f) / 32.0f, 0.001f); }
+    static float next_float(uint32_t& s) {
+        s ^= s << 13;
+        s ^= s >> 17;
+        s ^= s << 5;
+        return (float)(s & 0x00FFFFFFu) / (float)0x01000000u;
+    }
+    static float integrate(float mu, float alpha, uint32_t seed) {
+        float st = std::sqrt(std::fmax(0.0f, 1.0f - mu * mu));
+        V3 v(st, 0.0f, mu);
+        double sum = 0.0;
+        uint32_t state = seed == 0 ? 1u : seed;
+        int hits = 0;
+        for (int s = 0; s < 2048; s++) {
+            float u1 = next_float(state);
+            float u2 = next_float(state);
+            V3 h = sample_ggx_vndf(v, alpha, alpha, u1, u2);
+            float vdh = dot(v, h);
+            V3 l = 2.0f * vdh * h - v;
+            if (l.z <= 0.0f) continue;
+            sum += (double)g1_ggx_aniso(l, alpha, alpha);
+            hits++;
+        }
+        if (hits == 0) return 0.0f;
+        double e = sum / 2048.0;
+        e = e < 0.0 ? 0.0 : (e > 1.0 ? 1.0 : e);
+        return (float)e;
+    }
+    EnergyLut() {
+        for (int ai = 0; ai < 32; ai++) {
+            float alpha = alpha_value(ai);
+            for (int mi = 0; mi < 32; mi++) {
+                float mu = mu_value(mi);
+                uint32_t seed = ((uint32_t)mi * 0x9E3779B9u) ^ ((uint32_t)ai * 0xB5297A4Du) ^ 0xC2B2AE3Du;
+                E[mi * 32 + ai] = integrate(mu, alpha, seed);
+            }
+            float avg = 0.0f;
+            for (int mi = 0; mi < 32; mi++) avg += 2.0f * E[mi * 32 + ai] * mu_value(mi);
+            Eavg[ai] = clampf(avg / 32.0f, 0.0f, 1.0f);
+        }
+    }
+    float sample_e(float mu, float alpha) const {
+        mu = clampf(mu, 0.0f, 1.0f);
+        alpha = clampf(alpha, 0.001f, 1.0f);
+        float fmu = mu * 32.0f - 0.5f;
+        float fal = alpha * 32.0f - 0.5f;
+        int mi0 = std::min(std::max((int)std::floor(fmu), 0), 31);
+        int ai0 = std::min(std::max((int)std::floor(fal), 0), 31);
+        int mi1 = std::min(mi0 + 1, 31);
+        int ai1 = std::min(ai0 + 1, 31);
+        float tu = clampf(fmu - (float)mi0, 0.0f, 1.0f);
+        float ta = clampf(fal - (float)ai0, 0.0f, 1.0f);
+        float e00 = E[mi0 * 32 + ai0], e01 = E[mi0 * 32 + ai1];
+        float e10 = E[mi1 * 32 + ai0], e11 = E[mi1 * 32 + ai1];
+        float e0 = e00 + (e01 - e00) * ta;
+        float e1 = e10 + (e11 - e10) * ta;
+        return e0 + (e1 - e0) * tu;
+    }
+    float sample_eavg(float alpha) const {
+        alpha = clampf(alpha, 0.001f, 1.0f);
+        float fal = alpha * 32.0f - 0.5f;
+        int ai0 = std::min(std::max((int)std::floor(fal), 0), 31);
+        int ai1 = std::min(ai0 + 1, 31);
+        float ta = clampf(fal - (float)ai0, 0.0f, 1.0f);
+        return Eavg[ai0] + (Eavg[ai1] - Eavg[ai0]) * ta;
+    }
+};
+static const EnergyLut& energy_lut() {
+    static EnergyLut lut;
+    return lut;
+}
+
+// SheenCharlie.cs
+static float charlie_d(float ndh, float alpha) {
+    alpha = std::fmax(alpha, 0.04f);
+    float inv = 1.0f / alpha;
+    float c2 = ndh * ndh;
+    float s2 = std::fmax(1.0f - c2, 0.0f);
+    float sp = std::pow(s2, 0.5f * inv);
+    return (2.0f + inv) * sp / (2.0f * kPi);
+}
+static float charlie_lambda_log(float c, float alpha) {
+    float a, b, cc, d, e;
+    if (alpha < 0.25f) { a = 25.3245f; b = 3.32435f; cc = 0.16801f; d = -1.27393f; e = -4.85967f; }
+    else { a = 21.5473f; b = 3.82987f; cc = 0.19823f; d = -1.97760f; e = -4.32054f; }
+    return a / (1.0f + b * std::pow(c, cc)) + d * c + e;
+}
+static float charlie_lambda(float c, float alpha) {
+    c = clampf(c, 1e-4f, 1.0f);
+    if (alpha <= 0.25f - 0.05f) return std::exp(charlie_lambda_log(c, 0.0f));
+    if (alpha >= 0.25f + 0.05f) return std::exp(charlie_lambda_log(c, 1.0f));
+    float t = (alpha - (0.25f - 0.05f)) / (2.0f * 0.05f);
+    float lo = std::exp(charlie_lambda_log(c, 0.0f));
+    float hi = std::exp(charlie_lambda_log(c, 1.0f));
+    return lo + (hi - lo) * t;
+}
+static float charlie_brdf(float ndv, float ndl, float ndh, float alpha) {
+    float den = 4.0f * std::fmax(ndv, 1e-6f) * std::fmax(ndl, 1e-6f);
+    float g = 1.0f / ((1.0f + charlie_lambda(ndv, alpha)) * (1.0f + charlie_lambda(ndl, alpha)));
+    return charlie_d(ndh, alpha) * g / den;
+}
+
+// ═════════════════════════════════════════════════════════════════════════
+// Materials (IMaterial.cs)
+// ═════════════════════════════════════════════════════════════════════════
+struct BsdfSample {
+    V3 wo, f;
+    float pdf = 0;
+    bool is_delta = false;
+    bool has_next_absorption = false;
+    V3 next_absorption;
+    int transition = 0;  // 0 none, 1 enter, 2 exit
+    bool caustic_caster = true;
+};
+
+struct Material {
+    virtual ~Material() {}
+    virtual bool needs_direct_lighting() const { return true; }
+    virtual bool is_delta_scatter() const { return false; }
+    virtual V3 evaluate_direct(V3, V3, V3, const HitRecord&) const { return V3(0.0f); }
+    virtual V3 shadow_transmittance(V3, const HitRecord&) const { return V3(0.0f); }
+    virtual V3 aov_albedo(const HitRecord&) const { return V3(1.0f); }
+    virtual V3 shadow_absorption(const HitRecord&) const { return V3(0.0f); }
+    virtual bool is_specular_transmissive(const HitRecord&) const { return false; }
+    virtual bool try_dielectric_ior(const HitRecord&, float& ior) const { ior = 1.0f; return false; }
+    virtual V3 evaluate(V3, V3, const HitRecord&) const { return V3(0.0f); }
+    virtual float pdf(V3, V3, const HitRecord&) const { return 0.0f; }
+    virtual bool sample(V3, const HitRecord&, BsdfSample&) const { return false; }
+    virtual V3 emit(bool) const { return V3(0.0f); }
+    virtual bool scatter(const Ray&, const HitRecord&, V3& att, Ray& scattered) const = 0;
+    virtual bool is_emissive() const { return false; }
+    virtual bool is_light_proxy() const { return false; }
+};
+
+// Lambertian.cs:37-96
+struct Lambertian : Material {
+    V3 albedo;
+    explicit Lambertian(V3 a) : albedo(a) {}
+    V3 aov_albedo(const HitRecord&) const override { return albedo; }
+    V3 evaluate_direct(V3 to_light, V3, V3 normal, const HitRecord&) const override {
+        float ndl = std::fmax(dot(normal, to_light), 0.0f);
+        if (ndl <= 0.0f) return V3(0.0f);
+        return albedo * (ndl / kPi);
+    }
+    bool scatter(const Ray&, const HitRecord& rec, V3& att, Ray& scattered) const override {
+        V3 dir = rec.normal + random_unit_vector();
+        if (near_zero(dir)) dir = rec.normal;
+        scattered = Ray(rec.point, dir);
+        att = albedo;
+        return true;
+    }
+    V3 evaluate(V3, V3 l, const HitRecord& rec) const override {
+        if (dot(rec.normal, l) <= 0.0f) return V3(0.0f);
+        return albedo * (1.0f / kPi);
+    }
+    float pdf(V3, V3 l, const HitRecord& rec) const override {
+        float c = dot(rec.normal, l);
+        return c > 0.0f ? c * (1.0f / kPi) : 0.0f;
+    }
+    bool sample(V3, const HitRecord& rec, BsdfSample& s) const override {
+        V3 n = rec.normal;
+        V3 dir = n + random_unit_vector();
+        if (near_zero(dir)) dir = n;
+        V3 wo = normalize(dir);
+        float ndw = dot(n, wo);
+        if (ndw <= 0.0f) return false;
+        s = BsdfSample();
+        s.wo = wo;
+        s.f = albedo * (1.0f / kPi);
+        s.pdf = ndw * (1.0f / kPi);
+        s.is_delta = false;
+        return true;
+    }
+};
+
+// Emissive.cs:56-104
+struct Emissive : Material {
+    V3 color;
+    float intensity;
+    bool proxy;
+    Emissive(V3 c, float i, bool p) : color(c), intensity(std::fmax(i, 0.0f)), proxy(p) {}
+    bool needs_direct_lighting() const override { return false; }
+    V3 aov_albedo(const HitRecord&) const override { return color; }
+    bool scatter(const Ray&, const HitRecord&, V3& att, Ray&) const override { att = V3(0.0f); return false; }
+    V3 emit(bool front) const override {
+        if (!front) return V3(0.0f);
+        return color * intensity;
+    }
+    V3 emission_at() const { return color * intensity; }
+    bool is_emissive() const override { return true; }
+    bool is_light_proxy() const override { return proxy; }
+};
+
+// Dielectric.cs:46-105
+struct Dielectric : Material {
+    float ior;
+    V3 albedo;
+    Dielectric(float i, V3 a) : ior(i), albedo(a) {}
+    bool is_delta_scatter() const override { return true; }
+    V3 aov_albedo(const HitRecord&) const override { return albedo; }
+    V3 shadow_transmittance(V3 wi, const HitRecord& rec) const override {
+        float ct = std::fmin(std::fabs(dot(wi, rec.normal)), 1.0f);
+        float fr = fresnel_dielectric(ct, 1.0f / ior);
+        return (1.0f - fr) * albedo;
+    }
+    bool is_specular_transmissive(const HitRecord&) const override { return true; }
+    bool try_dielectric_ior(const HitRecord&, float& i) const override { i = ior; return true; }
+    bool scatter(const Ray& rin, const HitRecord& rec, V3& att, Ray& scattered) const override {
+        att = albedo;
+        float ri = rec.relative_eta > 0.0f ? rec.relative_eta : (rec.front_face ? (1.0f / ior) : ior);
+        V3 ud = normalize(rin.d);
+        V3 n = rec.normal;
+        float ct = std::fmin(dot(-ud, n), 1.0f);
+        float st = std::sqrt(std::fmax(0.0f, 1.0f - ct * ct));
+        bool cannot = ri * st > 1.0f;
+        V3 dir;
+        if (cannot || schlick(ct, ri) > random_float()) dir = reflect(ud, n);
+        else dir = refract(ud, n, ri);
+        V3 on = dot(dir, rec.normal) >= 0.0f ? rec.normal : -rec.normal;
+        scattered = Ray(offset_origin(rec.point, on), dir);
+        return true;
+    }
+};
+
+// DisneyBsdf.cs
+struct Disney : Material {
+    V3 base;
+    float metallic_, roughness_, specular_, specular_tint_, sheen_, sheen_tint_, sheen_roughness_;
+    float clearcoat_, clearcoat_gloss_, spec_trans_, ior_, aniso_, aniso_rot_, diff_trans_;
+    float coat_ior_, coat_roughness_, tf_thick_, tf_ior_, trans_depth_;
+    bool has_trans_color, has_coat_rough, thin_walled;
+    V3 trans_color;
+
+    // ShadingParams (DisneyBsdf.cs:213-293)
+    struct SP {
+        float metallic, roughness, specular, specular_tint, sheen, sheen_tint, sheen_roughness;
+        float clearcoat, clearcoat_gloss, spec_trans, ior, aniso, aniso_rot, diff_trans;
+        float alpha, ax, ay, cc_alpha, cc_f0, tf_thick, tf_ior;
+    } sp;
+
+    explicit Disney(const rt_material& m) {
+        base = V3(m.color[0], m.color[1], m.color[2]);
+        metallic_ = m.metallic; roughness_ = m.roughness; specular_ = m.specular;
+        specular_tint_ = m.specular_tint; sheen_ = m.sheen; sheen_tint_ = m.sheen_tint;
+        sheen_roughness_ = m.sheen_roughness; clearcoat_ = m.clearcoat; clearcoat_gloss_ = m.clearcoat_gloss;
+        spec_trans_ = m.spec_trans; ior_ = m.ior; aniso_ = m.anisotropic; aniso_rot_ = m.anisotropic_rotation;
+        diff_trans_ = m.diff_trans; coat_ior_ = m.coat_ior; coat_roughness_ = m.coat_roughness;
+        tf_thick_ = m.thin_film_thickness; tf_ior_ = m.thin_film_ior; trans_depth_ = m.transmission_depth;
+        has_trans_color = (m.flags & RT_MATF_TRANSMISSION_COLOR) != 0;
+        has_coat_rough = (m.flags & RT_MATF_COAT_ROUGHNESS) != 0;
+        thin_walled = (m.flags & RT_MATF_THIN_WALLED) != 0;
+        trans_color = V3(m.transmission_color[0], m.transmission_color[1], m.transmission_color[2]);
+        SP& s = sp;
+        s.metallic = clampf(metallic_, 0, 1);
+        s.roughness = clampf(roughness_, 0, 1);
+        s.specular = clampf(specular_, 0, 2);
+        s.specular_tint = clampf(specular_tint_, 0, 1);
+        s.sheen = clampf(sheen_, 0, 1);
+        s.sheen_tint = clampf(sheen_tint_, 0, 1);
+        s.sheen_roughness = clampf(sheen_roughness_, 0.04f, 1);
+        s.clearcoat = clampf(clearcoat_, 0, 1);
+        s.clearcoat_gloss = clampf(clearcoat_gloss_, 0, 1);
+        s.spec_trans = clampf(spec_trans_, 0, 1);
+        s.ior = std::fmax(ior_, 1.0001f);
+        s.aniso = clampf(aniso_, 0, 1);
+        s.aniso_rot = aniso_rot_ - std::floor(aniso_rot_);
+        s.diff_trans = clampf(diff_trans_, 0, 1);
+        s.alpha = std::fmax(s.roughness * s.roughness, 0.001f);
+        float aspect = std::sqrt(1.0f - 0.9f * s.aniso);
+        s.ax = std::fmax(s.alpha / aspect, 0.001f);
+        s.ay = std::fmax(s.alpha * aspect, 0.001f);
+        float cr = has_coat_rough ? coat_roughness_ : -1.0f;
+        s.cc_alpha = cr >= 0.0f ? std::fmax(cr * cr, 0.001f) : lerpf(0.1f, 0.001f, s.clearcoat_gloss);
+        float ec = std::fmax(coat_ior_, 1.0001f);
+        float r0 = (ec - 1.0f) / (ec + 1.0f);
+        s.cc_f0 = r0 * r0;
+        s.tf_thick = std::fmax(tf_thick_, 0.0f);
+        s.tf_ior = std::fmax(tf_ior_, 1.0001f);
+    }
+
+    static constexpr float kDeltaAlpha = 2.5e-3f;
+    static float schlick_weight(float c) {
+        float x = clampf(1.0f - c, 0.0f, 1.0f);
+        float x2 = x * x;
+        return x2 * x2 * x;
+    }
+    V3 compute_f0() const {
+        float r = (sp.ior - 1.0f) / (sp.ior + 1.0f);
+        float f0d = r * r;
+        float scaled = f0d * 2.0f * sp.specular;
+        float lum = luminance(base);
+        V3 tint = lum > 0.0f ? base / lum : V3(1.0f);
+        V3 diel = lerp(V3(scaled), scaled * tint, sp.specular_tint);
+        return lerp(diel, base, sp.metallic);
+    }
+    V3 eval_fresnel(float c, V3 f0) const {
+        float w = schlick_weight(c);
+        return f0 + (V3(1.0f) - f0) * w;
+    }
+    V3 multiscatter_weight(V3 f0, float ndv, float ndl) const {
+        const EnergyLut& L = energy_lut();
+        float ai = std::sqrt(sp.ax * sp.ay);
+        float eo = L.sample_e(ndv, ai), ei = L.sample_e(ndl, ai), ea = L.sample_eavg(ai);
+        float om = std::fmax(1.0f - ea, 1e-6f);
+        V3 favg = f0 + (V3(1.0f) - f0) * (1.0f / 21.0f);
+        V3 den = V3(1.0f) - favg * (1.0f - ea);
+        den = V3(std::fmax(den.x, 1e-6f), std::fmax(den.y, 1e-6f), std::fmax(den.z, 1e-6f));
+        V3 fms = favg * favg * ea / den;
+        return fms * ((1.0f - eo) * (1.0f - ei) / om);
+    }
+    V3 eval_multiscatter(V3 f0, float ndv, float ndl) const { return multiscatter_weight(f0, ndv, ndl) * (1.0f / kPi); }
+
+    struct Frame {
+        V3 t, b, n;
+        V3 to_local(V3 w) const { return V3(dot(w, t), dot(w, b), dot(w, n)); }
+        V3 to_world(V3 w) const { return w.x * t + w.y * b + w.z * n; }
+    };
+    // BuildShadingFrame (DisneyBsdf.cs:1825-1866)
+    Frame shading_frame(const HitRecord& rec) const {
+        V3 n = rec.normal, t, b;
+        if (length_sq(rec.tangent) > 1e-10f && length_sq(rec.bitangent) > 1e-10f) {
+            t = rec.tangent - dot(rec.tangent, n) * n;
+            float tl = length_sq(t);
+            if (tl > 1e-10f) { t = t / std::sqrt(tl); b = cross(n, t); }
+            else build_tangent_frame(n, t, b);
+        } else build_tangent_frame(n, t, b);
+        if (sp.aniso_rot != 0.0f) {
+            float ang = 2.0f * kPi * sp.aniso_rot;
+            float c = std::cos(ang), s = std::sin(ang);
+            V3 tr = c * t + s * b;
+            V3 br = -s * t + c * b;
+            t = tr; b = br;
+        }
+        return Frame{t, b, n};
+    }
+    Frame clearcoat_frame(const HitRecord& rec) const {
+        Frame f;
+        f.n = rec.normal;
+        build_tangent_frame(f.n, f.t, f.b);
+        return f;
+    }
+
+    struct Lobes {
+        float d, s, t, c, sh, ms, dt, total;
+        float pd() const { return d / total; }
+        float ps() const { return s / total; }
+        float pt() const { return t / total; }
+        float pc() const { return c / total; }
+        float psh() const { return sh / total; }
+        float pms() const { return ms / total; }
+        float pdt() const { return dt / total; }
+    };
+    // ComputeLobeWeights (DisneyBsdf.cs:1097-1139)
+    Lobes lobe_weights() const {
+        float dall = (1.0f - sp.metallic) * (1.0f - sp.spec_trans);
+        float dw = dall * (1.0f - sp.diff_trans);
+        float dtw = dall * sp.diff_trans;
+        float sf0;
+        if (sp.metallic > 0.5f) sf0 = luminance(base);
+        else {
+            float r = (sp.ior - 1.0f) / (sp.ior + 1.0f);
+            sf0 = r * r * 2.0f * sp.specular;
+        }
+        float floor_ = 0.1f * (1.0f - sp.spec_trans * 0.9f);
+        float sw = std::fmax(floor_, lerpf(sf0, 1.0f, sp.metallic));
+        float tw = (1.0f - sp.metallic) * sp.spec_trans;
+        float cw = sp.clearcoat * 0.04f;
+        float shw = sp.sheen * 0.25f * dall;
+        float ai = std::sqrt(sp.ax * sp.ay);
+        float ea = energy_lut().sample_eavg(ai);
+        float msw = sw * (1.0f - ea);
+        Lobes L{dw, sw, tw, cw, shw, msw, dtw, 0};
+        float sum = dw + sw + tw + cw + shw + msw + dtw;
+        L.total = sum < 1e-6f ? 1.0f : sum;
+        return L;
+    }
+    // ResolveTransmission (DisneyBsdf.cs:1016-1049)
+    void resolve_transmission(V3& tint, V3& sigma) const {
+        if (!has_trans_color) {
+            tint = V3(std::sqrt(std::fmax(base.x, 0.0f)), std::sqrt(std::fmax(base.y, 0.0f)), std::sqrt(std::fmax(base.z, 0.0f)));
+            sigma = V3(0.0f);
+            return;
+        }
+        float td = std::fmax(trans_depth_, 0.0f);
+        if (td <= 0.0f) { tint = trans_color; sigma = V3(0.0f); return; }
+        V3 c(std::fmax(trans_color.x, 1e-6f), std::fmax(trans_color.y, 1e-6f), std::fmax(trans_color.z, 1e-6f));
+        float inv = 1.0f / td;
+        sigma = V3(-std::log(c.x) * inv, -std::log(c.y) * inv, -std::log(c.z) * inv);
+        tint = V3(1.0f);
+    }
+
+    V3 aov_albedo(const HitRecord&) const override { return base; }
+
+    // EvaluateDirect (DisneyBsdf.cs:418-535)
+    V3 evaluate_direct(V3 to_light, V3 to_eye, V3 normal, const HitRecord& rec) const override {
+        float ndl = std::fmax(dot(normal, to_light), 0.0f);
+        if (ndl <= 0.0f) return V3(0.0f);
+        float ndv = std::fmax(dot(normal, to_eye), 0.001f);
+        V3 h = normalize(to_light + to_eye);
+        float ndh = std::fmax(dot(normal, h), 0.0f);
+        float vdh = std::fmax(dot(to_eye, h), 0.0f);
+        float ldh = std::fmax(dot(to_light, h), 0.0f);
+        float dw = (1.0f - sp.metallic) * (1.0f - sp.spec_trans) * (1.0f - sp.diff_trans);
+        V3 diffuse(0.0f);
+        if (dw > 0.0f) {
+            float fd90 = 0.5f + 2.0f * sp.roughness * ldh * ldh;
+            float fi = schlick_weight(ndv), fo = schlick_weight(ndl);
+            float fd = (1.0f + (fd90 - 1.0f) * fi) * (1.0f + (fd90 - 1.0f) * fo);
+            diffuse = base * fd * (dw * ndl / kPi);
+        }
+        V3 f0 = compute_f0();
+        V3 spec(0.0f);
+        if (sp.alpha > kDeltaAlpha) {
+            Frame fr = shading_frame(rec);
+            V3 vl = fr.to_local(to_eye), ll = fr.to_local(to_light), hl = fr.to_local(h);
+            float D = d_ggx_aniso(hl, sp.ax, sp.ay);
+            float G = g1_ggx_aniso(vl, sp.ax, sp.ay) * g1_ggx_aniso(ll, sp.ax, sp.ay);
+            V3 F = eval_fresnel(vdh, f0);
+            spec = D * G * F / std::fmax(4.0f * ndv, 1e-6f);
+        }
+        V3 cc(0.0f);
+        if (sp.clearcoat > 0.0f && sp.cc_alpha > kDeltaAlpha) {
+            V3 cn = rec.normal;
+            float cndv = std::fmax(dot(cn, to_eye), 0.001f);
+            float cndl = dot(cn, to_light);
+            if (cndl > 0.0f) {
+                float cndh = std::fmax(dot(cn, h), 0.0f);
+                float cvdh = std::fmax(dot(to_eye, h), 0.0f);
+                float a2 = sp.cc_alpha * sp.cc_alpha;
+                float den = cndh * cndh * (a2 - 1.0f) + 1.0f;
+                float cD = a2 / (kPi * den * den);
+                float cG = smith_g1(cndv, sp.cc_alpha) * smith_g1(cndl, sp.cc_alpha);
+                float cF = sp.cc_f0 + (1.0f - sp.cc_f0) * schlick_weight(cvdh);
+                cc = V3(sp.clearcoat * 0.25f * cD * cG * cF / std::fmax(4.0f * cndv, 1e-6f));
+            }
+        }
+        V3 sheen(0.0f);
+        if (sp.sheen > 0.0f && dw > 0.0f) {
+            float lum = luminance(base);
+            V3 tint = lum > 0.0f ? base / lum : V3(1.0f);
+            V3 scol = lerp(V3(1.0f), tint, sp.sheen_tint);
+            float sb = charlie_brdf(ndv, ndl, ndh, sp.sheen_roughness);
+            sheen = sp.sheen * sb * scol * ndl;
+        }
+        V3 ms = eval_multiscatter(f0, ndv, ndl) * ndl;
+        return diffuse + sheen + spec + ms + cc;
+    }
+
+    // ShadowTransmittance (DisneyBsdf.cs:564-585)
+    V3 shadow_transmittance(V3 wi, const HitRecord& rec) const override {
+        float st = spec_trans_;
+        if (st <= 0.0f) return V3(0.0f);
+        float ior = std::fmax(ior_, 1.0001f);
+        float eta = 1.0f / ior;
+        float ct = std::fmin(std::fabs(dot(wi, rec.normal)), 1.0f);
+        float fr = fresnel_dielectric(ct, eta);
+        V3 tint, sig;
+        resolve_transmission(tint, sig);
+        return st * (1.0f - fr) * tint;
+    }
+    bool is_specular_transmissive(const HitRecord&) const override {
+        if (thin_walled) return false;
+        return spec_trans_ >= 0.5f && roughness_ <= 0.05f;
+    }
+    V3 shadow_absorption(const HitRecord&) const override {
+        V3 tint, sig;
+        resolve_transmission(tint, sig);
+        return sig;
+    }
+    bool try_dielectric_ior(const HitRecord&, float& i) const override {
+        if (thin_walled || spec_trans_ <= 0.0f) { i = 1.0f; return false; }
+        i = std::fmax(ior_, 1.0001f);
+        return true;
+    }
+
+    // Evaluate (DisneyBsdf.cs:649-771)
+    V3 evaluate(V3 V, V3 L, const HitRecord& rec) const override {
+        V3 N = rec.normal;
+        float ndl = dot(N, L), ndv = dot(N, V);
+        if (ndv <= 0.0f) return V3(0.0f);
+        if (ndl <= 0.0f) {
+            float dall = (1.0f - sp.metallic) * (1.0f - sp.spec_trans);
+            if (dall <= 0.0f || sp.diff_trans <= 0.0f) return V3(0.0f);
+            return base * (dall * sp.diff_trans / kPi);
+        }
+        V3 hr = V + L;
+        float hl2 = length_sq(hr);
+        if (hl2 < 1e-14f) return V3(0.0f);
+        V3 H = hr / std::sqrt(hl2);
+        float ndh = std::fmax(dot(N, H), 0.0f);
+        float vdh = std::fmax(dot(V, H), 0.0f);
+        float ldh = std::fmax(dot(L, H), 0.0f);
+        float ds = (1.0f - sp.metallic) * (1.0f - sp.spec_trans) * (1.0f - sp.diff_trans);
+        V3 diffuse(0.0f);
+        if (ds > 0.0f) {
+            float fd90 = 0.5f + 2.0f * sp.roughness * ldh * ldh;
+            float fi = schlick_weight(ndv), fo = schlick_weight(ndl);
+            float fd = (1.0f + (fd90 - 1.0f) * fi) * (1.0f + (fd90 - 1.0f) * fo);
+            diffuse = base * fd * (ds / kPi);
+        }
+        V3 sheen(0.0f);
+        if (sp.sheen > 0.0f && ds > 0.0f) {
+            float lum = luminance(base);
+            V3 tint = lum > 0.0f ? base / lum : V3(1.0f);
+            V3 scol = lerp(V3(1.0f), tint, sp.sheen_tint);
+            float sb = charlie_brdf(ndv, ndl, ndh, sp.sheen_roughness);
+            sheen = sp.sheen * sb * scol;
+        }
+        Frame fr = shading_frame(rec);
+        V3 vl = fr.to_local(V), ll = fr.to_local(L), hlc = fr.to_local(H);
+        float D = d_ggx_aniso(hlc, sp.ax, sp.ay);
+        float G = g1_ggx_aniso(vl, sp.ax, sp.ay) * g1_ggx_aniso(ll, sp.ax, sp.ay);
+        V3 f0 = compute_f0();
+        V3 F = eval_fresnel(vdh, f0);
+        V3 spec = D * G / std::fmax(4.0f * ndv * ndl, 1e-7f) * F;
+        V3 cc(0.0f);
+        if (sp.clearcoat > 0.0f) {
+            V3 cn = rec.normal;
+            float cndv = dot(cn, V), cndl = dot(cn, L);
+            if (cndv > 0.0f && cndl > 0.0f) {
+                float cndh = std::fmax(dot(cn, H), 0.0f);
+                float cvdh = std::fmax(dot(V, H), 0.0f);
+                float a2 = sp.cc_alpha * sp.cc_alpha;
+                float den = cndh * cndh * (a2 - 1.0f) + 1.0f;
+                float cD = a2 / (kPi * den * den);
+                float cG = smith_g1(cndv, sp.cc_alpha) * smith_g1(cndl, sp.cc_alpha);
+                float cF = sp.cc_f0 + (1.0f - sp.cc_f0) * schlick_weight(cvdh);
+                float c = sp.clearcoat * 0.25f * cD * cG * cF / std::fmax(4.0f * cndv * cndl, 1e-7f);
+                cc = V3(c);
+            }
+        }
+        V3 ms = eval_multiscatter(f0, ndv, ndl);
+        return diffuse + sheen + spec + ms + cc;
+    }
+
+    // Pdf (DisneyBsdf.cs:783-870)
+    float pdf(V3 V, V3 L, const HitRecord& rec) const override {
+        V3 N = rec.normal;
+        float ndl = dot(N, L), ndv = dot(N, V);
+        if (ndv <= 0.0f) return 0.0f;
+        Lobes w = lobe_weights();
+        float cos_pdf = ndl > 0.0f ? ndl / kPi : 0.0f;
+        float dt_pdf = ndl < 0.0f ? -ndl / kPi : 0.0f;
+        float spec_pdf = 0.0f, cc_pdf = 0.0f;
+        V3 hr = V + L;
+        float hl2 = length_sq(hr);
+        if (hl2 >= 1e-14f) {
+            V3 H = hr / std::sqrt(hl2);
+            float ndh = dot(N, H);
+            if (ndh > 0.0f && sp.alpha > kDeltaAlpha) {
+                float sndv = std::fmax(ndv, 1e-7f);
+                Frame fr = shading_frame(rec);
+                V3 vl = fr.to_local(V), hlc = fr.to_local(H);
+                float D = d_ggx_aniso(hlc, sp.ax, sp.ay);
+                float g1v = g1_ggx_aniso(vl, sp.ax, sp.ay);
+                spec_pdf = g1v * D / (4.0f * sndv);
+            }
+            if (sp.cc_alpha > kDeltaAlpha) {
+                V3 cn = rec.normal;
+                float cndv = dot(cn, V), cndh = dot(cn, H);
+                if (cndv > 0.0f && cndh > 0.0f) {
+                    float s = std::fmax(cndv, 1e-7f);
+                    float a2 = sp.cc_alpha * sp.cc_alpha;
+                    float den = cndh * cndh * (a2 - 1.0f) + 1.0f;
+                    float cD = a2 / (kPi * den * den);
+                    float g = smith_g1(cndv, sp.cc_alpha);
+                    cc_pdf = g * cD / (4.0f * s);
+                }
+            }
+        }
+        return w.pd() * cos_pdf + w.psh() * cos_pdf + w.pms() * cos_pdf + w.ps() * spec_pdf + w.pc() * cc_pdf +
+               w.pdt() * dt_pdf;
+    }
+
+    enum Lobe { kDiffuse, kSpecular, kTransmission, kSheen, kMultiscatter, kDiffTrans, kClearcoat };
+
+    // Lobe samplers (DisneyBsdf.cs:1263-1709)
+    bool scatter_diffuse(const HitRecord& rec, V3 N, V3 V, float prob, V3& att, V3& dir) const {
+        V3 sd = N + random_unit_vector();
+        if (near_zero(sd)) sd = N;
+        sd = normalize(sd);
+        dir = sd;
+        V3 L = sd;
+        float ndv = std::fmax(dot(N, V), 0.001f), ndl = std::fmax(dot(N, L), 0.001f);
+        V3 hr = V + L;
+        float hl2 = length_sq(hr);
+        V3 H = hl2 > 1e-7f ? hr / std::sqrt(hl2) : N;
+        float ldh = std::fmax(dot(L, H), 0.0f);
+        float fd90 = 0.5f + 2.0f * sp.roughness * ldh * ldh;
+        float fi = schlick_weight(ndv), fo = schlick_weight(ndl);
+        float fd = (1.0f + (fd90 - 1.0f) * fi) * (1.0f + (fd90 - 1.0f) * fo);
+        att = base * fd;
+        att *= (1.0f - sp.diff_trans);
+        att /= std::fmax(prob, 0.1f);
+        (void)rec;
+        return true;
+    }
+    bool scatter_diff_trans(V3 N, float prob, V3& att, V3& dir) const {
+        V3 sd = -N + random_unit_vector();
+        if (near_zero(sd)) sd = -N;
+        dir = normalize(sd);
+        att = base * sp.diff_trans;
+        att /= std::fmax(prob, 0.05f);
+        return true;
+    }
+    bool scatter_sheen(V3 N, V3 V, float prob, V3& att, V3& dir) const {
+        V3 sd = N + random_unit_vector();
+        if (near_zero(sd)) sd = N;
+        sd = normalize(sd);
+        dir = sd;
+        V3 L = sd;
+        float ndv = std::fmax(dot(N, V), 1e-4f), ndl = std::fmax(dot(N, L), 1e-4f);
+        V3 hr = V + L;
+        float hl2 = length_sq(hr);
+        V3 H = hl2 > 1e-7f ? hr / std::sqrt(hl2) : N;
+        float ndh = std::fmax(dot(N, H), 0.0f);
+        float sb = charlie_brdf(ndv, ndl, ndh, sp.sheen_roughness);
+        float lum = luminance(base);
+        V3 tint = lum > 0.0f ? base / lum : V3(1.0f);
+        V3 scol = lerp(V3(1.0f), tint, sp.sheen_tint);
+        att = sp.sheen * kPi * sb * scol;
+        att /= std::fmax(prob, 0.1f);
+        return true;
+    }
+    bool scatter_specular(const HitRecord& rec, V3 N, V3 V, float prob, V3& att, V3& dir) const {
+        Frame fr = shading_frame(rec);
+        V3 vl = fr.to_local(V);
+        float u1 = random_float();
+        float u2 = random_float();
+        V3 hl = sample_ggx_vndf(vl, sp.ax, sp.ay, u1, u2);
+        V3 H = fr.to_world(hl);
+        V3 L = reflect(-V, H);
+        if (dot(L, N) <= 0.0f) { att = V3(0.0f); dir = N; return false; }
+        dir = L;
+        float vdh = std::fmax(dot(V, H), 1e-4f);
+        V3 f0 = compute_f0();
+        V3 F = eval_fresnel(vdh, f0);
+        V3 ll = fr.to_local(L);
+        float g1l = g1_ggx_aniso(ll, sp.ax, sp.ay);
+        att = F * g1l;
+        att /= std::fmax(prob, 0.1f);
+        return true;
+    }
+    bool scatter_transmission(const Ray& rin, const HitRecord& rec, V3 N, V3 V, float prob, V3& att, V3& dir, bool& is_refl) const {
+        float eta = thin_walled ? (1.0f / sp.ior)
+                                : (rec.relative_eta > 0.0f ? rec.relative_eta : (rec.front_face ? (1.0f / sp.ior) : sp.ior));
+        V3 ud = normalize(rin.d);
+        Frame fr = shading_frame(rec);
+        bool rough = sp.roughness > 0.01f;
+        V3 Ht;
+        if (rough) {
+            V3 vl = fr.to_local(V);
+            float u1 = random_float();
+            float u2 = random_float();
+            V3 hl = sample_ggx_vndf(vl, sp.ax, sp.ay, u1, u2);
+            Ht = fr.to_world(hl);
+        } else Ht = N;
+        if (dot(Ht, ud) > 0.0f) Ht = -Ht;
+        float ct = std::fmin(dot(-ud, Ht), 1.0f);
+        float st = std::sqrt(std::fmax(0.0f, 1.0f - ct * ct));
+        bool cannot = !thin_walled && eta * st > 1.0f;
+        V3 d;
+        float frs = cannot ? 1.0f : fresnel_dielectric(ct, eta);
+        if (frs > random_float()) { d = reflect(ud, Ht); is_refl = true; }
+        else if (thin_walled) { d = ud; is_refl = false; }
+        else { d = refract(ud, Ht, eta); is_refl = false; }
+        if (near_zero(d)) { d = reflect(ud, N); is_refl = true; }
+        dir = normalize(d);
+        V3 col;
+        if (is_refl) col = V3(1.0f);
+        else { V3 tint, sig; resolve_transmission(tint, sig); col = tint; }
+        att = col;
+        if (rough) {
+            V3 ll = fr.to_local(dir);
+            ll.z = std::fabs(ll.z);
+            att *= g1_ggx_aniso(ll, sp.ax, sp.ay);
+        }
+        att /= std::fmax(prob, 0.1f);
+        return true;
+    }
+    bool scatter_clearcoat(const HitRecord& rec, V3 N, V3 V, float prob, V3& att, V3& dir) const {
+        Frame fr = clearcoat_frame(rec);
+        V3 cn = fr.n;
+        if (dot(cn, V) <= 0.0f) { att = V3(0.0f); dir = N; return false; }
+        V3 vl = fr.to_local(V);
+        float u1 = random_float();
+        float u2 = random_float();
+        V3 hl = sample_ggx_vndf(vl, sp.cc_alpha, sp.cc_alpha, u1, u2);
+        V3 H = fr.to_world(hl);
+        V3 L = reflect(-V, H);
+        if (dot(L, N) <= 0.0f || dot(L, cn) <= 0.0f) { att = V3(0.0f); dir = N; return false; }
+        dir = L;
+        float cndl = std::fmax(dot(cn, L), 1e-4f);
+        float cvdh = std::fmax(dot(V, H), 1e-4f);
+        float fres = sp.cc_f0 + (1.0f - sp.cc_f0) * schlick_weight(cvdh);
+        float g1l = smith_g1(cndl, sp.cc_alpha);
+        att = V3(fres * g1l);
+        att /= std::fmax(prob, 0.1f);
+        return true;
+    }
+    bool scatter_multiscatter(V3 N, V3 V, float prob, V3& att, V3& dir) const {
+        V3 sd = N + random_unit_vector();
+        if (near_zero(sd)) sd = N;
+        sd = normalize(sd);
+        dir = sd;
+        float ndv = std::fmax(dot(N, V), 1e-3f), ndl = std::fmax(dot(N, sd), 1e-3f);
+        V3 f0 = compute_f0();
+        V3 w = multiscatter_weight(f0, ndv, ndl);
+        att = w / std::fmax(prob, 0.05f);
+        return true;
+    }
+
+    // ScatterInternal (DisneyBsdf.cs:1176-1248)
+    bool scatter_internal(const Ray& rin, const HitRecord& rec, V3& att, V3& dir, Lobe& lobe, bool& is_refl) const {
+        V3 N = rec.normal;
+        V3 V = normalize(-rin.d);
+        Lobes w = lobe_weights();
+        float pD = w.pd(), pS = w.ps(), pT = w.pt(), pSh = w.psh(), pMs = w.pms(), pDt = w.pdt();
+        float rnd = random_float();
+        bool res;
+        is_refl = false;
+        if (rnd < pD) { lobe = kDiffuse; res = scatter_diffuse(rec, N, V, pD, att, dir); }
+        else if ((rnd -= pD) < pS) { lobe = kSpecular; res = scatter_specular(rec, N, V, pS, att, dir); }
+        else if ((rnd -= pS) < pT) { lobe = kTransmission; res = scatter_transmission(rin, rec, N, V, pT, att, dir, is_refl); }
+        else if ((rnd -= pT) < pSh) { lobe = kSheen; res = scatter_sheen(N, V, pSh, att, dir); }
+        else if ((rnd -= pSh) < pMs) { lobe = kMultiscatter; res = scatter_multiscatter(N, V, pMs, att, dir); }
+        else if ((rnd -= pMs) < pDt) { lobe = kDiffTrans; res = scatter_diff_trans(N, pDt, att, dir); }
+        else { lobe = kClearcoat; res = scatter_clearcoat(rec, N, V, w.pc(), att, dir); }
+        if (std::isnan(att.x) || std::isinf(att.x) || std::isnan(att.y) || std::isinf(att.y) || std::isnan(att.z) ||
+            std::isinf(att.z))
+            att = base;
+        return res;
+    }
+
+    bool scatter(const Ray& rin, const HitRecord& rec, V3& att, Ray& scattered) const override {
+        V3 dir;
+        Lobe l;
+        bool refl;
+        bool r = scatter_internal(rin, rec, att, dir, l, refl);
+        scattered = Ray(rec.point, dir);
+        return r;
+    }
+
+    // Sample (DisneyBsdf.cs:893-997)
+    bool sample(V3 V, const HitRecord& rec, BsdfSample& out) const override {
+        Ray incoming(rec.point, -V);
+        V3 att, wo;
+        Lobe lobe;
+        bool refl;
+        if (!scatter_internal(incoming, rec, att, wo, lobe, refl)) return false;
+        float ndw = dot(rec.normal, wo);
+        out = BsdfSample();
+        out.wo = wo;
+        if (lobe == kDiffTrans) {
+            V3 fb = evaluate(V, wo, rec);
+            float pb = pdf(V, wo, rec);
+            if (pb <= 0.0f) { out.f = att; out.pdf = 1.0f; out.is_delta = true; return true; }
+            out.f = fb; out.pdf = pb; out.is_delta = false;
+            return true;
+        }
+        if (lobe == kTransmission) {
+            out.f = att;
+            out.pdf = 1.0f;
+            out.is_delta = true;
+            if (ndw < 0.0f && !thin_walled) {
+                V3 tint, sigma;
+                resolve_transmission(tint, sigma);
+                bool has = sigma.x > 0.0f || sigma.y > 0.0f || sigma.z > 0.0f;
+                if (rec.front_face && has) { out.has_next_absorption = true; out.next_absorption = sigma; }
+                else if (!rec.front_face) { out.has_next_absorption = true; out.next_absorption = V3(0.0f); }
+                out.transition = rec.front_face ? 1 : 2;
+            }
+            return true;
+        }
+        if (ndw <= 0.0f) return false;
+        bool nd_spec = lobe == kSpecular && sp.alpha <= kDeltaAlpha;
+        bool nd_coat = lobe == kClearcoat && sp.cc_alpha <= kDeltaAlpha;
+        if (nd_spec || nd_coat) {
+            out.f = att;
+            out.pdf = 1.0f;
+            out.is_delta = true;
+            out.caustic_caster = sp.metallic >= 0.5f;
+            return true;
+        }
+        out.f = evaluate(V, wo, rec);
+        out.pdf = pdf(V, wo, rec);
+        out.is_delta = false;
+        return true;
+    }
+};
+
+static Material* make_material(const rt_material& m) {
+    V3 c(m.color[0], m.color[1], m.color[2]);
+    switch (m.kind) {
+        case RT_MAT_LAMBERTIAN: return new Lambertian(c);
+        case RT_MAT_EMISSIVE: return new Emissive(c, m.intensity, (m.flags & RT_MATF_LIGHT_PROXY) != 0);
+        case RT_MAT_DIELECTRIC: return new Dielectric(m.ior, c);
+        case RT_MAT_DISNEY: return new Disney(m);
+        default: return nullptr;
+    }
+}
+
+// ═════════════════════════════════════════════════════════════════════════
+// Transparent shadow rays (ShadowRay.cs:59-158)
+// ═════════════════════════════════════════════════════════════════════════
+static thread_local bool t_block_spec_trans = false;
+static thread_local int64_t t_shadow_segments = 0;
+
+static V3 shadow_transmittance(const Hittable* world, const Ray& ray, float tmin, float tmax) {
+    V3 thr(1.0f);
+    V3 o = ray.o, d = ray.d;
+    float rem = tmax;
+    V3 sigma(0.0f), entry;
+    bool in_med = false;
+    for (int i = 0; i < 8; i++) {
+        HitRecord rec;
+        Ray step(o, d);
+        t_shadow_segments++;
+        if (!world->hit(step, tmin, rem, rec)) return thr;
+        if (rec.mat == nullptr) return V3(0.0f);
+        if (t_block_spec_trans && rec.mat->is_specular_transmissive(rec)) return V3(0.0f);
+        if (in_med) {
+            float dd = length(rec.point - entry);
+            thr = V3(thr.x * std::exp(-sigma.x * dd), thr.y * std::exp(-sigma.y * dd), thr.z * std::exp(-sigma.z * dd));
+            in_med = false;
+            if (near_zero(thr)) return V3(0.0f);
+        }
+        V3 t = rec.mat->shadow_transmittance(d, rec);
+        if (near_zero(t)) return V3(0.0f);
+        thr *= t;
+        if (near_zero(thr)) return V3(0.0f);
+        if (rec.front_face) {
+            V3 s = rec.mat->shadow_absorption(rec);
+            if (!near_zero(s)) { in_med = true; entry = rec.point; sigma = s; }
+        }
+        V3 on = dot(d, rec.normal) >= 0.0f ? rec.normal : -rec.normal;
+        o = offset_origin(rec.point, on);
+        rem -= rec.t + 2.0f * kEpsilon;
+        if (rem <= tmin) return thr;
+    }
+    return V3(0.0f);
+}
+
+// ═════════════════════════════════════════════════════════════════════════
+// Lights (ILight.cs)
+// ═════════════════════════════════════════════════════════════════════════
+struct LightSample {
+    bool in_shadow;
+    V3 color, dir;
+    float dist;
+};
+
+struct Light {
+    int shadow_samples = 1;
+    const Material* proxy = nullptr;
+    virtual ~Light() {}
+    virtual bool is_delta() const { return true; }
+    virtual float pdf_solid_angle(V3, V3) const { return 0.0f; }
+    virtual float approximate_power(const AABB&) const = 0;
+    virtual LightSample illuminate(V3 p, V3 n, const Hittable* world, int si) const = 0;
+};
+
+// SphereLight.cs:112-271
+struct SphereLight : Light {
+    V3 c, color;
+    float r, inten;
+    int sq;
+    float inv_sq;
+    SphereLight(V3 center, float radius, V3 col, float i, int ss, const Material* px) {
+        c = center; r = std::fmax(radius, kEpsilon); color = col; inten = i;
+        shadow_samples = std::max(1, ss);
+        proxy = px;
+        sq = (int)std::ceil(std::sqrt((float)shadow_samples));
+        inv_sq = 1.0f / (float)sq;
+    }
+    float approximate_power(const AABB&) const override { return 4.0f * kPi * luminance(color) * inten; }
+    bool is_delta() const override { return false; }
+    LightSample illuminate(V3 hp, V3 sn, const Hittable* world, int si) const override {
+        V3 to_c = c - hp;
+        float d2 = length_sq(to_c);
+        if (d2 < kEpsilon * kEpsilon) return {true, V3(0.0f), V3(0, 1, 0), 0.0f};
+        float dist = std::sqrt(d2);
+        V3 w = to_c / dist;
+        float s2 = r * r / d2;
+        float ctm = s2 >= 1.0f ? -1.0f : std::sqrt(1.0f - s2);
+        float xi1, xi2;
+        if (si >= 0) {
+            int su = si % sq, sv = si / sq;
+            xi1 = ((float)su + random_float()) * inv_sq;
+            xi2 = ((float)sv + random_float()) * inv_sq;
+        } else { xi1 = random_float(); xi2 = random_float(); }
+        float ct = 1.0f - xi1 * (1.0f - ctm);
+        float st = std::sqrt(std::fmax(0.0f, 1.0f - ct * ct));
+        float phi = 2.0f * kPi * xi2;
+        V3 dl(st * std::cos(phi), st * std::sin(phi), ct);
+        V3 dw = local_to_world(dl, w);
+        V3 oc = hp - c;
+        float b = dot(oc, dw);
+        float cc = length_sq(oc) - r * r;
+        float disc = b * b - cc;
+        if (disc < 0.0f) {
+            V3 cp = c + normalize(dw) * r;
+            return {true, V3(0.0f), dw, length(cp - hp)};
+        }
+        float sd = std::sqrt(disc);
+        float t = -b - sd;
+        if (t < kEpsilon) t = -b + sd;
+        if (t < kEpsilon) return {true, V3(0.0f), dw, 0.0f};
+        V3 sp = hp + t * dw;
+        V3 ln = (sp - c) / r;
+        float cl = std::fmax(0.0f, dot(-dw, ln));
+        if (cl <= 0.0f) return {false, V3(0.0f), dw, t};
+        V3 so = offset_origin(hp, sn);
+        Ray sr(so, dw);
+        float stmax = length(sp - so) - kEpsilon;
+        V3 tr = shadow_transmittance(world, sr, kEpsilon, stmax);
+        if (near_zero(tr)) return {true, V3(0.0f), dw, t};
+        float sa = 2.0f * kPi * (1.0f - ctm);
+        float att = inten * sa / (float)shadow_samples;
+        return {false, color * att * tr, dw, t};
+    }
+    float pdf_solid_angle(V3 hp, V3 wi) const override {
+        V3 to_c = c - hp;
+        float d2 = length_sq(to_c);
+        float r2 = r * r;
+        if (d2 <= r2) return 1.0f / (4.0f * kPi);
+        float s2 = r2 / d2;
+        float ctm = std::sqrt(std::fmax(0.0f, 1.0f - s2));
+        float wl = length(wi);
+        if (wl < kEpsilon) return 0.0f;
+        float dist = std::sqrt(d2);
+        V3 w = to_c / dist;
+        float ct = dot(wi, w) / wl;
+        if (ct < ctm) return 0.0f;
+        float sa = 2.0f * kPi * (1.0f - ctm);
+        return sa > kEpsilon ? 1.0f / sa : 0.0f;
+    }
+};
+
+// AreaLight.cs:97-226
+struct AreaLight : Light {
+    V3 corner, u, v, color, n;
+    float inten, soft, area;
+    int sq;
+    float inv_sq;
+    AreaLight(V3 c, V3 U, V3 V, V3 col, float i, int ss, float sr, const Material* px) {
+        corner = c; u = U; v = V; color = col; inten = i;
+        shadow_samples = std::max(1, ss);
+        soft = std::fmax(0.0f, sr);
+        proxy = px;
+        V3 cr = cross(u, v);
+        area = length(cr);
+        n = cr / area;
+        sq = (int)std::ceil(std::sqrt((float)shadow_samples));
+        inv_sq = 1.0f / (float)sq;
+    }
+    bool is_delta() const override { return false; }
+    float approximate_power(const AABB&) const override { return kPi * area * luminance(color) * inten; }
+    LightSample illuminate(V3 hp, V3 sn, const Hittable* world, int si) const override {
+        V3 sp;
+        if (si >= 0) {
+            int su = si % sq, sv = si / sq;
+            float ru = ((float)su + random_float()) * inv_sq;
+            float rv = ((float)sv + random_float()) * inv_sq;
+            sp = (corner + ru * u) + rv * v;
+        } else {
+            float a = random_float();
+            float b = random_float();
+            sp = (corner + a * u) + b * v;
+        }
+        V3 tl = sp - hp;
+        float d2 = length_sq(tl);
+        if (d2 < kEpsilon * kEpsilon) return {true, V3(0.0f), V3(0, 1, 0), 0.0f};
+        float dist = std::sqrt(d2);
+        V3 dl = tl / dist;
+        float cl = std::fmax(0.0f, dot(-dl, n));
+        if (cl <= 0.0f) return {false, V3(0.0f), dl, dist};
+        V3 so = offset_origin(hp, sn);
+        Ray sr(so, dl);
+        float stmax = length(sp - so) - kEpsilon;
+        V3 tr = shadow_transmittance(world, sr, kEpsilon, stmax);
+        if (near_zero(tr)) return {true, V3(0.0f), dl, dist};
+        float ad2 = d2;
+        if (soft > 0.0f) ad2 = std::fmax(ad2, soft * soft);
+        float att = inten * area * cl / (ad2 * (float)shadow_samples);
+        return {false, color * att * tr, dl, dist};
+    }
+    float pdf_solid_angle(V3 hp, V3 wi) const override {
+        float denom = dot(n, wi);
+        if (std::fabs(denom) < 1e-7f) return 0.0f;
+        float t = dot(corner - hp, n) / denom;
+        if (t <= kEpsilon) return 0.0f;
+        V3 p = hp + t * wi;
+        V3 lc = p - corner;
+        float uu = dot(u, u), vv = dot(v, v), uv = dot(u, v);
+        float det = uu * vv - uv * uv;
+        if (det < 1e-12f) return 0.0f;
+        float lu = dot(lc, u), lv = dot(lc, v);
+        float a = (lu * vv - lv * uv) / det;
+        float b = (lv * uu - lu * uv) / det;
+        if (a < 0.0f || a > 1.0f || b < 0.0f || b > 1.0f) return 0.0f;
+        float cl = std::fabs(denom);
+        if (cl < 1e-6f) return 0.0f;
+        return (t * t) / (area * cl);
+    }
+};
+
+// DirectionalLight.cs:60-181
+struct DirectionalLight : Light {
+    V3 dir, color;
+    float inten, ang_deg, cos_ang, solid;
+    int sq;
+    float inv_sq;
+    DirectionalLight(V3 d, V3 col, float i, float ang, int ss) {
+        dir = normalize(d); color = col; inten = i;
+        ang_deg = std::fmax(0.0f, ang);
+        bool disc = ang_deg > 0.0f;
+        float rad = ang_deg * kPi / 180.0f;
+        cos_ang = disc ? std::cos(rad) : 1.0f;
+        solid = disc ? 2.0f * kPi * (1.0f - cos_ang) : 0.0f;
+        shadow_samples = ss > 0 ? ss : (disc ? 4 : 1);
+        sq = (int)std::ceil(std::sqrt((float)shadow_samples));
+        inv_sq = 1.0f / (float)sq;
+    }
+    bool is_delta() const override { return ang_deg <= 0.0f; }
+    float approximate_power(const AABB& b) const override {
+        V3 e = b.mx - b.mn;
+        float rad = 0.5f * length(e);
+        float cs = kPi * rad * rad;
+        return luminance(color) * inten * cs;
+    }
+    LightSample illuminate(V3 hp, V3 sn, const Hittable* world, int si) const override {
+        V3 tl;
+        if (ang_deg > 0.0f) {
+            float xi1, xi2;
+            if (si >= 0) {
+                int su = si % sq, sv = si / sq;
+                xi1 = ((float)su + random_float()) * inv_sq;
+                xi2 = ((float)sv + random_float()) * inv_sq;
+            } else { xi1 = random_float(); xi2 = random_float(); }
+            float ct = 1.0f - xi1 * (1.0f - cos_ang);
+            float st = std::sqrt(std::fmax(0.0f, 1.0f - ct * ct));
+            float phi = 2.0f * kPi * xi2;
+            V3 w = -dir;
+            V3 ls(st * std::cos(phi), st * std::sin(phi), ct);
+            tl = local_to_world(ls, w);
+        } else tl = -dir;
+        V3 so = offset_origin(hp, sn);
+        Ray sr(so, tl);
+        V3 tr = shadow_transmittance(world, sr, kEpsilon, kInfinity);
+        if (near_zero(tr)) return {true, V3(0.0f), tl, kInfinity};
+        float e = inten / (float)shadow_samples;
+        return {false, color * e * tr, tl, kInfinity};
+    }
+    float pdf_solid_angle(V3, V3 wi) const override {
+        if (ang_deg <= 0.0f) return 0.0f;
+        if (solid <= 0.0f) return 0.0f;
+        float wl = length(wi);
+        if (wl < kEpsilon) return 0.0f;
+        float ct = dot(wi / wl, -dir);
+        if (ct < cos_ang) return 0.0f;
+        return 1.0f / solid;
+    }
+};
+
+// GeometryLight.cs:60-279 for TRIANGLE / QUAD (area sampling) and SPHERE
+// (solid-angle sampling)
+struct GeometryLight : Light {
+    const Triangle* tri = nullptr;
+    const Quad* quad = nullptr;
+    const Sphere* sph = nullptr;
+    const Emissive* em;
+    float soft;
+    int sq;
+    V3 rep_point;
+    float rep_area;
+    GeometryLight(const Hittable* g, const Emissive* m, int ss, float sr) : em(m) {
+        tri = dynamic_cast<const Triangle*>(g);
+        quad = dynamic_cast<const Quad*>(g);
+        sph = dynamic_cast<const Sphere*>(g);
+        shadow_samples = std::max(1, ss);
+        soft = std::fmax(0.0f, sr);
+        sq = (int)std::ceil(std::sqrt((float)shadow_samples));
+        AABB b = g->box();
+        rep_point = 0.5f * (b.mn + b.mx);
+        rep_area = tri ? tri->area : (quad ? quad->area : 4.0f * kPi * sph->r * sph->r);
+    }
+    bool is_delta() const override { return false; }
+    float approximate_power(const AABB&) const override { return kPi * rep_area * luminance(em->emission_at()); }
+    LightSample illuminate(V3 hp, V3 sn, const Hittable* world, int si) const override {
+        if (sph) {
+            float inv = 1.0f / (float)sq;
+            int su = si % sq, sv = si / sq;
+            float xi1 = ((float)su + random_float()) * inv;
+            float xi2 = ((float)sv + random_float()) * inv;
+            V3 sp, ln;
+            float pdf;
+            sph->sample_solid_angle(hp, xi1, xi2, sp, ln, pdf);
+            if (pdf <= 0.0f) return {true, V3(0.0f), V3(0, 1, 0), 0.0f};
+            V3 tl = sp - hp;
+            float d2 = length_sq(tl);
+            if (d2 < kEpsilon * kEpsilon) return {true, V3(0.0f), V3(0, 1, 0), 0.0f};
+            float dist = std::sqrt(d2);
+            V3 dl = tl / dist;
+            float cl = std::fmax(0.0f, dot(-dl, ln));
+            if (cl <= 0.0f) return {false, V3(0.0f), dl, dist};
+            V3 so = offset_origin(hp, sn);
+            Ray sr(so, dl);
+            float stmax = length(sp - so) - kEpsilon;
+            V3 tr = shadow_transmittance(world, sr, kEpsilon, stmax);
+            if (near_zero(tr)) return {true, V3(0.0f), dl, dist};
+            float att = 1.0f / (pdf * (float)shadow_samples);
+            return {false, em->emission_at() * att * tr, dl, dist};
+        }
+        V3 sp, ln;
+        float ar;
+        if (tri) tri->sample_stratified(si, sq, sp, ln, ar);
+        else quad->sample_stratified(si, sq, sp, ln, ar);
+        V3 tl = sp - hp;
+        float d2 = length_sq(tl);
+        if (d2 < kEpsilon * kEpsilon) return {true, V3(0.0f), V3(0, 1, 0), 0.0f};
+        float dist = std::sqrt(d2);
+        V3 dl = tl / dist;
+        float cl = std::fmax(0.0f, dot(-dl, ln));
+        if (cl <= 0.0f) return {true, V3(0.0f), dl, dist};
+        V3 so = offset_origin(hp, sn);
+        Ray sr(so, dl);
+        float stmax = length(sp - so) - kEpsilon;
+        V3 tr = shadow_transmittance(world, sr, kEpsilon, stmax);
+        if (near_zero(tr)) return {true, V3(0.0f), dl, dist};
+        float ad2 = d2;
+        if (soft > 0.0f) ad2 = std::fmax(ad2, soft * soft);
+        float att = ar * cl / (ad2 * (float)shadow_samples);
+        return {false, em->emission_at() * att * tr, dl, dist};
+    }
+    float pdf_solid_angle(V3 hp, V3 wi) const override {
+        if (sph) return sph->solid_angle_pdf(hp, wi);
+        const Hittable* h = tri ? (const Hittable*)tri : (const Hittable*)quad;
+        Ray r(hp, wi);
+        HitRecord rec;
+        if (!h->hit(r, kEpsilon, kInfinity, rec)) return 0.0f;
+        float cl = std::fabs(dot(-wi, rec.normal));
+        if (cl < 1e-6f) return 0.0f;
+        float d2 = rec.t * rec.t;
+        if (rep_area < 1e-12f) return 0.0f;
+        return d2 / (rep_area * cl);
+    }
+};
+
+// LightDistribution.cs:41-135
+struct LightDist {
+    std::vector<float> cdf, pdf;
+    LightDist(const std::vector<Light*>& lights, const AABB& b, bool uniform) {
+        int n = (int)lights.size();
+        cdf.assign(n, 0.0f);
+        pdf.assign(n, 0.0f);
+        if (n == 0) return;
+        std::vector<float> pw(n, 0.0f);
+        float sum = 0.0f;
+        if (!uniform)
+            for (int i = 0; i < n; i++) { pw[i] = std::fmax(0.0f, lights[i]->approximate_power(b)); sum += pw[i]; }
+        if (sum > 0.0f) {
+            float inv = 1.0f / sum, cum = 0.0f;
+            for (int i = 0; i < n; i++) { pdf[i] = pw[i] * inv; cum += pdf[i]; cdf[i] = cum; }
+        } else {
+            float u = 1.0f / (float)n, cum = 0.0f;
+            for (int i = 0; i < n; i++) { pdf[i] = u; cum += u; cdf[i] = cum; }
+        }
+        cdf[n - 1] = 1.0f;
+    }
+    void sample(float xi, int& idx, float& p) const {
+        int n = (int)cdf.size();
+        if (n == 0) { idx = 0; p = 1.0f; return; }
+        if (n == 1) { idx = 0; p = pdf[0]; return; }
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            if (cdf[mid] < xi) lo = mid + 1;
+            else hi = mid;
+        }
+        idx = lo;
+        p = pdf[lo];
+    }
+};
+
+// ═════════════════════════════════════════════════════════════════════════
+// Scene (SceneLoader.cs:282-553 world assembly) + camera (Camera.cs:26-106)
+// ═════════════════════════════════════════════════════════════════════════
+struct Camera {
+    V3 origin, llc, horiz, vert, u, v;
+    float lens_radius;
+    Camera() {}
+    explicit Camera(const rt_camera& c) {
+        V3 lf(c.look_from[0], c.look_from[1], c.look_from[2]);
+        V3 la(c.look_at[0], c.look_at[1], c.look_at[2]);
+        V3 vup(c.vup[0], c.vup[1], c.vup[2]);
+        float theta = c.vfov_deg * kPi / 180.0f;
+        float h = std::tan(theta / 2.0f);
+        float vh = 2.0f * h;
+        float vw = c.aspect * vh;
+        V3 w = normalize(lf - la);
+        if (length_sq(cross(w, vup)) < 1e-8f) vup = std::fabs(w.x) < 0.9f ? V3(1, 0, 0) : V3(0, 0, 1);
+        u = normalize(cross(vup, w));
+        v = cross(w, u);
+        origin = lf;
+        horiz = c.focus_dist * vw * u;
+        vert = c.focus_dist * vh * v;
+        llc = origin - horiz / 2.0f - vert / 2.0f - c.focus_dist * w;
+        lens_radius = c.aperture / 2.0f;
+    }
+    // GetRayWithDifferentials without motion (Camera.cs:127-153); the
+    // differentials only feed texture filtering and are not carried.
+    Ray get_ray(float s, float t) const {
+        V3 target = llc + s * horiz + t * vert;
+        if (lens_radius == 0.0f) return Ray(origin, target - origin);
+        V3 rd = lens_radius * random_in_unit_disk();
+        V3 off = u * rd.x + v * rd.y;
+        V3 lo = origin + off;
+        return Ray(lo, target - lo);
+    }
+};
+
+struct Scene {
+    std::vector<std::unique_ptr<Material>> materials;
+    std::vector<Hittable*> owned;        // every allocated Hittable (deleted at the end)
+    std::vector<Hittable*> prim_objs;    // per primitive id
+    std::vector<Hittable*> objects;      // SceneLoader object list
+    Hittable* bvh = nullptr;             // BvhNode over finite objects (or null)
+    HittableList* world = nullptr;
+    std::vector<std::unique_ptr<Light>> lights;
+    std::vector<Light*> light_ptrs;
+    Camera camera;
+    V3 sky;
+    bool medium = false;
+    V3 sig_a, sig_s, sig_t;
+    AABB bounds;
+    float scene_radius = 0, mean_irr = 0, total_flux = 0;
+    int rr_min_bounces = 3;
+    float rr_min_survival = 0.1f;
+    std::vector<std::pair<const Material*, const Light*>> emitter_to_light;
+    ~Scene() {
+        delete world;
+        for (Hittable* h : owned) delete h;
+    }
+    const Light* light_for(const Material* m) const {
+        const Light* r = nullptr;
+        for (auto& p : emitter_to_light)
+            if (p.first == m) r = p.second;
+        return r;
+    }
+};
+
+}  // namespace orc
+
+namespace orc {
+
+// IorStack.cs:33-107
+struct IorStack {
+    float s[8];
+    int n = 0;
+    float top() const { return n == 0 ? 1.0f : s[n - 1]; }
+    float enclosing() const { return n >= 2 ? s[n - 2] : 1.0f; }
+    void push(float v) {
+        if (n >= 8) {
+            for (int i = 0; i < 7; i++) s[i] = s[i + 1];
+            s[7] = v;
+            return;
+        }
+        s[n++] = v;
+    }
+    void pop() { if (n > 0) n--; }
+};
+
+// Renderer.AovProbe (Renderer.cs:777-815)
+struct AovProbe {
+    bool pending = true, have_depth = false;
+    float depth = -1.0f;
+    V3 alb_w = V3(1.0f), tent_alb = V3(1.0f), tent_nrm = V3(0.0f), albedo = V3(0.0f), normal = V3(0.0f);
+    void commit() { albedo = alb_w * tent_alb; normal = tent_nrm; pending = false; }
+    void commit_as(V3 a, V3 n) { albedo = alb_w * a; normal = n; pending = false; }
+};
+static V3 clamp01(V3 v) { return V3(clampf(v.x, 0, 1), clampf(v.y, 0, 1), clampf(v.z, 0, 1)); }
+
+enum RayCat { kCatCamera = 0, kCatDiffuse, kCatGlossy, kCatTransmission };
+
+struct Renderer {
+    const Scene& sc;
+    int max_depth, spp;
+    float max_rad, ind_max_rad, exposure;
+    bool power_mis;
+    std::unique_ptr<LightDist> ldist;
+    bool capture_aovs;
+    int rr_min_b;
+    float rr_min_s;
+    AovProbe* probe = nullptr;   // per-thread (set by the render loop)
+
+    Renderer(const Scene& s, const rt_settings& st) : sc(s) {
+        max_depth = st.max_depth;
+        spp = st.samples_per_pixel;
+        max_rad = st.max_sample_radiance;
+        ind_max_rad = max_rad * std::fmax(0.0f, st.indirect_clamp_factor);
+        exposure = std::pow(2.0f, st.exposure_ev);
+        power_mis = st.mis == RT_MIS_POWER;
+        capture_aovs = st.capture_aovs != 0;
+        if (st.light_sampling != RT_LIGHTS_ALL && !s.light_ptrs.empty())
+            ldist.reset(new LightDist(s.light_ptrs, s.bounds, st.light_sampling == RT_LIGHTS_UNIFORM));
+        rr_min_b = st.rr_min_bounces >= 0 ? st.rr_min_bounces : s.rr_min_bounces;
+        rr_min_s = st.rr_min_survival >= 0.0f ? st.rr_min_survival : s.rr_min_survival;
+    }
+
+    static V3 sanitize(V3 c) {
+        if (std::isnan(c.x) || std::isinf(c.x)) c.x = 0.0f;
+        if (std::isnan(c.y) || std::isinf(c.y)) c.y = 0.0f;
+        if (std::isnan(c.z) || std::isinf(c.z)) c.z = 0.0f;
+        return vmax(c, V3(0.0f));
+    }
+    V3 clamp_radiance(V3 c) const {
+        c = sanitize(c);
+        float l = luminance(c);
+        if (l > max_rad) c *= max_rad / l;
+        return c;
+    }
+    V3 clamp_indirect(V3 c) const {
+        c = sanitize(c);
+        float l = luminance(c);
+        if (l > ind_max_rad && l > 0.0f) c *= ind_max_rad / l;
+        return c;
+    }
+    float mis_weight(float p, float q) const {
+        if (power_mis) { float p2 = p * p, q2 = q * q; return p2 / (p2 + q2 + 1e-30f); }
+        return p / (p + q + 1e-30f);
+    }
+    // WeightEmission (Renderer.cs:1549-1565)
+    V3 weight_emission(V3 raw, const Material* m, const Ray& ray, float prev_pdf, bool prev_delta) const {
+        if (prev_delta) return raw;
+        const Light* l = m->is_emissive() ? sc.light_for(m) : nullptr;
+        if (l) {
+            float pl = l->pdf_solid_angle(ray.o, ray.d);
+            if (prev_pdf + pl <= 1e-20f) return V3(0.0f);
+            return raw * mis_weight(prev_pdf, pl);
+        }
+        return raw;
+    }
+    static V3 beer_lambert(V3 r, V3 sig, float t) {
+        if (sig.x <= 0.0f && sig.y <= 0.0f && sig.z <= 0.0f) return r;
+        if (std::isinf(t) && t > 0) return V3(0.0f);
+        if (t <= 0.0f) return r;
+        return r * V3(sig.x > 0.0f ? std::exp(-sig.x * t) : 1.0f, sig.y > 0.0f ? std::exp(-sig.y * t) : 1.0f,
+                      sig.z > 0.0f ? std::exp(-sig.z * t) : 1.0f);
+    }
+    V3 medium_tr(float tmax) const {
+        float d = std::fmin(std::fmax(tmax, 0.0f), 1e30f);
+        return V3(std::exp(-sc.sig_t.x * d), std::exp(-sc.sig_t.y * d), std::exp(-sc.sig_t.z * d));
+    }
+    // HomogeneousMedium.Sample (HomogeneousMedium.cs:50-76)
+    bool medium_sample(float tmax, float& t, V3& beta) const {
+        int ch = (int)(random_float() * 3.0f);
+        if (ch > 2) ch = 2;
+        float st = ch == 0 ? sc.sig_t.x : (ch == 1 ? sc.sig_t.y : sc.sig_t.z);
+        if (st <= 0.0f) { t = tmax; beta = V3(1.0f); return false; }
+        float dist = -std::log(1.0f - random_float()) / st;
+        bool scat = dist < tmax;
+        t = scat ? dist : tmax;
+        V3 tr = medium_tr(t);
+        V3 dens = scat ? sc.sig_t * tr : tr;
+        float pdf = (dens.x + dens.y + dens.z) / 3.0f;
+        if (pdf < 1e-20f) pdf = 1.0f;
+        beta = scat ? (tr * sc.sig_s / pdf) : (tr / pdf);
+        return scat;
+    }
+
+    // SampleSky (Renderer.cs:1820-1861), flat sky, no environment light
+    V3 sample_sky() const {
+        V3 sky = sc.sky;
+        if (capture_aovs && probe && probe->pending) probe->commit_as(clamp01(sky), V3(0.0f));
+        return sky;
+    }
+
+    // ComputeDirectLighting (Renderer.cs:1894-2036)
+    V3 direct_lighting(const HitRecord& rec, V3 view, const Material* m) const {
+        V3 result(0.0f);
+        float view_term = shadow_terminator(rec.shading_ng, rec.normal, view);
+        auto one = [&](const Light* light, int s, float ppick, V3& acc) {
+            LightSample ls = light->illuminate(rec.point, rec.normal, sc.world, s);
+            if (ls.in_shadow) return;
+            V3 brdf = m ? m->evaluate_direct(ls.dir, view, rec.normal, rec)
+                        : V3(std::fmax(dot(rec.normal, ls.dir), 0.0f) / kPi);
+            brdf *= view_term * shadow_terminator(rec.shading_ng, rec.normal, ls.dir);
+            if (brdf == V3(0.0f)) return;
+            V3 tr(1.0f);
+            if (sc.medium) {
+                float sd = std::isinf(ls.dist) ? 1e30f : ls.dist;
+                tr = medium_tr(sd);
+            }
+            float wn = 1.0f;
+            if (!light->is_delta() && m) {
+                float pb = m->pdf(view, ls.dir, rec);
+                if (pb > 0.0f) {
+                    float pl = light->pdf_solid_angle(rec.point, ls.dir);
+                    float pn = ppick > 0.0f ? ppick * pl : pl;
+                    wn = (pn + pb > 0.0f) ? mis_weight(pn, pb) : 1.0f;
+                }
+            }
+            acc += wn * ls.color * brdf * tr;
+        };
+        if (ldist) {
+            if (sc.light_ptrs.empty()) return result;
+            float xi = random_float();
+            int li;
+            float pp;
+            ldist->sample(xi, li, pp);
+            const Light* light = sc.light_ptrs[li];
+            V3 acc(0.0f);
+            for (int s = 0; s < light->shadow_samples; s++) one(light, s, pp, acc);
+            if (pp > 0.0f) result += acc / pp;
+        } else {
+            for (const Light* light : sc.light_ptrs) {
+                V3 acc(0.0f);
+                for (int s = 0; s < light->shadow_samples; s++) one(light, s, -1.0f, acc);
+                result += acc;
+            }
+        }
+        return result;
+    }
+
+    // ComputeDirectLightingMedium (Renderer.cs:2099-2206), isotropic phase
+    V3 direct_lighting_medium(V3 p) const {
+        V3 result(0.0f);
+        V3 dn(0.0f);
+        auto one = [&](const Light* light, int s, float ppick, V3& acc) {
+            LightSample ls = light->illuminate(p, dn, sc.world, s);
+            if (ls.in_shadow) return;
+            float pv = kInv4Pi;
+            float sd = std::isinf(ls.dist) ? 1e30f : ls.dist;
+            V3 tr = medium_tr(sd);
+            float wn = 1.0f;
+            if (!light->is_delta()) {
+                float pp = kInv4Pi;
+                if (pp > 0.0f) {
+                    float pl = light->pdf_solid_angle(p, ls.dir);
+                    float pn = ppick > 0.0f ? ppick * pl : pl;
+                    if (pn + pp > 0.0f) wn = mis_weight(pn, pp);
+                }
+            }
+            acc += wn * ls.color * pv * tr;
+        };
+        if (ldist) {
+            if (sc.light_ptrs.empty()) return result;
+            float xi = random_float();
+            int li;
+            float pp;
+            ldist->sample(xi, li, pp);
+            const Light* light = sc.light_ptrs[li];
+            V3 acc(0.0f);
+            for (int s = 0; s < light->shadow_samples; s++) one(light, s, pp, acc);
+            if (pp > 0.0f) result += acc / pp;
+        } else {
+            for (const Light* light : sc.light_ptrs) {
+                V3 acc(0.0f);
+                for (int s = 0; s < light->shadow_samples; s++) one(light, s, -1.0f, acc);
+                result += acc;
+            }
+        }
+        return result;
+    }
+
+    static int classify(V3 n, V3 wo, bool delta) {
+        if (dot(n, wo) < 0.0f) return kCatTransmission;
+        return delta ? kCatGlossy : kCatDiffuse;
+    }
+
+    // ShadeSurface (Renderer.cs:1008-1228)
+    V3 shade_surface(const Ray& ray, HitRecord& rec, int depth, float prev_pdf, bool prev_delta, V3 absorption, V3 beta,
+                     IorStack& iors) const {
+        const Material* m = rec.mat;
+        rec.shading_ng = rec.normal;
+        if (capture_aovs && probe && probe->pending) {
+            probe->tent_nrm = rec.normal;
+            probe->tent_alb = m ? clamp01(m->aov_albedo(rec)) : V3(1.0f);
+        }
+        V3 emitted(0.0f);
+        if (m) {
+            V3 raw = m->emit(rec.front_face);
+            if (raw.x > 0.0f || raw.y > 0.0f || raw.z > 0.0f) emitted = weight_emission(raw, m, ray, prev_pdf, prev_delta);
+        }
+        V3 view = normalize(-ray.d);
+        float mior;
+        if (m && m->try_dielectric_ior(rec, mior) && mior > 0.0f)
+            rec.relative_eta = rec.front_face ? (iors.top() / mior) : (mior / iors.enclosing());
+        bool needs = m ? m->needs_direct_lighting() : true;
+        V3 direct = needs ? direct_lighting(rec, view, m) : V3(0.0f);
+        if (!m) return emitted + direct;
+
+        BsdfSample s;
+        if (m->sample(view, rec, s)) return shade_sample_bounce(m, rec, s, depth, emitted, direct, absorption, beta, iors);
+
+        V3 att;
+        Ray scat;
+        if (m->scatter(ray, rec, att, scat)) {
+            if (capture_aovs && probe && probe->pending) {
+                if (m->is_delta_scatter()) probe->alb_w *= clamp01(att);
+                else probe->commit();
+            }
+            V3 nt = beta * att;
+            int used = max_depth - depth;
+            if (used >= rr_min_b) {
+                float sp = fmax_p(nt.x, fmax_p(nt.y, nt.z));
+                sp = fmax_p(sp, rr_min_s);
+                sp = fmin_p(sp, 0.95f);
+                if (random_float() > sp) return emitted + direct;
+                float inv = 1.0f / sp;
+                att *= inv;
+                nt *= inv;
+            }
+            bool nd = m->is_delta_scatter();
+            int cat = classify(rec.normal, scat.d, nd);
+            IorStack ni = iors;
+            float sior;
+            if (m->try_dielectric_ior(rec, sior) && dot(scat.d, rec.normal) < 0.0f) {
+                if (rec.front_face) ni.push(sior);
+                else ni.pop();
+            }
+            V3 ind = trace_ray(scat, depth - 1, 0.0f, nd, ni, absorption, nt, cat);
+            ind = sanitize(ind);
+            V3 contrib = att * ind;
+            if (depth == max_depth) contrib = clamp_indirect(contrib);
+            return emitted + direct + contrib;
+        }
+        return emitted + direct;
+    }
+
+    // ShadeSampleBounce (Renderer.cs:1235-1383)
+    V3 shade_sample_bounce(const Material* m, const HitRecord& rec, const BsdfSample& s, int depth, V3 emitted, V3 direct,
+                           V3 absorption, V3 beta, IorStack& iors) const {
+        V3 att;
+        if (s.is_delta) att = s.f;
+        else {
+            float ndw = dot(rec.normal, s.wo);
+            float a = std::fabs(ndw);
+            if (a <= 0.0f || s.pdf <= 0.0f) return emitted + direct;
+            att = s.f * a / s.pdf;
+        }
+        if (capture_aovs && probe && probe->pending) {
+            if (s.is_delta) probe->alb_w *= clamp01(s.f);
+            else probe->commit();
+        }
+        V3 nt = beta * att;
+        int used = max_depth - depth;
+        if (used >= rr_min_b) {
+            float sp = fmax_p(nt.x, fmax_p(nt.y, nt.z));
+            sp = fmax_p(sp, rr_min_s);
+            sp = fmin_p(sp, 0.95f);
+            if (random_float() > sp) return emitted + direct;
+            float inv = 1.0f / sp;
+            att *= inv;
+            nt *= inv;
+        }
+        V3 od = dot(s.wo, rec.normal) >= 0.0f ? rec.normal : -rec.normal;
+        Ray scat(offset_origin(rec.point, od), s.wo);
+        float npdf = s.is_delta ? 0.0f : s.pdf;
+        V3 nabs = s.has_next_absorption ? s.next_absorption : absorption;
+        int cat = classify(rec.normal, s.wo, s.is_delta);
+        V3 ind;
+        if (s.transition == 1 || s.transition == 2) {
+            IorStack ni = iors;
+            if (s.transition == 1) {
+                float pi;
+                ni.push(m->try_dielectric_ior(rec, pi) ? pi : iors.top());
+            } else ni.pop();
+            ind = trace_ray(scat, depth - 1, npdf, s.is_delta, ni, nabs, nt, cat);
+        } else {
+            ind = trace_ray(scat, depth - 1, npdf, s.is_delta, iors, nabs, nt, cat);
+        }
+        ind = sanitize(ind);
+        V3 contrib = att * ind;
+        if (depth == max_depth) contrib = clamp_indirect(contrib);
+        return emitted + direct + contrib;
+    }
+
+    // TraceRay (Renderer.cs:1581-1783)
+    V3 trace_ray(const Ray& ray, int depth, float prev_pdf, bool prev_delta, IorStack& iors, V3 absorption, V3 beta,
+                 int cat) const {
+        if (depth <= 0) return V3(0.0f);
+        float bm = fmax_p(beta.x, fmax_p(beta.y, beta.z));
+        if (bm < 1e-4f) return V3(0.0f);
+        Ray cur = ray;
+        HitRecord rec;
+        bool hit;
+        int skips = 0;
+        while (true) {
+            rec = HitRecord();
+            hit = sc.world->hit(cur, kEpsilon, kInfinity, rec);
+            if (!hit || cat != kCatCamera || !rec.camera_invisible) break;
+            cur = Ray(offset_origin(rec.point, cur.d), cur.d);
+            if (++skips >= 8) break;
+        }
+        if (capture_aovs && hit && cat == kCatCamera && probe && probe->pending && !probe->have_depth) {
+            probe->depth = rec.t * length(cur.d);
+            probe->have_depth = true;
+        }
+        if (!sc.medium) {
+            V3 r = !hit ? sample_sky() : shade_surface(cur, rec, depth, prev_pdf, prev_delta, absorption, beta, iors);
+            return beer_lambert(r, absorption, hit ? rec.t : std::numeric_limits<float>::infinity());
+        }
+        float tmax = hit ? rec.t : 1e30f;
+        float tmed;
+        V3 mb;
+        bool scat = medium_sample(tmax, tmed, mb);
+        if (scat) {
+            V3 p = cur.o + cur.d * tmed;
+            if (capture_aovs && probe && probe->pending) {
+                if (!probe->have_depth) { probe->depth = tmed * length(cur.d); probe->have_depth = true; }
+                probe->commit_as(V3(1.0f), V3(0.0f));
+            }
+            V3 lnee = direct_lighting_medium(p);
+            V3 lind(0.0f);
+            V3 mt = beta * mb;
+            int used = max_depth - depth;
+            float iscale = 1.0f;
+            bool kill = false;
+            if (used >= rr_min_b) {
+                float sp = fmax_p(mt.x, fmax_p(mt.y, mt.z));
+                sp = fmax_p(sp, rr_min_s);
+                sp = fmin_p(sp, 0.95f);
+                if (random_float() > sp) kill = true;
+                else iscale = 1.0f / sp;
+            }
+            if (!kill) {
+                float u1 = random_float(), u2 = random_float();
+                float z = 1.0f - 2.0f * u1;
+                float rr = std::sqrt(std::fmax(0.0f, 1.0f - z * z));
+                float phi = 2.0f * kPi * u2;
+                V3 wi(rr * std::cos(phi), rr * std::sin(phi), z);
+                float ppdf = kInv4Pi, pval = kInv4Pi;
+                float pw = ppdf > 1e-20f ? pval / ppdf : 0.0f;
+                V3 nt = mt * (pw * iscale);
+                lind = pw * iscale * trace_ray(Ray(p, wi), depth - 1, ppdf, false, iors, absorption, nt, kCatCamera);
+            }
+            return beer_lambert(mb * (lnee + lind), absorption, tmed);
+        }
+        V3 r = !hit ? mb * sample_sky()
+                    : mb * shade_surface(cur, rec, depth, prev_pdf, prev_delta, absorption, beta * mb, iors);
+        return beer_lambert(r, absorption, hit ? rec.t : std::numeric_limits<float>::infinity());
+    }
+
+    // AcesToneMap (Renderer.cs:893-907)
+    static V3 aces(V3 c) {
+        c = vmax(c, V3(0.0f));
+        V3 a = c * (2.51f * c + V3(0.03f));
+        V3 b = c * (2.43f * c + V3(0.59f)) + V3(0.14f);
+        V3 m(a.x / b.x, a.y / b.y, a.z / b.z);
+        const float ig = 1.0f / 2.2f;
+        return V3(std::pow(clampf(m.x, 0, 1), ig), std::pow(clampf(m.y, 0, 1), ig), std::pow(clampf(m.z, 0, 1), ig));
+    }
+};
+
+// ── Scene construction ───────────────────────────────────────────────────
+static std::string g_err;
+
+static Scene* build_scene(const rt_scene_desc* d) {
+    if (!d || (d->num_primitives > 0 && !d->primitives) || (d->num_materials > 0 && !d->materials) ||
+        (d->num_lights > 0 && !d->lights)) {
+        g_err = "null table";
+        return nullptr;
+    }
+    auto sc = std::make_unique<Scene>();
+    for (int64_t i = 0; i < d->num_materials; i++) {
+        Material* m = make_material(d->materials[i]);
+        if (!m) { g_err = "bad material kind"; return nullptr; }
+        sc->materials.emplace_back(m);
+    }
+    int64_t np = d->num_primitives;
+    sc->prim_objs.assign(np, nullptr);
+    Mesh* cur_mesh = nullptr;
+    int cur_mesh_id = -1;
+    for (int64_t i = 0; i < np; i++) {
+        const rt_primitive& p = d->primitives[i];
+        if (p.material < 0 || p.material >= d->num_materials) { g_err = "bad material index"; return nullptr; }
+        const Material* m = sc->materials[p.material].get();
+        const float* v = p.v;
+        Hittable* h = nullptr;
+        switch (p.kind) {
+            case RT_PRIM_TRIANGLE: h = new Triangle(V3(v[0], v[1], v[2]), V3(v[3], v[4], v[5]), V3(v[6], v[7], v[8]), m, (int)i); break;
+            case RT_PRIM_SPHERE: h = new Sphere(V3(v[0], v[1], v[2]), v[3], m, (int)i); break;
+            case RT_PRIM_QUAD:
+                h = new Quad(V3(v[0], v[1], v[2]), V3(v[3], v[4], v[5]), V3(v[6], v[7], v[8]), m, (int)i,
+                             (p.flags & RT_PRIMF_BACKFACE_CULLED) != 0);
+                break;
+            case RT_PRIM_PLANE: h = new Plane(V3(v[0], v[1], v[2]), V3(v[3], v[4], v[5]), m, (int)i); break;
+            default: g_err = "bad primitive kind"; return nullptr;
+        }
+        sc->owned.push_back(h);
+        sc->prim_objs[i] = h;
+        Hittable* obj = h;
+        if (p.flags & RT_PRIMF_CAMERA_INVISIBLE) { obj = new CameraInvisible(h); sc->owned.push_back(obj); }
+        if (p.mesh >= 0) {
+            if (p.kind != RT_PRIM_TRIANGLE) { g_err = "mesh faces must be triangles"; return nullptr; }
+            if (p.mesh != cur_mesh_id) {
+                cur_mesh = new Mesh();
+                cur_mesh_id = p.mesh;
+                sc->owned.push_back(cur_mesh);
+                sc->objects.push_back(cur_mesh);
+            }
+            cur_mesh->tris.push_back(obj);
+        } else {
+            cur_mesh = nullptr;
+            cur_mesh_id = -1;
+            sc->objects.push_back(obj);
+        }
+    }
+    for (Hittable* o : sc->objects)
+        if (Mesh* me = dynamic_cast<Mesh*>(o)) me->finish();
+    // World assembly (SceneLoader.cs:460-487)
+    sc->world = new HittableList();
+    if (sc->objects.size() > 4) {
+        std::vector<Hittable*> fin, inf;
+        for (Hittable* o : sc->objects) (o->is_plane() ? inf : fin).push_back(o);
+        if (!fin.empty()) {
+            sc->bvh = build_bvh(fin);
+            sc->owned.push_back(sc->bvh);
+            sc->world->objs.push_back(sc->bvh);
+        }
+        for (Hittable* o : inf) sc->world->objs.push_back(o);
+    } else sc->world->objs = sc->objects;
+
+    sc->camera = Camera(d->camera);
+    sc->sky = V3(d->sky_color[0], d->sky_color[1], d->sky_color[2]);
+    if (d->global_medium.enabled) {
+        sc->medium = true;
+        const rt_medium& gm = d->global_medium;
+        sc->sig_a = vmax(V3(gm.sigma_a[0], gm.sigma_a[1], gm.sigma_a[2]), V3(0.0f));
+        sc->sig_s = vmax(V3(gm.sigma_s[0], gm.sigma_s[1], gm.sigma_s[2]), V3(0.0f));
+        sc->sig_t = sc->sig_a + sc->sig_s;
+    }
+    // Lights
+    for (int64_t i = 0; i < d->num_lights; i++) {
+        const rt_light& l = d->lights[i];
+        V3 col(l.color[0], l.color[1], l.color[2]);
+        V3 pos(l.position[0], l.position[1], l.position[2]);
+        const Material* px = (l.material >= 0 && l.material < d->num_materials) ? sc->materials[l.material].get() : nullptr;
+        Light* L = nullptr;
+        switch (l.kind) {
+            case RT_LIGHT_SPHERE: L = new SphereLight(pos, l.radius, col, l.intensity, l.shadow_samples, px); break;
+            case RT_LIGHT_AREA:
+                L = new AreaLight(pos, V3(l.u[0], l.u[1], l.u[2]), V3(l.v[0], l.v[1], l.v[2]), col, l.intensity, l.shadow_samples,
+                                  l.soft_radius, px);
+                break;
+            case RT_LIGHT_DIRECTIONAL: L = new DirectionalLight(pos, col, l.intensity, l.radius, l.shadow_samples); break;
+            case RT_LIGHT_GEOMETRY: {
+                if (l.primitive < 0 || l.primitive >= np) { g_err = "geometry light primitive"; return nullptr; }
+                const Hittable* g = sc->prim_objs[l.primitive];
+                const Material* gm = sc->materials[d->primitives[l.primitive].material].get();
+                const Emissive* em = dynamic_cast<const Emissive*>(gm);
+                if (!em || d->primitives[l.primitive].kind == RT_PRIM_PLANE) { g_err = "geometry light needs an emissive tri/quad/sphere"; return nullptr; }
+                L = new GeometryLight(g, em, l.shadow_samples, l.soft_radius);
+                L->proxy = nullptr;
+                break;
+            }
+            default: g_err = "bad light kind"; return nullptr;
+        }
+        sc->lights.emplace_back(L);
+        sc->light_ptrs.push_back(L);
+    }
+    // Emitter → light map (Renderer.cs:409-426): geometry lights first, then proxies
+    for (size_t i = 0; i < sc->lights.size(); i++) {
+        const rt_light& l = d->lights[i];
+        if (l.kind == RT_LIGHT_GEOMETRY)
+            sc->emitter_to_light.push_back({sc->materials[d->primitives[l.primitive].material].get(), sc->lights[i].get()});
+    }
+    for (size_t i = 0; i < sc->lights.size(); i++)
+        if (sc->lights[i]->proxy) sc->emitter_to_light.push_back({sc->lights[i]->proxy, sc->lights[i].get()});
+
+    // ComputeFiniteSceneBounds + classifier (Renderer.cs:361-378,476-502)
+    AABB acc = AABB::empty();
+    bool any = false;
+    for (Hittable* o : sc->world->objs) {
+        AABB b = o->box();
+        V3 dd = b.mx - b.mn;
+        if (dd.x < 1e5f && dd.y < 1e5f && dd.z < 1e5f) { acc = any ? surround(acc, b) : b; any = true; }
+    }
+    if (!any) {
+        AABB raw = sc->world->box();
+        acc = AABB(vmax(raw.mn, V3(-1e3f)), vmin(raw.mx, V3(1e3f)));
+    }
+    sc->bounds = acc;
+    float flux = 0.0f;
+    for (Light* l : sc->light_ptrs) flux += l->approximate_power(acc);
+    V3 ext = acc.mx - acc.mn;
+    float R = std::fmax(0.5f * length(ext), 1e-3f);
+    float area = 4.0f * kPi * R * R;
+    sc->total_flux = flux;
+    sc->scene_radius = R;
+    sc->mean_irr = flux / area;
+    bool ind = sc->mean_irr < 0.5f;
+    sc->rr_min_bounces = ind ? 6 : 3;
+    sc->rr_min_survival = ind ? 0.40f : 0.10f;
+    return sc.release();
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ═════════════════════════════════════════════════════════════════════════
+// C exports for the ctypes test harness
+// ═════════════════════════════════════════════════════════════════════════
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+float orc_sobol(uint32_t index, uint32_t dim, uint32_t seed) { return sobol_sample(index, dim, seed); }
+
+void orc_sobol_batch(const uint32_t* index, const uint32_t* dim, const uint32_t* seed, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; i++) out[i] = sobol_sample(index[i], dim[i], seed[i]);
+}
+
+void* orc_scene_create(const rt_scene_desc* d) { return build_scene(d); }
+void orc_scene_destroy(void* s) { delete (Scene*)s; }
+
+int orc_scene_info(void* sp, rt_scene_info* info) {
+    Scene* s = (Scene*)sp;
+    std::memset(info, 0, sizeof(*info));
+    int64_t nodes = 0, leaves = 0;
+    int maxd = 0;
+    auto walk = [&](Hittable* h) {
+        if (BvhNode* b = dynamic_cast<BvhNode*>(h)) b->stats(0, nodes, leaves, maxd);
+    };
+    if (s->bvh) walk(s->bvh);
+    for (Hittable* o : s->objects)
+        if (Mesh* m = dynamic_cast<Mesh*>(o)) walk(m->accel);
+    info->num_primitives = (int64_t)s->prim_objs.size();
+    info->num_objects = (int64_t)s->objects.size();
+    info->num_nodes = nodes;
+    info->num_leaves = leaves;
+    info->max_depth = maxd;
+    info->has_bvh = s->bvh != nullptr;
+    info->rr_min_bounces = s->rr_min_bounces;
+    info->rr_min_survival = s->rr_min_survival;
+    info->scene_radius = s->scene_radius;
+    info->mean_irradiance = s->mean_irr;
+    return 0;
+}
+
+// Pre-order dump of the top-level BVH: internal node → (0, axis, 6 box
+// floats as bits); leaf → (1, count, 6 box floats, entries...) where an
+// entry is the primitive id or -(mesh_ordinal+2) for a nested mesh.
+int64_t orc_bvh_dump(void* sp, int32_t* out, int64_t cap) {
+    Scene* s = (Scene*)sp;
+    if (!s->bvh) return 0;
+    std::vector<int32_t> v;
+    std::vector<const Hittable*> meshes;
+    for (Hittable* o : s->objects)
+        if (dynamic_cast<Mesh*>(o)) meshes.push_back(o);
+    auto box = [&](const AABB& b) {
+        float f[6] = {b.mn.x, b.mn.y, b.mn.z, b.mx.x, b.mx.y, b.mx.z};
+        for (float x : f) { int32_t i; std::memcpy(&i, &x, 4); v.push_back(i); }
+    };
+    std::function<void(const BvhNode*)> rec = [&](const BvhNode* n) {
+        if (!n->prims.empty()) {
+            v.push_back(1);
+            v.push_back((int32_t)n->prims.size());
+            box(n->bbox);
+            for (const Hittable* p : n->prims) {
+                const Hittable* q = p;
+                if (auto* ci = dynamic_cast<const CameraInvisible*>(q)) q = ci->inner;
+                int32_t id = -1;
+                if (auto* t = dynamic_cast<const Triangle*>(q)) id = t->prim;
+                else if (auto* sp2 = dynamic_cast<const Sphere*>(q)) id = sp2->prim;
+                else if (auto* qu = dynamic_cast<const Quad*>(q)) id = qu->prim;
+                else if (auto* pl = dynamic_cast<const Plane*>(q)) id = pl->prim;
+                else {
+                    for (size_t k = 0; k < meshes.size(); k++)
+                        if (meshes[k] == q) id = -(int32_t)k - 2;
+                }
+                v.push_back(id);
+            }
+            return;
+        }
+        v.push_back(0);
+        v.push_back(n->axis);
+        box(n->bbox);
+        rec(n->left);
+        rec(n->right);
+    };
+    rec((const BvhNode*)s->bvh);
+    int64_t n = (int64_t)v.size();
+    if (out && cap >= n) std::memcpy(out, v.data(), (size_t)n * 4);
+    return n;
+}
+
+}  // extern "C"
+
+namespace orc {
+
+static int check_settings(const rt_settings* st) {
+    if (!st || st->width <= 0 || st->height <= 0 || st->samples_per_pixel <= 0 || st->max_depth < 0) {
+        g_err = "invalid settings";
+        return RT_ERR_INVALID_ARGUMENT;
+    }
+    if (st->sampler != RT_SAMPLER_SOBOL) { g_err = "only the Sobol sampler is implemented"; return RT_ERR_UNSUPPORTED; }
+    if (st->caustics) { g_err = "photon caustics not implemented"; return RT_ERR_UNSUPPORTED; }
+    return 0;
+}
+
+static inline uint32_t pixel_seed(int i, int j, uint32_t x) {
+    return (((uint32_t)i * 73856093u) ^ ((uint32_t)j * 19349663u)) ^ x;
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// world.Hit(ray, 1e-4, float.MaxValue) for a batch of rays; optional
+// per-ray node-test / primitive-test counters (reference traversal order).
+int orc_trace_rays(void* sp, const float* rays, int64_t n, int32_t* prim, float* t, float* u, float* v,
+                   int64_t* node_tests, int64_t* prim_tests) {
+    Scene* s = (Scene*)sp;
+    for (int64_t i = 0; i < n; i++) {
+        const float* r = rays + 6 * i;
+        Ray ray(V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5]));
+        HitRecord rec;
+        t_node_tests = 0;
+        t_prim_tests = 0;
+        bool h = s->world->hit(ray, kEpsilon, kInfinity, rec);
+        if (prim) prim[i] = h ? rec.prim : -1;
+        if (t) t[i] = h ? rec.t : 0.0f;
+        if (u) u[i] = h ? rec.u : 0.0f;
+        if (v) v[i] = h ? rec.v : 0.0f;
+        if (node_tests) node_tests[i] = t_node_tests;
+        if (prim_tests) prim_tests[i] = t_prim_tests;
+    }
+    return 0;
+}
+
+// Camera rays of Renderer.cs:630-653 + the camera-visibility skip loop of
+// TraceRay (Renderer.cs:1609-1644).  Output index s*W*H + y*W + x.
+int orc_trace_primary(void* sp, const rt_settings* st, int32_t sbegin, int32_t scount, int32_t* prim, float* t,
+                      float* u, float* v, int64_t* node_tests, int64_t* prim_tests) {
+    Scene* s = (Scene*)sp;
+    int rc = check_settings(st);
+    if (rc) return rc;
+    int W = st->width, H = st->height;
+    int64_t np = (int64_t)W * H;
+    SamplerCtx& c = t_smp;
+    int64_t nt = 0, pt = 0;
+    for (int j = 0; j < H; j++)
+        for (int i = 0; i < W; i++) {
+            uint32_t ps = pixel_seed(i, j, st->seed_xor);
+            for (int k = 0; k < scount; k++) {
+                c.active = true; c.seed = ps; c.index = (uint32_t)(sbegin + k); c.dim = 0; c.fixed = nullptr;
+                float ju = random_float();
+                float jv = random_float();
+                float uu = ((float)i + ju) / (float)W;
+                float vv = ((float)(H - j - 1) + jv) / (float)H;
+                Ray ray = s->camera.get_ray(uu, vv);
+                HitRecord rec;
+                bool hit;
+                int skips = 0;
+                t_node_tests = 0;
+                t_prim_tests = 0;
+                while (true) {
+                    rec = HitRecord();
+                    hit = s->world->hit(ray, kEpsilon, kInfinity, rec);
+                    if (!hit || !rec.camera_invisible) break;
+                    ray = Ray(offset_origin(rec.point, ray.d), ray.d);
+                    if (++skips >= 8) break;
+                }
+                nt += t_node_tests;
+                pt += t_prim_tests;
+                int64_t o = (int64_t)k * np + (int64_t)j * W + i;
+                if (prim) prim[o] = hit ? rec.prim : -1;
+                if (t) t[o] = hit ? rec.t : 0.0f;
+                if (u) u[o] = hit ? rec.u : 0.0f;
+                if (v) v[o] = hit ? rec.v : 0.0f;
+            }
+            c.active = false;
+        }
+    if (node_tests) *node_tests = nt;
+    if (prim_tests) *prim_tests = pt;
+    return 0;
+}
+
+// Renderer.Render (Renderer.cs:522-733), Sobol mode, 16×16 tiles over
+// `nthreads` workers.  row_step > 1 renders only rows j % row_step == 0
+// (bounded CPU-baseline samples); other rows are left untouched.
+int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nthreads, int32_t row_step,
+               rt_render_stats* stats) {
+    Scene* s = (Scene*)sp;
+    int rc = check_settings(st);
+    if (rc) return rc;
+    if (row_step < 1) row_step = 1;
+    Renderer R(*s, *st);
+    const int W = st->width, H = st->height, spp = st->samples_per_pixel;
+    const int64_t np = (int64_t)W * H;
+    const int T = 16;
+    int tx = (W + T - 1) / T, ty = (H + T - 1) / T;
+    int total = tx * ty;
+    std::atomic<int> next(0);
+    std::atomic<int64_t> seg_closest(0), seg_shadow(0);
+    const int nA = (spp + 1) / 2, nB = spp / 2;
+    auto worker = [&]() {
+        AovProbe probe;
+        Renderer RR(*s, *st);
+        RR.probe = &probe;
+        t_block_spec_trans = false;
+        t_shadow_segments = 0;
+        for (;;) {
+            int tile = next.fetch_add(1);
+            if (tile >= total) break;
+            int x0 = (tile % tx) * T, y0 = (tile / tx) * T;
+            int x1 = std::min(x0 + T, W), y1 = std::min(y0 + T, H);
+            for (int j = y0; j < y1; j++) {
+                if (j % row_step != 0) continue;
+                for (int i = x0; i < x1; i++) {
+                    V3 cum(0.0f), sa(0.0f), sb(0.0f), aa(0.0f), ab(0.0f), na(0.0f), nb(0.0f);
+                    float da = 0, db = 0;
+                    int dca = 0, dcb = 0;
+                    uint32_t ps = pixel_seed(i, j, st->seed_xor);
+                    for (int smp = 0; smp < spp; smp++) {
+                        SamplerCtx& c = t_smp;
+                        c.active = true; c.seed = ps; c.index = (uint32_t)smp; c.dim = 0; c.fixed = nullptr;
+                        float ju = random_float();
+                        float jv = random_float();
+                        float uu = ((float)i + ju) / (float)W;
+                        float vv = ((float)(H - j - 1) + jv) / (float)H;
+                        Ray ray = s->camera.get_ray(uu, vv);
+                        IorStack iors;
+                        probe = AovProbe();
+                        V3 L = RR.trace_ray(ray, RR.max_depth, 0.0f, true, iors, V3(0.0f), V3(1.0f), kCatCamera);
+                        c.active = false;
+                        L = RR.clamp_radiance(L);
+                        cum += L;
+                        bool isA = (smp & 1) == 0;
+                        if (isA) sa += L; else sb += L;
+                        if (RR.capture_aovs) {
+                            if (probe.pending) probe.commit();
+                            if (isA) { aa += probe.albedo; na += probe.normal; if (probe.have_depth) { da += probe.depth; dca++; } }
+                            else { ab += probe.albedo; nb += probe.normal; if (probe.have_depth) { db += probe.depth; dcb++; } }
+                        }
+                    }
+                    V3 lin = cum / (float)spp;
+                    int64_t px = (int64_t)j * W + i;
+                    auto put3 = [&](float* p, V3 c) { if (p) { p[px] = c.x; p[np + px] = c.y; p[2 * np + px] = c.z; } };
+                    if (film) {
+                        put3(film->beauty, lin);
+                        V3 mA = sa / (float)nA;
+                        V3 mB = nB > 0 ? sb / (float)nB : mA;
+                        put3(film->beauty_a, mA);
+                        put3(film->beauty_b, mB);
+                        if (film->sample_count) film->sample_count[px] = spp;
+                        put3(film->display, Renderer::aces(lin * RR.exposure));
+                        if (RR.capture_aovs) {
+                            V3 alA = aa / (float)nA, alB = nB > 0 ? ab / (float)nB : alA;
+                            V3 nrA = na / (float)nA, nrB = nB > 0 ? nb / (float)nB : nrA;
+                            float dA = dca > 0 ? da / (float)dca : -1.0f;
+                            float dB = nB > 0 ? (dcb > 0 ? db / (float)dcb : -1.0f) : dA;
+                            put3(film->albedo_a, alA);
+                            put3(film->albedo_b, alB);
+                            put3(film->normal_a, nrA);
+                            put3(film->normal_b, nrB);
+                            if (film->depth_a) film->depth_a[px] = dA;
+                            if (film->depth_b) film->depth_b[px] = dB;
+                        }
+                    }
+                }
+            }
+        }
+        seg_shadow += t_shadow_segments;
+    };
+    if (nthreads <= 1) worker();
+    else {
+        std::vector<std::thread> th;
+        for (int k = 0; k < nthreads; k++) th.emplace_back(worker);
+        for (auto& x : th) x.join();
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        int64_t rows = (H + row_step - 1) / row_step;
+        stats->camera_samples = rows * (int64_t)W * spp;
+        stats->rays_shadow = seg_shadow.load();
+    }
+    (void)seg_closest;
+    return 0;
+}
+
+// ── Unit probes for material / light / LUT tests ─────────────────────────
+// A shading point is described by (normal, tangent, bitangent, front_face,
+// relative_eta); random draws come from the caller's `rnd` list.
+static HitRecord probe_rec(const float* frame) {
+    HitRecord rec;
+    rec.point = V3(0.0f);
+    rec.normal = V3(frame[0], frame[1], frame[2]);
+    rec.tangent = V3(frame[3], frame[4], frame[5]);
+    rec.bitangent = V3(frame[6], frame[7], frame[8]);
+    rec.front_face = frame[9] != 0.0f;
+    rec.relative_eta = frame[10];
+    rec.shading_ng = rec.normal;
+    return rec;
+}
+
+// out: f[3] = Evaluate(V,L), pdf = Pdf(V,L), fd[3] = EvaluateDirect(L,V,N)
+int orc_bsdf_eval(const rt_material* m, const float* frame, const float* V, const float* L, float* f, float* pdf,
+                  float* fd) {
+    std::unique_ptr<Material> mat(make_material(*m));
+    if (!mat) return RT_ERR_INVALID_ARGUMENT;
+    HitRecord rec = probe_rec(frame);
+    V3 v(V[0], V[1], V[2]), l(L[0], L[1], L[2]);
+    V3 e = mat->evaluate(v, l, rec);
+    f[0] = e.x; f[1] = e.y; f[2] = e.z;
+    *pdf = mat->pdf(v, l, rec);
+    V3 d = mat->evaluate_direct(l, v, rec.normal, rec);
+    fd[0] = d.x; fd[1] = d.y; fd[2] = d.z;
+    return 0;
+}
+
+// out: wo[3], f[3], pdf, flags (bit0 valid, bit1 delta, bit2 has_next_abs,
+// bits 4-5 transition, bit6 caustic caster), used = random draws consumed
+int orc_bsdf_sample(const rt_material* m, const float* frame, const float* V, const float* rnd, int32_t nrnd, float* wo,
+                    float* f, float* pdf, int32_t* flags, int32_t* used) {
+    std::unique_ptr<Material> mat(make_material(*m));
+    if (!mat) return RT_ERR_INVALID_ARGUMENT;
+    HitRecord rec = probe_rec(frame);
+    SamplerCtx& c = t_smp;
+    c.fixed = rnd; c.nfixed = nrnd; c.fixed_pos = 0;
+    BsdfSample s;
+    bool ok = mat->sample(V3(V[0], V[1], V[2]), rec, s);
+    *used = c.fixed_pos;
+    c.fixed = nullptr;
+    int fl = 0;
+    if (ok) {
+        fl |= 1;
+        if (s.is_delta) fl |= 2;
+        if (s.has_next_absorption) fl |= 4;
+        fl |= (s.transition & 3) << 4;
+        if (s.caustic_caster) fl |= 64;
+        wo[0] = s.wo.x; wo[1] = s.wo.y; wo[2] = s.wo.z;
+        f[0] = s.f.x; f[1] = s.f.y; f[2] = s.f.z;
+        *pdf = s.pdf;
+    }
+    *flags = fl;
+    return 0;
+}
+
+int orc_energy_lut(float* E, float* Eavg) {
+    const EnergyLut& L = energy_lut();
+    std::memcpy(E, L.E, sizeof(L.E));
+    std::memcpy(Eavg, L.Eavg, sizeof(L.Eavg));
+    return 0;
+}
+
+// ShadowTransmittance / ShadowAbsorption probes
+int orc_shadow_tr(const rt_material* m, const float* frame, const float* wi, float* tr, float* sig) {
+    std::unique_ptr<Material> mat(make_material(*m));
+    if (!mat) return RT_ERR_INVALID_ARGUMENT;
+    HitRecord rec = probe_rec(frame);
+    V3 t = mat->shadow_transmittance(V3(wi[0], wi[1], wi[2]), rec);
+    V3 s = mat->shadow_absorption(rec);
+    tr[0] = t.x; tr[1] = t.y; tr[2] = t.z;
+    sig[0] = s.x; sig[1] = s.y; sig[2] = s.z;
+    return 0;
+}
+
+// One NEE sample of light `li` from point p with normal n (sample index si),
+// random draws from `rnd`.  out: in_shadow, color[3], dir[3], dist, pdf
+// (PdfSolidAngle of the returned direction).
+int orc_light_sample(void* sp, int32_t li, const float* p, const float* n, int32_t si, const float* rnd, int32_t nrnd,
+                     int32_t* in_shadow, float* color, float* dir, float* dist, float* pdf) {
+    Scene* s = (Scene*)sp;
+    if (li < 0 || li >= (int)s->light_ptrs.size()) return RT_ERR_INVALID_ARGUMENT;
+    SamplerCtx& c = t_smp;
+    c.fixed = rnd; c.nfixed = nrnd; c.fixed_pos = 0;
+    const Light* L = s->light_ptrs[li];
+    V3 P(p[0], p[1], p[2]);
+    LightSample ls = L->illuminate(P, V3(n[0], n[1], n[2]), s->world, si);
+    c.fixed = nullptr;
+    *in_shadow = ls.in_shadow;
+    color[0] = ls.color.x; color[1] = ls.color.y; color[2] = ls.color.z;
+    dir[0] = ls.dir.x; dir[1] = ls.dir.y; dir[2] = ls.dir.z;
+    *dist = ls.dist;
+    *pdf = L->pdf_solid_angle(P, ls.dir);
+    return 0;
+}
+
+float orc_light_pdf(void* sp, int32_t li, const float* p, const float* wi) {
+    Scene* s = (Scene*)sp;
+    return s->light_ptrs[li]->pdf_solid_angle(V3(p[0], p[1], p[2]), V3(wi[0], wi[1], wi[2]));
+}
+
+// Camera basis (Camera.cs:26-51): origin, llc, horizontal, vertical, u, v, lens radius
+int orc_camera_basis(const rt_camera* cam, float* out) {
+    Camera c(*cam);
+    V3 vs[6] = {c.origin, c.llc, c.horiz, c.vert, c.u, c.v};
+    for (int k = 0; k < 6; k++) { out[3 * k] = vs[k].x; out[3 * k + 1] = vs[k].y; out[3 * k + 2] = vs[k].z; }
+    out[18] = c.lens_radius;
+    return 0;
+}
+
+}  // extern "C"

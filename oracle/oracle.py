@@ -1,0 +1,211 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY — imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product.  The oracle restates the
+reference's C# render path on the CPU (see oracle.cpp).
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.join(os.path.dirname(_HERE), "3d-ray_amd")
+if _PKG not in sys.path:
+    sys.path.insert(0, _PKG)
+
+from raytracer import abi  # noqa: E402
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(path)
+        vp, fp, ip, i64p = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_sobol.restype = C.c_float
+        L.orc_sobol.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_sobol_batch.argtypes = [C.POINTER(C.c_uint32)] * 3 + [C.c_int64, fp]
+        L.orc_scene_create.restype = vp
+        L.orc_scene_create.argtypes = [C.POINTER(abi.rt_scene_desc)]
+        L.orc_scene_destroy.argtypes = [vp]
+        L.orc_scene_info.argtypes = [vp, C.POINTER(abi.rt_scene_info)]
+        L.orc_bvh_dump.restype = C.c_int64
+        L.orc_bvh_dump.argtypes = [vp, ip, C.c_int64]
+        L.orc_trace_rays.argtypes = [vp, fp, C.c_int64, ip, fp, fp, fp, i64p, i64p]
+        L.orc_trace_primary.argtypes = [vp, C.POINTER(abi.rt_settings), C.c_int32, C.c_int32, ip, fp, fp, fp, i64p, i64p]
+        L.orc_render.argtypes = [vp, C.POINTER(abi.rt_settings), C.POINTER(abi.rt_film), C.c_int32, C.c_int32,
+                                 C.POINTER(abi.rt_render_stats)]
+        L.orc_bsdf_eval.argtypes = [C.POINTER(abi.rt_material), fp, fp, fp, fp, fp, fp]
+        L.orc_bsdf_sample.argtypes = [C.POINTER(abi.rt_material), fp, fp, fp, C.c_int32, fp, fp, fp, ip, ip]
+        L.orc_energy_lut.argtypes = [fp, fp]
+        L.orc_shadow_tr.argtypes = [C.POINTER(abi.rt_material), fp, fp, fp, fp]
+        L.orc_light_sample.argtypes = [vp, C.c_int32, fp, fp, C.c_int32, fp, C.c_int32, ip, fp, fp, fp, fp]
+        L.orc_light_pdf.restype = C.c_float
+        L.orc_light_pdf.argtypes = [vp, C.c_int32, fp, fp]
+        L.orc_camera_basis.argtypes = [C.POINTER(abi.rt_camera), fp]
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+
+
+def _i64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64)) if a is not None else None
+
+
+def sobol(index, dim, seed):
+    index, dim, seed = (np.ascontiguousarray(np.broadcast_to(x, np.broadcast(index, dim, seed).shape), np.uint32)
+                        for x in (index, dim, seed))
+    out = np.empty(index.shape, np.float32)
+    u32 = C.POINTER(C.c_uint32)
+    lib().orc_sobol_batch(index.ctypes.data_as(u32), dim.ctypes.data_as(u32), seed.ctypes.data_as(u32),
+                          index.size, _f(out))
+    return out
+
+
+def energy_lut():
+    E = np.empty(32 * 32, np.float32)
+    A = np.empty(32, np.float32)
+    lib().orc_energy_lut(_f(E), _f(A))
+    return E.reshape(32, 32), A
+
+
+def camera_basis(cam):
+    out = np.empty(19, np.float32)
+    lib().orc_camera_basis(C.byref(cam), _f(out))
+    return out
+
+
+class OracleScene:
+    def __init__(self, desc):
+        self.desc = desc
+        self.h = lib().orc_scene_create(desc.ptr)
+        if not self.h:
+            raise RuntimeError("oracle scene: " + lib().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        i = abi.rt_scene_info()
+        lib().orc_scene_info(self.h, C.byref(i))
+        return i
+
+    def bvh_dump(self):
+        n = lib().orc_bvh_dump(self.h, None, 0)
+        out = np.empty(n, np.int32)
+        lib().orc_bvh_dump(self.h, _i(out), n)
+        return out
+
+    def trace_rays(self, rays, counters=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = len(rays)
+        prim = np.empty(n, np.int32)
+        t, u, v = (np.empty(n, np.float32) for _ in range(3))
+        nt = np.empty(n, np.int64) if counters else None
+        pt = np.empty(n, np.int64) if counters else None
+        lib().orc_trace_rays(self.h, _f(rays), n, _i(prim), _f(t), _f(u), _f(v), _i64(nt), _i64(pt))
+        return (prim, t, u, v, nt, pt) if counters else (prim, t, u, v)
+
+    def trace_primary(self, settings, sample_begin=0, sample_count=1):
+        n = settings.width * settings.height * sample_count
+        prim = np.empty(n, np.int32)
+        t, u, v = (np.empty(n, np.float32) for _ in range(3))
+        nt, pt = np.zeros(1, np.int64), np.zeros(1, np.int64)
+        rc = lib().orc_trace_primary(self.h, C.byref(settings), sample_begin, sample_count, _i(prim), _f(t), _f(u),
+                                     _f(v), _i64(nt), _i64(pt))
+        if rc:
+            raise RuntimeError("orc_trace_primary: " + lib().orc_last_error().decode())
+        return prim, t, u, v, int(nt[0]), int(pt[0])
+
+    def render(self, settings, threads=None, row_step=1, aovs=False):
+        W, H = settings.width, settings.height
+        npx = W * H
+        planes = {k: np.zeros(3 * npx, np.float32) for k in ("beauty", "beauty_a", "beauty_b", "display")}
+        if aovs:
+            for k in ("albedo_a", "albedo_b", "normal_a", "normal_b"):
+                planes[k] = np.zeros(3 * npx, np.float32)
+            planes["depth_a"] = np.zeros(npx, np.float32)
+            planes["depth_b"] = np.zeros(npx, np.float32)
+        count = np.zeros(npx, np.int32)
+        film = abi.rt_film()
+        for k, a in planes.items():
+            setattr(film, k, _f(a))
+        film.sample_count = _i(count)
+        stats = abi.rt_render_stats()
+        threads = threads or os.cpu_count() or 1
+        rc = lib().orc_render(self.h, C.byref(settings), C.byref(film), threads, row_step, C.byref(stats))
+        if rc:
+            raise RuntimeError("orc_render: " + lib().orc_last_error().decode())
+        out = {k: (a.reshape(3, H, W) if a.size == 3 * npx else a.reshape(H, W)) for k, a in planes.items()}
+        out["sample_count"] = count.reshape(H, W)
+        out["stats"] = stats
+        return out
+
+    def light_sample(self, li, p, n, si, rnd):
+        rnd = np.ascontiguousarray(rnd, np.float32)
+        p = np.ascontiguousarray(p, np.float32)
+        n = np.ascontiguousarray(n, np.float32)
+        ins = np.zeros(1, np.int32)
+        col, dr = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        dist, pdf = np.zeros(1, np.float32), np.zeros(1, np.float32)
+        lib().orc_light_sample(self.h, li, _f(p), _f(n), si, _f(rnd), len(rnd), _i(ins), _f(col), _f(dr), _f(dist), _f(pdf))
+        return bool(ins[0]), col, dr, float(dist[0]), float(pdf[0])
+
+    def light_pdf(self, li, p, wi):
+        p = np.ascontiguousarray(p, np.float32)
+        wi = np.ascontiguousarray(wi, np.float32)
+        return lib().orc_light_pdf(self.h, li, _f(p), _f(wi))
+
+
+def _frame(n, t=None, b=None, front=True, eta=0.0):
+    n = np.asarray(n, np.float32)
+    if t is None:
+        t = np.zeros(3, np.float32)
+        b = np.zeros(3, np.float32)
+    return np.concatenate([n, t, b, [1.0 if front else 0.0, eta]]).astype(np.float32)
+
+
+def bsdf_eval(mat, n, V, L, t=None, b=None, front=True):
+    fr = _frame(n, t, b, front)
+    V = np.ascontiguousarray(V, np.float32)
+    L = np.ascontiguousarray(L, np.float32)
+    f, fd = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    lib().orc_bsdf_eval(C.byref(mat), _f(fr), _f(V), _f(L), _f(f), _f(pdf), _f(fd))
+    return f, float(pdf[0]), fd
+
+
+def bsdf_sample(mat, n, V, rnd, t=None, b=None, front=True, eta=0.0):
+    fr = _frame(n, t, b, front, eta)
+    V = np.ascontiguousarray(V, np.float32)
+    rnd = np.ascontiguousarray(rnd, np.float32)
+    wo, f = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    flags, used = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    lib().orc_bsdf_sample(C.byref(mat), _f(fr), _f(V), _f(rnd), len(rnd), _f(wo), _f(f), _f(pdf), _i(flags), _i(used))
+    return dict(valid=bool(flags[0] & 1), delta=bool(flags[0] & 2), wo=wo, f=f, pdf=float(pdf[0]),
+                flags=int(flags[0]), used=int(used[0]))
+
+
+def shadow_tr(mat, n, wi, front=True):
+    fr = _frame(n, None, None, front)
+    wi = np.ascontiguousarray(wi, np.float32)
+    tr, sig = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().orc_shadow_tr(C.byref(mat), _f(fr), _f(wi), _f(tr), _f(sig))
+    return tr, sig
