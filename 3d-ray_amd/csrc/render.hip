@@ -76,8 +76,23 @@ struct PathBufs {
     float4 *sh_o, *sh_d, *sh_c, *sh_r;     // shadow slot records (N × maxs)
     int32_t *q_a, *q_b, *q_sh;
     int32_t* counters;          // [0] next queue, [1] shadow queue, [2] stack overflow
+    unsigned long long* tcount; // traversal counters: trace node/tri/other, shadow node/tri/other
     int n;
 };
+
+__device__ __forceinline__ void flush_counts(unsigned long long* dst, const TravCount& tc) {
+    unsigned long long a = tc.node, b = tc.tri, c = tc.other;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+        c += __shfl_down(c, off, 64);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(dst, a);
+        atomicAdd(dst + 1, b);
+        atomicAdd(dst + 2, c);
+    }
+}
 
 struct FilmBufs {
     float *sum, *sa, *sb, *alba, *albb, *nrma, *nrmb, *depa, *depb;
@@ -539,11 +554,15 @@ __global__ void __launch_bounds__(kBlock) k_raygen(DevScene S, RParams R, PathBu
     P.q_a[p] = p;
 }
 
-template <int SD>
+template <int SD, bool COUNT>
 __global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P, const int32_t* __restrict__ q, int count) {
     __shared__ uint2 stk[SD * kBlock];
+    TravCount tc{0u, 0u, 0u};
     int i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= count) return;
+    if (i >= count) {
+        if (COUNT) flush_counts(P.tcount, tc);
+        return;
+    }
     int p = q[i];
     float4 o4 = P.o[p], d4 = P.d[p];
     f3 o = f3of(o4), d = f3of(d4);
@@ -551,8 +570,8 @@ __global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P, const 
     HitOut h;
     bool moved = false;
     for (int skip = 0;;) {
-        h = trace_closest<SD, kBlock>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps, kInfinity,
-                                      &stk[threadIdx.x], (uint32_t*)&P.counters[2]);
+        h = trace_closest<SD, kBlock, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
+                                             kInfinity, &stk[threadIdx.x], (uint32_t*)&P.counters[2], tc);
         if (h.entry < 0 || !camera) break;
         int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
         if (!(fl & kEntryCamInvisible)) break;
@@ -563,6 +582,7 @@ __global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P, const 
     if (moved) P.o[p] = make_float4(o.x, o.y, o.z, o4.w);
     P.hit_e[p] = h.entry;
     P.hit_tuv[p] = make_float4(h.t, h.u, h.v, 0.0f);
+    if (COUNT) flush_counts(P.tcount, tc);
 }
 
 __device__ __forceinline__ void add_target(PathBufs& P, int p, bool ind, f3 c) {
@@ -1029,11 +1049,15 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, RParams R, PathBuf
 }
 
 // Transparent shadow walk (ShadowRay.Transmittance, ShadowRay.cs:59-158)
-template <int SD>
+template <int SD, bool COUNT>
 __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, const int32_t* __restrict__ q, int count) {
     __shared__ uint2 stk[SD * kBlock];
+    TravCount tc{0u, 0u, 0u};
     int i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= count) return;
+    if (i >= count) {
+        if (COUNT) flush_counts(P.tcount + 3, tc);
+        return;
+    }
     int64_t sl = q[i];
     float4 o4 = P.sh_o[sl], d4 = P.sh_d[sl];
     f3 org = f3of(o4), dir = f3of(d4);
@@ -1042,8 +1066,8 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, const
     bool in_med = false;
     bool done = false;
     for (int it = 0; it < 8 && !done; it++) {
-        HitOut h = trace_closest<SD, kBlock>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org, dir, kEps, rem,
-                                             &stk[threadIdx.x], (uint32_t*)&P.counters[2]);
+        HitOut h = trace_closest<SD, kBlock, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org, dir,
+                                                    kEps, rem, &stk[threadIdx.x], (uint32_t*)&P.counters[2], tc);
         if (h.entry < 0) { done = true; break; }
         Surf s = make_surf(S, h.entry, org, dir, h.t, h.u, h.v);
         const GMat g = S.mats[s.mat];
@@ -1069,6 +1093,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, const
     if (!done || near_zero(thr)) thr = splat(0.0f);
     f3 c = f3of(P.sh_c[sl]) * thr;
     P.sh_r[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+    if (COUNT) flush_counts(P.tcount + 3, tc);
 }
 
 // Per-pixel film accumulation in sample order (Renderer.cs:668-672,
@@ -1166,9 +1191,10 @@ __global__ void __launch_bounds__(kBlock) k_trace_primary(DevScene S, RParams R,
     f3 o, d;
     camera_ray(S, R, i, j, rng, o, d);
     HitOut h;
+    TravCount tc{0u, 0u, 0u};
     for (int skip = 0;;) {
-        h = trace_closest<SD, kBlock>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps, kInfinity,
-                                      &stk[threadIdx.x], (uint32_t*)(prim + n));
+        h = trace_closest<SD, kBlock, false>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps, kInfinity,
+                                             &stk[threadIdx.x], (uint32_t*)(prim + n), tc);
         if (h.entry < 0) break;
         int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
         if (!(fl & kEntryCamInvisible)) break;
@@ -1188,9 +1214,10 @@ __global__ void __launch_bounds__(kBlock) k_trace_rays(DevScene S, const float* 
     int p = xcd_block() * kBlock + threadIdx.x;
     if (p >= n) return;
     const float* r = rays + 6 * (int64_t)p;
-    HitOut h = trace_closest<SD, kBlock>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, mk(r[0], r[1], r[2]),
-                                         mk(r[3], r[4], r[5]), kEps, kInfinity, &stk[threadIdx.x],
-                                         (uint32_t*)(prim + n));
+    TravCount tc{0u, 0u, 0u};
+    HitOut h = trace_closest<SD, kBlock, false>(S.nodes, S.entries, S.aux, S.top_start, S.top_count,
+                                                mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), kEps, kInfinity,
+                                                &stk[threadIdx.x], (uint32_t*)(prim + n), tc);
     prim[p] = h.entry < 0 ? -1 : __float_as_int(S.entries[h.entry].b.w);
     to[p] = h.entry < 0 ? 0.0f : h.t;
     uo[p] = h.entry < 0 ? 0.0f : h.u;
@@ -1281,6 +1308,18 @@ static DevScene dev_view(const rt_scene* sc) {
 }
 
 // Launches a traversal kernel with the LDS stack depth the scene needs.
+#define DISPATCH_SD_C(sd, cnt, KERNEL, grid, ...)                                                     \
+    do {                                                                                               \
+        if (cnt) DISPATCH_SD_T(sd, KERNEL, true, grid, __VA_ARGS__);                                   \
+        else DISPATCH_SD_T(sd, KERNEL, false, grid, __VA_ARGS__);                                      \
+    } while (0)
+#define DISPATCH_SD_T(sd, KERNEL, C_, grid, ...)                                                          \
+    do {                                                                                                  \
+        if ((sd) <= 24) hipLaunchKernelGGL((KERNEL<24, C_>), grid, dim3(kBlock), 0, __VA_ARGS__);         \
+        else if ((sd) <= 32) hipLaunchKernelGGL((KERNEL<32, C_>), grid, dim3(kBlock), 0, __VA_ARGS__);    \
+        else if ((sd) <= 48) hipLaunchKernelGGL((KERNEL<48, C_>), grid, dim3(kBlock), 0, __VA_ARGS__);    \
+        else hipLaunchKernelGGL((KERNEL<64, C_>), grid, dim3(kBlock), 0, __VA_ARGS__);                    \
+    } while (0)
 #define DISPATCH_SD(sd, KERNEL, grid, ...)                                                       \
     do {                                                                                         \
         if ((sd) <= 24) hipLaunchKernelGGL(KERNEL<24>, grid, dim3(kBlock), 0, __VA_ARGS__);      \
@@ -1558,6 +1597,7 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     PathBufs P{};
     P.n = (int)N;
     P.counters = (int32_t*)take(64);
+    P.tcount = (unsigned long long*)take(64);
     P.o = (float4*)take(16 * N);
     P.d = (float4*)take(16 * N);
     P.thr = (float4*)take(16 * N);
@@ -1669,7 +1709,9 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     R.chunk_samples = K;
 
     double t_trace = 0, t_shade = 0, t_shadow = 0;
-    int64_t n_closest = 0, n_shadow_rays = 0, launches = 0;
+    int64_t n_closest = 0, n_shadow_rays = 0, launches = 0, shadow_launches = 0;
+    const bool counting = s->count_traversal != 0;
+    HIP_TRY(hipMemsetAsync(P.tcount, 0, 64, ctx->stream));
     hipEvent_t* ev = ctx->ev;
     HIP_TRY(hipEventRecord(ev[6], ctx->stream));
     for (int base = 0; base < spp; base += K) {
@@ -1685,7 +1727,7 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
             HIP_TRY(hipMemsetAsync(P.counters, 0, 8, ctx->stream));
             dim3 grid((count + kBlock - 1) / kBlock);
             HIP_TRY(hipEventRecord(ev[0], ctx->stream));
-            DISPATCH_SD(sc->stack_depth, k_trace, grid, ctx->stream, S, P, qa, count);
+            DISPATCH_SD_C(sc->stack_depth, counting, k_trace, grid, ctx->stream, S, P, qa, count);
             HIP_TRY(hipEventRecord(ev[1], ctx->stream));
             PathBufs Pb = P;
             Pb.q_b = qb;
@@ -1704,8 +1746,9 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
             if (cnt[2]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
             if (cnt[1] > 0) {
                 HIP_TRY(hipEventRecord(ev[3], ctx->stream));
-                DISPATCH_SD(sc->stack_depth, k_shadow, dim3((cnt[1] + kBlock - 1) / kBlock), ctx->stream, S, P, P.q_sh,
-                            cnt[1]);
+                DISPATCH_SD_C(sc->stack_depth, counting, k_shadow, dim3((cnt[1] + kBlock - 1) / kBlock), ctx->stream, S,
+                              P, P.q_sh, cnt[1]);
+                shadow_launches++;
                 HIP_TRY(hipEventRecord(ev[4], ctx->stream));
                 HIP_TRY(hipEventSynchronize(ev[4]));
                 hipEventElapsedTime(&ms, ev[3], ev[4]);
@@ -1755,6 +1798,17 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
         stats->rays_closest = n_closest;
         stats->rays_shadow = n_shadow_rays;
         stats->trace_launches = launches;
+        stats->shadow_launches = shadow_launches;
+        if (counting) {
+            unsigned long long tcv[6];
+            HIP_TRY(hipMemcpy(tcv, P.tcount, sizeof(tcv), hipMemcpyDeviceToHost));
+            stats->trace_node_tests = (int64_t)tcv[0];
+            stats->trace_tri_tests = (int64_t)tcv[1];
+            stats->trace_other_tests = (int64_t)tcv[2];
+            stats->shadow_node_tests = (int64_t)tcv[3];
+            stats->shadow_tri_tests = (int64_t)tcv[4];
+            stats->shadow_other_tests = (int64_t)tcv[5];
+        }
     }
     return RT_OK;
 }

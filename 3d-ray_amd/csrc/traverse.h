@@ -116,9 +116,21 @@ struct HitOut {
     float t, u, v;
 };
 
+// Reference-traversal work counters (instrumented kernels only):
+// node = BvhNode.Hit calls (box tests); tri = Hit calls on 64-B primitive
+// records (triangles, quads, planes); other = Hit calls on spheres (16 B).
+struct TravCount {
+    uint32_t node, tri, other;
+};
+
 // Tests one primitive entry; updates `h` and `closest` on a hit.
+template <bool COUNT>
 __device__ __forceinline__ void test_prim(const GEntry& e, uint32_t idx, int32_t kind, int32_t flags, f3 o, f3 d,
-                                          float tmin, const float4* aux, float& closest, HitOut& h) {
+                                          float tmin, const float4* aux, float& closest, HitOut& h, TravCount& tc) {
+    if (COUNT) {
+        if (kind == kSphere) tc.other++;
+        else tc.tri++;
+    }
     float t, u = 0.0f, v = 0.0f;
     bool hit;
     if (kind == kTri) hit = hit_triangle(o, d, f3of(e.a), f3of(e.b), f3of(e.c), tmin, closest, t, u, v);
@@ -135,11 +147,11 @@ __device__ __forceinline__ void test_prim(const GEntry& e, uint32_t idx, int32_t
 }
 
 // Stack layout: stk[k * kStride + lane] (uint2: ref, tEnter bits).
-template <int SD, int STRIDE>
+template <int SD, int STRIDE, bool COUNT>
 __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes, const GEntry* __restrict__ entries,
                                                 const float4* __restrict__ aux, uint32_t top_start, uint32_t top_count,
                                                 f3 o, f3 d, float tmin, float tmax, uint2* stk,
-                                                uint32_t* overflow) {
+                                                uint32_t* overflow, TravCount& tc) {
     HitOut h;
     h.entry = -1;
     h.t = 0.0f;
@@ -153,10 +165,11 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
         int32_t tag = tag_of(e.a);
         int32_t kind = tag & 0xff, flags = tag >> 8;
         if (kind != kBvh) {
-            test_prim(e, eidx, kind, flags, o, d, tmin, aux, closest, h);
+            test_prim<COUNT>(e, eidx, kind, flags, o, d, tmin, aux, closest, h, tc);
             continue;
         }
         float te;
+        if (COUNT && !(flags & kEntryNoBox)) tc.node++;
         if (!(flags & kEntryNoBox) && !slab(f3of(e.a), f3of(e.b), o, inv, tmin, closest, te)) continue;
         uint32_t cur = (uint32_t)__float_as_int(e.b.w);
         int sp = 0;
@@ -170,10 +183,11 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
                     int32_t ltag = tag_of(le.a);
                     int32_t lk = ltag & 0xff, lf = ltag >> 8;
                     if (lk != kBvh) {
-                        test_prim(le, idx, lk, lf, o, d, tmin, aux, closest, h);
+                        test_prim<COUNT>(le, idx, lk, lf, o, d, tmin, aux, closest, h, tc);
                         continue;
                     }
                     float lte;
+                    if (COUNT && !(lf & kEntryNoBox)) tc.node++;
                     if (!(lf & kEntryNoBox) && !slab(f3of(le.a), f3of(le.b), o, inv, tmin, closest, lte)) continue;
                     if (k + 1 < cnt) {
                         if (sp >= SD) { *overflow = 1u; return h; }
@@ -187,6 +201,7 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
                 if (descended) continue;
             } else {
                 const GNode nd = nodes[cur];
+                if (COUNT) tc.node += 2;
                 f3 c0mn = mk(nd.a.x, nd.a.y, nd.a.z), c0mx = mk(nd.a.w, nd.b.x, nd.b.y);
                 f3 c1mn = mk(nd.b.z, nd.b.w, nd.c.x), c1mx = mk(nd.c.y, nd.c.z, nd.c.w);
                 float te0, te1;

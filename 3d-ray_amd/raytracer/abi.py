@@ -75,7 +75,7 @@ class rt_settings(C.Structure):
                 ("exposure_ev", C.c_float), ("rr_min_bounces", C.c_int32),
                 ("rr_min_survival", C.c_float), ("seed_xor", C.c_uint32), ("caustics", C.c_int32),
                 ("caustic_photons", C.c_int32), ("caustic_gather_radius", C.c_float),
-                ("paths_in_flight", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("paths_in_flight", C.c_int32), ("count_traversal", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 FP = C.POINTER(C.c_float)
@@ -91,7 +91,9 @@ class rt_render_stats(C.Structure):
     _fields_ = [("wall_ms", C.c_double), ("device_ms", C.c_double), ("trace_ms", C.c_double),
                 ("shadow_ms", C.c_double), ("shade_ms", C.c_double), ("camera_samples", C.c_int64),
                 ("rays_closest", C.c_int64), ("rays_shadow", C.c_int64), ("trace_launches", C.c_int64),
-                ("reserved", C.c_int64 * 7)]
+                ("shadow_launches", C.c_int64), ("trace_node_tests", C.c_int64), ("trace_tri_tests", C.c_int64),
+                ("trace_other_tests", C.c_int64), ("shadow_node_tests", C.c_int64), ("shadow_tri_tests", C.c_int64),
+                ("shadow_other_tests", C.c_int64)]
 
 
 class rt_scene_info(C.Structure):

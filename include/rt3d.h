@@ -217,7 +217,9 @@ typedef struct rt_settings {
     int32_t caustic_photons;
     float caustic_gather_radius;
     int32_t paths_in_flight;        /* wavefront size; 0 = library default */
-    int32_t reserved[6];
+    int32_t count_traversal;        /* 1: count BVH node/primitive tests (slower
+                                       instrumented kernels; fills rt_render_stats) */
+    int32_t reserved[5];
 } rt_settings;          /* 96 bytes */
 
 /* ── Film (RenderBuffers.cs:52-169; FrameBuffer planar layout
@@ -251,7 +253,12 @@ typedef struct rt_render_stats {
     int64_t rays_shadow;         /* shadow segments (each transparent-shadow
                                     continuation counts once) */
     int64_t trace_launches;      /* closest-hit kernel launches */
-    int64_t reserved[7];
+    int64_t shadow_launches;     /* shadow kernel launches */
+    /* with rt_settings.count_traversal: the reference traversal's work —
+       BvhNode.Hit calls (node box tests) and primitive Hit calls — of the
+       closest-hit and shadow kernels */
+    int64_t trace_node_tests, trace_tri_tests, trace_other_tests;
+    int64_t shadow_node_tests, shadow_tri_tests, shadow_other_tests;
 } rt_render_stats;
 
 typedef struct rt_scene_info {

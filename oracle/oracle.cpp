@@ -2814,3 +2814,76 @@ int orc_camera_basis(const rt_camera* cam, float* out) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Batched BSDF probes over n (V, L) pairs with one shading frame.
+int orc_bsdf_eval_batch(const rt_material* m, const float* frame, const float* V, const float* L, int64_t n, float* f,
+                        float* pdf, float* fd) {
+    std::unique_ptr<Material> mat(make_material(*m));
+    if (!mat) return RT_ERR_INVALID_ARGUMENT;
+    HitRecord rec = probe_rec(frame);
+    for (int64_t i = 0; i < n; i++) {
+        V3 v(V[3 * i], V[3 * i + 1], V[3 * i + 2]), l(L[3 * i], L[3 * i + 1], L[3 * i + 2]);
+        V3 e = mat->evaluate(v, l, rec);
+        f[3 * i] = e.x; f[3 * i + 1] = e.y; f[3 * i + 2] = e.z;
+        pdf[i] = mat->pdf(v, l, rec);
+        V3 d = mat->evaluate_direct(l, v, rec.normal, rec);
+        fd[3 * i] = d.x; fd[3 * i + 1] = d.y; fd[3 * i + 2] = d.z;
+    }
+    return 0;
+}
+
+// n Sample() calls with the PCG stream seeded by `seed` (the reference's
+// PRNG fallback of Sampler.Sample1D when no Sobol context is open).
+int orc_bsdf_sample_batch(const rt_material* m, const float* frame, const float* V, int64_t n, uint64_t seed, float* wo,
+                          float* f, float* pdf, int32_t* flags) {
+    std::unique_ptr<Material> mat(make_material(*m));
+    if (!mat) return RT_ERR_INVALID_ARGUMENT;
+    HitRecord rec = probe_rec(frame);
+    SamplerCtx& c = t_smp;
+    c.active = false;
+    c.fixed = nullptr;
+    c.rng = seed * 2654435761ull + 0x9E3779B97F4A7C15ull;
+    V3 v(V[0], V[1], V[2]);
+    for (int64_t i = 0; i < n; i++) {
+        BsdfSample s;
+        bool ok = mat->sample(v, rec, s);
+        int fl = 0;
+        if (ok) {
+            fl |= 1;
+            if (s.is_delta) fl |= 2;
+            if (s.has_next_absorption) fl |= 4;
+            fl |= (s.transition & 3) << 4;
+            if (s.caustic_caster) fl |= 64;
+            wo[3 * i] = s.wo.x; wo[3 * i + 1] = s.wo.y; wo[3 * i + 2] = s.wo.z;
+            f[3 * i] = s.f.x; f[3 * i + 1] = s.f.y; f[3 * i + 2] = s.f.z;
+            pdf[i] = s.pdf;
+        }
+        flags[i] = fl;
+    }
+    return 0;
+}
+
+// Closest hit with explicit [tmin, tmax] through the world (BVH) or through a
+// flat HittableList over every object (the brute-force reference of
+// BvhEquivalenceTests.cs:54-72).
+int orc_trace_rays_ex(void* sp, const float* rays, int64_t n, float tmin, float tmax, int32_t linear, int32_t* prim,
+                      float* t) {
+    Scene* s = (Scene*)sp;
+    HittableList flat;
+    flat.objs = s->objects;
+    const Hittable* w = linear ? (const Hittable*)&flat : (const Hittable*)s->world;
+    for (int64_t i = 0; i < n; i++) {
+        const float* r = rays + 6 * i;
+        Ray ray(V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5]));
+        HitRecord rec;
+        bool h = w->hit(ray, tmin, tmax, rec);
+        prim[i] = h ? rec.prim : -1;
+        t[i] = h ? rec.t : 0.0f;
+    }
+    flat.objs.clear();
+    return 0;
+}
+
+}  // extern "C"

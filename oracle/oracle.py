@@ -50,6 +50,9 @@ def lib():
         L.orc_light_pdf.restype = C.c_float
         L.orc_light_pdf.argtypes = [vp, C.c_int32, fp, fp]
         L.orc_camera_basis.argtypes = [C.POINTER(abi.rt_camera), fp]
+        L.orc_bsdf_eval_batch.argtypes = [C.POINTER(abi.rt_material), fp, fp, fp, C.c_int64, fp, fp, fp]
+        L.orc_bsdf_sample_batch.argtypes = [C.POINTER(abi.rt_material), fp, fp, C.c_int64, C.c_uint64, fp, fp, fp, ip]
+        L.orc_trace_rays_ex.argtypes = [vp, fp, C.c_int64, C.c_float, C.c_float, C.c_int32, ip, fp]
         _lib = L
     return _lib
 
@@ -209,3 +212,34 @@ def shadow_tr(mat, n, wi, front=True):
     tr, sig = np.zeros(3, np.float32), np.zeros(3, np.float32)
     lib().orc_shadow_tr(C.byref(mat), _f(fr), _f(wi), _f(tr), _f(sig))
     return tr, sig
+
+
+def bsdf_eval_batch(mat, n, V, L, t=None, b=None, front=True):
+    """Evaluate / Pdf / EvaluateDirect for arrays V, L of shape (k, 3)."""
+    fr = _frame(n, t, b, front)
+    V = np.ascontiguousarray(V, np.float32).reshape(-1, 3)
+    L = np.ascontiguousarray(L, np.float32).reshape(-1, 3)
+    k = len(V)
+    f, fd = np.zeros((k, 3), np.float32), np.zeros((k, 3), np.float32)
+    pdf = np.zeros(k, np.float32)
+    lib().orc_bsdf_eval_batch(C.byref(mat), _f(fr), _f(V), _f(L), k, _f(f), _f(pdf), _f(fd))
+    return f, pdf, fd
+
+
+def bsdf_sample_batch(mat, n, V, count, seed=1, t=None, b=None, front=True, eta=0.0):
+    fr = _frame(n, t, b, front, eta)
+    V = np.ascontiguousarray(V, np.float32)
+    wo, f = np.zeros((count, 3), np.float32), np.zeros((count, 3), np.float32)
+    pdf = np.zeros(count, np.float32)
+    flags = np.zeros(count, np.int32)
+    lib().orc_bsdf_sample_batch(C.byref(mat), _f(fr), _f(V), count, seed, _f(wo), _f(f), _f(pdf), _i(flags))
+    return wo, f, pdf, flags
+
+
+def trace_rays_ex(scene, rays, tmin, tmax, linear):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = len(rays)
+    prim = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    lib().orc_trace_rays_ex(scene.h, _f(rays), n, tmin, tmax, 1 if linear else 0, _i(prim), _f(t))
+    return prim, t

@@ -1,0 +1,177 @@
+"""Radiance parity of the HIP wavefront integrator against the oracle's
+recursive restatement of Renderer.TraceRay.
+
+Two kinds of checks:
+  * same sampler streams (seed_xor = 0 on both sides): the two integrators
+    follow the same paths, so pixels agree up to fp32 re-association
+    (tolerances below, per pixel);
+  * independent streams (different seed_xor): per-pixel |mu_gpu - mu_cpu| <=
+    3 sigma for >= 99% of pixels and none beyond 6 sigma, sigma from the run
+    to run spread of several independent renders on each side.
+"""
+import numpy as np
+import pytest
+
+from raytracer import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+SAME_STREAM_REL = 1e-3     # per-pixel relative tolerance (same sampler streams)
+SAME_STREAM_FRAC = 0.97    # fraction of pixels that must meet it
+MEAN_REL = 2e-3            # image-mean relative tolerance
+
+
+def _same_stream(native, oracle, ctx, desc, W, H, spp, depth, **kw):
+    sc = native.Scene(ctx, desc)
+    r = native.Renderer(sc, spp, depth, **kw).render(W, H)
+    st = abi.settings_defaults(W, H, spp, depth)
+    for k, v in kw.items():
+        if k == "light_sampling_strategy":
+            st.light_sampling = {"all": 0, "power": 1, "uniform": 2}[v]
+        elif k == "mis_heuristic":
+            st.mis = {"balance": 0, "power": 1}[v]
+        else:
+            setattr(st, k, v)
+    o = oracle.OracleScene(desc).render(st)
+    g, c = r.buffers.beauty.astype(np.float64), o["beauty"].astype(np.float64)
+    lum_g = (0.2126 * g[0] + 0.7152 * g[1] + 0.0722 * g[2])
+    lum_c = (0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2])
+    rel = np.abs(lum_g - lum_c) / np.maximum(np.maximum(lum_c, lum_g), 1e-3)
+    frac = (rel <= SAME_STREAM_REL).mean()
+    mrel = abs(lum_g.mean() - lum_c.mean()) / max(lum_c.mean(), 1e-6)
+    print(f"pixels within {SAME_STREAM_REL}: {frac:.4f}; mean rel diff {mrel:.2e}; "
+          f"gpu mean {lum_g.mean():.6f} cpu mean {lum_c.mean():.6f}", flush=True)
+    assert lum_c.mean() > 0
+    assert frac >= SAME_STREAM_FRAC
+    assert mrel <= MEAN_REL
+    return r, o
+
+
+def test_cornell_same_streams(native, oracle, gpu_ctx):
+    _same_stream(native, oracle, gpu_ctx, scenes.cornell(96, 96), 96, 96, 16, 8)
+
+
+def test_cornell_area_light_same_streams(native, oracle, gpu_ctx):
+    _same_stream(native, oracle, gpu_ctx, scenes.cornell(64, 64, light="area"), 64, 64, 8, 8)
+
+
+def test_cornell_rr_override_same_streams(native, oracle, gpu_ctx):
+    # the config's "Russian roulette from depth 3" run
+    _same_stream(native, oracle, gpu_ctx, scenes.cornell(64, 64), 64, 64, 8, 8, rr_min_bounces=3,
+                 rr_min_survival=0.1)
+
+
+def test_disney_icosphere_same_streams(native, oracle, gpu_ctx):
+    d = scenes.displaced_icosphere_scene(96, 54, subdiv=4)
+    _same_stream(native, oracle, gpu_ctx, d, 96, 54, 8, 8)
+
+
+def test_power_light_sampling_same_streams(native, oracle, gpu_ctx):
+    d = scenes.displaced_icosphere_scene(64, 36, subdiv=3)
+    _same_stream(native, oracle, gpu_ctx, d, 64, 36, 8, 6, light_sampling_strategy="power",
+                 mis_heuristic="power")
+
+
+def test_fog_same_streams(native, oracle, gpu_ctx):
+    base = scenes.cornell(64, 64)
+    d = scenes.SceneDesc(base.prims, base.mats, base.lights, base.camera,
+                         medium=((0.05, 0.05, 0.05), (0.3, 0.3, 0.3)))
+    _same_stream(native, oracle, gpu_ctx, d, 64, 64, 8, 8)
+
+
+def test_cornell_independent_streams_within_3_sigma(native, oracle, gpu_ctx):
+    W = H = 48
+    spp, depth, runs = 8, 8, 12
+    desc = scenes.cornell(W, H)
+    sc = native.Scene(gpu_ctx, desc)
+    osc = oracle.OracleScene(desc)
+    g, c = [], []
+    for k in range(runs):
+        g.append(native.Renderer(sc, spp, depth, seed_xor=0x1000 + k).render(W, H).buffers.beauty.astype(np.float64))
+        st = abi.settings_defaults(W, H, spp, depth)
+        st.seed_xor = 0x9000 + k
+        c.append(osc.render(st)["beauty"].astype(np.float64))
+    g, c = np.stack(g), np.stack(c)
+    mg, mc = g.mean(0), c.mean(0)
+    se = np.sqrt(g.var(0, ddof=1) / runs + c.var(0, ddof=1) / runs)
+    z = np.abs(mg - mc) / np.maximum(se, 1e-6)
+    frac3 = (z <= 3).mean()
+    print(f"within 3 sigma: {frac3:.4f}; max z {z.max():.2f}", flush=True)
+    assert frac3 >= 0.99
+    assert z.max() <= 6.0
+
+
+def test_determinism_and_halves(native, gpu_ctx):
+    desc = scenes.cornell(64, 64)
+    sc = native.Scene(gpu_ctx, desc)
+    a = native.Renderer(sc, 9, 8).render(64, 64)
+    b = native.Renderer(sc, 9, 8).render(64, 64)
+    assert np.array_equal(a.buffers.beauty, b.buffers.beauty)
+    # RenderCaptureTests: A/B halves recombine to the beauty mean
+    nA, nB = 5, 4
+    comb = (nA * a.buffers.beauty_a + nB * a.buffers.beauty_b) / 9
+    assert np.allclose(comb, a.buffers.beauty, atol=1e-4)
+    assert np.all(a.buffers.sample_count == 9)
+    assert a.pixels.shape == (64, 64, 3) and a.pixels.min() >= 0 and a.pixels.max() <= 1
+
+
+def test_single_sample_b_half_mirrors_a(native, gpu_ctx):
+    sc = native.Scene(gpu_ctx, scenes.cornell(32, 32))
+    r = native.Renderer(sc, 1, 8).render(32, 32, capture_aovs=True)
+    assert np.array_equal(r.buffers.beauty_a, r.buffers.beauty_b)
+    assert np.array_equal(r.buffers.depth_a, r.buffers.depth_b)
+
+
+def test_aovs_match_oracle(native, oracle, gpu_ctx):
+    W = H = 48
+    desc = scenes.cornell(W, H)
+    sc = native.Scene(gpu_ctx, desc)
+    r = native.Renderer(sc, 4, 8).render(W, H, capture_aovs=True)
+    st = abi.settings_defaults(W, H, 4, 8)
+    st.capture_aovs = 1
+    o = oracle.OracleScene(desc).render(st, aovs=True)
+    for k in ("albedo_a", "normal_a", "depth_a", "albedo_b", "normal_b", "depth_b"):
+        a, b = getattr(r.buffers, k), o[k]
+        frac = (np.abs(a - b) <= 1e-3 + 1e-3 * np.abs(b)).mean()
+        print(k, frac, flush=True)
+        assert frac >= 0.98, k
+    # depth: first-hit distance of camera rays hitting the box (the open +z
+    # side misses nothing: every pixel sees a wall)
+    assert np.all(r.buffers.depth_a > 0)
+
+
+def test_invalid_settings_return_errors(native, gpu_ctx):
+    sc = native.Scene(gpu_ctx, scenes.cornell(16, 16))
+    r = native.Renderer(sc, 4, 8)
+    with pytest.raises(native.RtError) as e:
+        r.render(0, 16)
+    assert e.value.code == abi.RT_ERR_INVALID_ARGUMENT
+    r.settings.sampler = abi.RT_SAMPLER_PRNG
+    with pytest.raises(native.RtError) as e:
+        r.render(16, 16)
+    assert e.value.code == abi.RT_ERR_UNSUPPORTED
+    r.settings.sampler = abi.RT_SAMPLER_SOBOL
+    r.settings.caustics = 1
+    with pytest.raises(native.RtError) as e:
+        r.render(16, 16)
+    assert e.value.code == abi.RT_ERR_UNSUPPORTED
+    r.settings.caustics = 0
+    assert r.render(16, 16).buffers.beauty.mean() > 0
+
+
+def test_render_stats_and_traversal_counts(native, oracle, gpu_ctx):
+    import ctypes as C
+    W, H = 64, 64
+    desc = scenes.cornell(W, H)
+    sc = native.Scene(gpu_ctx, desc)
+    r = native.Renderer(sc, 2, 1)
+    s = r.settings
+    s.width, s.height, s.count_traversal = W, H, 1
+    st = abi.rt_render_stats()
+    gpu_ctx.check(native.load().rt_render(gpu_ctx.h, sc.h, C.byref(s), None, C.byref(st)))
+    assert st.camera_samples == W * H * 2 and st.rays_closest == W * H * 2
+    # depth 1: the closest-hit launch traces exactly the camera rays, so its
+    # reference-traversal counts equal the oracle's counters on those rays
+    _, _, _, _, nt, pt = oracle.OracleScene(desc).trace_primary(abi.settings_defaults(W, H, 2, 1), 0, 2)
+    assert st.trace_node_tests == nt
+    assert st.trace_tri_tests + st.trace_other_tests == pt
