@@ -22,6 +22,7 @@
 // (the reference's pathThroughput, which excludes Beer-Lambert) and a
 // separate radiance weight (which includes it).
 #include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -48,6 +49,7 @@ struct DevScene {
     uint32_t top_start, top_count;
     int num_lights;
     int medium;
+    int opaque_shadows;
     f3 cam_o, cam_llc, cam_h, cam_v, cam_u, cam_vv;
     float lens_radius;
     f3 sig_t, sig_s;
@@ -554,9 +556,14 @@ __global__ void __launch_bounds__(kBlock) k_raygen(DevScene S, RParams R, PathBu
     P.q_a[p] = p;
 }
 
-template <int SD, bool COUNT>
+// Traversal stack placement: LDS (lane-interleaved) or the wave's private
+// scratch (PRIV); chosen per context.
+template <int SD, bool COUNT, bool PRIV>
 __global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P, const int32_t* __restrict__ q, int count) {
-    __shared__ uint2 stk[SD * kBlock];
+    __shared__ uint2 lds_stk[PRIV ? 1 : SD * kBlock];
+    uint2 priv_stk[PRIV ? SD : 1];
+    uint2* stk = PRIV ? priv_stk : &lds_stk[threadIdx.x];
+    constexpr int STRIDE = PRIV ? 1 : kBlock;
     TravCount tc{0u, 0u, 0u};
     int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= count) {
@@ -570,8 +577,8 @@ __global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P, const 
     HitOut h;
     bool moved = false;
     for (int skip = 0;;) {
-        h = trace_closest<SD, kBlock, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
-                                             kInfinity, &stk[threadIdx.x], (uint32_t*)&P.counters[2], tc);
+        h = trace_closest<SD, STRIDE, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
+                                             kInfinity, stk, (uint32_t*)&P.counters[2], tc);
         if (h.entry < 0 || !camera) break;
         int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
         if (!(fl & kEntryCamInvisible)) break;
@@ -1049,9 +1056,12 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, RParams R, PathBuf
 }
 
 // Transparent shadow walk (ShadowRay.Transmittance, ShadowRay.cs:59-158)
-template <int SD, bool COUNT>
+template <int SD, bool COUNT, bool PRIV>
 __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, const int32_t* __restrict__ q, int count) {
-    __shared__ uint2 stk[SD * kBlock];
+    __shared__ uint2 lds_stk[PRIV ? 1 : SD * kBlock];
+    uint2 priv_stk[PRIV ? SD : 1];
+    uint2* stk = PRIV ? priv_stk : &lds_stk[threadIdx.x];
+    constexpr int STRIDE = PRIV ? 1 : kBlock;
     TravCount tc{0u, 0u, 0u};
     int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= count) {
@@ -1062,12 +1072,23 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, const
     float4 o4 = P.sh_o[sl], d4 = P.sh_d[sl];
     f3 org = f3of(o4), dir = f3of(d4);
     float rem = o4.w;
+    if (S.opaque_shadows) {
+        // Every material is opaque to shadow rays (no Dielectric, no Disney
+        // spec_trans > 0), so ShadowRay.Transmittance is 0 at the first
+        // interface and 1 without one: an any-hit query.
+        HitOut h = trace_closest<SD, STRIDE, COUNT, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org,
+                                                          dir, kEps, rem, stk, (uint32_t*)&P.counters[2], tc);
+        f3 c = h.entry < 0 ? f3of(P.sh_c[sl]) : splat(0.0f);
+        P.sh_r[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+        if (COUNT) flush_counts(P.tcount + 3, tc);
+        return;
+    }
     f3 thr = splat(1.0f), sigma = splat(0.0f), entry = splat(0.0f);
     bool in_med = false;
     bool done = false;
     for (int it = 0; it < 8 && !done; it++) {
-        HitOut h = trace_closest<SD, kBlock, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org, dir,
-                                                    kEps, rem, &stk[threadIdx.x], (uint32_t*)&P.counters[2], tc);
+        HitOut h = trace_closest<SD, STRIDE, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org, dir,
+                                                    kEps, rem, stk, (uint32_t*)&P.counters[2], tc);
         if (h.entry < 0) { done = true; break; }
         Surf s = make_surf(S, h.entry, org, dir, h.t, h.u, h.v);
         const GMat g = S.mats[s.mat];
@@ -1201,10 +1222,12 @@ __global__ void __launch_bounds__(kBlock) k_trace_primary(DevScene S, RParams R,
         o = offset_origin(o + h.t * d, d);
         if (++skip >= 8) break;
     }
+    float hu, hv;
+    report_uv(S.entries, h, o, d, hu, hv);
     prim[p] = h.entry < 0 ? -1 : __float_as_int(S.entries[h.entry].b.w);
     to[p] = h.entry < 0 ? 0.0f : h.t;
-    uo[p] = h.entry < 0 ? 0.0f : h.u;
-    vo[p] = h.entry < 0 ? 0.0f : h.v;
+    uo[p] = h.entry < 0 ? 0.0f : hu;
+    vo[p] = h.entry < 0 ? 0.0f : hv;
 }
 
 template <int SD>
@@ -1215,13 +1238,15 @@ __global__ void __launch_bounds__(kBlock) k_trace_rays(DevScene S, const float* 
     if (p >= n) return;
     const float* r = rays + 6 * (int64_t)p;
     TravCount tc{0u, 0u, 0u};
-    HitOut h = trace_closest<SD, kBlock, false>(S.nodes, S.entries, S.aux, S.top_start, S.top_count,
-                                                mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), kEps, kInfinity,
-                                                &stk[threadIdx.x], (uint32_t*)(prim + n), tc);
+    f3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
+    HitOut h = trace_closest<SD, kBlock, false>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
+                                                kInfinity, &stk[threadIdx.x], (uint32_t*)(prim + n), tc);
+    float hu, hv;
+    report_uv(S.entries, h, o, d, hu, hv);
     prim[p] = h.entry < 0 ? -1 : __float_as_int(S.entries[h.entry].b.w);
     to[p] = h.entry < 0 ? 0.0f : h.t;
-    uo[p] = h.entry < 0 ? 0.0f : h.u;
-    vo[p] = h.entry < 0 ? 0.0f : h.v;
+    uo[p] = h.entry < 0 ? 0.0f : hu;
+    vo[p] = h.entry < 0 ? 0.0f : hv;
 }
 
 }  // namespace rt
@@ -1241,6 +1266,7 @@ struct rt_ctx {
     void* film_mem = nullptr;
     size_t film_bytes = 0;
     hipEvent_t ev[8] = {};
+    bool priv_stack = false;   // traversal stack in private scratch instead of LDS
 };
 
 struct rt_scene {
@@ -1295,6 +1321,7 @@ static DevScene dev_view(const rt_scene* sc) {
     S.top_count = h.top_count;
     S.num_lights = (int)h.lights.size();
     S.medium = h.medium ? 1 : 0;
+    S.opaque_shadows = h.opaque_shadows ? 1 : 0;
     S.cam_o = h.cam_origin;
     S.cam_llc = h.cam_llc;
     S.cam_h = h.cam_horiz;
@@ -1308,17 +1335,21 @@ static DevScene dev_view(const rt_scene* sc) {
 }
 
 // Launches a traversal kernel with the LDS stack depth the scene needs.
-#define DISPATCH_SD_C(sd, cnt, KERNEL, grid, ...)                                                     \
+#define DISPATCH_SD_C(sd, cnt, priv, KERNEL, grid, ...)                                               \
     do {                                                                                               \
-        if (cnt) DISPATCH_SD_T(sd, KERNEL, true, grid, __VA_ARGS__);                                   \
-        else DISPATCH_SD_T(sd, KERNEL, false, grid, __VA_ARGS__);                                      \
+        if (cnt) {                                                                                     \
+            if (priv) DISPATCH_SD_T(sd, KERNEL, true, true, grid, __VA_ARGS__);                        \
+            else DISPATCH_SD_T(sd, KERNEL, true, false, grid, __VA_ARGS__);                            \
+        } else {                                                                                       \
+            if (priv) DISPATCH_SD_T(sd, KERNEL, false, true, grid, __VA_ARGS__);                       \
+            else DISPATCH_SD_T(sd, KERNEL, false, false, grid, __VA_ARGS__);                           \
+        }                                                                                              \
     } while (0)
-#define DISPATCH_SD_T(sd, KERNEL, C_, grid, ...)                                                          \
-    do {                                                                                                  \
-        if ((sd) <= 24) hipLaunchKernelGGL((KERNEL<24, C_>), grid, dim3(kBlock), 0, __VA_ARGS__);         \
-        else if ((sd) <= 32) hipLaunchKernelGGL((KERNEL<32, C_>), grid, dim3(kBlock), 0, __VA_ARGS__);    \
-        else if ((sd) <= 48) hipLaunchKernelGGL((KERNEL<48, C_>), grid, dim3(kBlock), 0, __VA_ARGS__);    \
-        else hipLaunchKernelGGL((KERNEL<64, C_>), grid, dim3(kBlock), 0, __VA_ARGS__);                    \
+#define DISPATCH_SD_T(sd, KERNEL, C_, P_, grid, ...)                                                        \
+    do {                                                                                                    \
+        if ((sd) <= 24) hipLaunchKernelGGL((KERNEL<24, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);       \
+        else if ((sd) <= 32) hipLaunchKernelGGL((KERNEL<32, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);  \
+        else hipLaunchKernelGGL((KERNEL<64, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);                  \
     } while (0)
 #define DISPATCH_SD(sd, KERNEL, grid, ...)                                                       \
     do {                                                                                         \
@@ -1387,6 +1418,8 @@ int rt_create(int32_t device, rt_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return RT_ERR_DEVICE;
     rt_ctx* ctx = new rt_ctx();
     ctx->device = device;
+    const char* stk = getenv("RT3D_STACK");
+    ctx->priv_stack = stk && std::string(stk) == "private";
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RT_ERR_DEVICE;
@@ -1727,7 +1760,7 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
             HIP_TRY(hipMemsetAsync(P.counters, 0, 8, ctx->stream));
             dim3 grid((count + kBlock - 1) / kBlock);
             HIP_TRY(hipEventRecord(ev[0], ctx->stream));
-            DISPATCH_SD_C(sc->stack_depth, counting, k_trace, grid, ctx->stream, S, P, qa, count);
+            DISPATCH_SD_C(sc->stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, S, P, qa, count);
             HIP_TRY(hipEventRecord(ev[1], ctx->stream));
             PathBufs Pb = P;
             Pb.q_b = qb;
@@ -1746,7 +1779,7 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
             if (cnt[2]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
             if (cnt[1] > 0) {
                 HIP_TRY(hipEventRecord(ev[3], ctx->stream));
-                DISPATCH_SD_C(sc->stack_depth, counting, k_shadow, dim3((cnt[1] + kBlock - 1) / kBlock), ctx->stream, S,
+                DISPATCH_SD_C(sc->stack_depth, counting, ctx->priv_stack, k_shadow, dim3((cnt[1] + kBlock - 1) / kBlock), ctx->stream, S,
                               P, P.q_sh, cnt[1]);
                 shadow_launches++;
                 HIP_TRY(hipEventRecord(ev[4], ctx->stream));

@@ -37,6 +37,7 @@ struct HostScene {
     float lens_radius = 0.0f;
     f3 sky;
     bool medium = false;
+    bool opaque_shadows = true;        // no material transmits shadow rays
     f3 sig_a, sig_s, sig_t;
     // scene analysis (Renderer.cs:361-378)
     Box bounds;

@@ -450,6 +450,10 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
             return RT_ERR_UNSUPPORTED;
         }
         hs.mats[i] = make_gmat(d->materials[i], hs.lut_eavg);
+        // ShadowTransmittance is non-zero only for Dielectric and for Disney
+        // with spec_trans > 0 (Dielectric.cs:46-54, DisneyBsdf.cs:570-571)
+        if (k == RT_MAT_DIELECTRIC || (k == RT_MAT_DISNEY && d->materials[i].spec_trans > 0.0f))
+            hs.opaque_shadows = false;
     }
 
     // objects (SceneLoader object list; a mesh is one object)

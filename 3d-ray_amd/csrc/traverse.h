@@ -123,20 +123,28 @@ struct TravCount {
     uint32_t node, tri, other;
 };
 
-// Tests one primitive entry; updates `h` and `closest` on a hit.
+// Tests one primitive entry (a, b, c already loaded; d is fetched only for
+// quads); updates `h` and `closest` on a hit and returns whether it hit.
 template <bool COUNT>
-__device__ __forceinline__ void test_prim(const GEntry& e, uint32_t idx, int32_t kind, int32_t flags, f3 o, f3 d,
-                                          float tmin, const float4* aux, float& closest, HitOut& h, TravCount& tc) {
+__device__ __forceinline__ bool test_prim(const GEntry* __restrict__ entries, float4 a, float4 b, float4 c,
+                                          uint32_t idx, int32_t kind, int32_t flags, f3 o, f3 d, float tmin,
+                                          const float4* aux, float& closest, HitOut& h, TravCount& tc) {
     if (COUNT) {
         if (kind == kSphere) tc.other++;
         else tc.tri++;
     }
     float t, u = 0.0f, v = 0.0f;
     bool hit;
-    if (kind == kTri) hit = hit_triangle(o, d, f3of(e.a), f3of(e.b), f3of(e.c), tmin, closest, t, u, v);
-    else if (kind == kSphere) hit = hit_sphere(o, d, f3of(e.a), e.b.x, tmin, closest, t);
-    else if (kind == kQuad) hit = hit_quad(o, d, e, aux, (flags & kEntryCulled) != 0, tmin, closest, t, u, v);
-    else hit = hit_plane(o, d, f3of(e.a), f3of(e.b), tmin, closest, t);
+    if (kind == kTri) hit = hit_triangle(o, d, f3of(a), f3of(b), f3of(c), tmin, closest, t, u, v);
+    else if (kind == kSphere) hit = hit_sphere(o, d, f3of(a), b.x, tmin, closest, t);
+    else if (kind == kQuad) {
+        GEntry e;
+        e.a = a;
+        e.b = b;
+        e.c = c;
+        e.d = entries[idx].d;
+        hit = hit_quad(o, d, e, aux, (flags & kEntryCulled) != 0, tmin, closest, t, u, v);
+    } else hit = hit_plane(o, d, f3of(a), f3of(b), tmin, closest, t);
     if (hit) {
         closest = t;
         h.entry = (int32_t)idx;
@@ -144,10 +152,15 @@ __device__ __forceinline__ void test_prim(const GEntry& e, uint32_t idx, int32_t
         h.u = u;
         h.v = v;
     }
+    return hit;
 }
 
 // Stack layout: stk[k * kStride + lane] (uint2: ref, tEnter bits).
-template <int SD, int STRIDE, bool COUNT>
+// ANY = true returns at the first primitive hit inside [tmin, tmax]: the
+// answer to "is there any hit" equals the closest-hit answer's existence,
+// because until the first hit both traversals visit the same nodes with
+// the same tMax.
+template <int SD, int STRIDE, bool COUNT, bool ANY = false>
 __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes, const GEntry* __restrict__ entries,
                                                 const float4* __restrict__ aux, uint32_t top_start, uint32_t top_count,
                                                 f3 o, f3 d, float tmin, float tmax, uint2* stk,
@@ -161,17 +174,18 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
     float closest = tmax;
     for (uint32_t ti = 0; ti < top_count; ti++) {
         uint32_t eidx = top_start + ti;
-        GEntry e = entries[eidx];
-        int32_t tag = tag_of(e.a);
+        float4 ea = entries[eidx].a, eb = entries[eidx].b, ec = entries[eidx].c;
+        int32_t tag = tag_of(ea);
         int32_t kind = tag & 0xff, flags = tag >> 8;
         if (kind != kBvh) {
-            test_prim<COUNT>(e, eidx, kind, flags, o, d, tmin, aux, closest, h, tc);
+            if (test_prim<COUNT>(entries, ea, eb, ec, eidx, kind, flags, o, d, tmin, aux, closest, h, tc) && ANY)
+                return h;
             continue;
         }
         float te;
         if (COUNT && !(flags & kEntryNoBox)) tc.node++;
-        if (!(flags & kEntryNoBox) && !slab(f3of(e.a), f3of(e.b), o, inv, tmin, closest, te)) continue;
-        uint32_t cur = (uint32_t)__float_as_int(e.b.w);
+        if (!(flags & kEntryNoBox) && !slab(f3of(ea), f3of(eb), o, inv, tmin, closest, te)) continue;
+        uint32_t cur = (uint32_t)__float_as_int(eb.w);
         int sp = 0;
         for (;;) {
             if (ref_is_leaf(cur)) {
@@ -179,22 +193,23 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
                 bool descended = false;
                 for (uint32_t k = 0; k < cnt; k++) {
                     uint32_t idx = start + k;
-                    GEntry le = entries[idx];
-                    int32_t ltag = tag_of(le.a);
+                    float4 la = entries[idx].a, lb = entries[idx].b, lc = entries[idx].c;
+                    int32_t ltag = tag_of(la);
                     int32_t lk = ltag & 0xff, lf = ltag >> 8;
                     if (lk != kBvh) {
-                        test_prim<COUNT>(le, idx, lk, lf, o, d, tmin, aux, closest, h, tc);
+                        if (test_prim<COUNT>(entries, la, lb, lc, idx, lk, lf, o, d, tmin, aux, closest, h, tc) && ANY)
+                            return h;
                         continue;
                     }
                     float lte;
                     if (COUNT && !(lf & kEntryNoBox)) tc.node++;
-                    if (!(lf & kEntryNoBox) && !slab(f3of(le.a), f3of(le.b), o, inv, tmin, closest, lte)) continue;
+                    if (!(lf & kEntryNoBox) && !slab(f3of(la), f3of(lb), o, inv, tmin, closest, lte)) continue;
                     if (k + 1 < cnt) {
                         if (sp >= SD) { *overflow = 1u; return h; }
                         stk[sp * STRIDE] = make_uint2(leaf_ref(idx + 1, cnt - k - 1), __float_as_uint(-FLT_MAX));
                         sp++;
                     }
-                    cur = (uint32_t)__float_as_int(le.b.w);
+                    cur = (uint32_t)__float_as_int(lb.w);
                     descended = true;
                     break;
                 }
@@ -243,6 +258,36 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
         }
     }
     return h;
+}
+
+// HitRecord.U/V of a reported hit: barycentrics for triangles, (alpha,
+// beta) for quads (both returned by the test), Sphere.GetSphereUV
+// (Sphere.cs:260-269) and InfinitePlane's tiled planar UV
+// (InfinitePlane.cs:45-55).
+__device__ __forceinline__ void report_uv(const GEntry* __restrict__ entries, const HitOut& h, f3 o, f3 d, float& u,
+                                          float& v) {
+    u = h.u;
+    v = h.v;
+    if (h.entry < 0) return;
+    float4 a = entries[h.entry].a, b = entries[h.entry].b;
+    int kind = tag_of(a) & 0xff;
+    if (kind == kSphere) {
+        f3 p = o + h.t * d;
+        f3 n = (p - f3of(a)) / b.x;
+        float theta = acosf(-n.y);
+        float phi = atan2f(-n.z, n.x) + kPi;
+        u = phi / (2.0f * kPi);
+        v = theta / kPi;
+    } else if (kind == kPlane) {
+        f3 n = f3of(b);
+        f3 ua = fabsf(n.y) < 0.99f ? normalize(cross(n, mk(0.0f, 1.0f, 0.0f))) : mk(1.0f, 0.0f, 0.0f);
+        f3 va = cross(n, ua);
+        f3 lp = (o + h.t * d) - f3of(a);
+        u = dot(lp, ua);
+        v = dot(lp, va);
+        u -= floorf(u);
+        v -= floorf(v);
+    }
 }
 
 }  // namespace rt

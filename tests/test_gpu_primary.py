@@ -17,8 +17,12 @@ def _primary_equal(native, oracle, ctx, desc, W, H, spp, begin=0):
     print(f"rays={p.size} hits={(p >= 0).sum()} prim-mismatch={(p != op).sum()}", flush=True)
     assert np.array_equal(p, op)
     assert np.array_equal(t.view(np.uint32), ot.view(np.uint32))
-    assert np.array_equal(u.view(np.uint32), ou.view(np.uint32))
-    assert np.array_equal(v.view(np.uint32), ov.view(np.uint32))
+    # (u, v) come from the intersection arithmetic (bit-exact) except on
+    # spheres, where GetSphereUV uses acos/atan2 (last-ulp libm differences)
+    sph = (p >= 0) & (desc.prims["kind"][np.maximum(p, 0)] == abi.RT_PRIM_SPHERE)
+    for a, b in ((u, ou), (v, ov)):
+        assert np.array_equal(a[~sph].view(np.uint32), b[~sph].view(np.uint32))
+        assert np.all(np.abs(a[sph] - b[sph]) <= 4e-7)
     return p
 
 
