@@ -64,9 +64,19 @@ struct RParams {
     uint32_t seed_xor;
     int aovs;
     int maxs;                   // shadow slots per path
-    int base_sample;            // first sample index of the chunk
-    int chunk_samples;
-    int n_paths;
+    int base_sample;            // first sample pass of the window
+};
+
+// counters[]: next-queue size, shadow-queue size (ungrouped form), one size
+// per light-sample segment of the grouped shadow queue, stack-overflow flag
+constexpr int kCntNext = 0, kCntShadow = 1, kCntRetire = 2, kCntSeg = 4, kMaxSeg = 32, kCntOverflow = 63;
+
+// Shadow queue view: segment j holds the shadow rays of light sample j
+// (in light-list order) at q_sh[j*stride ...]; pre[] is the prefix of the
+// segment sizes, so launch index i maps to segment j with pre[j] <= i < pre[j+1].
+struct ShQ {
+    int nseg, stride;
+    int pre[kMaxSeg + 1];
 };
 
 struct PathBufs {
@@ -76,8 +86,10 @@ struct PathBufs {
     float4* hit_tuv;
     float4 *av0, *av1, *av2, *av3, *av4;   // AOV probe
     float4 *sh_o, *sh_d, *sh_c, *sh_r;     // shadow slot records (N × maxs)
-    int32_t *q_a, *q_b, *q_sh;
-    int32_t* counters;          // [0] next queue, [1] shadow queue, [2] stack overflow
+    int32_t *q_a, *q_b, *q_sh, *q_r;       // trace queues, shadow queue, retired paths
+    int32_t* sid;               // sample of the window each path slot is tracing
+    float4 *res_l, *res_a, *res_n;         // per-sample results of the window: (L, depth), albedo, normal
+    int32_t* counters;          // kCnt* slots below
     unsigned long long* tcount; // traversal counters: trace node/tri/other, shadow node/tri/other
     int n;
 };
@@ -496,9 +508,10 @@ __device__ __forceinline__ f3 medium_tr(const DevScene& S, float tmax) {
 // ═════════════════════════════════════════════════════════════════════════
 // Kernels
 // ═════════════════════════════════════════════════════════════════════════
-__device__ __forceinline__ void path_sample(const RParams& R, int p, int& i, int& j, int& s) {
-    int pix = p % R.npix;
-    s = R.base_sample + p / R.npix;
+// Sample `sid` of the window: pass-major (sid = pass_local * npix + pixel).
+__device__ __forceinline__ void sample_of(const RParams& R, int sid, int& i, int& j, int& s) {
+    int pix = sid % R.npix;
+    s = R.base_sample + sid / R.npix;
     i = pix % R.W;
     j = pix / R.W;
 }
@@ -530,11 +543,10 @@ __device__ __forceinline__ void camera_ray(const DevScene& S, const RParams& R, 
     d = target - o;
 }
 
-__global__ void __launch_bounds__(kBlock) k_raygen(DevScene S, RParams R, PathBufs P) {
-    int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= R.n_paths) return;
+// Starts the camera path of window sample `sid` in slot p.
+__device__ __forceinline__ void init_path(const DevScene& S, const RParams& R, PathBufs& P, int p, int sid) {
     int i, j, s;
-    path_sample(R, p, i, j, s);
+    sample_of(R, sid, i, j, s);
     Rng rng{(uint32_t)s, pixel_seed(i, j, R.seed_xor), 0u};
     f3 o, d;
     camera_ray(S, R, i, j, rng, o, d);
@@ -553,6 +565,14 @@ __global__ void __launch_bounds__(kBlock) k_raygen(DevScene S, RParams R, PathBu
         P.av3[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         P.av4[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    P.sid[p] = sid;
+}
+
+// Initial fill of the path pool: slot p traces window sample p.
+__global__ void __launch_bounds__(kBlock) k_raygen(DevScene S, RParams R, PathBufs P, int n) {
+    int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    init_path(S, R, P, p, p);
     P.q_a[p] = p;
 }
 
@@ -578,7 +598,7 @@ __global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P, const 
     bool moved = false;
     for (int skip = 0;;) {
         h = trace_closest<SD, STRIDE, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
-                                             kInfinity, stk, (uint32_t*)&P.counters[2], tc);
+                                             kInfinity, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
         if (h.entry < 0 || !camera) break;
         int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
         if (!(fl & kEntryCamInvisible)) break;
@@ -609,12 +629,14 @@ __device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, i
     return flags & ~(0xff << kPfPendShift) & ~kPfPendInd;
 }
 
-__global__ void __launch_bounds__(kBlock) k_shade(DevScene S, RParams R, PathBufs P, const int32_t* __restrict__ q,
-                                                  int count) {
+template <int WPE>
+__global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, PathBufs P,
+                                                       const int32_t* __restrict__ q, int count) {
     int i = xcd_block() * kBlock + threadIdx.x;
     bool active = i < count;
     int p = active ? q[i] : 0;
     int n_shadow = 0;
+    uint32_t smask = 0;   // bit j: light sample j produced a shadow ray
     bool cont = false;
     // per-lane shadow records are written straight into the path's slots
     if (active) {
@@ -623,7 +645,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, RParams R, PathBuf
         flags = fold_shadows(P, R, p, flags);
         int depth = __float_as_int(thr4.w);
         int pi, pj, ps;
-        path_sample(R, p, pi, pj, ps);
+        sample_of(R, P.sid[p], pi, pj, ps);
         Rng rng{(uint32_t)ps, pixel_seed(pi, pj, R.seed_xor), __float_as_uint(wt4.w)};
         f3 o = f3of(o4), d = f3of(d4);
         float prev_pdf = o4.w;
@@ -728,9 +750,10 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, RParams R, PathBuf
                 last_l = lo + 1;
                 ppick = S.light_pdf[lo];
             }
+            int jn = 0;
             for (int li = first_l; li < last_l; li++) {
                 GLight L = S.lights[li];
-                for (int si = 0; si < L.l0.y; si++) {
+                for (int si = 0; si < L.l0.y; si++, jn++) {
                     LSample ls = light_sample(L, ps3, splat(0.0f), si, rng);
                     if (ls.skip) continue;
                     f3 tr = medium_tr(S, ls.dist);
@@ -748,6 +771,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, RParams R, PathBuf
                     P.sh_o[sl] = make_float4(ls.so.x, ls.so.y, ls.so.z, ls.tmax);
                     P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
                     P.sh_c[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+                    if (jn < kMaxSeg) smask |= 1u << jn;
                     n_shadow++;
                 }
             }
@@ -835,9 +859,10 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, RParams R, PathBuf
                     last_l = lo + 1;
                     ppick = S.light_pdf[lo];
                 }
+                int jn = 0;
                 for (int li = first_l; li < last_l; li++) {
                     GLight L = S.lights[li];
-                    for (int si = 0; si < L.l0.y; si++) {
+                    for (int si = 0; si < L.l0.y; si++, jn++) {
                         LSample ls = light_sample(L, s.p, s.n, si, rng);
                         if (ls.skip || !has_brdf) continue;
                         f3 brdf;
@@ -871,6 +896,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, RParams R, PathBuf
                         P.sh_o[sl] = make_float4(ls.so.x, ls.so.y, ls.so.z, ls.tmax);
                         P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
                         P.sh_c[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+                        if (jn < kMaxSeg) smask |= 1u << jn;
                         n_shadow++;
                     }
                 }
@@ -1049,26 +1075,42 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, RParams R, PathBuf
         }
     }
     // compaction: next queue and shadow queue (wave-aggregated)
-    int nb = wave_alloc(&P.counters[0], cont ? 1 : 0);
+    int nb = wave_alloc(&P.counters[kCntNext], cont ? 1 : 0);
     if (cont) P.q_b[nb] = p;
-    int sb = wave_alloc(&P.counters[1], n_shadow);
-    for (int k = 0; k < n_shadow; k++) P.q_sh[sb + k] = (int32_t)((int64_t)p * R.maxs + k);
+    bool retire = active && !cont;
+    int rb = wave_alloc(&P.counters[kCntRetire], retire ? 1 : 0);
+    if (retire) P.q_r[rb] = p;
+    if (R.maxs <= kMaxSeg) {
+        // shadow rays grouped by light sample: neighbouring paths' rays
+        // towards the same light share a wave (coherent traversal)
+        for (int j = 0; j < R.maxs; j++) {
+            bool has = (smask >> j) & 1u;
+            int pos = wave_alloc(&P.counters[kCntSeg + j], has ? 1 : 0);
+            if (has)
+                P.q_sh[(int64_t)j * P.n + pos] = (int32_t)((int64_t)p * R.maxs + __popc(smask & ((1u << j) - 1u)));
+        }
+    } else {
+        int sb = wave_alloc(&P.counters[kCntShadow], n_shadow);
+        for (int k = 0; k < n_shadow; k++) P.q_sh[sb + k] = (int32_t)((int64_t)p * R.maxs + k);
+    }
 }
 
 // Transparent shadow walk (ShadowRay.Transmittance, ShadowRay.cs:59-158)
 template <int SD, bool COUNT, bool PRIV>
-__global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, const int32_t* __restrict__ q, int count) {
+__global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, ShQ sq) {
     __shared__ uint2 lds_stk[PRIV ? 1 : SD * kBlock];
     uint2 priv_stk[PRIV ? SD : 1];
     uint2* stk = PRIV ? priv_stk : &lds_stk[threadIdx.x];
     constexpr int STRIDE = PRIV ? 1 : kBlock;
     TravCount tc{0u, 0u, 0u};
     int i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= count) {
+    if (i >= sq.pre[sq.nseg]) {
         if (COUNT) flush_counts(P.tcount + 3, tc);
         return;
     }
-    int64_t sl = q[i];
+    int j = 0;
+    while (j + 1 < sq.nseg && i >= sq.pre[j + 1]) j++;
+    int64_t sl = P.q_sh[(int64_t)j * sq.stride + (i - sq.pre[j])];
     float4 o4 = P.sh_o[sl], d4 = P.sh_d[sl];
     f3 org = f3of(o4), dir = f3of(d4);
     float rem = o4.w;
@@ -1077,7 +1119,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, const
         // spec_trans > 0), so ShadowRay.Transmittance is 0 at the first
         // interface and 1 without one: an any-hit query.
         HitOut h = trace_closest<SD, STRIDE, COUNT, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org,
-                                                          dir, kEps, rem, stk, (uint32_t*)&P.counters[2], tc);
+                                                          dir, kEps, rem, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
         f3 c = h.entry < 0 ? f3of(P.sh_c[sl]) : splat(0.0f);
         P.sh_r[sl] = make_float4(c.x, c.y, c.z, 0.0f);
         if (COUNT) flush_counts(P.tcount + 3, tc);
@@ -1088,7 +1130,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, const
     bool done = false;
     for (int it = 0; it < 8 && !done; it++) {
         HitOut h = trace_closest<SD, STRIDE, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org, dir,
-                                                    kEps, rem, stk, (uint32_t*)&P.counters[2], tc);
+                                                    kEps, rem, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
         if (h.entry < 0) { done = true; break; }
         Surf s = make_surf(S, h.entry, org, dir, h.t, h.u, h.v);
         const GMat g = S.mats[s.mat];
@@ -1117,46 +1159,78 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, const
     if (COUNT) flush_counts(P.tcount + 3, tc);
 }
 
+// Ends the path in slot p: folds its last shadow results and stores the
+// camera sample's radiance L = w0 * (Le0 + Ld0 + clampInd(Lind)), clamped
+// (Renderer.cs:658-666, 1222-1223), and its AOV probe, under its sample id.
+__device__ __forceinline__ void finish_path(const RParams& R, PathBufs& P, int p) {
+    int32_t flags = __float_as_int(P.d[p].w);
+    fold_shadows(P, R, p, flags);
+    f3 a0 = f3of(P.a0[p]), ai = f3of(P.ai[p]), w0 = f3of(P.w0[p]);
+    f3 L = w0 * (sanitize(a0) + clamp_lum(ai, R.ind_max_rad, true));
+    L = clamp_lum(L, R.max_rad, false);
+    int sid = P.sid[p];
+    float depth = -1.0f;
+    if (R.aovs) {
+        float4 av0 = P.av0[p], av1 = P.av1[p];
+        int aovf = __float_as_int(av0.w);
+        f3 alb, nrm;
+        if (aovf & kAovPending) {
+            alb = f3of(av0) * f3of(av1);
+            nrm = f3of(P.av2[p]);
+        } else {
+            alb = f3of(P.av3[p]);
+            nrm = f3of(P.av4[p]);
+        }
+        if (aovf & kAovHaveDepth) depth = av1.w;
+        P.res_a[sid] = make_float4(alb.x, alb.y, alb.z, 0.0f);
+        P.res_n[sid] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+    }
+    P.res_l[sid] = make_float4(L.x, L.y, L.z, depth);
+}
+
+// Retires the paths that ended in this iteration and restarts the first
+// n_regen of their slots on the next window samples (sid_base + i), appended
+// to the next trace queue.
+__global__ void __launch_bounds__(kBlock) k_retire(DevScene S, RParams R, PathBufs P, const int32_t* __restrict__ q,
+                                                   int n_ret, int n_regen, int sid_base, int32_t* q_out) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_ret) return;
+    int p = q[i];
+    finish_path(R, P, p);
+    if (i < n_regen) {
+        init_path(S, R, P, p, sid_base + i);
+        q_out[i] = p;
+    }
+}
+
 // Per-pixel film accumulation in sample order (Renderer.cs:668-672,
-// CaptureAccumulator.Add, Renderer.cs:829-856)
-__global__ void __launch_bounds__(kBlock) k_finalize(RParams R, PathBufs P, FilmBufs F) {
+// CaptureAccumulator.Add, Renderer.cs:829-856): the window's samples of a
+// pixel are summed pass by pass, even passes into half A, odd into half B.
+__global__ void __launch_bounds__(kBlock) k_accum(RParams R, PathBufs P, FilmBufs F, int passes) {
     int pix = blockIdx.x * kBlock + threadIdx.x;
     if (pix >= R.npix) return;
     const int np = R.npix;
     f3 sum = mk(F.sum[pix], F.sum[np + pix], F.sum[2 * np + pix]);
     f3 sa = mk(F.sa[pix], F.sa[np + pix], F.sa[2 * np + pix]);
     f3 sb = mk(F.sb[pix], F.sb[np + pix], F.sb[2 * np + pix]);
-    for (int sl = 0; sl < R.chunk_samples; sl++) {
+    for (int sl = 0; sl < passes; sl++) {
         int s = R.base_sample + sl;
-        if (s >= R.spp) break;
-        int p = sl * np + pix;
-        int32_t flags = __float_as_int(P.d[p].w);
-        fold_shadows(P, R, p, flags);
-        f3 a0 = f3of(P.a0[p]), ai = f3of(P.ai[p]), w0 = f3of(P.w0[p]);
-        f3 L = w0 * (sanitize(a0) + clamp_lum(ai, R.ind_max_rad, true));
-        L = clamp_lum(L, R.max_rad, false);
+        int64_t k = (int64_t)sl * np + pix;
+        float4 l = P.res_l[k];
+        f3 L = f3of(l);
         sum += L;
         bool isA = (s & 1) == 0;
         if (isA) sa += L;
         else sb += L;
         if (R.aovs) {
-            float4 av0 = P.av0[p], av1 = P.av1[p];
-            int aovf = __float_as_int(av0.w);
-            f3 alb, nrm;
-            if (aovf & kAovPending) {
-                alb = f3of(av0) * f3of(av1);
-                nrm = f3of(P.av2[p]);
-            } else {
-                alb = f3of(P.av3[p]);
-                nrm = f3of(P.av4[p]);
-            }
+            f3 alb = f3of(P.res_a[k]), nrm = f3of(P.res_n[k]);
             float* A = isA ? F.alba : F.albb;
             float* N = isA ? F.nrma : F.nrmb;
             A[pix] += alb.x; A[np + pix] += alb.y; A[2 * np + pix] += alb.z;
             N[pix] += nrm.x; N[np + pix] += nrm.y; N[2 * np + pix] += nrm.z;
-            if (aovf & kAovHaveDepth) {
-                if (isA) { F.depa[pix] += av1.w; F.dca[pix]++; }
-                else { F.depb[pix] += av1.w; F.dcb[pix]++; }
+            if (l.w >= 0.0f) {
+                if (isA) { F.depa[pix] += l.w; F.dca[pix]++; }
+                else { F.depb[pix] += l.w; F.dcb[pix]++; }
             }
         }
     }
@@ -1165,8 +1239,6 @@ __global__ void __launch_bounds__(kBlock) k_finalize(RParams R, PathBufs P, Film
     F.sb[pix] = sb.x; F.sb[np + pix] = sb.y; F.sb[2 * np + pix] = sb.z;
 }
 
-// Means, halves and the display transform (CaptureAccumulator.Store,
-// Renderer.cs:862-883; AcesToneMap, Renderer.cs:893-907)
 __global__ void k_resolve(RParams R, FilmBufs F, float exposure) {
     int pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= R.npix) return;
@@ -1207,7 +1279,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_primary(DevScene S, RParams R,
     int p = xcd_block() * kBlock + threadIdx.x;
     if (p >= n) return;
     int i, j, s;
-    path_sample(R, p, i, j, s);
+    sample_of(R, p, i, j, s);
     Rng rng{(uint32_t)s, pixel_seed(i, j, R.seed_xor), 0u};
     f3 o, d;
     camera_ray(S, R, i, j, rng, o, d);
@@ -1266,7 +1338,8 @@ struct rt_ctx {
     void* film_mem = nullptr;
     size_t film_bytes = 0;
     hipEvent_t ev[8] = {};
-    bool priv_stack = false;   // traversal stack in private scratch instead of LDS
+    bool priv_stack = true;    // traversal stack in private scratch (false: LDS)
+    int shade_wpe = 1;         // minimum waves per SIMD the shading kernel is compiled for
 };
 
 struct rt_scene {
@@ -1419,7 +1492,9 @@ int rt_create(int32_t device, rt_ctx** out) {
     rt_ctx* ctx = new rt_ctx();
     ctx->device = device;
     const char* stk = getenv("RT3D_STACK");
-    ctx->priv_stack = stk && std::string(stk) == "private";
+    ctx->priv_stack = !(stk && std::string(stk) == "lds");
+    const char* wpe = getenv("RT3D_SHADE_WPE");
+    if (wpe) ctx->shade_wpe = std::max(1, std::min(4, atoi(wpe)));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RT_ERR_DEVICE;
@@ -1603,17 +1678,24 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     const HostScene& h = sc->hs;
     const int W = s->width, H = s->height, spp = s->samples_per_pixel;
     const int npix = W * H;
-    // wavefront size: whole sample passes over the image
+    // Path pool of N slots; samples are taken in windows of K whole passes
+    // over the image whose per-sample results are summed into the film in
+    // sample order when the window completes.  A slot whose path ends is
+    // restarted on the window's next sample, so the pool stays full.
     int64_t target = s->paths_in_flight > 0 ? s->paths_in_flight : (1 << 21);
-    int K = (int)std::max<int64_t>(1, std::min<int64_t>(spp, target / npix));
-    const int64_t N = (int64_t)K * npix;
-    if (N > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "wavefront too large");
+    const int64_t kWindowCap = std::max<int64_t>(target, 1ll << 26);
+    int K = (int)std::max<int64_t>(1, std::min<int64_t>(spp, kWindowCap / npix));
+    const int64_t WS = (int64_t)K * npix;
+    const int64_t N = std::min<int64_t>(target, WS);
+    if (WS > (1ll << 31) - 1 || N > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
     const int maxs = std::max(1, s->light_sampling != RT_LIGHTS_ALL ? h.max_light_samples : h.max_shadow_per_vertex);
+    if ((int64_t)maxs * N > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many shadow slots");
     const bool aovs = s->capture_aovs != 0;
 
     // ── path buffers (grow-only cache in the context) ──────────────────────
-    size_t per_path = 8 * 16 + 8 * 4 + 4 + 16 + (aovs ? 5 * 16 : 0) + 4 * 16 * (size_t)maxs + 3 * 4 + 4 * (size_t)maxs;
-    size_t need = (size_t)N * per_path + 32 * 256 + 16384;
+    size_t per_path = 8 * 16 + 8 * 4 + 4 + 16 + (aovs ? 5 * 16 : 0) + 4 * 16 * (size_t)maxs + 5 * 4 + 4 * (size_t)maxs;
+    size_t per_sample = aovs ? 48 : 16;
+    size_t need = (size_t)N * per_path + (size_t)WS * per_sample + 40 * 256 + 16384;
     if (need > ctx->path_bytes) {
         if (ctx->path_mem) hipFree(ctx->path_mem);
         ctx->path_mem = nullptr;
@@ -1629,7 +1711,7 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     };
     PathBufs P{};
     P.n = (int)N;
-    P.counters = (int32_t*)take(64);
+    P.counters = (int32_t*)take(256);
     P.tcount = (unsigned long long*)take(64);
     P.o = (float4*)take(16 * N);
     P.d = (float4*)take(16 * N);
@@ -1656,6 +1738,13 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     P.q_a = (int32_t*)take(4 * N);
     P.q_b = (int32_t*)take(4 * N);
     P.q_sh = (int32_t*)take(4 * N * maxs);
+    P.q_r = (int32_t*)take(4 * N);
+    P.sid = (int32_t*)take(4 * N);
+    P.res_l = (float4*)take(16 * WS);
+    if (aovs) {
+        P.res_a = (float4*)take(16 * WS);
+        P.res_n = (float4*)take(16 * WS);
+    }
     if ((size_t)(m - (char*)ctx->path_mem) > ctx->path_bytes) return fail(ctx, RT_ERR_INTERNAL, "path buffer layout");
 
     // ── film accumulators ───────────────────────────────────────────────────
@@ -1739,7 +1828,6 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     R.seed_xor = s->seed_xor;
     R.aovs = aovs ? 1 : 0;
     R.maxs = maxs;
-    R.chunk_samples = K;
 
     double t_trace = 0, t_shade = 0, t_shadow = 0;
     int64_t n_closest = 0, n_shadow_rays = 0, launches = 0, shadow_launches = 0;
@@ -1747,51 +1835,88 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     HIP_TRY(hipMemsetAsync(P.tcount, 0, 64, ctx->stream));
     hipEvent_t* ev = ctx->ev;
     HIP_TRY(hipEventRecord(ev[6], ctx->stream));
+    bool shadow_timed = false;   // ev[3]..ev[4] bracket a shadow launch not yet read
+    auto read_shadow_time = [&]() {
+        if (!shadow_timed) return;
+        float ms = 0;
+        hipEventElapsedTime(&ms, ev[3], ev[4]);
+        t_shadow += ms;
+        shadow_timed = false;
+    };
     for (int base = 0; base < spp; base += K) {
-        int live_samples = std::min(K, spp - base);
+        const int passes = std::min(K, spp - base);
+        const int ws = passes * npix;
         R.base_sample = base;
-        R.n_paths = live_samples * npix;
-        HIP_TRY(hipMemsetAsync(P.counters, 0, 64, ctx->stream));
-        hipLaunchKernelGGL(k_raygen, dim3((R.n_paths + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, S, R, P);
-        int count = s->max_depth > 0 ? R.n_paths : 0;
+        const int n0 = (int)std::min<int64_t>(N, ws);
+        HIP_TRY(hipMemsetAsync(P.counters, 0, 256, ctx->stream));
+        hipLaunchKernelGGL(k_raygen, dim3((n0 + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, S, R, P, n0);
+        int next_sid = n0;
+        int count = n0;
         int32_t* qa = P.q_a;
         int32_t* qb = P.q_b;
-        for (int bounce = 0; bounce < s->max_depth && count > 0; bounce++) {
-            HIP_TRY(hipMemsetAsync(P.counters, 0, 8, ctx->stream));
-            dim3 grid((count + kBlock - 1) / kBlock);
-            HIP_TRY(hipEventRecord(ev[0], ctx->stream));
-            DISPATCH_SD_C(sc->stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, S, P, qa, count);
-            HIP_TRY(hipEventRecord(ev[1], ctx->stream));
-            PathBufs Pb = P;
-            Pb.q_b = qb;
-            hipLaunchKernelGGL(k_shade, grid, dim3(kBlock), 0, ctx->stream, S, R, Pb, qa, count);
-            HIP_TRY(hipEventRecord(ev[2], ctx->stream));
-            int32_t cnt[3];
-            HIP_TRY(hipMemcpyAsync(cnt, P.counters, 12, hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
-            float ms = 0;
-            hipEventElapsedTime(&ms, ev[0], ev[1]);
-            t_trace += ms;
-            hipEventElapsedTime(&ms, ev[1], ev[2]);
-            t_shade += ms;
-            n_closest += count;
-            launches++;
-            if (cnt[2]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
-            if (cnt[1] > 0) {
-                HIP_TRY(hipEventRecord(ev[3], ctx->stream));
-                DISPATCH_SD_C(sc->stack_depth, counting, ctx->priv_stack, k_shadow, dim3((cnt[1] + kBlock - 1) / kBlock), ctx->stream, S,
-                              P, P.q_sh, cnt[1]);
-                shadow_launches++;
-                HIP_TRY(hipEventRecord(ev[4], ctx->stream));
-                HIP_TRY(hipEventSynchronize(ev[4]));
-                hipEventElapsedTime(&ms, ev[3], ev[4]);
-                t_shadow += ms;
-                n_shadow_rays += cnt[1];
+        while (count > 0) {
+            int32_t cnt[64] = {};
+            const int32_t* retired = qa;
+            int n_ret = count;
+            if (s->max_depth > 0) {
+                HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * (kCntSeg + kMaxSeg), ctx->stream));
+                dim3 grid((count + kBlock - 1) / kBlock);
+                HIP_TRY(hipEventRecord(ev[0], ctx->stream));
+                DISPATCH_SD_C(sc->stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, S, P, qa, count);
+                HIP_TRY(hipEventRecord(ev[1], ctx->stream));
+                PathBufs Pb = P;
+                Pb.q_b = qb;
+                switch (ctx->shade_wpe) {
+                    case 2: hipLaunchKernelGGL(k_shade<2>, grid, dim3(kBlock), 0, ctx->stream, S, R, Pb, qa, count); break;
+                    case 3: hipLaunchKernelGGL(k_shade<3>, grid, dim3(kBlock), 0, ctx->stream, S, R, Pb, qa, count); break;
+                    case 4: hipLaunchKernelGGL(k_shade<4>, grid, dim3(kBlock), 0, ctx->stream, S, R, Pb, qa, count); break;
+                    default: hipLaunchKernelGGL(k_shade<1>, grid, dim3(kBlock), 0, ctx->stream, S, R, Pb, qa, count);
+                }
+                HIP_TRY(hipEventRecord(ev[2], ctx->stream));
+                HIP_TRY(hipMemcpyAsync(cnt, P.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+                HIP_TRY(hipStreamSynchronize(ctx->stream));
+                read_shadow_time();
+                float ms = 0;
+                hipEventElapsedTime(&ms, ev[0], ev[1]);
+                t_trace += ms;
+                hipEventElapsedTime(&ms, ev[1], ev[2]);
+                t_shade += ms;
+                n_closest += count;
+                launches++;
+                if (cnt[kCntOverflow]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+                ShQ sq{};
+                if (maxs <= kMaxSeg) {
+                    sq.nseg = maxs;
+                    sq.stride = (int)N;
+                    for (int j = 0; j < maxs; j++) sq.pre[j + 1] = sq.pre[j] + cnt[kCntSeg + j];
+                } else {
+                    sq.nseg = 1;
+                    sq.pre[1] = cnt[kCntShadow];
+                }
+                const int n_sh = sq.pre[sq.nseg];
+                if (n_sh > 0) {
+                    HIP_TRY(hipEventRecord(ev[3], ctx->stream));
+                    DISPATCH_SD_C(sc->stack_depth, counting, ctx->priv_stack, k_shadow, dim3((n_sh + kBlock - 1) / kBlock),
+                                  ctx->stream, S, P, sq);
+                    HIP_TRY(hipEventRecord(ev[4], ctx->stream));
+                    shadow_timed = true;
+                    shadow_launches++;
+                    n_shadow_rays += n_sh;
+                }
+                retired = P.q_r;
+                n_ret = cnt[kCntRetire];
             }
+            // retire ended paths; restart their slots on the next samples
+            const int n_regen = std::min(n_ret, ws - next_sid);
+            if (n_ret > 0)
+                hipLaunchKernelGGL(k_retire, dim3((n_ret + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, S, R, P,
+                                   retired, n_ret, n_regen, next_sid, qb + cnt[kCntNext]);
+            HIP_TRY(hipGetLastError());
+            next_sid += n_regen;
+            count = cnt[kCntNext] + n_regen;
             std::swap(qa, qb);
-            count = cnt[0];
         }
-        hipLaunchKernelGGL(k_finalize, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, R, P, F);
+        hipLaunchKernelGGL(k_accum, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, R, P, F, passes);
         HIP_TRY(hipGetLastError());
     }
     float exposure = powf(2.0f, s->exposure_ev);
@@ -1799,10 +1924,11 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[7], ctx->stream));
     HIP_TRY(hipEventSynchronize(ev[7]));
+    read_shadow_time();
     float dev_ms = 0;
     hipEventElapsedTime(&dev_ms, ev[6], ev[7]);
     int32_t ovf = 0;
-    HIP_TRY(hipMemcpy(&ovf, P.counters + 2, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&ovf, P.counters + kCntOverflow, 4, hipMemcpyDeviceToHost));
     if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
     // ── film D2H ────────────────────────────────────────────────────────────
     if (film) {

@@ -1459,6 +1459,31 @@ static V3 shadow_transmittance(const Hittable* world, const Ray& ray, float tmin
     return V3(0.0f);
 }
 
+// ── CausticPhotonTracer sampling helpers (CausticPhotonTracer.cs:348-410) ──
+// FaceToward: flips n to point from `from` toward the scene-bounds centre
+static V3 face_toward(V3 n, V3 from, const AABB& b) {
+    V3 c = 0.5f * (b.mn + b.mx);
+    return dot(n, c - from) < 0.0f ? -n : n;
+}
+// CosineDir: normalize(n + RandomUnitVector), degenerate → n
+static V3 cosine_dir(V3 n) {
+    V3 d = n + random_unit_vector();
+    if (length_sq(d) < 1e-8f) d = n;
+    return normalize(d);
+}
+static void photon_onb(V3 w, V3& t, V3& b) {
+    V3 a = std::fabs(w.x) > 0.9f ? V3(0, 1, 0) : V3(1, 0, 0);
+    t = normalize(cross(a, w));
+    b = cross(w, t);
+}
+// RandomDisk: r = sqrt(xi1), angle = 2 pi xi2
+static void photon_disk(float& x, float& y) {
+    float r = std::sqrt(random_float());
+    float a = 2.0f * kPi * random_float();
+    x = r * std::cos(a);
+    y = r * std::sin(a);
+}
+
 // ═════════════════════════════════════════════════════════════════════════
 // Lights (ILight.cs)
 // ═════════════════════════════════════════════════════════════════════════
@@ -1476,6 +1501,9 @@ struct Light {
     virtual float pdf_solid_angle(V3, V3) const { return 0.0f; }
     virtual float approximate_power(const AABB&) const = 0;
     virtual LightSample illuminate(V3 p, V3 n, const Hittable* world, int si) const = 0;
+    // CausticPhotonTracer.TryEmit (CausticPhotonTracer.cs:279-345): one photon
+    // ray and the flux of the whole light
+    virtual bool emit_photon(const AABB& b, Ray& ray, V3& flux) const = 0;
 };
 
 // SphereLight.cs:112-271
@@ -1493,6 +1521,13 @@ struct SphereLight : Light {
     }
     float approximate_power(const AABB&) const override { return 4.0f * kPi * luminance(color) * inten; }
     bool is_delta() const override { return false; }
+    bool emit_photon(const AABB&, Ray& ray, V3& flux) const override {
+        V3 dir = random_unit_vector();
+        V3 p = c + dir * r;
+        ray = Ray(offset_origin(p, dir), cosine_dir(dir));
+        flux = color * inten * (4.0f * kPi);
+        return true;
+    }
     LightSample illuminate(V3 hp, V3 sn, const Hittable* world, int si) const override {
         V3 to_c = c - hp;
         float d2 = length_sq(to_c);
@@ -1574,6 +1609,17 @@ struct AreaLight : Light {
     }
     bool is_delta() const override { return false; }
     float approximate_power(const AABB&) const override { return kPi * area * luminance(color) * inten; }
+    bool emit_photon(const AABB& b, Ray& ray, V3& flux) const override {
+        float r1 = random_float();
+        float r2 = random_float();
+        V3 p = (corner + r1 * u) + r2 * v;
+        V3 nn = normalize(cross(u, v));
+        nn = face_toward(nn, (corner + 0.5f * u) + 0.5f * v, b);
+        float ar = length(cross(u, v));
+        ray = Ray(offset_origin(p, nn), cosine_dir(nn));
+        flux = color * inten * ar * kPi;
+        return true;
+    }
     LightSample illuminate(V3 hp, V3 sn, const Hittable* world, int si) const override {
         V3 sp;
         if (si >= 0) {
@@ -1647,6 +1693,19 @@ struct DirectionalLight : Light {
         float cs = kPi * rad * rad;
         return luminance(color) * inten * cs;
     }
+    bool emit_photon(const AABB& b, Ray& ray, V3& flux) const override {
+        V3 c = 0.5f * (b.mn + b.mx);
+        float r = std::fmax(0.5f * length(b.mx - b.mn), 1e-3f);
+        V3 w = dir;
+        V3 t, bb;
+        photon_onb(w, t, bb);
+        float dx, dy;
+        photon_disk(dx, dy);
+        V3 origin = (c - w * (r * 1.5f)) + (t * dx + bb * dy) * r;
+        ray = Ray(origin, w);
+        flux = color * inten * (kPi * r * r);
+        return true;
+    }
     LightSample illuminate(V3 hp, V3 sn, const Hittable* world, int si) const override {
         V3 tl;
         if (ang_deg > 0.0f) {
@@ -1705,6 +1764,34 @@ struct GeometryLight : Light {
     }
     bool is_delta() const override { return false; }
     float approximate_power(const AABB&) const override { return kPi * rep_area * luminance(em->emission_at()); }
+    bool emit_photon(const AABB& b, Ray& ray, V3& flux) const override {
+        V3 p, nn;
+        float ar;
+        if (sph) {
+            V3 d = random_unit_vector();
+            ar = 4.0f * kPi * sph->r * sph->r;
+            p = sph->c + d * sph->r;
+            nn = d;
+        } else if (tri) {
+            float xi1 = random_float(), xi2 = random_float();
+            float r1 = std::sqrt(xi1);
+            float uu = 1.0f - r1, vv = xi2 * r1;
+            p = tri->v0 + uu * tri->e1 + vv * tri->e2;
+            nn = tri->n;
+            ar = tri->area;
+        } else {
+            float uu = random_float(), vv = random_float();
+            p = (quad->q + uu * quad->u) + vv * quad->v;
+            nn = quad->n;
+            ar = quad->area;
+        }
+        V3 e = em->emission_at();
+        if (e.x <= 0.0f && e.y <= 0.0f && e.z <= 0.0f) return false;
+        nn = face_toward(nn, p, b);
+        ray = Ray(offset_origin(p, nn), cosine_dir(nn));
+        flux = e * ar * kPi;
+        return true;
+    }
     LightSample illuminate(V3 hp, V3 sn, const Hittable* world, int si) const override {
         if (sph) {
             float inv = 1.0f / (float)sq;
@@ -1868,6 +1955,269 @@ struct Scene {
     }
 };
 
+// ═════════════════════════════════════════════════════════════════════════
+// Caustic photon pass (CausticPhotonTracer.cs:60-356, PhotonMap.cs:39-317)
+// ═════════════════════════════════════════════════════════════════════════
+struct PhotonO {
+    V3 pos, inc, pow;
+};
+
+// CausticPhotonTracer.Trace (CausticPhotonTracer.cs:134-231): follows delta
+// caustic-caster bounces (and delta legacy Scatter) until the first
+// non-specular hit, which deposits the photon iff at least one specular
+// bounce happened.  Returns whether a photon was deposited.
+static bool photon_trace(const Hittable* world, Ray ray, V3 power, PhotonO& out) {
+    float emax = fmax_p(power.x, fmax_p(power.y, power.z));
+    if (emax <= 0.0f) return false;
+    float inv_emax = 1.0f / emax;
+    int spec = 0;
+    bool in_med = false;
+    V3 sig(0.0f), entry(0.0f);
+    for (int bounce = 0; bounce < 24; bounce++) {
+        HitRecord rec;
+        if (!world->hit(ray, kEpsilon, kInfinity, rec)) return false;
+        if (in_med) {
+            float d = length(rec.point - entry);
+            power = V3(power.x * std::exp(-sig.x * d), power.y * std::exp(-sig.y * d), power.z * std::exp(-sig.z * d));
+            in_med = false;
+        }
+        const Material* m = rec.mat;
+        if (!m) return false;
+        V3 view = normalize(-ray.d);
+        auto enter_medium = [&](V3 wo) {
+            bool transmitted = dot(wo, rec.normal) < 0.0f;
+            if (!transmitted || !rec.front_face) return;
+            V3 sa = m->shadow_absorption(rec);
+            if (sa.x <= 0.0f && sa.y <= 0.0f && sa.z <= 0.0f) return;
+            in_med = true;
+            sig = sa;
+            entry = rec.point;
+        };
+        auto deposit = [&]() {
+            if (spec < 1) return false;
+            if (power.x <= 0.0f && power.y <= 0.0f && power.z <= 0.0f) return false;
+            out.pos = rec.point;
+            out.inc = normalize(-ray.d);
+            out.pow = power;
+            return true;
+        };
+        BsdfSample bs;
+        if (m->sample(view, rec, bs)) {
+            if (bs.is_delta && bs.caustic_caster) {
+                power *= bs.f;
+                enter_medium(bs.wo);
+                V3 od = dot(bs.wo, rec.normal) >= 0.0f ? rec.normal : -rec.normal;
+                ray = Ray(offset_origin(rec.point, od), bs.wo);
+                spec++;
+            } else {
+                return deposit();
+            }
+        } else {
+            V3 att;
+            Ray sc;
+            if (!m->scatter(ray, rec, att, sc)) return false;
+            if (!m->is_delta_scatter()) return deposit();
+            power *= att;
+            enter_medium(sc.d);
+            ray = sc;
+            spec++;
+        }
+        if (spec >= 4) {
+            float survive = std::fmin(1.0f, fmax_p(power.x, fmax_p(power.y, power.z)) * inv_emax);
+            if (survive <= 1e-6f || random_float() >= survive) return false;
+            power = power / survive;
+        }
+    }
+    return false;
+}
+
+// PhotonMap (PhotonMap.cs:39-317): Teschner-hashed uniform grid in CSR form.
+// Photons are kept in global photon-index order (the reference merges
+// per-thread lists in scheduling order; any order gives the same map up to
+// the order of photons inside a bucket).
+struct PhotonMapO {
+    std::vector<PhotonO> ph;
+    std::vector<int32_t> start, idx, key;
+    V3 origin;
+    float cell = 0, inv_cell = 0;
+    int mask = 0;
+
+    void build(float cell_size) {
+        cell = std::fmax(cell_size, 1e-6f);
+        inv_cell = 1.0f / cell;
+        int n = (int)ph.size();
+        int buckets = 1;
+        while (buckets < n) buckets <<= 1;
+        mask = buckets - 1;
+        origin = V3(0.0f);
+        if (n > 0) {
+            V3 mn(std::numeric_limits<float>::infinity());
+            for (auto& p : ph) mn = V3(std::fmin(mn.x, p.pos.x), std::fmin(mn.y, p.pos.y), std::fmin(mn.z, p.pos.z));
+            origin = mn;
+        }
+        // stable counting sort into CSR buckets
+        start.assign(buckets + 1, 0);
+        key.resize(n);
+        for (int i = 0; i < n; i++) {
+            key[i] = bucket_of(ph[i].pos);
+            start[key[i] + 1]++;
+        }
+        for (int b = 1; b <= buckets; b++) start[b] += start[b - 1];
+        idx.assign(n, 0);
+        std::vector<int32_t> cur(start.begin(), start.end() - 1);
+        for (int i = 0; i < n; i++) idx[cur[key[i]]++] = i;
+    }
+    void cell_of(V3 p, int& x, int& y, int& z) const {
+        x = (int)std::floor((p.x - origin.x) * inv_cell);
+        y = (int)std::floor((p.y - origin.y) * inv_cell);
+        z = (int)std::floor((p.z - origin.z) * inv_cell);
+    }
+    int bucket(int x, int y, int z) const {
+        uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+        return (int)(h & (uint32_t)mask);
+    }
+    int bucket_of(V3 p) const {
+        int x, y, z;
+        cell_of(p, x, y, z);
+        return bucket(x, y, z);
+    }
+    float cell_min_d2(V3 p, int x, int y, int z) const {
+        float bx = origin.x + (float)x * cell, by = origin.y + (float)y * cell, bz = origin.z + (float)z * cell;
+        float dx = std::fmax(std::fmax(bx - p.x, p.x - (bx + cell)), 0.0f);
+        float dy = std::fmax(std::fmax(by - p.y, p.y - (by + cell)), 0.0f);
+        float dz = std::fmax(std::fmax(bz - p.z, p.z - (bz + cell)), 0.0f);
+        return (dx * dx + dy * dy) + dz * dz;
+    }
+    static void heap_insert(int* hi, float* hd, int& count, int k, int pi, float d2) {
+        if (count < k) {
+            int i = count++;
+            hi[i] = pi; hd[i] = d2;
+            while (i > 0) {
+                int parent = (i - 1) >> 1;
+                if (hd[parent] >= hd[i]) break;
+                std::swap(hd[parent], hd[i]);
+                std::swap(hi[parent], hi[i]);
+                i = parent;
+            }
+        } else if (d2 < hd[0]) {
+            hi[0] = pi; hd[0] = d2;
+            int i = 0;
+            for (;;) {
+                int l = 2 * i + 1, r = 2 * i + 2, lg = i;
+                if (l < count && hd[l] > hd[lg]) lg = l;
+                if (r < count && hd[r] > hd[lg]) lg = r;
+                if (lg == i) break;
+                std::swap(hd[lg], hd[i]);
+                std::swap(hi[lg], hi[i]);
+                i = lg;
+            }
+        }
+    }
+    // GatherKNearest (PhotonMap.cs:209-240) with ScanShell (:243-276)
+    int gather(V3 p, int k, float max_r, int* hi, float* hd, float& radius) const {
+        radius = 0.0f;
+        if (ph.empty() || k <= 0) return 0;
+        int cx, cy, cz;
+        cell_of(p, cx, cy, cz);
+        float max_r2 = max_r * max_r;
+        int count = 0;
+        int max_rings = (int)std::ceil(max_r * inv_cell) + 1;
+        for (int ring = 0; ring <= max_rings; ring++) {
+            for (int z = cz - ring; z <= cz + ring; z++)
+                for (int y = cy - ring; y <= cy + ring; y++)
+                    for (int x = cx - ring; x <= cx + ring; x++) {
+                        int cheb = std::max(std::abs(x - cx), std::max(std::abs(y - cy), std::abs(z - cz)));
+                        if (cheb != ring) continue;
+                        float md2 = cell_min_d2(p, x, y, z);
+                        if (md2 > max_r2) continue;
+                        if (count >= k && md2 >= hd[0]) continue;
+                        int b = bucket(x, y, z);
+                        for (int kk = start[b]; kk < start[b + 1]; kk++) {
+                            int pi = idx[kk];
+                            V3 pos = ph[pi].pos;
+                            int px, py, pz;
+                            cell_of(pos, px, py, pz);
+                            if (px != x || py != y || pz != z) continue;
+                            float d2 = length_sq(pos - p);
+                            if (d2 > max_r2) continue;
+                            heap_insert(hi, hd, count, k, pi, d2);
+                        }
+                    }
+            float reach = (float)ring * cell;
+            if (count >= k && hd[0] <= reach * reach) break;
+            if (reach > max_r) break;
+        }
+        radius = count > 0 ? std::sqrt(hd[0]) : 0.0f;
+        return count;
+    }
+};
+
+// CausticPhotonTracer.Build (CausticPhotonTracer.cs:60-127): per-light photon
+// budgets by luminous power, a Sobol sub-sequence per source
+// (seed = si * 0x9E3779B1 ^ 0xC2B2AE35, index = photon index within the source).
+static bool build_photon_map(const Scene& sc, int budget, float cell_size, int nthreads, PhotonMapO& map,
+                             int64_t* emitted = nullptr) {
+    map = PhotonMapO();
+    if (budget <= 0) return false;
+    std::vector<const Light*> src;
+    std::vector<float> lum;
+    float total = 0.0f;
+    for (const Light* l : sc.light_ptrs) {
+        float p = std::fmax(0.0f, l->approximate_power(sc.bounds));
+        if (p <= 0.0f) continue;
+        src.push_back(l);
+        lum.push_back(p);
+        total += p;
+    }
+    if (src.empty() || total <= 0.0f) return false;
+    std::vector<int> counts(src.size()), offs(src.size() + 1, 0);
+    for (size_t i = 0; i < src.size(); i++) {
+        int n = (int)std::nearbyint((float)budget * (lum[i] / total));
+        counts[i] = std::max(1, n);
+        offs[i + 1] = offs[i] + counts[i];
+    }
+    const int ntot = offs[src.size()];
+    if (emitted) *emitted = ntot;
+    std::vector<PhotonO> slot(ntot);
+    std::vector<uint8_t> ok(ntot, 0);
+    std::atomic<int> next(0);
+    auto worker = [&]() {
+        for (;;) {
+            int g0 = next.fetch_add(4096);
+            if (g0 >= ntot) break;
+            int g1 = std::min(ntot, g0 + 4096);
+            for (int g = g0; g < g1; g++) {
+                int si = (int)(std::upper_bound(offs.begin(), offs.end(), g) - offs.begin()) - 1;
+                int local = g - offs[si];
+                SamplerCtx& c = t_smp;
+                c.active = true;
+                c.seed = ((uint32_t)si * 0x9E3779B1u) ^ 0xC2B2AE35u;
+                c.index = (uint32_t)local;
+                c.dim = 0;
+                c.fixed = nullptr;
+                Ray ray;
+                V3 flux;
+                if (src[si]->emit_photon(sc.bounds, ray, flux)) {
+                    V3 scale = V3(1.0f) / (float)counts[si];
+                    ok[g] = photon_trace(sc.world, ray, flux * scale, slot[g]) ? 1 : 0;
+                }
+                c.active = false;
+            }
+        }
+    };
+    if (nthreads <= 1) worker();
+    else {
+        std::vector<std::thread> th;
+        for (int k = 0; k < nthreads; k++) th.emplace_back(worker);
+        for (auto& x : th) x.join();
+    }
+    for (int g = 0; g < ntot; g++)
+        if (ok[g]) map.ph.push_back(slot[g]);
+    if (map.ph.empty()) return false;
+    map.build(cell_size);
+    return true;
+}
+
 }  // namespace orc
 
 namespace orc {
@@ -1911,6 +2261,43 @@ struct Renderer {
     int rr_min_b;
     float rr_min_s;
     AovProbe* probe = nullptr;   // per-thread (set by the render loop)
+    // photon caustics (Renderer.cs:130-152, 380-406)
+    const PhotonMapO* pmap = nullptr;
+    bool caustics = false;
+    float gather_r = 0.0f;
+    enum { kCausticNone = 0, kCausticDiffuse = 1, kCausticSuppress = 2 };
+    static int next_caustic(int st, bool delta) {
+        if (!delta) return kCausticDiffuse;
+        return st == kCausticNone ? kCausticNone : kCausticSuppress;
+    }
+    // GatherCaustics (Renderer.cs:1449-1508) + CausticOccupancyWeight (:1520-1526)
+    V3 gather_caustics(const HitRecord& rec, V3 view, const Material* m) const {
+        const int K = 40;
+        int hi[K];
+        float hd[K];
+        float radius;
+        int count = pmap->gather(rec.point, K, gather_r, hi, hd, radius);
+        if (count == 0) return V3(0.0f);
+        bool under = count < K;
+        if (under) radius = gather_r;
+        if (radius <= 0.0f) return V3(0.0f);
+        float inv_r = 1.0f / radius;
+        V3 sum(0.0f);
+        for (int i = 0; i < count; i++) {
+            const PhotonO& p = pmap->ph[hi[i]];
+            if (dot(p.inc, rec.normal) <= 0.0f) continue;
+            float w = 1.0f - length(p.pos - rec.point) * inv_r / 1.0f;
+            if (w <= 0.0f) continue;
+            sum += m->evaluate(view, p.inc, rec) * (p.pow * w);
+        }
+        const float norm = 1.0f - 2.0f / (3.0f * 1.0f);
+        V3 est = sum * (1.0f / (norm * kPi * radius * radius));
+        if (under) {
+            float t = (float)count / (float)K;
+            est *= t * t * (3.0f - 2.0f * t);
+        }
+        return est;
+    }
 
     Renderer(const Scene& s, const rt_settings& st) : sc(s) {
         max_depth = st.max_depth;
@@ -2089,7 +2476,7 @@ struct Renderer {
 
     // ShadeSurface (Renderer.cs:1008-1228)
     V3 shade_surface(const Ray& ray, HitRecord& rec, int depth, float prev_pdf, bool prev_delta, V3 absorption, V3 beta,
-                     IorStack& iors) const {
+                     IorStack& iors, int cst) const {
         const Material* m = rec.mat;
         rec.shading_ng = rec.normal;
         if (capture_aovs && probe && probe->pending) {
@@ -2097,7 +2484,7 @@ struct Renderer {
             probe->tent_alb = m ? clamp01(m->aov_albedo(rec)) : V3(1.0f);
         }
         V3 emitted(0.0f);
-        if (m) {
+        if (m && !(caustics && cst == kCausticSuppress)) {
             V3 raw = m->emit(rec.front_face);
             if (raw.x > 0.0f || raw.y > 0.0f || raw.z > 0.0f) emitted = weight_emission(raw, m, ray, prev_pdf, prev_delta);
         }
@@ -2107,10 +2494,12 @@ struct Renderer {
             rec.relative_eta = rec.front_face ? (iors.top() / mior) : (mior / iors.enclosing());
         bool needs = m ? m->needs_direct_lighting() : true;
         V3 direct = needs ? direct_lighting(rec, view, m) : V3(0.0f);
+        if (caustics && needs && m && !m->is_delta_scatter()) direct += gather_caustics(rec, view, m);
         if (!m) return emitted + direct;
 
         BsdfSample s;
-        if (m->sample(view, rec, s)) return shade_sample_bounce(m, rec, s, depth, emitted, direct, absorption, beta, iors);
+        if (m->sample(view, rec, s))
+            return shade_sample_bounce(m, rec, s, depth, emitted, direct, absorption, beta, iors, cst);
 
         V3 att;
         Ray scat;
@@ -2138,7 +2527,7 @@ struct Renderer {
                 if (rec.front_face) ni.push(sior);
                 else ni.pop();
             }
-            V3 ind = trace_ray(scat, depth - 1, 0.0f, nd, ni, absorption, nt, cat);
+            V3 ind = trace_ray(scat, depth - 1, 0.0f, nd, ni, absorption, nt, cat, next_caustic(cst, nd));
             ind = sanitize(ind);
             V3 contrib = att * ind;
             if (depth == max_depth) contrib = clamp_indirect(contrib);
@@ -2149,7 +2538,7 @@ struct Renderer {
 
     // ShadeSampleBounce (Renderer.cs:1235-1383)
     V3 shade_sample_bounce(const Material* m, const HitRecord& rec, const BsdfSample& s, int depth, V3 emitted, V3 direct,
-                           V3 absorption, V3 beta, IorStack& iors) const {
+                           V3 absorption, V3 beta, IorStack& iors, int cst) const {
         V3 att;
         if (s.is_delta) att = s.f;
         else {
@@ -2185,9 +2574,9 @@ struct Renderer {
                 float pi;
                 ni.push(m->try_dielectric_ior(rec, pi) ? pi : iors.top());
             } else ni.pop();
-            ind = trace_ray(scat, depth - 1, npdf, s.is_delta, ni, nabs, nt, cat);
+            ind = trace_ray(scat, depth - 1, npdf, s.is_delta, ni, nabs, nt, cat, next_caustic(cst, s.is_delta));
         } else {
-            ind = trace_ray(scat, depth - 1, npdf, s.is_delta, iors, nabs, nt, cat);
+            ind = trace_ray(scat, depth - 1, npdf, s.is_delta, iors, nabs, nt, cat, next_caustic(cst, s.is_delta));
         }
         ind = sanitize(ind);
         V3 contrib = att * ind;
@@ -2197,7 +2586,7 @@ struct Renderer {
 
     // TraceRay (Renderer.cs:1581-1783)
     V3 trace_ray(const Ray& ray, int depth, float prev_pdf, bool prev_delta, IorStack& iors, V3 absorption, V3 beta,
-                 int cat) const {
+                 int cat, int cst = kCausticNone) const {
         if (depth <= 0) return V3(0.0f);
         float bm = fmax_p(beta.x, fmax_p(beta.y, beta.z));
         if (bm < 1e-4f) return V3(0.0f);
@@ -2217,7 +2606,7 @@ struct Renderer {
             probe->have_depth = true;
         }
         if (!sc.medium) {
-            V3 r = !hit ? sample_sky() : shade_surface(cur, rec, depth, prev_pdf, prev_delta, absorption, beta, iors);
+            V3 r = !hit ? sample_sky() : shade_surface(cur, rec, depth, prev_pdf, prev_delta, absorption, beta, iors, cst);
             return beer_lambert(r, absorption, hit ? rec.t : std::numeric_limits<float>::infinity());
         }
         float tmax = hit ? rec.t : 1e30f;
@@ -2252,12 +2641,13 @@ struct Renderer {
                 float ppdf = kInv4Pi, pval = kInv4Pi;
                 float pw = ppdf > 1e-20f ? pval / ppdf : 0.0f;
                 V3 nt = mt * (pw * iscale);
-                lind = pw * iscale * trace_ray(Ray(p, wi), depth - 1, ppdf, false, iors, absorption, nt, kCatCamera);
+                lind = pw * iscale * trace_ray(Ray(p, wi), depth - 1, ppdf, false, iors, absorption, nt, kCatCamera,
+                                               kCausticDiffuse);
             }
             return beer_lambert(mb * (lnee + lind), absorption, tmed);
         }
         V3 r = !hit ? mb * sample_sky()
-                    : mb * shade_surface(cur, rec, depth, prev_pdf, prev_delta, absorption, beta * mb, iors);
+                    : mb * shade_surface(cur, rec, depth, prev_pdf, prev_delta, absorption, beta * mb, iors, cst);
         return beer_lambert(r, absorption, hit ? rec.t : std::numeric_limits<float>::infinity());
     }
 
@@ -2516,7 +2906,7 @@ static int check_settings(const rt_settings* st) {
         return RT_ERR_INVALID_ARGUMENT;
     }
     if (st->sampler != RT_SAMPLER_SOBOL) { g_err = "only the Sobol sampler is implemented"; return RT_ERR_UNSUPPORTED; }
-    if (st->caustics) { g_err = "photon caustics not implemented"; return RT_ERR_UNSUPPORTED; }
+    if (st->caustics && st->caustic_photons < 0) { g_err = "negative photon budget"; return RT_ERR_INVALID_ARGUMENT; }
     return 0;
 }
 
@@ -2607,6 +2997,15 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
     int rc = check_settings(st);
     if (rc) return rc;
     if (row_step < 1) row_step = 1;
+    // photon pre-pass (Renderer.cs:385-406): gather-radius cap 6 % of the
+    // scene radius (or the settings override), grid cell half of it
+    PhotonMapO pm;
+    bool caus = false;
+    float gr = 0.0f;
+    if (st->caustics && st->caustic_photons > 0 && !s->light_ptrs.empty()) {
+        gr = st->caustic_gather_radius > 0.0f ? st->caustic_gather_radius : std::fmax(s->scene_radius * 0.06f, 1e-4f);
+        caus = build_photon_map(*s, st->caustic_photons, gr * 0.5f, std::max(1, (int)nthreads), pm);
+    }
     Renderer R(*s, *st);
     const int W = st->width, H = st->height, spp = st->samples_per_pixel;
     const int64_t np = (int64_t)W * H;
@@ -2620,7 +3019,10 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
         AovProbe probe;
         Renderer RR(*s, *st);
         RR.probe = &probe;
-        t_block_spec_trans = false;
+        RR.pmap = caus ? &pm : nullptr;
+        RR.caustics = caus;
+        RR.gather_r = gr;
+        t_block_spec_trans = caus;
         t_shadow_segments = 0;
         for (;;) {
             int tile = next.fetch_add(1);
@@ -2700,6 +3102,78 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
     (void)seg_closest;
     return 0;
 }
+
+// ── Photon map probes ──────────────────────────────────────────────────────
+// Builds the caustic photon map of a scene (CausticPhotonTracer.Build +
+// PhotonMap ctor).  Returns a handle (null when no photon was deposited);
+// *n_photons / *n_emitted / *n_buckets describe it.
+void* orc_photon_build(void* sp, int32_t budget, float cell, int32_t nthreads, int64_t* n_photons, int64_t* n_emitted,
+                       int64_t* n_buckets) {
+    Scene* s = (Scene*)sp;
+    PhotonMapO* m = new PhotonMapO();
+    int64_t em = 0;
+    bool ok = build_photon_map(*s, budget, cell, std::max(1, (int)nthreads), *m, &em);
+    if (n_emitted) *n_emitted = em;
+    if (!ok) {
+        delete m;
+        if (n_photons) *n_photons = 0;
+        if (n_buckets) *n_buckets = 0;
+        return nullptr;
+    }
+    if (n_photons) *n_photons = (int64_t)m->ph.size();
+    if (n_buckets) *n_buckets = (int64_t)m->mask + 1;
+    return m;
+}
+
+// Copies the map out: pos/inc/pow 3 floats per photon, key (bucket) and
+// idx per photon, start (n_buckets + 1), grid = origin xyz, cell.
+void orc_photon_get(void* h, float* pos, float* inc, float* pw, int32_t* key, int32_t* idx, int32_t* start,
+                    float* grid) {
+    PhotonMapO* m = (PhotonMapO*)h;
+    for (size_t i = 0; i < m->ph.size(); i++) {
+        const PhotonO& p = m->ph[i];
+        if (pos) { pos[3 * i] = p.pos.x; pos[3 * i + 1] = p.pos.y; pos[3 * i + 2] = p.pos.z; }
+        if (inc) { inc[3 * i] = p.inc.x; inc[3 * i + 1] = p.inc.y; inc[3 * i + 2] = p.inc.z; }
+        if (pw) { pw[3 * i] = p.pow.x; pw[3 * i + 1] = p.pow.y; pw[3 * i + 2] = p.pow.z; }
+        if (key) key[i] = m->key[i];
+        if (idx) idx[i] = m->idx[i];
+    }
+    if (start) std::copy(m->start.begin(), m->start.end(), start);
+    if (grid) { grid[0] = m->origin.x; grid[1] = m->origin.y; grid[2] = m->origin.z; grid[3] = m->cell; }
+}
+
+// Builds a photon map from given photons (positions only matter for the
+// grid) — the host sort the device grid is compared with.
+void* orc_photon_map_from(const float* pos, int64_t n, float cell) {
+    PhotonMapO* m = new PhotonMapO();
+    m->ph.resize(n);
+    for (int64_t i = 0; i < n; i++) {
+        m->ph[i].pos = V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        m->ph[i].inc = V3(0, 1, 0);
+        m->ph[i].pow = V3(0.0f);
+    }
+    m->build(cell);
+    return m;
+}
+
+// k-nearest gather for a batch of points: count, kernel radius and the
+// gathered photon indices (k per point, -1 padded).
+void orc_photon_gather(void* h, const float* pts, int64_t n, int32_t k, float max_r, int32_t* count, float* radius,
+                       int32_t* out_idx) {
+    PhotonMapO* m = (PhotonMapO*)h;
+    std::vector<int> hi(std::max(1, (int)k));
+    std::vector<float> hd(std::max(1, (int)k));
+    for (int64_t i = 0; i < n; i++) {
+        float r;
+        int c = m->gather(V3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), k, max_r, hi.data(), hd.data(), r);
+        if (count) count[i] = c;
+        if (radius) radius[i] = r;
+        if (out_idx)
+            for (int j = 0; j < k; j++) out_idx[i * k + j] = j < c ? hi[j] : -1;
+    }
+}
+
+void orc_photon_free(void* h) { delete (PhotonMapO*)h; }
 
 // ── Unit probes for material / light / LUT tests ─────────────────────────
 // A shading point is described by (normal, tangent, bitangent, front_face,

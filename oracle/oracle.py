@@ -53,6 +53,16 @@ def lib():
         L.orc_bsdf_eval_batch.argtypes = [C.POINTER(abi.rt_material), fp, fp, fp, C.c_int64, fp, fp, fp]
         L.orc_bsdf_sample_batch.argtypes = [C.POINTER(abi.rt_material), fp, fp, C.c_int64, C.c_uint64, fp, fp, fp, ip]
         L.orc_trace_rays_ex.argtypes = [vp, fp, C.c_int64, C.c_float, C.c_float, C.c_int32, ip, fp]
+        L.orc_photon_build.restype = vp
+        L.orc_photon_build.argtypes = [vp, C.c_int32, C.c_float, C.c_int32, i64p, i64p, i64p]
+        L.orc_photon_get.argtypes = [vp, fp, fp, fp, ip, ip, ip, fp]
+        L.orc_photon_get.restype = None
+        L.orc_photon_map_from.restype = vp
+        L.orc_photon_map_from.argtypes = [fp, C.c_int64, C.c_float]
+        L.orc_photon_gather.argtypes = [vp, fp, C.c_int64, C.c_int32, C.c_float, ip, fp, ip]
+        L.orc_photon_gather.restype = None
+        L.orc_photon_free.argtypes = [vp]
+        L.orc_photon_free.restype = None
         _lib = L
     return _lib
 
@@ -170,6 +180,13 @@ class OracleScene:
         lib().orc_light_sample(self.h, li, _f(p), _f(n), si, _f(rnd), len(rnd), _i(ins), _f(col), _f(dr), _f(dist), _f(pdf))
         return bool(ins[0]), col, dr, float(dist[0]), float(pdf[0])
 
+    def photon_map(self, budget, cell, threads=None):
+        """CausticPhotonTracer.Build + PhotonMap: the deposited photons (in
+        global photon-index order) and the CSR hash grid."""
+        n, em, nb = np.zeros(1, np.int64), np.zeros(1, np.int64), np.zeros(1, np.int64)
+        h = lib().orc_photon_build(self.h, budget, cell, threads or os.cpu_count() or 1, _i64(n), _i64(em), _i64(nb))
+        return PhotonMap(h, int(n[0]), int(nb[0]), emitted=int(em[0]))
+
     def light_pdf(self, li, p, wi):
         p = np.ascontiguousarray(p, np.float32)
         wi = np.ascontiguousarray(wi, np.float32)
@@ -243,3 +260,43 @@ def trace_rays_ex(scene, rays, tmin, tmax, linear):
     t = np.empty(n, np.float32)
     lib().orc_trace_rays_ex(scene.h, _f(rays), n, tmin, tmax, 1 if linear else 0, _i(prim), _f(t))
     return prim, t
+
+
+class PhotonMap:
+    """Oracle photon map (PhotonMap.cs:39-317)."""
+
+    def __init__(self, h, n, n_buckets, emitted=0):
+        self.h, self.n, self.n_buckets, self.emitted = h, n, n_buckets, emitted
+        self.pos = np.zeros((n, 3), np.float32)
+        self.inc = np.zeros((n, 3), np.float32)
+        self.power = np.zeros((n, 3), np.float32)
+        self.key = np.zeros(n, np.int32)
+        self.idx = np.zeros(n, np.int32)
+        self.start = np.zeros(n_buckets + 1, np.int32)
+        self.grid = np.zeros(4, np.float32)
+        if h:
+            lib().orc_photon_get(h, _f(self.pos), _f(self.inc), _f(self.power), _i(self.key), _i(self.idx),
+                                 _i(self.start), _f(self.grid))
+
+    @classmethod
+    def from_positions(cls, pos, cell):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        h = lib().orc_photon_map_from(_f(pos), len(pos), cell)
+        nb = 1
+        while nb < len(pos):
+            nb <<= 1
+        return cls(h, len(pos), nb)
+
+    def gather(self, pts, k, max_r):
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        n = len(pts)
+        cnt, rad = np.zeros(n, np.int32), np.zeros(n, np.float32)
+        idx = np.full((n, k), -1, np.int32)
+        if self.h:
+            lib().orc_photon_gather(self.h, _f(pts), n, k, max_r, _i(cnt), _f(rad), _i(idx))
+        return cnt, rad, idx
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_photon_free(self.h)
+            self.h = None

@@ -17,13 +17,17 @@ def _primary_equal(native, oracle, ctx, desc, W, H, spp, begin=0):
     print(f"rays={p.size} hits={(p >= 0).sum()} prim-mismatch={(p != op).sum()}", flush=True)
     assert np.array_equal(p, op)
     assert np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+    _uv_equal(desc, p, u, v, ou, ov)
+    return p
+
+
+def _uv_equal(desc, p, u, v, ou, ov):
     # (u, v) come from the intersection arithmetic (bit-exact) except on
     # spheres, where GetSphereUV uses acos/atan2 (last-ulp libm differences)
     sph = (p >= 0) & (desc.prims["kind"][np.maximum(p, 0)] == abi.RT_PRIM_SPHERE)
     for a, b in ((u, ou), (v, ov)):
         assert np.array_equal(a[~sph].view(np.uint32), b[~sph].view(np.uint32))
         assert np.all(np.abs(a[sph] - b[sph]) <= 4e-7)
-    return p
 
 
 def test_cornell_primary_hits_bit_exact(native, oracle, gpu_ctx):
@@ -75,8 +79,8 @@ def test_random_rays_bit_exact_triangles(native, oracle, gpu_ctx):
     op, ot, ou, ov = oracle.OracleScene(d).trace_rays(rays)
     assert (p >= 0).sum() > 1000
     assert np.array_equal(p, op)
-    for a, b in ((t, ot), (u, ou), (v, ov)):
-        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+    _uv_equal(d, p, u, v, ou, ov)
 
 
 def test_scene_info_matches_oracle(native, oracle, gpu_ctx):

@@ -115,6 +115,19 @@ def test_determinism_and_halves(native, gpu_ctx):
     assert a.pixels.shape == (64, 64, 3) and a.pixels.min() >= 0 and a.pixels.max() <= 1
 
 
+def test_path_pool_size_does_not_change_the_film(native, gpu_ctx):
+    # a small pool restarts slots on new samples many times per window; the
+    # per-sample results and their sample-order sum must be bit-identical
+    desc = scenes.cornell(48, 40)
+    sc = native.Scene(gpu_ctx, desc)
+    big = native.Renderer(sc, 6, 8).render(48, 40, capture_aovs=True)
+    for pool in (64, 777):
+        small = native.Renderer(sc, 6, 8, paths_in_flight=pool).render(48, 40, capture_aovs=True)
+        print(f"pool={pool}", flush=True)
+        for k in ("beauty", "beauty_a", "beauty_b", "albedo_a", "normal_b", "depth_a"):
+            assert np.array_equal(getattr(big.buffers, k), getattr(small.buffers, k)), (pool, k)
+
+
 def test_single_sample_b_half_mirrors_a(native, gpu_ctx):
     sc = native.Scene(gpu_ctx, scenes.cornell(32, 32))
     r = native.Renderer(sc, 1, 8).render(32, 32, capture_aovs=True)
