@@ -50,6 +50,8 @@ struct DevScene {
     int num_lights;
     int medium;
     int opaque_shadows;
+    int caustics;               // photon map active: gather at diffuse/glossy hits, suppress L S+ re-hits
+    int block_spec;             // ShadowRay.BlockSpecularTransmission
     f3 cam_o, cam_llc, cam_h, cam_v, cam_u, cam_vv;
     float lens_radius;
     f3 sig_t, sig_s;
@@ -69,7 +71,8 @@ struct RParams {
 
 // counters[]: next-queue size, shadow-queue size (ungrouped form), one size
 // per light-sample segment of the grouped shadow queue, stack-overflow flag
-constexpr int kCntNext = 0, kCntShadow = 1, kCntRetire = 2, kCntSeg = 4, kMaxSeg = 32, kCntOverflow = 63;
+constexpr int kCntNext = 0, kCntShadow = 1, kCntRetire = 2, kCntGather = 3, kCntSeg = 4, kMaxSeg = 32,
+              kCntOverflow = 63;
 
 // Shadow queue view: segment j holds the shadow rays of light sample j
 // (in light-list order) at q_sh[j*stride ...]; pre[] is the prefix of the
@@ -89,6 +92,7 @@ struct PathBufs {
     int32_t *q_a, *q_b, *q_sh, *q_r;       // trace queues, shadow queue, retired paths
     int32_t* sid;               // sample of the window each path slot is tracing
     float4 *res_l, *res_a, *res_n;         // per-sample results of the window: (L, depth), albedo, normal
+    float4 *gq0, *gq1, *gq2, *gq3;         // caustic gather requests: (o,t) (d,entry) (u,v,eta,path) (weight,target)
     int32_t* counters;          // kCnt* slots below
     unsigned long long* tcount; // traversal counters: trace node/tri/other, shadow node/tri/other
     int n;
@@ -121,7 +125,14 @@ enum : int32_t {
     kPfTgtInd = 8,             // contributions go to the indirect accumulator
     kPfPendShift = 8,          // 8 bits: pending shadow slots
     kPfPendInd = 1 << 16,      // pending shadows belong to the indirect accumulator
+    kPfCstShift = 17,          // 2 bits: caustic-chain state (none, diffuse, suppress)
 };
+enum : int { kCstNone = 0, kCstDiffuse = 1, kCstSuppress = 2 };
+// NextCausticState (Renderer.cs:1434-1438)
+__device__ __forceinline__ int next_cst(int st, bool delta) {
+    if (!delta) return kCstDiffuse;
+    return st == kCstNone ? kCstNone : kCstSuppress;
+}
 // AOV probe flag bits (av0.w)
 enum : int32_t { kAovPending = 1, kAovHaveDepth = 2 };
 
@@ -652,6 +663,8 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
         bool prev_delta = (flags & kPfPrevDelta) != 0;
         int cat = pf_cat(flags);
         bool tgt = (flags & kPfTgtInd) != 0;
+        const int cst = (flags >> kPfCstShift) & 3;
+        int next_cs = cst;
         f3 thr = f3of(thr4), wt = f3of(wt4);
         float4 ab4 = P.ab[p];
         f3 sigma_abs = f3of(ab4);
@@ -799,6 +812,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
                 next_pdf = kInv4Pi;
                 next_delta = false;
                 next_cat = 0;
+                next_cs = kCstDiffuse;
                 cont = true;
             }
         } else if (!hit) {
@@ -818,7 +832,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
                 av1 = make_float4(ta.x, ta.y, ta.z, av1.w);
             }
             // emission with MIS (WeightEmission, Renderer.cs:1549-1565)
-            if (kind == RT_MAT_EMISSIVE && s.front) {
+            if (kind == RT_MAT_EMISSIVE && s.front && !(S.caustics && cst == kCstSuppress)) {
                 f3 raw = f3of(g.m1);
                 if (raw.x > 0.0f || raw.y > 0.0f || raw.z > 0.0f) {
                     f3 em = raw;
@@ -901,6 +915,14 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
                     }
                 }
             }
+            // ── caustic gather request (Renderer.cs:1108-1109) ────────────
+            if (S.caustics && (kind == RT_MAT_LAMBERTIAN || kind == RT_MAT_DISNEY)) {
+                int gi = atomicAdd(&P.counters[kCntGather], 1);
+                P.gq0[gi] = make_float4(o.x, o.y, o.z, htuv.x);
+                P.gq1[gi] = make_float4(d.x, d.y, d.z, __int_as_float(he));
+                P.gq2[gi] = make_float4(htuv.y, htuv.z, s.rel_eta, __int_as_float(p));
+                P.gq3[gi] = make_float4(vw.x, vw.y, vw.z, __int_as_float(tgt ? 1 : 0));
+            }
             // ── indirect bounce ─────────────────────────────────────────
             bool have = false, legacy = false, is_delta = false;
             f3 att = splat(0.0f), wo = splat(0.0f), origin = s.p;
@@ -934,20 +956,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
                     if (disney_scatter(DP, S.lut_e, s, d, rng, att, wo, lobe)) legacy = true;
                 }
             } else if (kind == RT_MAT_DIELECTRIC) {
-                // Dielectric.Scatter (Dielectric.cs:75-105)
-                float ior = g.m3.w;
-                float ri = s.rel_eta > 0.0f ? s.rel_eta : (s.front ? (1.0f / ior) : ior);
-                f3 ud = normalize(d);
-                float ct = fminf(dot(-ud, s.n), 1.0f);
-                float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
-                bool cannot = ri * st > 1.0f;
-                f3 dir;
-                if (cannot || schlick(ct, ri) > rng.next()) dir = reflect(ud, s.n);
-                else dir = refract(ud, s.n, ri);
-                f3 on = dot(dir, s.n) >= 0.0f ? s.n : -s.n;
-                origin = offset_origin(s.p, on);
-                wo = dir;
-                att = f3of(g.m1);
+                dielectric_scatter(g, s, d, rng, att, wo, origin);
                 legacy = true;
                 is_delta = true;
             }
@@ -1002,6 +1011,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
                         thr_next = nt;
                         wt_next = primary ? att : wt * att;
                         if (primary) next_tgt = true;
+                        next_cs = next_cst(cst, bs.delta);
                         cont = true;
                     }
                 }
@@ -1047,6 +1057,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
                     thr_next = nt;
                     wt_next = primary ? att : wt * att;
                     if (primary) next_tgt = true;
+                    next_cs = next_cst(cst, is_delta);
                     cont = true;
                 }
             }
@@ -1059,8 +1070,9 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
             if (depth - 1 <= 0 || fmax3(thr_next) < 1e-4f) cont = false;
         }
         if (cont) {
-            flags &= ~(kPfPrevDelta | (3 << kPfCatShift) | kPfTgtInd);
-            flags |= (next_delta ? kPfPrevDelta : 0) | (next_cat << kPfCatShift) | (next_tgt ? kPfTgtInd : 0);
+            flags &= ~(kPfPrevDelta | (3 << kPfCatShift) | kPfTgtInd | (3 << kPfCstShift));
+            flags |= (next_delta ? kPfPrevDelta : 0) | (next_cat << kPfCatShift) | (next_tgt ? kPfTgtInd : 0) |
+                     (next_cs << kPfCstShift);
             P.o[p] = make_float4(next_o.x, next_o.y, next_o.z, next_pdf);
             P.d[p] = make_float4(next_d.x, next_d.y, next_d.z, __int_as_float(flags));
             P.thr[p] = make_float4(thr_next.x, thr_next.y, thr_next.z, __int_as_float(depth - 1));
@@ -1093,6 +1105,58 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
         int sb = wave_alloc(&P.counters[kCntShadow], n_shadow);
         for (int k = 0; k < n_shadow; k++) P.q_sh[sb + k] = (int32_t)((int64_t)p * R.maxs + k);
     }
+}
+
+}  // namespace rt
+#include "photon.h"
+namespace rt {
+
+// Caustic radiance at the diffuse/glossy vertices shaded this iteration
+// (Renderer.GatherCaustics, Renderer.cs:1449-1508; CausticOccupancyWeight
+// :1520-1526): k nearest photons, cone filter k_f = 1 (normalisation 1/3),
+// adaptive radius, occupancy fade for sparse gathers.  Added to the target
+// accumulator of the path with the vertex weight.
+__global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPhotons M, int count) {
+    int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= count) return;
+    float4 a = P.gq0[i], b = P.gq1[i], c = P.gq2[i], w4 = P.gq3[i];
+    f3 o = f3of(a), d = f3of(b);
+    Surf s = make_surf(S, __float_as_int(b.w), o, d, a.w, c.x, c.y);
+    s.rel_eta = c.z;
+    const int p = __float_as_int(c.w);
+    int hi[kGatherK];
+    float hd[kGatherK];
+    float radius;
+    int n = photon_knn(M, s.p, hi, hd, radius);
+    if (n == 0) return;
+    bool under = n < kGatherK;
+    if (under) radius = M.max_r;
+    if (radius <= 0.0f) return;
+    const GMat g = S.mats[s.mat];
+    const bool disney = g.m0.x == RT_MAT_DISNEY;
+    DisneyP DP;
+    if (disney) DP = load_disney(g);
+    f3 view = normalize(-d);
+    float inv_r = 1.0f / radius;
+    f3 sum = splat(0.0f);
+    for (int k = 0; k < n; k++) {
+        int pi = hi[k];
+        f3 inc = f3of(M.inc[pi]);
+        if (dot(inc, s.n) <= 0.0f) continue;
+        float wgt = 1.0f - length(f3of(M.pos[pi]) - s.p) * inv_r / 1.0f;
+        if (wgt <= 0.0f) continue;
+        f3 f;
+        if (disney) f = disney_eval(DP, S.lut_e, view, inc, s);
+        else f = dot(s.n, inc) <= 0.0f ? splat(0.0f) : f3of(g.m1) * (1.0f / kPi);
+        sum += f * (f3of(M.pw[pi]) * wgt);
+    }
+    const float norm = 1.0f - 2.0f / (3.0f * 1.0f);
+    f3 est = sum * (1.0f / (norm * kPi * radius * radius));
+    if (under) {
+        float t = (float)n / (float)kGatherK;
+        est *= t * t * (3.0f - 2.0f * t);
+    }
+    add_target(P, p, __float_as_int(w4.w) != 0, sanitize(f3of(w4) * est));
 }
 
 // Transparent shadow walk (ShadowRay.Transmittance, ShadowRay.cs:59-158)
@@ -1134,6 +1198,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, ShQ s
         if (h.entry < 0) { done = true; break; }
         Surf s = make_surf(S, h.entry, org, dir, h.t, h.u, h.v);
         const GMat g = S.mats[s.mat];
+        if (S.block_spec && (g.m0.y & kMatSpecTransmissive)) { thr = splat(0.0f); done = true; break; }
         if (in_med) {
             float dd = length(s.p - entry);
             thr = mk(thr.x * expf(-sigma.x * dd), thr.y * expf(-sigma.y * dd), thr.z * expf(-sigma.z * dd));
@@ -1339,7 +1404,9 @@ struct rt_ctx {
     size_t film_bytes = 0;
     hipEvent_t ev[8] = {};
     bool priv_stack = true;    // traversal stack in private scratch (false: LDS)
-    int shade_wpe = 1;         // minimum waves per SIMD the shading kernel is compiled for
+    int shade_wpe = 2;         // minimum waves per SIMD the shading kernel is compiled for
+    void* ph_mem = nullptr;    // photon pass buffers (grow-only)
+    size_t ph_bytes = 0;
 };
 
 struct rt_scene {
@@ -1404,6 +1471,8 @@ static DevScene dev_view(const rt_scene* sc) {
     S.lens_radius = h.lens_radius;
     S.sig_t = h.sig_t;
     S.sig_s = h.sig_s;
+    S.caustics = 0;
+    S.block_spec = 0;
     return S;
 }
 
@@ -1432,13 +1501,163 @@ static DevScene dev_view(const rt_scene* sc) {
         else hipLaunchKernelGGL(KERNEL<64>, grid, dim3(kBlock), 0, __VA_ARGS__);                 \
     } while (0)
 
+// ── caustic photon pass (host orchestration) ───────────────────────────────
+struct PhotonPass {
+    DevPhotons M{};
+    int64_t emitted = 0;
+    int n = 0, nb = 0;
+    const float4* slot_pos = nullptr;
+    const uint32_t* key0 = nullptr;   // bucket of each photon, photon order
+};
+
+// Exclusive scan of n ints (out may alias nothing); bsum needs
+// ceil(n / kScanTile) + 1 ints; the total lands in bsum[tiles].
+static void dev_scan(hipStream_t st, const int32_t* in, int32_t* out, int n, int32_t* bsum) {
+    int tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles == 0) return;
+    hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(kScanBlock), 0, st, in, out, n, bsum);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, st, bsum, tiles);
+    hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(kScanBlock), 0, st, out, n, bsum);
+}
+
+// CausticPhotonTracer.Build + PhotonMap construction on the device
+// (Renderer.cs:385-406).  Per-light budgets are set on the host exactly as
+// the reference does (luminous power shares, banker's rounding, >= 1 each).
+static int photon_pass(rt_ctx* ctx, const rt_scene* sc, const DevScene& S, int budget, float cell_size,
+                       PhotonPass& out) {
+    const HostScene& h = sc->hs;
+    std::vector<PhotonSrc> src;
+    std::vector<float> lum;
+    float total = 0.0f;
+    for (size_t i = 0; i < h.lights.size(); i++) {
+        float p = std::max(0.0f, h.light_power[i]);
+        if (p <= 0.0f) continue;
+        src.push_back(PhotonSrc{(int)i, 0, 0, 0});
+        lum.push_back(p);
+        total += p;
+    }
+    out = PhotonPass{};
+    if (budget <= 0 || src.empty() || total <= 0.0f) return RT_OK;
+    int64_t ntot64 = 0;
+    for (size_t i = 0; i < src.size(); i++) {
+        int n = (int)nearbyintf((float)budget * (lum[i] / total));
+        src[i].count = std::max(1, n);
+        src[i].offset = (int)ntot64;
+        ntot64 += src[i].count;
+    }
+    if (ntot64 > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "photon budget too large");
+    const int ntot = (int)ntot64;
+    out.emitted = ntot;
+    int nbmax = 1;
+    while (nbmax < ntot) nbmax <<= 1;
+    const int tiles_max = (std::max(ntot, nbmax) + 1023) / 1024;
+    const size_t hist_n = (size_t)256 * ((ntot + 1023) / 1024);
+    size_t need = 0;
+    auto lay = [&](size_t bytes) { size_t o = need; need += (bytes + 255) & ~(size_t)255; return o; };
+    size_t o_src = lay(sizeof(PhotonSrc) * src.size());
+    size_t o_sp = lay(16ull * ntot), o_si = lay(16ull * ntot), o_sw = lay(16ull * ntot);
+    size_t o_flag = lay(4ull * ntot), o_posof = lay(4ull * ntot);
+    size_t o_cp = lay(16ull * ntot), o_ci = lay(16ull * ntot), o_cw = lay(16ull * ntot);
+    size_t o_k0 = lay(4ull * ntot), o_ka = lay(4ull * ntot), o_kb = lay(4ull * ntot);
+    size_t o_v0 = lay(4ull * ntot), o_va = lay(4ull * ntot), o_vb = lay(4ull * ntot);
+    size_t o_cnt = lay(4ull * nbmax), o_start = lay(4ull * (nbmax + 1));
+    size_t o_hist = lay(4 * hist_n), o_hscan = lay(4 * hist_n);
+    size_t o_bsum = lay(4ull * ((std::max(hist_n, (size_t)std::max(ntot, nbmax)) + kScanTile - 1) / kScanTile + 2));
+    size_t o_part = lay(16 * 256), o_org = lay(16), o_ovf = lay(16);
+    (void)tiles_max;
+    if (need > ctx->ph_bytes) {
+        if (ctx->ph_mem) hipFree(ctx->ph_mem);
+        ctx->ph_mem = nullptr;
+        ctx->ph_bytes = 0;
+        HIP_TRY(hipMalloc(&ctx->ph_mem, need));
+        ctx->ph_bytes = need;
+    }
+    char* m = (char*)ctx->ph_mem;
+    auto at = [&](size_t o) { return (void*)(m + o); };
+    PhotonSrc* d_src = (PhotonSrc*)at(o_src);
+    float4 *sp = (float4*)at(o_sp), *si = (float4*)at(o_si), *sw = (float4*)at(o_sw);
+    int32_t *flag = (int32_t*)at(o_flag), *posof = (int32_t*)at(o_posof);
+    float4 *cp = (float4*)at(o_cp), *ci = (float4*)at(o_ci), *cw = (float4*)at(o_cw);
+    uint32_t *k0 = (uint32_t*)at(o_k0), *ka = (uint32_t*)at(o_ka), *kb = (uint32_t*)at(o_kb);
+    int32_t *v0 = (int32_t*)at(o_v0), *va = (int32_t*)at(o_va), *vb = (int32_t*)at(o_vb);
+    int32_t *cnt = (int32_t*)at(o_cnt), *start = (int32_t*)at(o_start);
+    int32_t *hist = (int32_t*)at(o_hist), *hscan = (int32_t*)at(o_hscan), *bsum = (int32_t*)at(o_bsum);
+    float4 *part = (float4*)at(o_part), *org = (float4*)at(o_org);
+    int32_t* ovf = (int32_t*)at(o_ovf);
+    hipStream_t st = ctx->stream;
+
+    HIP_TRY(hipMemcpyAsync(d_src, src.data(), sizeof(PhotonSrc) * src.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(ovf, 0, 4, st));
+    dim3 g((ntot + kBlock - 1) / kBlock);
+    DISPATCH_SD(sc->stack_depth, k_photon_trace, g, st, S, d_src, (int)src.size(), ntot, h.bounds.mn, h.bounds.mx, sp,
+                si, sw, flag, (uint32_t*)ovf);
+    HIP_TRY(hipGetLastError());
+    dev_scan(st, flag, posof, ntot, bsum);
+    int32_t hv[2] = {0, 0};
+    const int tiles = (ntot + kScanTile - 1) / kScanTile;
+    HIP_TRY(hipMemcpyAsync(&hv[0], bsum + tiles, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&hv[1], ovf, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (hv[1]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow in the photon pass");
+    const int n = hv[0];
+    out.slot_pos = sp;
+    if (n == 0) return RT_OK;
+    hipLaunchKernelGGL(k_photon_compact, g, dim3(kBlock), 0, st, ntot, flag, posof, sp, si, sw, cp, ci, cw);
+    // grid: origin = min position, cell, power-of-two bucket table
+    int nb = 1;
+    while (nb < n) nb <<= 1;
+    const float cell = std::max(cell_size, 1e-6f);
+    GridParams G{cell, 1.0f / cell, (uint32_t)(nb - 1), n};
+    int pb = std::min(256, (n + 255) / 256);
+    hipLaunchKernelGGL(k_min3, dim3(pb), dim3(256), 0, st, cp, n, part);
+    hipLaunchKernelGGL(k_min3, dim3(1), dim3(256), 0, st, part, pb, org);
+    HIP_TRY(hipMemsetAsync(cnt, 0, 4ull * nb, st));
+    hipLaunchKernelGGL(k_photon_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cp, org, G, k0, v0, cnt);
+    // stable LSD radix sort of (bucket, photon index)
+    int bits = 0;
+    while ((1 << bits) < nb) bits++;
+    const uint32_t* kin = k0;
+    const int32_t* vin = v0;
+    const int rtiles = (n + 1023) / 1024;
+    for (int shift = 0, pass = 0; shift < bits; shift += 8, pass++) {
+        uint32_t* kout = (pass & 1) ? kb : ka;
+        int32_t* vout = (pass & 1) ? vb : va;
+        hipLaunchKernelGGL(k_rs_hist, dim3(rtiles), dim3(256), 0, st, kin, n, shift, hist);
+        dev_scan(st, hist, hscan, 256 * rtiles, bsum);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(rtiles), dim3(256), 0, st, kin, vin, kout, vout, n, shift, hscan);
+        kin = kout;
+        vin = vout;
+    }
+    // CSR bucket starts
+    dev_scan(st, cnt, start, nb, bsum);
+    HIP_TRY(hipMemcpyAsync(start + nb, &hv[0], 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    out.n = n;
+    out.nb = nb;
+    out.key0 = k0;
+    DevPhotons& M = out.M;
+    M.pos = cp;
+    M.inc = ci;
+    M.pw = cw;
+    M.start = start;
+    M.idx = vin;
+    M.origin = org;
+    M.cell = G.cell;
+    M.inv_cell = G.inv_cell;
+    M.mask = G.mask;
+    M.n = n;
+    return RT_OK;
+}
+
 static int check_settings(rt_ctx* ctx, const rt_settings* s) {
     if (!s) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "settings is null");
     if (s->width <= 0 || s->height <= 0 || s->samples_per_pixel <= 0 || s->max_depth < 0)
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "width, height and samples_per_pixel must be positive, max_depth >= 0");
     if ((int64_t)s->width * s->height > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
     if (s->sampler != RT_SAMPLER_SOBOL) return fail(ctx, RT_ERR_UNSUPPORTED, "only the Sobol sampler is implemented");
-    if (s->caustics) return fail(ctx, RT_ERR_UNSUPPORTED, "photon caustics are not implemented");
+    if (s->caustics && (s->caustic_photons < 0 || s->caustic_gather_radius < 0.0f))
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "negative photon budget or gather radius");
     if (s->light_sampling < RT_LIGHTS_ALL || s->light_sampling > RT_LIGHTS_UNIFORM)
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bad light_sampling");
     return RT_OK;
@@ -1515,6 +1734,7 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     if (ctx->path_mem) hipFree(ctx->path_mem);
     if (ctx->film_mem) hipFree(ctx->film_mem);
+    if (ctx->ph_mem) hipFree(ctx->ph_mem);
     for (auto& e : ctx->ev)
         if (e) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -1669,6 +1889,52 @@ int rt_trace_rays(rt_ctx* ctx, const rt_scene* sc, const float* rays, int64_t nu
     return RT_OK;
 }
 
+int rt_photon_map(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, int64_t cap_photons, int64_t cap_buckets,
+                  int64_t* n_photons, int64_t* n_emitted, int64_t* n_buckets, float* pos, float* inc, float* power,
+                  int32_t* key, int32_t* idx, int32_t* start, float* grid) {
+    if (!ctx || !sc || !s) return RT_ERR_INVALID_ARGUMENT;
+    if (s->caustic_photons < 0 || s->caustic_gather_radius < 0.0f)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "negative photon budget or gather radius");
+    hipSetDevice(ctx->device);
+    const HostScene& h = sc->hs;
+    float gr = s->caustic_gather_radius > 0.0f ? s->caustic_gather_radius : std::max(h.scene_radius * 0.06f, 1e-4f);
+    DevScene S = dev_view(sc);
+    PhotonPass PP;
+    int rc = photon_pass(ctx, sc, S, s->caustic_photons, gr * 0.5f, PP);
+    if (rc) return rc;
+    if (n_photons) *n_photons = PP.n;
+    if (n_emitted) *n_emitted = PP.emitted;
+    if (n_buckets) *n_buckets = PP.n > 0 ? PP.nb : 0;
+    if (PP.n == 0) return RT_OK;
+    const int n = PP.n;
+    if (cap_photons >= n) {
+        std::vector<float4> tmp(n);
+        const float4* src3[3] = {PP.M.pos, PP.M.inc, PP.M.pw};
+        float* dst3[3] = {pos, inc, power};
+        for (int a = 0; a < 3; a++) {
+            if (!dst3[a]) continue;
+            HIP_TRY(hipMemcpy(tmp.data(), src3[a], 16ull * n, hipMemcpyDeviceToHost));
+            for (int i = 0; i < n; i++) {
+                dst3[a][3 * i] = tmp[i].x;
+                dst3[a][3 * i + 1] = tmp[i].y;
+                dst3[a][3 * i + 2] = tmp[i].z;
+            }
+        }
+        if (key) HIP_TRY(hipMemcpy(key, PP.key0, 4ull * n, hipMemcpyDeviceToHost));
+        if (idx) HIP_TRY(hipMemcpy(idx, PP.M.idx, 4ull * n, hipMemcpyDeviceToHost));
+    }
+    if (start && cap_buckets >= PP.nb + 1) HIP_TRY(hipMemcpy(start, PP.M.start, 4ull * (PP.nb + 1), hipMemcpyDeviceToHost));
+    if (grid) {
+        float4 o;
+        HIP_TRY(hipMemcpy(&o, PP.M.origin, 16, hipMemcpyDeviceToHost));
+        grid[0] = o.x;
+        grid[1] = o.y;
+        grid[2] = o.z;
+        grid[3] = PP.M.cell;
+    }
+    return RT_OK;
+}
+
 int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_film* film, rt_render_stats* stats) {
     auto t_start = std::chrono::steady_clock::now();
     if (!ctx || !sc) return RT_ERR_INVALID_ARGUMENT;
@@ -1691,9 +1957,11 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     const int maxs = std::max(1, s->light_sampling != RT_LIGHTS_ALL ? h.max_light_samples : h.max_shadow_per_vertex);
     if ((int64_t)maxs * N > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many shadow slots");
     const bool aovs = s->capture_aovs != 0;
+    const bool want_caustics = s->caustics && s->caustic_photons > 0 && !h.lights.empty();
 
     // ── path buffers (grow-only cache in the context) ──────────────────────
-    size_t per_path = 8 * 16 + 8 * 4 + 4 + 16 + (aovs ? 5 * 16 : 0) + 4 * 16 * (size_t)maxs + 5 * 4 + 4 * (size_t)maxs;
+    size_t per_path = 8 * 16 + 8 * 4 + 4 + 16 + (aovs ? 5 * 16 : 0) + 4 * 16 * (size_t)maxs + 5 * 4 + 4 * (size_t)maxs +
+                      (want_caustics ? 64 : 0);
     size_t per_sample = aovs ? 48 : 16;
     size_t need = (size_t)N * per_path + (size_t)WS * per_sample + 40 * 256 + 16384;
     if (need > ctx->path_bytes) {
@@ -1741,6 +2009,12 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     P.q_r = (int32_t*)take(4 * N);
     P.sid = (int32_t*)take(4 * N);
     P.res_l = (float4*)take(16 * WS);
+    if (want_caustics) {
+        P.gq0 = (float4*)take(16 * N);
+        P.gq1 = (float4*)take(16 * N);
+        P.gq2 = (float4*)take(16 * N);
+        P.gq3 = (float4*)take(16 * N);
+    }
     if (aovs) {
         P.res_a = (float4*)take(16 * WS);
         P.res_n = (float4*)take(16 * WS);
@@ -1829,6 +2103,19 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     R.aovs = aovs ? 1 : 0;
     R.maxs = maxs;
 
+    // ── caustic photon pre-pass (Renderer.cs:385-406) ──────────────────────
+    PhotonPass PP;
+    if (want_caustics) {
+        float gr = s->caustic_gather_radius > 0.0f ? s->caustic_gather_radius : std::max(h.scene_radius * 0.06f, 1e-4f);
+        rc = photon_pass(ctx, sc, S, s->caustic_photons, gr * 0.5f, PP);
+        if (rc) return rc;
+        PP.M.max_r = gr;
+        if (PP.n > 0) {
+            S.caustics = 1;
+            S.block_spec = 1;
+        }
+    }
+
     double t_trace = 0, t_shade = 0, t_shadow = 0;
     int64_t n_closest = 0, n_shadow_rays = 0, launches = 0, shadow_launches = 0;
     const bool counting = s->count_traversal != 0;
@@ -1875,6 +2162,9 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
                 HIP_TRY(hipEventRecord(ev[2], ctx->stream));
                 HIP_TRY(hipMemcpyAsync(cnt, P.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
                 HIP_TRY(hipStreamSynchronize(ctx->stream));
+                if (S.caustics && cnt[kCntGather] > 0)
+                    hipLaunchKernelGGL(k_gather, dim3((cnt[kCntGather] + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                                       ctx->stream, S, P, PP.M, cnt[kCntGather]);
                 read_shadow_time();
                 float ms = 0;
                 hipEventElapsedTime(&ms, ev[0], ev[1]);

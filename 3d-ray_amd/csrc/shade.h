@@ -528,6 +528,7 @@ struct BsdfS {
     f3 wo, f;
     float pdf;
     bool delta;
+    bool caster;      // BsdfSample.CausticCaster (followed by the photon walk)
     bool has_next_abs;
     f3 next_abs;
     int transition;   // 0 none, 1 enter, 2 exit
@@ -543,6 +544,7 @@ __device__ bool disney_sample(const DisneyP& P, const float* E, f3 V, const Surf
     o.has_next_abs = false;
     o.next_abs = splat(0.0f);
     o.transition = 0;
+    o.caster = true;
     if (lobe == kLDiffTrans) {
         f3 fb = disney_eval(P, E, V, wo, s);
         float pb = disney_pdf(P, V, wo, s);
@@ -567,12 +569,33 @@ __device__ bool disney_sample(const DisneyP& P, const float* E, f3 V, const Surf
         o.f = att;
         o.pdf = 1.0f;
         o.delta = true;
+        // only a mirror-like (metallic >= 0.5) delta reflection casts caustics
+        o.caster = P.metallic >= 0.5f;
         return true;
     }
     o.f = disney_eval(P, E, V, wo, s);
     o.pdf = disney_pdf(P, V, wo, s);
     o.delta = false;
     return true;
+}
+
+// Dielectric.Scatter (Dielectric.cs:75-105): Schlick-weighted reflect /
+// refract with the relative IOR (IorStack when set, else the material's).
+__device__ __forceinline__ void dielectric_scatter(const GMat& g, const Surf& s, f3 d, Rng& rng, f3& att, f3& wo,
+                                                   f3& origin) {
+    float ior = g.m3.w;
+    float ri = s.rel_eta > 0.0f ? s.rel_eta : (s.front ? (1.0f / ior) : ior);
+    f3 ud = normalize(d);
+    float ct = fminf(dot(-ud, s.n), 1.0f);
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    bool cannot = ri * st > 1.0f;
+    f3 dir;
+    if (cannot || schlick(ct, ri) > rng.next()) dir = reflect(ud, s.n);
+    else dir = refract(ud, s.n, ri);
+    f3 on = dot(dir, s.n) >= 0.0f ? s.n : -s.n;
+    origin = offset_origin(s.p, on);
+    wo = dir;
+    att = f3of(g.m1);
 }
 
 // ShadowTransmittance / ShadowAbsorption of any material kind

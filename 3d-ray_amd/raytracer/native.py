@@ -61,6 +61,9 @@ def load():
     L.rt_render.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.POINTER(abi.rt_film), C.POINTER(abi.rt_render_stats)]
     L.rt_trace_primary.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.c_int32, C.c_int32, ip, fp, fp, fp]
     L.rt_trace_rays.argtypes = [vp, vp, fp, C.c_int64, ip, fp, fp, fp]
+    i64p = C.POINTER(C.c_int64)
+    L.rt_photon_map.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.c_int64, C.c_int64, i64p, i64p, i64p, fp, fp, fp,
+                                ip, ip, ip, fp]
     L.rt_debug_bvh_dump.argtypes = [C.POINTER(abi.rt_scene_desc), ip, C.c_int64]
     L.rt_debug_bvh_dump.restype = C.c_int64
     _lib = L
@@ -69,7 +72,7 @@ def load():
 
 EXPORTED_SYMBOLS = ("rt_get_version", "rt_material_defaults", "rt_settings_defaults", "rt_create", "rt_destroy",
                     "rt_last_error", "rt_scene_create", "rt_scene_destroy", "rt_scene_get_info", "rt_render",
-                    "rt_trace_primary", "rt_trace_rays", "rt_debug_bvh_dump")
+                    "rt_trace_primary", "rt_trace_rays", "rt_photon_map", "rt_debug_bvh_dump")
 
 
 def _f(a):
@@ -147,6 +150,25 @@ class Scene:
         self.ctx.check(load().rt_trace_rays(self.ctx.h, self.h, _f(rays), n, _i(prim), _f(t), _f(u), _f(v)))
         return prim, t, u, v
 
+    def photon_map(self, settings):
+        """Device photon map of the caustic pre-pass (rt_photon_map): dict of
+        pos/inc/power (n,3), key/idx (n,), start (n_buckets+1,), grid (4,)."""
+        L = load()
+        n, em, nb = (C.c_int64(0) for _ in range(3))
+        self.ctx.check(L.rt_photon_map(self.ctx.h, self.h, C.byref(settings), 0, 0, C.byref(n), C.byref(em), C.byref(nb),
+                                       None, None, None, None, None, None, None))
+        n, em, nb = n.value, em.value, nb.value
+        out = {"n": n, "emitted": em, "n_buckets": nb}
+        pos, inc, pw = (np.zeros((n, 3), np.float32) for _ in range(3))
+        key, idx = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        start = np.zeros(nb + 1, np.int32)
+        grid = np.zeros(4, np.float32)
+        a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        self.ctx.check(L.rt_photon_map(self.ctx.h, self.h, C.byref(settings), n, nb + 1, C.byref(a), C.byref(b),
+                                       C.byref(c), _f(pos), _f(inc), _f(pw), _i(key), _i(idx), _i(start), _f(grid)))
+        out.update(pos=pos, inc=inc, power=pw, key=key, idx=idx, start=start, grid=grid)
+        return out
+
     def close(self):
         if getattr(self, "h", None):
             load().rt_scene_destroy(self.h)
@@ -181,7 +203,8 @@ class Renderer:
 
     def __init__(self, scene, samples_per_pixel, max_depth, max_sample_radiance=10.0, mis_heuristic="balance",
                  light_sampling_strategy="all", indirect_clamp_factor=0.25, exposure_ev=0.0, rr_min_bounces=-1,
-                 rr_min_survival=-1.0, seed_xor=0, paths_in_flight=0):
+                 rr_min_survival=-1.0, seed_xor=0, paths_in_flight=0, enable_caustics=False, caustic_photons=0,
+                 caustic_gather_radius=0.0):
         self.scene = scene
         s = abi.rt_settings()
         load().rt_settings_defaults(C.byref(s))
@@ -196,6 +219,9 @@ class Renderer:
         s.rr_min_survival = rr_min_survival
         s.seed_xor = seed_xor
         s.paths_in_flight = paths_in_flight
+        s.caustics = 1 if enable_caustics else 0
+        s.caustic_photons = caustic_photons
+        s.caustic_gather_radius = caustic_gather_radius
         self.settings = s
 
     def render(self, width, height, capture_aovs=False):

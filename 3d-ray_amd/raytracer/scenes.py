@@ -388,3 +388,57 @@ def displaced_icosphere_scene(width=1920, height=1080, subdiv=8, seed=0xC0FFEE, 
         b.directional_light((-0.4, -1.0, -0.3), (1.0, 1.0, 1.0), 3.0)
     cam = make_camera((0.0, 1.2, 3.5), (0.0, 0.9, 0.0), 40.0, width / height)
     return b.build(cam)
+
+
+def gerstner_water(n_cells, half_extent=2.0, height=0.45, waves=8, seed=0x5EA):
+    """Water heightfield: an n_cells x n_cells grid over [-half, half]^2
+    (2 triangles per cell) displaced by the sum of `waves` seeded Gerstner
+    waves (horizontal + vertical displacement).  Returns (vertices, faces)."""
+    rng = np.random.default_rng(seed)
+    g = np.linspace(-half_extent, half_extent, n_cells + 1)
+    X, Z = np.meshgrid(g, g, indexing="xy")
+    x, z = X.ravel().astype(np.float64), Z.ravel().astype(np.float64)
+    px, py, pz = x.copy(), np.full_like(x, height), z.copy()
+    lam = rng.uniform(0.4, 1.6, waves)
+    amp = rng.uniform(0.006, 0.02, waves)
+    ang = rng.uniform(0.0, 2.0 * np.pi, waves)
+    phase = rng.uniform(0.0, 2.0 * np.pi, waves)
+    for k in range(waves):
+        kk = 2.0 * np.pi / lam[k]
+        dx, dz = np.cos(ang[k]), np.sin(ang[k])
+        q = 0.6 / (kk * amp[k] * waves)          # steepness kept below folding
+        th = kk * (dx * x + dz * z) + phase[k]
+        px += q * amp[k] * dx * np.cos(th)
+        pz += q * amp[k] * dz * np.cos(th)
+        py += amp[k] * np.sin(th)
+    v = np.stack([px, py, pz], 1).astype(np.float32)
+    n1 = n_cells + 1
+    i, j = np.meshgrid(np.arange(n_cells), np.arange(n_cells), indexing="xy")
+    a = (j * n1 + i).ravel()
+    b, c, d = a + 1, a + n1, a + n1 + 1
+    f = np.concatenate([np.stack([a, c, b], 1), np.stack([b, c, d], 1)]).astype(np.int64)
+    return v, f
+
+
+def caustics_scene(width=1920, height=1080, water_cells=200, seed=0x5EA):
+    """Caustics + volumes workload: three Disney glass spheres (ior 1.5,
+    Beer-Lambert absorption 0.5 per unit: transmission colour e^-0.5 at
+    depth 1), an 80k-triangle Gerstner-wave water sheet (Disney glass, ior
+    1.33) over a diffuse floor, a sun plus one sphere light, and a global
+    homogeneous fog with sigma_t = 0.05 and albedo 0.9."""
+    b = SceneBuilder()
+    floor = b.lambertian((0.75, 0.72, 0.68))
+    t = float(np.exp(-0.5))
+    glass = b.disney((1.0, 1.0, 1.0), spec_trans=1.0, ior=1.5, roughness=0.0, transmission_color=(t, t, t),
+                     transmission_depth=1.0)
+    water = b.disney((1.0, 1.0, 1.0), spec_trans=1.0, ior=1.33, roughness=0.0)
+    b.quad((-6.0, 0.0, -6.0), (0.0, 0.0, 12.0), (12.0, 0.0, 0.0), floor)
+    v, f = gerstner_water(water_cells, seed=seed)
+    b.mesh(v, f, water)
+    for cx, cz, r in ((-0.9, -0.3, 0.35), (0.1, 0.4, 0.3), (1.0, -0.5, 0.4)):
+        b.sphere((cx, 0.45 + r + 0.35, cz), r, glass)
+    b.directional_light((-0.35, -1.0, -0.25), intensity=3.0)
+    b.sphere_light((1.6, 3.0, 1.2), 0.15, intensity=40.0, shadow_samples=1)
+    cam = make_camera((0.0, 2.4, 5.2), (0.0, 0.6, 0.0), 40.0, width / height)
+    sig_t, albedo = 0.05, 0.9
+    return b.build(cam, medium=((sig_t * (1 - albedo),) * 3, (sig_t * albedo,) * 3))

@@ -40,13 +40,15 @@ def traversal_bytes(node, tri, sph, rays):
     return 64.0 * node + 64.0 * tri + 16.0 * sph + 48.0 * rays
 
 
-def cpu_baseline(desc, width, height, spp, depth, rows_target_s):
+def cpu_baseline(desc, width, height, spp, depth, rows_target_s, caustics=None):
     """Times the CPU restatement of the reference (oracle/) on every k-th
     scanline of the same frame at full spp, on the host's cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
     st = scenes_settings(width, height, spp, depth)
+    if caustics:
+        st.caustics, st.caustic_photons, st.caustic_gather_radius = 1, caustics[0], caustics[1]
     osc = O.OracleScene(desc)
     # probe on a sparse subset to pick a stride giving a bounded sample
     probe_step = max(1, height // 4)
@@ -82,6 +84,11 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--subdiv", type=int, default=8)
+    ap.add_argument("--workload", choices=("icosphere", "caustics"), default="icosphere",
+                    help="icosphere: the 1.31M-triangle Disney scene (headline); caustics: glass spheres, "
+                         "Gerstner water, fog, photon-map caustics")
+    ap.add_argument("--photons", type=int, default=20_000_000)
+    ap.add_argument("--gather-radius", type=float, default=0.01)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the instrumented traversal-count render")
@@ -93,16 +100,28 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    caus = args.workload == "caustics"
+    if caus:
+        # caustics + volumes: 1920x1080, 128 spp, depth 12, 20M photons
+        if args.spp == 64:
+            args.spp = 128
+        if args.depth == 8:
+            args.depth = 12
     W, H, spp, depth = args.width, args.height, args.spp, args.depth
-    log(f"generating scene (subdiv {args.subdiv})")
-    desc = scenes.displaced_icosphere_scene(W, H, subdiv=args.subdiv)
+    if caus:
+        log("generating scene (glass spheres, 80k-triangle Gerstner water, fog)")
+        desc = scenes.caustics_scene(W, H)
+    else:
+        log(f"generating scene (subdiv {args.subdiv})")
+        desc = scenes.displaced_icosphere_scene(W, H, subdiv=args.subdiv)
     ntri = int((desc.prims["kind"] == 0).sum())
     ctx = native.Context(0)
     t0 = time.perf_counter()
     sc = native.Scene(ctx, desc)
     log(f"scene resident: {ntri} triangles, build+upload {time.perf_counter() - t0:.2f} s, "
         f"{sc.info().device_bytes / 1e6:.0f} MB")
-    r = native.Renderer(sc, spp, depth)
+    r = native.Renderer(sc, spp, depth, enable_caustics=caus, caustic_photons=args.photons if caus else 0,
+                        caustic_gather_radius=args.gather_radius if caus else 0.0)
     s = r.settings
     s.width, s.height = W, H
     import ctypes as C
@@ -167,12 +186,13 @@ def main():
     if not args.no_cpu_baseline:
         log("CPU baseline (C++ restatement of the reference, row subset)")
         try:
-            cpu = cpu_baseline(desc, W, H, spp, depth, args.cpu_seconds)
+            cpu = cpu_baseline(desc, W, H, spp, depth, args.cpu_seconds,
+                               caustics=(args.photons, args.gather_radius) if caus else None)
         except Exception as e:  # the baseline must not hide the GPU result
             log(f"cpu baseline failed: {e}")
 
     out = {
-        "metric": "Msamples/s @1M-tri 1080p vs CPU",
+        "metric": "Msamples/s caustics+volumes 1080p vs CPU" if caus else "Msamples/s @1M-tri 1080p vs CPU",
         "value": round(value, 2),
         "unit": "Msamples/s",
         "n_gpus": 1,
@@ -184,8 +204,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": f"displaced icosphere {ntri} tris + 2-tri ground, per-triangle Disney, "
-                               f"4 sphere lights + sun, {W}x{H}, {spp} spp, depth {depth}, NEE+MIS balance, Sobol",
+        "config": {"workload": (f"caustics + volumes: 3 glass spheres, {ntri}-tri Gerstner water, fog, sun + sphere "
+                                f"light, {args.photons} photons, gather radius {args.gather_radius}, {W}x{H}, {spp} spp, "
+                                f"depth {depth}") if caus else
+                               (f"displaced icosphere {ntri} tris + 2-tri ground, per-triangle Disney, "
+                                f"4 sphere lights + sun, {W}x{H}, {spp} spp, depth {depth}, NEE+MIS balance, Sobol"),
                    "width": W, "height": H, "spp": spp, "max_depth": depth, "triangles": ntri},
         "mrays_per_s": round(args.steps * rays / elapsed / 1e6, 1),
         "kernel_ms_per_step": {k: round(v / args.steps, 2) for k, v in per.items()},

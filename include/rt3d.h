@@ -213,9 +213,9 @@ typedef struct rt_settings {
     uint32_t seed_xor;              /* 0 = the reference's Sobol streams; any other
                                        value XORs every pixel seed (independent
                                        streams for statistical tests) */
-    int32_t caustics;               /* reserved: photon caustics (not yet built) */
-    int32_t caustic_photons;
-    float caustic_gather_radius;
+    int32_t caustics;               /* --caustics: photon-map caustics (Renderer.cs:380-406) */
+    int32_t caustic_photons;        /* --caustic-photons: emitted photon budget */
+    float caustic_gather_radius;    /* gather-radius cap; 0 = 6 % of the scene radius */
     int32_t paths_in_flight;        /* wavefront size; 0 = library default */
     int32_t count_traversal;        /* 1: count BVH node/primitive tests (slower
                                        instrumented kernels; fills rt_render_stats) */
@@ -327,6 +327,21 @@ int rt_trace_primary(rt_ctx* ctx, const rt_scene* scene, const rt_settings* sett
  * Replaces: IHittable.Hit on the assembled world (HittableList.cs:19-38). */
 int rt_trace_rays(rt_ctx* ctx, const rt_scene* scene, const float* rays, int64_t num_rays,
                   int32_t* prim_id, float* t, float* u, float* v);
+
+/* Parity hook: runs the caustic photon pre-pass rt_render runs for
+ * `settings` (caustic_photons; caustic_gather_radius, or 6 % of the scene
+ * radius when 0, Renderer.cs:394; grid cell = half of it) and copies the
+ * device photon map out.  Counts are always written; the arrays (NULL =
+ * skip) are filled when cap_photons >= n_photons (pos/inc/power 3 floats
+ * and key/idx 1 int per photon) and cap_buckets >= n_buckets + 1 (start) —
+ * call once with zero caps to size them.  key = bucket of each photon in
+ * photon order, idx/start = the CSR grid, grid = origin xyz and cell size.
+ * Photons are in emission order (global photon index).  No reference
+ * counterpart (PhotonMap keeps these private, PhotonMap.cs:27-35); it lets
+ * a test check the device grid bit-exactly against a host sort. */
+int rt_photon_map(rt_ctx* ctx, const rt_scene* scene, const rt_settings* settings, int64_t cap_photons,
+                  int64_t cap_buckets, int64_t* n_photons, int64_t* n_emitted, int64_t* n_buckets, float* pos,
+                  float* inc, float* power, int32_t* key, int32_t* idx, int32_t* start, float* grid);
 
 /* Diagnostics (host only, no device needed): builds the scene on the host and
  * serialises the flattened world BVH in pre-order — internal node: (0, axis,
