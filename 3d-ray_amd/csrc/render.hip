@@ -72,7 +72,8 @@ struct RParams {
 // counters[]: next-queue size, shadow-queue size (ungrouped form), one size
 // per light-sample segment of the grouped shadow queue, stack-overflow flag
 constexpr int kCntNext = 0, kCntShadow = 1, kCntRetire = 2, kCntGather = 3, kCntSeg = 4, kMaxSeg = 32,
-              kCntOverflow = 63;
+              kCntRank = 36, kCntClear = 40, kCntOverflow = 63;
+enum : int32_t { kSlotIdle = 0, kSlotActive = 1, kSlotRetired = 2 };
 
 // Shadow queue view: segment j holds the shadow rays of light sample j
 // (in light-list order) at q_sh[j*stride ...]; pre[] is the prefix of the
@@ -88,8 +89,10 @@ struct PathBufs {
     int32_t* hit_e;
     float4* hit_tuv;
     float4 *av0, *av1, *av2, *av3, *av4;   // AOV probe
-    float4 *sh_o, *sh_d, *sh_c, *sh_r;     // shadow slot records (N × maxs)
-    int32_t *q_a, *q_b, *q_sh, *q_r;       // trace queues, shadow queue, retired paths
+    float4 *sh_o, *sh_d, *sh_c, *sh_r;     // shadow records, record k of slot p at k*N + p (N × maxs)
+    int32_t* state;             // per slot: kSlotIdle, kSlotActive (to trace), kSlotRetired
+    int32_t* smask;             // per slot: light samples with a pending shadow ray (grouped form)
+    int32_t* q_sh;              // shadow queue: shadow record indices
     int32_t* sid;               // sample of the window each path slot is tracing
     float4 *res_l, *res_a, *res_n;         // per-sample results of the window: (L, depth), albedo, normal
     float4 *gq0, *gq1, *gq2, *gq3;         // caustic gather requests: (o,t) (d,entry) (u,v,eta,path) (weight,target)
@@ -579,29 +582,29 @@ __device__ __forceinline__ void init_path(const DevScene& S, const RParams& R, P
     P.sid[p] = sid;
 }
 
-// Initial fill of the path pool: slot p traces window sample p.
+// Initial fill of the path pool: slot p < n traces window sample p; the
+// rest of the pool stays idle.
 __global__ void __launch_bounds__(kBlock) k_raygen(DevScene S, RParams R, PathBufs P, int n) {
     int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n) return;
-    init_path(S, R, P, p, p);
-    P.q_a[p] = p;
+    if (p >= P.n) return;
+    if (p < n) init_path(S, R, P, p, p);
+    P.state[p] = p < n ? (R.max_depth > 0 ? kSlotActive : kSlotRetired) : kSlotIdle;
 }
 
 // Traversal stack placement: LDS (lane-interleaved) or the wave's private
 // scratch (PRIV); chosen per context.
 template <int SD, bool COUNT, bool PRIV>
-__global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P, const int32_t* __restrict__ q, int count) {
+__global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P) {
     __shared__ uint2 lds_stk[PRIV ? 1 : SD * kBlock];
     uint2 priv_stk[PRIV ? SD : 1];
     uint2* stk = PRIV ? priv_stk : &lds_stk[threadIdx.x];
     constexpr int STRIDE = PRIV ? 1 : kBlock;
     TravCount tc{0u, 0u, 0u};
-    int i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= count) {
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    if (p >= P.n || P.state[p] != kSlotActive) {
         if (COUNT) flush_counts(P.tcount, tc);
         return;
     }
-    int p = q[i];
     float4 o4 = P.o[p], d4 = P.d[p];
     f3 o = f3of(o4), d = f3of(d4);
     bool camera = pf_cat(__float_as_int(d4.w)) == 0;
@@ -635,17 +638,21 @@ __device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, i
     int pend = pf_pend(flags);
     if (pend == 0) return flags;
     f3 s = splat(0.0f);
-    for (int k = 0; k < pend; k++) s += f3of(P.sh_r[(int64_t)p * R.maxs + k]);
+    if (R.maxs <= kMaxSeg) {
+        // record of light sample j at j*N + p, summed in light-sample order
+        for (uint32_t m = (uint32_t)P.smask[p]; m != 0u; m &= m - 1u)
+            s += f3of(P.sh_r[(int64_t)(__ffs(m) - 1) * P.n + p]);
+    } else {
+        for (int k = 0; k < pend; k++) s += f3of(P.sh_r[(int64_t)k * P.n + p]);
+    }
     add_target(P, p, (flags & kPfPendInd) != 0, s);
     return flags & ~(0xff << kPfPendShift) & ~kPfPendInd;
 }
 
 template <int WPE>
-__global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, PathBufs P,
-                                                       const int32_t* __restrict__ q, int count) {
-    int i = xcd_block() * kBlock + threadIdx.x;
-    bool active = i < count;
-    int p = active ? q[i] : 0;
+__global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, PathBufs P) {
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    const bool active = p < P.n && P.state[p] == kSlotActive;
     int n_shadow = 0;
     uint32_t smask = 0;   // bit j: light sample j produced a shadow ray
     bool cont = false;
@@ -732,7 +739,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
             aovf &= ~kAovPending;
         };
 
-        const int64_t slot0 = (int64_t)p * R.maxs;
+        const bool grouped = R.maxs <= kMaxSeg;
         f3 next_o = o, next_d = d;
         float next_pdf = 0.0f;
         bool next_delta = false;
@@ -780,7 +787,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
                     if (ppick > 0.0f) c = c / ppick;
                     c = seg_w * c;
                     if (is_zero(c)) continue;
-                    int64_t sl = slot0 + n_shadow;
+                    int64_t sl = (int64_t)(grouped ? jn : n_shadow) * P.n + p;
                     P.sh_o[sl] = make_float4(ls.so.x, ls.so.y, ls.so.z, ls.tmax);
                     P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
                     P.sh_c[sl] = make_float4(c.x, c.y, c.z, 0.0f);
@@ -906,7 +913,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
                         if (ppick > 0.0f) c = c / ppick;
                         c = vw * c;
                         if (is_zero(c)) continue;
-                        int64_t sl = slot0 + n_shadow;
+                        int64_t sl = (int64_t)(grouped ? jn : n_shadow) * P.n + p;
                         P.sh_o[sl] = make_float4(ls.so.x, ls.so.y, ls.so.z, ls.tmax);
                         P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
                         P.sh_c[sl] = make_float4(c.x, c.y, c.z, 0.0f);
@@ -1065,6 +1072,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
         // pending shadows of this vertex belong to this vertex's target
         flags = flags & ~(0xff << kPfPendShift) & ~kPfPendInd;
         flags |= (n_shadow << kPfPendShift) | (tgt ? kPfPendInd : 0);
+        if (grouped) P.smask[p] = (int32_t)smask;
         // TraceRay entry checks of the next segment (Renderer.cs:1589-1603)
         if (cont) {
             if (depth - 1 <= 0 || fmax3(thr_next) < 1e-4f) cont = false;
@@ -1086,24 +1094,22 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
             P.av1[p] = av1;
         }
     }
-    // compaction: next queue and shadow queue (wave-aggregated)
-    int nb = wave_alloc(&P.counters[kCntNext], cont ? 1 : 0);
-    if (cont) P.q_b[nb] = p;
+    // slot state and counts; shadow queue (wave-aggregated)
     bool retire = active && !cont;
-    int rb = wave_alloc(&P.counters[kCntRetire], retire ? 1 : 0);
-    if (retire) P.q_r[rb] = p;
+    if (retire) P.state[p] = kSlotRetired;
+    wave_alloc(&P.counters[kCntNext], cont ? 1 : 0);
+    wave_alloc(&P.counters[kCntRetire], retire ? 1 : 0);
     if (R.maxs <= kMaxSeg) {
         // shadow rays grouped by light sample: neighbouring paths' rays
         // towards the same light share a wave (coherent traversal)
         for (int j = 0; j < R.maxs; j++) {
             bool has = (smask >> j) & 1u;
             int pos = wave_alloc(&P.counters[kCntSeg + j], has ? 1 : 0);
-            if (has)
-                P.q_sh[(int64_t)j * P.n + pos] = (int32_t)((int64_t)p * R.maxs + __popc(smask & ((1u << j) - 1u)));
+            if (has) P.q_sh[(int64_t)j * P.n + pos] = (int32_t)((int64_t)j * P.n + p);
         }
     } else {
         int sb = wave_alloc(&P.counters[kCntShadow], n_shadow);
-        for (int k = 0; k < n_shadow; k++) P.q_sh[sb + k] = (int32_t)((int64_t)p * R.maxs + k);
+        for (int k = 0; k < n_shadow; k++) P.q_sh[sb + k] = (int32_t)((int64_t)k * P.n + p);
     }
 }
 
@@ -1253,18 +1259,21 @@ __device__ __forceinline__ void finish_path(const RParams& R, PathBufs& P, int p
     P.res_l[sid] = make_float4(L.x, L.y, L.z, depth);
 }
 
-// Retires the paths that ended in this iteration and restarts the first
-// n_regen of their slots on the next window samples (sid_base + i), appended
-// to the next trace queue.
-__global__ void __launch_bounds__(kBlock) k_retire(DevScene S, RParams R, PathBufs P, const int32_t* __restrict__ q,
-                                                   int n_ret, int n_regen, int sid_base, int32_t* q_out) {
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n_ret) return;
-    int p = q[i];
-    finish_path(R, P, p);
-    if (i < n_regen) {
-        init_path(S, R, P, p, sid_base + i);
-        q_out[i] = p;
+// Retires the paths that ended in this iteration and restarts n_regen of
+// their slots on the next window samples (sid_base + rank); the others go
+// idle.  Which slot traces which sample does not change any result.
+__global__ void __launch_bounds__(kBlock) k_retire(DevScene S, RParams R, PathBufs P, int n_regen, int sid_base) {
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    const bool ret = p < P.n && P.state[p] == kSlotRetired;
+    if (ret) finish_path(R, P, p);
+    int r = wave_alloc(&P.counters[kCntRank], ret ? 1 : 0);
+    if (ret) {
+        if (r < n_regen) {
+            init_path(S, R, P, p, sid_base + r);
+            P.state[p] = R.max_depth > 0 ? kSlotActive : kSlotRetired;
+        } else {
+            P.state[p] = kSlotIdle;
+        }
     }
 }
 
@@ -1956,6 +1965,7 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     if (WS > (1ll << 31) - 1 || N > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
     const int maxs = std::max(1, s->light_sampling != RT_LIGHTS_ALL ? h.max_light_samples : h.max_shadow_per_vertex);
     if ((int64_t)maxs * N > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many shadow slots");
+    if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
     const bool aovs = s->capture_aovs != 0;
     const bool want_caustics = s->caustics && s->caustic_photons > 0 && !h.lights.empty();
 
@@ -2003,10 +2013,9 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     P.sh_d = (float4*)take(16 * N * maxs);
     P.sh_c = (float4*)take(16 * N * maxs);
     P.sh_r = (float4*)take(16 * N * maxs);
-    P.q_a = (int32_t*)take(4 * N);
-    P.q_b = (int32_t*)take(4 * N);
     P.q_sh = (int32_t*)take(4 * N * maxs);
-    P.q_r = (int32_t*)take(4 * N);
+    P.state = (int32_t*)take(4 * N);
+    P.smask = (int32_t*)take(4 * N);
     P.sid = (int32_t*)take(4 * N);
     P.res_l = (float4*)take(16 * WS);
     if (want_caustics) {
@@ -2135,29 +2144,25 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
         const int ws = passes * npix;
         R.base_sample = base;
         const int n0 = (int)std::min<int64_t>(N, ws);
+        const int pool_blocks = (int)((N + kBlock - 1) / kBlock);
         HIP_TRY(hipMemsetAsync(P.counters, 0, 256, ctx->stream));
-        hipLaunchKernelGGL(k_raygen, dim3((n0 + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, S, R, P, n0);
+        hipLaunchKernelGGL(k_raygen, dim3(pool_blocks), dim3(kBlock), 0, ctx->stream, S, R, P, n0);
         int next_sid = n0;
         int count = n0;
-        int32_t* qa = P.q_a;
-        int32_t* qb = P.q_b;
         while (count > 0) {
             int32_t cnt[64] = {};
-            const int32_t* retired = qa;
-            int n_ret = count;
+            int n_ret = count, n_next = 0;
+            HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * kCntClear, ctx->stream));
             if (s->max_depth > 0) {
-                HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * (kCntSeg + kMaxSeg), ctx->stream));
-                dim3 grid((count + kBlock - 1) / kBlock);
+                dim3 grid(pool_blocks);
                 HIP_TRY(hipEventRecord(ev[0], ctx->stream));
-                DISPATCH_SD_C(sc->stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, S, P, qa, count);
+                DISPATCH_SD_C(sc->stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, S, P);
                 HIP_TRY(hipEventRecord(ev[1], ctx->stream));
-                PathBufs Pb = P;
-                Pb.q_b = qb;
                 switch (ctx->shade_wpe) {
-                    case 2: hipLaunchKernelGGL(k_shade<2>, grid, dim3(kBlock), 0, ctx->stream, S, R, Pb, qa, count); break;
-                    case 3: hipLaunchKernelGGL(k_shade<3>, grid, dim3(kBlock), 0, ctx->stream, S, R, Pb, qa, count); break;
-                    case 4: hipLaunchKernelGGL(k_shade<4>, grid, dim3(kBlock), 0, ctx->stream, S, R, Pb, qa, count); break;
-                    default: hipLaunchKernelGGL(k_shade<1>, grid, dim3(kBlock), 0, ctx->stream, S, R, Pb, qa, count);
+                    case 2: hipLaunchKernelGGL(k_shade<2>, grid, dim3(kBlock), 0, ctx->stream, S, R, P); break;
+                    case 3: hipLaunchKernelGGL(k_shade<3>, grid, dim3(kBlock), 0, ctx->stream, S, R, P); break;
+                    case 4: hipLaunchKernelGGL(k_shade<4>, grid, dim3(kBlock), 0, ctx->stream, S, R, P); break;
+                    default: hipLaunchKernelGGL(k_shade<1>, grid, dim3(kBlock), 0, ctx->stream, S, R, P);
                 }
                 HIP_TRY(hipEventRecord(ev[2], ctx->stream));
                 HIP_TRY(hipMemcpyAsync(cnt, P.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
@@ -2193,18 +2198,16 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
                     shadow_launches++;
                     n_shadow_rays += n_sh;
                 }
-                retired = P.q_r;
                 n_ret = cnt[kCntRetire];
+                n_next = cnt[kCntNext];
             }
             // retire ended paths; restart their slots on the next samples
             const int n_regen = std::min(n_ret, ws - next_sid);
             if (n_ret > 0)
-                hipLaunchKernelGGL(k_retire, dim3((n_ret + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, S, R, P,
-                                   retired, n_ret, n_regen, next_sid, qb + cnt[kCntNext]);
+                hipLaunchKernelGGL(k_retire, dim3(pool_blocks), dim3(kBlock), 0, ctx->stream, S, R, P, n_regen, next_sid);
             HIP_TRY(hipGetLastError());
             next_sid += n_regen;
-            count = cnt[kCntNext] + n_regen;
-            std::swap(qa, qb);
+            count = n_next + n_regen;
         }
         hipLaunchKernelGGL(k_accum, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, R, P, F, passes);
         HIP_TRY(hipGetLastError());

@@ -92,7 +92,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the instrumented traversal-count render")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 4 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.empty(n, dtype=torch.float32, device="cuda:0").uniform_()
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "copies": 4}), flush=True)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU scaling is a design note "
