@@ -27,6 +27,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "rt_scene.h"
@@ -69,10 +70,46 @@ struct RParams {
     int base_sample;            // first sample pass of the window
 };
 
-// counters[]: next-queue size, shadow-queue size (ungrouped form), one size
-// per light-sample segment of the grouped shadow queue, stack-overflow flag
+// A window is a run of whole sample passes of one or more jobs; part k
+// holds passes [pass0, pass0 + npass) of job `job` at window samples
+// [off, off + npass * npix), pass-major.
+struct PartDev {
+    int job, pass0, off, npix, npass, pad[3];
+};
+
+// Batch job on the device: its scene view and render parameters.
+struct JobDev {
+    DevScene S;
+    RParams R;
+};
+
+// Scene/parameter source of the kernels: the kernel arguments of a single
+// render (uniform, scalar registers) or the path's job record of a batch.
+struct OneJob {
+    DevScene S;
+    RParams R;
+    __device__ __forceinline__ int job_of(const int32_t*, int) const { return 0; }
+    __device__ __forceinline__ const DevScene& scene(int) const { return S; }
+    __device__ __forceinline__ const RParams& params(int) const { return R; }
+};
+struct ManyJobs {
+    const JobDev* J;
+    __device__ __forceinline__ int job_of(const int32_t* job, int p) const { return job[p]; }
+    __device__ __forceinline__ const DevScene& scene(int j) const { return J[j].S; }
+    __device__ __forceinline__ const RParams& params(int j) const { return J[j].R; }
+};
+
+// counters[] (device-resident loop state):
+//   per iteration (cleared by k_iter_begin): paths active in the next
+//   iteration, shadow-queue size (ungrouped form), retired paths, gather
+//   requests, one size per light-sample segment of the grouped shadow queue;
+//   per window: next window sample to start, window size, paths active in
+//   the current iteration; per render: stack-overflow flag.
 constexpr int kCntNext = 0, kCntShadow = 1, kCntRetire = 2, kCntGather = 3, kCntSeg = 4, kMaxSeg = 32,
-              kCntRank = 36, kCntClear = 40, kCntOverflow = 63;
+              kCntClear = 40, kCntSid = 40, kCntWs = 41, kCntActive = 42, kCntWindow = 48, kCntOverflow = 63;
+// tcount[] (64-bit): traversal tests (trace node/tri/other, shadow node/tri/other), closest-hit segments, shadow rays
+constexpr int kTcRays = 6, kTcShadowRays = 7, kTcIters = 8, kTcShadowIters = 9;
+constexpr int kIterBatch = 4;   // bounce iterations enqueued between host checks
 enum : int32_t { kSlotIdle = 0, kSlotActive = 1, kSlotRetired = 2 };
 
 // Shadow queue view: segment j holds the shadow rays of light sample j
@@ -94,6 +131,9 @@ struct PathBufs {
     int32_t* smask;             // per slot: light samples with a pending shadow ray (grouped form)
     int32_t* q_sh;              // shadow queue: shadow record indices
     int32_t* sid;               // sample of the window each path slot is tracing
+    int32_t *job, *pix, *pass;  // its job (batch), pixel index and sample pass
+    const struct PartDev* parts;   // window parts (job, first pass, window offset)
+    int nparts;
     float4 *res_l, *res_a, *res_n;         // per-sample results of the window: (L, depth), albedo, normal
     float4 *gq0, *gq1, *gq2, *gq3;         // caustic gather requests: (o,t) (d,entry) (u,v,eta,path) (weight,target)
     int32_t* counters;          // kCnt* slots below
@@ -147,11 +187,11 @@ __device__ __forceinline__ int pf_pend(int32_t f) { return (f >> kPfPendShift) &
 // XCD-aware block order: gfx950 dispatches consecutive workgroups round-robin
 // over its 8 XCDs; renumber so each XCD walks one contiguous range of the
 // queue (neighbouring rays share BVH nodes in that XCD's L2).
-__device__ __forceinline__ uint32_t xcd_block() {
-    uint32_t b = blockIdx.x, nb = gridDim.x;
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
     uint32_t q = nb / 8u, r = nb % 8u, x = b % 8u;
     return x * q + min(x, r) + b / 8u;
 }
+__device__ __forceinline__ uint32_t xcd_block() { return xcd_remap(blockIdx.x, gridDim.x); }
 
 // Wave-aggregated queue allocation: returns the first slot for this lane's
 // `n` items (one atomic per wave).
@@ -557,11 +597,27 @@ __device__ __forceinline__ void camera_ray(const DevScene& S, const RParams& R, 
     d = target - o;
 }
 
-// Starts the camera path of window sample `sid` in slot p.
-__device__ __forceinline__ void init_path(const DevScene& S, const RParams& R, PathBufs& P, int p, int sid) {
-    int i, j, s;
-    sample_of(R, sid, i, j, s);
-    Rng rng{(uint32_t)s, pixel_seed(i, j, R.seed_xor), 0u};
+// Window sample `sid` → (part, job, pixel, pass) through the part table.
+__device__ __forceinline__ void resolve_sample(const PathBufs& P, int sid, int& job, int& pix, int& pass) {
+    int lo = 0, hi = P.nparts - 1;
+    while (lo < hi) {   // last part with off <= sid
+        int mid = (lo + hi + 1) / 2;
+        if (P.parts[mid].off <= sid) lo = mid;
+        else hi = mid - 1;
+    }
+    const PartDev pt = P.parts[lo];
+    int local = sid - pt.off;
+    job = pt.job;
+    pix = local % pt.npix;
+    pass = pt.pass0 + local / pt.npix;
+}
+
+// Starts the camera path of window sample `sid` (pixel `pix`, sample pass
+// `pass` of job `job`) in slot p.
+__device__ __forceinline__ void init_path(const DevScene& S, const RParams& R, PathBufs& P, int p, int sid, int job,
+                                          int pix, int pass) {
+    int i = pix % R.W, j = pix / R.W;
+    Rng rng{(uint32_t)pass, pixel_seed(i, j, R.seed_xor), 0u};
     f3 o, d;
     camera_ray(S, R, i, j, rng, o, d);
     P.o[p] = make_float4(o.x, o.y, o.z, 0.0f);
@@ -580,21 +636,59 @@ __device__ __forceinline__ void init_path(const DevScene& S, const RParams& R, P
         P.av4[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     P.sid[p] = sid;
+    P.job[p] = job;
+    P.pix[p] = pix;
+    P.pass[p] = pass;
 }
 
 // Initial fill of the path pool: slot p < n traces window sample p; the
 // rest of the pool stays idle.
-__global__ void __launch_bounds__(kBlock) k_raygen(DevScene S, RParams R, PathBufs P, int n) {
+template <class JOBS>
+__global__ void __launch_bounds__(kBlock) k_raygen(JOBS jobs, PathBufs P, int n, int ws) {
     int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= P.n) return;
-    if (p < n) init_path(S, R, P, p, p);
-    P.state[p] = p < n ? (R.max_depth > 0 ? kSlotActive : kSlotRetired) : kSlotIdle;
+    int st = kSlotIdle;
+    if (p < n) {
+        int job, pix, pass;
+        resolve_sample(P, p, job, pix, pass);
+        const RParams& R = jobs.params(job);
+        init_path(jobs.scene(job), R, P, p, p, job, pix, pass);
+        st = R.max_depth > 0 ? kSlotActive : kSlotRetired;
+    }
+    P.state[p] = st;
+    if (p == 0) {
+        P.counters[kCntNext] = n;
+        P.counters[kCntSid] = n;
+        P.counters[kCntWs] = ws;
+    }
+}
+
+// Start of one bounce iteration: snapshot the active-path count, add the
+// previous iteration's traced and shadow segments to the statistics, clear
+// the per-iteration counters.  One wave.
+__global__ void __launch_bounds__(kBlock) k_iter_begin(PathBufs P, int traced) {
+    if (threadIdx.x == 0) {
+        int active = P.counters[kCntNext];
+        long long sh = P.counters[kCntShadow];
+        for (int j = 0; j < kMaxSeg; j++) sh += P.counters[kCntSeg + j];
+        P.counters[kCntActive] = active;
+        if (traced && active > 0) {
+            P.tcount[kTcRays] += (unsigned long long)active;
+            P.tcount[kTcIters] += 1ull;
+        }
+        if (sh > 0) {
+            P.tcount[kTcShadowRays] += (unsigned long long)sh;
+            P.tcount[kTcShadowIters] += 1ull;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kCntClear; k += kBlock) P.counters[k] = 0;
 }
 
 // Traversal stack placement: LDS (lane-interleaved) or the wave's private
 // scratch (PRIV); chosen per context.
-template <int SD, bool COUNT, bool PRIV>
-__global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P) {
+template <class JOBS, int SD, bool COUNT, bool PRIV>
+__global__ void __launch_bounds__(kBlock) k_trace(JOBS jobs, PathBufs P) {
     __shared__ uint2 lds_stk[PRIV ? 1 : SD * kBlock];
     uint2 priv_stk[PRIV ? SD : 1];
     uint2* stk = PRIV ? priv_stk : &lds_stk[threadIdx.x];
@@ -605,6 +699,7 @@ __global__ void __launch_bounds__(kBlock) k_trace(DevScene S, PathBufs P) {
         if (COUNT) flush_counts(P.tcount, tc);
         return;
     }
+    const DevScene& S = jobs.scene(jobs.job_of(P.job, p));
     float4 o4 = P.o[p], d4 = P.d[p];
     f3 o = f3of(o4), d = f3of(d4);
     bool camera = pf_cat(__float_as_int(d4.w)) == 0;
@@ -649,10 +744,13 @@ __device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, i
     return flags & ~(0xff << kPfPendShift) & ~kPfPendInd;
 }
 
-template <int WPE>
-__global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, PathBufs P) {
+template <class JOBS, int WPE>
+__global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, int maxs) {
     const int p = xcd_block() * kBlock + threadIdx.x;
     const bool active = p < P.n && P.state[p] == kSlotActive;
+    const int job = active ? jobs.job_of(P.job, p) : 0;
+    const DevScene& S = jobs.scene(job);
+    const RParams& R = jobs.params(job);
     int n_shadow = 0;
     uint32_t smask = 0;   // bit j: light sample j produced a shadow ray
     bool cont = false;
@@ -662,9 +760,8 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
         int32_t flags = __float_as_int(d4.w);
         flags = fold_shadows(P, R, p, flags);
         int depth = __float_as_int(thr4.w);
-        int pi, pj, ps;
-        sample_of(R, P.sid[p], pi, pj, ps);
-        Rng rng{(uint32_t)ps, pixel_seed(pi, pj, R.seed_xor), __float_as_uint(wt4.w)};
+        const int pix = P.pix[p];
+        Rng rng{(uint32_t)P.pass[p], pixel_seed(pix % R.W, pix / R.W, R.seed_xor), __float_as_uint(wt4.w)};
         f3 o = f3of(o4), d = f3of(d4);
         float prev_pdf = o4.w;
         bool prev_delta = (flags & kPfPrevDelta) != 0;
@@ -739,7 +836,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
             aovf &= ~kAovPending;
         };
 
-        const bool grouped = R.maxs <= kMaxSeg;
+        const bool grouped = maxs <= kMaxSeg;
         f3 next_o = o, next_d = d;
         float next_pdf = 0.0f;
         bool next_delta = false;
@@ -1099,10 +1196,10 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(DevScene S, RParams R, Pa
     if (retire) P.state[p] = kSlotRetired;
     wave_alloc(&P.counters[kCntNext], cont ? 1 : 0);
     wave_alloc(&P.counters[kCntRetire], retire ? 1 : 0);
-    if (R.maxs <= kMaxSeg) {
+    if (maxs <= kMaxSeg) {
         // shadow rays grouped by light sample: neighbouring paths' rays
         // towards the same light share a wave (coherent traversal)
-        for (int j = 0; j < R.maxs; j++) {
+        for (int j = 0; j < maxs; j++) {
             bool has = (smask >> j) & 1u;
             int pos = wave_alloc(&P.counters[kCntSeg + j], has ? 1 : 0);
             if (has) P.q_sh[(int64_t)j * P.n + pos] = (int32_t)((int64_t)j * P.n + p);
@@ -1122,8 +1219,11 @@ namespace rt {
 // :1520-1526): k nearest photons, cone filter k_f = 1 (normalisation 1/3),
 // adaptive radius, occupancy fade for sparse gathers.  Added to the target
 // accumulator of the path with the vertex weight.
-__global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPhotons M, int count) {
-    int i = xcd_block() * kBlock + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPhotons M) {
+    const int count = P.counters[kCntGather];
+    const uint32_t nbk = (uint32_t)((count + kBlock - 1) / kBlock);
+    if (blockIdx.x >= nbk) return;
+    int i = xcd_remap(blockIdx.x, nbk) * kBlock + threadIdx.x;
     if (i >= count) return;
     float4 a = P.gq0[i], b = P.gq1[i], c = P.gq2[i], w4 = P.gq3[i];
     f3 o = f3of(a), d = f3of(b);
@@ -1166,21 +1266,41 @@ __global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPh
 }
 
 // Transparent shadow walk (ShadowRay.Transmittance, ShadowRay.cs:59-158)
-template <int SD, bool COUNT, bool PRIV>
-__global__ void __launch_bounds__(kBlock) k_shadow(DevScene S, PathBufs P, ShQ sq) {
+template <class JOBS, int SD, bool COUNT, bool PRIV>
+__global__ void __launch_bounds__(kBlock) k_shadow(JOBS jobs, PathBufs P, int maxs) {
     __shared__ uint2 lds_stk[PRIV ? 1 : SD * kBlock];
     uint2 priv_stk[PRIV ? SD : 1];
     uint2* stk = PRIV ? priv_stk : &lds_stk[threadIdx.x];
     constexpr int STRIDE = PRIV ? 1 : kBlock;
     TravCount tc{0u, 0u, 0u};
-    int i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= sq.pre[sq.nseg]) {
+    // shadow queue view from the segment sizes (uniform over the grid)
+    const bool grouped = maxs <= kMaxSeg;
+    int total = grouped ? 0 : P.counters[kCntShadow];
+    if (grouped)
+        for (int j = 0; j < maxs; j++) total += P.counters[kCntSeg + j];
+    const uint32_t nb = (uint32_t)((total + kBlock - 1) / kBlock);
+    if (blockIdx.x >= nb) {
         if (COUNT) flush_counts(P.tcount + 3, tc);
         return;
     }
-    int j = 0;
-    while (j + 1 < sq.nseg && i >= sq.pre[j + 1]) j++;
-    int64_t sl = P.q_sh[(int64_t)j * sq.stride + (i - sq.pre[j])];
+    int i = xcd_remap(blockIdx.x, nb) * kBlock + threadIdx.x;
+    if (i >= total) {
+        if (COUNT) flush_counts(P.tcount + 3, tc);
+        return;
+    }
+    int64_t qi = i;
+    if (grouped) {
+        int j = 0, pre = 0;
+        for (;;) {
+            int c = P.counters[kCntSeg + j];
+            if (i < pre + c || j + 1 >= maxs) break;
+            pre += c;
+            j++;
+        }
+        qi = (int64_t)j * P.n + (i - pre);
+    }
+    int64_t sl = P.q_sh[qi];
+    const DevScene& S = jobs.scene(jobs.job_of(P.job, (int)(sl % P.n)));
     float4 o4 = P.sh_o[sl], d4 = P.sh_d[sl];
     f3 org = f3of(o4), dir = f3of(d4);
     float rem = o4.w;
@@ -1259,28 +1379,36 @@ __device__ __forceinline__ void finish_path(const RParams& R, PathBufs& P, int p
     P.res_l[sid] = make_float4(L.x, L.y, L.z, depth);
 }
 
-// Retires the paths that ended in this iteration and restarts n_regen of
-// their slots on the next window samples (sid_base + rank); the others go
-// idle.  Which slot traces which sample does not change any result.
-__global__ void __launch_bounds__(kBlock) k_retire(DevScene S, RParams R, PathBufs P, int n_regen, int sid_base) {
+// Retires the paths that ended in this iteration and restarts their slots
+// on the next window samples (allocated from the window's sample counter);
+// once the window is exhausted the slots go idle.  Which slot traces which
+// sample does not change any result.
+template <class JOBS>
+__global__ void __launch_bounds__(kBlock) k_retire(JOBS jobs, PathBufs P) {
     const int p = xcd_block() * kBlock + threadIdx.x;
     const bool ret = p < P.n && P.state[p] == kSlotRetired;
-    if (ret) finish_path(R, P, p);
-    int r = wave_alloc(&P.counters[kCntRank], ret ? 1 : 0);
+    if (ret) finish_path(jobs.params(jobs.job_of(P.job, p)), P, p);
+    const int ws = P.counters[kCntWs];
+    int sid = wave_alloc(&P.counters[kCntSid], ret ? 1 : 0);
+    bool start = ret && sid < ws;
     if (ret) {
-        if (r < n_regen) {
-            init_path(S, R, P, p, sid_base + r);
-            P.state[p] = R.max_depth > 0 ? kSlotActive : kSlotRetired;
-        } else {
-            P.state[p] = kSlotIdle;
+        int st = kSlotIdle;
+        if (start) {
+            int job, pix, pass;
+            resolve_sample(P, sid, job, pix, pass);
+            const RParams& R = jobs.params(job);
+            init_path(jobs.scene(job), R, P, p, sid, job, pix, pass);
+            st = R.max_depth > 0 ? kSlotActive : kSlotRetired;
         }
+        P.state[p] = st;
     }
+    wave_alloc(&P.counters[kCntNext], start ? 1 : 0);
 }
 
 // Per-pixel film accumulation in sample order (Renderer.cs:668-672,
 // CaptureAccumulator.Add, Renderer.cs:829-856): the window's samples of a
 // pixel are summed pass by pass, even passes into half A, odd into half B.
-__global__ void __launch_bounds__(kBlock) k_accum(RParams R, PathBufs P, FilmBufs F, int passes) {
+__global__ void __launch_bounds__(kBlock) k_accum(RParams R, PathBufs P, FilmBufs F, int pass0, int passes, int off) {
     int pix = blockIdx.x * kBlock + threadIdx.x;
     if (pix >= R.npix) return;
     const int np = R.npix;
@@ -1288,8 +1416,8 @@ __global__ void __launch_bounds__(kBlock) k_accum(RParams R, PathBufs P, FilmBuf
     f3 sa = mk(F.sa[pix], F.sa[np + pix], F.sa[2 * np + pix]);
     f3 sb = mk(F.sb[pix], F.sb[np + pix], F.sb[2 * np + pix]);
     for (int sl = 0; sl < passes; sl++) {
-        int s = R.base_sample + sl;
-        int64_t k = (int64_t)sl * np + pix;
+        int s = pass0 + sl;
+        int64_t k = (int64_t)off + (int64_t)sl * np + pix;
         float4 l = P.res_l[k];
         f3 L = f3of(l);
         sum += L;
@@ -1416,6 +1544,14 @@ struct rt_ctx {
     int shade_wpe = 2;         // minimum waves per SIMD the shading kernel is compiled for
     void* ph_mem = nullptr;    // photon pass buffers (grow-only)
     size_t ph_bytes = 0;
+    hipEvent_t iev[2][kIterBatch][5] = {};   // per-iteration kernel timing, two batches
+    hipEvent_t chk[2] = {};                   // batch completion
+    int32_t* pinned = nullptr;                // host-visible pending counts (hipHostMalloc)
+    void* parts_mem = nullptr;                // window part tables (grow-only)
+    size_t parts_bytes = 0;
+    void* jobs_mem = nullptr;                 // batch job table (grow-only)
+    size_t jobs_bytes = 0;
+    bool use_graph = false;                   // replay captured bounce iterations (hipGraph)
 };
 
 struct rt_scene {
@@ -1486,21 +1622,28 @@ static DevScene dev_view(const rt_scene* sc) {
 }
 
 // Launches a traversal kernel with the LDS stack depth the scene needs.
-#define DISPATCH_SD_C(sd, cnt, priv, KERNEL, grid, ...)                                               \
+// Traversal kernels: job source J, stack depth, traversal counting, stack
+// placement (batches always use the private stack).
+#define DISPATCH_SD_C(J, sd, cnt, priv, KERNEL, grid, ...)                                            \
     do {                                                                                               \
-        if (cnt) {                                                                                     \
-            if (priv) DISPATCH_SD_T(sd, KERNEL, true, true, grid, __VA_ARGS__);                        \
-            else DISPATCH_SD_T(sd, KERNEL, true, false, grid, __VA_ARGS__);                            \
+        if constexpr (std::is_same<J, OneJob>::value) {                                                \
+            if (cnt) {                                                                                 \
+                if (priv) DISPATCH_SD_T(J, sd, KERNEL, true, true, grid, __VA_ARGS__);                 \
+                else DISPATCH_SD_T(J, sd, KERNEL, true, false, grid, __VA_ARGS__);                     \
+            } else {                                                                                   \
+                if (priv) DISPATCH_SD_T(J, sd, KERNEL, false, true, grid, __VA_ARGS__);                \
+                else DISPATCH_SD_T(J, sd, KERNEL, false, false, grid, __VA_ARGS__);                    \
+            }                                                                                          \
         } else {                                                                                       \
-            if (priv) DISPATCH_SD_T(sd, KERNEL, false, true, grid, __VA_ARGS__);                       \
-            else DISPATCH_SD_T(sd, KERNEL, false, false, grid, __VA_ARGS__);                           \
+            if (cnt) DISPATCH_SD_T(J, sd, KERNEL, true, true, grid, __VA_ARGS__);                      \
+            else DISPATCH_SD_T(J, sd, KERNEL, false, true, grid, __VA_ARGS__);                         \
         }                                                                                              \
     } while (0)
-#define DISPATCH_SD_T(sd, KERNEL, C_, P_, grid, ...)                                                        \
-    do {                                                                                                    \
-        if ((sd) <= 24) hipLaunchKernelGGL((KERNEL<24, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);       \
-        else if ((sd) <= 32) hipLaunchKernelGGL((KERNEL<32, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);  \
-        else hipLaunchKernelGGL((KERNEL<64, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);                  \
+#define DISPATCH_SD_T(J, sd, KERNEL, C_, P_, grid, ...)                                                        \
+    do {                                                                                                       \
+        if ((sd) <= 24) hipLaunchKernelGGL((KERNEL<J, 24, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);       \
+        else if ((sd) <= 32) hipLaunchKernelGGL((KERNEL<J, 32, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);  \
+        else hipLaunchKernelGGL((KERNEL<J, 64, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);                  \
     } while (0)
 #define DISPATCH_SD(sd, KERNEL, grid, ...)                                                       \
     do {                                                                                         \
@@ -1509,6 +1652,145 @@ static DevScene dev_view(const rt_scene* sc) {
         else if ((sd) <= 48) hipLaunchKernelGGL(KERNEL<48>, grid, dim3(kBlock), 0, __VA_ARGS__); \
         else hipLaunchKernelGGL(KERNEL<64>, grid, dim3(kBlock), 0, __VA_ARGS__);                 \
     } while (0)
+
+// ── bounce loop over sample windows (single render and batch) ─────────────
+struct WindowPlan {
+    std::vector<PartDev> parts;            // parts of all windows, in window order
+    std::vector<int> first, count, ws;     // per window: first part, part count, samples
+    void add_window(const std::vector<PartDev>& w, int samples) {
+        first.push_back((int)parts.size());
+        count.push_back((int)w.size());
+        ws.push_back(samples);
+        parts.insert(parts.end(), w.begin(), w.end());
+    }
+};
+
+struct LoopTiming {
+    double trace = 0, shade = 0, shadow = 0;
+};
+
+// Runs every window through the device-driven bounce loop.  Iterations are
+// enqueued in batches of kIterBatch without host waits (every grid is sized
+// to the pool, every count lives on the device); after each batch the
+// pending-path count goes to pinned memory, the host waits for the previous
+// batch's count (one batch stays queued) and ends the window once it is
+// zero.  accum(part) enqueues the film accumulation of one window part.
+template <class JOBS, class ACCUM>
+static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool traced, bool counting,
+                       int stack_depth, const DevScene* gather_scene, const DevPhotons* M, const WindowPlan& plan,
+                       ACCUM accum, LoopTiming& tm) {
+    const int64_t N = P.n;
+    const int pool_blocks = (int)((N + kBlock - 1) / kBlock);
+    const int shadow_blocks = (int)(((int64_t)N * maxs + kBlock - 1) / kBlock);
+    size_t pbytes = sizeof(PartDev) * std::max<size_t>(plan.parts.size(), 1);
+    if (pbytes > ctx->parts_bytes) {
+        if (ctx->parts_mem) hipFree(ctx->parts_mem);
+    if (ctx->jobs_mem) hipFree(ctx->jobs_mem);
+        ctx->parts_mem = nullptr;
+        ctx->parts_bytes = 0;
+        HIP_TRY(hipMalloc(&ctx->parts_mem, pbytes));
+        ctx->parts_bytes = pbytes;
+    }
+    PartDev* d_parts = (PartDev*)ctx->parts_mem;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(d_parts, plan.parts.data(), sizeof(PartDev) * plan.parts.size(), hipMemcpyHostToDevice));
+    auto enqueue_iteration = [&](const PathBufs& Pw, hipEvent_t* e) -> int {
+        hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(kBlock), 0, ctx->stream, Pw, traced ? 1 : 0);
+        if (traced) {
+            dim3 grid(pool_blocks);
+            if (e) HIP_TRY(hipEventRecord(e[0], ctx->stream));
+            DISPATCH_SD_C(JOBS, stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, jobs, Pw);
+            if (e) HIP_TRY(hipEventRecord(e[1], ctx->stream));
+            switch (ctx->shade_wpe) {
+                case 2: hipLaunchKernelGGL((k_shade<JOBS, 2>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
+                case 3: hipLaunchKernelGGL((k_shade<JOBS, 3>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
+                case 4: hipLaunchKernelGGL((k_shade<JOBS, 4>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
+                default: hipLaunchKernelGGL((k_shade<JOBS, 1>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+            }
+            if (e) HIP_TRY(hipEventRecord(e[2], ctx->stream));
+            if (gather_scene) hipLaunchKernelGGL(k_gather, grid, dim3(kBlock), 0, ctx->stream, *gather_scene, Pw, *M);
+            if (e) HIP_TRY(hipEventRecord(e[3], ctx->stream));
+            DISPATCH_SD_C(JOBS, stack_depth, counting, ctx->priv_stack, k_shadow, dim3(shadow_blocks), ctx->stream, jobs,
+                          Pw, maxs);
+            if (e) HIP_TRY(hipEventRecord(e[4], ctx->stream));
+        }
+        hipLaunchKernelGGL((k_retire<JOBS>), dim3(pool_blocks), dim3(kBlock), 0, ctx->stream, jobs, Pw);
+        HIP_TRY(hipGetLastError());
+        return RT_OK;
+    };
+    auto collect = [&](int set) {
+        if (!traced) return;
+        for (int k = 0; k < kIterBatch; k++) {
+            hipEvent_t* e = ctx->iev[set][k];
+            float ms = 0;
+            hipEventElapsedTime(&ms, e[0], e[1]);
+            tm.trace += ms;
+            hipEventElapsedTime(&ms, e[1], e[2]);
+            tm.shade += ms;
+            hipEventElapsedTime(&ms, e[3], e[4]);
+            tm.shadow += ms;
+        }
+    };
+    for (size_t w = 0; w < plan.ws.size(); w++) {
+        PathBufs Pw = P;
+        Pw.parts = d_parts + plan.first[w];
+        Pw.nparts = plan.count[w];
+        const int ws = plan.ws[w];
+        const int n0 = (int)std::min<int64_t>(N, ws);
+        HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * kCntWindow, ctx->stream));
+        hipLaunchKernelGGL((k_raygen<JOBS>), dim3(pool_blocks), dim3(kBlock), 0, ctx->stream, jobs, Pw, n0, ws);
+        // graph mode: the kIterBatch iterations of a batch are captured once
+        // per window (no memset nodes: k_iter_begin clears the counters) and
+        // replayed; per-kernel timings are not recorded
+        hipGraphExec_t gexec = nullptr;
+        if (ctx->use_graph) {
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            const bool saved = traced;
+            int rc = RT_OK;
+            for (int k = 0; k < kIterBatch && rc == RT_OK; k++) rc = enqueue_iteration(Pw, nullptr);
+            hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+            (void)saved;
+            if (rc) return rc;
+            if (ce != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+            ce = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (ce != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ce));
+        }
+        struct GraphGuard {
+            hipGraphExec_t g;
+            ~GraphGuard() { if (g) hipGraphExecDestroy(g); }
+        } guard{gexec};
+        for (int b = 0;; b++) {
+            const int set = b & 1;
+            if (gexec) {
+                HIP_TRY(hipGraphLaunch(gexec, ctx->stream));
+            } else {
+                for (int k = 0; k < kIterBatch; k++) {
+                    int rc = enqueue_iteration(Pw, ctx->iev[set][k]);
+                    if (rc) return rc;
+                }
+            }
+            HIP_TRY(hipMemcpyAsync(&ctx->pinned[2 * set], &P.counters[kCntNext], 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(&ctx->pinned[2 * set + 1], &P.counters[kCntOverflow], 4, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            HIP_TRY(hipEventRecord(ctx->chk[set], ctx->stream));
+            if (b == 0) continue;
+            const int prev = set ^ 1;
+            HIP_TRY(hipEventSynchronize(ctx->chk[prev]));
+            if (!gexec) collect(prev);
+            if (ctx->pinned[2 * prev + 1]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+            if (ctx->pinned[2 * prev] == 0) {
+                HIP_TRY(hipEventSynchronize(ctx->chk[set]));
+                if (!gexec) collect(set);
+                break;
+            }
+        }
+        for (int k = 0; k < plan.count[w]; k++) accum(plan.parts[plan.first[w] + k]);
+        HIP_TRY(hipGetLastError());
+    }
+    return RT_OK;
+}
 
 // ── caustic photon pass (host orchestration) ───────────────────────────────
 struct PhotonPass {
@@ -1721,17 +2003,24 @@ int rt_create(int32_t device, rt_ctx** out) {
     ctx->device = device;
     const char* stk = getenv("RT3D_STACK");
     ctx->priv_stack = !(stk && std::string(stk) == "lds");
+    const char* gr = getenv("RT3D_GRAPH");
+    ctx->use_graph = gr && std::string(gr) == "1";
     const char* wpe = getenv("RT3D_SHADE_WPE");
     if (wpe) ctx->shade_wpe = std::max(1, std::min(4, atoi(wpe)));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RT_ERR_DEVICE;
     }
-    for (auto& e : ctx->ev) {
-        if (hipEventCreate(&e) != hipSuccess) {
-            delete ctx;
-            return RT_ERR_DEVICE;
-        }
+    bool ok = true;
+    for (auto& e : ctx->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    for (auto& set : ctx->iev)
+        for (auto& it : set)
+            for (auto& e : it) ok = ok && hipEventCreate(&e) == hipSuccess;
+    for (auto& e : ctx->chk) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&ctx->pinned, 64, hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        rt_destroy(ctx);
+        return RT_ERR_DEVICE;
     }
     *out = ctx;
     return RT_OK;
@@ -1746,6 +2035,15 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->ph_mem) hipFree(ctx->ph_mem);
     for (auto& e : ctx->ev)
         if (e) hipEventDestroy(e);
+    for (auto& set : ctx->iev)
+        for (auto& it : set)
+            for (auto& e : it)
+                if (e) hipEventDestroy(e);
+    for (auto& e : ctx->chk)
+        if (e) hipEventDestroy(e);
+    if (ctx->pinned) hipHostFree(ctx->pinned);
+    if (ctx->parts_mem) hipFree(ctx->parts_mem);
+    if (ctx->jobs_mem) hipFree(ctx->jobs_mem);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1944,34 +2242,14 @@ int rt_photon_map(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, int64_t
     return RT_OK;
 }
 
-int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_film* film, rt_render_stats* stats) {
-    auto t_start = std::chrono::steady_clock::now();
-    if (!ctx || !sc) return RT_ERR_INVALID_ARGUMENT;
-    int rc = check_settings(ctx, s);
-    if (rc) return rc;
-    hipSetDevice(ctx->device);
-    const HostScene& h = sc->hs;
-    const int W = s->width, H = s->height, spp = s->samples_per_pixel;
-    const int npix = W * H;
-    // Path pool of N slots; samples are taken in windows of K whole passes
-    // over the image whose per-sample results are summed into the film in
-    // sample order when the window completes.  A slot whose path ends is
-    // restarted on the window's next sample, so the pool stays full.
-    int64_t target = s->paths_in_flight > 0 ? s->paths_in_flight : (1 << 21);
-    const int64_t kWindowCap = std::max<int64_t>(target, 1ll << 26);
-    int K = (int)std::max<int64_t>(1, std::min<int64_t>(spp, kWindowCap / npix));
-    const int64_t WS = (int64_t)K * npix;
-    const int64_t N = std::min<int64_t>(target, WS);
-    if (WS > (1ll << 31) - 1 || N > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
-    const int maxs = std::max(1, s->light_sampling != RT_LIGHTS_ALL ? h.max_light_samples : h.max_shadow_per_vertex);
-    if ((int64_t)maxs * N > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many shadow slots");
-    if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
-    const bool aovs = s->capture_aovs != 0;
-    const bool want_caustics = s->caustics && s->caustic_photons > 0 && !h.lights.empty();
+}  // extern "C"
 
-    // ── path buffers (grow-only cache in the context) ──────────────────────
-    size_t per_path = 8 * 16 + 8 * 4 + 4 + 16 + (aovs ? 5 * 16 : 0) + 4 * 16 * (size_t)maxs + 5 * 4 + 4 * (size_t)maxs +
-                      (want_caustics ? 64 : 0);
+// ── render set-up helpers shared by rt_render and rt_render_batch ─────────
+// Path pool of N slots (and per-sample results of a window of WS samples)
+// in the context's grow-only buffer.
+static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, bool gather, PathBufs& P) {
+    size_t per_path = 8 * 16 + 8 * 4 + 4 + 16 + (aovs ? 5 * 16 : 0) + 4 * 16 * (size_t)maxs + 8 * 4 + 4 * (size_t)maxs +
+                      (gather ? 64 : 0);
     size_t per_sample = aovs ? 48 : 16;
     size_t need = (size_t)N * per_path + (size_t)WS * per_sample + 40 * 256 + 16384;
     if (need > ctx->path_bytes) {
@@ -1987,10 +2265,10 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
         m += (bytes + 255) & ~(size_t)255;
         return (void*)r;
     };
-    PathBufs P{};
+    P = PathBufs{};
     P.n = (int)N;
     P.counters = (int32_t*)take(256);
-    P.tcount = (unsigned long long*)take(64);
+    P.tcount = (unsigned long long*)take(128);
     P.o = (float4*)take(16 * N);
     P.d = (float4*)take(16 * N);
     P.thr = (float4*)take(16 * N);
@@ -2016,9 +2294,12 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     P.q_sh = (int32_t*)take(4 * N * maxs);
     P.state = (int32_t*)take(4 * N);
     P.smask = (int32_t*)take(4 * N);
+    P.job = (int32_t*)take(4 * N);
+    P.pix = (int32_t*)take(4 * N);
+    P.pass = (int32_t*)take(4 * N);
     P.sid = (int32_t*)take(4 * N);
     P.res_l = (float4*)take(16 * WS);
-    if (want_caustics) {
+    if (gather) {
         P.gq0 = (float4*)take(16 * N);
         P.gq1 = (float4*)take(16 * N);
         P.gq2 = (float4*)take(16 * N);
@@ -2029,24 +2310,19 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
         P.res_n = (float4*)take(16 * WS);
     }
     if ((size_t)(m - (char*)ctx->path_mem) > ctx->path_bytes) return fail(ctx, RT_ERR_INTERNAL, "path buffer layout");
+    return RT_OK;
+}
 
-    // ── film accumulators ───────────────────────────────────────────────────
-    // 25 accumulator + 26 output floats per pixel, 21 regions padded to 256 B
-    size_t fneed = (size_t)npix * 4 * 51 + 32 * 256;
-    if (fneed > ctx->film_bytes) {
-        if (ctx->film_mem) hipFree(ctx->film_mem);
-        ctx->film_mem = nullptr;
-        ctx->film_bytes = 0;
-        HIP_TRY(hipMalloc(&ctx->film_mem, fneed));
-        ctx->film_bytes = fneed;
-    }
-    char* fm = (char*)ctx->film_mem;
+// Film accumulators + output planes of an npix image at `base` (null: size
+// only).  Returns the bytes used; *acc_bytes = the accumulator part.
+static size_t film_layout(char* base, int64_t npix, FilmBufs& F, size_t* acc_bytes) {
+    size_t off = 0;
     auto ftake = [&](size_t bytes) {
-        char* r = fm;
-        fm += (bytes + 255) & ~(size_t)255;
+        char* r = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
         return (void*)r;
     };
-    FilmBufs F{};
+    F = FilmBufs{};
     F.sum = (float*)ftake(12 * (size_t)npix);
     F.sa = (float*)ftake(12 * (size_t)npix);
     F.sb = (float*)ftake(12 * (size_t)npix);
@@ -2058,7 +2334,7 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     F.depb = (float*)ftake(4 * (size_t)npix);
     F.dca = (int32_t*)ftake(4 * (size_t)npix);
     F.dcb = (int32_t*)ftake(4 * (size_t)npix);
-    size_t acc_bytes = (size_t)(fm - (char*)ctx->film_mem);
+    if (acc_bytes) *acc_bytes = off;
     F.out_beauty = (float*)ftake(12 * (size_t)npix);
     F.out_a = (float*)ftake(12 * (size_t)npix);
     F.out_b = (float*)ftake(12 * (size_t)npix);
@@ -2069,48 +2345,147 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     F.out_nrmb = (float*)ftake(12 * (size_t)npix);
     F.out_depa = (float*)ftake(4 * (size_t)npix);
     F.out_depb = (float*)ftake(4 * (size_t)npix);
-    if ((size_t)(fm - (char*)ctx->film_mem) > ctx->film_bytes) return fail(ctx, RT_ERR_INTERNAL, "film buffer layout");
-    HIP_TRY(hipMemsetAsync(ctx->film_mem, 0, acc_bytes, ctx->stream));
+    return off;
+}
 
-    // ── scene view, light distribution ──────────────────────────────────────
-    rt_scene* msc = const_cast<rt_scene*>(sc);
+static int grow(rt_ctx* ctx, void** mem, size_t* bytes, size_t need) {
+    if (need <= *bytes) return RT_OK;
+    if (*mem) hipFree(*mem);
+    *mem = nullptr;
+    *bytes = 0;
+    HIP_TRY(hipMalloc(mem, need));
+    *bytes = need;
+    return RT_OK;
+}
+
+// LightDistribution (LightDistribution.cs:41-99) for the power / uniform
+// strategies, uploaded into the scene's table.
+static int upload_light_dist(rt_ctx* ctx, const rt_scene* sc, int strategy) {
+    const HostScene& h = sc->hs;
     int nl = (int)h.lights.size();
-    bool picked = s->light_sampling != RT_LIGHTS_ALL && nl > 0;
-    if (picked) {
-        // LightDistribution (LightDistribution.cs:41-99)
-        std::vector<float> cdf(nl), pdf(nl), pw(nl, 0.0f);
-        float sum = 0.0f;
-        if (s->light_sampling == RT_LIGHTS_POWER)
-            for (int i = 0; i < nl; i++) { pw[i] = std::max(0.0f, h.light_power[i]); sum += pw[i]; }
-        float cum = 0.0f;
-        if (sum > 0.0f) {
-            float inv = 1.0f / sum;
-            for (int i = 0; i < nl; i++) { pdf[i] = pw[i] * inv; cum += pdf[i]; cdf[i] = cum; }
-        } else {
-            float un = 1.0f / (float)nl;
-            for (int i = 0; i < nl; i++) { pdf[i] = un; cum += un; cdf[i] = cum; }
-        }
-        cdf[nl - 1] = 1.0f;
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        HIP_TRY(hipMemcpy(msc->d_ldist, cdf.data(), 4 * nl, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(msc->d_ldist + nl, pdf.data(), 4 * nl, hipMemcpyHostToDevice));
+    if (strategy == RT_LIGHTS_ALL || nl == 0) return RT_OK;
+    std::vector<float> cdf(nl), pdf(nl), pw(nl, 0.0f);
+    float sum = 0.0f;
+    if (strategy == RT_LIGHTS_POWER)
+        for (int i = 0; i < nl; i++) { pw[i] = std::max(0.0f, h.light_power[i]); sum += pw[i]; }
+    float cum = 0.0f;
+    if (sum > 0.0f) {
+        float inv = 1.0f / sum;
+        for (int i = 0; i < nl; i++) { pdf[i] = pw[i] * inv; cum += pdf[i]; cdf[i] = cum; }
+    } else {
+        float un = 1.0f / (float)nl;
+        for (int i = 0; i < nl; i++) { pdf[i] = un; cum += un; cdf[i] = cum; }
     }
-    DevScene S = dev_view(sc);
+    cdf[nl - 1] = 1.0f;
+    rt_scene* msc = const_cast<rt_scene*>(sc);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(msc->d_ldist, cdf.data(), 4 * nl, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(msc->d_ldist + nl, pdf.data(), 4 * nl, hipMemcpyHostToDevice));
+    return RT_OK;
+}
+
+static int shadow_slots(const rt_settings* s, const HostScene& h) {
+    return std::max(1, s->light_sampling != RT_LIGHTS_ALL ? h.max_light_samples : h.max_shadow_per_vertex);
+}
+
+// Renderer constructor arguments (Renderer.cs:300-378) as device parameters.
+static RParams make_params(const rt_settings* s, const HostScene& h, int maxs) {
     RParams R{};
-    R.W = W;
-    R.H = H;
-    R.spp = spp;
+    R.W = s->width;
+    R.H = s->height;
+    R.spp = s->samples_per_pixel;
     R.max_depth = s->max_depth;
-    R.npix = npix;
+    R.npix = s->width * s->height;
     R.max_rad = s->max_sample_radiance;
     R.ind_max_rad = s->max_sample_radiance * std::max(0.0f, s->indirect_clamp_factor);
     R.power_mis = s->mis == RT_MIS_POWER;
-    R.picked = picked ? 1 : 0;
+    R.picked = (s->light_sampling != RT_LIGHTS_ALL && !h.lights.empty()) ? 1 : 0;
     R.rr_min_b = s->rr_min_bounces >= 0 ? s->rr_min_bounces : h.rr_min_bounces;
     R.rr_min_s = s->rr_min_survival >= 0.0f ? s->rr_min_survival : h.rr_min_survival;
     R.seed_xor = s->seed_xor;
-    R.aovs = aovs ? 1 : 0;
+    R.aovs = s->capture_aovs ? 1 : 0;
     R.maxs = maxs;
+    return R;
+}
+
+// Resolve (mean, halves, display) and copy the requested planes out.
+static int finish_film(rt_ctx* ctx, const RParams& R, const FilmBufs& F, const rt_settings* s, const rt_film* film) {
+    const int64_t npix = (int64_t)R.npix;
+    float exposure = powf(2.0f, s->exposure_ev);
+    hipLaunchKernelGGL(k_resolve, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, ctx->stream, R, F, exposure);
+    HIP_TRY(hipGetLastError());
+    if (!film) return RT_OK;
+    const bool aovs = R.aovs != 0;
+    size_t c3 = 12 * (size_t)npix, c1 = 4 * (size_t)npix;
+    struct { void* dst; const void* src; size_t n; } copies[] = {
+        {film->beauty, F.out_beauty, c3}, {film->beauty_a, F.out_a, c3}, {film->beauty_b, F.out_b, c3},
+        {film->display, F.out_disp, c3},
+        {aovs ? film->albedo_a : nullptr, F.out_alba, c3}, {aovs ? film->albedo_b : nullptr, F.out_albb, c3},
+        {aovs ? film->normal_a : nullptr, F.out_nrma, c3}, {aovs ? film->normal_b : nullptr, F.out_nrmb, c3},
+        {aovs ? film->depth_a : nullptr, F.out_depa, c1}, {aovs ? film->depth_b : nullptr, F.out_depb, c1},
+    };
+    for (auto& c : copies)
+        if (c.dst) HIP_TRY(hipMemcpyAsync(c.dst, c.src, c.n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (film->sample_count)
+        for (int64_t k = 0; k < npix; k++) film->sample_count[k] = s->samples_per_pixel;
+    return RT_OK;
+}
+
+static void fill_counts(const unsigned long long* tcv, bool counting, rt_render_stats* st) {
+    st->rays_closest = (int64_t)tcv[kTcRays];
+    st->rays_shadow = (int64_t)tcv[kTcShadowRays];
+    st->trace_launches = (int64_t)tcv[kTcIters];
+    st->shadow_launches = (int64_t)tcv[kTcShadowIters];
+    if (counting) {
+        st->trace_node_tests = (int64_t)tcv[0];
+        st->trace_tri_tests = (int64_t)tcv[1];
+        st->trace_other_tests = (int64_t)tcv[2];
+        st->shadow_node_tests = (int64_t)tcv[3];
+        st->shadow_tri_tests = (int64_t)tcv[4];
+        st->shadow_other_tests = (int64_t)tcv[5];
+    }
+}
+
+extern "C" {
+
+int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_film* film, rt_render_stats* stats) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (!ctx || !sc) return RT_ERR_INVALID_ARGUMENT;
+    int rc = check_settings(ctx, s);
+    if (rc) return rc;
+    hipSetDevice(ctx->device);
+    const HostScene& h = sc->hs;
+    const int spp = s->samples_per_pixel;
+    const int npix = s->width * s->height;
+    // Path pool of N slots; samples are taken in windows of K whole passes
+    // over the image whose per-sample results are summed into the film in
+    // sample order when the window completes.  A slot whose path ends is
+    // restarted on the window's next sample, so the pool stays full.
+    int64_t target = s->paths_in_flight > 0 ? s->paths_in_flight : (1 << 21);
+    const int64_t kWindowCap = std::max<int64_t>(target, std::min<int64_t>(1ll << 26, 32 * target));
+    int K = (int)std::max<int64_t>(1, std::min<int64_t>(spp, kWindowCap / npix));
+    const int64_t WS = (int64_t)K * npix;
+    const int64_t N = std::min<int64_t>(target, WS);
+    if (WS > (1ll << 31) - 1 || N > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
+    const int maxs = shadow_slots(s, h);
+    if ((int64_t)maxs * N > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many shadow slots");
+    if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
+    const bool want_caustics = s->caustics && s->caustic_photons > 0 && !h.lights.empty();
+
+    PathBufs P;
+    rc = alloc_paths(ctx, N, WS, maxs, s->capture_aovs != 0, want_caustics, P);
+    if (rc) return rc;
+    FilmBufs F;
+    size_t acc_bytes = 0;
+    rc = grow(ctx, &ctx->film_mem, &ctx->film_bytes, film_layout(nullptr, npix, F, nullptr) + 256);
+    if (rc) return rc;
+    film_layout((char*)ctx->film_mem, npix, F, &acc_bytes);
+    HIP_TRY(hipMemsetAsync(ctx->film_mem, 0, acc_bytes, ctx->stream));
+    rc = upload_light_dist(ctx, sc, s->light_sampling);
+    if (rc) return rc;
+    DevScene S = dev_view(sc);
+    RParams R = make_params(s, h, maxs);
 
     // ── caustic photon pre-pass (Renderer.cs:385-406) ──────────────────────
     PhotonPass PP;
@@ -2125,120 +2500,36 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
         }
     }
 
-    double t_trace = 0, t_shade = 0, t_shadow = 0;
-    int64_t n_closest = 0, n_shadow_rays = 0, launches = 0, shadow_launches = 0;
     const bool counting = s->count_traversal != 0;
-    HIP_TRY(hipMemsetAsync(P.tcount, 0, 64, ctx->stream));
+    HIP_TRY(hipMemsetAsync(P.tcount, 0, 128, ctx->stream));
+    HIP_TRY(hipMemsetAsync(P.counters, 0, 256, ctx->stream));
     hipEvent_t* ev = ctx->ev;
     HIP_TRY(hipEventRecord(ev[6], ctx->stream));
-    bool shadow_timed = false;   // ev[3]..ev[4] bracket a shadow launch not yet read
-    auto read_shadow_time = [&]() {
-        if (!shadow_timed) return;
-        float ms = 0;
-        hipEventElapsedTime(&ms, ev[3], ev[4]);
-        t_shadow += ms;
-        shadow_timed = false;
-    };
+    WindowPlan plan;
     for (int base = 0; base < spp; base += K) {
         const int passes = std::min(K, spp - base);
-        const int ws = passes * npix;
-        R.base_sample = base;
-        const int n0 = (int)std::min<int64_t>(N, ws);
-        const int pool_blocks = (int)((N + kBlock - 1) / kBlock);
-        HIP_TRY(hipMemsetAsync(P.counters, 0, 256, ctx->stream));
-        hipLaunchKernelGGL(k_raygen, dim3(pool_blocks), dim3(kBlock), 0, ctx->stream, S, R, P, n0);
-        int next_sid = n0;
-        int count = n0;
-        while (count > 0) {
-            int32_t cnt[64] = {};
-            int n_ret = count, n_next = 0;
-            HIP_TRY(hipMemsetAsync(P.counters, 0, 4 * kCntClear, ctx->stream));
-            if (s->max_depth > 0) {
-                dim3 grid(pool_blocks);
-                HIP_TRY(hipEventRecord(ev[0], ctx->stream));
-                DISPATCH_SD_C(sc->stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, S, P);
-                HIP_TRY(hipEventRecord(ev[1], ctx->stream));
-                switch (ctx->shade_wpe) {
-                    case 2: hipLaunchKernelGGL(k_shade<2>, grid, dim3(kBlock), 0, ctx->stream, S, R, P); break;
-                    case 3: hipLaunchKernelGGL(k_shade<3>, grid, dim3(kBlock), 0, ctx->stream, S, R, P); break;
-                    case 4: hipLaunchKernelGGL(k_shade<4>, grid, dim3(kBlock), 0, ctx->stream, S, R, P); break;
-                    default: hipLaunchKernelGGL(k_shade<1>, grid, dim3(kBlock), 0, ctx->stream, S, R, P);
-                }
-                HIP_TRY(hipEventRecord(ev[2], ctx->stream));
-                HIP_TRY(hipMemcpyAsync(cnt, P.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
-                HIP_TRY(hipStreamSynchronize(ctx->stream));
-                if (S.caustics && cnt[kCntGather] > 0)
-                    hipLaunchKernelGGL(k_gather, dim3((cnt[kCntGather] + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                                       ctx->stream, S, P, PP.M, cnt[kCntGather]);
-                read_shadow_time();
-                float ms = 0;
-                hipEventElapsedTime(&ms, ev[0], ev[1]);
-                t_trace += ms;
-                hipEventElapsedTime(&ms, ev[1], ev[2]);
-                t_shade += ms;
-                n_closest += count;
-                launches++;
-                if (cnt[kCntOverflow]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
-                ShQ sq{};
-                if (maxs <= kMaxSeg) {
-                    sq.nseg = maxs;
-                    sq.stride = (int)N;
-                    for (int j = 0; j < maxs; j++) sq.pre[j + 1] = sq.pre[j] + cnt[kCntSeg + j];
-                } else {
-                    sq.nseg = 1;
-                    sq.pre[1] = cnt[kCntShadow];
-                }
-                const int n_sh = sq.pre[sq.nseg];
-                if (n_sh > 0) {
-                    HIP_TRY(hipEventRecord(ev[3], ctx->stream));
-                    DISPATCH_SD_C(sc->stack_depth, counting, ctx->priv_stack, k_shadow, dim3((n_sh + kBlock - 1) / kBlock),
-                                  ctx->stream, S, P, sq);
-                    HIP_TRY(hipEventRecord(ev[4], ctx->stream));
-                    shadow_timed = true;
-                    shadow_launches++;
-                    n_shadow_rays += n_sh;
-                }
-                n_ret = cnt[kCntRetire];
-                n_next = cnt[kCntNext];
-            }
-            // retire ended paths; restart their slots on the next samples
-            const int n_regen = std::min(n_ret, ws - next_sid);
-            if (n_ret > 0)
-                hipLaunchKernelGGL(k_retire, dim3(pool_blocks), dim3(kBlock), 0, ctx->stream, S, R, P, n_regen, next_sid);
-            HIP_TRY(hipGetLastError());
-            next_sid += n_regen;
-            count = n_next + n_regen;
-        }
-        hipLaunchKernelGGL(k_accum, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, R, P, F, passes);
-        HIP_TRY(hipGetLastError());
+        plan.add_window({PartDev{0, base, 0, npix, passes, {0, 0, 0}}}, passes * npix);
     }
-    float exposure = powf(2.0f, s->exposure_ev);
-    hipLaunchKernelGGL(k_resolve, dim3((npix + 255) / 256), dim3(256), 0, ctx->stream, R, F, exposure);
-    HIP_TRY(hipGetLastError());
+    LoopTiming tm;
+    OneJob one{S, R};
+    rc = run_windows(ctx, one, P, maxs, s->max_depth > 0, counting, sc->stack_depth, S.caustics ? &S : nullptr, &PP.M,
+                     plan,
+                     [&](const PartDev& pt) {
+                         hipLaunchKernelGGL(k_accum, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream,
+                                            R, P, F, pt.pass0, pt.npass, pt.off);
+                     },
+                     tm);
+    if (rc) return rc;
+    const double t_trace = tm.trace, t_shade = tm.shade, t_shadow = tm.shadow;
+    rc = finish_film(ctx, R, F, s, film);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(ev[7], ctx->stream));
     HIP_TRY(hipEventSynchronize(ev[7]));
-    read_shadow_time();
     float dev_ms = 0;
     hipEventElapsedTime(&dev_ms, ev[6], ev[7]);
     int32_t ovf = 0;
     HIP_TRY(hipMemcpy(&ovf, P.counters + kCntOverflow, 4, hipMemcpyDeviceToHost));
     if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
-    // ── film D2H ────────────────────────────────────────────────────────────
-    if (film) {
-        size_t c3 = 12 * (size_t)npix, c1 = 4 * (size_t)npix;
-        struct { void* dst; const void* src; size_t n; } copies[] = {
-            {film->beauty, F.out_beauty, c3}, {film->beauty_a, F.out_a, c3}, {film->beauty_b, F.out_b, c3},
-            {film->display, F.out_disp, c3},
-            {aovs ? film->albedo_a : nullptr, F.out_alba, c3}, {aovs ? film->albedo_b : nullptr, F.out_albb, c3},
-            {aovs ? film->normal_a : nullptr, F.out_nrma, c3}, {aovs ? film->normal_b : nullptr, F.out_nrmb, c3},
-            {aovs ? film->depth_a : nullptr, F.out_depa, c1}, {aovs ? film->depth_b : nullptr, F.out_depb, c1},
-        };
-        for (auto& c : copies)
-            if (c.dst) HIP_TRY(hipMemcpyAsync(c.dst, c.src, c.n, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (film->sample_count)
-            for (int k = 0; k < npix; k++) film->sample_count[k] = spp;
-    }
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -2247,21 +2538,165 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
         stats->shade_ms = t_shade;
         stats->shadow_ms = t_shadow;
         stats->camera_samples = (int64_t)npix * spp;
-        stats->rays_closest = n_closest;
-        stats->rays_shadow = n_shadow_rays;
-        stats->trace_launches = launches;
-        stats->shadow_launches = shadow_launches;
-        if (counting) {
-            unsigned long long tcv[6];
-            HIP_TRY(hipMemcpy(tcv, P.tcount, sizeof(tcv), hipMemcpyDeviceToHost));
-            stats->trace_node_tests = (int64_t)tcv[0];
-            stats->trace_tri_tests = (int64_t)tcv[1];
-            stats->trace_other_tests = (int64_t)tcv[2];
-            stats->shadow_node_tests = (int64_t)tcv[3];
-            stats->shadow_tri_tests = (int64_t)tcv[4];
-            stats->shadow_other_tests = (int64_t)tcv[5];
+        unsigned long long tcv[16];
+        HIP_TRY(hipMemcpy(tcv, P.tcount, sizeof(tcv), hipMemcpyDeviceToHost));
+        fill_counts(tcv, counting, stats);
+    }
+    return RT_OK;
+}
+
+
+int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32_t pool_paths,
+                    rt_render_stats* total) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (!ctx || !jobs || n_jobs <= 0) return RT_ERR_INVALID_ARGUMENT;
+    hipSetDevice(ctx->device);
+    int rc;
+    // validate; shadow-slot count, traversal stack depth and AOV buffers are
+    // shared by the pool, so they take the maximum over the jobs
+    int maxs = 1, stack_depth = 0;
+    bool aovs = false, counting = false, traced = false;
+    std::vector<std::pair<const rt_scene*, int>> strategy;
+    for (int j = 0; j < n_jobs; j++) {
+        const rt_batch_job& b = jobs[j];
+        if (!b.scene || !b.settings) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "batch job without scene or settings");
+        rc = check_settings(ctx, b.settings);
+        if (rc) return rc;
+        if (b.settings->caustics) return fail(ctx, RT_ERR_UNSUPPORTED, "photon caustics are not supported in a batch");
+        for (auto& p : strategy)
+            if (p.first == b.scene && p.second != b.settings->light_sampling)
+                return fail(ctx, RT_ERR_UNSUPPORTED, "one scene with two light-sampling strategies in a batch");
+        strategy.push_back({b.scene, b.settings->light_sampling});
+        maxs = std::max(maxs, shadow_slots(b.settings, b.scene->hs));
+        stack_depth = std::max(stack_depth, b.scene->stack_depth);
+        aovs = aovs || b.settings->capture_aovs;
+        counting = counting || b.settings->count_traversal;
+        traced = traced || b.settings->max_depth > 0;
+    }
+    if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
+    // windows over the job-major sample sequence, split at pass boundaries
+    const int64_t N0 = pool_paths > 0 ? pool_paths : (1 << 21);
+    const int64_t cap = std::max<int64_t>(N0, std::min<int64_t>(1ll << 26, 32 * N0));
+    WindowPlan plan;
+    {
+        std::vector<PartDev> cur;
+        int64_t used = 0;
+        for (int j = 0; j < n_jobs; j++) {
+            const rt_settings* st = jobs[j].settings;
+            const int64_t npix = (int64_t)st->width * st->height;
+            int pass = 0;
+            while (pass < st->samples_per_pixel) {
+                int64_t room = (cap - used) / npix;
+                if (room <= 0 && !cur.empty()) {
+                    plan.add_window(cur, (int)used);
+                    cur.clear();
+                    used = 0;
+                    continue;
+                }
+                int np = (int)std::max<int64_t>(1, std::min<int64_t>(room, st->samples_per_pixel - pass));
+                if (used + np * npix > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
+                cur.push_back(PartDev{j, pass, (int)used, (int)npix, np, {0, 0, 0}});
+                used += np * npix;
+                pass += np;
+            }
+        }
+        if (!cur.empty()) plan.add_window(cur, (int)used);
+    }
+    int64_t WS = 1;
+    for (int w : plan.ws) WS = std::max<int64_t>(WS, w);
+    const int64_t N = std::min<int64_t>(N0, WS);
+    if (N > (1ll << 30) || (int64_t)maxs * N > (1ll << 31) - 1)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "path pool too large");
+    PathBufs P;
+    rc = alloc_paths(ctx, N, WS, maxs, aovs, false, P);
+    if (rc) return rc;
+    // per-job films, one buffer
+    std::vector<FilmBufs> F(n_jobs);
+    std::vector<size_t> foff(n_jobs + 1, 0);
+    for (int j = 0; j < n_jobs; j++) {
+        const int64_t npix = (int64_t)jobs[j].settings->width * jobs[j].settings->height;
+        foff[j + 1] = foff[j] + ((film_layout(nullptr, npix, F[j], nullptr) + 255) & ~(size_t)255);
+    }
+    rc = grow(ctx, &ctx->film_mem, &ctx->film_bytes, foff[n_jobs] + 256);
+    if (rc) return rc;
+    for (int j = 0; j < n_jobs; j++) {
+        size_t acc = 0;
+        const int64_t npix = (int64_t)jobs[j].settings->width * jobs[j].settings->height;
+        film_layout((char*)ctx->film_mem + foff[j], npix, F[j], &acc);
+        HIP_TRY(hipMemsetAsync((char*)ctx->film_mem + foff[j], 0, acc, ctx->stream));
+    }
+    // job table
+    std::vector<JobDev> jd(n_jobs);
+    std::vector<RParams> R(n_jobs);
+    for (int j = 0; j < n_jobs; j++) {
+        rc = upload_light_dist(ctx, jobs[j].scene, jobs[j].settings->light_sampling);
+        if (rc) return rc;
+        jd[j].S = dev_view(jobs[j].scene);
+        R[j] = make_params(jobs[j].settings, jobs[j].scene->hs, maxs);
+        jd[j].R = R[j];
+    }
+    rc = grow(ctx, &ctx->jobs_mem, &ctx->jobs_bytes, sizeof(JobDev) * n_jobs);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(ctx->jobs_mem, jd.data(), sizeof(JobDev) * n_jobs, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemsetAsync(P.tcount, 0, 128, ctx->stream));
+    HIP_TRY(hipMemsetAsync(P.counters, 0, 256, ctx->stream));
+    hipEvent_t* ev = ctx->ev;
+    HIP_TRY(hipEventRecord(ev[6], ctx->stream));
+    LoopTiming tm;
+    ManyJobs many{(const JobDev*)ctx->jobs_mem};
+    rc = run_windows(ctx, many, P, maxs, traced, counting, stack_depth, nullptr, nullptr, plan,
+                     [&](const PartDev& pt) {
+                         const int64_t npix = pt.npix;
+                         hipLaunchKernelGGL(k_accum, dim3((unsigned)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                                            ctx->stream, R[pt.job], P, F[pt.job], pt.pass0, pt.npass, pt.off);
+                     },
+                     tm);
+    if (rc) return rc;
+    for (int j = 0; j < n_jobs; j++) {
+        rc = finish_film(ctx, R[j], F[j], jobs[j].settings, jobs[j].film);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(ev[7], ctx->stream));
+    HIP_TRY(hipEventSynchronize(ev[7]));
+    float dev_ms = 0;
+    hipEventElapsedTime(&dev_ms, ev[6], ev[7]);
+    int32_t ovf = 0;
+    HIP_TRY(hipMemcpy(&ovf, P.counters + kCntOverflow, 4, hipMemcpyDeviceToHost));
+    if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+    int64_t samples = 0;
+    for (int j = 0; j < n_jobs; j++) {
+        int64_t sj = (int64_t)jobs[j].settings->width * jobs[j].settings->height * jobs[j].settings->samples_per_pixel;
+        samples += sj;
+        if (jobs[j].stats) {
+            std::memset(jobs[j].stats, 0, sizeof(rt_render_stats));
+            jobs[j].stats->camera_samples = sj;
         }
     }
+    if (total) {
+        std::memset(total, 0, sizeof(*total));
+        total->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        total->device_ms = dev_ms;
+        total->trace_ms = tm.trace;
+        total->shade_ms = tm.shade;
+        total->shadow_ms = tm.shadow;
+        total->camera_samples = samples;
+        unsigned long long tcv[16];
+        HIP_TRY(hipMemcpy(tcv, P.tcount, sizeof(tcv), hipMemcpyDeviceToHost));
+        fill_counts(tcv, counting, total);
+    }
+    return RT_OK;
+}
+
+
+int rt_context_tune(rt_ctx* ctx, int32_t shade_waves_per_simd, int32_t traversal_stack, int32_t bounce_graph) {
+    if (!ctx) return RT_ERR_INVALID_ARGUMENT;
+    if (shade_waves_per_simd < 0 || shade_waves_per_simd > 4 || traversal_stack < 0 || traversal_stack > 2 ||
+        bounce_graph < 0 || bounce_graph > 2)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "shade waves per SIMD in [0, 4], traversal stack and graph in [0, 2]");
+    if (shade_waves_per_simd > 0) ctx->shade_wpe = shade_waves_per_simd;
+    if (traversal_stack > 0) ctx->priv_stack = traversal_stack == 1;
+    if (bounce_graph > 0) ctx->use_graph = bounce_graph == 2;
     return RT_OK;
 }
 

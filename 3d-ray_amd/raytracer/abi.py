@@ -96,6 +96,11 @@ class rt_render_stats(C.Structure):
                 ("shadow_other_tests", C.c_int64)]
 
 
+class rt_batch_job(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("settings", C.POINTER(rt_settings)), ("film", C.POINTER(rt_film)),
+                ("stats", C.POINTER(rt_render_stats))]
+
+
 class rt_scene_info(C.Structure):
     _fields_ = [("num_primitives", C.c_int64), ("num_objects", C.c_int64), ("num_nodes", C.c_int64),
                 ("num_leaves", C.c_int64), ("max_depth", C.c_int32), ("has_bvh", C.c_int32),

@@ -64,6 +64,8 @@ def load():
     i64p = C.POINTER(C.c_int64)
     L.rt_photon_map.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.c_int64, C.c_int64, i64p, i64p, i64p, fp, fp, fp,
                                 ip, ip, ip, fp]
+    L.rt_render_batch.argtypes = [vp, C.POINTER(abi.rt_batch_job), C.c_int32, C.c_int32, C.POINTER(abi.rt_render_stats)]
+    L.rt_context_tune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
     L.rt_debug_bvh_dump.argtypes = [C.POINTER(abi.rt_scene_desc), ip, C.c_int64]
     L.rt_debug_bvh_dump.restype = C.c_int64
     _lib = L
@@ -72,7 +74,8 @@ def load():
 
 EXPORTED_SYMBOLS = ("rt_get_version", "rt_material_defaults", "rt_settings_defaults", "rt_create", "rt_destroy",
                     "rt_last_error", "rt_scene_create", "rt_scene_destroy", "rt_scene_get_info", "rt_render",
-                    "rt_trace_primary", "rt_trace_rays", "rt_photon_map", "rt_debug_bvh_dump")
+                    "rt_trace_primary", "rt_trace_rays", "rt_photon_map", "rt_render_batch", "rt_context_tune",
+                    "rt_debug_bvh_dump")
 
 
 def _f(a):
@@ -109,6 +112,11 @@ class Context:
     def check(self, rc):
         if rc != 0:
             raise RtError(rc, load().rt_last_error(self.h).decode())
+
+    def tune(self, shade_waves_per_simd=0, traversal_stack=0, bounce_graph=0):
+        """rt_context_tune: kernel configuration (0 = keep); stack 1 private,
+        2 LDS; bounce_graph 1 direct launches, 2 hipGraph replay."""
+        self.check(load().rt_context_tune(self.h, shade_waves_per_simd, traversal_stack, bounce_graph))
 
     def close(self):
         if getattr(self, "h", None):
@@ -224,9 +232,7 @@ class Renderer:
         s.caustic_gather_radius = caustic_gather_radius
         self.settings = s
 
-    def render(self, width, height, capture_aovs=False):
-        s = self.settings
-        s.width, s.height, s.capture_aovs = width, height, 1 if capture_aovs else 0
+    def _film(self, width, height, capture_aovs):
         npx = width * height
         planes = {k: np.zeros(3 * npx, np.float32) for k in ("beauty", "beauty_a", "beauty_b", "display")}
         if capture_aovs:
@@ -239,11 +245,50 @@ class Renderer:
         for k, a in planes.items():
             setattr(film, k, _f(a))
         film.sample_count = _i(count)
-        stats = abi.rt_render_stats()
-        ctx = self.scene.ctx
-        ctx.check(load().rt_render(ctx.h, self.scene.h, C.byref(s), C.byref(film), C.byref(stats)))
+        return film, planes, count
+
+    def _result(self, width, height, planes, count, stats):
+        npx = width * height
         shaped = {k: (a.reshape(3, height, width) if a.size == 3 * npx else a.reshape(height, width))
                   for k, a in planes.items()}
         shaped["sample_count"] = count.reshape(height, width)
         pixels = np.transpose(shaped["display"], (1, 2, 0))
-        return RenderResult(pixels, RenderBuffers(shaped, s.samples_per_pixel), stats)
+        return RenderResult(pixels, RenderBuffers(shaped, self.settings.samples_per_pixel), stats)
+
+    def render(self, width, height, capture_aovs=False):
+        s = self.settings
+        s.width, s.height, s.capture_aovs = width, height, 1 if capture_aovs else 0
+        film, planes, count = self._film(width, height, capture_aovs)
+        stats = abi.rt_render_stats()
+        ctx = self.scene.ctx
+        ctx.check(load().rt_render(ctx.h, self.scene.h, C.byref(s), C.byref(film), C.byref(stats)))
+        return self._result(width, height, planes, count, stats)
+
+
+def render_batch(items, pool_paths=0, films=True):
+    """Renders several (Renderer, width, height[, capture_aovs]) items through
+    one path pool (rt_render_batch).  Returns (list of RenderResult or None
+    per item, total rt_render_stats)."""
+    L = load()
+    jobs = (abi.rt_batch_job * len(items))()
+    keep, outs = [], []
+    ctx = None
+    for k, it in enumerate(items):
+        r, w, h = it[0], it[1], it[2]
+        aov = bool(it[3]) if len(it) > 3 else False
+        st = abi.rt_settings()
+        C.memmove(C.byref(st), C.byref(r.settings), C.sizeof(st))
+        st.width, st.height, st.capture_aovs = w, h, 1 if aov else 0
+        jobs[k].scene = r.scene.h
+        jobs[k].settings = C.pointer(st)
+        keep.append(st)
+        if films:
+            film, planes, count = r._film(w, h, aov)
+            keep.append(film)
+            jobs[k].film = C.pointer(film)
+            outs.append((r, w, h, planes, count))
+        ctx = r.scene.ctx if ctx is None else ctx
+    total = abi.rt_render_stats()
+    ctx.check(L.rt_render_batch(ctx.h, jobs, len(items), pool_paths, C.byref(total)))
+    results = [r._result(w, h, planes, count, None) for r, w, h, planes, count in outs] if films else None
+    return results, total

@@ -442,3 +442,87 @@ def caustics_scene(width=1920, height=1080, water_cells=200, seed=0x5EA):
     cam = make_camera((0.0, 2.4, 5.2), (0.0, 0.6, 0.0), 40.0, width / height)
     sig_t, albedo = 0.05, 0.9
     return b.build(cam, medium=((sig_t * (1 - albedo),) * 3, (sig_t * albedo,) * 3))
+
+
+_BOX_TRIS = np.array([
+    [[0, 0, 0], [1, 1, 0], [1, 0, 0]], [[0, 0, 0], [0, 1, 0], [1, 1, 0]],      # z = 0
+    [[0, 0, 1], [1, 0, 1], [1, 1, 1]], [[0, 0, 1], [1, 1, 1], [0, 1, 1]],      # z = 1
+    [[0, 0, 0], [1, 0, 0], [1, 0, 1]], [[0, 0, 0], [1, 0, 1], [0, 0, 1]],      # y = 0
+    [[0, 1, 0], [0, 1, 1], [1, 1, 1]], [[0, 1, 0], [1, 1, 1], [1, 1, 0]],      # y = 1
+    [[0, 0, 0], [0, 0, 1], [0, 1, 1]], [[0, 0, 0], [0, 1, 1], [0, 1, 0]],      # x = 0
+    [[1, 0, 0], [1, 1, 0], [1, 1, 1]], [[1, 0, 0], [1, 1, 1], [1, 0, 1]],      # x = 1
+], np.float64) - 0.5
+
+
+def _random_rotations(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def batch_scene(target_tris, width, height, seed, lights):
+    """One scene of the batched workload: a displaced icosphere of the
+    largest subdivision within `target_tris` plus randomly rotated, scaled
+    and placed boxes (each box transform applied on the host) making up the
+    remaining triangle count, on a ground quad; random per-triangle Disney
+    materials; `lights` sphere lights (r 0.1, radiance U[5,20])."""
+    rng = np.random.default_rng(seed)
+    s = 3
+    while s < 8 and 20 * 4 ** (s + 1) <= target_tris:
+        s += 1
+    v, f = icosphere(s)
+    v = v * (1.0 + 0.05 * fbm(v, 6, seed))[:, None] + np.array([0.0, 1.0, 0.0])
+    tris = [v[f]]
+    nbox = max(0, (int(target_tris) - len(f)) // 12)
+    if nbox:
+        rot = _random_rotations(rng, nbox)
+        scale = rng.uniform(0.04, 0.18, nbox)
+        ang = rng.uniform(0.0, 2.0 * np.pi, nbox)
+        rad = rng.uniform(1.2, 4.0, nbox)
+        pos = np.stack([rad * np.cos(ang), rng.uniform(0.05, 1.5, nbox), rad * np.sin(ang)], 1)
+        boxes = np.einsum("nij,tkj->ntki", rot, _BOX_TRIS) * scale[:, None, None, None] + pos[:, None, None, :]
+        tris.append(boxes.reshape(-1, 3, 3))
+    tris = np.concatenate(tris).astype(np.float32)
+    n = len(tris)
+    b = SceneBuilder()
+    b.shadow_samples_override = 1
+    ground = b.disney((0.5, 0.5, 0.5), roughness=0.8)
+    b.quad((-10.0, 0.0, -10.0), (0.0, 0.0, 20.0), (20.0, 0.0, 0.0), ground)
+    mats = material_records(n)
+    mats["kind"] = abi.RT_MAT_DISNEY
+    mats["color"] = rng.uniform(0.05, 0.95, (n, 3))
+    mats["roughness"] = rng.uniform(0.05, 1.0, n)
+    mats["metallic"] = (rng.random(n) < 0.3).astype(np.float32)
+    mats["specular"] = 0.5
+    m0 = b.add_materials(mats)
+    b.triangles(tris, m0 + np.arange(n))
+    for k in range(lights):
+        ang = rng.uniform(0.0, 2.0 * np.pi)
+        pos = (rng.uniform(1.0, 2.5) * np.cos(ang), rng.uniform(2.0, 3.0), rng.uniform(1.0, 2.5) * np.sin(ang))
+        b.sphere_light(pos, 0.1, (1.0, 1.0, 1.0), float(rng.uniform(5.0, 20.0)), shadow_samples=1)
+    cam = make_camera((0.0, 1.4, 4.5), (0.0, 0.9, 0.0), 45.0, width / height)
+    return b.build(cam)
+
+
+def batch_workload(count=64, seed=0xBA7C4, min_tris=1e4, max_tris=2e6,
+                   resolutions=((256, 256), (512, 512), (1024, 1024), (1920, 1080)), spps=(4, 16, 64),
+                   depths=(4, 8, 12)):
+    """The batched workload: `count` scenes with log-uniform triangle counts
+    in [min_tris, max_tris], resolution / spp / depth drawn from the given
+    sets, 1-8 sphere lights each.  Returns a list of dicts with the scene
+    description and its render settings."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        tris = float(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris))))
+        w, h = resolutions[rng.integers(len(resolutions))]
+        spp = int(spps[rng.integers(len(spps))])
+        depth = int(depths[rng.integers(len(depths))])
+        lights = int(rng.integers(1, 9))
+        out.append({"desc": batch_scene(tris, w, h, int(rng.integers(1 << 30)), lights), "width": int(w),
+                    "height": int(h), "spp": spp, "depth": depth, "target_tris": int(tris)})
+    return out

@@ -74,6 +74,89 @@ def scenes_settings(width, height, spp, depth):
     return abi.settings_defaults(width, height, spp, depth)
 
 
+def run_batch(args):
+    """Batched workload: every scene rendered once per scene (plain: one
+    rt_render call each) and once all together (rt_render_batch, one path
+    pool); both at the scenes' full spp, so the films are identical and the
+    RMSE equal by construction.  value = plain time / batched time."""
+    import ctypes as C
+    from raytracer import abi, autotune
+    L = native.load()
+    log(f"generating {args.batch_scenes} scenes")
+    work = scenes.batch_workload(count=args.batch_scenes)
+    ctx = native.Context(0)
+    t0 = time.perf_counter()
+    items = []
+    for j in work:
+        sc = native.Scene(ctx, j["desc"])
+        items.append((native.Renderer(sc, j["spp"], j["depth"]), j["width"], j["height"]))
+    ntri = sum(int((j["desc"].prims["kind"] == 0).sum()) for j in work)
+    samples = sum(w * h * r.settings.samples_per_pixel for r, w, h in items)
+    log(f"{len(items)} scenes resident ({ntri} triangles), build+upload {time.perf_counter() - t0:.1f} s; "
+        f"{samples / 1e6:.0f}M camera samples")
+    tuned = None
+    if not args.no_tune:
+        log("autotuning the batched form")
+        tuned = autotune.tune(items, batch=True, log=log)
+        log(f"autotune: {tuned}")
+    pool = tuned["pool"] if tuned else 0
+
+    def plain():
+        st = abi.rt_render_stats()
+        for r, w, h in items:
+            s = r.settings
+            s.width, s.height = w, h
+            ctx.check(L.rt_render(ctx.h, r.scene.h, C.byref(s), None, C.byref(st)))
+
+    def batched():
+        _, total = native.render_batch(items, pool_paths=pool, films=False)
+        return total
+
+    log("warmup")
+    for k in range(args.warmup):
+        plain()
+        batched()
+    tp = tb = 0.0
+    total = None
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        plain()
+        tp += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        total = batched()
+        tb += time.perf_counter() - t0
+        log(f"step {k}: plain {tp / (k + 1) * 1e3:.0f} ms, batched {tb / (k + 1) * 1e3:.0f} ms")
+    tp /= args.steps
+    tb /= args.steps
+    per = {"trace": total.trace_ms, "shadow": total.shadow_ms, "shade": total.shade_ms}
+    out = {
+        "metric": "batched/plain speedup at equal RMSE",
+        "value": round(tp / tb, 3),
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tb * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scenes)",
+        "config": {"workload": f"{len(items)} scenes, {ntri} triangles, log-uniform 10k-2M per scene, resolutions "
+                               f"256^2..1920x1080, spp 4/16/64, depth 4/8/12, 1-8 sphere lights",
+                   "scenes": len(items), "camera_samples": samples, "triangles": ntri, "autotune": tuned},
+        "plain_msamples_per_s": round(samples / tp / 1e6, 2),
+        "batched_msamples_per_s": round(samples / tb / 1e6, 2),
+        "plain_ms": round(tp * 1e3, 1),
+        "batched_ms": round(tb * 1e3, 1),
+        "films": "identical per scene (bit-exact, tests/test_gpu_batch.py)",
+        "kernel_ms_batched": {k: round(v, 1) for k, v in per.items()},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,9 +167,12 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--subdiv", type=int, default=8)
-    ap.add_argument("--workload", choices=("icosphere", "caustics"), default="icosphere",
+    ap.add_argument("--workload", choices=("icosphere", "caustics", "batch"), default="icosphere",
                     help="icosphere: the 1.31M-triangle Disney scene (headline); caustics: glass spheres, "
-                         "Gerstner water, fog, photon-map caustics")
+                         "Gerstner water, fog, photon-map caustics; batch: 64 scenes of uneven size, batched "
+                         "vs plain per-scene renders")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     ap.add_argument("--photons", type=int, default=20_000_000)
     ap.add_argument("--gather-radius", type=float, default=0.01)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
@@ -119,6 +205,8 @@ def main():
             args.spp = 128
         if args.depth == 8:
             args.depth = 12
+    if args.workload == "batch":
+        return run_batch(args)
     W, H, spp, depth = args.width, args.height, args.spp, args.depth
     if caus:
         log("generating scene (glass spheres, 80k-triangle Gerstner water, fog)")

@@ -311,6 +311,33 @@ int rt_scene_get_info(const rt_scene* scene, rt_scene_info* info);
 int rt_render(rt_ctx* ctx, const rt_scene* scene, const rt_settings* settings,
               const rt_film* film, rt_render_stats* stats);
 
+/* Kernel configuration of a context (no reference counterpart; set by an
+ * autotuner): shade_waves_per_simd = minimum waves per SIMD the shading
+ * kernel is compiled for (1-4), traversal_stack = 1 private scratch, 2 LDS,
+ * bounce_graph = 1 direct launches, 2 replay of hipGraph-captured bounce
+ * iterations (no per-kernel timings then); 0 keeps a setting.  Results do
+ * not depend on any of them. */
+int rt_context_tune(rt_ctx* ctx, int32_t shade_waves_per_simd, int32_t traversal_stack, int32_t bounce_graph);
+
+/* ── Batched rendering (no reference counterpart: the reference renders one
+ *    scene per process run, Program.cs:474-490) ─────────────────────────── */
+typedef struct rt_batch_job {
+    const rt_scene* scene;
+    const rt_settings* settings;   /* caustics are not supported in a batch */
+    const rt_film* film;           /* NULL: the film stays on the device */
+    rt_render_stats* stats;        /* optional: per-job camera_samples */
+} rt_batch_job;
+
+/* Renders n_jobs (scene, settings) pairs through one path pool: the camera
+ * samples of all jobs stream through the same device-driven bounce loop,
+ * every path carrying its job, so small and large scenes fill the GPU
+ * together.  Each job's film is identical to rt_render of that job (same
+ * per-sample values, same summation order).  pool_paths = path slots of the
+ * pool (0 = library default).  total (optional) receives summed counts and
+ * the batch's timings. */
+int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32_t pool_paths,
+                    rt_render_stats* total);
+
 /* Parity hook: for samples [sample_begin, sample_begin+sample_count) of every
  * pixel, the camera ray of Renderer.cs:630-653 (+ the camera-visibility skip
  * of TraceRay, Renderer.cs:1609-1644) is traced and its closest hit written as
