@@ -106,9 +106,14 @@ struct ManyJobs {
 //   per window: next window sample to start, window size, paths active in
 //   the current iteration; per render: stack-overflow flag.
 constexpr int kCntNext = 0, kCntShadow = 1, kCntRetire = 2, kCntGather = 3, kCntSeg = 4, kMaxSeg = 32,
-              kCntClear = 40, kCntSid = 40, kCntWs = 41, kCntActive = 42, kCntWindow = 48, kCntOverflow = 63;
+              kCntClear = 40, kCntSid = 40, kCntWs = 41, kCntActive = 42, kCntSidSnap = 43, kCntWindow = 48,
+              kCntOverflow = 63;
 // tcount[] (64-bit): traversal tests (trace node/tri/other, shadow node/tri/other), closest-hit segments, shadow rays
 constexpr int kTcRays = 6, kTcShadowRays = 7, kTcIters = 8, kTcShadowIters = 9;
+// per-wave count arrays (wc): continuing paths, retired paths, gather
+// requests, shadow rays (ungrouped form), then one per light-sample segment
+constexpr int kWcNext = 0, kWcRetire = 1, kWcGather = 2, kWcShadow = 3, kWcSeg = 4;
+constexpr int kCntSidBase = 44;   // first window sample of this iteration's restarts
 constexpr int kIterBatch = 4;   // bounce iterations enqueued between host checks
 enum : int32_t { kSlotIdle = 0, kSlotActive = 1, kSlotRetired = 2 };
 
@@ -128,6 +133,10 @@ struct PathBufs {
     float4 *av0, *av1, *av2, *av3, *av4;   // AOV probe
     float4 *sh_o, *sh_d, *sh_c, *sh_r;     // shadow records, record k of slot p at k*N + p (N × maxs)
     int32_t* state;             // per slot: kSlotIdle, kSlotActive (to trace), kSlotRetired
+    int32_t* gflag;             // per slot: a caustic gather request was queued this iteration
+    int32_t* q_g;               // gather queue: slots with a request
+    int32_t* wc;                // per-wave counts, kWc* arrays of nw = N/64 entries, scanned in place
+    int nw;
     int32_t* smask;             // per slot: light samples with a pending shadow ray (grouped form)
     int32_t* q_sh;              // shadow queue: shadow record indices
     int32_t* sid;               // sample of the window each path slot is tracing
@@ -193,21 +202,6 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 }
 __device__ __forceinline__ uint32_t xcd_block() { return xcd_remap(blockIdx.x, gridDim.x); }
 
-// Wave-aggregated queue allocation: returns the first slot for this lane's
-// `n` items (one atomic per wave).
-__device__ __forceinline__ int wave_alloc(int32_t* counter, int n) {
-    int lane = __lane_id();
-    int incl = n;
-    for (int off = 1; off < 64; off <<= 1) {
-        int v = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += v;
-    }
-    int total = __shfl(incl, 63, 64);
-    int base = 0;
-    if (lane == 63 && total > 0) base = atomicAdd(counter, total);
-    base = __shfl(base, 63, 64);
-    return base + incl - n;
-}
 
 // ── surface reconstruction from an entry (primitive Hit bodies) ─────────
 __device__ __forceinline__ Surf make_surf(const DevScene& S, int32_t eidx, f3 o, f3 d, float t, float u, float v) {
@@ -659,16 +653,22 @@ __global__ void __launch_bounds__(kBlock) k_raygen(JOBS jobs, PathBufs P, int n,
     if (p == 0) {
         P.counters[kCntNext] = n;
         P.counters[kCntSid] = n;
+        P.counters[kCntSidSnap] = n;
         P.counters[kCntWs] = ws;
     }
 }
 
-// Start of one bounce iteration: snapshot the active-path count, add the
-// previous iteration's traced and shadow segments to the statistics, clear
-// the per-iteration counters.  One wave.
+// Start of one bounce iteration: count the paths pending in it (paths that
+// continued from the previous iteration + slots the previous k_retire
+// restarted, derived from the window's sample counter), add the previous
+// iteration's traced and shadow segments to the statistics, clear the
+// per-iteration counters.  One wave.
 __global__ void __launch_bounds__(kBlock) k_iter_begin(PathBufs P, int traced) {
     if (threadIdx.x == 0) {
-        int active = P.counters[kCntNext];
+        const int sid = P.counters[kCntSid], snap = P.counters[kCntSidSnap], ws = P.counters[kCntWs];
+        const int started = min(sid - snap, max(0, ws - snap));
+        int active = P.counters[kCntNext] + started;
+        P.counters[kCntSidSnap] = sid;
         long long sh = P.counters[kCntShadow];
         for (int j = 0; j < kMaxSeg; j++) sh += P.counters[kCntSeg + j];
         P.counters[kCntActive] = active;
@@ -735,6 +735,7 @@ __device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, i
     f3 s = splat(0.0f);
     if (R.maxs <= kMaxSeg) {
         // record of light sample j at j*N + p, summed in light-sample order
+        // (the slot's mask of light samples with a shadow ray)
         for (uint32_t m = (uint32_t)P.smask[p]; m != 0u; m &= m - 1u)
             s += f3of(P.sh_r[(int64_t)(__ffs(m) - 1) * P.n + p]);
     } else {
@@ -747,13 +748,15 @@ __device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, i
 template <class JOBS, int WPE>
 __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, int maxs) {
     const int p = xcd_block() * kBlock + threadIdx.x;
-    const bool active = p < P.n && P.state[p] == kSlotActive;
+    const int st_in = p < P.n ? P.state[p] : kSlotIdle;
+    const bool active = st_in == kSlotActive;
     const int job = active ? jobs.job_of(P.job, p) : 0;
     const DevScene& S = jobs.scene(job);
     const RParams& R = jobs.params(job);
     int n_shadow = 0;
     uint32_t smask = 0;   // bit j: light sample j produced a shadow ray
     bool cont = false;
+    bool gather = false;  // a caustic gather request was queued
     // per-lane shadow records are written straight into the path's slots
     if (active) {
         float4 o4 = P.o[p], d4 = P.d[p], thr4 = P.thr[p], wt4 = P.wt[p];
@@ -1021,11 +1024,11 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             }
             // ── caustic gather request (Renderer.cs:1108-1109) ────────────
             if (S.caustics && (kind == RT_MAT_LAMBERTIAN || kind == RT_MAT_DISNEY)) {
-                int gi = atomicAdd(&P.counters[kCntGather], 1);
-                P.gq0[gi] = make_float4(o.x, o.y, o.z, htuv.x);
-                P.gq1[gi] = make_float4(d.x, d.y, d.z, __int_as_float(he));
-                P.gq2[gi] = make_float4(htuv.y, htuv.z, s.rel_eta, __int_as_float(p));
-                P.gq3[gi] = make_float4(vw.x, vw.y, vw.z, __int_as_float(tgt ? 1 : 0));
+                gather = true;
+                P.gq0[p] = make_float4(o.x, o.y, o.z, htuv.x);
+                P.gq1[p] = make_float4(d.x, d.y, d.z, __int_as_float(he));
+                P.gq2[p] = make_float4(htuv.y, htuv.z, s.rel_eta, __int_as_float(p));
+                P.gq3[p] = make_float4(vw.x, vw.y, vw.z, __int_as_float(tgt ? 1 : 0));
             }
             // ── indirect bounce ─────────────────────────────────────────
             bool have = false, legacy = false, is_delta = false;
@@ -1169,7 +1172,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         // pending shadows of this vertex belong to this vertex's target
         flags = flags & ~(0xff << kPfPendShift) & ~kPfPendInd;
         flags |= (n_shadow << kPfPendShift) | (tgt ? kPfPendInd : 0);
-        if (grouped) P.smask[p] = (int32_t)smask;
+        (void)grouped;
         // TraceRay entry checks of the next segment (Renderer.cs:1589-1603)
         if (cont) {
             if (depth - 1 <= 0 || fmax3(thr_next) < 1e-4f) cont = false;
@@ -1191,23 +1194,117 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             P.av1[p] = av1;
         }
     }
-    // slot state and counts; shadow queue (wave-aggregated)
+    // slot state, per-slot outputs and this wave's counts (no atomics: the
+    // queues are laid out by k_wave_scan + k_compact from the counts)
     bool retire = active && !cont;
-    if (retire) P.state[p] = kSlotRetired;
-    wave_alloc(&P.counters[kCntNext], cont ? 1 : 0);
-    wave_alloc(&P.counters[kCntRetire], retire ? 1 : 0);
-    if (maxs <= kMaxSeg) {
-        // shadow rays grouped by light sample: neighbouring paths' rays
-        // towards the same light share a wave (coherent traversal)
+    if (p < P.n) {
+        if (retire) P.state[p] = kSlotRetired;
+        P.smask[p] = maxs <= kMaxSeg ? (int32_t)smask : n_shadow;
+        P.gflag[p] = gather ? 1 : 0;
+    }
+    const int w = p >> 6;
+    // retired this iteration, plus slots restarted as already-retired
+    // (samples of depth-0 jobs), which k_retire finishes as well
+    const int n_cont = __popcll(__ballot(cont)), n_ret = __popcll(__ballot(retire || st_in == kSlotRetired));
+    const int n_g = __popcll(__ballot(gather));
+    int n_sh = n_shadow;
+    for (int off = 32; off > 0; off >>= 1) n_sh += __shfl_xor(n_sh, off, 64);
+    if (threadIdx.x == 0) {
+        P.wc[kWcNext * P.nw + w] = n_cont;
+        P.wc[kWcRetire * P.nw + w] = n_ret;
+        P.wc[kWcGather * P.nw + w] = n_g;
+        P.wc[kWcShadow * P.nw + w] = maxs <= kMaxSeg ? 0 : n_sh;
+    }
+    if (maxs <= kMaxSeg)
         for (int j = 0; j < maxs; j++) {
-            bool has = (smask >> j) & 1u;
-            int pos = wave_alloc(&P.counters[kCntSeg + j], has ? 1 : 0);
-            if (has) P.q_sh[(int64_t)j * P.n + pos] = (int32_t)((int64_t)j * P.n + p);
+            int c = __popcll(__ballot((smask >> j) & 1u));
+            if (threadIdx.x == 0) P.wc[(kWcSeg + j) * P.nw + w] = c;
+        }
+}
+
+// Per-wave counts when no path is traced (depth-0 renders): every pending
+// slot is retired; the other count arrays are zero.
+__global__ void __launch_bounds__(kBlock) k_count_retired(PathBufs P, int narr) {
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    const bool ret = p < P.n && P.state[p] == kSlotRetired;
+    const int n = __popcll(__ballot(ret));
+    if (threadIdx.x == 0)
+        for (int a = 0; a < narr; a++) P.wc[a * P.nw + (p >> 6)] = a == kWcRetire ? n : 0;
+}
+
+// Exclusive scan of one per-wave count array (block a scans array a) and
+// its total into the counters; the retire array also advances the window's
+// sample counter (the restarted slots take samples [sid, sid + total)).
+__global__ void __launch_bounds__(1024) k_wave_scan(PathBufs P) {
+    __shared__ int32_t part[1024 / 64];
+    __shared__ int32_t blocksum;
+    const int a = blockIdx.x;
+    int32_t* x = P.wc + (int64_t)a * P.nw;
+    const int nw = P.nw;
+    const int per = (nw + 1023) / 1024;
+    const int b0 = threadIdx.x * per, b1 = min(nw, b0 + per);
+    int t = 0;
+    for (int i = b0; i < b1; i++) t += x[i];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl = t;
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) part[wid] = incl;
+    __syncthreads();
+    int wpre = 0;
+    for (int k = 0; k < wid; k++) wpre += part[k];
+    if (threadIdx.x == 1023) blocksum = wpre + incl;
+    int run = wpre + incl - t;
+    for (int i = b0; i < b1; i++) {
+        int v = x[i];
+        x[i] = run;
+        run += v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int total = blocksum;
+        const int slot = a == kWcNext ? kCntNext : a == kWcRetire ? kCntRetire : a == kWcGather ? kCntGather
+                       : a == kWcShadow ? kCntShadow : kCntSeg + (a - kWcSeg);
+        P.counters[slot] = total;
+        if (a == kWcRetire) {
+            P.counters[kCntSidBase] = P.counters[kCntSid];
+            P.counters[kCntSid] += total;
+        }
+    }
+}
+
+// Lays out the shadow queue (grouped by light sample, slot order within a
+// segment) and the gather queue from the scanned per-wave counts.
+__global__ void __launch_bounds__(kBlock) k_compact(PathBufs P, int maxs) {
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    const bool ok = p < P.n;
+    const int w = p >> 6;
+    const unsigned long long lt = (1ull << (threadIdx.x & 63)) - 1ull;
+    const int st = ok ? P.state[p] : kSlotIdle;
+    const bool shaded = st != kSlotIdle;
+    const int m = shaded ? P.smask[p] : 0;
+    if (maxs <= kMaxSeg) {
+        for (int j = 0; j < maxs; j++) {
+            bool has = (m >> j) & 1;
+            unsigned long long b = __ballot(has);
+            if (has)
+                P.q_sh[(int64_t)j * P.n + P.wc[(kWcSeg + j) * P.nw + w] + __popcll(b & lt)] =
+                    (int32_t)((int64_t)j * P.n + p);
         }
     } else {
-        int sb = wave_alloc(&P.counters[kCntShadow], n_shadow);
-        for (int k = 0; k < n_shadow; k++) P.q_sh[sb + k] = (int32_t)((int64_t)k * P.n + p);
+        int incl = m;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(incl, off, 64);
+            if ((int)(threadIdx.x & 63) >= off) incl += y;
+        }
+        int base = P.wc[kWcShadow * P.nw + w] + incl - m;
+        for (int k = 0; k < m; k++) P.q_sh[base + k] = (int32_t)((int64_t)k * P.n + p);
     }
+    const bool g = shaded && P.gflag[p];
+    unsigned long long gb = __ballot(g);
+    if (g) P.q_g[P.wc[kWcGather * P.nw + w] + __popcll(gb & lt)] = p;
 }
 
 }  // namespace rt
@@ -1225,7 +1322,8 @@ __global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPh
     if (blockIdx.x >= nbk) return;
     int i = xcd_remap(blockIdx.x, nbk) * kBlock + threadIdx.x;
     if (i >= count) return;
-    float4 a = P.gq0[i], b = P.gq1[i], c = P.gq2[i], w4 = P.gq3[i];
+    const int slot = P.q_g[i];
+    float4 a = P.gq0[slot], b = P.gq1[slot], c = P.gq2[slot], w4 = P.gq3[slot];
     f3 o = f3of(a), d = f3of(b);
     Surf s = make_surf(S, __float_as_int(b.w), o, d, a.w, c.x, c.y);
     s.rel_eta = c.z;
@@ -1389,7 +1487,9 @@ __global__ void __launch_bounds__(kBlock) k_retire(JOBS jobs, PathBufs P) {
     const bool ret = p < P.n && P.state[p] == kSlotRetired;
     if (ret) finish_path(jobs.params(jobs.job_of(P.job, p)), P, p);
     const int ws = P.counters[kCntWs];
-    int sid = wave_alloc(&P.counters[kCntSid], ret ? 1 : 0);
+    const unsigned long long lt = (1ull << (threadIdx.x & 63)) - 1ull;
+    const unsigned long long rb = __ballot(ret);
+    const int sid = P.counters[kCntSidBase] + P.wc[kWcRetire * P.nw + (p >> 6)] + __popcll(rb & lt);
     bool start = ret && sid < ws;
     if (ret) {
         int st = kSlotIdle;
@@ -1402,7 +1502,6 @@ __global__ void __launch_bounds__(kBlock) k_retire(JOBS jobs, PathBufs P) {
         }
         P.state[p] = st;
     }
-    wave_alloc(&P.counters[kCntNext], start ? 1 : 0);
 }
 
 // Per-pixel film accumulation in sample order (Renderer.cs:668-672,
@@ -1685,7 +1784,6 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
     size_t pbytes = sizeof(PartDev) * std::max<size_t>(plan.parts.size(), 1);
     if (pbytes > ctx->parts_bytes) {
         if (ctx->parts_mem) hipFree(ctx->parts_mem);
-    if (ctx->jobs_mem) hipFree(ctx->jobs_mem);
         ctx->parts_mem = nullptr;
         ctx->parts_bytes = 0;
         HIP_TRY(hipMalloc(&ctx->parts_mem, pbytes));
@@ -1708,11 +1806,17 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
                 default: hipLaunchKernelGGL((k_shade<JOBS, 1>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
             }
             if (e) HIP_TRY(hipEventRecord(e[2], ctx->stream));
+            hipLaunchKernelGGL(k_wave_scan, dim3(kWcSeg + (maxs <= kMaxSeg ? maxs : 0)), dim3(1024), 0, ctx->stream, Pw);
+            hipLaunchKernelGGL(k_compact, grid, dim3(kBlock), 0, ctx->stream, Pw, maxs);
             if (gather_scene) hipLaunchKernelGGL(k_gather, grid, dim3(kBlock), 0, ctx->stream, *gather_scene, Pw, *M);
             if (e) HIP_TRY(hipEventRecord(e[3], ctx->stream));
             DISPATCH_SD_C(JOBS, stack_depth, counting, ctx->priv_stack, k_shadow, dim3(shadow_blocks), ctx->stream, jobs,
                           Pw, maxs);
             if (e) HIP_TRY(hipEventRecord(e[4], ctx->stream));
+        } else {
+            const int narr = kWcSeg + (maxs <= kMaxSeg ? maxs : 0);
+            hipLaunchKernelGGL(k_count_retired, dim3(pool_blocks), dim3(kBlock), 0, ctx->stream, Pw, narr);
+            hipLaunchKernelGGL(k_wave_scan, dim3(narr), dim3(1024), 0, ctx->stream, Pw);
         }
         hipLaunchKernelGGL((k_retire<JOBS>), dim3(pool_blocks), dim3(kBlock), 0, ctx->stream, jobs, Pw);
         HIP_TRY(hipGetLastError());
@@ -1771,7 +1875,8 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
                     if (rc) return rc;
                 }
             }
-            HIP_TRY(hipMemcpyAsync(&ctx->pinned[2 * set], &P.counters[kCntNext], 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(&ctx->pinned[2 * set], &P.counters[kCntActive], 4, hipMemcpyDeviceToHost,
+                                   ctx->stream));
             HIP_TRY(hipMemcpyAsync(&ctx->pinned[2 * set + 1], &P.counters[kCntOverflow], 4, hipMemcpyDeviceToHost,
                                    ctx->stream));
             HIP_TRY(hipEventRecord(ctx->chk[set], ctx->stream));
@@ -2251,7 +2356,8 @@ static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, 
     size_t per_path = 8 * 16 + 8 * 4 + 4 + 16 + (aovs ? 5 * 16 : 0) + 4 * 16 * (size_t)maxs + 8 * 4 + 4 * (size_t)maxs +
                       (gather ? 64 : 0);
     size_t per_sample = aovs ? 48 : 16;
-    size_t need = (size_t)N * per_path + (size_t)WS * per_sample + 40 * 256 + 16384;
+    size_t need = (size_t)N * per_path + (size_t)WS * per_sample + 40 * 256 + 16384 + 8 * N +
+                  4 * (size_t)((N + kBlock - 1) / kBlock) * (kWcSeg + kMaxSeg);
     if (need > ctx->path_bytes) {
         if (ctx->path_mem) hipFree(ctx->path_mem);
         ctx->path_mem = nullptr;
@@ -2294,6 +2400,10 @@ static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, 
     P.q_sh = (int32_t*)take(4 * N * maxs);
     P.state = (int32_t*)take(4 * N);
     P.smask = (int32_t*)take(4 * N);
+    P.gflag = (int32_t*)take(4 * N);
+    P.q_g = (int32_t*)take(4 * N);
+    P.nw = (int)((N + kBlock - 1) / kBlock);
+    P.wc = (int32_t*)take(4 * (size_t)P.nw * (kWcSeg + std::min(maxs, kMaxSeg)));
     P.job = (int32_t*)take(4 * N);
     P.pix = (int32_t*)take(4 * N);
     P.pass = (int32_t*)take(4 * N);

@@ -24,7 +24,7 @@ import numpy as np
 from . import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib3dray_amd.so")
+LIB_PATH = os.environ.get("RT3D_LIB") or os.path.join(os.path.dirname(_HERE), "lib3dray_amd.so")
 _lib = None
 
 

@@ -33,6 +33,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 counters
+    (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated), from the committed
+    summary profiles/hbm_traffic_<workload>.txt; None when not measured."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == kernel and parts[1] == "traffic_bytes_per_launch":
+                return float(parts[2])
+    return None
+
+
 def traversal_bytes(node, tri, sph, rays):
     # algorithmic bytes of a closest-hit segment: 64 B per BVH node visit,
     # 64 B per triangle/quad/plane record, 16 B per sphere, 32 B ray read and
@@ -277,8 +292,9 @@ def main():
         t_avg_s = per[dom] / args.steps / max(launches, 1) / 1000.0
         bytes_per_launch = b / max(launches, 1)
         achieved = bytes_per_launch / t_avg_s / 1e9
+        traffic = measured_traffic(f"k_{dom}", "caustics" if caus else "icosphere1310k_1080p64")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_{dom}",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_{dom}",
                 "bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(t_avg_s * 1000.0, 4),
                 "launches_per_step": launches}
 
