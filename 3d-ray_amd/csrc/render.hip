@@ -32,6 +32,7 @@
 
 #include "rt_scene.h"
 #include "shade.h"
+#include "adjoint.h"
 
 namespace rt {
 
@@ -56,6 +57,11 @@ struct DevScene {
     f3 cam_o, cam_llc, cam_h, cam_v, cam_u, cam_vv;
     float lens_radius;
     f3 sig_t, sig_s;
+    // texture set of textured Disney materials (kMatTextured)
+    const rt_material* raw;     // material records as given (knobs the texture does not replace)
+    const float4* tuv;          // 2 per primitive or null (HitRecord U/V)
+    const float* tex;           // planar [kTexChannels][tex_h][tex_w]
+    int tex_w, tex_h;
 };
 
 struct RParams {
@@ -109,7 +115,7 @@ constexpr int kCntNext = 0, kCntShadow = 1, kCntRetire = 2, kCntGather = 3, kCnt
               kCntClear = 40, kCntSid = 40, kCntWs = 41, kCntActive = 42, kCntSidSnap = 43, kCntWindow = 48,
               kCntOverflow = 63;
 // tcount[] (64-bit): traversal tests (trace node/tri/other, shadow node/tri/other), closest-hit segments, shadow rays
-constexpr int kTcRays = 6, kTcShadowRays = 7, kTcIters = 8, kTcShadowIters = 9;
+constexpr int kTcRays = 6, kTcShadowRays = 7;
 // per-wave count arrays (wc): continuing paths, retired paths, gather
 // requests, shadow rays (ungrouped form), then one per light-sample segment
 constexpr int kWcNext = 0, kWcRetire = 1, kWcGather = 2, kWcShadow = 3, kWcSeg = 4;
@@ -148,6 +154,14 @@ struct PathBufs {
     int32_t* counters;          // kCnt* slots below
     unsigned long long* tcount; // traversal counters: trace node/tri/other, shadow node/tri/other
     int n;
+    // path-replay adjoint: adj_pass 1 = primal (res_l[sample] = the sample's
+    // radiance sum a0 + ai), 2 = replay (texture gradients); 0 = plain render
+    int adj_pass;
+    float adj_inv_spp;
+    float4* adj_q;              // 5 per slot: pending bounce gradient (15 coefficients), (u, v, flag, -)
+    float4* adj_g;              // 2 per shadow record: NEE gradient (5), (u, v, flag)
+    const float* adj_db;        // dLoss/dBeauty, planar [3][npix]
+    float* adj_dt;              // texture gradient, planar [kTexChannels][tex_h][tex_w]
 };
 
 __device__ __forceinline__ void flush_counts(unsigned long long* dst, const TravCount& tc) {
@@ -243,6 +257,33 @@ __device__ __forceinline__ Surf make_surf(const DevScene& S, int32_t eidx, f3 o,
     (void)u;
     (void)v;
     return s;
+}
+
+// Texture coordinate of a hit: the primitive's interpolated vertex UVs when
+// the scene has them (SmoothTriangle.cs:168,186), else HitRecord.U/V.
+__device__ __forceinline__ float2 tex_coord(const DevScene& S, int32_t he, f3 o, f3 d, float4 htuv) {
+    const float4 ea = S.entries[he].a;
+    if (S.tuv && (tag_of(ea) & 0xff) == kTri) {
+        const int prim = __float_as_int(S.entries[he].b.w);
+        const float4 a = S.tuv[2 * prim], b = S.tuv[2 * prim + 1];
+        const float w1 = htuv.y, w2 = htuv.z, w0 = 1.0f - w1 - w2;
+        return make_float2((w0 * a.x + w1 * a.z) + w2 * b.x, (w0 * a.y + w1 * a.w) + w2 * b.y);
+    }
+    HitOut h{he, htuv.x, htuv.y, htuv.z};
+    float u, v;
+    report_uv(S.entries, h, o, d, u, v);
+    return make_float2(u, v);
+}
+
+// Material record of a textured Disney hit: the material's knobs with base
+// colour, roughness and metallic from the texture set.
+__device__ __forceinline__ GMat textured_gmat(const DevScene& S, const GMat& g, int mat, float2 uv, float tp[kTexChannels]) {
+    const TexFoot foot = tex_footprint(S.tex_w, S.tex_h, uv.x, uv.y);
+#pragma unroll
+    for (int c = 0; c < kTexChannels; c++) tp[c] = tex_value(S.tex, S.tex_w, S.tex_h, foot, c);
+    GMat t = disney_gmat(S.raw[mat], mk(tp[0], tp[1], tp[2]), tp[4], tp[3], S.lut_e + 32 * 32);
+    t.m0.z = g.m0.z;
+    return t;
 }
 
 // ── lights (SphereLight/AreaLight/DirectionalLight/GeometryLight) ───────
@@ -672,14 +713,8 @@ __global__ void __launch_bounds__(kBlock) k_iter_begin(PathBufs P, int traced) {
         long long sh = P.counters[kCntShadow];
         for (int j = 0; j < kMaxSeg; j++) sh += P.counters[kCntSeg + j];
         P.counters[kCntActive] = active;
-        if (traced && active > 0) {
-            P.tcount[kTcRays] += (unsigned long long)active;
-            P.tcount[kTcIters] += 1ull;
-        }
-        if (sh > 0) {
-            P.tcount[kTcShadowRays] += (unsigned long long)sh;
-            P.tcount[kTcShadowIters] += 1ull;
-        }
+        if (traced && active > 0) P.tcount[kTcRays] += (unsigned long long)active;
+        if (sh > 0) P.tcount[kTcShadowRays] += (unsigned long long)sh;
     }
     __syncthreads();
     for (int k = threadIdx.x; k < kCntClear; k += kBlock) P.counters[k] = 0;
@@ -745,7 +780,14 @@ __device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, i
     return flags & ~(0xff << kPfPendShift) & ~kPfPendInd;
 }
 
-template <class JOBS, int WPE>
+// ADJ: the replay pass of the path-replay adjoint (rt_adjoint) — the same
+// shading plus, at textured Disney hits, the texture-gradient terms: NEE
+// records carry dBrdf/dp contracted with the pixel's adjoint weight (added by
+// k_adj_nee when the shadow ray is unoccluded), and a sampled bounce leaves
+// dF/dp / F (per channel) pending until the next shade, where the radiance
+// gathered after this vertex, D = (primal a0 + ai) - (a0 + ai so far), is
+// known (path replay backpropagation, Vicini, Speierer and Jakob 2021).
+template <class JOBS, int WPE, bool ADJ = false>
 __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, int maxs) {
     const int p = xcd_block() * kBlock + threadIdx.x;
     const int st_in = p < P.n ? P.state[p] : kSlotIdle;
@@ -762,6 +804,31 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         float4 o4 = P.o[p], d4 = P.d[p], thr4 = P.thr[p], wt4 = P.wt[p];
         int32_t flags = __float_as_int(d4.w);
         flags = fold_shadows(P, R, p, flags);
+        bool adj_pend = false;   // this vertex leaves a bounce gradient pending
+        float adj_qv[16];
+        float2 adj_uv = make_float2(0.0f, 0.0f);
+        if constexpr (ADJ) {
+            const float4 qf = P.adj_q[5 * (int64_t)p + 4];
+            if (qf.z != 0.0f) {
+                // the previous vertex's bounce: everything gathered after it
+                const float4 T = P.res_l[P.sid[p]];
+                const float4 a0 = P.a0[p], ai = P.ai[p];
+                const float D[3] = {T.x - (a0.x + ai.x), T.y - (a0.y + ai.y), T.z - (a0.z + ai.z)};
+                float q[16];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    float4 v = P.adj_q[5 * (int64_t)p + k];
+                    q[4 * k] = v.x;
+                    q[4 * k + 1] = v.y;
+                    q[4 * k + 2] = v.z;
+                    q[4 * k + 3] = v.w;
+                }
+                float gsum[kNd];
+#pragma unroll
+                for (int j = 0; j < kNd; j++) gsum[j] = (D[0] * q[j] + D[1] * q[kNd + j]) + D[2] * q[2 * kNd + j];
+                tex_scatter(P.adj_dt, S.tex_w, S.tex_h, qf.x, qf.y, gsum);
+            }
+        }
         int depth = __float_as_int(thr4.w);
         const int pix = P.pix[p];
         Rng rng{(uint32_t)P.pass[p], pixel_seed(pix % R.W, pix / R.W, R.seed_xor), __float_as_uint(wt4.w)};
@@ -928,7 +995,14 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         } else {
             // ── surface (ShadeSurface, Renderer.cs:1008-1228) ──────────
             Surf s = make_surf(S, he, o, d, htuv.x, htuv.y, htuv.z);
-            const GMat g = S.mats[s.mat];
+            GMat g = S.mats[s.mat];
+            float tp[kTexChannels];
+            float2 tuv = make_float2(0.0f, 0.0f);
+            const bool textured = (g.m0.y & kMatTextured) != 0;
+            if (textured) {
+                tuv = tex_coord(S, he, o, d, htuv);
+                g = textured_gmat(S, g, s.mat, tuv, tp);
+            }
             int kind = g.m0.x;
             int mflags = g.m0.y;
             f3 thr_s = thr * mb;
@@ -961,6 +1035,19 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             }
             DisneyP DP;
             if (kind == RT_MAT_DISNEY) DP = load_disney(g);
+            // adjoint weight of this vertex's terms: dLoss/dPixel * w0 / spp
+            DisneyD DD;
+            f3 adj_w = splat(0.0f);
+            const bool adj_on = ADJ && P.adj_pass == 2 && textured && kind == RT_MAT_DISNEY;
+            if constexpr (ADJ) {
+                if (adj_on) {
+                    DD = disney_d(S.raw[s.mat], tp, S.lut_e + 32 * 32);
+                    const int pix = P.pix[p];
+                    const f3 w0v = primary ? seg : f3of(P.w0[p]);
+                    adj_w = mk(P.adj_db[pix], P.adj_db[R.npix + pix], P.adj_db[2 * R.npix + pix]) * w0v * P.adj_inv_spp;
+                    adj_uv = tuv;
+                }
+            }
             // ── NEE (ComputeDirectLighting, Renderer.cs:1894-2036) ──────
             if (kind != RT_MAT_EMISSIVE) {
                 float view_term = shadow_terminator(s.n, s.n, view);
@@ -1017,6 +1104,24 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                         P.sh_o[sl] = make_float4(ls.so.x, ls.so.y, ls.so.z, ls.tmax);
                         P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
                         P.sh_c[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+                        if constexpr (ADJ) {
+                            float4 g0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g1 = g0;
+                            if (adj_on) {
+                                // c = vw * (wn Le tr / ppick) * st * brdf(p): dc/dp = k * dbrdf/dp
+                                const D3 db = disney_eval_direct_d(DP, DD, S.lut_e, ls.dir, view, s);
+                                const float st = view_term * shadow_terminator(s.n, s.n, ls.dir);
+                                f3 k = wn * ls.color * tr;
+                                if (ppick > 0.0f) k = k / ppick;
+                                k = vw * k * adj_w * st;
+                                float gj[kNd];
+#pragma unroll
+                                for (int j = 0; j < kNd; j++) gj[j] = (k.x * db.x.d[j] + k.y * db.y.d[j]) + k.z * db.z.d[j];
+                                g0 = make_float4(gj[0], gj[1], gj[2], gj[3]);
+                                g1 = make_float4(gj[4], tuv.x, tuv.y, 1.0f);
+                            }
+                            P.adj_g[2 * sl] = g0;
+                            P.adj_g[2 * sl + 1] = g1;
+                        }
                         if (jn < kMaxSeg) smask |= 1u << jn;
                         n_shadow++;
                     }
@@ -1032,6 +1137,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             }
             // ── indirect bounce ─────────────────────────────────────────
             bool have = false, legacy = false, is_delta = false;
+            int legacy_lobe = -1;   // Disney lobe of a fallback (Scatter) bounce
             f3 att = splat(0.0f), wo = splat(0.0f), origin = s.p;
             float spdf = 0.0f;
             BsdfS bs;
@@ -1059,8 +1165,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             } else if (kind == RT_MAT_DISNEY) {
                 if (disney_sample(DP, S.lut_e, view, s, rng, bs)) have = true;
                 else {
-                    int lobe;
-                    if (disney_scatter(DP, S.lut_e, s, d, rng, att, wo, lobe)) legacy = true;
+                    if (disney_scatter(DP, S.lut_e, s, d, rng, att, wo, legacy_lobe)) legacy = true;
                 }
             } else if (kind == RT_MAT_DIELECTRIC) {
                 dielectric_scatter(g, s, d, rng, att, wo, origin);
@@ -1120,6 +1225,16 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                         if (primary) next_tgt = true;
                         next_cs = next_cst(cst, bs.delta);
                         cont = true;
+                        if constexpr (ADJ) {
+                            if (adj_on) {
+                                // non-delta: att = F(wo) |cos| / pdf / q, so d(att)/dp / att =
+                                // dF/dp / F; delta: the lobe weight over its fixed probability
+                                const D3 fw = bs.delta ? lobe_att_d(bs.lobe, DP, DD, S.lut_e, view, bs.wo, s)
+                                                       : disney_eval_d(DP, DD, S.lut_e, view, bs.wo, s);
+                                log_deriv(fw, adj_w, adj_qv);
+                                adj_pend = true;
+                            }
+                        }
                     }
                 }
             } else if (legacy) {
@@ -1166,6 +1281,13 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                     if (primary) next_tgt = true;
                     next_cs = next_cst(cst, is_delta);
                     cont = true;
+                    if constexpr (ADJ) {
+                        if (adj_on && legacy_lobe >= 0) {
+                            const f3 V = normalize(-d);
+                            log_deriv(lobe_att_d(legacy_lobe, DP, DD, S.lut_e, V, wo, s), adj_w, adj_qv);
+                            adj_pend = true;
+                        }
+                    }
                 }
             }
         }
@@ -1192,6 +1314,17 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         if (R.aovs) {
             P.av0[p] = make_float4(av0.x, av0.y, av0.z, __int_as_float(aovf));
             P.av1[p] = av1;
+        }
+        if constexpr (ADJ) {
+            if (adj_pend && cont) {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    P.adj_q[5 * (int64_t)p + k] =
+                        make_float4(adj_qv[4 * k], adj_qv[4 * k + 1], adj_qv[4 * k + 2], adj_qv[4 * k + 3]);
+                P.adj_q[5 * (int64_t)p + 4] = make_float4(adj_uv.x, adj_uv.y, 1.0f, 0.0f);
+            } else {
+                P.adj_q[5 * (int64_t)p + 4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
         }
     }
     // slot state, per-slot outputs and this wave's counts (no atomics: the
@@ -1448,6 +1581,38 @@ __global__ void __launch_bounds__(kBlock) k_shadow(JOBS jobs, PathBufs P, int ma
     if (COUNT) flush_counts(P.tcount + 3, tc);
 }
 
+// Replay pass of the adjoint: the NEE gradient of every unoccluded shadow
+// record of a textured hit goes to the texture (opaque shadows only, so a
+// record's result is its full contribution or zero).  Same queue view as
+// k_shadow.
+__global__ void __launch_bounds__(kBlock) k_adj_nee(PathBufs P, int maxs, int tex_w, int tex_h) {
+    const bool grouped = maxs <= kMaxSeg;
+    int total = grouped ? 0 : P.counters[kCntShadow];
+    if (grouped)
+        for (int j = 0; j < maxs; j++) total += P.counters[kCntSeg + j];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    int64_t qi = i;
+    if (grouped) {
+        int j = 0, pre = 0;
+        for (;;) {
+            int c = P.counters[kCntSeg + j];
+            if (i < pre + c || j + 1 >= maxs) break;
+            pre += c;
+            j++;
+        }
+        qi = (int64_t)j * P.n + (i - pre);
+    }
+    const int64_t sl = P.q_sh[qi];
+    const float4 g1 = P.adj_g[2 * sl + 1];
+    if (g1.w == 0.0f) return;
+    const float4 r = P.sh_r[sl];
+    if (r.x == 0.0f && r.y == 0.0f && r.z == 0.0f) return;
+    const float4 g0 = P.adj_g[2 * sl];
+    const float g[kNd] = {g0.x, g0.y, g0.z, g0.w, g1.x};
+    tex_scatter(P.adj_dt, tex_w, tex_h, g1.y, g1.z, g);
+}
+
 // Ends the path in slot p: folds its last shadow results and stores the
 // camera sample's radiance L = w0 * (Le0 + Ld0 + clampInd(Lind)), clamped
 // (Renderer.cs:658-666, 1222-1223), and its AOV probe, under its sample id.
@@ -1455,6 +1620,14 @@ __device__ __forceinline__ void finish_path(const RParams& R, PathBufs& P, int p
     int32_t flags = __float_as_int(P.d[p].w);
     fold_shadows(P, R, p, flags);
     f3 a0 = f3of(P.a0[p]), ai = f3of(P.ai[p]), w0 = f3of(P.w0[p]);
+    if (P.adj_pass) {
+        // adjoint primal: the unweighted radiance sum the replay subtracts from
+        if (P.adj_pass == 1) {
+            f3 t = a0 + ai;
+            P.res_l[P.sid[p]] = make_float4(t.x, t.y, t.z, 0.0f);
+        }
+        return;
+    }
     f3 L = w0 * (sanitize(a0) + clamp_lum(ai, R.ind_max_rad, true));
     L = clamp_lum(L, R.max_rad, false);
     int sid = P.sid[p];
@@ -1651,6 +1824,8 @@ struct rt_ctx {
     void* jobs_mem = nullptr;                 // batch job table (grow-only)
     size_t jobs_bytes = 0;
     bool use_graph = false;                   // replay captured bounce iterations (hipGraph)
+    void* adj_mem = nullptr;                  // adjoint buffers (grow-only)
+    size_t adj_bytes = 0;
 };
 
 struct rt_scene {
@@ -1663,6 +1838,9 @@ struct rt_scene {
     GLight* d_lights = nullptr;
     float* d_lut = nullptr;
     float* d_ldist = nullptr;   // cdf[n], pdf[n]
+    rt_material* d_raw = nullptr;
+    float4* d_tuv = nullptr;
+    float* d_tex = nullptr;
     int stack_depth = 0;
     int64_t device_bytes = 0;
 };
@@ -1717,6 +1895,11 @@ static DevScene dev_view(const rt_scene* sc) {
     S.sig_s = h.sig_s;
     S.caustics = 0;
     S.block_spec = 0;
+    S.raw = sc->d_raw;
+    S.tuv = h.tex_uv.empty() ? nullptr : sc->d_tuv;
+    S.tex = h.textured ? sc->d_tex : nullptr;
+    S.tex_w = h.tex_w;
+    S.tex_h = h.tex_h;
     return S;
 }
 
@@ -1766,6 +1949,7 @@ struct WindowPlan {
 
 struct LoopTiming {
     double trace = 0, shade = 0, shadow = 0;
+    int64_t launches = 0;   // bounce iterations enqueued (one k_trace and one k_shadow each)
 };
 
 // Runs every window through the device-driven bounce loop.  Iterations are
@@ -1799,7 +1983,12 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
             if (e) HIP_TRY(hipEventRecord(e[0], ctx->stream));
             DISPATCH_SD_C(JOBS, stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, jobs, Pw);
             if (e) HIP_TRY(hipEventRecord(e[1], ctx->stream));
-            switch (ctx->shade_wpe) {
+            bool adj_replay = false;
+            if constexpr (std::is_same<JOBS, OneJob>::value) adj_replay = Pw.adj_pass == 2;
+            if (adj_replay) {
+                if constexpr (std::is_same<JOBS, OneJob>::value)
+                    hipLaunchKernelGGL((k_shade<JOBS, 1, true>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+            } else switch (ctx->shade_wpe) {
                 case 2: hipLaunchKernelGGL((k_shade<JOBS, 2>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
                 case 3: hipLaunchKernelGGL((k_shade<JOBS, 3>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
                 case 4: hipLaunchKernelGGL((k_shade<JOBS, 4>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
@@ -1813,6 +2002,10 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
             DISPATCH_SD_C(JOBS, stack_depth, counting, ctx->priv_stack, k_shadow, dim3(shadow_blocks), ctx->stream, jobs,
                           Pw, maxs);
             if (e) HIP_TRY(hipEventRecord(e[4], ctx->stream));
+            if constexpr (std::is_same<JOBS, OneJob>::value)
+                if (adj_replay)
+                    hipLaunchKernelGGL(k_adj_nee, dim3(shadow_blocks), dim3(kBlock), 0, ctx->stream, Pw, maxs, jobs.S.tex_w,
+                                       jobs.S.tex_h);
         } else {
             const int narr = kWcSeg + (maxs <= kMaxSeg ? maxs : 0);
             hipLaunchKernelGGL(k_count_retired, dim3(pool_blocks), dim3(kBlock), 0, ctx->stream, Pw, narr);
@@ -1867,6 +2060,7 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
         } guard{gexec};
         for (int b = 0;; b++) {
             const int set = b & 1;
+            if (traced) tm.launches += kIterBatch;
             if (gexec) {
                 HIP_TRY(hipGraphLaunch(gexec, ctx->stream));
             } else {
@@ -2149,6 +2343,7 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->pinned) hipHostFree(ctx->pinned);
     if (ctx->parts_mem) hipFree(ctx->parts_mem);
     if (ctx->jobs_mem) hipFree(ctx->jobs_mem);
+    if (ctx->adj_mem) hipFree(ctx->adj_mem);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -2186,7 +2381,11 @@ int rt_scene_create(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
     if ((rc = upload(ctx, &sc->d_mats, h.mats.data(), h.mats.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_lights, h.lights.data(), h.lights.size(), b))) return cleanup(rc);
     std::vector<float> lut(h.lut_e, h.lut_e + 32 * 32);
+    lut.insert(lut.end(), h.lut_eavg, h.lut_eavg + 32);   // E then Eavg
     if ((rc = upload(ctx, &sc->d_lut, lut.data(), lut.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_raw, h.raw_mats.data(), h.raw_mats.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_tuv, h.tex_uv.data(), h.tex_uv.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_tex, h.texels.data(), h.texels.size(), b))) return cleanup(rc);
     std::vector<float> ld(2 * std::max<size_t>(h.lights.size(), 1), 0.0f);
     if ((rc = upload(ctx, &sc->d_ldist, ld.data(), ld.size(), b))) return cleanup(rc);
     *out = sc;
@@ -2203,7 +2402,33 @@ void rt_scene_destroy(rt_scene* sc) {
     hipFree(sc->d_lights);
     hipFree(sc->d_lut);
     hipFree(sc->d_ldist);
+    hipFree(sc->d_raw);
+    hipFree(sc->d_tuv);
+    hipFree(sc->d_tex);
     delete sc;
+}
+
+int rt_scene_set_textures(rt_ctx* ctx, rt_scene* sc, int32_t width, int32_t height, const float* texels) {
+    if (!ctx || !sc || !texels || width <= 0 || height <= 0 || (int64_t)width * height > (1ll << 28))
+        return RT_ERR_INVALID_ARGUMENT;
+    if (!sc->hs.textured) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "scene has no textured material");
+    hipSetDevice(ctx->device);
+    HostScene& h = sc->hs;
+    const size_t n = (size_t)kTexChannels * width * height;
+    if (width != h.tex_w || height != h.tex_h) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        float* d = nullptr;
+        HIP_TRY(hipMalloc(&d, n * sizeof(float)));
+        hipFree(sc->d_tex);
+        sc->device_bytes += (int64_t)(n - h.texels.size()) * (int64_t)sizeof(float);
+        sc->d_tex = d;
+        h.tex_w = width;
+        h.tex_h = height;
+    }
+    h.texels.assign(texels, texels + n);
+    HIP_TRY(hipMemcpyAsync(sc->d_tex, h.texels.data(), n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RT_OK;
 }
 
 int rt_scene_get_info(const rt_scene* sc, rt_scene_info* info) {
@@ -2542,11 +2767,11 @@ static int finish_film(rt_ctx* ctx, const RParams& R, const FilmBufs& F, const r
     return RT_OK;
 }
 
-static void fill_counts(const unsigned long long* tcv, bool counting, rt_render_stats* st) {
+static void fill_counts(const unsigned long long* tcv, bool counting, int64_t launches, rt_render_stats* st) {
     st->rays_closest = (int64_t)tcv[kTcRays];
     st->rays_shadow = (int64_t)tcv[kTcShadowRays];
-    st->trace_launches = (int64_t)tcv[kTcIters];
-    st->shadow_launches = (int64_t)tcv[kTcShadowIters];
+    st->trace_launches = launches;
+    st->shadow_launches = launches;
     if (counting) {
         st->trace_node_tests = (int64_t)tcv[0];
         st->trace_tri_tests = (int64_t)tcv[1];
@@ -2582,6 +2807,8 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     if ((int64_t)maxs * N > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many shadow slots");
     if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
     const bool want_caustics = s->caustics && s->caustic_photons > 0 && !h.lights.empty();
+    if (want_caustics && h.textured)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "photon caustics with textured materials are not supported");
 
     PathBufs P;
     rc = alloc_paths(ctx, N, WS, maxs, s->capture_aovs != 0, want_caustics, P);
@@ -2650,11 +2877,109 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
         stats->camera_samples = (int64_t)npix * spp;
         unsigned long long tcv[16];
         HIP_TRY(hipMemcpy(tcv, P.tcount, sizeof(tcv), hipMemcpyDeviceToHost));
-        fill_counts(tcv, counting, stats);
+        fill_counts(tcv, counting, tm.launches, stats);
     }
     return RT_OK;
 }
 
+
+int rt_adjoint(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const float* d_beauty, float* d_texels,
+               rt_render_stats* stats) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (!ctx || !sc || !d_beauty || !d_texels) return RT_ERR_INVALID_ARGUMENT;
+    int rc = check_settings(ctx, s);
+    if (rc) return rc;
+    hipSetDevice(ctx->device);
+    const HostScene& h = sc->hs;
+    if (!h.textured) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "scene has no textured material");
+    if (h.medium) return fail(ctx, RT_ERR_UNSUPPORTED, "adjoint: participating media are not supported");
+    if (!h.opaque_shadows) return fail(ctx, RT_ERR_UNSUPPORTED, "adjoint: transmissive materials are not supported");
+    if (s->caustics) return fail(ctx, RT_ERR_UNSUPPORTED, "adjoint: photon caustics are not supported");
+    if ((double)s->max_sample_radiance * (double)s->indirect_clamp_factor < 1e30)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "adjoint: radiance clamps must be inactive "
+                                             "(max_sample_radiance * indirect_clamp_factor >= 1e30)");
+    const int spp = s->samples_per_pixel;
+    const int npix = s->width * s->height;
+    const int64_t plane = (int64_t)h.tex_w * h.tex_h;
+    int64_t target = s->paths_in_flight > 0 ? s->paths_in_flight : (1 << 21);
+    const int64_t kWindowCap = std::max<int64_t>(target, std::min<int64_t>(1ll << 26, 32 * target));
+    int K = (int)std::max<int64_t>(1, std::min<int64_t>(spp, kWindowCap / npix));
+    const int64_t WS = (int64_t)K * npix;
+    const int64_t N = std::min<int64_t>(target, WS);
+    if (WS > (1ll << 31) - 1 || N > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
+    const int maxs = shadow_slots(s, h);
+    if ((int64_t)maxs * N > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many shadow slots");
+    if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
+
+    PathBufs P;
+    rc = alloc_paths(ctx, N, WS, maxs, false, false, P);
+    if (rc) return rc;
+    // adjoint buffers: pending bounce terms, NEE terms, dLoss/dBeauty, gradient
+    const size_t b_q = 16 * 5 * (size_t)N, b_g = 16 * 2 * (size_t)N * std::max(maxs, 1);
+    const size_t b_db = 4 * 3 * (size_t)npix, b_dt = 4 * (size_t)kTexChannels * plane;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    rc = grow(ctx, &ctx->adj_mem, &ctx->adj_bytes, al(b_q) + al(b_g) + al(b_db) + al(b_dt));
+    if (rc) return rc;
+    char* am = (char*)ctx->adj_mem;
+    P.adj_q = (float4*)am;
+    P.adj_g = (float4*)(am + al(b_q));
+    float* d_db = (float*)(am + al(b_q) + al(b_g));
+    P.adj_db = d_db;
+    P.adj_dt = (float*)(am + al(b_q) + al(b_g) + al(b_db));
+    P.adj_inv_spp = 1.0f / (float)spp;
+    HIP_TRY(hipMemcpyAsync(d_db, d_beauty, b_db, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(P.adj_dt, 0, b_dt, ctx->stream));
+    rc = upload_light_dist(ctx, sc, s->light_sampling);
+    if (rc) return rc;
+    DevScene S = dev_view(sc);
+    RParams R = make_params(s, h, maxs);
+    HIP_TRY(hipMemsetAsync(P.tcount, 0, 128, ctx->stream));
+    HIP_TRY(hipMemsetAsync(P.counters, 0, 256, ctx->stream));
+    hipEvent_t* ev = ctx->ev;
+    HIP_TRY(hipEventRecord(ev[6], ctx->stream));
+    LoopTiming tm_primal, tm_replay;
+    OneJob one{S, R};
+    auto no_accum = [](const PartDev&) {};
+    if (s->max_depth > 0) {
+        // per window: the primal pass stores every sample's radiance sum, the
+        // replay pass retraces the same samples (same streams) with gradients
+        for (int base = 0; base < spp; base += K) {
+            const int passes = std::min(K, spp - base);
+            WindowPlan plan;
+            plan.add_window({PartDev{0, base, 0, npix, passes, {0, 0, 0}}}, passes * npix);
+            P.adj_pass = 1;
+            rc = run_windows(ctx, one, P, maxs, true, false, sc->stack_depth, nullptr, nullptr, plan, no_accum,
+                             tm_primal);
+            if (rc) return rc;
+            HIP_TRY(hipMemsetAsync(P.adj_q, 0, b_q, ctx->stream));
+            P.adj_pass = 2;
+            rc = run_windows(ctx, one, P, maxs, true, false, sc->stack_depth, nullptr, nullptr, plan, no_accum,
+                             tm_replay);
+            if (rc) return rc;
+        }
+    }
+    HIP_TRY(hipEventRecord(ev[7], ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d_texels, P.adj_dt, b_dt, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float dev_ms = 0;
+    hipEventElapsedTime(&dev_ms, ev[6], ev[7]);
+    int32_t ovf = 0;
+    HIP_TRY(hipMemcpy(&ovf, P.counters + kCntOverflow, 4, hipMemcpyDeviceToHost));
+    if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        stats->device_ms = dev_ms;
+        stats->trace_ms = tm_primal.trace + tm_replay.trace;
+        stats->shade_ms = tm_primal.shade + tm_replay.shade;
+        stats->shadow_ms = tm_primal.shadow + tm_replay.shadow;
+        stats->camera_samples = (int64_t)npix * spp;
+        unsigned long long tcv[16];
+        HIP_TRY(hipMemcpy(tcv, P.tcount, sizeof(tcv), hipMemcpyDeviceToHost));
+        fill_counts(tcv, false, tm_primal.launches + tm_replay.launches, stats);
+    }
+    return RT_OK;
+}
 
 int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32_t pool_paths,
                     rt_render_stats* total) {
@@ -2793,7 +3118,7 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
         total->camera_samples = samples;
         unsigned long long tcv[16];
         HIP_TRY(hipMemcpy(tcv, P.tcount, sizeof(tcv), hipMemcpyDeviceToHost));
-        fill_counts(tcv, counting, total);
+        fill_counts(tcv, counting, tm.launches, total);
     }
     return RT_OK;
 }

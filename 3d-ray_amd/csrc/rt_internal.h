@@ -52,6 +52,7 @@ enum MatFlags : int32_t {
     kMatSpecTransmissive = 4,   // IsSpecularTransmissive
     kMatDielectricIor = 8,      // TryGetDielectricIor returns true
     kMatEmissiveProxy = 16,
+    kMatTextured = 32,          // baseColor/roughness/metallic from the scene's texture set
 };
 
 // Device material record (8 × 16 B).  Disney ShadingParams and the

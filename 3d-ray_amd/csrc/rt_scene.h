@@ -5,6 +5,7 @@
 
 #include "../../include/rt3d.h"
 #include "rt_internal.h"
+#include "material.h"
 
 namespace rt {
 
@@ -46,6 +47,12 @@ struct HostScene {
     float rr_min_survival = 0.1f;
     int max_shadow_per_vertex = 0;     // Σ_l shadow_samples
     int max_light_samples = 0;         // max_l shadow_samples
+    // texture set of RT_MATF_TEXTURED materials
+    std::vector<rt_material> raw_mats; // the material records as given
+    bool textured = false;
+    int tex_w = 0, tex_h = 0;
+    std::vector<float> texels;         // planar [kTexChannels][tex_h][tex_w]
+    std::vector<float4> tex_uv;        // 2 per primitive: (uv0, uv1), (uv2, -, -); empty = HitRecord U/V
 };
 
 // Builds the host scene; returns RT_OK or an error with `err` set.

@@ -309,14 +309,7 @@ void build_energy_lut(float* E, float* Eavg) {
     }
 }
 
-float lut_sample_eavg(const float* Eavg, float alpha) {
-    alpha = clampf(alpha, 0.001f, 1.0f);
-    float fal = alpha * 32.0f - 0.5f;
-    int a0 = std::min(std::max((int)floorf(fal), 0), 31);
-    int a1 = std::min(a0 + 1, 31);
-    float ta = clampf(fal - (float)a0, 0.0f, 1.0f);
-    return Eavg[a0] + (Eavg[a1] - Eavg[a0]) * ta;
-}
+float lut_sample_eavg(const float* Eavg, float alpha) { return eavg_lookup(Eavg, alpha); }
 
 // ── materials (DisneyBsdf.cs:213-293, 1016-1049, 1097-1139, 1943-1959) ──
 static GMat make_gmat(const rt_material& m, const float* Eavg) {
@@ -340,81 +333,8 @@ static GMat make_gmat(const rt_material& m, const float* Eavg) {
             g.m3 = make_float4(0.0f, 0.0f, 0.0f, m.ior);
             flags |= kMatSpecTransmissive | kMatDielectricIor;
             break;
-        default: {
-            float metallic = clampf(m.metallic, 0, 1), roughness = clampf(m.roughness, 0, 1);
-            float specular = clampf(m.specular, 0, 2), spec_tint = clampf(m.specular_tint, 0, 1);
-            float sheen = clampf(m.sheen, 0, 1), sheen_tint = clampf(m.sheen_tint, 0, 1);
-            float sheen_rough = clampf(m.sheen_roughness, 0.04f, 1);
-            float clearcoat = clampf(m.clearcoat, 0, 1), cc_gloss = clampf(m.clearcoat_gloss, 0, 1);
-            float spec_trans = clampf(m.spec_trans, 0, 1), ior = mx(m.ior, 1.0001f);
-            float aniso = clampf(m.anisotropic, 0, 1);
-            float rot = m.anisotropic_rotation - floorf(m.anisotropic_rotation);
-            float diff_trans = clampf(m.diff_trans, 0, 1);
-            float alpha = mx(roughness * roughness, 0.001f);
-            float aspect = sqrtf(1.0f - 0.9f * aniso);
-            float ax = mx(alpha / aspect, 0.001f), ay = mx(alpha * aspect, 0.001f);
-            float cr = (m.flags & RT_MATF_COAT_ROUGHNESS) ? m.coat_roughness : -1.0f;
-            float cc_alpha = cr >= 0.0f ? mx(cr * cr, 0.001f) : lerpf(0.1f, 0.001f, cc_gloss);
-            float ec = mx(m.coat_ior, 1.0001f);
-            float r0 = (ec - 1.0f) / (ec + 1.0f);
-            float cc_f0 = r0 * r0;
-            // ComputeF0
-            float r = (ior - 1.0f) / (ior + 1.0f);
-            float scaled = r * r * 2.0f * specular;
-            float lum = luminance(c);
-            f3 tint = lum > 0.0f ? c / lum : splat(1.0f);
-            f3 diel = lerp3(splat(scaled), scaled * tint, spec_tint);
-            f3 f0 = lerp3(diel, c, metallic);
-            // ComputeLobeWeights
-            float dall = (1.0f - metallic) * (1.0f - spec_trans);
-            float dw = dall * (1.0f - diff_trans), dtw = dall * diff_trans;
-            float sf0;
-            if (metallic > 0.5f) sf0 = luminance(c);
-            else sf0 = r * r * 2.0f * specular;
-            float sfloor = 0.1f * (1.0f - spec_trans * 0.9f);
-            float sw = mx(sfloor, lerpf(sf0, 1.0f, metallic));
-            float tw = (1.0f - metallic) * spec_trans;
-            float cw = clearcoat * 0.04f;
-            float shw = sheen * 0.25f * dall;
-            float alpha_iso = sqrtf(ax * ay);
-            float eavg = lut_sample_eavg(Eavg, alpha_iso);
-            float msw = sw * (1.0f - eavg);
-            float sum = dw + sw + tw + cw + shw + msw + dtw;
-            float tot = sum < 1e-6f ? 1.0f : sum;
-            // ResolveTransmission
-            f3 ttint, sigma = splat(0.0f);
-            bool has_tc = (m.flags & RT_MATF_TRANSMISSION_COLOR) != 0;
-            bool thin = (m.flags & RT_MATF_THIN_WALLED) != 0;
-            if (!has_tc) ttint = mk(sqrtf(mx(c.x, 0.0f)), sqrtf(mx(c.y, 0.0f)), sqrtf(mx(c.z, 0.0f)));
-            else {
-                f3 tc = ld3(m.transmission_color);
-                float td = mx(m.transmission_depth, 0.0f);
-                if (td <= 0.0f) ttint = tc;
-                else {
-                    f3 cc = mk(mx(tc.x, 1e-6f), mx(tc.y, 1e-6f), mx(tc.z, 1e-6f));
-                    float inv = 1.0f / td;
-                    sigma = mk(-logf(cc.x) * inv, -logf(cc.y) * inv, -logf(cc.z) * inv);
-                    ttint = splat(1.0f);
-                }
-            }
-            if (thin) flags |= kMatThinWalled;
-            if (has_tc) flags |= kMatHasTransColor;
-            // IsSpecularTransmissive / TryGetDielectricIor read the raw knobs
-            if (!thin && m.spec_trans >= 0.5f && m.roughness <= 0.05f) flags |= kMatSpecTransmissive;
-            if (!thin && m.spec_trans > 0.0f) flags |= kMatDielectricIor;
-            g.m1 = make_float4(c.x, c.y, c.z, metallic);
-            g.m2 = make_float4(f0.x, f0.y, f0.z, roughness);
-            g.m3 = make_float4(alpha, ax, ay, ior);
-            g.m4 = make_float4(dw / tot, sw / tot, tw / tot, shw / tot);
-            g.m5 = make_float4(msw / tot, dtw / tot, cw / tot, spec_trans);
-            g.m6 = make_float4(sheen, sheen_tint, sheen_rough, diff_trans);
-            g.m7 = make_float4(clearcoat, cc_alpha, cc_f0, rot);
-            g.m8 = make_float4(ttint.x, ttint.y, ttint.z, alpha_iso);
-            g.m9 = make_float4(sigma.x, sigma.y, sigma.z, eavg);
-            // raw spec_trans for ShadowTransmittance (DisneyBsdf.cs:570-584)
-            g.m0.w = fbits(m.spec_trans);
-            break;
-        }
+        default:
+            return disney_gmat(m, c, m.metallic, m.roughness, Eavg);
     }
     g.m0.y = flags;
     return g;
@@ -449,11 +369,39 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
             err = "thin-film Disney materials are not supported";
             return RT_ERR_UNSUPPORTED;
         }
+        if (d->materials[i].flags & RT_MATF_TEXTURED) {
+            const rt_material& m = d->materials[i];
+            if (k != RT_MAT_DISNEY) { err = "only Disney materials can be textured"; return RT_ERR_UNSUPPORTED; }
+            if (m.spec_trans != 0.0f || m.diff_trans != 0.0f) {
+                err = "textured Disney materials need spec_trans = 0 and diff_trans = 0";
+                return RT_ERR_UNSUPPORTED;
+            }
+            if (!d->texels || d->tex_width <= 0 || d->tex_height <= 0 ||
+                (int64_t)d->tex_width * d->tex_height > (1ll << 28)) {
+                err = "textured material without a valid texture set";
+                return RT_ERR_INVALID_ARGUMENT;
+            }
+            hs.textured = true;
+        }
         hs.mats[i] = make_gmat(d->materials[i], hs.lut_eavg);
         // ShadowTransmittance is non-zero only for Dielectric and for Disney
         // with spec_trans > 0 (Dielectric.cs:46-54, DisneyBsdf.cs:570-571)
         if (k == RT_MAT_DIELECTRIC || (k == RT_MAT_DISNEY && d->materials[i].spec_trans > 0.0f))
             hs.opaque_shadows = false;
+    }
+    hs.raw_mats.assign(d->materials, d->materials + d->num_materials);
+    if (hs.textured) {
+        hs.tex_w = d->tex_width;
+        hs.tex_h = d->tex_height;
+        hs.texels.assign(d->texels, d->texels + (size_t)kTexChannels * d->tex_width * d->tex_height);
+        if (d->texcoords) {
+            hs.tex_uv.resize(2 * (size_t)np);
+            for (int64_t i = 0; i < np; i++) {
+                const float* t = d->texcoords + 6 * i;
+                hs.tex_uv[2 * i] = make_float4(t[0], t[1], t[2], t[3]);
+                hs.tex_uv[2 * i + 1] = make_float4(t[4], t[5], 0.0f, 0.0f);
+            }
+        }
     }
 
     // objects (SceneLoader object list; a mesh is one object)

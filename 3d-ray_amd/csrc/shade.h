@@ -532,6 +532,7 @@ struct BsdfS {
     bool has_next_abs;
     f3 next_abs;
     int transition;   // 0 none, 1 enter, 2 exit
+    int lobe;         // Disney lobe that produced the sample (Lobe)
 };
 
 // Sample (DisneyBsdf.cs:893-997)
@@ -541,6 +542,7 @@ __device__ bool disney_sample(const DisneyP& P, const float* E, f3 V, const Surf
     if (!disney_scatter(P, E, s, -V, rng, att, wo, lobe)) return false;
     float ndw = dot(s.n, wo);
     o.wo = wo;
+    o.lobe = lobe;
     o.has_next_abs = false;
     o.next_abs = splat(0.0f);
     o.transition = 0;

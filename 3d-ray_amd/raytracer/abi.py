@@ -17,6 +17,8 @@ RT_PRIM_TRIANGLE, RT_PRIM_SPHERE, RT_PRIM_QUAD, RT_PRIM_PLANE = 0, 1, 2, 3
 RT_PRIMF_BACKFACE_CULLED, RT_PRIMF_CAMERA_INVISIBLE = 1, 2
 RT_MAT_LAMBERTIAN, RT_MAT_EMISSIVE, RT_MAT_DIELECTRIC, RT_MAT_DISNEY = 0, 1, 2, 3
 RT_MATF_LIGHT_PROXY, RT_MATF_THIN_WALLED, RT_MATF_TRANSMISSION_COLOR, RT_MATF_COAT_ROUGHNESS = 1, 2, 4, 8
+RT_MATF_TEXTURED = 16
+TEX_CHANNELS = 5   # texture set planes: base colour r, g, b, roughness, metallic
 RT_LIGHT_SPHERE, RT_LIGHT_AREA, RT_LIGHT_DIRECTIONAL, RT_LIGHT_GEOMETRY = 0, 1, 2, 3
 RT_SAMPLER_SOBOL, RT_SAMPLER_PRNG = 0, 1
 RT_MIS_BALANCE, RT_MIS_POWER = 0, 1
@@ -64,7 +66,8 @@ class rt_scene_desc(C.Structure):
                 ("materials", C.POINTER(rt_material)), ("num_materials", C.c_int64),
                 ("lights", C.POINTER(rt_light)), ("num_lights", C.c_int64),
                 ("camera", rt_camera), ("sky_color", F3), ("reserved0", C.c_int32),
-                ("global_medium", rt_medium)]
+                ("global_medium", rt_medium), ("texcoords", C.POINTER(C.c_float)),
+                ("texels", C.POINTER(C.c_float)), ("tex_width", C.c_int32), ("tex_height", C.c_int32)]
 
 
 class rt_settings(C.Structure):

@@ -58,6 +58,7 @@ def load():
     L.rt_scene_destroy.argtypes = [vp]
     L.rt_scene_destroy.restype = None
     L.rt_scene_get_info.argtypes = [vp, C.POINTER(abi.rt_scene_info)]
+    L.rt_scene_set_textures.argtypes = [vp, vp, C.c_int32, C.c_int32, fp]
     L.rt_render.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.POINTER(abi.rt_film), C.POINTER(abi.rt_render_stats)]
     L.rt_trace_primary.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.c_int32, C.c_int32, ip, fp, fp, fp]
     L.rt_trace_rays.argtypes = [vp, vp, fp, C.c_int64, ip, fp, fp, fp]
@@ -66,6 +67,7 @@ def load():
                                 ip, ip, ip, fp]
     L.rt_render_batch.argtypes = [vp, C.POINTER(abi.rt_batch_job), C.c_int32, C.c_int32, C.POINTER(abi.rt_render_stats)]
     L.rt_context_tune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+    L.rt_adjoint.argtypes = [vp, vp, C.POINTER(abi.rt_settings), fp, fp, C.POINTER(abi.rt_render_stats)]
     L.rt_debug_bvh_dump.argtypes = [C.POINTER(abi.rt_scene_desc), ip, C.c_int64]
     L.rt_debug_bvh_dump.restype = C.c_int64
     _lib = L
@@ -73,8 +75,10 @@ def load():
 
 
 EXPORTED_SYMBOLS = ("rt_get_version", "rt_material_defaults", "rt_settings_defaults", "rt_create", "rt_destroy",
-                    "rt_last_error", "rt_scene_create", "rt_scene_destroy", "rt_scene_get_info", "rt_render",
+                    "rt_last_error", "rt_scene_create", "rt_scene_destroy", "rt_scene_get_info",
+                    "rt_scene_set_textures", "rt_render",
                     "rt_trace_primary", "rt_trace_rays", "rt_photon_map", "rt_render_batch", "rt_context_tune",
+                    "rt_adjoint",
                     "rt_debug_bvh_dump")
 
 
@@ -141,6 +145,14 @@ class Scene:
         i = abi.rt_scene_info()
         self.ctx.check(load().rt_scene_get_info(self.h, C.byref(i)))
         return i
+
+    def set_textures(self, texture):
+        """Replaces the texture set ((5, h, w) fp32: base r, g, b, roughness,
+        metallic) of a scene with textured materials."""
+        t = np.ascontiguousarray(texture, np.float32)
+        assert t.ndim == 3 and t.shape[0] == abi.TEX_CHANNELS
+        self.ctx.check(load().rt_scene_set_textures(self.ctx.h, self.h, t.shape[2], t.shape[1], _f(t)))
+        self.desc.texture = t
 
     def trace_primary(self, settings, sample_begin=0, sample_count=1):
         n = settings.width * settings.height * sample_count
@@ -263,6 +275,20 @@ class Renderer:
         ctx = self.scene.ctx
         ctx.check(load().rt_render(ctx.h, self.scene.h, C.byref(s), C.byref(film), C.byref(stats)))
         return self._result(width, height, planes, count, stats)
+
+    def adjoint(self, width, height, d_beauty):
+        """Texture gradient (rt_adjoint): dLoss/dTexels (5, th, tw) of a loss
+        with dLoss/dBeauty = d_beauty (3, height, width); returns (gradient,
+        rt_render_stats)."""
+        s = self.settings
+        s.width, s.height, s.capture_aovs = width, height, 0
+        db = np.ascontiguousarray(d_beauty, np.float32).reshape(3, height, width)
+        tex = self.scene.desc.texture
+        out = np.zeros(tex.shape, np.float32)
+        stats = abi.rt_render_stats()
+        ctx = self.scene.ctx
+        ctx.check(load().rt_adjoint(ctx.h, self.scene.h, C.byref(s), _f(db), _f(out), C.byref(stats)))
+        return out, stats
 
 
 def render_batch(items, pool_paths=0, films=True):

@@ -55,7 +55,8 @@ def material_records(n):
 class SceneDesc:
     """Owns the numpy tables behind one rt_scene_desc (keeps them alive)."""
 
-    def __init__(self, prims, mats, lights, camera, sky=(0.0, 0.0, 0.0), medium=None):
+    def __init__(self, prims, mats, lights, camera, sky=(0.0, 0.0, 0.0), medium=None, texcoords=None,
+                 texture=None):
         self.prims = np.ascontiguousarray(prims, PRIM_DTYPE)
         self.mats = np.ascontiguousarray(mats, MAT_DTYPE)
         self.lights = np.ascontiguousarray(lights, LIGHT_DTYPE)
@@ -75,6 +76,16 @@ class SceneDesc:
             d.global_medium.enabled = 1
             d.global_medium.sigma_a[:] = medium[0]
             d.global_medium.sigma_s[:] = medium[1]
+        # texture set (5, h, w) fp32 and optional per-primitive vertex UVs (n, 6)
+        self.texture = None if texture is None else np.ascontiguousarray(texture, np.float32)
+        self.texcoords = None if texcoords is None else np.ascontiguousarray(texcoords, np.float32).reshape(-1, 6)
+        if self.texture is not None:
+            assert self.texture.ndim == 3 and self.texture.shape[0] == abi.TEX_CHANNELS
+            d.texels = C.cast(self.texture.ctypes.data, C.POINTER(C.c_float))
+            d.tex_height, d.tex_width = self.texture.shape[1], self.texture.shape[2]
+        if self.texcoords is not None:
+            assert len(self.texcoords) == len(self.prims)
+            d.texcoords = C.cast(self.texcoords.ctypes.data, C.POINTER(C.c_float))
         self.desc = d
 
     @property
@@ -126,6 +137,8 @@ class SceneBuilder:
             elif k == "coat_roughness":
                 flags |= abi.RT_MATF_COAT_ROUGHNESS
                 m["coat_roughness"] = v
+            elif k == "textured":
+                flags |= abi.RT_MATF_TEXTURED if v else 0
             else:
                 m[k] = v
         m["flags"] = flags
@@ -388,6 +401,71 @@ def displaced_icosphere_scene(width=1920, height=1080, subdiv=8, seed=0xC0FFEE, 
         b.directional_light((-0.4, -1.0, -0.3), (1.0, 1.0, 1.0), 3.0)
     cam = make_camera((0.0, 1.2, 3.5), (0.0, 0.9, 0.0), 40.0, width / height)
     return b.build(cam)
+
+
+def sphere_uv(dirs):
+    """Sphere.GetSphereUV (Sphere.cs:260-269) of unit directions (n, 3)."""
+    theta = np.arccos(np.clip(-dirs[:, 1], -1.0, 1.0))
+    phi = np.arctan2(-dirs[:, 2], dirs[:, 0]) + np.pi
+    return np.stack([phi / (2.0 * np.pi), theta / np.pi], 1)
+
+
+def material_texture(size=512, seed=0x7E47, cell=16, base=(0.05, 0.95), rough=(0.1, 1.0), p_metal=0.3):
+    """Seeded texture set (5, size, size): piecewise-constant cells of
+    `cell` texels with baseColor U[base]^3, roughness U[rough] and metallic
+    Bernoulli(p_metal) in {0, 1}."""
+    rng = np.random.default_rng(seed)
+    n = size // cell
+    c = rng.uniform(base[0], base[1], (3, n, n))
+    r = rng.uniform(rough[0], rough[1], (1, n, n))
+    m = (rng.random((1, n, n)) < p_metal).astype(np.float64)
+    t = np.concatenate([c, r, m]).repeat(cell, 1).repeat(cell, 2)
+    return np.ascontiguousarray(t, np.float32)
+
+
+def textured_icosphere_scene(width=256, height=256, subdiv=6, seed=0xC0FFEE, texture=None, sphere_lights=4,
+                             sun=True):
+    """Differentiation workload: the displaced icosphere of the single large
+    scene at 20·4^subdiv triangles (81,920 at subdiv 6) with ONE textured
+    Disney material (specular 0.5) whose baseColor, roughness and metallic
+    come from a texture set mapped spherically (vertex UVs = GetSphereUV of
+    the undisplaced vertex direction, u unwrapped across the seam per
+    triangle); same ground, lights and camera.  texture: (5, h, w), default
+    the constant 0.5 start texture at 512^2."""
+    rng = np.random.default_rng(seed)
+    v, f = icosphere(subdiv)
+    uvv = sphere_uv(v)
+    d = fbm(v, 6, seed)
+    v = v * (1.0 + 0.05 * d)[:, None]
+    v = v + np.array([0.0, 1.0, 0.0])
+    tris = v[f].astype(np.float32)
+    n = len(tris)
+    tuv = uvv[f]                                   # (n, 3, 2)
+    u = tuv[:, :, 0]
+    wrap = (u.max(1) - u.min(1)) > 0.5
+    u[wrap] = np.where(u[wrap] < 0.5, u[wrap] + 1.0, u[wrap])
+    b = SceneBuilder()
+    b.shadow_samples_override = 1
+    ground = b.disney((0.5, 0.5, 0.5), roughness=0.8)
+    g = np.array([[[-10, 0, -10], [-10, 0, 10], [10, 0, 10]], [[-10, 0, -10], [10, 0, 10], [10, 0, -10]]], np.float32)
+    b.triangles(g, ground)
+    first = b._count
+    mat = b.disney((0.5, 0.5, 0.5), specular=0.5, textured=True)
+    b.triangles(tris, mat)
+    for k in range(sphere_lights):
+        ang = 2.0 * np.pi * (k + 0.25) / max(sphere_lights, 1)
+        pos = (1.8 * np.cos(ang), 2.3 + 0.2 * (k % 2), 1.8 * np.sin(ang))
+        b.sphere_light(pos, 0.1, (1.0, 1.0, 1.0), float(rng.uniform(5.0, 20.0)), shadow_samples=1)
+    if sun:
+        b.directional_light((-0.4, -1.0, -0.3), (1.0, 1.0, 1.0), 3.0)
+    cam = make_camera((0.0, 1.2, 3.5), (0.0, 0.9, 0.0), 40.0, width / height)
+    desc = b.build(cam)
+    tc = np.zeros((len(desc.prims), 6), np.float32)
+    tc[first:first + n] = tuv.reshape(n, 6)
+    if texture is None:
+        texture = np.full((abi.TEX_CHANNELS, 512, 512), 0.5, np.float32)
+    return SceneDesc(desc.prims, desc.mats, desc.lights, desc.camera, desc.sky, desc.medium, texcoords=tc,
+                     texture=texture)
 
 
 def gerstner_water(n_cells, half_extent=2.0, height=0.45, waves=8, seed=0x5EA):

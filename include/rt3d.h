@@ -107,7 +107,12 @@ typedef enum rt_material_flags {
     RT_MATF_LIGHT_PROXY = 1,          /* Emissive.IsLightProxy */
     RT_MATF_THIN_WALLED = 2,          /* DisneyBsdf.ThinWalled */
     RT_MATF_TRANSMISSION_COLOR = 4,   /* DisneyBsdf.TransmissionColor != null */
-    RT_MATF_COAT_ROUGHNESS = 8        /* DisneyBsdf.CoatRoughness != null */
+    RT_MATF_COAT_ROUGHNESS = 8,       /* DisneyBsdf.CoatRoughness != null */
+    RT_MATF_TEXTURED = 16             /* DISNEY only: baseColor, roughness and metallic
+                                         come from the scene's texture set at the
+                                         hit's texture coordinate (no reference
+                                         counterpart; spec_trans and diff_trans
+                                         must be 0) */
 } rt_material_flags;
 
 typedef struct rt_material {
@@ -189,6 +194,20 @@ typedef struct rt_scene_desc {
     float sky_color[3];   /* flat sky (FlatSky.cs); must be black (see rt_scene_create) */
     int32_t reserved0;
     rt_medium global_medium;
+    /* Texture set of RT_MATF_TEXTURED materials (no reference counterpart:
+     * the reference's Disney parameters are constant FloatTexture/SolidColor
+     * values, DisneyBsdf.cs:55-73).  texels: planar fp32
+     * [5][tex_height][tex_width] = base colour r, g, b, roughness, metallic;
+     * bilinear, texel centres at (i + 0.5) / width, u wraps, v clamps.
+     * texcoords (optional): 6 floats per primitive, the (u, v) of a
+     * TRIANGLE's three vertices, interpolated as SmoothTriangle does
+     * (SmoothTriangle.cs:186: (1-b1-b2) uv0 + b1 uv1 + b2 uv2); without them a
+     * hit's texture coordinate is HitRecord.U/V (barycentrics, quad
+     * (alpha, beta), Sphere.GetSphereUV, the plane's tiled UV). */
+    const float* texcoords;
+    const float* texels;
+    int32_t tex_width;
+    int32_t tex_height;
 } rt_scene_desc;
 
 /* ── Render settings (Renderer ctor arguments + Render(width, height)) ───── */
@@ -252,8 +271,9 @@ typedef struct rt_render_stats {
     int64_t rays_closest;        /* primary + extension segments traced */
     int64_t rays_shadow;         /* shadow segments (each transparent-shadow
                                     continuation counts once) */
-    int64_t trace_launches;      /* closest-hit kernel launches */
-    int64_t shadow_launches;     /* shadow kernel launches */
+    int64_t trace_launches;      /* closest-hit kernel launches (bounce iterations
+                                    enqueued, trailing empty ones included) */
+    int64_t shadow_launches;     /* shadow kernel launches (same count) */
     /* with rt_settings.count_traversal: the reference traversal's work —
        BvhNode.Hit calls (node box tests) and primitive Hit calls — of the
        closest-hit and shadow kernels */
@@ -306,10 +326,32 @@ int rt_scene_create(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out);
 void rt_scene_destroy(rt_scene* scene);
 int rt_scene_get_info(const rt_scene* scene, rt_scene_info* info);
 
+/* Replaces the texture set of a scene with textured materials (same layout
+ * as rt_scene_desc.texels; the size may change).  No reference counterpart:
+ * the optimisation loop of the adjoint updates textures between renders
+ * without rebuilding the BVH. */
+int rt_scene_set_textures(rt_ctx* ctx, rt_scene* scene, int32_t width, int32_t height, const float* texels);
+
 /* Renders settings->samples_per_pixel samples per pixel into the film.
  * Replaces: Renderer.Render(width, height, options) (Renderer.cs:522-733). */
 int rt_render(rt_ctx* ctx, const rt_scene* scene, const rt_settings* settings,
               const rt_film* film, rt_render_stats* stats);
+
+/* Path-replay adjoint of the texture set (no reference counterpart).
+ * Given dLoss/dBeauty (d_beauty: planar [3][height][width], host memory) of
+ * a loss on the beauty film rendered by rt_render with the same scene and
+ * settings, writes dLoss/dTexels (d_texels: planar [5][tex_height][tex_width],
+ * host memory; overwritten).  Each window of samples is traced twice with
+ * the same sampler streams: a primal pass storing every sample's radiance,
+ * then a replay pass adding, at textured Disney hits, the derivative of the
+ * NEE BSDF terms and of the sampled-bounce weight (detached sampling: pdfs,
+ * MIS weights, lobe choice and Russian roulette held fixed; delta and
+ * fallback bounces add no term).  Requires: no medium, no transmissive
+ * material, no caustics, inactive clamps (max_sample_radiance *
+ * indirect_clamp_factor >= 1e30); else RT_ERR_UNSUPPORTED.  Texel gradients
+ * are summed with float atomics (order not fixed). */
+int rt_adjoint(rt_ctx* ctx, const rt_scene* scene, const rt_settings* settings, const float* d_beauty,
+               float* d_texels, rt_render_stats* stats);
 
 /* Kernel configuration of a context (no reference counterpart; set by an
  * autotuner): shade_waves_per_simd = minimum waves per SIMD the shading

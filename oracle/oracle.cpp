@@ -7,7 +7,7 @@
 // a difference between it and the wavefront GPU design shows up as a test
 // failure rather than being shared by both.
 //
-// Parity status per function (what pins it) is listed in DESIGN.md; the
+// Each function is pinned by the tests under tests/ that exercise it; the
 // functions whose bits depend on .NET's Vector3 reduction order or MathF
 // transcendentals are statistically pinned only.
 #include <atomic>
@@ -748,6 +748,9 @@ struct Material {
     virtual bool scatter(const Ray&, const HitRecord&, V3& att, Ray& scattered) const = 0;
     virtual bool is_emissive() const { return false; }
     virtual bool is_light_proxy() const { return false; }
+    // textured Disney: the material record whose baseColor / roughness /
+    // metallic the texture set replaces per hit (null: not textured)
+    virtual const rt_material* texture_raw() const { return nullptr; }
 };
 
 // Lambertian.cs:37-96
@@ -856,7 +859,10 @@ struct Disney : Material {
         float alpha, ax, ay, cc_alpha, cc_f0, tf_thick, tf_ior;
     } sp;
 
+    rt_material raw{};
+    const rt_material* texture_raw() const override { return (raw.flags & RT_MATF_TEXTURED) ? &raw : nullptr; }
     explicit Disney(const rt_material& m) {
+        raw = m;
         base = V3(m.color[0], m.color[1], m.color[2]);
         metallic_ = m.metallic; roughness_ = m.roughness; specular_ = m.specular;
         specular_tint_ = m.specular_tint; sheen_ = m.sheen; sheen_tint_ = m.sheen_tint;
@@ -1347,7 +1353,47 @@ struct Disney : Material {
         if (std::isnan(att.x) || std::isinf(att.x) || std::isnan(att.y) || std::isinf(att.y) || std::isnan(att.z) ||
             std::isinf(att.z))
             att = base;
+        if (raw.flags & RT_MATF_TEXTURED) last_lobe = lobe;   // per-hit instance (adjoint)
         return res;
+    }
+    mutable int last_lobe = -1;
+    // The weight the lobe sampler returned for direction dir, without the
+    // division by the lobe probability (restated from the samplers above).
+    V3 lobe_att(int lobe, const HitRecord& rec, V3 V, V3 dir) const {
+        V3 N = rec.normal;
+        if (lobe == kDiffuse) {
+            float ndv = std::fmax(dot(N, V), 0.001f), ndl = std::fmax(dot(N, dir), 0.001f);
+            V3 hr = V + dir;
+            float hl2 = length_sq(hr);
+            V3 H = hl2 > 1e-7f ? hr / std::sqrt(hl2) : N;
+            float ldh = std::fmax(dot(dir, H), 0.0f);
+            float fd90 = 0.5f + 2.0f * sp.roughness * ldh * ldh;
+            float fi = schlick_weight(ndv), fo = schlick_weight(ndl);
+            float fd = (1.0f + (fd90 - 1.0f) * fi) * (1.0f + (fd90 - 1.0f) * fo);
+            return base * fd * (1.0f - sp.diff_trans);
+        }
+        if (lobe == kSpecular) {
+            Frame fr = shading_frame(rec);
+            V3 H = normalize(V + dir);
+            float vdh = std::fmax(dot(V, H), 1e-4f);
+            return eval_fresnel(vdh, compute_f0()) * g1_ggx_aniso(fr.to_local(dir), sp.ax, sp.ay);
+        }
+        if (lobe == kSheen) {
+            float ndv = std::fmax(dot(N, V), 1e-4f), ndl = std::fmax(dot(N, dir), 1e-4f);
+            V3 hr = V + dir;
+            float hl2 = length_sq(hr);
+            V3 H = hl2 > 1e-7f ? hr / std::sqrt(hl2) : N;
+            float ndh = std::fmax(dot(N, H), 0.0f);
+            float lum = luminance(base);
+            V3 tint = lum > 0.0f ? base / lum : V3(1.0f);
+            return sp.sheen * kPi * charlie_brdf(ndv, ndl, ndh, sp.sheen_roughness) * lerp(V3(1.0f), tint, sp.sheen_tint);
+        }
+        if (lobe == kMultiscatter) {
+            float ndv = std::fmax(dot(N, V), 1e-3f), ndl = std::fmax(dot(N, dir), 1e-3f);
+            return multiscatter_weight(compute_f0(), ndv, ndl);
+        }
+        if (lobe == kDiffTrans) return base * sp.diff_trans;
+        return V3(0.0f);
     }
 
     bool scatter(const Ray& rin, const HitRecord& rec, V3& att, Ray& scattered) const override {
@@ -1943,6 +1989,69 @@ struct Scene {
     int rr_min_bounces = 3;
     float rr_min_survival = 0.1f;
     std::vector<std::pair<const Material*, const Light*>> emitter_to_light;
+    // texture set of textured Disney materials: planar [5][tex_h][tex_w],
+    // optional vertex UVs (6 per primitive)
+    std::vector<float> tex, tuv;
+    int tex_w = 0, tex_h = 0;
+    // Texture coordinate of a hit: interpolated vertex UVs of a triangle
+    // (SmoothTriangle.cs:168,186) when given, else HitRecord.U/V.
+    void tex_coord(const HitRecord& rec, bool is_tri, float& u, float& v) const {
+        u = rec.u;
+        v = rec.v;
+        if (tuv.empty() || !is_tri || rec.prim < 0) return;
+        const float* t = &tuv[6 * (size_t)rec.prim];
+        float w1 = rec.u, w2 = rec.v, w0 = 1.0f - w1 - w2;
+        u = (w0 * t[0] + w1 * t[2]) + w2 * t[4];
+        v = (w0 * t[1] + w1 * t[3]) + w2 * t[5];
+    }
+    // Bilinear footprint: texel centres at (i + 0.5) / width, u wraps, v clamps.
+    void tex_foot(float u, float v, int idx[4], float w[4]) const {
+        float x = u * (float)tex_w - 0.5f, y = v * (float)tex_h - 0.5f;
+        float x0 = std::floor(x), y0 = std::floor(y);
+        float fx = x - x0, fy = y - y0;
+        long long ix = (long long)x0;
+        int i0 = (int)(((ix % tex_w) + tex_w) % tex_w), i1 = (i0 + 1) % tex_w;
+        int j0 = (int)std::min<float>(std::max<float>(y0, 0.0f), (float)(tex_h - 1));
+        int j1 = (int)std::min<float>(std::max<float>(y0 + 1.0f, 0.0f), (float)(tex_h - 1));
+        idx[0] = j0 * tex_w + i0; idx[1] = j0 * tex_w + i1; idx[2] = j1 * tex_w + i0; idx[3] = j1 * tex_w + i1;
+        w[0] = (1.0f - fx) * (1.0f - fy); w[1] = fx * (1.0f - fy); w[2] = (1.0f - fx) * fy; w[3] = fx * fy;
+    }
+    float tex_at(const int idx[4], const float w[4], int ch) const {
+        const float* pl = &tex[(size_t)ch * tex_w * tex_h];
+        return ((w[0] * pl[idx[0]] + w[1] * pl[idx[1]]) + w[2] * pl[idx[2]]) + w[3] * pl[idx[3]];
+    }
+    std::vector<uint8_t> tri_flag;   // per primitive: TRIANGLE
+    bool textured = false;
+    // Texture lookup of a textured hit: filtered parameters and footprint.
+    struct TexHit {
+        const rt_material* raw;
+        float p[5];        // base r, g, b, roughness, metallic
+        int idx[4];
+        float w[4];
+    };
+    TexHit tex_hit(const rt_material& raw, const HitRecord& rec) const {
+        TexHit t;
+        t.raw = &raw;
+        float u, v;
+        bool tri = rec.prim >= 0 && rec.prim < (int)tri_flag.size() && tri_flag[rec.prim];
+        tex_coord(rec, tri, u, v);
+        tex_foot(u, v, t.idx, t.w);
+        for (int c = 0; c < 5; c++) t.p[c] = tex_at(t.idx, t.w, c);
+        return t;
+    }
+    // The per-hit Disney of a textured material: Disney(rt_material) with
+    // the texture's baseColor / roughness / metallic, parameter j moved by dp.
+    static Disney material_at(const TexHit& t, int j = -1, float dp = 0.0f) {
+        float q[5];
+        for (int c = 0; c < 5; c++) q[c] = t.p[c] + (c == j ? dp : 0.0f);
+        rt_material m = *t.raw;
+        m.color[0] = q[0];
+        m.color[1] = q[1];
+        m.color[2] = q[2];
+        m.roughness = q[3];
+        m.metallic = q[4];
+        return Disney(m);
+    }
     ~Scene() {
         delete world;
         for (Hittable* h : owned) delete h;
@@ -2251,6 +2360,48 @@ static V3 clamp01(V3 v) { return V3(clampf(v.x, 0, 1), clampf(v.y, 0, 1), clampf
 
 enum RayCat { kCatCamera = 0, kCatDiffuse, kCatGlossy, kCatTransmission };
 
+// ═════════════════════════════════════════════════════════════════════════
+// FP64 adjoint of the texture parameters (no reference counterpart).  The
+// render recursion runs unchanged; aw is the weight (dLoss/dPixel / spp times
+// every bounce weight down to the current vertex) of the radiance the current
+// ShadeSurface returns.  At a textured hit the derivative of the BSDF with
+// respect to the filtered parameters p = (base r, g, b, roughness, metallic)
+// is taken by central differences of the fp32 Disney evaluation (step
+// kAdjH, sampling, pdfs and MIS weights held at p), accumulated in double and
+// spread over the bilinear footprint:
+//   NEE:    aw · k_l · dBrdf(wl)/dp      (k_l = wn Le tr terminator / ppick)
+//   bounce: aw · dF(wo)/dp |cos|/pdf / q · Li(wo)   (Li = the returned radiance)
+// Delta and fallback (Scatter) bounces carry the weight but add no term.
+// ═════════════════════════════════════════════════════════════════════════
+constexpr float kAdjH = 1e-3f;
+struct AdjState {
+    double aw[3] = {0, 0, 0};
+    const Scene::TexHit* hit = nullptr;   // set while a textured hit's NEE runs
+    double* grad = nullptr;               // [5][tex_h][tex_w]
+    int64_t plane = 0;
+    void add(const Scene::TexHit& t, const double g[5]) {
+        for (int k = 0; k < 4; k++)
+            for (int j = 0; j < 5; j++) grad[j * plane + t.idx[k]] += (double)t.w[k] * g[j];
+    }
+};
+static thread_local AdjState* t_adj = nullptr;
+// Scales the adjoint weight by a bounce weight for the duration of a call.
+struct AdjWeight {
+    AdjState* A;
+    double saved[3];
+    AdjWeight(AdjState* a, V3 att) : A(a) {
+        if (!A) return;
+        for (int c = 0; c < 3; c++) saved[c] = A->aw[c];
+        A->aw[0] *= att.x;
+        A->aw[1] *= att.y;
+        A->aw[2] *= att.z;
+    }
+    void restore() {
+        if (A)
+            for (int c = 0; c < 3; c++) A->aw[c] = saved[c];
+    }
+};
+
 struct Renderer {
     const Scene& sc;
     int max_depth, spp;
@@ -2407,6 +2558,23 @@ struct Renderer {
                 }
             }
             acc += wn * ls.color * brdf * tr;
+            if (t_adj && t_adj->hit) {
+                // NEE term: d(brdf)/dp at the light direction, MIS weight held fixed
+                const Scene::TexHit& th = *t_adj->hit;
+                float stl = view_term * shadow_terminator(rec.shading_ng, rec.normal, ls.dir);
+                V3 k = wn * ls.color * tr * stl;
+                double inv = ppick > 0.0f ? 1.0 / (double)ppick : 1.0;
+                double g[5];
+                for (int j = 0; j < 5; j++) {
+                    V3 fp = Scene::material_at(th, j, kAdjH).evaluate_direct(ls.dir, view, rec.normal, rec);
+                    V3 fm = Scene::material_at(th, j, -kAdjH).evaluate_direct(ls.dir, view, rec.normal, rec);
+                    double h2 = 2.0 * (double)kAdjH;
+                    g[j] = inv * (t_adj->aw[0] * k.x * (((double)fp.x - fm.x) / h2) +
+                                  t_adj->aw[1] * k.y * (((double)fp.y - fm.y) / h2) +
+                                  t_adj->aw[2] * k.z * (((double)fp.z - fm.z) / h2));
+                }
+                t_adj->add(th, g);
+            }
         };
         if (ldist) {
             if (sc.light_ptrs.empty()) return result;
@@ -2477,6 +2645,15 @@ struct Renderer {
     // ShadeSurface (Renderer.cs:1008-1228)
     V3 shade_surface(const Ray& ray, HitRecord& rec, int depth, float prev_pdf, bool prev_delta, V3 absorption, V3 beta,
                      IorStack& iors, int cst) const {
+        std::unique_ptr<Disney> tex_mat;
+        Scene::TexHit th;
+        const Scene::TexHit* thp = nullptr;
+        if (rec.mat && rec.mat->texture_raw()) {
+            th = sc.tex_hit(*rec.mat->texture_raw(), rec);
+            thp = &th;
+            tex_mat.reset(new Disney(Scene::material_at(th)));
+            rec.mat = tex_mat.get();
+        }
         const Material* m = rec.mat;
         rec.shading_ng = rec.normal;
         if (capture_aovs && probe && probe->pending) {
@@ -2493,13 +2670,16 @@ struct Renderer {
         if (m && m->try_dielectric_ior(rec, mior) && mior > 0.0f)
             rec.relative_eta = rec.front_face ? (iors.top() / mior) : (mior / iors.enclosing());
         bool needs = m ? m->needs_direct_lighting() : true;
+        AdjState* A = t_adj;
+        if (A) A->hit = thp;
         V3 direct = needs ? direct_lighting(rec, view, m) : V3(0.0f);
+        if (A) A->hit = nullptr;
         if (caustics && needs && m && !m->is_delta_scatter()) direct += gather_caustics(rec, view, m);
         if (!m) return emitted + direct;
 
         BsdfSample s;
         if (m->sample(view, rec, s))
-            return shade_sample_bounce(m, rec, s, depth, emitted, direct, absorption, beta, iors, cst);
+            return shade_sample_bounce(m, rec, s, depth, emitted, direct, absorption, beta, iors, cst, view, thp);
 
         V3 att;
         Ray scat;
@@ -2527,8 +2707,11 @@ struct Renderer {
                 if (rec.front_face) ni.push(sior);
                 else ni.pop();
             }
+            AdjWeight aw(A, att);
             V3 ind = trace_ray(scat, depth - 1, 0.0f, nd, ni, absorption, nt, cat, next_caustic(cst, nd));
+            aw.restore();
             ind = sanitize(ind);
+            if (A && thp) adj_lobe_term(A, th, static_cast<const Disney*>(m)->last_lobe, rec, view, scat.d, att, ind);
             V3 contrib = att * ind;
             if (depth == max_depth) contrib = clamp_indirect(contrib);
             return emitted + direct + contrib;
@@ -2537,8 +2720,32 @@ struct Renderer {
     }
 
     // ShadeSampleBounce (Renderer.cs:1235-1383)
+    // Bounce term of a delta or fallback (Scatter) bounce: d(lobe weight)/dp
+    // at the sampled direction, times the fixed 1/(lobe probability * q).
+    static void adj_lobe_term(AdjState* A, const Scene::TexHit& th, int lobe, const HitRecord& rec, V3 V, V3 dir, V3 att,
+                              V3 ind) {
+        V3 w0 = Scene::material_at(th).lobe_att(lobe, rec, V, dir);
+        double g[5];
+        for (int j = 0; j < 5; j++) {
+            V3 wp = Scene::material_at(th, j, kAdjH).lobe_att(lobe, rec, V, dir);
+            V3 wm = Scene::material_at(th, j, -kAdjH).lobe_att(lobe, rec, V, dir);
+            double acc = 0.0;
+            for (int c = 0; c < 3; c++) {
+                double wv = c == 0 ? w0.x : (c == 1 ? w0.y : w0.z);
+                if (wv == 0.0) continue;
+                double scale = (c == 0 ? att.x : (c == 1 ? att.y : att.z)) / wv;   // 1 / (prob * q)
+                double dw = ((c == 0 ? wp.x : (c == 1 ? wp.y : wp.z)) - (c == 0 ? wm.x : (c == 1 ? wm.y : wm.z))) /
+                            (2.0 * (double)kAdjH);
+                acc += A->aw[c] * dw * scale * (c == 0 ? ind.x : (c == 1 ? ind.y : ind.z));
+            }
+            g[j] = acc;
+        }
+        A->add(th, g);
+    }
+
     V3 shade_sample_bounce(const Material* m, const HitRecord& rec, const BsdfSample& s, int depth, V3 emitted, V3 direct,
-                           V3 absorption, V3 beta, IorStack& iors, int cst) const {
+                           V3 absorption, V3 beta, IorStack& iors, int cst, V3 view = V3(0.0f),
+                           const Scene::TexHit* th = nullptr) const {
         V3 att;
         if (s.is_delta) att = s.f;
         else {
@@ -2553,12 +2760,14 @@ struct Renderer {
         }
         V3 nt = beta * att;
         int used = max_depth - depth;
+        float rr_inv = 1.0f;
         if (used >= rr_min_b) {
             float sp = fmax_p(nt.x, fmax_p(nt.y, nt.z));
             sp = fmax_p(sp, rr_min_s);
             sp = fmin_p(sp, 0.95f);
             if (random_float() > sp) return emitted + direct;
             float inv = 1.0f / sp;
+            rr_inv = inv;
             att *= inv;
             nt *= inv;
         }
@@ -2568,6 +2777,8 @@ struct Renderer {
         V3 nabs = s.has_next_absorption ? s.next_absorption : absorption;
         int cat = classify(rec.normal, s.wo, s.is_delta);
         V3 ind;
+        AdjState* A = t_adj;
+        AdjWeight aw(A, att);
         if (s.transition == 1 || s.transition == 2) {
             IorStack ni = iors;
             if (s.transition == 1) {
@@ -2578,7 +2789,27 @@ struct Renderer {
         } else {
             ind = trace_ray(scat, depth - 1, npdf, s.is_delta, iors, nabs, nt, cat, next_caustic(cst, s.is_delta));
         }
+        aw.restore();
         ind = sanitize(ind);
+        if (A && th && s.is_delta) {
+            const Disney* dm = static_cast<const Disney*>(m);
+            adj_lobe_term(A, *th, dm->last_lobe, rec, view, s.wo, att, ind);
+        }
+        if (A && th && !s.is_delta) {
+            // bounce term: d(att)/dp = d f(view, wo)/dp * |cos| / pdf (/ RR
+            // survival), pdf and sampling held fixed
+            double scale = (double)std::fabs(dot(rec.normal, s.wo)) / (double)s.pdf;
+            double rr = (double)rr_inv;
+            double g[5];
+            for (int j = 0; j < 5; j++) {
+                V3 fp = Scene::material_at(*th, j, kAdjH).evaluate(view, s.wo, rec);
+                V3 fm = Scene::material_at(*th, j, -kAdjH).evaluate(view, s.wo, rec);
+                double k = scale * rr / (2.0 * (double)kAdjH);
+                g[j] = A->aw[0] * ((double)fp.x - fm.x) * k * ind.x + A->aw[1] * ((double)fp.y - fm.y) * k * ind.y +
+                       A->aw[2] * ((double)fp.z - fm.z) * k * ind.z;
+            }
+            A->add(*th, g);
+        }
         V3 contrib = att * ind;
         if (depth == max_depth) contrib = clamp_indirect(contrib);
         return emitted + direct + contrib;
@@ -2715,6 +2946,24 @@ static Scene* build_scene(const rt_scene_desc* d) {
             cur_mesh_id = -1;
             sc->objects.push_back(obj);
         }
+    }
+    sc->tri_flag.assign(np, 0);
+    for (int64_t i = 0; i < np; i++) sc->tri_flag[i] = d->primitives[i].kind == RT_PRIM_TRIANGLE;
+    for (int64_t i = 0; i < d->num_materials; i++) {
+        const rt_material& m = d->materials[i];
+        if (!(m.flags & RT_MATF_TEXTURED)) continue;
+        if (m.kind != RT_MAT_DISNEY || m.spec_trans != 0.0f || m.diff_trans != 0.0f) {
+            g_err = "unsupported textured material";
+            return nullptr;
+        }
+        if (!d->texels || d->tex_width <= 0 || d->tex_height <= 0) { g_err = "textured material without texture set"; return nullptr; }
+        sc->textured = true;
+    }
+    if (sc->textured) {
+        sc->tex_w = d->tex_width;
+        sc->tex_h = d->tex_height;
+        sc->tex.assign(d->texels, d->texels + (size_t)5 * d->tex_width * d->tex_height);
+        if (d->texcoords) sc->tuv.assign(d->texcoords, d->texcoords + 6 * (size_t)np);
     }
     for (Hittable* o : sc->objects)
         if (Mesh* me = dynamic_cast<Mesh*>(o)) me->finish();
@@ -3002,6 +3251,10 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
     PhotonMapO pm;
     bool caus = false;
     float gr = 0.0f;
+    if (st->caustics && st->caustic_photons > 0 && !s->light_ptrs.empty() && s->textured) {
+        g_err = "photon caustics with textured materials are not supported";
+        return RT_ERR_UNSUPPORTED;
+    }
     if (st->caustics && st->caustic_photons > 0 && !s->light_ptrs.empty()) {
         gr = st->caustic_gather_radius > 0.0f ? st->caustic_gather_radius : std::fmax(s->scene_radius * 0.06f, 1e-4f);
         caus = build_photon_map(*s, st->caustic_photons, gr * 0.5f, std::max(1, (int)nthreads), pm);
@@ -3100,6 +3353,81 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
         stats->rays_shadow = seg_shadow.load();
     }
     (void)seg_closest;
+    return 0;
+}
+
+// Filtered texture set at n texture coordinates (uv pairs) -> out[n][5].
+int orc_texture_sample(void* sp, const float* uv, int64_t n, float* out) {
+    Scene* s = (Scene*)sp;
+    if (!s->textured) return RT_ERR_INVALID_ARGUMENT;
+    for (int64_t k = 0; k < n; k++) {
+        int idx[4];
+        float w[4];
+        s->tex_foot(uv[2 * k], uv[2 * k + 1], idx, w);
+        for (int c = 0; c < 5; c++) out[5 * k + c] = s->tex_at(idx, w, c);
+    }
+    return 0;
+}
+
+// FP64 CPU adjoint: d_tex[5][tex_h][tex_w] = sum over pixels and samples of
+// dLoss/dBeauty / spp propagated through the render recursion (see AdjState).
+// Needs a textured scene without medium or transmissive materials and
+// inactive clamps (max_sample_radiance * indirect_clamp_factor >= 1e30).
+int orc_adjoint(void* sp, const rt_settings* st, const float* d_beauty, double* d_tex, int32_t nthreads) {
+    Scene* s = (Scene*)sp;
+    int rc = check_settings(st);
+    if (rc) return rc;
+    if (!s->textured) { g_err = "scene has no textured material"; return RT_ERR_INVALID_ARGUMENT; }
+    if (s->medium || st->caustics) { g_err = "adjoint: no medium or caustics"; return RT_ERR_UNSUPPORTED; }
+    if ((double)st->max_sample_radiance * (double)st->indirect_clamp_factor < 1e30) {
+        g_err = "adjoint: radiance clamps must be inactive";
+        return RT_ERR_UNSUPPORTED;
+    }
+    const int W = st->width, H = st->height, spp = st->samples_per_pixel;
+    const int64_t np = (int64_t)W * H, plane = (int64_t)s->tex_w * s->tex_h;
+    std::atomic<int> next(0);
+    std::mutex mu;
+    std::fill(d_tex, d_tex + 5 * plane, 0.0);
+    auto worker = [&]() {
+        std::vector<double> grad(5 * plane, 0.0);
+        AdjState A;
+        A.grad = grad.data();
+        A.plane = plane;
+        Renderer RR(*s, *st);
+        for (;;) {
+            int j = next.fetch_add(1);
+            if (j >= H) break;
+            for (int i = 0; i < W; i++) {
+                int64_t px = (int64_t)j * W + i;
+                double db[3] = {d_beauty[px], d_beauty[np + px], d_beauty[2 * np + px]};
+                if (db[0] == 0.0 && db[1] == 0.0 && db[2] == 0.0) continue;
+                uint32_t ps = pixel_seed(i, j, st->seed_xor);
+                for (int smp = 0; smp < spp; smp++) {
+                    SamplerCtx& c = t_smp;
+                    c.active = true; c.seed = ps; c.index = (uint32_t)smp; c.dim = 0; c.fixed = nullptr;
+                    float ju = random_float();
+                    float jv = random_float();
+                    float uu = ((float)i + ju) / (float)W;
+                    float vv = ((float)(H - j - 1) + jv) / (float)H;
+                    Ray ray = s->camera.get_ray(uu, vv);
+                    IorStack iors;
+                    for (int k = 0; k < 3; k++) A.aw[k] = db[k] / (double)spp;
+                    t_adj = &A;
+                    RR.trace_ray(ray, RR.max_depth, 0.0f, true, iors, V3(0.0f), V3(1.0f), kCatCamera);
+                    t_adj = nullptr;
+                    c.active = false;
+                }
+            }
+        }
+        std::lock_guard<std::mutex> g(mu);
+        for (int64_t k = 0; k < 5 * plane; k++) d_tex[k] += grad[k];
+    };
+    if (nthreads <= 1) worker();
+    else {
+        std::vector<std::thread> th;
+        for (int k = 0; k < nthreads; k++) th.emplace_back(worker);
+        for (auto& x : th) x.join();
+    }
     return 0;
 }
 

@@ -63,6 +63,8 @@ def lib():
         L.orc_photon_gather.restype = None
         L.orc_photon_free.argtypes = [vp]
         L.orc_photon_free.restype = None
+        L.orc_adjoint.argtypes = [vp, C.POINTER(abi.rt_settings), fp, C.POINTER(C.c_double), C.c_int32]
+        L.orc_texture_sample.argtypes = [vp, fp, C.c_int64, fp]
         _lib = L
     return _lib
 
@@ -134,6 +136,26 @@ class OracleScene:
         pt = np.empty(n, np.int64) if counters else None
         lib().orc_trace_rays(self.h, _f(rays), n, _i(prim), _f(t), _f(u), _f(v), _i64(nt), _i64(pt))
         return (prim, t, u, v, nt, pt) if counters else (prim, t, u, v)
+
+    def texture_sample(self, uv):
+        """Bilinear texture-set values (n, 5) at texture coordinates uv (n, 2)."""
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((len(uv), 5), np.float32)
+        if lib().orc_texture_sample(self.h, _f(uv), len(uv), _f(out)):
+            raise RuntimeError("scene has no texture set")
+        return out
+
+    def adjoint(self, settings, d_beauty, nthreads=None):
+        """FP64 CPU adjoint: gradient of a loss with dLoss/dBeauty =
+        d_beauty (3, H, W) with respect to the texture set (5, th, tw)."""
+        t = self.desc.texture
+        db = np.ascontiguousarray(d_beauty, np.float32).reshape(3, -1)
+        out = np.zeros((5,) + t.shape[1:], np.float64)
+        rc = lib().orc_adjoint(self.h, C.byref(settings), _f(db), out.ctypes.data_as(C.POINTER(C.c_double)),
+                               nthreads or os.cpu_count() or 1)
+        if rc:
+            raise RuntimeError(f"orc_adjoint {rc}: " + lib().orc_last_error().decode())
+        return out
 
     def trace_primary(self, settings, sample_begin=0, sample_count=1):
         n = settings.width * settings.height * sample_count
