@@ -41,6 +41,7 @@ namespace rt {
 // ═════════════════════════════════════════════════════════════════════════
 struct DevScene {
     const GNode* nodes;
+    const GNode4* nodes4;       // any-hit BVH (null: any-hit queries use the binary tree)
     const GEntry* entries;
     const float4* aux;
     const GMat* mats;
@@ -91,18 +92,70 @@ struct JobDev {
 
 // Scene/parameter source of the kernels: the kernel arguments of a single
 // render (uniform, scalar registers) or the path's job record of a batch.
+// run(act, j, f) calls f(scene, params) for the lanes with act set; the
+// batch form loads the job record with scalar loads, once per distinct job
+// among those lanes (a single pass when they share a job, the common case:
+// slots are refilled in sample order and jobs are contiguous sample ranges).
+// It contains ballots, so f must not use wave-level operations.  run_lane
+// is the same with per-lane record loads.
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+    // the record is read through the constant address space: with a
+    // wave-uniform address these become scalar loads into SGPRs
+    typedef const __attribute__((address_space(4))) uint32_t CU;
+    CU* src = (CU*)p;
+    uint32_t buf[sizeof(T) / 4];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++) buf[i] = src[i];
+    T r;
+    __builtin_memcpy(&r, buf, sizeof(T));
+    return r;
+}
 struct OneJob {
     DevScene S;
     RParams R;
     __device__ __forceinline__ int job_of(const int32_t*, int) const { return 0; }
     __device__ __forceinline__ const DevScene& scene(int) const { return S; }
     __device__ __forceinline__ const RParams& params(int) const { return R; }
+    template <class F>
+    __device__ __forceinline__ void run(bool act, int, F&& f) const {
+        if (act) f(S, R);
+    }
+    template <class F>
+    __device__ __forceinline__ void run_lane(bool act, int, F&& f) const {
+        if (act) f(S, R);
+    }
 };
 struct ManyJobs {
     const JobDev* J;
     __device__ __forceinline__ int job_of(const int32_t* job, int p) const { return job[p]; }
     __device__ __forceinline__ const DevScene& scene(int j) const { return J[j].S; }
     __device__ __forceinline__ const RParams& params(int j) const { return J[j].R; }
+    template <class F>
+    __device__ __forceinline__ void run(bool act, int j, F&& f) const {
+        bool done = !act;
+        for (;;) {
+            const unsigned long long pend = __ballot(!done);
+            if (pend == 0ull) break;
+            // the job of the lowest pending lane, as a wave-uniform value; the
+            // pointer goes through an empty asm so the compiler cannot rewrite
+            // it from the per-lane job inside the branch below
+            const int ju = __builtin_amdgcn_readfirstlane(__shfl(j, __ffsll((long long)pend) - 1, 64));
+            const JobDev* pj = J + ju;
+            asm volatile("" : "+s"(pj));
+            if (!done && j == ju) {
+                const JobDev jd = load_uniform(pj);
+                f(jd.S, jd.R);
+                done = true;
+            }
+        }
+    }
+    // per-lane records (vector loads): for kernels whose register pressure
+    // leaves no room for the scalar copy of the record (the shading kernel)
+    template <class F>
+    __device__ __forceinline__ void run_lane(bool act, int j, F&& f) const {
+        if (act) f(J[j].S, J[j].R);
+    }
 };
 
 // counters[] (device-resident loop state):
@@ -734,25 +787,26 @@ __global__ void __launch_bounds__(kBlock) k_trace(JOBS jobs, PathBufs P) {
         if (COUNT) flush_counts(P.tcount, tc);
         return;
     }
-    const DevScene& S = jobs.scene(jobs.job_of(P.job, p));
-    float4 o4 = P.o[p], d4 = P.d[p];
-    f3 o = f3of(o4), d = f3of(d4);
-    bool camera = pf_cat(__float_as_int(d4.w)) == 0;
-    HitOut h;
-    bool moved = false;
-    for (int skip = 0;;) {
-        h = trace_closest<SD, STRIDE, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
-                                             kInfinity, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
-        if (h.entry < 0 || !camera) break;
-        int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
-        if (!(fl & kEntryCamInvisible)) break;
-        o = offset_origin(o + h.t * d, d);
-        moved = true;
-        if (++skip >= 8) break;
-    }
-    if (moved) P.o[p] = make_float4(o.x, o.y, o.z, o4.w);
-    P.hit_e[p] = h.entry;
-    P.hit_tuv[p] = make_float4(h.t, h.u, h.v, 0.0f);
+    jobs.run(true, jobs.job_of(P.job, p), [&](const DevScene& S, const RParams&) {
+        float4 o4 = P.o[p], d4 = P.d[p];
+        f3 o = f3of(o4), d = f3of(d4);
+        bool camera = pf_cat(__float_as_int(d4.w)) == 0;
+        HitOut h;
+        bool moved = false;
+        for (int skip = 0;;) {
+            h = trace_closest<SD, STRIDE, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
+                                                 kInfinity, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
+            if (h.entry < 0 || !camera) break;
+            int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
+            if (!(fl & kEntryCamInvisible)) break;
+            o = offset_origin(o + h.t * d, d);
+            moved = true;
+            if (++skip >= 8) break;
+        }
+        if (moved) P.o[p] = make_float4(o.x, o.y, o.z, o4.w);
+        P.hit_e[p] = h.entry;
+        P.hit_tuv[p] = make_float4(h.t, h.u, h.v, 0.0f);
+    });
     if (COUNT) flush_counts(P.tcount, tc);
 }
 
@@ -793,14 +847,12 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
     const int st_in = p < P.n ? P.state[p] : kSlotIdle;
     const bool active = st_in == kSlotActive;
     const int job = active ? jobs.job_of(P.job, p) : 0;
-    const DevScene& S = jobs.scene(job);
-    const RParams& R = jobs.params(job);
     int n_shadow = 0;
     uint32_t smask = 0;   // bit j: light sample j produced a shadow ray
     bool cont = false;
     bool gather = false;  // a caustic gather request was queued
     // per-lane shadow records are written straight into the path's slots
-    if (active) {
+    jobs.run_lane(active, job, [&](const DevScene& S, const RParams& R) {
         float4 o4 = P.o[p], d4 = P.d[p], thr4 = P.thr[p], wt4 = P.wt[p];
         int32_t flags = __float_as_int(d4.w);
         flags = fold_shadows(P, R, p, flags);
@@ -1326,7 +1378,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                 P.adj_q[5 * (int64_t)p + 4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
         }
-    }
+    });
     // slot state, per-slot outputs and this wave's counts (no atomics: the
     // queues are laid out by k_wave_scan + k_compact from the counts)
     bool retire = active && !cont;
@@ -1498,7 +1550,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPh
 
 // Transparent shadow walk (ShadowRay.Transmittance, ShadowRay.cs:59-158)
 template <class JOBS, int SD, bool COUNT, bool PRIV>
-__global__ void __launch_bounds__(kBlock) k_shadow(JOBS jobs, PathBufs P, int maxs) {
+__global__ void __launch_bounds__(kBlock, 7) k_shadow(JOBS jobs, PathBufs P, int maxs) {
     __shared__ uint2 lds_stk[PRIV ? 1 : SD * kBlock];
     uint2 priv_stk[PRIV ? SD : 1];
     uint2* stk = PRIV ? priv_stk : &lds_stk[threadIdx.x];
@@ -1520,64 +1572,74 @@ __global__ void __launch_bounds__(kBlock) k_shadow(JOBS jobs, PathBufs P, int ma
         return;
     }
     int64_t qi = i;
+    int seg = 0;
     if (grouped) {
-        int j = 0, pre = 0;
+        int pre = 0;
         for (;;) {
-            int c = P.counters[kCntSeg + j];
-            if (i < pre + c || j + 1 >= maxs) break;
+            int c = P.counters[kCntSeg + seg];
+            if (i < pre + c || seg + 1 >= maxs) break;
             pre += c;
-            j++;
+            seg++;
         }
-        qi = (int64_t)j * P.n + (i - pre);
+        qi = (int64_t)seg * P.n + (i - pre);
     }
     int64_t sl = P.q_sh[qi];
-    const DevScene& S = jobs.scene(jobs.job_of(P.job, (int)(sl % P.n)));
-    float4 o4 = P.sh_o[sl], d4 = P.sh_d[sl];
-    f3 org = f3of(o4), dir = f3of(d4);
-    float rem = o4.w;
-    if (S.opaque_shadows) {
-        // Every material is opaque to shadow rays (no Dielectric, no Disney
-        // spec_trans > 0), so ShadowRay.Transmittance is 0 at the first
-        // interface and 1 without one: an any-hit query.
-        HitOut h = trace_closest<SD, STRIDE, COUNT, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org,
-                                                          dir, kEps, rem, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
-        f3 c = h.entry < 0 ? f3of(P.sh_c[sl]) : splat(0.0f);
-        P.sh_r[sl] = make_float4(c.x, c.y, c.z, 0.0f);
-        if (COUNT) flush_counts(P.tcount + 3, tc);
-        return;
-    }
-    f3 thr = splat(1.0f), sigma = splat(0.0f), entry = splat(0.0f);
-    bool in_med = false;
-    bool done = false;
-    for (int it = 0; it < 8 && !done; it++) {
-        HitOut h = trace_closest<SD, STRIDE, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org, dir,
-                                                    kEps, rem, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
-        if (h.entry < 0) { done = true; break; }
-        Surf s = make_surf(S, h.entry, org, dir, h.t, h.u, h.v);
-        const GMat g = S.mats[s.mat];
-        if (S.block_spec && (g.m0.y & kMatSpecTransmissive)) { thr = splat(0.0f); done = true; break; }
-        if (in_med) {
-            float dd = length(s.p - entry);
-            thr = mk(thr.x * expf(-sigma.x * dd), thr.y * expf(-sigma.y * dd), thr.z * expf(-sigma.z * dd));
-            in_med = false;
+    // slot of the record: segment j holds light sample j's records at j*N + slot
+    const int slot = grouped ? (int)(sl - (int64_t)seg * P.n) : (int)(sl % P.n);
+    jobs.run(true, jobs.job_of(P.job, slot), [&](const DevScene& S, const RParams&) {
+        float4 o4 = P.sh_o[sl], d4 = P.sh_d[sl];
+        f3 org = f3of(o4), dir = f3of(d4);
+        float rem = o4.w;
+        if (S.opaque_shadows) {
+            // Every material is opaque to shadow rays (no Dielectric, no Disney
+            // spec_trans > 0), so ShadowRay.Transmittance is 0 at the first
+            // interface and 1 without one: an any-hit query.
+            bool blocked;
+            if (!COUNT && S.nodes4)
+                blocked = trace_any4<SD, STRIDE>(S.nodes4, S.entries, S.aux, S.top_start, S.top_count, org, dir, kEps,
+                                                 rem, stk, (uint32_t*)&P.counters[kCntOverflow]);
+            else
+                blocked = trace_closest<SD, STRIDE, COUNT, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count,
+                                                                 org, dir, kEps, rem, stk,
+                                                                 (uint32_t*)&P.counters[kCntOverflow], tc)
+                              .entry >= 0;
+            f3 c = blocked ? splat(0.0f) : f3of(P.sh_c[sl]);
+            P.sh_r[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+            return;
+        }
+        f3 thr = splat(1.0f), sigma = splat(0.0f), entry = splat(0.0f);
+        bool in_med = false;
+        bool done = false;
+        for (int it = 0; it < 8 && !done; it++) {
+            HitOut h = trace_closest<SD, STRIDE, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org, dir,
+                                                        kEps, rem, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
+            if (h.entry < 0) { done = true; break; }
+            Surf s = make_surf(S, h.entry, org, dir, h.t, h.u, h.v);
+            const GMat g = S.mats[s.mat];
+            if (S.block_spec && (g.m0.y & kMatSpecTransmissive)) { thr = splat(0.0f); done = true; break; }
+            if (in_med) {
+                float dd = length(s.p - entry);
+                thr = mk(thr.x * expf(-sigma.x * dd), thr.y * expf(-sigma.y * dd), thr.z * expf(-sigma.z * dd));
+                in_med = false;
+                if (near_zero(thr)) { thr = splat(0.0f); done = true; break; }
+            }
+            f3 t = mat_shadow_tr(g, dir, s.n);
+            if (near_zero(t)) { thr = splat(0.0f); done = true; break; }
+            thr *= t;
             if (near_zero(thr)) { thr = splat(0.0f); done = true; break; }
+            if (s.front) {
+                f3 sg = mat_shadow_abs(g);
+                if (!near_zero(sg)) { in_med = true; entry = s.p; sigma = sg; }
+            }
+            f3 on = dot(dir, s.n) >= 0.0f ? s.n : -s.n;
+            org = offset_origin(s.p, on);
+            rem -= h.t + 2.0f * kEps;
+            if (rem <= kEps) { done = true; break; }
         }
-        f3 t = mat_shadow_tr(g, dir, s.n);
-        if (near_zero(t)) { thr = splat(0.0f); done = true; break; }
-        thr *= t;
-        if (near_zero(thr)) { thr = splat(0.0f); done = true; break; }
-        if (s.front) {
-            f3 sg = mat_shadow_abs(g);
-            if (!near_zero(sg)) { in_med = true; entry = s.p; sigma = sg; }
-        }
-        f3 on = dot(dir, s.n) >= 0.0f ? s.n : -s.n;
-        org = offset_origin(s.p, on);
-        rem -= h.t + 2.0f * kEps;
-        if (rem <= kEps) { done = true; break; }
-    }
-    if (!done || near_zero(thr)) thr = splat(0.0f);
-    f3 c = f3of(P.sh_c[sl]) * thr;
-    P.sh_r[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+        if (!done || near_zero(thr)) thr = splat(0.0f);
+        f3 c = f3of(P.sh_c[sl]) * thr;
+        P.sh_r[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+    });
     if (COUNT) flush_counts(P.tcount + 3, tc);
 }
 
@@ -1795,6 +1857,28 @@ __global__ void __launch_bounds__(kBlock) k_trace_rays(DevScene S, const float* 
     vo[p] = h.entry < 0 ? 0.0f : hv;
 }
 
+// Occlusion queries (rt_occluded): the any-hit traversal the shadow kernel
+// runs for opaque scenes, on caller-given segments.
+template <int SD>
+__global__ void __launch_bounds__(kBlock) k_occluded(DevScene S, const float* __restrict__ rays, int n, int32_t* occ) {
+    __shared__ uint2 stk[SD * kBlock];
+    int p = xcd_block() * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const float* r = rays + 8 * (int64_t)p;
+    f3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
+    bool b;
+    if (S.nodes4) {
+        b = trace_any4<SD, kBlock>(S.nodes4, S.entries, S.aux, S.top_start, S.top_count, o, d, r[6], r[7],
+                                   &stk[threadIdx.x], (uint32_t*)(occ + n));
+    } else {
+        TravCount tc{0u, 0u, 0u};
+        b = trace_closest<SD, kBlock, false, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, r[6], r[7],
+                                                   &stk[threadIdx.x], (uint32_t*)(occ + n), tc)
+                .entry >= 0;
+    }
+    occ[p] = b ? 1 : 0;
+}
+
 }  // namespace rt
 
 // ═════════════════════════════════════════════════════════════════════════
@@ -1832,6 +1916,7 @@ struct rt_scene {
     rt_ctx* ctx = nullptr;
     HostScene hs;
     GNode* d_nodes = nullptr;
+    GNode4* d_nodes4 = nullptr;
     GEntry* d_entries = nullptr;
     float4* d_aux = nullptr;
     GMat* d_mats = nullptr;
@@ -1872,6 +1957,7 @@ static DevScene dev_view(const rt_scene* sc) {
     const HostScene& h = sc->hs;
     DevScene S;
     S.nodes = sc->d_nodes;
+    S.nodes4 = h.nodes4.empty() ? nullptr : sc->d_nodes4;
     S.entries = sc->d_entries;
     S.aux = sc->d_aux;
     S.mats = sc->d_mats;
@@ -2366,6 +2452,10 @@ int rt_scene_create(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
     // Stack bound: deepest world path + a leaf continuation + the deepest
     // nested mesh path (build_host_scene records the maximum over all BVHs).
     sc->stack_depth = h.world_depth + (h.mesh_depth >= 0 ? 1 + h.mesh_depth : 0) + 1;
+    // the any-hit BVH needs up to 3 pushes per level; use it only when its
+    // bound fits the stacks the binary traversal's depth class provides
+    if (h.any_stack > 64) h.nodes4.clear();
+    else sc->stack_depth = std::max(sc->stack_depth, h.any_stack);
     if (sc->stack_depth > 64) {
         delete sc;
         return fail(ctx, RT_ERR_UNSUPPORTED, "BVH too deep for the 64-entry traversal stack");
@@ -2376,6 +2466,7 @@ int rt_scene_create(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
         return code;
     };
     if ((rc = upload(ctx, &sc->d_nodes, h.nodes.data(), h.nodes.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_nodes4, h.nodes4.data(), h.nodes4.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_entries, h.entries.data(), h.entries.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_aux, h.quad_aux.data(), h.quad_aux.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_mats, h.mats.data(), h.mats.size(), b))) return cleanup(rc);
@@ -2396,6 +2487,7 @@ void rt_scene_destroy(rt_scene* sc) {
     if (!sc) return;
     if (sc->ctx) hipSetDevice(sc->ctx->device);
     hipFree(sc->d_nodes);
+    hipFree(sc->d_nodes4);
     hipFree(sc->d_entries);
     hipFree(sc->d_aux);
     hipFree(sc->d_mats);
@@ -2486,6 +2578,34 @@ int rt_trace_primary(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, int3
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(mem);
     if (e != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string("trace_primary: ") + hipGetErrorString(e));
+    if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+    return RT_OK;
+}
+
+int rt_occluded(rt_ctx* ctx, const rt_scene* sc, const float* rays, int64_t num_rays, int32_t* occluded) {
+    if (!ctx || !sc || num_rays < 0 || (num_rays > 0 && (!rays || !occluded))) return RT_ERR_INVALID_ARGUMENT;
+    if (num_rays == 0) return RT_OK;
+    if (num_rays > (1ll << 31) - 2) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many rays");
+    hipSetDevice(ctx->device);
+    int n = (int)num_rays;
+    DevScene S = dev_view(sc);
+    char* mem = nullptr;
+    HIP_TRY(hipMalloc((void**)&mem, (size_t)n * 32 + 4 * (size_t)(n + 4)));
+    float* d_rays = (float*)mem;
+    int32_t* d_occ = (int32_t*)(mem + (size_t)n * 32);
+    hipError_t e = hipMemcpyAsync(d_rays, rays, (size_t)n * 32, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_occ, 0, (size_t)(n + 4) * 4, ctx->stream);
+    if (e == hipSuccess) {
+        dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+        DISPATCH_SD(sc->stack_depth, k_occluded, grid, ctx->stream, S, d_rays, n, d_occ);
+        e = hipGetLastError();
+    }
+    int32_t ovf = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&ovf, d_occ + n, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(occluded, d_occ, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(mem);
+    if (e != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string("occluded: ") + hipGetErrorString(e));
     if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
     return RT_OK;
 }
@@ -2724,6 +2844,12 @@ static int shadow_slots(const rt_settings* s, const HostScene& h) {
 }
 
 // Renderer constructor arguments (Renderer.cs:300-378) as device parameters.
+// Samples per window for a pool of `pool` slots.  A window's per-sample
+// results stay resident until it completes (16 B per sample, 48 B with
+// AOVs: 4 GiB / 12 GiB at the cap), and every window ends with a drain of
+// the pool, so windows are made as large as HBM comfortably allows.
+static int64_t window_cap(int64_t pool) { return std::max<int64_t>(pool, std::min<int64_t>(1ll << 28, 128 * pool)); }
+
 static RParams make_params(const rt_settings* s, const HostScene& h, int maxs) {
     RParams R{};
     R.W = s->width;
@@ -2798,7 +2924,7 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     // sample order when the window completes.  A slot whose path ends is
     // restarted on the window's next sample, so the pool stays full.
     int64_t target = s->paths_in_flight > 0 ? s->paths_in_flight : (1 << 21);
-    const int64_t kWindowCap = std::max<int64_t>(target, std::min<int64_t>(1ll << 26, 32 * target));
+    const int64_t kWindowCap = window_cap(target);
     int K = (int)std::max<int64_t>(1, std::min<int64_t>(spp, kWindowCap / npix));
     const int64_t WS = (int64_t)K * npix;
     const int64_t N = std::min<int64_t>(target, WS);
@@ -2902,7 +3028,7 @@ int rt_adjoint(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const floa
     const int npix = s->width * s->height;
     const int64_t plane = (int64_t)h.tex_w * h.tex_h;
     int64_t target = s->paths_in_flight > 0 ? s->paths_in_flight : (1 << 21);
-    const int64_t kWindowCap = std::max<int64_t>(target, std::min<int64_t>(1ll << 26, 32 * target));
+    const int64_t kWindowCap = window_cap(target);
     int K = (int)std::max<int64_t>(1, std::min<int64_t>(spp, kWindowCap / npix));
     const int64_t WS = (int64_t)K * npix;
     const int64_t N = std::min<int64_t>(target, WS);
@@ -3011,7 +3137,7 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
     if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
     // windows over the job-major sample sequence, split at pass boundaries
     const int64_t N0 = pool_paths > 0 ? pool_paths : (1 << 21);
-    const int64_t cap = std::max<int64_t>(N0, std::min<int64_t>(1ll << 26, 32 * N0));
+    const int64_t cap = window_cap(N0);
     WindowPlan plan;
     {
         std::vector<PartDev> cur;

@@ -45,6 +45,16 @@ struct alignas(64) GEntry {
     float4 a, b, c, d;
 };
 
+// 4-wide node (128 B, one L2 line) of the any-hit BVH: the binary tree with
+// every internal node's subtree collapsed to up to four children (the
+// largest-area internal child is expanded first).  Child boxes in SoA order;
+// ch = node index or leaf reference (same encoding as the binary tree); an
+// unused slot has an empty box.  A BVH entry's any-hit root is in c.x.
+struct alignas(128) GNode4 {
+    float4 mnx, mny, mnz, mxx, mxy, mxz;
+    int4 ch, pad;
+};
+
 // Material kinds on the device (rt_material_kind) and derived flags.
 enum MatFlags : int32_t {
     kMatThinWalled = 1,

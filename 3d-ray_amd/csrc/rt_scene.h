@@ -16,6 +16,8 @@ struct Box {
 // Host copy of everything rt_scene_create uploads.
 struct HostScene {
     std::vector<GNode> nodes;
+    std::vector<GNode4> nodes4;        // any-hit BVH (collapsed binary tree)
+    int any_stack = 0;                 // traversal stack bound of the any-hit BVH
     std::vector<GEntry> entries;
     std::vector<float4> quad_aux;      // 4 per quad: (U, area) (V, 0) (tangent, 0) (bitangent, 0)
     std::vector<GMat> mats;

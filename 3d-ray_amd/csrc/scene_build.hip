@@ -340,6 +340,104 @@ static GMat make_gmat(const rt_material& m, const float* Eavg) {
     return g;
 }
 
+// ── any-hit BVH: 4-wide collapse of every binary tree ──────────────────
+static float half_area(const Box& b) {
+    f3 e = b.mx - b.mn;
+    return e.x * e.y + e.y * e.z + e.z * e.x;
+}
+static Box child_box(const GNode& n, int k) {
+    Box b;
+    if (k == 0) {
+        b.mn = mk(n.a.x, n.a.y, n.a.z);
+        b.mx = mk(n.a.w, n.b.x, n.b.y);
+    } else {
+        b.mn = mk(n.b.z, n.b.w, n.c.x);
+        b.mx = mk(n.c.y, n.c.z, n.c.w);
+    }
+    return b;
+}
+// Returns the any-hit reference of binary reference `ref` and, in `need`,
+// the stack entries a traversal below it can hold (3 siblings per level,
+// one leaf continuation per nested BVH entry).
+static uint32_t collapse4(HostScene& hs, uint32_t ref, int& need) {
+    if (ref_is_leaf(ref)) {
+        need = 0;
+        uint32_t st = leaf_start(ref), cnt = leaf_count(ref);
+        for (uint32_t k = 0; k < cnt; k++) {
+            GEntry& e = hs.entries[st + k];
+            if ((__builtin_bit_cast(int32_t, e.a.w) & 0xff) != kBvh) continue;
+            int sub = 0;
+            uint32_t r4 = collapse4(hs, (uint32_t)__builtin_bit_cast(int32_t, e.b.w), sub);
+            e.c.x = __builtin_bit_cast(float, (int32_t)r4);
+            need = std::max(need, (k + 1 < cnt ? 1 : 0) + sub);
+        }
+        return ref;
+    }
+    struct C {
+        uint32_t ref;
+        Box box;
+    };
+    std::vector<C> ch;
+    const GNode n = hs.nodes[ref];
+    ch.push_back({(uint32_t)n.d.x, child_box(n, 0)});
+    ch.push_back({(uint32_t)n.d.y, child_box(n, 1)});
+    while (ch.size() < 4) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int i = 0; i < (int)ch.size(); i++)
+            if (!ref_is_leaf(ch[i].ref) && half_area(ch[i].box) > ba) {
+                ba = half_area(ch[i].box);
+                best = i;
+            }
+        if (best < 0) break;
+        const GNode m = hs.nodes[ch[best].ref];
+        ch[best] = {(uint32_t)m.d.x, child_box(m, 0)};
+        ch.insert(ch.begin() + best + 1, C{(uint32_t)m.d.y, child_box(m, 1)});
+    }
+    const uint32_t idx = (uint32_t)hs.nodes4.size();
+    hs.nodes4.push_back(GNode4{});
+    float mnx[4], mny[4], mnz[4], mxx[4], mxy[4], mxz[4];
+    int32_t cr[4];
+    int deepest = 0;
+    for (int k = 0; k < 4; k++) {
+        if (k < (int)ch.size()) {
+            int sub = 0;
+            cr[k] = (int32_t)collapse4(hs, ch[k].ref, sub);
+            deepest = std::max(deepest, sub);
+            mnx[k] = ch[k].box.mn.x; mny[k] = ch[k].box.mn.y; mnz[k] = ch[k].box.mn.z;
+            mxx[k] = ch[k].box.mx.x; mxy[k] = ch[k].box.mx.y; mxz[k] = ch[k].box.mx.z;
+        } else {
+            cr[k] = -1;
+            mnx[k] = mny[k] = mnz[k] = FLT_MAX;
+            mxx[k] = mxy[k] = mxz[k] = -FLT_MAX;
+        }
+    }
+    GNode4& g = hs.nodes4[idx];
+    g.mnx = make_float4(mnx[0], mnx[1], mnx[2], mnx[3]);
+    g.mny = make_float4(mny[0], mny[1], mny[2], mny[3]);
+    g.mnz = make_float4(mnz[0], mnz[1], mnz[2], mnz[3]);
+    g.mxx = make_float4(mxx[0], mxx[1], mxx[2], mxx[3]);
+    g.mxy = make_float4(mxy[0], mxy[1], mxy[2], mxy[3]);
+    g.mxz = make_float4(mxz[0], mxz[1], mxz[2], mxz[3]);
+    g.ch = make_int4(cr[0], cr[1], cr[2], cr[3]);
+    g.pad = make_int4(0, 0, 0, 0);
+    need = (int)ch.size() - 1 + deepest;
+    return idx;
+}
+static void build_any_bvh(HostScene& hs) {
+    hs.nodes4.clear();
+    int need = 0;
+    for (uint32_t t = 0; t < hs.top_count; t++) {
+        GEntry& e = hs.entries[hs.top_start + t];
+        if ((__builtin_bit_cast(int32_t, e.a.w) & 0xff) != kBvh) continue;
+        int sub = 0;
+        uint32_t r4 = collapse4(hs, (uint32_t)__builtin_bit_cast(int32_t, e.b.w), sub);
+        e.c.x = __builtin_bit_cast(float, (int32_t)r4);
+        need = std::max(need, sub);
+    }
+    hs.any_stack = need + 1;
+}
+
 int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
     if (!d) { err = "desc is null"; return RT_ERR_INVALID_ARGUMENT; }
     if (d->num_primitives < 0 || d->num_materials < 0 || d->num_lights < 0 ||
@@ -502,6 +600,7 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
     hs.top_count = (uint32_t)top.size();
     for (auto& e : top) hs.entries.push_back(e);
     if (hs.entries.size() >= kLeafStartMask) { err = "scene too large"; return RT_ERR_UNSUPPORTED; }
+    build_any_bvh(hs);
 
     // Camera basis (Camera.cs:26-51)
     {

@@ -260,6 +260,98 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
     return h;
 }
 
+// Any-hit query on the 4-wide BVH (GNode4): is there a primitive hit with
+// t in [tmin, tmax]?  The answer equals the closest-hit traversal's
+// existence of a hit (every primitive whose boxes the segment enters is
+// tested; the visit order does not matter for a yes/no answer), with a
+// quarter of the dependent node fetches.  Hit children: the first is
+// entered, the others pushed.
+template <int SD, int STRIDE>
+__device__ __forceinline__ bool trace_any4(const GNode4* __restrict__ nodes4, const GEntry* __restrict__ entries,
+                                           const float4* __restrict__ aux, uint32_t top_start, uint32_t top_count,
+                                           f3 o, f3 d, float tmin, float tmax, uint2* stk, uint32_t* overflow) {
+    HitOut h;
+    h.entry = -1;
+    TravCount tc{0u, 0u, 0u};
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float closest = tmax;
+    for (uint32_t ti = 0; ti < top_count; ti++) {
+        uint32_t eidx = top_start + ti;
+        float4 ea = entries[eidx].a, eb = entries[eidx].b, ec = entries[eidx].c;
+        int32_t tag = tag_of(ea);
+        int32_t kind = tag & 0xff, flags = tag >> 8;
+        if (kind != kBvh) {
+            if (test_prim<false>(entries, ea, eb, ec, eidx, kind, flags, o, d, tmin, aux, closest, h, tc)) return true;
+            continue;
+        }
+        float te;
+        if (!(flags & kEntryNoBox) && !slab(f3of(ea), f3of(eb), o, inv, tmin, closest, te)) continue;
+        uint32_t cur = (uint32_t)__float_as_int(ec.x);
+        int sp = 0;
+        for (;;) {
+            if (ref_is_leaf(cur)) {
+                uint32_t start = leaf_start(cur), cnt = leaf_count(cur);
+                bool descended = false;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    uint32_t idx = start + k;
+                    float4 la = entries[idx].a, lb = entries[idx].b, lc = entries[idx].c;
+                    int32_t ltag = tag_of(la);
+                    int32_t lk = ltag & 0xff, lf = ltag >> 8;
+                    if (lk != kBvh) {
+                        if (test_prim<false>(entries, la, lb, lc, idx, lk, lf, o, d, tmin, aux, closest, h, tc))
+                            return true;
+                        continue;
+                    }
+                    float lte;
+                    if (!(lf & kEntryNoBox) && !slab(f3of(la), f3of(lb), o, inv, tmin, closest, lte)) continue;
+                    if (k + 1 < cnt) {
+                        if (sp >= SD) { *overflow = 1u; return false; }
+                        stk[sp * STRIDE] = make_uint2(leaf_ref(idx + 1, cnt - k - 1), 0u);
+                        sp++;
+                    }
+                    cur = (uint32_t)__float_as_int(lc.x);
+                    descended = true;
+                    break;
+                }
+                if (descended) continue;
+            } else {
+                const GNode4 nd = nodes4[cur];
+                const float bx0[4] = {nd.mnx.x, nd.mnx.y, nd.mnx.z, nd.mnx.w};
+                const float by0[4] = {nd.mny.x, nd.mny.y, nd.mny.z, nd.mny.w};
+                const float bz0[4] = {nd.mnz.x, nd.mnz.y, nd.mnz.z, nd.mnz.w};
+                const float bx1[4] = {nd.mxx.x, nd.mxx.y, nd.mxx.z, nd.mxx.w};
+                const float by1[4] = {nd.mxy.x, nd.mxy.y, nd.mxy.z, nd.mxy.w};
+                const float bz1[4] = {nd.mxz.x, nd.mxz.y, nd.mxz.z, nd.mxz.w};
+                const uint32_t cref[4] = {(uint32_t)nd.ch.x, (uint32_t)nd.ch.y, (uint32_t)nd.ch.z, (uint32_t)nd.ch.w};
+                bool have = false;
+                uint32_t next = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    float tk;
+                    if (!slab(mk(bx0[k], by0[k], bz0[k]), mk(bx1[k], by1[k], bz1[k]), o, inv, tmin, closest, tk))
+                        continue;
+                    if (!have) {
+                        next = cref[k];
+                        have = true;
+                    } else {
+                        if (sp >= SD) { *overflow = 1u; return false; }
+                        stk[sp * STRIDE] = make_uint2(cref[k], 0u);
+                        sp++;
+                    }
+                }
+                if (have) {
+                    cur = next;
+                    continue;
+                }
+            }
+            if (sp == 0) break;
+            sp--;
+            cur = stk[sp * STRIDE].x;
+        }
+    }
+    return false;
+}
+
 // HitRecord.U/V of a reported hit: barycentrics for triangles, (alpha,
 // beta) for quads (both returned by the test), Sphere.GetSphereUV
 // (Sphere.cs:260-269) and InfinitePlane's tiled planar UV

@@ -17,8 +17,8 @@ import time
 
 from . import native
 
-POOLS = (1 << 19, 1 << 20, 1 << 21, 1 << 22)
-SHADE_WAVES = (1, 2, 3)
+POOLS = (1 << 21, 1 << 22)   # smaller pools win short trials only (fewer idle slots in the drain)
+SHADE_WAVES = (2, 3)
 STACKS = (1, 2)   # 1 private scratch, 2 LDS
 GRAPHS = (1, 2)   # 1 direct launches, 2 hipGraph replay of the bounce iterations
 
@@ -49,11 +49,12 @@ def _trial(ctx, items, pool, waves, stack, graph, batch):
     return time.perf_counter() - t0
 
 
-def tune(items, batch=True, trial_spp=1, pools=POOLS, shade_waves=SHADE_WAVES, stacks=STACKS, graphs=GRAPHS,
-         use_cache=True, log=None):
-    """Picks (pool, shade waves, stack) for `items` = [(Renderer, w, h, ...)]
-    from timed trial renders at `trial_spp` samples per pixel, applies it to
-    the renderers' context and returns it as a dict."""
+def tune(items, batch=True, trial_spp=4, pools=POOLS, shade_waves=SHADE_WAVES, stacks=STACKS, graphs=GRAPHS,
+         use_cache=True, log=None, repeats=2):
+    """Picks (pool, shade waves, stack, graph) for `items` = [(Renderer, w,
+    h, ...)] from timed trial renders at `trial_spp` samples per pixel (the
+    fastest of `repeats` runs per configuration), applies it to the
+    renderers' context and returns it as a dict."""
     ctx = items[0][0].scene.ctx
     key = f"{signature(items)}:{'batch' if batch else 'plain'}"
     path = _cache_path()
@@ -79,7 +80,7 @@ def tune(items, batch=True, trial_spp=1, pools=POOLS, shade_waves=SHADE_WAVES, s
                 for waves in shade_waves:
                     for stack in stacks:
                         for graph in graphs:
-                            dt = _trial(ctx, items, pool, waves, stack, graph, batch)
+                            dt = min(_trial(ctx, items, pool, waves, stack, graph, batch) for _ in range(repeats))
                             trials.append((dt, pool, waves, stack, graph))
                             if log:
                                 log(f"autotune pool={pool} shade_waves={waves} stack={stack} graph={graph}: "

@@ -62,6 +62,7 @@ def load():
     L.rt_render.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.POINTER(abi.rt_film), C.POINTER(abi.rt_render_stats)]
     L.rt_trace_primary.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.c_int32, C.c_int32, ip, fp, fp, fp]
     L.rt_trace_rays.argtypes = [vp, vp, fp, C.c_int64, ip, fp, fp, fp]
+    L.rt_occluded.argtypes = [vp, vp, fp, C.c_int64, ip]
     i64p = C.POINTER(C.c_int64)
     L.rt_photon_map.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.c_int64, C.c_int64, i64p, i64p, i64p, fp, fp, fp,
                                 ip, ip, ip, fp]
@@ -78,7 +79,7 @@ EXPORTED_SYMBOLS = ("rt_get_version", "rt_material_defaults", "rt_settings_defau
                     "rt_last_error", "rt_scene_create", "rt_scene_destroy", "rt_scene_get_info",
                     "rt_scene_set_textures", "rt_render",
                     "rt_trace_primary", "rt_trace_rays", "rt_photon_map", "rt_render_batch", "rt_context_tune",
-                    "rt_adjoint",
+                    "rt_adjoint", "rt_occluded",
                     "rt_debug_bvh_dump")
 
 
@@ -169,6 +170,14 @@ class Scene:
         t, u, v = (np.empty(n, np.float32) for _ in range(3))
         self.ctx.check(load().rt_trace_rays(self.ctx.h, self.h, _f(rays), n, _i(prim), _f(t), _f(u), _f(v)))
         return prim, t, u, v
+
+    def occluded(self, segs):
+        """Any-hit queries (rt_occluded): segs (n, 8) = origin, direction,
+        tmin, tmax -> bool (n,)."""
+        segs = np.ascontiguousarray(segs, np.float32).reshape(-1, 8)
+        out = np.zeros(len(segs), np.int32)
+        self.ctx.check(load().rt_occluded(self.ctx.h, self.h, _f(segs), len(segs), _i(out)))
+        return out.astype(bool)
 
     def photon_map(self, settings):
         """Device photon map of the caustic pre-pass (rt_photon_map): dict of

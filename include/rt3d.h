@@ -390,6 +390,15 @@ int rt_trace_primary(rt_ctx* ctx, const rt_scene* scene, const rt_settings* sett
                      int32_t sample_begin, int32_t sample_count,
                      int32_t* prim_id, float* t, float* u, float* v);
 
+/* Occlusion of n segments (rays: n x 8 floats = origin, direction, tmin,
+ * tmax): occluded[i] = 1 when any primitive is hit with tmin <= t <= tmax.
+ * The any-hit query the shadow kernel runs for opaque scenes (a 4-wide BVH
+ * collapsed from the binary one); the answer equals the existence of a
+ * closest hit on [tmin, tmax].
+ * Replaces: ShadowRay.Transmittance's occlusion test for opaque materials
+ * (src/RayTracer/Rendering/ShadowRay.cs:59-158, BvhNode.Hit with the segment). */
+int rt_occluded(rt_ctx* ctx, const rt_scene* scene, const float* rays, int64_t num_rays, int32_t* occluded);
+
 /* Closest hit of arbitrary rays (origin xyz, direction xyz per ray, 6 floats)
  * against the scene, tMin = 1e-4, tMax = float.MaxValue, exactly as
  * world.Hit(ray, MathUtils.Epsilon, MathUtils.Infinity) (Renderer.cs:1629).
