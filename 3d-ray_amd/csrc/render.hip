@@ -169,6 +169,8 @@ constexpr int kCntNext = 0, kCntShadow = 1, kCntRetire = 2, kCntGather = 3, kCnt
               kCntOverflow = 63;
 // tcount[] (64-bit): traversal tests (trace node/tri/other, shadow node/tri/other), closest-hit segments, shadow rays
 constexpr int kTcRays = 6, kTcShadowRays = 7;
+// any-hit (4-wide BVH) traversal work of the shadow kernel: node fetches, triangle/quad/plane tests, sphere tests
+constexpr int kTcAnyNode = 8, kTcAnyTri = 9, kTcAnyOther = 10;
 // per-wave count arrays (wc): continuing paths, retired paths, gather
 // requests, shadow rays (ungrouped form), then one per light-sample segment
 constexpr int kWcNext = 0, kWcRetire = 1, kWcGather = 2, kWcShadow = 3, kWcSeg = 4;
@@ -1595,10 +1597,20 @@ __global__ void __launch_bounds__(kBlock, 7) k_shadow(JOBS jobs, PathBufs P, int
             // spec_trans > 0), so ShadowRay.Transmittance is 0 at the first
             // interface and 1 without one: an any-hit query.
             bool blocked;
-            if (!COUNT && S.nodes4)
-                blocked = trace_any4<SD, STRIDE>(S.nodes4, S.entries, S.aux, S.top_start, S.top_count, org, dir, kEps,
-                                                 rem, stk, (uint32_t*)&P.counters[kCntOverflow]);
-            else
+            if (S.nodes4) {
+                TravCount t4{0u, 0u, 0u};
+                blocked = trace_any4<SD, STRIDE, COUNT>(S.nodes4, S.entries, S.aux, S.top_start, S.top_count, org, dir,
+                                                        kEps, rem, stk, (uint32_t*)&P.counters[kCntOverflow], t4);
+                if (COUNT) {
+                    // the reference traversal's work on the binary tree, for
+                    // the algorithmic-bytes figures (same answer)
+                    trace_closest<SD, STRIDE, COUNT, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org,
+                                                           dir, kEps, rem, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
+                    atomicAdd(&P.tcount[kTcAnyNode], (unsigned long long)t4.node);
+                    atomicAdd(&P.tcount[kTcAnyTri], (unsigned long long)t4.tri);
+                    atomicAdd(&P.tcount[kTcAnyOther], (unsigned long long)t4.other);
+                }
+            } else
                 blocked = trace_closest<SD, STRIDE, COUNT, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count,
                                                                  org, dir, kEps, rem, stk,
                                                                  (uint32_t*)&P.counters[kCntOverflow], tc)
@@ -1868,8 +1880,9 @@ __global__ void __launch_bounds__(kBlock) k_occluded(DevScene S, const float* __
     f3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
     bool b;
     if (S.nodes4) {
+        TravCount tc{0u, 0u, 0u};
         b = trace_any4<SD, kBlock>(S.nodes4, S.entries, S.aux, S.top_start, S.top_count, o, d, r[6], r[7],
-                                   &stk[threadIdx.x], (uint32_t*)(occ + n));
+                                   &stk[threadIdx.x], (uint32_t*)(occ + n), tc);
     } else {
         TravCount tc{0u, 0u, 0u};
         b = trace_closest<SD, kBlock, false, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, r[6], r[7],
@@ -2905,6 +2918,9 @@ static void fill_counts(const unsigned long long* tcv, bool counting, int64_t la
         st->shadow_node_tests = (int64_t)tcv[3];
         st->shadow_tri_tests = (int64_t)tcv[4];
         st->shadow_other_tests = (int64_t)tcv[5];
+        st->any_node_fetches = (int64_t)tcv[kTcAnyNode];
+        st->any_tri_tests = (int64_t)tcv[kTcAnyTri];
+        st->any_other_tests = (int64_t)tcv[kTcAnyOther];
     }
 }
 

@@ -49,7 +49,8 @@ struct alignas(64) GEntry {
 // every internal node's subtree collapsed to up to four children (the
 // largest-area internal child is expanded first).  Child boxes in SoA order;
 // ch = node index or leaf reference (same encoding as the binary tree); an
-// unused slot has an empty box.  A BVH entry's any-hit root is in c.x.
+// unused slot holds a point box at 1e30 and an empty leaf.  A BVH entry's
+// any-hit root is in c.x.
 struct alignas(128) GNode4 {
     float4 mnx, mny, mnz, mxx, mxy, mxz;
     int4 ch, pad;

@@ -29,6 +29,20 @@ static float box_area(const Box& b) {
     if (d.x < 0.0f || d.y < 0.0f || d.z < 0.0f) return 0.0f;
     return 2.0f * (d.x * d.y + d.x * d.z + d.y * d.z);
 }
+// The C ABI structs have fixed sizes (C# [StructLayout(Sequential)] and
+// ctypes mirror them field for field).
+static_assert(sizeof(rt_primitive) == 64, "rt_primitive layout");
+static_assert(sizeof(rt_material) == 128, "rt_material layout");
+static_assert(sizeof(rt_light) == 80, "rt_light layout");
+static_assert(sizeof(rt_camera) == 56, "rt_camera layout");
+static_assert(sizeof(rt_medium) == 32, "rt_medium layout");
+static_assert(sizeof(rt_scene_desc) == 176, "rt_scene_desc layout");
+static_assert(sizeof(rt_settings) == 96, "rt_settings layout");
+static_assert(sizeof(rt_film) == 88, "rt_film layout");
+static_assert(sizeof(rt_render_stats) == 152, "rt_render_stats layout");
+static_assert(sizeof(rt_scene_info) == 96, "rt_scene_info layout");
+static_assert(sizeof(rt_batch_job) == 32, "rt_batch_job layout");
+
 static f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
 static int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
 static float ibits(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
@@ -407,9 +421,10 @@ static uint32_t collapse4(HostScene& hs, uint32_t ref, int& need) {
             mnx[k] = ch[k].box.mn.x; mny[k] = ch[k].box.mn.y; mnz[k] = ch[k].box.mn.z;
             mxx[k] = ch[k].box.mx.x; mxy[k] = ch[k].box.mx.y; mxz[k] = ch[k].box.mx.z;
         } else {
-            cr[k] = -1;
-            mnx[k] = mny[k] = mnz[k] = FLT_MAX;
-            mxx[k] = mxy[k] = mxz[k] = -FLT_MAX;
+            // unused slot: a point box far outside any scene, empty leaf
+            cr[k] = (int32_t)leaf_ref(0, 0);
+            mnx[k] = mny[k] = mnz[k] = 1e30f;
+            mxx[k] = mxy[k] = mxz[k] = 1e30f;
         }
     }
     GNode4& g = hs.nodes4[idx];

@@ -260,19 +260,33 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
     return h;
 }
 
+// Box test of the any-hit traversal: the same slab distances as slab(),
+// without the NaN and empty-box guards.  A NaN slab distance only arises for
+// an axis-parallel ray whose origin lies exactly on a box plane (the
+// hardware min/max then drop it and the box is entered, where the reference
+// would skip it); unused child slots hold a box far outside the scene whose
+// reference is an empty leaf.
+__device__ __forceinline__ bool slab_any(f3 mn_, f3 mx_, f3 o, f3 inv, float tmin, float tmax) {
+    float t0x = (mn_.x - o.x) * inv.x, t0y = (mn_.y - o.y) * inv.y, t0z = (mn_.z - o.z) * inv.z;
+    float t1x = (mx_.x - o.x) * inv.x, t1y = (mx_.y - o.y) * inv.y, t1z = (mx_.z - o.z) * inv.z;
+    float te = fmaxf(tmin, fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z))));
+    float tx = fminf(tmax, fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z))));
+    return tx >= te;
+}
+
 // Any-hit query on the 4-wide BVH (GNode4): is there a primitive hit with
 // t in [tmin, tmax]?  The answer equals the closest-hit traversal's
 // existence of a hit (every primitive whose boxes the segment enters is
 // tested; the visit order does not matter for a yes/no answer), with a
 // quarter of the dependent node fetches.  Hit children: the first is
 // entered, the others pushed.
-template <int SD, int STRIDE>
+template <int SD, int STRIDE, bool COUNT = false>
 __device__ __forceinline__ bool trace_any4(const GNode4* __restrict__ nodes4, const GEntry* __restrict__ entries,
                                            const float4* __restrict__ aux, uint32_t top_start, uint32_t top_count,
-                                           f3 o, f3 d, float tmin, float tmax, uint2* stk, uint32_t* overflow) {
+                                           f3 o, f3 d, float tmin, float tmax, uint2* stk, uint32_t* overflow,
+                                           TravCount& tc) {
     HitOut h;
     h.entry = -1;
-    TravCount tc{0u, 0u, 0u};
     f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float closest = tmax;
     for (uint32_t ti = 0; ti < top_count; ti++) {
@@ -281,10 +295,11 @@ __device__ __forceinline__ bool trace_any4(const GNode4* __restrict__ nodes4, co
         int32_t tag = tag_of(ea);
         int32_t kind = tag & 0xff, flags = tag >> 8;
         if (kind != kBvh) {
-            if (test_prim<false>(entries, ea, eb, ec, eidx, kind, flags, o, d, tmin, aux, closest, h, tc)) return true;
+            if (test_prim<COUNT>(entries, ea, eb, ec, eidx, kind, flags, o, d, tmin, aux, closest, h, tc)) return true;
             continue;
         }
         float te;
+        if (COUNT && !(flags & kEntryNoBox)) tc.node++;
         if (!(flags & kEntryNoBox) && !slab(f3of(ea), f3of(eb), o, inv, tmin, closest, te)) continue;
         uint32_t cur = (uint32_t)__float_as_int(ec.x);
         int sp = 0;
@@ -298,11 +313,12 @@ __device__ __forceinline__ bool trace_any4(const GNode4* __restrict__ nodes4, co
                     int32_t ltag = tag_of(la);
                     int32_t lk = ltag & 0xff, lf = ltag >> 8;
                     if (lk != kBvh) {
-                        if (test_prim<false>(entries, la, lb, lc, idx, lk, lf, o, d, tmin, aux, closest, h, tc))
+                        if (test_prim<COUNT>(entries, la, lb, lc, idx, lk, lf, o, d, tmin, aux, closest, h, tc))
                             return true;
                         continue;
                     }
                     float lte;
+                    if (COUNT && !(lf & kEntryNoBox)) tc.node++;
                     if (!(lf & kEntryNoBox) && !slab(f3of(la), f3of(lb), o, inv, tmin, closest, lte)) continue;
                     if (k + 1 < cnt) {
                         if (sp >= SD) { *overflow = 1u; return false; }
@@ -316,6 +332,7 @@ __device__ __forceinline__ bool trace_any4(const GNode4* __restrict__ nodes4, co
                 if (descended) continue;
             } else {
                 const GNode4 nd = nodes4[cur];
+                if (COUNT) tc.node++;   // one 128-B node fetch
                 const float bx0[4] = {nd.mnx.x, nd.mnx.y, nd.mnx.z, nd.mnx.w};
                 const float by0[4] = {nd.mny.x, nd.mny.y, nd.mny.z, nd.mny.w};
                 const float bz0[4] = {nd.mnz.x, nd.mnz.y, nd.mnz.z, nd.mnz.w};
@@ -327,8 +344,7 @@ __device__ __forceinline__ bool trace_any4(const GNode4* __restrict__ nodes4, co
                 uint32_t next = 0;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    float tk;
-                    if (!slab(mk(bx0[k], by0[k], bz0[k]), mk(bx1[k], by1[k], bz1[k]), o, inv, tmin, closest, tk))
+                    if (!slab_any(mk(bx0[k], by0[k], bz0[k]), mk(bx1[k], by1[k], bz1[k]), o, inv, tmin, closest))
                         continue;
                     if (!have) {
                         next = cref[k];

@@ -96,7 +96,8 @@ class rt_render_stats(C.Structure):
                 ("rays_closest", C.c_int64), ("rays_shadow", C.c_int64), ("trace_launches", C.c_int64),
                 ("shadow_launches", C.c_int64), ("trace_node_tests", C.c_int64), ("trace_tri_tests", C.c_int64),
                 ("trace_other_tests", C.c_int64), ("shadow_node_tests", C.c_int64), ("shadow_tri_tests", C.c_int64),
-                ("shadow_other_tests", C.c_int64)]
+                ("shadow_other_tests", C.c_int64), ("any_node_fetches", C.c_int64), ("any_tri_tests", C.c_int64),
+                ("any_other_tests", C.c_int64)]
 
 
 class rt_batch_job(C.Structure):

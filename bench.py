@@ -346,6 +346,7 @@ def main():
     per = {"trace": agg["trace_ms"], "shadow": agg["shadow_ms"], "shade": agg["shade_ms"]}
     dom = max(per, key=per.get)
     roof = None
+    ref_b = None
     if counts is not None:
         if dom == "trace":
             b = traversal_bytes(counts.trace_node_tests, counts.trace_tri_tests, counts.trace_other_tests,
@@ -355,6 +356,12 @@ def main():
             b = traversal_bytes(counts.shadow_node_tests, counts.shadow_tri_tests, counts.shadow_other_tests,
                                 counts.rays_shadow)
             launches = counts.shadow_launches
+            if counts.any_node_fetches > 0:
+                # the kernel's own any-hit traversal on the 4-wide BVH: 128-B
+                # node fetches; the reference-traversal figure is kept beside it
+                ref_b = b
+                b = (128.0 * counts.any_node_fetches + 64.0 * counts.any_tri_tests + 16.0 * counts.any_other_tests
+                     + 48.0 * counts.rays_shadow)
         else:
             # material record + path state read/write per shading event
             b = (96.0 + 2 * 128.0) * counts.rays_closest
@@ -369,6 +376,11 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_{dom}",
                 "bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(t_avg_s * 1000.0, 4),
                 "launches_per_step": launches}
+        if ref_b is not None:
+            roof["reference_traversal"] = {
+                "bytes_per_launch": round(ref_b / max(launches, 1)),
+                "gbps": round(ref_b / max(launches, 1) / t_avg_s / 1e9, 1),
+                "note": "same per-unit bytes over the reference traversal's visits on the binary tree"}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -279,6 +279,10 @@ typedef struct rt_render_stats {
        closest-hit and shadow kernels */
     int64_t trace_node_tests, trace_tri_tests, trace_other_tests;
     int64_t shadow_node_tests, shadow_tri_tests, shadow_other_tests;
+    /* with count_traversal: the work the shadow kernel's own any-hit
+       traversal does on the 4-wide BVH (128-B node fetches, triangle/quad/
+       plane tests, sphere tests); zero when it uses the binary tree */
+    int64_t any_node_fetches, any_tri_tests, any_other_tests;
 } rt_render_stats;
 
 typedef struct rt_scene_info {

@@ -37,6 +37,11 @@ def test_struct_sizes():
     assert C.sizeof(abi.rt_material) == 128
     assert C.sizeof(abi.rt_light) == 80
     assert C.sizeof(abi.rt_settings) == 96
+    # the same sizes are static_asserts in the library (scene_build.hip)
+    sizes = {"rt_camera": 56, "rt_medium": 32, "rt_scene_desc": 176, "rt_film": 88, "rt_render_stats": 152,
+             "rt_scene_info": 96, "rt_batch_job": 32}
+    for name, size in sizes.items():
+        assert C.sizeof(getattr(abi, name)) == size, name
 
 
 def test_version_and_defaults(native):
