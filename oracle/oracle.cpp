@@ -2423,6 +2423,10 @@ struct Renderer {
     }
     // GatherCaustics (Renderer.cs:1449-1508) + CausticOccupancyWeight (:1520-1526)
     V3 gather_caustics(const HitRecord& rec, V3 view, const Material* m) const {
+        return caustic_estimate(*pmap, rec, view, m, gather_r);
+    }
+    static V3 caustic_estimate(const PhotonMapO& map, const HitRecord& rec, V3 view, const Material* m, float gather_r) {
+        const PhotonMapO* pmap = &map;
         const int K = 40;
         int hi[K];
         float hd[K];
@@ -2509,19 +2513,24 @@ struct Renderer {
         return V3(std::exp(-sc.sig_t.x * d), std::exp(-sc.sig_t.y * d), std::exp(-sc.sig_t.z * d));
     }
     // HomogeneousMedium.Sample (HomogeneousMedium.cs:50-76)
-    bool medium_sample(float tmax, float& t, V3& beta) const {
+    bool medium_sample(float tmax, float& t, V3& beta) const { return medium_sample_of(sc.sig_t, sc.sig_s, tmax, t, beta); }
+    static V3 medium_tr_of(V3 sig_t, float tmax) {
+        float d = std::fmin(std::fmax(tmax, 0.0f), 1e30f);
+        return V3(std::exp(-sig_t.x * d), std::exp(-sig_t.y * d), std::exp(-sig_t.z * d));
+    }
+    static bool medium_sample_of(V3 sig_t, V3 sig_s, float tmax, float& t, V3& beta) {
         int ch = (int)(random_float() * 3.0f);
         if (ch > 2) ch = 2;
-        float st = ch == 0 ? sc.sig_t.x : (ch == 1 ? sc.sig_t.y : sc.sig_t.z);
+        float st = ch == 0 ? sig_t.x : (ch == 1 ? sig_t.y : sig_t.z);
         if (st <= 0.0f) { t = tmax; beta = V3(1.0f); return false; }
         float dist = -std::log(1.0f - random_float()) / st;
         bool scat = dist < tmax;
         t = scat ? dist : tmax;
-        V3 tr = medium_tr(t);
-        V3 dens = scat ? sc.sig_t * tr : tr;
+        V3 tr = medium_tr_of(sig_t, t);
+        V3 dens = scat ? sig_t * tr : tr;
         float pdf = (dens.x + dens.y + dens.z) / 3.0f;
         if (pdf < 1e-20f) pdf = 1.0f;
-        beta = scat ? (tr * sc.sig_s / pdf) : (tr / pdf);
+        beta = scat ? (tr * sig_s / pdf) : (tr / pdf);
         return scat;
     }
 
@@ -3356,6 +3365,39 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
     return 0;
 }
 
+// HomogeneousMedium.Sample with explicit coefficients and random numbers
+// rnd[0] (channel pick), rnd[1] (distance): t, beta[3], scattered.
+int orc_medium_sample(const float* sig_t, const float* sig_s, float tmax, const float* rnd, float* t, float* beta,
+                      int32_t* scat) {
+    SamplerCtx& c = t_smp;
+    c.fixed = rnd; c.nfixed = 2; c.fixed_pos = 0;
+    V3 b;
+    bool sc = Renderer::medium_sample_of(V3(sig_t[0], sig_t[1], sig_t[2]), V3(sig_s[0], sig_s[1], sig_s[2]), tmax, *t, b);
+    c.fixed = nullptr;
+    beta[0] = b.x; beta[1] = b.y; beta[2] = b.z;
+    *scat = sc ? 1 : 0;
+    return 0;
+}
+
+// GatherCaustics at n points of a surface with normal nrm[k] and view
+// direction view[k], Lambertian BRDF of albedo `albedo`, gather-radius cap
+// max_r (the photon map from orc_photon_map_from / orc_photon_build).
+int orc_caustic_estimate(void* h, const float* pts, const float* nrm, const float* view, int64_t n, const float* albedo,
+                         float max_r, float* out) {
+    const PhotonMapO* map = (const PhotonMapO*)h;
+    Lambertian lam(V3(albedo[0], albedo[1], albedo[2]));
+    for (int64_t k = 0; k < n; k++) {
+        HitRecord rec;
+        rec.point = V3(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
+        rec.normal = V3(nrm[3 * k], nrm[3 * k + 1], nrm[3 * k + 2]);
+        rec.front_face = true;
+        rec.mat = &lam;
+        V3 e = Renderer::caustic_estimate(*map, rec, V3(view[3 * k], view[3 * k + 1], view[3 * k + 2]), &lam, max_r);
+        out[3 * k] = e.x; out[3 * k + 1] = e.y; out[3 * k + 2] = e.z;
+    }
+    return 0;
+}
+
 // Filtered texture set at n texture coordinates (uv pairs) -> out[n][5].
 int orc_texture_sample(void* sp, const float* uv, int64_t n, float* out) {
     Scene* s = (Scene*)sp;
@@ -3482,6 +3524,15 @@ void* orc_photon_map_from(const float* pos, int64_t n, float cell) {
     }
     m->build(cell);
     return m;
+}
+
+// Sets every photon's power and incoming direction (synthetic maps).
+void orc_photon_set_all(void* h, const float* power, const float* inc) {
+    PhotonMapO* m = (PhotonMapO*)h;
+    for (auto& p : m->ph) {
+        p.pow = V3(power[0], power[1], power[2]);
+        p.inc = V3(inc[0], inc[1], inc[2]);
+    }
 }
 
 // k-nearest gather for a batch of points: count, kernel radius and the

@@ -65,6 +65,10 @@ def lib():
         L.orc_photon_free.restype = None
         L.orc_adjoint.argtypes = [vp, C.POINTER(abi.rt_settings), fp, C.POINTER(C.c_double), C.c_int32]
         L.orc_texture_sample.argtypes = [vp, fp, C.c_int64, fp]
+        L.orc_medium_sample.argtypes = [fp, fp, C.c_float, fp, fp, fp, ip]
+        L.orc_photon_set_all.argtypes = [vp, fp, fp]
+        L.orc_photon_set_all.restype = None
+        L.orc_caustic_estimate.argtypes = [vp, fp, fp, fp, C.c_int64, fp, C.c_float, fp]
         _lib = L
     return _lib
 
@@ -309,6 +313,10 @@ class PhotonMap:
             nb <<= 1
         return cls(h, len(pos), nb)
 
+    def set_all(self, power, inc):
+        """Gives every photon the same power (3,) and incoming direction (3,)."""
+        lib().orc_photon_set_all(self.h, _f(np.asarray(power, np.float32)), _f(np.asarray(inc, np.float32)))
+
     def gather(self, pts, k, max_r):
         pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
         n = len(pts)
@@ -322,3 +330,26 @@ class PhotonMap:
         if getattr(self, "h", None):
             lib().orc_photon_free(self.h)
             self.h = None
+
+
+def medium_sample(sig_t, sig_s, tmax, u_channel, u_dist):
+    """HomogeneousMedium.Sample with fixed random numbers: (t, beta (3,), scattered)."""
+    st = np.ascontiguousarray(sig_t, np.float32)
+    ss = np.ascontiguousarray(sig_s, np.float32)
+    rnd = np.array([u_channel, u_dist], np.float32)
+    t = np.zeros(1, np.float32)
+    beta = np.zeros(3, np.float32)
+    sc = np.zeros(1, np.int32)
+    lib().orc_medium_sample(_f(st), _f(ss), float(tmax), _f(rnd), _f(t), _f(beta), _i(sc))
+    return float(t[0]), beta, bool(sc[0])
+
+
+def caustic_estimate(pmap, points, normals, views, albedo, max_r):
+    """GatherCaustics with a Lambertian BRDF at each point: (n, 3)."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    nrm = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    vw = np.ascontiguousarray(views, np.float32).reshape(-1, 3)
+    alb = np.ascontiguousarray(albedo, np.float32)
+    out = np.zeros((len(pts), 3), np.float32)
+    lib().orc_caustic_estimate(pmap.h, _f(pts), _f(nrm), _f(vw), len(pts), _f(alb), float(max_r), _f(out))
+    return out
