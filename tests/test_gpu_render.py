@@ -191,3 +191,16 @@ def test_render_stats_and_traversal_counts(native, oracle, gpu_ctx):
     _, _, _, _, nt, pt = oracle.OracleScene(desc).trace_primary(abi.settings_defaults(W, H, 2, 1), 0, 2)
     assert st.trace_node_tests == nt
     assert st.trace_tri_tests + st.trace_other_tests == pt
+
+
+@pytest.mark.parametrize("name", ["cornell", "icosphere"])
+def test_16k_spp_render_matches_oracle(native, oracle, gpu_ctx, name):
+    # relative RMSE of the GPU render against the oracle's 16k-spp render
+    # of the same sampler streams (every Sobol index up to 16384 replayed)
+    W, H, spp = 16, 16, 16384
+    desc = scenes.cornell(W, H) if name == "cornell" else scenes.displaced_icosphere_scene(W, H, subdiv=3)
+    g = native.Renderer(native.Scene(gpu_ctx, desc), spp, 8).render(W, H).buffers.beauty.astype(np.float64)
+    c = oracle.OracleScene(desc).render(abi.settings_defaults(W, H, spp, 8))["beauty"].astype(np.float64)
+    rel = np.sqrt(((g - c) ** 2).mean()) / np.sqrt((c ** 2).mean())
+    print(f"{name}: relative RMSE vs the oracle at {spp} spp {rel:.2e}", flush=True)
+    assert rel <= 1e-3
