@@ -3151,36 +3151,65 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
         traced = traced || b.settings->max_depth > 0;
     }
     if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
-    // windows over the job-major sample sequence, split at pass boundaries
+    // Segments of the batch, in job order: a job of at least two pools of
+    // samples runs alone (single-job kernels, its windows like rt_render's);
+    // smaller jobs are packed job-major into shared windows (batch kernels),
+    // cut at pass boundaries, so they fill the pool together.
     const int64_t N0 = pool_paths > 0 ? pool_paths : (1 << 21);
     const int64_t cap = window_cap(N0);
-    WindowPlan plan;
+    struct Segment {
+        int job;   // >= 0: that job alone; -1: packed jobs
+        WindowPlan plan;
+    };
+    std::vector<Segment> segs;
     {
         std::vector<PartDev> cur;
         int64_t used = 0;
+        WindowPlan packed;
+        auto flush_packed = [&]() {
+            if (!cur.empty()) packed.add_window(cur, (int)used);
+            cur.clear();
+            used = 0;
+            if (!packed.ws.empty()) segs.push_back(Segment{-1, packed});
+            packed = WindowPlan();
+        };
         for (int j = 0; j < n_jobs; j++) {
             const rt_settings* st = jobs[j].settings;
             const int64_t npix = (int64_t)st->width * st->height;
+            const int spp = st->samples_per_pixel;
+            if (npix * spp >= 2 * N0) {
+                flush_packed();
+                Segment sg{j, WindowPlan()};
+                const int K = (int)std::max<int64_t>(1, std::min<int64_t>(spp, cap / npix));
+                if ((int64_t)K * npix > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
+                for (int base = 0; base < spp; base += K) {
+                    const int passes = std::min(K, spp - base);
+                    sg.plan.add_window({PartDev{j, base, 0, (int)npix, passes, {0, 0, 0}}}, passes * (int)npix);
+                }
+                segs.push_back(sg);
+                continue;
+            }
             int pass = 0;
-            while (pass < st->samples_per_pixel) {
+            while (pass < spp) {
                 int64_t room = (cap - used) / npix;
                 if (room <= 0 && !cur.empty()) {
-                    plan.add_window(cur, (int)used);
+                    packed.add_window(cur, (int)used);
                     cur.clear();
                     used = 0;
                     continue;
                 }
-                int np = (int)std::max<int64_t>(1, std::min<int64_t>(room, st->samples_per_pixel - pass));
+                int np = (int)std::max<int64_t>(1, std::min<int64_t>(room, spp - pass));
                 if (used + np * npix > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
                 cur.push_back(PartDev{j, pass, (int)used, (int)npix, np, {0, 0, 0}});
                 used += np * npix;
                 pass += np;
             }
         }
-        if (!cur.empty()) plan.add_window(cur, (int)used);
+        flush_packed();
     }
     int64_t WS = 1;
-    for (int w : plan.ws) WS = std::max<int64_t>(WS, w);
+    for (const Segment& sg : segs)
+        for (int w : sg.plan.ws) WS = std::max<int64_t>(WS, w);
     const int64_t N = std::min<int64_t>(N0, WS);
     if (N > (1ll << 30) || (int64_t)maxs * N > (1ll << 31) - 1)
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "path pool too large");
@@ -3222,14 +3251,21 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
     HIP_TRY(hipEventRecord(ev[6], ctx->stream));
     LoopTiming tm;
     ManyJobs many{(const JobDev*)ctx->jobs_mem};
-    rc = run_windows(ctx, many, P, maxs, traced, counting, stack_depth, nullptr, nullptr, plan,
-                     [&](const PartDev& pt) {
-                         const int64_t npix = pt.npix;
-                         hipLaunchKernelGGL(k_accum, dim3((unsigned)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                                            ctx->stream, R[pt.job], P, F[pt.job], pt.pass0, pt.npass, pt.off);
-                     },
-                     tm);
-    if (rc) return rc;
+    auto accum = [&](const PartDev& pt) {
+        const int64_t npix = pt.npix;
+        hipLaunchKernelGGL(k_accum, dim3((unsigned)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
+                           R[pt.job], P, F[pt.job], pt.pass0, pt.npass, pt.off);
+    };
+    for (const Segment& sg : segs) {
+        if (sg.job >= 0) {
+            OneJob one{jd[sg.job].S, R[sg.job]};
+            rc = run_windows(ctx, one, P, maxs, jobs[sg.job].settings->max_depth > 0, counting, stack_depth, nullptr,
+                             nullptr, sg.plan, accum, tm);
+        } else {
+            rc = run_windows(ctx, many, P, maxs, traced, counting, stack_depth, nullptr, nullptr, sg.plan, accum, tm);
+        }
+        if (rc) return rc;
+    }
     for (int j = 0; j < n_jobs; j++) {
         rc = finish_film(ctx, R[j], F[j], jobs[j].settings, jobs[j].film);
         if (rc) return rc;

@@ -375,9 +375,10 @@ typedef struct rt_batch_job {
 } rt_batch_job;
 
 /* Renders n_jobs (scene, settings) pairs through one path pool: the camera
- * samples of all jobs stream through the same device-driven bounce loop,
- * every path carrying its job, so small and large scenes fill the GPU
- * together.  Each job's film is identical to rt_render of that job (same
+ * samples of the small jobs (fewer than two pools of samples) stream through
+ * the same device-driven bounce loop together, every path carrying its job,
+ * so they fill the GPU together; a larger job runs alone in the pool with
+ * the single-scene kernels, in job order, without a host round trip.  Each job's film is identical to rt_render of that job (same
  * per-sample values, same summation order).  pool_paths = path slots of the
  * pool (0 = library default).  total (optional) receives summed counts and
  * the batch's timings. */
