@@ -17,7 +17,7 @@ import time
 
 from . import native
 
-POOLS = (1 << 21, 1 << 22)   # smaller pools win short trials only (fewer idle slots in the drain)
+POOLS = (1 << 22, 1 << 23, 1 << 24, 1 << 25)
 SHADE_WAVES = (2, 3)
 STACKS = (1, 2)   # 1 private scratch, 2 LDS
 GRAPHS = (1, 2)   # 1 direct launches, 2 hipGraph replay of the bounce iterations

@@ -249,6 +249,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--subdiv", type=int, default=8)
+    ap.add_argument("--pool", type=int, default=0, help="path slots in flight (0 = library default, 2^21)")
     ap.add_argument("--workload", choices=("icosphere", "caustics", "batch", "adjoint"), default="icosphere",
                     help="icosphere: the 1.31M-triangle Disney scene (headline); caustics: glass spheres, "
                          "Gerstner water, fog, photon-map caustics; batch: 64 scenes of uneven size, batched "
@@ -305,7 +306,8 @@ def main():
     sc = native.Scene(ctx, desc)
     log(f"scene resident: {ntri} triangles, build+upload {time.perf_counter() - t0:.2f} s, "
         f"{sc.info().device_bytes / 1e6:.0f} MB")
-    r = native.Renderer(sc, spp, depth, enable_caustics=caus, caustic_photons=args.photons if caus else 0,
+    r = native.Renderer(sc, spp, depth, paths_in_flight=args.pool, enable_caustics=caus,
+                        caustic_photons=args.photons if caus else 0,
                         caustic_gather_radius=args.gather_radius if caus else 0.0)
     s = r.settings
     s.width, s.height = W, H
