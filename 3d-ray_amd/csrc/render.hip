@@ -166,7 +166,7 @@ struct ManyJobs {
 //   the current iteration; per render: stack-overflow flag.
 constexpr int kCntNext = 0, kCntShadow = 1, kCntRetire = 2, kCntGather = 3, kCntSeg = 4, kMaxSeg = 32,
               kCntClear = 40, kCntSid = 40, kCntWs = 41, kCntActive = 42, kCntSidSnap = 43, kCntWindow = 48,
-              kCntOverflow = 63;
+              kCntPend = 52, kCntOverflow = 63;
 // tcount[] (64-bit): traversal tests (trace node/tri/other, shadow node/tri/other), closest-hit segments, shadow rays
 constexpr int kTcRays = 6, kTcShadowRays = 7;
 // any-hit (4-wide BVH) traversal work of the shadow kernel: node fetches, triangle/quad/plane tests, sphere tests
@@ -175,6 +175,8 @@ constexpr int kTcAnyNode = 8, kTcAnyTri = 9, kTcAnyOther = 10;
 // requests, shadow rays (ungrouped form), then one per light-sample segment
 constexpr int kWcNext = 0, kWcRetire = 1, kWcGather = 2, kWcShadow = 3, kWcSeg = 4;
 constexpr int kCntSidBase = 44;   // first window sample of this iteration's restarts
+// kCntPend..+2 (written by k_batch_end for the host): paths pending in the next
+// iteration, the stack-overflow flag, and whether the window's samples are all issued
 constexpr int kIterBatch = 4;   // bounce iterations enqueued between host checks
 enum : int32_t { kSlotIdle = 0, kSlotActive = 1, kSlotRetired = 2 };
 
@@ -196,8 +198,8 @@ struct PathBufs {
     int32_t* state;             // per slot: kSlotIdle, kSlotActive (to trace), kSlotRetired
     int32_t* gflag;             // per slot: a caustic gather request was queued this iteration
     int32_t* q_g;               // gather queue: slots with a request
-    int32_t* wc;                // per-wave counts, kWc* arrays of nw = N/64 entries, scanned in place
-    int nw;
+    int32_t* wc;                // per-wave counts, kWc* arrays of ceil(N/64) entries, scanned in place
+    int nw;                     // row stride of wc: ceil(N/64) rounded up to a multiple of 4
     int32_t* smask;             // per slot: light samples with a pending shadow ray (grouped form)
     int32_t* q_sh;              // shadow queue: shadow record indices
     int32_t* sid;               // sample of the window each path slot is tracing
@@ -773,6 +775,19 @@ __global__ void __launch_bounds__(kBlock) k_iter_begin(PathBufs P, int traced) {
     }
     __syncthreads();
     for (int k = threadIdx.x; k < kCntClear; k += kBlock) P.counters[k] = 0;
+}
+
+// End of a batch of iterations: the paths pending in the next iteration
+// (k_iter_begin's count, without its side effects), the overflow flag and
+// whether every sample of the window has been issued, for one host copy.
+__global__ void __launch_bounds__(kBlock) k_batch_end(PathBufs P) {
+    if (threadIdx.x == 0) {
+        const int sid = P.counters[kCntSid], snap = P.counters[kCntSidSnap], ws = P.counters[kCntWs];
+        const int started = min(sid - snap, max(0, ws - snap));
+        P.counters[kCntPend] = P.counters[kCntNext] + started;
+        P.counters[kCntPend + 1] = P.counters[kCntOverflow];
+        P.counters[kCntPend + 2] = sid >= ws ? 1 : 0;
+    }
 }
 
 // Traversal stack placement: LDS (lane-interleaved) or the wave's private
@@ -1422,36 +1437,75 @@ __global__ void __launch_bounds__(kBlock) k_count_retired(PathBufs P, int narr) 
 // Exclusive scan of one per-wave count array (block a scans array a) and
 // its total into the counters; the retire array also advances the window's
 // sample counter (the restarted slots take samples [sid, sid + total)).
+// Each thread owns kScanPer consecutive counts, read and written as int4
+// (a wave covers 8 KB of whole cache lines per round); a round scans
+// 1024 * kScanPer counts and the rounds carry the running total.
+constexpr int kScanPer = 32;
 __global__ void __launch_bounds__(1024) k_wave_scan(PathBufs P) {
     __shared__ int32_t part[1024 / 64];
-    __shared__ int32_t blocksum;
     const int a = blockIdx.x;
-    int32_t* x = P.wc + (int64_t)a * P.nw;
-    const int nw = P.nw;
-    const int per = (nw + 1023) / 1024;
-    const int b0 = threadIdx.x * per, b1 = min(nw, b0 + per);
-    int t = 0;
-    for (int i = b0; i < b1; i++) t += x[i];
+    int32_t* x = P.wc + (int64_t)a * P.nw;          // row stride P.nw: a multiple of 4
+    const int nwr = (P.n + kBlock - 1) / kBlock;    // counts written this iteration
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int incl = t;
-    for (int off = 1; off < 64; off <<= 1) {
-        int y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
+    int carry = 0;
+    for (int r0 = 0; r0 < nwr; r0 += 1024 * kScanPer) {
+        const int b0 = r0 + (int)threadIdx.x * kScanPer;
+        int v[kScanPer];
+#pragma unroll
+        for (int k = 0; k < kScanPer; k += 4) {
+            const int i = b0 + k;
+            int4 q = make_int4(0, 0, 0, 0);
+            if (i + 3 < nwr) {
+                q = *(const int4*)(x + i);
+            } else if (i < nwr) {
+                q.x = x[i];
+                if (i + 1 < nwr) q.y = x[i + 1];
+                if (i + 2 < nwr) q.z = x[i + 2];
+            }
+            v[k] = q.x;
+            v[k + 1] = q.y;
+            v[k + 2] = q.z;
+            v[k + 3] = q.w;
+        }
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; k++) t += v[k];
+        int incl = t;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        if (lane == 63) part[wid] = incl;
+        __syncthreads();
+        int wpre = 0, round = 0;
+        for (int k = 0; k < 1024 / 64; k++) {
+            const int c = part[k];
+            wpre += k < wid ? c : 0;
+            round += c;
+        }
+        int run = carry + wpre + incl - t;
+#pragma unroll
+        for (int k = 0; k < kScanPer; k += 4) {
+            const int i = b0 + k;
+            int4 q;
+            q.x = run;
+            q.y = q.x + v[k];
+            q.z = q.y + v[k + 1];
+            q.w = q.z + v[k + 2];
+            run = q.w + v[k + 3];
+            if (i + 3 < nwr) {
+                *(int4*)(x + i) = q;
+            } else if (i < nwr) {
+                x[i] = q.x;
+                if (i + 1 < nwr) x[i + 1] = q.y;
+                if (i + 2 < nwr) x[i + 2] = q.z;
+            }
+        }
+        carry += round;
+        __syncthreads();   // part[] is rewritten by the next round
     }
-    if (lane == 63) part[wid] = incl;
-    __syncthreads();
-    int wpre = 0;
-    for (int k = 0; k < wid; k++) wpre += part[k];
-    if (threadIdx.x == 1023) blocksum = wpre + incl;
-    int run = wpre + incl - t;
-    for (int i = b0; i < b1; i++) {
-        int v = x[i];
-        x[i] = run;
-        run += v;
-    }
-    __syncthreads();
     if (threadIdx.x == 0) {
-        const int total = blocksum;
+        const int total = carry;
         const int slot = a == kWcNext ? kCntNext : a == kWcRetire ? kCntRetire : a == kWcGather ? kCntGather
                        : a == kWcShadow ? kCntShadow : kCntSeg + (a - kWcSeg);
         P.counters[slot] = total;
@@ -2114,9 +2168,9 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
         HIP_TRY(hipGetLastError());
         return RT_OK;
     };
-    auto collect = [&](int set) {
+    auto collect = [&](int set, int n) {
         if (!traced) return;
-        for (int k = 0; k < kIterBatch; k++) {
+        for (int k = 0; k < n; k++) {
             hipEvent_t* e = ctx->iev[set][k];
             float ms = 0;
             hipEventElapsedTime(&ms, e[0], e[1]);
@@ -2157,32 +2211,40 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
             hipGraphExec_t g;
             ~GraphGuard() { if (g) hipGraphExecDestroy(g); }
         } guard{gexec};
+        // Once every sample of the window has been issued (known one batch
+        // late), batches shrink to one iteration, so the window ends at most
+        // one empty iteration after its last path instead of a whole batch.
+        int nb[2] = {0, 0};   // iterations with timing events in each in-flight batch
+        bool drain = false;
         for (int b = 0;; b++) {
             const int set = b & 1;
-            if (traced) tm.launches += kIterBatch;
-            if (gexec) {
+            const int iters = drain ? 1 : kIterBatch;
+            if (traced) tm.launches += iters;
+            if (gexec && !drain) {
                 HIP_TRY(hipGraphLaunch(gexec, ctx->stream));
+                nb[set] = 0;
             } else {
-                for (int k = 0; k < kIterBatch; k++) {
-                    int rc = enqueue_iteration(Pw, ctx->iev[set][k]);
+                for (int k = 0; k < iters; k++) {
+                    int rc = enqueue_iteration(Pw, gexec ? nullptr : ctx->iev[set][k]);
                     if (rc) return rc;
                 }
+                nb[set] = gexec ? 0 : iters;
             }
-            HIP_TRY(hipMemcpyAsync(&ctx->pinned[2 * set], &P.counters[kCntActive], 4, hipMemcpyDeviceToHost,
-                                   ctx->stream));
-            HIP_TRY(hipMemcpyAsync(&ctx->pinned[2 * set + 1], &P.counters[kCntOverflow], 4, hipMemcpyDeviceToHost,
+            hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(kBlock), 0, ctx->stream, Pw);
+            HIP_TRY(hipMemcpyAsync(&ctx->pinned[4 * set], &P.counters[kCntPend], 12, hipMemcpyDeviceToHost,
                                    ctx->stream));
             HIP_TRY(hipEventRecord(ctx->chk[set], ctx->stream));
             if (b == 0) continue;
             const int prev = set ^ 1;
             HIP_TRY(hipEventSynchronize(ctx->chk[prev]));
-            if (!gexec) collect(prev);
-            if (ctx->pinned[2 * prev + 1]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
-            if (ctx->pinned[2 * prev] == 0) {
+            collect(prev, nb[prev]);
+            if (ctx->pinned[4 * prev + 1]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+            if (ctx->pinned[4 * prev] == 0) {
                 HIP_TRY(hipEventSynchronize(ctx->chk[set]));
-                if (!gexec) collect(set);
+                collect(set, nb[set]);
                 break;
             }
+            drain = ctx->pinned[4 * prev + 2] != 0;
         }
         for (int k = 0; k < plan.count[w]; k++) accum(plan.parts[plan.first[w] + k]);
         HIP_TRY(hipGetLastError());
@@ -2715,7 +2777,7 @@ static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, 
                       (gather ? 64 : 0);
     size_t per_sample = aovs ? 48 : 16;
     size_t need = (size_t)N * per_path + (size_t)WS * per_sample + 40 * 256 + 16384 + 8 * N +
-                  4 * (size_t)((N + kBlock - 1) / kBlock) * (kWcSeg + kMaxSeg);
+                  4 * (size_t)((N + kBlock - 1) / kBlock + 3) * (kWcSeg + kMaxSeg);
     if (need > ctx->path_bytes) {
         if (ctx->path_mem) hipFree(ctx->path_mem);
         ctx->path_mem = nullptr;
@@ -2760,7 +2822,7 @@ static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, 
     P.smask = (int32_t*)take(4 * N);
     P.gflag = (int32_t*)take(4 * N);
     P.q_g = (int32_t*)take(4 * N);
-    P.nw = (int)((N + kBlock - 1) / kBlock);
+    P.nw = (int)(((N + kBlock - 1) / kBlock + 3) & ~(int64_t)3);   // row stride of the count arrays (int4 rows)
     P.wc = (int32_t*)take(4 * (size_t)P.nw * (kWcSeg + std::min(maxs, kMaxSeg)));
     P.job = (int32_t*)take(4 * N);
     P.pix = (int32_t*)take(4 * N);
