@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib3dray_amd.so")
-SOURCES = ["scene_build.hip", "render.hip"]
+SOURCES = ["scene_build.hip", "render.hip", "film_io.hip"]
 HEADERS = ["rt_common.h", "rt_internal.h", "rt_scene.h", "traverse.h", "shade.h", "material.h", "adjoint.h", "photon.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -25,7 +25,7 @@ FLAGS = [
 
 
 def build(verbose=False):
-    cmd = [HIPCC] + FLAGS + ["-x", "hip"] + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", LIB + ".tmp"]
+    cmd = [HIPCC] + FLAGS + ["-x", "hip"] + [os.path.join(CSRC, s) for s in SOURCES] + ["-lz", "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -1977,6 +1977,8 @@ struct rt_ctx {
     bool use_graph = false;                   // replay captured bounce iterations (hipGraph)
     void* adj_mem = nullptr;                  // adjoint buffers (grow-only)
     size_t adj_bytes = 0;
+    void* io_mem = nullptr;                   // film-file encoder buffers (grow-only)
+    size_t io_bytes = 0;
 };
 
 struct rt_scene {
@@ -2010,6 +2012,29 @@ static int fail(rt_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;
     return code;
 }
+
+// Context services for the film-file encoders (film_io.hip).
+namespace rt {
+hipStream_t ctx_stream(rt_ctx* ctx) { return ctx->stream; }
+int ctx_device(rt_ctx* ctx) { return ctx->device; }
+int ctx_fail(rt_ctx* ctx, int code, const std::string& msg) { return fail(ctx, code, msg); }
+// Grow-only device buffer of at least `bytes` (contents not kept across growth).
+int ctx_io_buffer(rt_ctx* ctx, size_t bytes, void** out) {
+    if (bytes > ctx->io_bytes) {
+        if (ctx->io_mem) {
+            hipStreamSynchronize(ctx->stream);
+            hipFree(ctx->io_mem);
+        }
+        ctx->io_mem = nullptr;
+        ctx->io_bytes = 0;
+        hipError_t e = hipMalloc(&ctx->io_mem, bytes);
+        if (e != hipSuccess) return fail(ctx, RT_ERR_OUT_OF_MEMORY, hipGetErrorString(e));
+        ctx->io_bytes = bytes;
+    }
+    *out = ctx->io_mem;
+    return RT_OK;
+}
+}  // namespace rt
 
 template <typename T>
 static int upload(rt_ctx* ctx, T** dst, const T* src, size_t n, int64_t& bytes) {
@@ -2505,6 +2530,7 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->parts_mem) hipFree(ctx->parts_mem);
     if (ctx->jobs_mem) hipFree(ctx->jobs_mem);
     if (ctx->adj_mem) hipFree(ctx->adj_mem);
+    if (ctx->io_mem) hipFree(ctx->io_mem);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -113,6 +113,13 @@ class rt_scene_info(C.Structure):
                 ("device_bytes", C.c_int64), ("reserved", C.c_int64 * 4)]
 
 
+RT_EXR_HALF, RT_EXR_FLOAT = 1, 2
+
+
+class rt_exr_channel(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("type", C.c_int32), ("pad", C.c_int32), ("plane", C.POINTER(C.c_float))]
+
+
 for _t, _n in ((rt_primitive, 64), (rt_material, 128), (rt_light, 80), (rt_camera, 56),
                (rt_medium, 32), (rt_settings, 96)):
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t))

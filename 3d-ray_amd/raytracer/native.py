@@ -71,6 +71,11 @@ def load():
     L.rt_adjoint.argtypes = [vp, vp, C.POINTER(abi.rt_settings), fp, fp, C.POINTER(abi.rt_render_stats)]
     L.rt_debug_bvh_dump.argtypes = [C.POINTER(abi.rt_scene_desc), ip, C.c_int64]
     L.rt_debug_bvh_dump.restype = C.c_int64
+    u8p = C.POINTER(C.c_uint8)
+    L.rt_exr_encode.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(abi.rt_exr_channel), C.c_int32, u8p, C.c_int64,
+                                i64p]
+    L.rt_pfm_encode.argtypes = [vp, fp, C.c_int32, C.c_int32, C.c_int32, u8p, C.c_int64, i64p]
+    L.rt_png_encode.argtypes = [vp, fp, C.c_int32, C.c_int32, u8p, C.c_int64, i64p]
     _lib = L
     return L
 
@@ -79,7 +84,7 @@ EXPORTED_SYMBOLS = ("rt_get_version", "rt_material_defaults", "rt_settings_defau
                     "rt_last_error", "rt_scene_create", "rt_scene_destroy", "rt_scene_get_info",
                     "rt_scene_set_textures", "rt_render",
                     "rt_trace_primary", "rt_trace_rays", "rt_photon_map", "rt_render_batch", "rt_context_tune",
-                    "rt_adjoint", "rt_occluded",
+                    "rt_adjoint", "rt_occluded", "rt_exr_encode", "rt_pfm_encode", "rt_png_encode",
                     "rt_debug_bvh_dump")
 
 

@@ -239,6 +239,87 @@ def run_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def _film_planes(W, H, seed=0xF11):
+    """A render-like 1080p film: smooth HDR gradients with a little noise
+    (beauty, albedo, normal, variance layers) and a depth plane with misses."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:H, 0:W].astype(np.float32)
+    u, v = x / W, y / H
+    base = [np.sin(6.3 * u + k) * np.cos(4.1 * v - k) for k in range(12)]
+    planes = [(1.5 + b + 0.05 * rng.standard_normal((H, W))).astype(np.float32) for b in base]
+    depth = (2.0 + 3.0 * v + 0.01 * rng.standard_normal((H, W))).astype(np.float32)
+    depth[: H // 8] = -1.0
+    return planes, depth
+
+
+def run_film(args):
+    """Film-file workload: the 1920x1080 multilayer EXR of `-o out.exr --aov
+    albedo,normal,depth,variance` (R,G,B + albedo.*, normal.*, variance.*
+    half, Z float: 13 channels) and the 8-bit PNG of the display planes,
+    encoded from host planes (PCIe in, file bytes out).  value = EXR encodes
+    per second x pixels (Mpixels/s); the oracle's numpy + zlib writer on one
+    core is the CPU baseline."""
+    from raytracer import film_io
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    W, H = 1920, 1080
+    planes, depth = _film_planes(W, H)
+    names = ["R", "G", "B", "albedo.R", "albedo.G", "albedo.B", "normal.X", "normal.Y", "normal.Z",
+             "variance.R", "variance.G", "variance.B"]
+    chans = [film_io.Channel(n, film_io.HALF, p) for n, p in zip(names, planes)]
+    chans.append(film_io.Channel("Z", film_io.FLOAT, depth))
+    display = np.clip(np.stack(planes[:3]) / 3.0, 0.0, 1.0).astype(np.float32)
+    ctx = native.Context(0)
+    for _ in range(args.warmup):
+        film_io.exr_encode(W, H, chans, ctx)
+        film_io.png_encode(display, W, H, ctx)
+    te = tp = 0.0
+    nbytes = npng = 0
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        data = film_io.exr_encode(W, H, chans, ctx)
+        t1 = time.perf_counter()
+        png = film_io.png_encode(display, W, H, ctx)
+        t2 = time.perf_counter()
+        te += t1 - t0
+        tp += t2 - t1
+        nbytes, npng = len(data), len(png)
+        log(f"step {k}: exr {1e3 * (t1 - t0):.1f} ms ({nbytes} B), png {1e3 * (t2 - t1):.1f} ms ({npng} B)")
+    mpx = W * H / (te / args.steps) / 1e6
+    cpu = None
+    if not args.no_cpu_baseline:
+        import film_oracle
+        t0 = time.perf_counter()
+        ref = film_oracle.exr_bytes(W, H, [(c.name, c.type, c.plane) for c in chans])
+        tc = time.perf_counter() - t0
+        assert ref == data, "EXR bytes differ from the oracle's"
+        cpu = {"value": round(W * H / tc / 1e6, 3), "unit": "Mpixels/s", "cores": 1, "kind": "port",
+               "sample": f"one 1920x1080 13-channel EXR with the oracle's numpy + zlib writer ({tc:.2f} s), "
+                         "byte-identical to the GPU encoder's file"}
+    out = {
+        "metric": "Mpixels/s multilayer EXR encode 1080p",
+        "value": round(mpx, 2),
+        "unit": "Mpixels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round((te + tp) * 1e3 / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16/f32 bytes",
+        "data": "synthetic (seeded smooth HDR planes)",
+        "config": {"workload": "1920x1080 EXR, 12 half + 1 float channels, ZIP 16-line blocks; 1920x1080 RGBA PNG",
+                   "width": W, "height": H, "channels": 13},
+        "exr_ms": round(te * 1e3 / args.steps, 2),
+        "exr_bytes": nbytes,
+        "png_ms": round(tp * 1e3 / args.steps, 2),
+        "png_bytes": npng,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,12 +330,13 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--subdiv", type=int, default=8)
-    ap.add_argument("--pool", type=int, default=0, help="path slots in flight (0 = library default, 2^21)")
-    ap.add_argument("--workload", choices=("icosphere", "caustics", "batch", "adjoint"), default="icosphere",
+    ap.add_argument("--pool", type=int, default=0, help="path slots in flight (0 = library default, 2^24)")
+    ap.add_argument("--workload", choices=("icosphere", "caustics", "batch", "adjoint", "film"), default="icosphere",
                     help="icosphere: the 1.31M-triangle Disney scene (headline); caustics: glass spheres, "
                          "Gerstner water, fog, photon-map caustics; batch: 64 scenes of uneven size, batched "
                          "vs plain per-scene renders; adjoint: texture gradients of the 81,920-triangle textured "
-                         "icosphere (256^2, 32 spp, depth 4, 512^2 texture set), adjoint vs forward cost")
+                         "icosphere (256^2, 32 spp, depth 4, 512^2 texture set), adjoint vs forward cost; film: "
+                         "1080p multilayer EXR and PNG encoding")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     ap.add_argument("--photons", type=int, default=20_000_000)
@@ -293,6 +375,8 @@ def main():
         return run_batch(args)
     if args.workload == "adjoint":
         return run_adjoint(args)
+    if args.workload == "film":
+        return run_film(args)
     W, H, spp, depth = args.width, args.height, args.spp, args.depth
     if caus:
         log("generating scene (glass spheres, 80k-triangle Gerstner water, fog)")

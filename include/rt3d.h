@@ -426,6 +426,47 @@ int rt_photon_map(rt_ctx* ctx, const rt_scene* scene, const rt_settings* setting
                   int64_t cap_buckets, int64_t* n_photons, int64_t* n_emitted, int64_t* n_buckets, float* pos,
                   float* inc, float* power, int32_t* key, int32_t* idx, int32_t* start, float* grid);
 
+/* ── Film files.  Each encoder writes a whole file image into `out` (the
+ * caller writes it to disk); with out = NULL it only stores in *size an upper
+ * bound of the bytes needed, and a capacity below that bound is
+ * RT_ERR_INVALID_ARGUMENT.  On success *size = bytes written.  Planes are
+ * plane-major host floats, row 0 = top (FrameBuffer.cs:20-41); they are
+ * converted on the device and compressed on host threads. */
+
+/* EXR channel (ExrImage.Channel, ExrImage.cs:45-46; ExrPixelType, :8-12). */
+#define RT_EXR_HALF 1
+#define RT_EXR_FLOAT 2
+typedef struct rt_exr_channel {
+    const char* name;      /* 1..31 printable ASCII characters, unique */
+    int32_t type;          /* RT_EXR_HALF or RT_EXR_FLOAT */
+    int32_t pad;
+    const float* plane;    /* width*height floats */
+} rt_exr_channel;
+
+/* Replaces ExrImage.Write (ExrImage.cs:52-137): single-part scanline OpenEXR
+ * v2, ZIP compression in 16-line blocks (zlib framing, a block that does not
+ * shrink is stored raw), channels stored in ordinal name order, half samples
+ * clamped to +-65504 with NaN kept, Rec.709 chromaticities.  Invalid channel
+ * sets (none, empty / too long / non-printable / duplicate names, bad pixel
+ * type) and dimensions are RT_ERR_INVALID_ARGUMENT, as the reference throws
+ * ArgumentException. */
+int rt_exr_encode(rt_ctx* ctx, int32_t width, int32_t height, const rt_exr_channel* channels, int32_t n_channels,
+                  uint8_t* out, int64_t capacity, int64_t* size);
+
+/* Replaces PfmImage.Write (PfmImage.cs:29-65): "PF" (3 channels) or "Pf"
+ * (1), "W H", scale -1.0 (little-endian), rows bottom-to-top, channels
+ * interleaved.  channels other than 1 or 3 are RT_ERR_INVALID_ARGUMENT. */
+int rt_pfm_encode(rt_ctx* ctx, const float* planes, int32_t width, int32_t height, int32_t channels, uint8_t* out,
+                  int64_t capacity, int64_t* size);
+
+/* Replaces Program.SaveImage for .png (Program.cs:723-756): the display
+ * planes (3 x W*H, rt_film.display) as 8-bit RGBA, channel byte =
+ * clamp((int)(clamp(v, 0, 1) * 256), 0, 255) (NaN -> 0), alpha 255.  The
+ * reference's PNG encoder is a library (ImageSharp); the pixels, not the
+ * compressed bytes, are the contract. */
+int rt_png_encode(rt_ctx* ctx, const float* display, int32_t width, int32_t height, uint8_t* out, int64_t capacity,
+                  int64_t* size);
+
 /* Diagnostics (host only, no device needed): builds the scene on the host and
  * serialises the flattened world BVH in pre-order — internal node: (0, axis,
  * 6 box floats as int bits); leaf: (1, count, 6 box floats, entry ids) with
