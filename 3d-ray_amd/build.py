@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib3dray_amd.so")
-SOURCES = ["scene_build.hip", "render.hip", "film_io.hip"]
+SOURCES = ["scene_build.hip", "render.hip", "film_io.hip", "denoise.hip"]
 HEADERS = ["rt_common.h", "rt_internal.h", "rt_scene.h", "traverse.h", "shade.h", "material.h", "adjoint.h", "photon.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

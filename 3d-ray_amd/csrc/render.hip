@@ -1979,6 +1979,8 @@ struct rt_ctx {
     size_t adj_bytes = 0;
     void* io_mem = nullptr;                   // film-file encoder buffers (grow-only)
     size_t io_bytes = 0;
+    void* dn_mem = nullptr;                   // denoiser buffers (grow-only)
+    size_t dn_bytes = 0;
 };
 
 struct rt_scene {
@@ -2019,21 +2021,23 @@ hipStream_t ctx_stream(rt_ctx* ctx) { return ctx->stream; }
 int ctx_device(rt_ctx* ctx) { return ctx->device; }
 int ctx_fail(rt_ctx* ctx, int code, const std::string& msg) { return fail(ctx, code, msg); }
 // Grow-only device buffer of at least `bytes` (contents not kept across growth).
-int ctx_io_buffer(rt_ctx* ctx, size_t bytes, void** out) {
-    if (bytes > ctx->io_bytes) {
-        if (ctx->io_mem) {
+static int grow_buffer(rt_ctx* ctx, void** mem, size_t* have, size_t bytes, void** out) {
+    if (bytes > *have) {
+        if (*mem) {
             hipStreamSynchronize(ctx->stream);
-            hipFree(ctx->io_mem);
+            hipFree(*mem);
         }
-        ctx->io_mem = nullptr;
-        ctx->io_bytes = 0;
-        hipError_t e = hipMalloc(&ctx->io_mem, bytes);
+        *mem = nullptr;
+        *have = 0;
+        hipError_t e = hipMalloc(mem, bytes);
         if (e != hipSuccess) return fail(ctx, RT_ERR_OUT_OF_MEMORY, hipGetErrorString(e));
-        ctx->io_bytes = bytes;
+        *have = bytes;
     }
-    *out = ctx->io_mem;
+    *out = *mem;
     return RT_OK;
 }
+int ctx_io_buffer(rt_ctx* ctx, size_t bytes, void** out) { return grow_buffer(ctx, &ctx->io_mem, &ctx->io_bytes, bytes, out); }
+int ctx_dn_buffer(rt_ctx* ctx, size_t bytes, void** out) { return grow_buffer(ctx, &ctx->dn_mem, &ctx->dn_bytes, bytes, out); }
 }  // namespace rt
 
 template <typename T>
@@ -2531,6 +2535,7 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->jobs_mem) hipFree(ctx->jobs_mem);
     if (ctx->adj_mem) hipFree(ctx->adj_mem);
     if (ctx->io_mem) hipFree(ctx->io_mem);
+    if (ctx->dn_mem) hipFree(ctx->dn_mem);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }

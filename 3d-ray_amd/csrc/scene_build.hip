@@ -43,6 +43,7 @@ static_assert(sizeof(rt_render_stats) == 152, "rt_render_stats layout");
 static_assert(sizeof(rt_scene_info) == 96, "rt_scene_info layout");
 static_assert(sizeof(rt_batch_job) == 32, "rt_batch_job layout");
 static_assert(sizeof(rt_exr_channel) == 24, "rt_exr_channel layout");
+static_assert(sizeof(rt_denoise_options) == 32, "rt_denoise_options layout");
 
 static f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
 static int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }

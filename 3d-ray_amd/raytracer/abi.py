@@ -120,6 +120,15 @@ class rt_exr_channel(C.Structure):
     _fields_ = [("name", C.c_char_p), ("type", C.c_int32), ("pad", C.c_int32), ("plane", C.POINTER(C.c_float))]
 
 
+RT_DENOISE_NLM, RT_DENOISE_NFOR = 1, 2
+RT_DENOISE_FAST, RT_DENOISE_HIGH = 0, 1
+
+
+class rt_denoise_options(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("quality", C.c_int32), ("selection_margin", C.c_float),
+                ("samples_a", C.c_int32), ("samples_b", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
 for _t, _n in ((rt_primitive, 64), (rt_material, 128), (rt_light, 80), (rt_camera, 56),
                (rt_medium, 32), (rt_settings, 96)):
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t))

@@ -76,6 +76,8 @@ def load():
                                 i64p]
     L.rt_pfm_encode.argtypes = [vp, fp, C.c_int32, C.c_int32, C.c_int32, u8p, C.c_int64, i64p]
     L.rt_png_encode.argtypes = [vp, fp, C.c_int32, C.c_int32, u8p, C.c_int64, i64p]
+    L.rt_denoise.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(abi.rt_film), C.POINTER(abi.rt_denoise_options),
+                             fp]
     _lib = L
     return L
 
@@ -85,6 +87,7 @@ EXPORTED_SYMBOLS = ("rt_get_version", "rt_material_defaults", "rt_settings_defau
                     "rt_scene_set_textures", "rt_render",
                     "rt_trace_primary", "rt_trace_rays", "rt_photon_map", "rt_render_batch", "rt_context_tune",
                     "rt_adjoint", "rt_occluded", "rt_exr_encode", "rt_pfm_encode", "rt_png_encode",
+                    "rt_denoise",
                     "rt_debug_bvh_dump")
 
 

@@ -467,6 +467,29 @@ int rt_pfm_encode(rt_ctx* ctx, const float* planes, int32_t width, int32_t heigh
 int rt_png_encode(rt_ctx* ctx, const float* display, int32_t width, int32_t height, uint8_t* out, int64_t capacity,
                   int64_t* size);
 
+/* ── Denoiser (replaces NforDenoiser.Denoise, NforDenoiser.cs:29-57, with
+ * DenoiserOptions, DenoiserOptions.cs:1-51).  Consumes the dual-buffer film
+ * of a render with AOVs (beauty_a/b, albedo_a/b, normal_a/b, depth_a/b host
+ * planes of rt_film; the other planes are not read) and writes the denoised
+ * linear beauty (3 x W*H floats).  A film without AOV planes is
+ * RT_ERR_INVALID_ARGUMENT, as the reference throws InvalidOperationException. */
+#define RT_DENOISE_NLM 1     /* DenoiserKind.Nlm: joint NL-means */
+#define RT_DENOISE_NFOR 2    /* DenoiserKind.Nfor: first-order regression (the default) */
+#define RT_DENOISE_FAST 0    /* DenoiseQuality.Fast */
+#define RT_DENOISE_HIGH 1    /* DenoiseQuality.High (the default) */
+typedef struct rt_denoise_options {
+    int32_t kind;              /* RT_DENOISE_NFOR or RT_DENOISE_NLM */
+    int32_t quality;           /* RT_DENOISE_HIGH or RT_DENOISE_FAST */
+    float selection_margin;    /* NFOR candidate-selection margin (NforRegression.SelectionMargin);
+                                  < 0: the Sobol sampler's 0.5 */
+    int32_t samples_a;         /* RenderBuffers.SamplesA = ceil(spp / 2) */
+    int32_t samples_b;         /* RenderBuffers.SamplesB = floor(spp / 2) */
+    int32_t reserved[3];
+} rt_denoise_options;          /* 32 bytes */
+
+int rt_denoise(rt_ctx* ctx, int32_t width, int32_t height, const rt_film* film, const rt_denoise_options* options,
+               float* beauty_out);
+
 /* Diagnostics (host only, no device needed): builds the scene on the host and
  * serialises the flattened world BVH in pre-order — internal node: (0, axis,
  * 6 box floats as int bits); leaf: (1, count, 6 box floats, entry ids) with

@@ -353,3 +353,90 @@ def caustic_estimate(pmap, points, normals, views, albedo, max_r):
     out = np.zeros((len(pts), 3), np.float32)
     lib().orc_caustic_estimate(pmap.h, _f(pts), _f(nrm), _f(vw), len(pts), _f(alb), float(max_r), _f(out))
     return out
+
+
+# ── denoiser (denoise.cpp) ────────────────────────────────────────────────
+
+def _dn_lib():
+    L = lib()
+    if not getattr(L, "_dn_typed", False):
+        fp, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
+        L.orc_nlm_filter.argtypes = [C.c_int, C.c_int, C.c_int, fp, fp, C.c_int, fp, C.c_int, C.c_int, C.c_float,
+                                     C.c_int, fp, fp, fp]
+        L.orc_nlm_filter.restype = None
+        L.orc_binomial_smooth7.argtypes = [C.c_int, C.c_int, fp, fp]
+        L.orc_binomial_smooth7.restype = None
+        L.orc_box_smooth.argtypes = [C.c_int, C.c_int, fp, fp, C.c_int]
+        L.orc_box_smooth.restype = None
+        L.orc_percentile_nonneg.argtypes = [C.c_int, fp, C.c_float, C.c_float]
+        L.orc_percentile_nonneg.restype = C.c_float
+        L.orc_regression_solve.argtypes = [dp, dp, C.c_int, C.c_int, C.c_double]
+        L.orc_regression_solve.restype = C.c_int
+        L.orc_denoise.argtypes = [C.c_int, C.c_int] + [fp] * 8 + [C.c_int] * 4 + [C.c_float, fp]
+        L.orc_denoise.restype = C.c_int
+        L._dn_typed = True
+    return L
+
+
+def _c32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def nlm_filter(guide, var, target, R, P, k, features=None, inv_bw=None):
+    """NlMeansCore.Filter: guide/var (gc, H, W), target (tc, H, W); features
+    (F, H, W) with per-plane inverse bandwidths."""
+    g, v, t = _c32(guide), _c32(var), _c32(target)
+    gc, h, w = g.shape
+    tc = t.shape[0]
+    out = np.zeros_like(t)
+    nf = 0 if features is None else features.shape[0]
+    f = _c32(features) if nf else None
+    bw = _c32(inv_bw) if nf else None
+    _dn_lib().orc_nlm_filter(w, h, gc, _f(g), _f(v), tc, _f(t), R, P, float(k), nf, _f(f) if nf else None,
+                             _f(bw) if nf else None, _f(out))
+    return out
+
+
+def binomial_smooth7(plane):
+    p = _c32(plane)
+    out = np.zeros_like(p)
+    _dn_lib().orc_binomial_smooth7(p.shape[1], p.shape[0], _f(p), _f(out))
+    return out
+
+
+def box_smooth(plane, r):
+    p = _c32(plane)
+    out = np.zeros_like(p)
+    _dn_lib().orc_box_smooth(p.shape[1], p.shape[0], _f(p), _f(out), r)
+    return out
+
+
+def percentile_nonneg(plane, pct=0.99, fallback=1.0):
+    p = _c32(plane).ravel()
+    return float(_dn_lib().orc_percentile_nonneg(p.size, _f(p), pct, fallback))
+
+
+def regression_solve(gram, rhs, lambda_scale=1e-3):
+    """RegressionSolver.Solve on copies: (solved, solution (rhs_count, dim))."""
+    g = np.ascontiguousarray(gram, np.float64).copy()
+    r = np.ascontiguousarray(rhs, np.float64).copy()
+    dim = g.shape[0]
+    cnt = r.size // dim
+    dp = C.POINTER(C.c_double)
+    ok = _dn_lib().orc_regression_solve(g.ctypes.data_as(dp), r.ctypes.data_as(dp), dim, cnt, lambda_scale)
+    return bool(ok), r.reshape(cnt, dim)
+
+
+def denoise(buffers, kind="nfor", quality="high", margin=0.5):
+    """NforDenoiser.Denoise on a RenderBuffers-like object (planes (C, H, W),
+    samples_a / samples_b); margin = the NFOR selection margin (0.5 for the
+    Sobol sampler)."""
+    b = buffers
+    h, w = b.beauty_a.shape[-2:]
+    planes = [_c32(x) for x in (b.beauty_a, b.beauty_b, b.albedo_a, b.albedo_b, b.normal_a, b.normal_b,
+                                b.depth_a, b.depth_b)]
+    out = np.zeros((3, h, w), np.float32)
+    rc = _dn_lib().orc_denoise(w, h, *[_f(p) for p in planes], b.samples_a, b.samples_b,
+                               {"nlm": 1, "nfor": 2}[kind], {"fast": 0, "high": 1}[quality], float(margin), _f(out))
+    assert rc == 0
+    return out
