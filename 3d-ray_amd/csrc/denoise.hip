@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rt3d.h"
@@ -251,6 +252,312 @@ __global__ void __launch_bounds__(kBlk) k_var_floor(const float* sm, const float
     half[i] = 2.0f * (s > fl ? s : fl);
 }
 
+// ── NFOR (NforRegression.cs) ──────────────────────────────────────────────
+constexpr int kDim = 8, kFeat = 7;                 // [1, albedo 3, normal 3, depth]
+constexpr int kTileCX = 16, kTileCY = 8;           // centres per k_nfor_dist workgroup
+constexpr int kBW = 2 * kTileCX - 1 + 2 * kPatch;  // its distance block
+constexpr int kBH = 2 * kTileCY - 1 + 2 * kPatch;
+constexpr int kSplatTile = 64;                     // NforRegression.TileSize
+
+struct NforGeom {
+    int w, h, R, side, offsets;
+    int cxn, cyn;      // centres: every (even x, even y) pixel (the stride-2 grid)
+};
+
+// Patch distance of every (centre, offset) pair (ProcessTile, NforRegression.cs:345-381):
+// per offset, pointwise distances of the block in LDS, then each centre's
+// clipped patch summed row by row and divided by its pixel count; INF when
+// the centre has no partner inside the image.  dist is offset-major.
+__global__ void __launch_bounds__(kBlk) k_nfor_dist(const float* __restrict__ g, const float* __restrict__ v, NforGeom G,
+                                                    float* __restrict__ dist) {
+    __shared__ float blk[kBW * kBH];
+    const int64_t n = (int64_t)G.w * G.h, nc = (int64_t)G.cxn * G.cyn;
+    const int tilesX = (G.cxn + kTileCX - 1) / kTileCX;
+    const int tcx = blockIdx.x % tilesX, tcy = blockIdx.x / tilesX;
+    const int px0 = 2 * kTileCX * tcx, py0 = 2 * kTileCY * tcy;
+    const int bx0 = px0 - kPatch, by0 = py0 - kPatch;
+    const int lcx = threadIdx.x % kTileCX, lcy = threadIdx.x / kTileCX;
+    const bool is_c = threadIdx.x < kTileCX * kTileCY;
+    const int gx = px0 + 2 * lcx, gy = py0 + 2 * lcy;
+    const bool has_c = is_c && gx < G.w && gy < G.h;
+    const int64_t ci = (int64_t)(gy / 2) * G.cxn + gx / 2;
+    const float invC = 1.0f / 3.0f;
+    constexpr int P = kPatch;
+    for (int o = 0; o < G.offsets; o++) {
+        const int dx = o % G.side - G.R, dy = o / G.side - G.R;
+        const int pxLo = max(0, -dx), pxHi = G.w - 1 - max(0, dx);
+        const int pyLo = max(0, -dy), pyHi = G.h - 1 - max(0, dy);
+        const int64_t qOff = (int64_t)dy * G.w + dx;
+        for (int i = threadIdx.x; i < kBW * kBH; i += blockDim.x) {
+            const int xx = bx0 + i % kBW, yy = by0 + i / kBW;
+            float d = 0.0f;
+            if (xx >= pxLo && xx <= pxHi && yy >= pyLo && yy <= pyHi) {
+                const int64_t p = (int64_t)yy * G.w + xx;
+                d = dpt_at(g, v, n, 3, invC, p, p + qOff);
+            }
+            blk[i] = d;
+        }
+        __syncthreads();
+        if (has_c) {
+            float r;
+            if (gx < pxLo || gx > pxHi || gy < pyLo || gy > pyHi) {
+                r = INFINITY;
+            } else {
+                const int xLo = max(gx - P, pxLo), xHi = min(gx + P, pxHi);
+                const int yLo = max(gy - P, pyLo), yHi = min(gy + P, pyHi);
+                float sum = 0.0f;
+                for (int yy = yLo; yy <= yHi; yy++)
+                    for (int xx = xLo; xx <= xHi; xx++) sum += blk[(yy - by0) * kBW + (xx - bx0)];
+                r = sum / (float)((xHi - xLo + 1) * (yHi - yLo + 1));
+            }
+            dist[(int64_t)o * nc + ci] = r;
+        }
+        __syncthreads();
+    }
+}
+
+// Per centre and candidate: weights, the weighted normal equations over the
+// window in offset order, the regularised Cholesky solve in double
+// (RegressionSolver.cs:19-66) and the coefficients (NforRegression.cs:383-461).
+// Only the lower triangle of the Gram matrix is formed: the solver reads no
+// other element.  Weights go to W (offset-major, 0 = no contribution).
+__global__ void __launch_bounds__(kBlk) k_nfor_solve(const float* __restrict__ dist, const float* __restrict__ feat,
+                                                     const float* __restrict__ t, NforGeom G, float invK2,
+                                                     float* __restrict__ W, float* __restrict__ beta) {
+    const int64_t nc = (int64_t)G.cxn * G.cyn, n = (int64_t)G.w * G.h;
+    const int64_t ci = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nc) return;
+    const int gx = 2 * (int)(ci % G.cxn), gy = 2 * (int)(ci / G.cxn);
+    const int64_t p = (int64_t)gy * G.w + gx;
+    float fp[kFeat];
+    for (int i = 0; i < kFeat; i++) fp[i] = feat[(int64_t)i * n + p];
+    float gram[kDim * (kDim + 1) / 2];   // lower triangle, row by row
+    float rhs[3 * kDim];
+    for (int i = 0; i < kDim * (kDim + 1) / 2; i++) gram[i] = 0.0f;
+    for (int i = 0; i < 3 * kDim; i++) rhs[i] = 0.0f;
+    for (int o = 0; o < G.offsets; o++) {
+        const float dd = dist[(int64_t)o * nc + ci];
+        float wgt = 0.0f;
+        if (!(isinf(dd) && dd > 0.0f)) {
+            const float d = maxz(dd) * invK2;
+            if (d < kCutoff) wgt = fast_exp_neg(d);
+        }
+        W[(int64_t)o * nc + ci] = wgt;
+        if (wgt == 0.0f) continue;
+        const int dx = o % G.side - G.R, dy = o / G.side - G.R;
+        const int64_t q = p + (int64_t)dy * G.w + dx;
+        float f[kDim];
+        f[0] = 1.0f;
+        for (int i = 0; i < kFeat; i++) f[i + 1] = feat[(int64_t)i * n + q] - fp[i];
+        const float wy0 = wgt * t[q], wy1 = wgt * t[n + q], wy2 = wgt * t[2 * n + q];
+        int k = 0;
+        for (int i = 0; i < kDim; i++) {
+            const float wf = wgt * f[i];
+            for (int j = 0; j <= i; j++) gram[k++] += wf * f[j];
+        }
+        for (int j = 0; j < kDim; j++) {
+            rhs[j] += wy0 * f[j];
+            rhs[kDim + j] += wy1 * f[j];
+            rhs[2 * kDim + j] += wy2 * f[j];
+        }
+    }
+    float* b = beta + ci * 3 * kDim;
+    const float sumW = gram[0];
+    if (sumW <= 0.0f) {
+        for (int i = 0; i < 3 * kDim; i++) b[i] = 0.0f;
+        return;
+    }
+    double L[kDim * (kDim + 1) / 2], B[3 * kDim];
+    for (int i = 0; i < kDim * (kDim + 1) / 2; i++) L[i] = gram[i];
+    for (int i = 0; i < 3 * kDim; i++) B[i] = rhs[i];
+    double mf[3];
+    for (int ch = 0; ch < 3; ch++) mf[ch] = B[ch * kDim] / (double)sumW;
+    auto at = [](int i, int j) { return i * (i + 1) / 2 + j; };   // j <= i
+    double trace = 0;
+    for (int i = 0; i < kDim; i++) trace += L[at(i, i)];
+    const double lambda = 1e-3 * trace / kDim + 1e-12;
+    for (int i = 0; i < kDim; i++) L[at(i, i)] += lambda;
+    bool ok = true;
+    for (int i = 0; i < kDim && ok; i++)
+        for (int j = 0; j <= i; j++) {
+            double sum = L[at(i, j)];
+            for (int k = 0; k < j; k++) sum -= L[at(i, k)] * L[at(j, k)];
+            if (i == j) {
+                if (sum <= 0) {
+                    ok = false;
+                    break;
+                }
+                L[at(i, i)] = sqrt(sum);
+            } else {
+                L[at(i, j)] = sum / L[at(j, j)];
+            }
+        }
+    if (ok) {
+        for (int r = 0; r < 3; r++) {
+            double* x = B + r * kDim;
+            for (int i = 0; i < kDim; i++) {
+                double sum = x[i];
+                for (int k = 0; k < i; k++) sum -= L[at(i, k)] * x[k];
+                x[i] = sum / L[at(i, i)];
+            }
+            for (int i = kDim - 1; i >= 0; i--) {
+                double sum = x[i];
+                for (int k = i + 1; k < kDim; k++) sum -= L[at(k, i)] * x[k];
+                x[i] = sum / L[at(i, i)];
+            }
+        }
+        for (int i = 0; i < 3 * kDim; i++) b[i] = (float)B[i];
+    } else {
+        for (int i = 0; i < 3 * kDim; i++) b[i] = 0.0f;
+        for (int ch = 0; ch < 3; ch++) b[ch * kDim] = (float)mf[ch];
+    }
+}
+
+// The collaborative splat as a gather (NforRegression.cs:463-516 and the
+// normalisation :288-311): pixel q takes the predictions of the centres
+// whose window holds it in the order the reference adds them — tiles by
+// checkerboard pass, centres row by row inside a tile — then divides.
+__global__ void __launch_bounds__(kBlk) k_nfor_gather(const float* __restrict__ W, const float* __restrict__ beta,
+                                                      const float* __restrict__ feat, const float* __restrict__ t,
+                                                      NforGeom G, float* __restrict__ out) {
+    const int64_t n = (int64_t)G.w * G.h, nc = (int64_t)G.cxn * G.cyn;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int x = (int)(q % G.w), y = (int)(q / G.w);
+    float fq[kFeat];
+    for (int i = 0; i < kFeat; i++) fq[i] = feat[(int64_t)i * n + q];
+    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, den = 0.0f;
+    const int xa = max(0, x - G.R), xb = min(G.w - 1, x + G.R);
+    const int ya = max(0, y - G.R), yb = min(G.h - 1, y + G.R);
+    const int txa = xa / kSplatTile, txb = xb / kSplatTile, tya = ya / kSplatTile, tyb = yb / kSplatTile;
+    for (int pass = 0; pass < 4; pass++)
+        for (int ty = tya; ty <= tyb; ty++) {
+            if ((ty & 1) != (pass >> 1)) continue;
+            for (int tx = txa; tx <= txb; tx++) {
+                if ((tx & 1) != (pass & 1)) continue;
+                const int gy0 = max(ty * kSplatTile, ya), gy1 = min(ty * kSplatTile + kSplatTile - 1, yb);
+                const int gx0 = max(tx * kSplatTile, xa), gx1 = min(tx * kSplatTile + kSplatTile - 1, xb);
+                for (int gy = gy0 + (gy0 & 1); gy <= gy1; gy += 2)
+                    for (int gx = gx0 + (gx0 & 1); gx <= gx1; gx += 2) {
+                        const int64_t ci = (int64_t)(gy / 2) * G.cxn + gx / 2;
+                        const int o = (y - gy + G.R) * G.side + (x - gx + G.R);
+                        const float wgt = W[(int64_t)o * nc + ci];
+                        if (wgt <= 0.0f) continue;
+                        const int64_t pc = (int64_t)gy * G.w + gx;
+                        float f[kDim];
+                        f[0] = 1.0f;
+                        for (int i = 0; i < kFeat; i++) f[i + 1] = fq[i] - feat[(int64_t)i * n + pc];
+                        const float* b = beta + ci * 3 * kDim;
+                        float pr[3];
+                        for (int ch = 0; ch < 3; ch++) {
+                            float e[kDim];
+                            for (int i = 0; i < kDim; i++) e[i] = b[ch * kDim + i] * f[i];
+                            pr[ch] = ((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7]));
+                        }
+                        acc0 += wgt * pr[0];
+                        acc1 += wgt * pr[1];
+                        acc2 += wgt * pr[2];
+                        den += wgt;
+                    }
+            }
+        }
+    if (den > 0.0f) {
+        const float inv = 1.0f / den;
+        out[q] = acc0 * inv;
+        out[n + q] = acc1 * inv;
+        out[2 * n + q] = acc2 * inv;
+    } else {
+        out[q] = t[q];
+        out[n + q] = t[n + q];
+        out[2 * n + q] = t[2 * n + q];
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_scale(const float* src, int64_t n, const float* scales, int planes,
+                                                float* dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * planes) return;
+    dst[i] = src[i] * scales[i / n];
+}
+
+// PlaneOps.BoxSmooth (PlaneOps.cs:55-85): running row sums, then column sums
+__global__ void __launch_bounds__(kBlk) k_box_rows(const float* src, float* tmp, int w, int h, int r) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= h) return;
+    const int64_t row = (int64_t)y * w;
+    float run = 0.0f;
+    for (int x = 0; x < min(r + 1, w); x++) run += src[row + x];
+    for (int x = 0; x < w; x++) {
+        const int lo = max(0, x - r), hi = min(w - 1, x + r);
+        tmp[row + x] = run / (float)(hi - lo + 1);
+        if (x + r + 1 < w) run += src[row + x + r + 1];
+        if (x - r >= 0) run -= src[row + x - r];
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_box_cols(const float* tmp, float* dst, int w, int h, int r) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)w * h) return;
+    const int x = (int)(i % w), y = (int)(i / w);
+    const int lo = max(0, y - r), hi = min(h - 1, y + r);
+    const float inv = 1.0f / (hi - lo + 1);
+    float sum = 0.0f;
+    for (int yy = lo; yy <= hi; yy++) sum += tmp[(int64_t)yy * w + x];
+    dst[i] = sum * inv;
+}
+
+// candidate MSE maps before smoothing (NforRegression.cs:95-133)
+__global__ void __launch_bounds__(kBlk) k_mse_noisy(const float* hv, int64_t n, float* tmp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v0 = 0.5f * hv[i], v1 = 0.5f * hv[n + i], v2 = 0.5f * hv[2 * n + i];
+    tmp[i] = (v0 + v1 + v2) * (1.0f / 3.0f);
+}
+
+__global__ void __launch_bounds__(kBlk) k_mse_filtered(const float* fa, const float* fb, const float* bA,
+                                                       const float* bB, const float* hv, int64_t n, float margin,
+                                                       float* tmp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float e = 0.0f, vSum = 0.0f;
+    for (int c = 0; c < 3; c++) {
+        const int64_t p = c * n + i;
+        const float dA = fa[p] - bB[p], dB = fb[p] - bA[p];
+        e += 0.5f * (dA * dA + dB * dB) - hv[p];
+        vSum += 0.5f * hv[p];
+    }
+    tmp[i] = (e + margin * vSum) * (1.0f / 3.0f);
+}
+
+// indicator of candidate ci being the (first) smallest smoothed MSE
+__global__ void __launch_bounds__(kBlk) k_indicator(const float* mse, int ncand, int ci, int64_t n, float* ind) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float best = mse[i];
+    int bi = 0;
+    for (int cj = 1; cj < ncand; cj++)
+        if (mse[cj * n + i] < best) {
+            best = mse[cj * n + i];
+            bi = cj;
+        }
+    ind[i] = bi == ci ? 1.0f : 0.0f;
+}
+
+// softened blend of the candidates, clamped at zero (NforRegression.cs:136-167)
+__global__ void __launch_bounds__(kBlk) k_blend(const float* wts, const float* cand, int ncand, int64_t n, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float wSum = 0.0f;
+    for (int c = 0; c < ncand; c++) wSum += wts[c * n + i];
+    const float inv = wSum > 0.0f ? 1.0f / wSum : 0.0f;
+    for (int ch = 0; ch < 3; ch++) {
+        float acc = 0.0f;
+        for (int c = 0; c < ncand; c++) acc += wts[c * n + i] * cand[(int64_t)c * 3 * n + ch * n + i];
+        const float r = acc * inv;
+        out[ch * n + i] = r < 0.0f ? 0.0f : r;
+    }
+}
+
 inline unsigned blocks(int64_t n) { return (unsigned)((n + kBlk - 1) / kBlk); }
 
 }  // namespace dn
@@ -378,6 +685,102 @@ int percentile99(Pipe& Q, const float* x, float* result) {
     return RT_OK;
 }
 
+// PlaneOps.BoxSmooth of one plane (tmp: one scratch plane)
+void box_smooth(Pipe& Q, const float* src, float* tmp, float* dst, int r) {
+    hipLaunchKernelGGL(k_box_rows, dim3(blocks(Q.h)), dim3(kBlk), 0, Q.st, src, tmp, Q.w, Q.h, r);
+    hipLaunchKernelGGL(k_box_cols, dim3(blocks(Q.n)), dim3(kBlk), 0, Q.st, tmp, dst, Q.w, Q.h, r);
+}
+
+// NforRegression.BuildScaledFeatures (NforRegression.cs:173-193): the global
+// standard deviation of each feature plane from sequential double sums, in
+// the reference's order (host threads, one per plane), then the scaling on
+// the device.
+int scale_features(Pipe& Q, const float* feat, float* scales_dev, float* out) {
+    rt_ctx* ctx = Q.ctx;
+    const int64_t n = Q.n;
+    std::vector<float> host((size_t)kFeat * n);
+    DN_TRY(hipMemcpyAsync(host.data(), feat, host.size() * 4, hipMemcpyDeviceToHost, Q.st));
+    DN_TRY(hipStreamSynchronize(Q.st));
+    float scales[kFeat];
+    std::vector<std::thread> th;
+    for (int pl = 0; pl < kFeat; pl++)
+        th.emplace_back([&, pl] {
+            const float* src = host.data() + (size_t)pl * n;
+            double sum = 0, sumSq = 0;
+            for (int64_t i = 0; i < n; i++) {
+                sum += src[i];
+                sumSq += (double)src[i] * src[i];
+            }
+            const double mean = sum / n;
+            const float sd = (float)std::sqrt(std::max(sumSq / n - mean * mean, 0.0));
+            scales[pl] = 1.0f / std::max(sd, 1e-4f);
+        });
+    for (auto& t : th) t.join();
+    DN_TRY(hipMemcpyAsync(scales_dev, scales, sizeof(scales), hipMemcpyHostToDevice, Q.st));
+    hipLaunchKernelGGL(k_scale, dim3(blocks(kFeat * n)), dim3(kBlk), 0, Q.st, feat, n, (const float*)scales_dev, kFeat,
+                       out);
+    // the host vector must outlive the upload of `scales`
+    DN_TRY(hipStreamSynchronize(Q.st));
+    return RT_OK;
+}
+
+struct NforBufs {
+    float *featS, *dist, *W, *beta, *reg, *cand, *mse, *wts, *tmp, *tmp2, *scales;
+};
+
+// NforRegression.Run (NforRegression.cs:54-170)
+int nfor(Pipe& Q, const NforBufs& B, const float* bA, const float* bB, const float* bmean, const float* bhv,
+         const float* feat, int quality, float margin, int nA, int nB, float* res) {
+    rt_ctx* ctx = Q.ctx;
+    const int64_t n = Q.n;
+    int rc = scale_features(Q, feat, B.scales, B.featS);
+    if (rc) return rc;
+    const float ksHigh[2] = {0.5f, 1.0f}, ksFast[1] = {0.7f};
+    const float* ks = quality == RT_DENOISE_HIGH ? ksHigh : ksFast;
+    const int nk = quality == RT_DENOISE_HIGH ? 2 : 1;
+    const int R = quality == RT_DENOISE_HIGH ? 9 : 7;
+    NforGeom G{Q.w, Q.h, R, 2 * R + 1, (2 * R + 1) * (2 * R + 1), (Q.w + 1) / 2, (Q.h + 1) / 2};
+    const int64_t nc = (int64_t)G.cxn * G.cyn;
+    const int tiles = ((G.cxn + kTileCX - 1) / kTileCX) * ((G.cyn + kTileCY - 1) / kTileCY);
+    // regressions: guide B -> target A (reg[0..nk)), guide A -> target B (reg[nk..2nk))
+    for (int half = 0; half < 2; half++) {
+        const float* g = half == 0 ? bB : bA;
+        const float* t = half == 0 ? bA : bB;
+        hipLaunchKernelGGL(k_nfor_dist, dim3(tiles), dim3(kBlk), 0, Q.st, g, bhv, G, B.dist);
+        for (int ki = 0; ki < nk; ki++) {
+            const float invK2 = 1.0f / (ks[ki] * ks[ki]);
+            hipLaunchKernelGGL(k_nfor_solve, dim3(blocks(nc)), dim3(kBlk), 0, Q.st, (const float*)B.dist,
+                               (const float*)B.featS, t, G, invK2, B.W, B.beta);
+            hipLaunchKernelGGL(k_nfor_gather, dim3(blocks(n)), dim3(kBlk), 0, Q.st, (const float*)B.W,
+                               (const float*)B.beta, (const float*)B.featS, t, G,
+                               B.reg + (int64_t)(half * nk + ki) * 3 * n);
+        }
+    }
+    // candidates: the unfiltered mean, then one per k
+    const int ncand = nk + 1;
+    DN_TRY(hipMemcpyAsync(B.cand, bmean, (size_t)n * 12, hipMemcpyDeviceToDevice, Q.st));
+    for (int ki = 0; ki < nk; ki++) {
+        rc = combine(Q, B.reg + (int64_t)ki * 3 * n, B.reg + (int64_t)(nk + ki) * 3 * n, 3 * n, nA, nB,
+                     B.cand + (int64_t)(ki + 1) * 3 * n);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_mse_noisy, dim3(blocks(n)), dim3(kBlk), 0, Q.st, bhv, n, B.tmp);
+    box_smooth(Q, B.tmp, B.tmp2, B.mse, 4);
+    for (int ki = 0; ki < nk; ki++) {
+        hipLaunchKernelGGL(k_mse_filtered, dim3(blocks(n)), dim3(kBlk), 0, Q.st,
+                           (const float*)(B.reg + (int64_t)ki * 3 * n), (const float*)(B.reg + (int64_t)(nk + ki) * 3 * n),
+                           bA, bB, bhv, n, margin, B.tmp);
+        box_smooth(Q, B.tmp, B.tmp2, B.mse + (int64_t)(ki + 1) * n, 4);
+    }
+    for (int ci = 0; ci < ncand; ci++) {
+        hipLaunchKernelGGL(k_indicator, dim3(blocks(n)), dim3(kBlk), 0, Q.st, (const float*)B.mse, ncand, ci, n, B.tmp);
+        box_smooth(Q, B.tmp, B.tmp2, B.wts + (int64_t)ci * n, 2);
+    }
+    hipLaunchKernelGGL(k_blend, dim3(blocks(n)), dim3(kBlk), 0, Q.st, (const float*)B.wts, (const float*)B.cand, ncand,
+                       n, res);
+    return RT_OK;
+}
+
 }  // namespace
 
 extern "C" int rt_denoise(rt_ctx* ctx, int32_t width, int32_t height, const rt_film* film,
@@ -393,7 +796,6 @@ extern "C" int rt_denoise(rt_ctx* ctx, int32_t width, int32_t height, const rt_f
         return ctx_fail(ctx, RT_ERR_INVALID_ARGUMENT, "unexpected denoiser kind");
     if (quality != RT_DENOISE_FAST && quality != RT_DENOISE_HIGH)
         return ctx_fail(ctx, RT_ERR_INVALID_ARGUMENT, "unexpected denoise quality");
-    if (kind == RT_DENOISE_NFOR) return ctx_fail(ctx, RT_ERR_UNSUPPORTED, "the NFOR regression is not built yet");
     const int nA = opts->samples_a, nB = opts->samples_b;
     if (nA <= 0 || nB < 0) return ctx_fail(ctx, RT_ERR_INVALID_ARGUMENT, "samples_a must be > 0 and samples_b >= 0");
     DN_TRY(hipSetDevice(ctx_device(ctx)));
@@ -403,6 +805,7 @@ extern "C" int rt_denoise(rt_ctx* ctx, int32_t width, int32_t height, const rt_f
     // device layout (floats): inputs 20 n, features 7 n, temporaries, chunk planes
     float *bA, *bB, *aA, *aB, *nmA, *nmB, *dA, *dB, *comb, *dNa, *dNb, *feat, *mean, *hv, *s0, *s1, *fa, *fb, *bmean,
         *bhv, *oa, *ob, *res;
+    NforBufs NB{};
     size_t total = 0;
     for (int pass = 0; pass < 2; pass++) {
         char* base = nullptr;
@@ -422,6 +825,14 @@ extern "C" int rt_denoise(rt_ctx* ctx, int32_t width, int32_t height, const rt_f
         Q.den = A.f(n);
         Q.hist = (int32_t*)A.f(65536);
         Q.bw = A.f(8);
+        if (kind == RT_DENOISE_NFOR) {
+            const int R = quality == RT_DENOISE_HIGH ? 9 : 7, nk = quality == RT_DENOISE_HIGH ? 2 : 1;
+            const int64_t offs = (int64_t)(2 * R + 1) * (2 * R + 1);
+            const int64_t nc = (int64_t)((width + 1) / 2) * ((height + 1) / 2);
+            NB.featS = A.f(kFeat * n), NB.dist = A.f(offs * nc), NB.W = A.f(offs * nc), NB.beta = A.f(nc * 3 * kDim);
+            NB.reg = A.f(2 * nk * 3 * n), NB.cand = A.f((nk + 1) * 3 * n), NB.mse = A.f((nk + 1) * n);
+            NB.wts = A.f((nk + 1) * n), NB.tmp = A.f(n), NB.tmp2 = A.f(n), NB.scales = A.f(8);
+        }
         total = A.off + 256;
     }
     const size_t p3 = (size_t)n * 12, p1 = (size_t)n * 4;
@@ -455,8 +866,11 @@ extern "C" int rt_denoise(rt_ctx* ctx, int32_t width, int32_t height, const rt_f
     // 3. beauty variance
     if (!rc) rc = combine(Q, bA, bB, 3 * n, nA, nB, bmean);
     if (!rc) rc = half_variance(Q, bA, bB, bmean, 3, s0, s1, bhv);
-    // 4. joint NL-means
-    if (!rc) {
+    // 4. main filter: NFOR regression or joint NL-means
+    if (!rc && kind == RT_DENOISE_NFOR) {
+        const float margin = opts->selection_margin >= 0.0f ? opts->selection_margin : 0.5f;
+        rc = nfor(Q, NB, bA, bB, bmean, bhv, feat, quality, margin, nA, nB, res);
+    } else if (!rc) {
         const float k = quality == RT_DENOISE_HIGH ? 1.0f : 0.7f;
         const int R = quality == RT_DENOISE_HIGH ? 9 : 7;
         rc = nlm(Q, bB, bhv, 3, bA, 3, R, k, 7, feat, oa);

@@ -25,24 +25,34 @@ def _check(got, want, what):
     assert bad[0].size == 0
 
 
-@pytest.mark.parametrize("W,H,spp", [(37, 29, 8), (64, 48, 3)])
-@pytest.mark.parametrize("quality", ["high", "fast"])
-def test_nlm_matches_oracle_bit_exact(gpu_ctx, oracle, W, H, spp, quality):
+@pytest.mark.parametrize("W,H,spp", [(37, 29, 8), (64, 48, 3), (150, 70, 4)])
+@pytest.mark.parametrize("kind,quality", [("nlm", "high"), ("nlm", "fast"), ("nfor", "high"), ("nfor", "fast")])
+def test_matches_oracle_bit_exact(gpu_ctx, oracle, W, H, spp, kind, quality):
+    # 150 x 70 spans several 64-pixel splat tiles in both directions
     buf = _film(gpu_ctx, scenes.cornell(W, H), W, H, spp)
-    got = denoise.denoise(buf, denoise.DenoiserOptions("nlm", quality), gpu_ctx)
-    want = oracle.denoise(buf, "nlm", quality)
-    _check(got, want, f"nlm {quality} {W}x{H} {spp} spp")
+    got = denoise.denoise(buf, denoise.DenoiserOptions(kind, quality), gpu_ctx)
+    want = oracle.denoise(buf, kind, quality)
+    _check(got, want, f"{kind} {quality} {W}x{H} {spp} spp")
 
 
-def test_nlm_with_sky_and_one_sample(gpu_ctx, oracle):
+@pytest.mark.parametrize("kind", ["nlm", "nfor"])
+def test_with_sky_and_one_sample(gpu_ctx, oracle, kind):
     # an open scene (sky pixels: depth -1) and spp = 1 (samples_b = 0)
     W, H = 40, 24
     desc = scenes.displaced_icosphere_scene(W, H, subdiv=3, sphere_lights=2)
     for spp in (1, 4):
         buf = _film(gpu_ctx, desc, W, H, spp)
         assert (buf.depth_a < 0).any()
-        got = denoise.denoise(buf, denoise.DenoiserOptions("nlm", "fast"), gpu_ctx)
-        _check(got, oracle.denoise(buf, "nlm", "fast"), f"nlm sky {spp} spp")
+        got = denoise.denoise(buf, denoise.DenoiserOptions(kind, "fast"), gpu_ctx)
+        _check(got, oracle.denoise(buf, kind, "fast"), f"{kind} sky {spp} spp")
+
+
+def test_nfor_selection_margin(gpu_ctx, oracle):
+    W, H = 33, 31
+    buf = _film(gpu_ctx, scenes.cornell(W, H), W, H, 6)
+    for margin in (0.0, 0.5, 2.0):
+        got = denoise.denoise(buf, denoise.DenoiserOptions("nfor", "high", margin), gpu_ctx)
+        _check(got, oracle.denoise(buf, "nfor", "high", margin), f"nfor margin {margin}")
 
 
 def test_denoise_argument_errors(gpu_ctx):
