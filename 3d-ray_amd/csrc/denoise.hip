@@ -15,14 +15,15 @@
 //              (NforDenoiser.JointNlm, :66-84)
 //
 // NL-means, offset-decomposed as the reference does it, in two kernels per
-// chunk of search offsets: k_nlm_rows gives each (row, offset) pair one
-// thread that walks the row with the reference's running patch sum (its
-// rounding depends on the whole walk, so a row cannot be split) and writes
-// one horizontally averaged plane per offset; k_nlm_accum gives each pixel
-// one thread that takes the offsets in the reference's order (dy major),
-// sums the vertical patch, forms the weight and accumulates.  The planes of
-// a chunk of up to 128 offsets (1 GB at 1920x1080) live in HBM between the
-// two kernels.  Every fp32 operation is the reference's, in its order; its
+// batch of filterings (the six feature-prefilter passes run as one batch,
+// the two halves of the main filter as another): k_nlm_rows gives each
+// (filtering, row, offset) one thread that walks the row with the
+// reference's running patch sum (its rounding depends on the whole walk, so
+// a row cannot be split) and writes one horizontally averaged plane per
+// offset; k_nlm_accum gives each (filtering, pixel) one thread that takes
+// the offsets in the reference's order (dy major), sums the vertical patch,
+// forms the weight and accumulates.  The planes of a batch (up to 768, 6.4
+// GB at 1920x1080) live in HBM between the two kernels.  Every fp32 operation is the reference's, in its order; its
 // Vector<float> body (8 wide on x64) and scalar tail differ in one rounding
 // (reciprocal multiply vs division of the vertical patch sum), and the
 // same split is made per row.
@@ -51,7 +52,6 @@ constexpr int kVecW = 8;            // Vector<float>.Count on x64
 constexpr float kEps = 1e-10f;      // NlMeansCore.Epsilon
 constexpr float kCutoff = 10.0f;    // NlMeansCore.WeightCutoff
 constexpr int kPatch = 3;           // DenoiserOptions.PatchRadius
-constexpr int kChunk = 128;         // offsets per k_nlm_rows / k_nlm_accum pair
 constexpr int kBlk = 256;
 
 __device__ inline float fast_exp_neg(float d) {
@@ -90,10 +90,24 @@ __device__ inline void offset_region(const Geom& G, int o, int& dx, int& dy, int
     y1 = G.h - 1 - max(0, dy);
 }
 
-// Pass 1 of NlMeansCore.Filter for every (row, offset) of a chunk: pointwise
-// distances and the running horizontal patch average (NlMeansCore.cs:177-195).
-__global__ void __launch_bounds__(kBlk) k_nlm_rows(const float* __restrict__ g, const float* __restrict__ v, int gc,
-                                                   Geom G, float* __restrict__ dh) {
+// One NL-means filtering of a batch: guide g / variance v (gc planes), target
+// t (tc <= 3 planes), output out; the dh planes of the batch's job j start at
+// dh + j * no * n.
+struct NlmJob {
+    const float* g;
+    const float* v;
+    const float* t;
+    float* out;
+    int gc, tc, pad0, pad1;
+};
+
+// Pass 1 of NlMeansCore.Filter for every (job, row, offset) of a chunk:
+// pointwise distances and the running horizontal patch average
+// (NlMeansCore.cs:177-195).  Each distance is evaluated once; the running
+// sum subtracts it 2P+1 steps later from a per-thread ring of 8 in LDS.
+__global__ void __launch_bounds__(kBlk) k_nlm_rows(const NlmJob* __restrict__ jobs, Geom G, float* __restrict__ dh) {
+    __shared__ float ring[kBlk * 8];
+    const NlmJob J = jobs[blockIdx.y];
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (int64_t)G.h * G.no) return;
     const int ol = (int)(tid % G.no);
@@ -104,37 +118,54 @@ __global__ void __launch_bounds__(kBlk) k_nlm_rows(const float* __restrict__ g, 
     const int64_t n = (int64_t)G.w * G.h;
     const int64_t row = (int64_t)y * G.w;
     const int64_t qOff = (int64_t)dy * G.w + dx;
-    const float invC = 1.0f / gc;
-    float* out = dh + (int64_t)ol * n + row;
+    const float invC = 1.0f / J.gc;
+    float* out = dh + ((int64_t)blockIdx.y * G.no + ol) * n + row;
+    float* rg = ring + threadIdx.x * 8;
     constexpr int P = kPatch;
+    static_assert(2 * P + 1 < 8, "ring of 8");
     float run = 0.0f;
     const int warm = min(x0 + P, x1);
-    for (int xx = x0; xx <= warm; xx++) run += dpt_at(g, v, n, gc, invC, row + xx, row + xx + qOff);
+    for (int xx = x0; xx <= warm; xx++) {
+        const float d = dpt_at(J.g, J.v, n, J.gc, invC, row + xx, row + xx + qOff);
+        rg[xx & 7] = d;
+        run += d;
+    }
     for (int x = x0; x <= x1; x++) {
         const int lo = max(x0, x - P), hi = min(x1, x + P);
         out[x] = run / (float)(hi - lo + 1);
-        if (x + P + 1 <= x1) run += dpt_at(g, v, n, gc, invC, row + x + P + 1, row + x + P + 1 + qOff);
-        if (x - P >= x0) run -= dpt_at(g, v, n, gc, invC, row + x - P, row + x - P + qOff);
+        if (x + P + 1 <= x1) {
+            const int m = x + P + 1;
+            const float d = dpt_at(J.g, J.v, n, J.gc, invC, row + m, row + m + qOff);
+            rg[m & 7] = d;
+            run += d;
+        }
+        if (x - P >= x0) run -= rg[(x - P) & 7];
     }
 }
 
 // Pass 2 of NlMeansCore.Filter for one chunk of offsets (NlMeansCore.cs:199-263),
-// per pixel in offset order; num/den carry over between chunks in memory, the
-// last chunk normalises (:268-281).  tc <= 3.
-__global__ void __launch_bounds__(kBlk) k_nlm_accum(Geom G, const float* __restrict__ dh, int tc,
-                                                    const float* __restrict__ t, float invK2, int nf,
+// per (job, pixel) in offset order; num/den carry over between chunks in
+// memory, the last chunk normalises (:268-281).
+__global__ void __launch_bounds__(kBlk) k_nlm_accum(const NlmJob* __restrict__ jobs, Geom G,
+                                                    const float* __restrict__ dh, float invK2, int nf,
                                                     const float* __restrict__ feat, const float* __restrict__ fbw,
                                                     float* __restrict__ num, float* __restrict__ den, int first,
-                                                    int last, float* __restrict__ out) {
+                                                    int last) {
+    const NlmJob J = jobs[blockIdx.y];
     const int64_t n = (int64_t)G.w * G.h;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    const int tc = J.tc;
+    const float* __restrict__ t = J.t;
+    float* nm = num + (int64_t)blockIdx.y * 3 * n;
+    float* dn = den + (int64_t)blockIdx.y * n;
+    const float* dhj = dh + (int64_t)blockIdx.y * G.no * n;
     const int x = (int)(p % G.w), y = (int)(p / G.w);
     float acc[3] = {0.0f, 0.0f, 0.0f};
     float dsum = 0.0f;
     if (!first) {
-        for (int c = 0; c < tc; c++) acc[c] = num[(int64_t)c * n + p];
-        dsum = den[p];
+        for (int c = 0; c < tc; c++) acc[c] = nm[(int64_t)c * n + p];
+        dsum = dn[p];
     }
     constexpr int P = kPatch;
     for (int ol = 0; ol < G.no; ol++) {
@@ -143,7 +174,7 @@ __global__ void __launch_bounds__(kBlk) k_nlm_accum(Geom G, const float* __restr
         if (x0 > x1 || y0 > y1 || x < x0 || x > x1 || y < y0 || y > y1) continue;
         const int vlo = max(y0, y - P), vhi = min(y1, y + P);
         const int cnt = vhi - vlo + 1;
-        const float* col = dh + (int64_t)ol * n + x;
+        const float* col = dhj + (int64_t)ol * n + x;
         float sum = 0.0f;
         for (int yy = vlo; yy <= vhi; yy++) sum += col[(int64_t)yy * G.w];
         const int nvec = ((x1 - x0 + 1) / kVecW) * kVecW;
@@ -160,10 +191,10 @@ __global__ void __launch_bounds__(kBlk) k_nlm_accum(Geom G, const float* __restr
     }
     if (last) {
         const float invDen = dsum > 0.0f ? 1.0f / dsum : 0.0f;
-        for (int c = 0; c < tc; c++) out[(int64_t)c * n + p] = acc[c] * invDen;
+        for (int c = 0; c < tc; c++) J.out[(int64_t)c * n + p] = acc[c] * invDen;
     } else {
-        for (int c = 0; c < tc; c++) num[(int64_t)c * n + p] = acc[c];
-        den[p] = dsum;
+        for (int c = 0; c < tc; c++) nm[(int64_t)c * n + p] = acc[c];
+        dn[p] = dsum;
     }
 }
 
@@ -190,20 +221,23 @@ __device__ inline uint32_t nonneg_key(float v) {
     return b == 0x80000000u ? 0u : b;
 }
 
-__global__ void __launch_bounds__(kBlk) k_pct_hist_hi(const float* x, int64_t n, int32_t* hist) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float v = x[i];
-    if (v >= 0.0f) atomicAdd(&hist[nonneg_key(v) >> 16], 1);
-}
-
-__global__ void __launch_bounds__(kBlk) k_pct_hist_lo(const float* x, int64_t n, uint32_t hi, int32_t* hist) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float v = x[i];
-    if (!(v >= 0.0f)) return;
-    const uint32_t k = nonneg_key(v);
-    if ((k >> 16) == hi) atomicAdd(&hist[k & 0xffffu], 1);
+// histogram of one digit of the order keys of the non-negative values whose
+// higher bits equal `prefix` (under `mask`); per-block counts in LDS
+constexpr int kDigitBins = 2048;
+__global__ void __launch_bounds__(kBlk) k_pct_hist(const float* x, int64_t n, int shift, uint32_t dmask,
+                                                   uint32_t mask, uint32_t prefix, int32_t* hist) {
+    __shared__ int32_t lh[kDigitBins];
+    for (int i = threadIdx.x; i < kDigitBins; i += blockDim.x) lh[i] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = x[i];
+        if (!(v >= 0.0f)) continue;
+        const uint32_t k = nonneg_key(v);
+        if ((k & mask) == prefix) atomicAdd(&lh[(k >> shift) & dmask], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kDigitBins; i += blockDim.x)
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
 }
 
 // NforDenoiser.NormaliseDepth: sky (-1) at the far plane, else min(v/far, 1)
@@ -320,7 +354,8 @@ __global__ void __launch_bounds__(kBlk) k_nfor_dist(const float* __restrict__ g,
 // window in offset order, the regularised Cholesky solve in double
 // (RegressionSolver.cs:19-66) and the coefficients (NforRegression.cs:383-461).
 // Only the lower triangle of the Gram matrix is formed: the solver reads no
-// other element.  Weights go to W (offset-major, 0 = no contribution).
+// other element.  Weights go to W (offset-major like dist, so a wave's
+// stores are contiguous; 0 = no contribution).
 __global__ void __launch_bounds__(kBlk) k_nfor_solve(const float* __restrict__ dist, const float* __restrict__ feat,
                                                      const float* __restrict__ t, NforGeom G, float invK2,
                                                      float* __restrict__ W, float* __restrict__ beta) {
@@ -484,7 +519,7 @@ __global__ void __launch_bounds__(kBlk) k_scale(const float* src, int64_t n, con
 __global__ void __launch_bounds__(kBlk) k_box_rows(const float* src, float* tmp, int w, int h, int r) {
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= h) return;
-    const int64_t row = (int64_t)y * w;
+    const int64_t row = (int64_t)blockIdx.y * w * h + (int64_t)y * w;
     float run = 0.0f;
     for (int x = 0; x < min(r + 1, w); x++) run += src[row + x];
     for (int x = 0; x < w; x++) {
@@ -498,12 +533,13 @@ __global__ void __launch_bounds__(kBlk) k_box_rows(const float* src, float* tmp,
 __global__ void __launch_bounds__(kBlk) k_box_cols(const float* tmp, float* dst, int w, int h, int r) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)w * h) return;
+    const int64_t base = (int64_t)blockIdx.y * w * h;
     const int x = (int)(i % w), y = (int)(i / w);
     const int lo = max(0, y - r), hi = min(h - 1, y + r);
     const float inv = 1.0f / (hi - lo + 1);
     float sum = 0.0f;
-    for (int yy = lo; yy <= hi; yy++) sum += tmp[(int64_t)yy * w + x];
-    dst[i] = sum * inv;
+    for (int yy = lo; yy <= hi; yy++) sum += tmp[base + (int64_t)yy * w + x];
+    dst[base + i] = sum * inv;
 }
 
 // candidate MSE maps before smoothing (NforRegression.cs:95-133)
@@ -529,8 +565,8 @@ __global__ void __launch_bounds__(kBlk) k_mse_filtered(const float* fa, const fl
     tmp[i] = (e + margin * vSum) * (1.0f / 3.0f);
 }
 
-// indicator of candidate ci being the (first) smallest smoothed MSE
-__global__ void __launch_bounds__(kBlk) k_indicator(const float* mse, int ncand, int ci, int64_t n, float* ind) {
+// indicator planes: 1 where candidate ci has the (first) smallest smoothed MSE
+__global__ void __launch_bounds__(kBlk) k_indicator(const float* mse, int ncand, int64_t n, float* ind) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float best = mse[i];
@@ -540,7 +576,7 @@ __global__ void __launch_bounds__(kBlk) k_indicator(const float* mse, int ncand,
             best = mse[cj * n + i];
             bi = cj;
         }
-    ind[i] = bi == ci ? 1.0f : 0.0f;
+    for (int ci = 0; ci < ncand; ci++) ind[ci * n + i] = bi == ci ? 1.0f : 0.0f;
 }
 
 // softened blend of the candidates, clamped at zero (NforRegression.cs:136-167)
@@ -574,6 +610,9 @@ namespace {
         if (e_ != hipSuccess) return ctx_fail(ctx, RT_ERR_DEVICE, hipGetErrorString(e_)); \
     } while (0)
 
+constexpr int kMaxPlanes = 768;   // dh planes per k_nlm_rows / k_nlm_accum pair (jobs x offsets)
+constexpr int kMaxJobs = 8;
+
 struct Arena {
     char* base;
     size_t off = 0;
@@ -589,11 +628,13 @@ struct Pipe {
     hipStream_t st;
     int w, h;
     int64_t n;
-    float* dh;       // kChunk planes
-    float* num;      // 3 planes
-    float* den;      // 1 plane
-    int32_t* hist;   // 65536 bins
-    float* bw;       // 8 inverse bandwidths (device)
+    float* dh;        // kMaxPlanes planes
+    float* num;       // kMaxJobs x 3 planes
+    float* den;       // kMaxJobs planes
+    int32_t* hist;    // kDigitBins bins
+    float* bw;        // 8 inverse bandwidths
+    NlmJob* jobs;     // kMaxJobs job records (device): slots 0-5 prefilter, 6-7 main filter
+    NlmJob host_jobs[8];   // their host copies, alive until the call's final synchronisation
 };
 
 // PlaneOps.Combine of two buffers of len floats (nB <= 0: copy of a)
@@ -621,74 +662,96 @@ int half_variance(Pipe& Q, const float* A, const float* B, const float* mean, in
     return RT_OK;
 }
 
-// NlMeansCore.Filter: guide g/var v (gc planes), target t (tc planes) -> out
-int nlm(Pipe& Q, const float* g, const float* v, int gc, const float* t, int tc, int R, float K, int nf,
-        const float* feat, float* out) {
-    const int side = 2 * R + 1, offsets = side * side;
+// NlMeansCore.Filter for a batch of jobs with one search radius, strength
+// and feature guide set, offsets in chunks of at most kMaxPlanes / jobs; the
+// batch's job records go to slots [slot0, slot0 + jobs) (stream-ordered copy)
+int nlm(Pipe& Q, const std::vector<NlmJob>& jobs, int slot0, int R, float K, int nf, const float* feat) {
+    rt_ctx* ctx = Q.ctx;
+    const int nj = (int)jobs.size();
+    if (slot0 + nj > kMaxJobs) return ctx_fail(ctx, RT_ERR_INTERNAL, "too many NL-means jobs");
+    for (int j = 0; j < nj; j++) Q.host_jobs[slot0 + j] = jobs[j];
+    DN_TRY(hipMemcpyAsync(Q.jobs + slot0, Q.host_jobs + slot0, sizeof(NlmJob) * nj, hipMemcpyHostToDevice, Q.st));
+    const NlmJob* dj = Q.jobs + slot0;
+    const int side = 2 * R + 1, offsets = side * side, chunk = kMaxPlanes / nj;
     const float invK2 = 1.0f / (K * K);
-    for (int o0 = 0; o0 < offsets; o0 += kChunk) {
-        Geom G{Q.w, Q.h, R, side, o0, std::min(kChunk, offsets - o0)};
+    for (int o0 = 0; o0 < offsets; o0 += chunk) {
+        Geom G{Q.w, Q.h, R, side, o0, std::min(chunk, offsets - o0)};
         const int64_t rows = (int64_t)Q.h * G.no;
-        hipLaunchKernelGGL(k_nlm_rows, dim3(blocks(rows)), dim3(kBlk), 0, Q.st, g, v, gc, G, Q.dh);
-        hipLaunchKernelGGL(k_nlm_accum, dim3(blocks(Q.n)), dim3(kBlk), 0, Q.st, G, (const float*)Q.dh, tc, t, invK2,
-                           nf, feat, (const float*)Q.bw, Q.num, Q.den, o0 == 0 ? 1 : 0,
-                           o0 + kChunk >= offsets ? 1 : 0, out);
+        hipLaunchKernelGGL(k_nlm_rows, dim3(blocks(rows), nj), dim3(kBlk), 0, Q.st, dj, G, Q.dh);
+        hipLaunchKernelGGL(k_nlm_accum, dim3(blocks(Q.n), nj), dim3(kBlk), 0, Q.st, dj, G,
+                           (const float*)Q.dh, invK2, nf, feat, (const float*)Q.bw, Q.num, Q.den, o0 == 0 ? 1 : 0,
+                           o0 + chunk >= offsets ? 1 : 0);
     }
     return RT_OK;
 }
 
-// FeaturePrefilter.Prefilter: cross NL-means of the halves (R 5, k 1)
-int prefilter(Pipe& Q, const float* A, const float* B, int c, int nA, int nB, float* mean, float* hv, float* s0,
-              float* s1, float* fa, float* fb, float* out) {
-    const int64_t len = (int64_t)c * Q.n;
-    int rc = combine(Q, A, B, len, nA, nB, mean);
-    if (!rc) rc = half_variance(Q, A, B, mean, c, s0, s1, hv);
-    if (!rc) rc = nlm(Q, B, hv, c, A, c, 5, 1.0f, 0, nullptr, fa);
-    if (!rc) rc = nlm(Q, A, hv, c, B, c, 5, 1.0f, 0, nullptr, fb);
-    if (!rc) rc = combine(Q, fa, fb, len, nA, nB, out);
+// FeaturePrefilter.Prefilter of the albedo, normal and depth halves at once:
+// per feature the mean, the half variance, then the six cross NL-means
+// filterings (R 5, k 1) in one batch, then the sample-weighted combination.
+struct FeatureHalves {
+    const float* A;
+    const float* B;
+    int c;
+    float *mean, *hv, *fa, *fb, *out;
+};
+
+int prefilter(Pipe& Q, FeatureHalves* F, int nfeat, int nA, int nB, float* s0, float* s1) {
+    std::vector<NlmJob> jobs;
+    for (int i = 0; i < nfeat; i++) {
+        const int64_t len = (int64_t)F[i].c * Q.n;
+        int rc = combine(Q, F[i].A, F[i].B, len, nA, nB, F[i].mean);
+        if (!rc) rc = half_variance(Q, F[i].A, F[i].B, F[i].mean, F[i].c, s0, s1, F[i].hv);
+        if (rc) return rc;
+        jobs.push_back(NlmJob{F[i].B, F[i].hv, F[i].A, F[i].fa, F[i].c, F[i].c, 0, 0});
+        jobs.push_back(NlmJob{F[i].A, F[i].hv, F[i].B, F[i].fb, F[i].c, F[i].c, 0, 0});
+    }
+    int rc = nlm(Q, jobs, 0, 5, 1.0f, 0, nullptr);
+    for (int i = 0; i < nfeat && !rc; i++) rc = combine(Q, F[i].fa, F[i].fb, (int64_t)F[i].c * Q.n, nA, nB, F[i].out);
     return rc;
 }
 
-// PlaneOps.PercentileOfNonNegative (0.99) by a two-pass radix select
+// PlaneOps.PercentileOfNonNegative (0.99): an exact radix select over the
+// order keys of the non-negative values, digits of 11, 11 and 10 bits
 int percentile99(Pipe& Q, const float* x, float* result) {
     rt_ctx* ctx = Q.ctx;
-    std::vector<int32_t> hist(65536);
-    DN_TRY(hipMemsetAsync(Q.hist, 0, 65536 * 4, Q.st));
-    hipLaunchKernelGGL(k_pct_hist_hi, dim3(blocks(Q.n)), dim3(kBlk), 0, Q.st, x, Q.n, Q.hist);
-    DN_TRY(hipMemcpyAsync(hist.data(), Q.hist, 65536 * 4, hipMemcpyDeviceToHost, Q.st));
-    DN_TRY(hipStreamSynchronize(Q.st));
-    int64_t count = 0;
-    for (int i = 0; i < 65536; i++) count += hist[i];
-    if (count == 0) {
-        *result = 1.0f;
-        return RT_OK;
+    std::vector<int32_t> hist(kDigitBins);
+    const int shifts[3] = {21, 10, 0};
+    uint32_t prefix = 0, mask = 0;
+    int64_t before = 0, k = 0;
+    const unsigned grid = (unsigned)std::min<int64_t>(1024, blocks(Q.n));
+    for (int pass = 0; pass < 3; pass++) {
+        const int nbins = pass == 2 ? 1024 : kDigitBins;
+        DN_TRY(hipMemsetAsync(Q.hist, 0, kDigitBins * 4, Q.st));
+        hipLaunchKernelGGL(k_pct_hist, dim3(grid), dim3(kBlk), 0, Q.st, x, Q.n, shifts[pass], (uint32_t)(nbins - 1),
+                           mask, prefix, Q.hist);
+        DN_TRY(hipMemcpyAsync(hist.data(), Q.hist, kDigitBins * 4, hipMemcpyDeviceToHost, Q.st));
+        DN_TRY(hipStreamSynchronize(Q.st));
+        if (pass == 0) {
+            int64_t count = 0;
+            for (int i = 0; i < kDigitBins; i++) count += hist[i];
+            if (count == 0) {
+                *result = 1.0f;
+                return RT_OK;
+            }
+            k = (int64_t)(0.99f * (float)(count - 1));
+            k = std::min<int64_t>(std::max<int64_t>(k, 0), count - 1);
+        }
+        int d = 0;
+        for (; d < nbins; d++) {
+            if (before + hist[d] > k) break;
+            before += hist[d];
+        }
+        prefix |= (uint32_t)d << shifts[pass];
+        mask |= (uint32_t)(nbins - 1) << shifts[pass];
     }
-    int64_t k = (int64_t)(0.99f * (float)(count - 1));
-    k = std::min<int64_t>(std::max<int64_t>(k, 0), count - 1);
-    uint32_t hi = 0;
-    int64_t before = 0;
-    for (; hi < 65536; hi++) {
-        if (before + hist[hi] > k) break;
-        before += hist[hi];
-    }
-    DN_TRY(hipMemsetAsync(Q.hist, 0, 65536 * 4, Q.st));
-    hipLaunchKernelGGL(k_pct_hist_lo, dim3(blocks(Q.n)), dim3(kBlk), 0, Q.st, x, Q.n, hi, Q.hist);
-    DN_TRY(hipMemcpyAsync(hist.data(), Q.hist, 65536 * 4, hipMemcpyDeviceToHost, Q.st));
-    DN_TRY(hipStreamSynchronize(Q.st));
-    uint32_t lo = 0;
-    for (; lo < 65536; lo++) {
-        if (before + hist[lo] > k) break;
-        before += hist[lo];
-    }
-    const uint32_t bits = (hi << 16) | lo;
-    memcpy(result, &bits, 4);
+    memcpy(result, &prefix, 4);
     return RT_OK;
 }
 
-// PlaneOps.BoxSmooth of one plane (tmp: one scratch plane)
-void box_smooth(Pipe& Q, const float* src, float* tmp, float* dst, int r) {
-    hipLaunchKernelGGL(k_box_rows, dim3(blocks(Q.h)), dim3(kBlk), 0, Q.st, src, tmp, Q.w, Q.h, r);
-    hipLaunchKernelGGL(k_box_cols, dim3(blocks(Q.n)), dim3(kBlk), 0, Q.st, tmp, dst, Q.w, Q.h, r);
+// PlaneOps.BoxSmooth of `planes` consecutive planes (tmp: as many scratch planes)
+void box_smooth(Pipe& Q, const float* src, float* tmp, float* dst, int r, int planes) {
+    hipLaunchKernelGGL(k_box_rows, dim3(blocks(Q.h), planes), dim3(kBlk), 0, Q.st, src, tmp, Q.w, Q.h, r);
+    hipLaunchKernelGGL(k_box_cols, dim3(blocks(Q.n), planes), dim3(kBlk), 0, Q.st, tmp, dst, Q.w, Q.h, r);
 }
 
 // NforRegression.BuildScaledFeatures (NforRegression.cs:173-193): the global
@@ -716,11 +779,9 @@ int scale_features(Pipe& Q, const float* feat, float* scales_dev, float* out) {
             scales[pl] = 1.0f / std::max(sd, 1e-4f);
         });
     for (auto& t : th) t.join();
-    DN_TRY(hipMemcpyAsync(scales_dev, scales, sizeof(scales), hipMemcpyHostToDevice, Q.st));
+    DN_TRY(hipMemcpy(scales_dev, scales, sizeof(scales), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_scale, dim3(blocks(kFeat * n)), dim3(kBlk), 0, Q.st, feat, n, (const float*)scales_dev, kFeat,
                        out);
-    // the host vector must outlive the upload of `scales`
-    DN_TRY(hipStreamSynchronize(Q.st));
     return RT_OK;
 }
 
@@ -764,18 +825,15 @@ int nfor(Pipe& Q, const NforBufs& B, const float* bA, const float* bB, const flo
                      B.cand + (int64_t)(ki + 1) * 3 * n);
         if (rc) return rc;
     }
+    // per-candidate MSE maps (tmp: ncand planes), smoothed together
     hipLaunchKernelGGL(k_mse_noisy, dim3(blocks(n)), dim3(kBlk), 0, Q.st, bhv, n, B.tmp);
-    box_smooth(Q, B.tmp, B.tmp2, B.mse, 4);
-    for (int ki = 0; ki < nk; ki++) {
+    for (int ki = 0; ki < nk; ki++)
         hipLaunchKernelGGL(k_mse_filtered, dim3(blocks(n)), dim3(kBlk), 0, Q.st,
                            (const float*)(B.reg + (int64_t)ki * 3 * n), (const float*)(B.reg + (int64_t)(nk + ki) * 3 * n),
-                           bA, bB, bhv, n, margin, B.tmp);
-        box_smooth(Q, B.tmp, B.tmp2, B.mse + (int64_t)(ki + 1) * n, 4);
-    }
-    for (int ci = 0; ci < ncand; ci++) {
-        hipLaunchKernelGGL(k_indicator, dim3(blocks(n)), dim3(kBlk), 0, Q.st, (const float*)B.mse, ncand, ci, n, B.tmp);
-        box_smooth(Q, B.tmp, B.tmp2, B.wts + (int64_t)ci * n, 2);
-    }
+                           bA, bB, bhv, n, margin, B.tmp + (int64_t)(ki + 1) * n);
+    box_smooth(Q, B.tmp, B.tmp2, B.mse, 4, ncand);
+    hipLaunchKernelGGL(k_indicator, dim3(blocks(n)), dim3(kBlk), 0, Q.st, (const float*)B.mse, ncand, n, B.tmp);
+    box_smooth(Q, B.tmp, B.tmp2, B.wts, 2, ncand);
     hipLaunchKernelGGL(k_blend, dim3(blocks(n)), dim3(kBlk), 0, Q.st, (const float*)B.wts, (const float*)B.cand, ncand,
                        n, res);
     return RT_OK;
@@ -798,13 +856,15 @@ extern "C" int rt_denoise(rt_ctx* ctx, int32_t width, int32_t height, const rt_f
         return ctx_fail(ctx, RT_ERR_INVALID_ARGUMENT, "unexpected denoise quality");
     const int nA = opts->samples_a, nB = opts->samples_b;
     if (nA <= 0 || nB < 0) return ctx_fail(ctx, RT_ERR_INVALID_ARGUMENT, "samples_a must be > 0 and samples_b >= 0");
+    if ((int64_t)width * height > (1ll << 28)) return ctx_fail(ctx, RT_ERR_INVALID_ARGUMENT, "film too large");
     DN_TRY(hipSetDevice(ctx_device(ctx)));
 
     Pipe Q{ctx, ctx_stream(ctx), width, height, (int64_t)width * height};
     const int64_t n = Q.n;
-    // device layout (floats): inputs 20 n, features 7 n, temporaries, chunk planes
-    float *bA, *bB, *aA, *aB, *nmA, *nmB, *dA, *dB, *comb, *dNa, *dNb, *feat, *mean, *hv, *s0, *s1, *fa, *fb, *bmean,
-        *bhv, *oa, *ob, *res;
+    // device layout (floats): inputs 20 n, per-feature temporaries 28 n,
+    // features 7 n, beauty temporaries, kMaxPlanes dh planes, NFOR buffers
+    float *bA, *bB, *aA, *aB, *nmA, *nmB, *dA, *dB, *comb, *dNa, *dNb, *feat, *ftmp, *s0, *s1, *bmean, *bhv, *oa,
+        *ob, *res;
     NforBufs NB{};
     size_t total = 0;
     for (int pass = 0; pass < 2; pass++) {
@@ -818,20 +878,21 @@ extern "C" int rt_denoise(rt_ctx* ctx, int32_t width, int32_t height, const rt_f
         Arena A{base};
         bA = A.f(3 * n), bB = A.f(3 * n), aA = A.f(3 * n), aB = A.f(3 * n), nmA = A.f(3 * n), nmB = A.f(3 * n);
         dA = A.f(n), dB = A.f(n), comb = A.f(n), dNa = A.f(n), dNb = A.f(n), feat = A.f(7 * n);
-        mean = A.f(3 * n), hv = A.f(3 * n), s0 = A.f(n), s1 = A.f(n), fa = A.f(3 * n), fb = A.f(3 * n);
+        ftmp = A.f(28 * n), s0 = A.f(n), s1 = A.f(n);
         bmean = A.f(3 * n), bhv = A.f(3 * n), oa = A.f(3 * n), ob = A.f(3 * n), res = A.f(3 * n);
-        Q.dh = A.f((int64_t)kChunk * n);
-        Q.num = A.f(3 * n);
-        Q.den = A.f(n);
-        Q.hist = (int32_t*)A.f(65536);
+        Q.dh = A.f((int64_t)kMaxPlanes * n);
+        Q.num = A.f((int64_t)kMaxJobs * 3 * n);
+        Q.den = A.f((int64_t)kMaxJobs * n);
+        Q.hist = (int32_t*)A.f(kDigitBins);
         Q.bw = A.f(8);
+        Q.jobs = (NlmJob*)A.f(kMaxJobs * (int64_t)sizeof(NlmJob) / 4);
         if (kind == RT_DENOISE_NFOR) {
             const int R = quality == RT_DENOISE_HIGH ? 9 : 7, nk = quality == RT_DENOISE_HIGH ? 2 : 1;
             const int64_t offs = (int64_t)(2 * R + 1) * (2 * R + 1);
             const int64_t nc = (int64_t)((width + 1) / 2) * ((height + 1) / 2);
             NB.featS = A.f(kFeat * n), NB.dist = A.f(offs * nc), NB.W = A.f(offs * nc), NB.beta = A.f(nc * 3 * kDim);
             NB.reg = A.f(2 * nk * 3 * n), NB.cand = A.f((nk + 1) * 3 * n), NB.mse = A.f((nk + 1) * n);
-            NB.wts = A.f((nk + 1) * n), NB.tmp = A.f(n), NB.tmp2 = A.f(n), NB.scales = A.f(8);
+            NB.wts = A.f((nk + 1) * n), NB.tmp = A.f((nk + 1) * n), NB.tmp2 = A.f((nk + 1) * n), NB.scales = A.f(8);
         }
         total = A.off + 256;
     }
@@ -844,8 +905,8 @@ extern "C" int rt_denoise(rt_ctx* ctx, int32_t width, int32_t height, const rt_f
     DN_TRY(hipMemcpyAsync(nmB, film->normal_b, p3, hipMemcpyHostToDevice, Q.st));
     DN_TRY(hipMemcpyAsync(dA, film->depth_a, p1, hipMemcpyHostToDevice, Q.st));
     DN_TRY(hipMemcpyAsync(dB, film->depth_b, p1, hipMemcpyHostToDevice, Q.st));
-    const float bw[8] = {1.0f / 2e-3f, 1.0f / 2e-3f, 1.0f / 2e-3f, 1.0f / 1e-2f,
-                         1.0f / 1e-2f, 1.0f / 1e-2f, 1.0f / 2e-3f, 0.0f};
+    static const float bw[8] = {1.0f / 2e-3f, 1.0f / 2e-3f, 1.0f / 2e-3f, 1.0f / 1e-2f,
+                                1.0f / 1e-2f, 1.0f / 1e-2f, 1.0f / 2e-3f, 0.0f};
     DN_TRY(hipMemcpyAsync(Q.bw, bw, sizeof(bw), hipMemcpyHostToDevice, Q.st));
 
     // 1. depth -> [0, 1]
@@ -859,22 +920,23 @@ extern "C" int rt_denoise(rt_ctx* ctx, int32_t width, int32_t height, const rt_f
         const float far = std::max(1.05f * p99, 1e-6f), invFar = 1.0f / far;
         hipLaunchKernelGGL(k_depth_norm, dim3(blocks(n)), dim3(kBlk), 0, Q.st, dA, dB, n, invFar, dNa, dNb);
     }
-    // 2. prefiltered features: albedo, normal, depth planes back to back
-    int rc = prefilter(Q, aA, aB, 3, nA, nB, mean, hv, s0, s1, fa, fb, feat);
-    if (!rc) rc = prefilter(Q, nmA, nmB, 3, nA, nB, mean, hv, s0, s1, fa, fb, feat + 3 * n);
-    if (!rc) rc = prefilter(Q, dNa, dNb, 1, nA, nB, mean, hv, s0, s1, fa, fb, feat + 6 * n);
+    // 2. prefiltered features: albedo, normal, depth planes back to back in feat
+    FeatureHalves F[3] = {{aA, aB, 3, ftmp, ftmp + 3 * n, ftmp + 6 * n, ftmp + 9 * n, feat},
+                          {nmA, nmB, 3, ftmp + 12 * n, ftmp + 15 * n, ftmp + 18 * n, ftmp + 21 * n, feat + 3 * n},
+                          {dNa, dNb, 1, ftmp + 24 * n, ftmp + 25 * n, ftmp + 26 * n, ftmp + 27 * n, feat + 6 * n}};
+    int rc = prefilter(Q, F, 3, nA, nB, s0, s1);
     // 3. beauty variance
     if (!rc) rc = combine(Q, bA, bB, 3 * n, nA, nB, bmean);
     if (!rc) rc = half_variance(Q, bA, bB, bmean, 3, s0, s1, bhv);
-    // 4. main filter: NFOR regression or joint NL-means
+    // 4. main filter: NFOR regression or joint NL-means (both halves in one batch)
     if (!rc && kind == RT_DENOISE_NFOR) {
         const float margin = opts->selection_margin >= 0.0f ? opts->selection_margin : 0.5f;
         rc = nfor(Q, NB, bA, bB, bmean, bhv, feat, quality, margin, nA, nB, res);
     } else if (!rc) {
         const float k = quality == RT_DENOISE_HIGH ? 1.0f : 0.7f;
         const int R = quality == RT_DENOISE_HIGH ? 9 : 7;
-        rc = nlm(Q, bB, bhv, 3, bA, 3, R, k, 7, feat, oa);
-        if (!rc) rc = nlm(Q, bA, bhv, 3, bB, 3, R, k, 7, feat, ob);
+        std::vector<NlmJob> jobs = {NlmJob{bB, bhv, bA, oa, 3, 3, 0, 0}, NlmJob{bA, bhv, bB, ob, 3, 3, 0, 0}};
+        rc = nlm(Q, jobs, 6, R, k, 7, feat);
         if (!rc) rc = combine(Q, oa, ob, 3 * n, nA, nB, res);
     }
     if (rc) return rc;

@@ -320,6 +320,75 @@ def run_film(args):
     print(json.dumps(out), flush=True)
 
 
+def run_denoise(args):
+    """Denoiser workload: a 1920x1080 film of the Cornell box at 8 spp with
+    AOVs, denoised with NFOR at high quality (the reference's default:
+    feature prefilter, two regression candidates over a 19x19 window,
+    per-pixel selection) and with joint NL-means.  value = NFOR megapixels
+    per second, host planes in and out (PCIe included).  The CPU baseline is
+    the oracle (the C++ restatement, one thread) on a 480x270 crop."""
+    from raytracer import denoise as dn
+    W, H, spp = 1920, 1080, 8
+    ctx = native.Context(0)
+    sc = native.Scene(ctx, scenes.cornell(W, H))
+    buf = native.Renderer(sc, spp, 8).render(W, H, capture_aovs=True).buffers
+    sc.close()
+    opt_nfor, opt_nlm = dn.DenoiserOptions("nfor", "high"), dn.DenoiserOptions("nlm", "high")
+    for _ in range(args.warmup):
+        dn.denoise(buf, opt_nfor, ctx)
+    tn = 0.0
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        dn.denoise(buf, opt_nfor, ctx)
+        dt = time.perf_counter() - t0
+        tn += dt
+        log(f"step {k}: nfor {1e3 * dt:.1f} ms")
+    t0 = time.perf_counter()
+    dn.denoise(buf, opt_nlm, ctx)
+    t_nlm = time.perf_counter() - t0
+    ms = tn * 1e3 / args.steps
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        class Crop:
+            pass
+
+        cw, ch = 480, 270
+        c = Crop()
+        for k in ("beauty_a", "beauty_b", "albedo_a", "albedo_b", "normal_a", "normal_b", "depth_a", "depth_b"):
+            a = getattr(buf, k)
+            setattr(c, k, np.ascontiguousarray(a[..., 405:405 + ch, 720:720 + cw]))
+        c.samples_a, c.samples_b = buf.samples_a, buf.samples_b
+        t0 = time.perf_counter()
+        O.denoise(c, "nfor", "high")
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(cw * ch / tc / 1e6, 4), "unit": "Mpixels/s", "cores": 1, "kind": "port",
+               "sample": f"NFOR high on a {cw}x{ch} crop of the same film with the oracle (C++ restatement, "
+                         f"one thread): {tc:.1f} s"}
+    out = {
+        "metric": "Mpixels/s NFOR denoise 1080p",
+        "value": round(W * H / (ms / 1e3) / 1e6, 3),
+        "unit": "Mpixels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 solves)",
+        "data": "synthetic (Cornell box film, 8 spp with albedo/normal/depth halves)",
+        "config": {"workload": "1920x1080 film, NFOR high: prefilter R5, 2 candidates, 19x19 window, 7x7 patches",
+                   "width": W, "height": H, "spp": spp},
+        "nlm_ms": round(t_nlm * 1e3, 2),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,12 +400,13 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--subdiv", type=int, default=8)
     ap.add_argument("--pool", type=int, default=0, help="path slots in flight (0 = library default, 2^24)")
-    ap.add_argument("--workload", choices=("icosphere", "caustics", "batch", "adjoint", "film"), default="icosphere",
+    ap.add_argument("--workload", choices=("icosphere", "caustics", "batch", "adjoint", "film", "denoise"),
+                    default="icosphere",
                     help="icosphere: the 1.31M-triangle Disney scene (headline); caustics: glass spheres, "
                          "Gerstner water, fog, photon-map caustics; batch: 64 scenes of uneven size, batched "
                          "vs plain per-scene renders; adjoint: texture gradients of the 81,920-triangle textured "
                          "icosphere (256^2, 32 spp, depth 4, 512^2 texture set), adjoint vs forward cost; film: "
-                         "1080p multilayer EXR and PNG encoding")
+                         "1080p multilayer EXR and PNG encoding; denoise: NFOR / NL-means on a 1080p film")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     ap.add_argument("--photons", type=int, default=20_000_000)
@@ -377,6 +447,8 @@ def main():
         return run_adjoint(args)
     if args.workload == "film":
         return run_film(args)
+    if args.workload == "denoise":
+        return run_denoise(args)
     W, H, spp, depth = args.width, args.height, args.spp, args.depth
     if caus:
         log("generating scene (glass spheres, 80k-triangle Gerstner water, fog)")
