@@ -1,0 +1,332 @@
+"""YAML scene front-end, shaped like the reference's SceneLoader.Load
+(src/RayTracer/Scene/SceneLoader.cs:122-553) for the part of the schema
+(SceneData.cs) the renderer implements:
+
+  world.sky        flat colour (BuildSkySettings :1595-1625; no sky = (0.5, 0.7, 1.0))
+  world.ground     infinite plane (BuildGround :1324-1374, ResolveGroundMaterial :1528-1564)
+  world.medium     homogeneous, isotropic phase (BuildMedium :4135, ParseSigmas :4206-4217)
+  materials        lambertian, dielectric, emissive, disney / disney_bsdf / pbr
+                   (CreateMaterial :1027-1074, MaterialData defaults SceneData.cs:470-717);
+                   an unknown type is a grey-default Lambertian, as in the reference
+  entities         sphere, quad, triangle, plane / infinite_plane (CreateEntity :2282-2325,
+                   CreateTriangleEntity :3430-3461) with scale / rotate / translate
+                   (ComputeTransformMatrix :3788-3822) and visible_to_camera
+  lights           sphere, area, directional / sun (CreateLight :3463-3636, with the
+                   emissive proxies it appends); unknown types are skipped
+  camera/cameras   ResolveCamera :731-781, focal_pos via ComputeFocusDistance :3744-3786
+  geometry lights  every non-proxy emissive primitive (ExtractGeometryLights :805),
+                   default 4 shadow samples (GeometryLight.DefaultShadowSamples)
+
+The assembled tables keep SceneLoader's object order — ground, entities,
+then the light proxies — which is the primitive order the renderer's IDs
+follow.  Transforms are applied to the primitive's defining points and
+vectors (the reference wraps the primitive in a ray-transforming
+Transform): the same surfaces up to float rounding.  Everything else in the
+schema (imports, templates, instances, groups, CSG, meshes, textures and
+maps, mix materials, metal, bound media, non-isotropic phases, point / spot
+/ portal lights, motion, sky models other than flat, ground shapes other
+than the plane) raises SceneLoadError naming the feature, so a scene is
+either loaded as the reference would assemble it or refused.
+"""
+import math
+
+import numpy as np
+
+from . import abi, scenes
+
+DEFAULT_SKY = (0.5, 0.7, 1.0)           # SceneLoader.DefaultSkyColor
+GEOMETRY_SHADOW_SAMPLES = 4             # GeometryLight.DefaultShadowSamples
+
+
+class SceneLoadError(ValueError):
+    pass
+
+
+def _v3(x, default=None):
+    if x is None:
+        return default
+    if len(x) < 3:
+        raise SceneLoadError(f"expected 3 components, got {x!r}")
+    return tuple(float(np.float32(c)) for c in x[:3])
+
+
+def _f(d, key, default):
+    v = d.get(key, default)
+    return default if v is None else float(np.float32(v))
+
+
+_UNSUPPORTED_MATERIAL_KEYS = ("texture", "normal_map", "bump_map", "coat_normal_map", "subsurface_radius",
+                              "displacement", "emission_texture")
+
+
+def _material(b, m):
+    """CreateMaterial for the supported kinds; returns the table index."""
+    typ = (m.get("type") or "").lower()
+    if typ in ("mix", "blend"):
+        raise SceneLoadError(f"material '{m.get('id')}': mix materials are not supported")
+    if typ == "metal":
+        raise SceneLoadError(f"material '{m.get('id')}': the metal material is not supported")
+    for k in m:
+        if k in _UNSUPPORTED_MATERIAL_KEYS or (k.endswith("_texture") and m[k] is not None):
+            raise SceneLoadError(f"material '{m.get('id')}': '{k}' (textures and maps) is not supported")
+    color = _v3(m.get("color"), (0.5, 0.5, 0.5))
+    if typ == "lambertian":
+        return b.lambertian(color)
+    if typ == "dielectric":
+        return b.dielectric(_f(m, "refraction_index", 1.5), color)
+    if typ == "emissive":
+        return b.emissive(color, _f(m, "intensity", 1.0))
+    if typ in ("disney", "disney_bsdf", "pbr"):
+        kw = dict(metallic=_f(m, "metallic", 0.0), roughness=_f(m, "roughness", 0.5),
+                  specular=_f(m, "specular", 0.5), specular_tint=_f(m, "specular_tint", 0.0),
+                  sheen=_f(m, "sheen", 0.0), sheen_tint=_f(m, "sheen_tint", 0.5),
+                  sheen_roughness=_f(m, "sheen_roughness", 0.3), clearcoat=_f(m, "clearcoat", 0.0),
+                  clearcoat_gloss=_f(m, "clearcoat_gloss", 1.0), spec_trans=_f(m, "spec_trans", 0.0),
+                  ior=_f(m, "ior", 1.5), anisotropic=_f(m, "anisotropic", 0.0),
+                  anisotropic_rotation=_f(m, "anisotropic_rotation", 0.0),
+                  transmission_depth=_f(m, "transmission_depth", 0.0), diff_trans=_f(m, "diff_trans", 0.0),
+                  coat_ior=_f(m, "coat_ior", 1.5), thin_film_thickness=_f(m, "thin_film_thickness", 0.0),
+                  thin_film_ior=_f(m, "thin_film_ior", 1.5))
+        if m.get("thin_walled"):
+            kw["thin_walled"] = True
+        if m.get("transmission_color") is not None:
+            kw["transmission_color"] = _v3(m["transmission_color"])
+        cr = _f(m, "coat_roughness", -1.0)
+        if cr >= 0.0:
+            kw["coat_roughness"] = max(cr, 0.0)
+        return b.disney(color, **kw)
+    return b.lambertian(color)
+
+
+def _transform(e):
+    """ComputeTransformMatrix: scale, then rotate about X, Y, Z (degrees),
+    then translate; as a 4x4 acting on column vectors, plus the uniform
+    scale factor (None when non-uniform)."""
+    M = np.eye(4)
+    uniform = 1.0
+    s = e.get("scale")
+    if s is not None:
+        sv = [float(s)] * 3 if not isinstance(s, (list, tuple)) else [float(x) for x in s[:3]]
+        M = np.diag(sv + [1.0]) @ M
+        uniform = sv[0] if sv[0] == sv[1] == sv[2] else None
+    r = e.get("rotate")
+    if r is not None and len(r) >= 3:
+        ax, ay, az = (math.radians(float(x)) for x in r[:3])
+        Rx = np.array([[1, 0, 0, 0], [0, math.cos(ax), -math.sin(ax), 0], [0, math.sin(ax), math.cos(ax), 0],
+                       [0, 0, 0, 1]])
+        Ry = np.array([[math.cos(ay), 0, math.sin(ay), 0], [0, 1, 0, 0], [-math.sin(ay), 0, math.cos(ay), 0],
+                       [0, 0, 0, 1]])
+        Rz = np.array([[math.cos(az), -math.sin(az), 0, 0], [math.sin(az), math.cos(az), 0, 0], [0, 0, 1, 0],
+                       [0, 0, 0, 1]])
+        M = Rz @ Ry @ Rx @ M
+    t = e.get("translate")
+    if t is not None and len(t) >= 3:
+        T = np.eye(4)
+        T[:3, 3] = [float(x) for x in t[:3]]
+        M = T @ M
+    return M, uniform
+
+
+def _pt(M, p):
+    return tuple((M @ np.array([*p, 1.0]))[:3])
+
+
+def _vec(M, v):
+    return tuple(M[:3, :3] @ np.array(v))
+
+
+_UNSUPPORTED_ENTITY_KEYS = ("motion", "interior_medium", "exterior_medium", "displacement", "template")
+
+
+def _entity(b, e, mat):
+    typ = (e.get("type") or "").lower()
+    for k in _UNSUPPORTED_ENTITY_KEYS:
+        if e.get(k) is not None:
+            raise SceneLoadError(f"entity '{e.get('name')}': '{k}' is not supported")
+    flags = 0 if e.get("visible_to_camera", True) else abi.RT_PRIMF_CAMERA_INVISIBLE
+    M, uniform = _transform(e)
+    ident = np.array_equal(M, np.eye(4))
+    if typ == "sphere":
+        c = _v3(e.get("center"), (0.0, 0.0, 0.0))
+        r = _f(e, "radius", 1.0)
+        if not ident:
+            if uniform is None:
+                raise SceneLoadError(f"entity '{e.get('name')}': non-uniform scale of a sphere is not supported")
+            c, r = _pt(M, c), r * abs(uniform)
+        return b.sphere(c, r, mat, flags)
+    if typ == "quad":
+        q = _v3(e.get("q"), (0.0, 0.0, 0.0))
+        u = _v3(e.get("u"), (1.0, 0.0, 0.0))
+        v = _v3(e.get("v"), (0.0, 1.0, 0.0))
+        if not ident:
+            q, u, v = _pt(M, q), _vec(M, u), _vec(M, v)
+        return b.quad(q, u, v, mat, flags)
+    if typ in ("triangle", "smooth_triangle"):
+        if all(e.get(k) is not None for k in ("n0", "n1", "n2")):
+            raise SceneLoadError(f"entity '{e.get('name')}': smooth triangles (vertex normals) are not supported")
+        vs = [_v3(e.get("v0"), (0.0, 0.0, 0.0)), _v3(e.get("v1"), (1.0, 0.0, 0.0)), _v3(e.get("v2"), (0.0, 1.0, 0.0))]
+        if not ident:
+            vs = [_pt(M, p) for p in vs]
+        return b.triangle(np.array(vs[0], np.float32), np.array(vs[1], np.float32), np.array(vs[2], np.float32),
+                          mat, flags)
+    if typ in ("plane", "infinite_plane"):
+        if flags:
+            raise SceneLoadError(f"entity '{e.get('name')}': camera-invisible planes are not supported")
+        p = _v3(e.get("point"), (0.0, 0.0, 0.0))
+        n = _v3(e.get("normal"), (0.0, 1.0, 0.0))
+        if not ident:
+            p = _pt(M, p)
+            n = tuple(np.linalg.inv(M[:3, :3]).T @ np.array(n))
+        return b.plane(p, n, mat)
+    raise SceneLoadError(f"entity '{e.get('name')}': type '{typ}' is not supported")
+
+
+def _camera(data, selector, width, height):
+    """ResolveCamera, then the Camera constructor's inputs."""
+    cams = data.get("cameras") or []
+    if not cams:
+        cd = data.get("camera") or {}
+    elif selector is None:
+        cd = cams[0]
+    else:
+        by_name = [c for c in cams if str(c.get("name", "")).lower() == str(selector).lower()]
+        if by_name:
+            cd = by_name[0]
+        elif str(selector).lstrip("-").isdigit() and 0 <= int(selector) < len(cams):
+            cd = cams[int(selector)]
+        else:
+            cd = cams[0]
+    for k in ("motion", "shutter"):
+        if cd.get(k) is not None:
+            raise SceneLoadError(f"camera '{k}' (motion blur) is not supported")
+    pos = _v3(cd.get("position"), (0.0, 1.0, -5.0))
+    look = _v3(cd.get("look_at"), (0.0, 0.0, 0.0))
+    vup = _v3(cd.get("vup"), (0.0, 1.0, 0.0))
+    focus = _f(cd, "focal_dist", 1.0)
+    fp = cd.get("focal_pos")
+    if fp is not None and len(fp) >= 3:
+        # ComputeFocusDistance in float32: the focus point projected on the optical axis
+        f32 = np.float32
+        fwd = np.array(look, f32) - np.array(pos, f32)
+        ln = f32(np.sqrt(f32(fwd[0] * fwd[0] + fwd[1] * fwd[1] + fwd[2] * fwd[2])))
+        if ln >= f32(1e-6):
+            fwd = fwd / ln
+            d = np.array([float(x) for x in fp[:3]], f32) - np.array(pos, f32)
+            proj = f32(f32(d[0] * fwd[0] + d[1] * fwd[1]) + f32(d[2] * fwd[2]))
+            if proj > f32(1e-4):
+                focus = float(proj)
+    aspect = float(np.float32(width) / np.float32(height))
+    return scenes.make_camera(pos, look, _f(cd, "fov", 60.0), aspect, vup=vup, aperture=_f(cd, "aperture", 0.0),
+                              focus_dist=focus)
+
+
+def load_data(data, width, height, shadow_samples_override=None, camera=None):
+    """SceneLoader.Load on an already parsed YAML document (a dict)."""
+    if not isinstance(data, dict):
+        raise SceneLoadError("not a scene document")
+    for k in ("imports", "templates", "mediums"):
+        if data.get(k):
+            raise SceneLoadError(f"'{k}' is not supported")
+    b = scenes.SceneBuilder()
+    b.shadow_samples_override = shadow_samples_override
+    b.geometry_shadow_samples = GEOMETRY_SHADOW_SAMPLES
+    world = data.get("world") or {}
+
+    materials = {}
+    for m in data.get("materials") or []:
+        if m.get("id") is None:
+            continue
+        materials[str(m["id"])] = _material(b, m)
+
+    def get_material(mid):
+        if mid is not None and str(mid) in materials:
+            return materials[str(mid)]
+        return b.lambertian((0.5, 0.5, 0.5))
+
+    sky = DEFAULT_SKY
+    sd = world.get("sky")
+    if sd is not None:
+        st = (sd.get("type") or "").lower()
+        if st not in ("flat", ""):
+            raise SceneLoadError(f"sky type '{st}' is not supported (flat only)")
+        if sd.get("background") is not None:
+            raise SceneLoadError("a separate sky background is not supported")
+        sky = _v3(sd.get("color"), DEFAULT_SKY)
+
+    medium = None
+    md = world.get("medium")
+    if md is not None:
+        mt = (md.get("type") or "homogeneous").lower()
+        if mt != "homogeneous":
+            raise SceneLoadError(f"medium type '{mt}' is not supported (homogeneous only)")
+        ph = (md.get("phase") or "isotropic").lower()
+        if ph != "isotropic":
+            raise SceneLoadError(f"medium phase '{ph}' is not supported (isotropic only)")
+        sa = tuple(max(x, 0.0) for x in _v3(md.get("sigma_a"), (0.0, 0.0, 0.0)))
+        ss = tuple(max(x, 0.0) for x in _v3(md.get("sigma_s"), (0.0, 0.0, 0.0)))
+        medium = (sa, ss)
+
+    # objects: the ground first
+    gd = world.get("ground")
+    if gd is not None:
+        gt = (gd.get("type") or "infinite_plane").strip().lower()
+        if gt not in ("infinite_plane", "plane", ""):
+            raise SceneLoadError(f"ground type '{gt}' is not supported (infinite plane only)")
+        if gd.get("visibility") is not None:
+            raise SceneLoadError("ground visibility masks are not supported")
+        if gd.get("material"):
+            gm = get_material(gd["material"])
+        elif gd.get("color") is not None or gd.get("roughness") is not None or gd.get("metallic") is not None:
+            gm = b.disney(_v3(gd.get("color"), (0.5, 0.5, 0.5)), metallic=_f(gd, "metallic", 0.0),
+                          roughness=_f(gd, "roughness", 0.8))
+        else:
+            sdd = world.get("sky") or {}
+            alb = _v3(sdd.get("ground_albedo")) or _v3(sdd.get("ground_color"))
+            gm = b.lambertian(alb if alb is not None else (0.5, 0.5, 0.5))
+        anchor = _v3(gd.get("point"), (0.0, float(np.float32(gd.get("y", 0.0) or 0.0)), 0.0))
+        n = np.array(_v3(gd.get("normal"), (0.0, 1.0, 0.0)), np.float32)
+        if float(n @ n) < 1e-12:
+            n = np.array([0.0, 1.0, 0.0], np.float32)
+        n = n / np.float32(np.sqrt(np.float32(n @ n)))
+        b.plane(anchor, tuple(float(x) for x in n), gm)
+
+    for e in data.get("entities") or []:
+        _entity(b, e, get_material(e.get("material")))
+
+    # lights (proxies appended after the entities)
+    lights_key = "lights" in data
+    for L in data.get("lights") or []:
+        lt = (L.get("type") or "").lower()
+        color = _v3(L.get("color"), (1.0, 1.0, 1.0))
+        inten = _f(L, "intensity", 1.0)
+        ss = int(L.get("shadow_samples", 4) or 0)
+        vis = bool(L.get("visible_to_camera", True))
+        if lt in ("directional", "sun"):
+            b.directional_light(_v3(L.get("direction"), (-1.0, -1.0, -1.0)), color, inten,
+                                angular_radius_deg=_f(L, "angular_radius", 0.0))
+        elif lt in ("area", "area_light", "rect", "rect_light"):
+            corner, u, v = _v3(L.get("corner")), _v3(L.get("u")), _v3(L.get("v"))
+            if corner is None or u is None or v is None:
+                continue   # the reference warns and skips the light
+            b.area_light(corner, u, v, color, inten, shadow_samples=ss, soft_radius=_f(L, "soft_radius", 0.0),
+                         visible_to_camera=vis)
+        elif lt in ("sphere", "sphere_light", "ball", "ball_light"):
+            b.sphere_light(_v3(L.get("position"), (0.0, 10.0, 0.0)), _f(L, "radius", 0.5), color, inten,
+                           shadow_samples=ss, visible_to_camera=vis)
+        elif lt in ("point", "spot", "spotlight", "portal", "portal_light"):
+            raise SceneLoadError(f"light type '{lt}' is not supported")
+        # other types: CreateLight returns null and the light is skipped
+    if not lights_key:
+        raise SceneLoadError("a scene without a 'lights' section gets the reference's default point light, "
+                             "which is not supported")
+    cam = _camera(data, camera, width, height)
+    return b.build(cam, sky=sky, medium=medium)
+
+
+def load(path, width, height, shadow_samples_override=None, camera=None):
+    """SceneLoader.Load(yamlPath, imageWidth, imageHeight, shadowSamplesOverride,
+    cameraSelector): a scenes.SceneDesc for the renderer."""
+    import yaml
+    with open(path, "r", encoding="utf-8") as f:
+        data = yaml.safe_load(f)
+    return load_data(data, width, height, shadow_samples_override, camera)
