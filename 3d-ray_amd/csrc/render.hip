@@ -61,6 +61,7 @@ struct DevScene {
     // texture set of textured Disney materials (kMatTextured)
     const rt_material* raw;     // material records as given (knobs the texture does not replace)
     const float4* tuv;          // 2 per primitive or null (HitRecord U/V)
+    const float4* smooth;       // 4 per primitive or null: (N0,Tx) (N1,Ty) (N2,Tz) (B, smooth)
     const float* tex;           // planar [kTexChannels][tex_h][tex_w]
     int tex_w, tex_h;
 };
@@ -290,6 +291,20 @@ __device__ __forceinline__ Surf make_surf(const DevScene& S, int32_t eidx, f3 o,
         s.n = face(d, f3of(e.d), s.front);
         s.tan = length_sq(e1) > 0.0f ? normalize(e1) : mk(1.0f, 0.0f, 0.0f);
         s.bitan = length_sq(e2) > 0.0f ? normalize(e2) : mk(0.0f, 0.0f, 1.0f);
+        if (S.smooth) {
+            // SmoothTriangle.Hit (SmoothTriangle.cs:166-193): interpolated
+            // shading normal, oriented by the face normal's front test
+            const float4* q = S.smooth + 4 * (size_t)s.prim;
+            const float4 q3 = q[3];
+            if (q3.w != 0.0f) {
+                const float4 q0 = q[0], q1 = q[1], q2 = q[2];
+                const float w0 = 1.0f - u - v;
+                f3 sn = normalize((w0 * f3of(q0) + u * f3of(q1)) + v * f3of(q2));
+                s.n = s.front ? sn : -sn;
+                s.tan = mk(q0.w, q1.w, q2.w);
+                s.bitan = f3of(q3);
+            }
+        }
     } else if (kind == kSphere) {
         f3 c = f3of(e.a);
         float r = e.b.x;
@@ -311,8 +326,6 @@ __device__ __forceinline__ Surf make_surf(const DevScene& S, int32_t eidx, f3 o,
         s.tan = ua;
         s.bitan = cross(n, ua);
     }
-    (void)u;
-    (void)v;
     return s;
 }
 
@@ -1996,6 +2009,7 @@ struct rt_scene {
     float* d_ldist = nullptr;   // cdf[n], pdf[n]
     rt_material* d_raw = nullptr;
     float4* d_tuv = nullptr;
+    float4* d_smooth = nullptr;
     float* d_tex = nullptr;
     int stack_depth = 0;
     int64_t device_bytes = 0;
@@ -2079,6 +2093,7 @@ static DevScene dev_view(const rt_scene* sc) {
     S.block_spec = 0;
     S.raw = sc->d_raw;
     S.tuv = h.tex_uv.empty() ? nullptr : sc->d_tuv;
+    S.smooth = h.smooth.empty() ? nullptr : sc->d_smooth;
     S.tex = h.textured ? sc->d_tex : nullptr;
     S.tex_w = h.tex_w;
     S.tex_h = h.tex_h;
@@ -2582,6 +2597,7 @@ int rt_scene_create(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
     if ((rc = upload(ctx, &sc->d_lut, lut.data(), lut.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_raw, h.raw_mats.data(), h.raw_mats.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_tuv, h.tex_uv.data(), h.tex_uv.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_smooth, h.smooth.data(), h.smooth.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_tex, h.texels.data(), h.texels.size(), b))) return cleanup(rc);
     std::vector<float> ld(2 * std::max<size_t>(h.lights.size(), 1), 0.0f);
     if ((rc = upload(ctx, &sc->d_ldist, ld.data(), ld.size(), b))) return cleanup(rc);
@@ -2602,6 +2618,7 @@ void rt_scene_destroy(rt_scene* sc) {
     hipFree(sc->d_ldist);
     hipFree(sc->d_raw);
     hipFree(sc->d_tuv);
+    hipFree(sc->d_smooth);
     hipFree(sc->d_tex);
     delete sc;
 }

@@ -55,6 +55,9 @@ struct HostScene {
     int tex_w = 0, tex_h = 0;
     std::vector<float> texels;         // planar [kTexChannels][tex_h][tex_w]
     std::vector<float4> tex_uv;        // 2 per primitive: (uv0, uv1), (uv2, -, -); empty = HitRecord U/V
+    // SmoothTriangle data, 4 per primitive: (N0, T.x) (N1, T.y) (N2, T.z)
+    // (B, 1); a zero record is a flat triangle; empty = no smooth triangles
+    std::vector<float4> smooth;
 };
 
 // Builds the host scene; returns RT_OK or an error with `err` set.

@@ -36,7 +36,7 @@ static_assert(sizeof(rt_material) == 128, "rt_material layout");
 static_assert(sizeof(rt_light) == 80, "rt_light layout");
 static_assert(sizeof(rt_camera) == 56, "rt_camera layout");
 static_assert(sizeof(rt_medium) == 32, "rt_medium layout");
-static_assert(sizeof(rt_scene_desc) == 176, "rt_scene_desc layout");
+static_assert(sizeof(rt_scene_desc) == 184, "rt_scene_desc layout");
 static_assert(sizeof(rt_settings) == 96, "rt_settings layout");
 static_assert(sizeof(rt_film) == 88, "rt_film layout");
 static_assert(sizeof(rt_render_stats) == 152, "rt_render_stats layout");
@@ -46,6 +46,28 @@ static_assert(sizeof(rt_exr_channel) == 24, "rt_exr_channel layout");
 static_assert(sizeof(rt_denoise_options) == 32, "rt_denoise_options layout");
 
 static f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+
+// Tangent frame of a triangle's UV mapping, E1 = dUV1.x T + dUV1.y B and
+// E2 = dUV2.x T + dUV2.y B solved for T, B; normalised edges when the mapping
+// is degenerate (SmoothTriangle.cs:98-129).  uv = u0 v0 u1 v1 u2 v2.
+static void uv_frame(f3 e1, f3 e2, const float* uv, f3& T, f3& B) {
+    const float d1x = uv[2] - uv[0], d1y = uv[3] - uv[1];
+    const float d2x = uv[4] - uv[0], d2y = uv[5] - uv[1];
+    const float det = d1x * d2y - d2x * d1y;
+    if (fabsf(det) > 1e-8f) {
+        const float inv = 1.0f / det;
+        f3 t = inv * (d2y * e1 - d1y * e2);
+        f3 b = inv * (-d2x * e1 + d1x * e2);
+        const float tl = length(t), bl = length(b);
+        if (tl > 1e-8f && bl > 1e-8f) {
+            T = t / tl;
+            B = b / bl;
+            return;
+        }
+    }
+    T = normalize(e1);
+    B = normalize(e2);
+}
 static int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
 static float ibits(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
 
@@ -519,6 +541,40 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
         }
     }
 
+    // smooth triangles (SmoothTriangle.cs:68-130): unit vertex normals and
+    // the tangent frame of the vertex UVs, barycentric UVs without texcoords
+    if (d->normals) {
+        hs.smooth.assign(4 * (size_t)np, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        bool any = false;
+        for (int64_t i = 0; i < np; i++) {
+            const float* n = d->normals + 9 * i;
+            bool smooth = false;
+            for (int k = 0; k < 9; k++) smooth |= n[k] != 0.0f;
+            if (!smooth) continue;
+            const rt_primitive& p = d->primitives[i];
+            if (p.kind != RT_PRIM_TRIANGLE) { err = "vertex normals on a primitive that is not a triangle"; return RT_ERR_INVALID_ARGUMENT; }
+            f3 nv[3];
+            for (int k = 0; k < 3; k++) {
+                f3 a = ld3(n + 3 * k);
+                float l2 = length_sq(a);
+                if (!(l2 > 0.0f) || !std::isfinite(l2)) { err = "smooth triangle with a zero or non-finite vertex normal"; return RT_ERR_INVALID_ARGUMENT; }
+                nv[k] = normalize(a);
+            }
+            static const float kBaryUv[6] = {0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 1.0f};
+            const float* uv = d->texcoords ? d->texcoords + 6 * i : kBaryUv;
+            f3 v0 = ld3(p.v), e1 = ld3(p.v + 3) - v0, e2 = ld3(p.v + 6) - v0;
+            f3 T, B;
+            uv_frame(e1, e2, uv, T, B);
+            float4* o = &hs.smooth[4 * i];
+            o[0] = make_float4(nv[0].x, nv[0].y, nv[0].z, T.x);
+            o[1] = make_float4(nv[1].x, nv[1].y, nv[1].z, T.y);
+            o[2] = make_float4(nv[2].x, nv[2].y, nv[2].z, T.z);
+            o[3] = make_float4(B.x, B.y, B.z, 1.0f);
+            any = true;
+        }
+        if (!any) hs.smooth.clear();
+    }
+
     // objects (SceneLoader object list; a mesh is one object)
     Flattener F{hs, d, {}, {}};
     F.quad_aux_of.assign(np, -1);
@@ -719,6 +775,10 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
                 const rt_material& pm = d->materials[p.material];
                 if (pm.kind != RT_MAT_EMISSIVE || p.kind == RT_PRIM_PLANE || p.mesh >= 0) {
                     err = "geometry light needs a standalone emissive triangle, quad or sphere";
+                    return RT_ERR_UNSUPPORTED;
+                }
+                if (!hs.smooth.empty() && hs.smooth[4 * (size_t)l.primitive + 3].w != 0.0f) {
+                    err = "a smooth triangle cannot be a geometry light";
                     return RT_ERR_UNSUPPORTED;
                 }
                 ss = std::max(1, l.shadow_samples);

@@ -67,7 +67,8 @@ class rt_scene_desc(C.Structure):
                 ("lights", C.POINTER(rt_light)), ("num_lights", C.c_int64),
                 ("camera", rt_camera), ("sky_color", F3), ("reserved0", C.c_int32),
                 ("global_medium", rt_medium), ("texcoords", C.POINTER(C.c_float)),
-                ("texels", C.POINTER(C.c_float)), ("tex_width", C.c_int32), ("tex_height", C.c_int32)]
+                ("texels", C.POINTER(C.c_float)), ("tex_width", C.c_int32), ("tex_height", C.c_int32),
+                ("normals", C.POINTER(C.c_float))]
 
 
 class rt_settings(C.Structure):

@@ -8,7 +8,8 @@
   materials        lambertian, dielectric, emissive, disney / disney_bsdf / pbr
                    (CreateMaterial :1027-1074, MaterialData defaults SceneData.cs:470-717);
                    an unknown type is a grey-default Lambertian, as in the reference
-  entities         sphere, quad, triangle, plane / infinite_plane (CreateEntity :2282-2325,
+  entities         sphere, quad, triangle / smooth_triangle (n0-n2 and uv0-uv2 make it
+                   a SmoothTriangle), plane / infinite_plane (CreateEntity :2282-2325,
                    CreateTriangleEntity :3430-3461) with scale / rotate / translate
                    (ComputeTransformMatrix :3788-3822) and visible_to_camera
   lights           sphere, area, directional / sun (CreateLight :3463-3636, with the
@@ -162,13 +163,26 @@ def _entity(b, e, mat):
             q, u, v = _pt(M, q), _vec(M, u), _vec(M, v)
         return b.quad(q, u, v, mat, flags)
     if typ in ("triangle", "smooth_triangle"):
-        if all(e.get(k) is not None for k in ("n0", "n1", "n2")):
-            raise SceneLoadError(f"entity '{e.get('name')}': smooth triangles (vertex normals) are not supported")
         vs = [_v3(e.get("v0"), (0.0, 0.0, 0.0)), _v3(e.get("v1"), (1.0, 0.0, 0.0)), _v3(e.get("v2"), (0.0, 1.0, 0.0))]
+        normals = uvs = None
+        if all(e.get(k) is not None for k in ("n0", "n1", "n2")):
+            # SmoothTriangle; its UVs only when all three are given
+            normals = [_v3(e.get(k)) for k in ("n0", "n1", "n2")]
+            if all(e.get(k) is not None for k in ("uv0", "uv1", "uv2")):
+                uvs = [tuple(float(np.float32(c)) for c in e.get(k)[:2]) for k in ("uv0", "uv1", "uv2")]
+            if not ident:
+                # the Transform wrapper maps the interpolated normal by the
+                # inverse transpose; with a uniform scale that commutes with
+                # the interpolation, so the vertex normals are mapped instead
+                if uniform is None:
+                    raise SceneLoadError(f"entity '{e.get('name')}': non-uniform scale of a smooth triangle "
+                                         "is not supported")
+                Nm = np.linalg.inv(M[:3, :3]).T
+                normals = [tuple(Nm @ np.array(n)) for n in normals]
         if not ident:
             vs = [_pt(M, p) for p in vs]
         return b.triangle(np.array(vs[0], np.float32), np.array(vs[1], np.float32), np.array(vs[2], np.float32),
-                          mat, flags)
+                          mat, flags, normals=normals, uvs=uvs)
     if typ in ("plane", "infinite_plane"):
         if flags:
             raise SceneLoadError(f"entity '{e.get('name')}': camera-invisible planes are not supported")

@@ -56,7 +56,7 @@ class SceneDesc:
     """Owns the numpy tables behind one rt_scene_desc (keeps them alive)."""
 
     def __init__(self, prims, mats, lights, camera, sky=(0.0, 0.0, 0.0), medium=None, texcoords=None,
-                 texture=None):
+                 texture=None, normals=None):
         self.prims = np.ascontiguousarray(prims, PRIM_DTYPE)
         self.mats = np.ascontiguousarray(mats, MAT_DTYPE)
         self.lights = np.ascontiguousarray(lights, LIGHT_DTYPE)
@@ -86,6 +86,11 @@ class SceneDesc:
         if self.texcoords is not None:
             assert len(self.texcoords) == len(self.prims)
             d.texcoords = C.cast(self.texcoords.ctypes.data, C.POINTER(C.c_float))
+        # optional per-primitive vertex normals (n, 9); an all-zero row is a flat triangle
+        self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
+        if self.normals is not None:
+            assert len(self.normals) == len(self.prims)
+            d.normals = C.cast(self.normals.ctypes.data, C.POINTER(C.c_float))
         self.desc = d
 
     @property
@@ -110,6 +115,8 @@ class SceneBuilder:
         self._mat_count = 0
         self._lights = []
         self._next_mesh = 0
+        self._normals = []        # (first primitive, (n, 9) vertex normals) of smooth triangles
+        self._uvs = []            # (first primitive, (n, 6) vertex UVs)
         self.geometry_shadow_samples = 4
         self.shadow_samples_override = None
 
@@ -170,8 +177,15 @@ class SceneBuilder:
         self._count += 1
         return first
 
-    def triangle(self, v0, v1, v2, mat, flags=0):
-        return self._append(abi.RT_PRIM_TRIANGLE, mat, np.concatenate([v0, v1, v2]), flags)
+    def triangle(self, v0, v1, v2, mat, flags=0, normals=None, uvs=None):
+        """A Triangle, or a SmoothTriangle when vertex normals (n0, n1, n2)
+        are given; uvs = (uv0, uv1, uv2) are its vertex texture coordinates."""
+        first = self._append(abi.RT_PRIM_TRIANGLE, mat, np.concatenate([v0, v1, v2]), flags)
+        if normals is not None:
+            self._normals.append((first, np.asarray(normals, np.float32).reshape(1, 9)))
+        if uvs is not None:
+            self._uvs.append((first, np.asarray(uvs, np.float32).reshape(1, 6)))
+        return first
 
     def triangles(self, tris, mats, flags=0):
         """tris: (n, 3, 3) standalone triangle entities; mats: int or (n,)."""
@@ -187,11 +201,20 @@ class SceneBuilder:
         self._count += len(tris)
         return first
 
-    def mesh(self, vertices, faces, mats):
-        """A Mesh object (nested BVH): faces contiguous, in face order."""
+    def mesh(self, vertices, faces, mats, normals=None, normal_faces=None, uvs=None, uv_faces=None):
+        """A Mesh object (nested BVH): faces contiguous, in face order.  With
+        vertex normals (indexed by normal_faces, default faces) the faces are
+        SmoothTriangles; uvs (indexed by uv_faces) are vertex texture
+        coordinates."""
         vertices = np.asarray(vertices, np.float32)
         faces = np.asarray(faces, np.int64)
         tris = vertices[faces].reshape(-1, 9)
+        if normals is not None:
+            nf = faces if normal_faces is None else np.asarray(normal_faces, np.int64)
+            self._normals.append((self._count, np.asarray(normals, np.float32)[nf].reshape(-1, 9)))
+        if uvs is not None:
+            tf = faces if uv_faces is None else np.asarray(uv_faces, np.int64)
+            self._uvs.append((self._count, np.asarray(uvs, np.float32)[tf].reshape(-1, 6)))
         mesh_id = self._next_mesh
         self._next_mesh += 1
         first = self._count
@@ -257,9 +280,21 @@ class SceneBuilder:
         kinds = mats["kind"][prims["material"]]
         proxy = (mats["flags"][prims["material"]] & abi.RT_MATF_LIGHT_PROXY) != 0
         emissive = np.nonzero((kinds == abi.RT_MAT_EMISSIVE) & ~proxy)[0]
+        normals = None
+        if self._normals:
+            normals = np.zeros((len(prims), 9), np.float32)
+            for first, n in self._normals:
+                normals[first:first + len(n)] = n
+        texcoords = None
+        if self._uvs:
+            texcoords = np.tile(np.array([0, 0, 1, 0, 0, 1], np.float32), (len(prims), 1))   # barycentric UVs
+            for first, t in self._uvs:
+                texcoords[first:first + len(t)] = t
         if len(emissive):
             if np.any(prims["mesh"][emissive] >= 0) or np.any(prims["kind"][emissive] == abi.RT_PRIM_PLANE):
                 raise ValueError("emissive meshes/planes are not supported as geometry lights")
+            if normals is not None and np.any(normals[emissive] != 0):
+                raise ValueError("emissive smooth triangles are not supported as geometry lights")
             g = np.zeros(len(emissive), LIGHT_DTYPE)
             g["kind"] = abi.RT_LIGHT_GEOMETRY
             g["shadow_samples"] = self.shadow_samples_override or self.geometry_shadow_samples
@@ -267,7 +302,7 @@ class SceneBuilder:
             g["material"] = prims["material"][emissive]
             lights.append(g)
         lights = np.concatenate(lights) if lights else np.zeros(0, LIGHT_DTYPE)
-        return SceneDesc(prims, mats, lights, camera, sky, medium)
+        return SceneDesc(prims, mats, lights, camera, sky, medium, texcoords=texcoords, normals=normals)
 
 
 # ═══════════════════════════════════════════════════════════════════════

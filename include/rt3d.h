@@ -208,6 +208,13 @@ typedef struct rt_scene_desc {
     const float* texels;
     int32_t tex_width;
     int32_t tex_height;
+    /* Vertex normals (optional): 9 floats per primitive, N0, N1, N2 of a
+     * TRIANGLE.  A triangle whose triple is not all zero is a SmoothTriangle
+     * (SmoothTriangle.cs:68-197): normals normalised once, shading normal
+     * normalize(w0 N0 + w1 N1 + w2 N2), front face from the face normal,
+     * tangent frame from the UV gradients of its texcoords (barycentric UVs
+     * without them).  Such a triangle may not be a GEOMETRY light. */
+    const float* normals;
 } rt_scene_desc;
 
 /* ── Render settings (Renderer ctor arguments + Render(width, height)) ───── */

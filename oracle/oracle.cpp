@@ -176,6 +176,47 @@ struct Triangle : Hittable {
     }
 };
 
+// SmoothTriangle.cs:68-130 (constructor), 147-197 (Hit).  The Möller-Trumbore
+// test is the flat triangle's; the hit carries the interpolated vertex normal,
+// oriented by the face normal's front test, and the frame of the UV mapping.
+// rec.u/v stay the barycentrics: the texture lookup interpolates the vertex
+// UVs from them exactly as SmoothTriangle.Hit fills HitRecord.U/V.
+struct SmoothTriangle : Triangle {
+    V3 n0, n1, n2;
+    SmoothTriangle(V3 a, V3 b, V3 c, V3 na, V3 nb, V3 nc, const float uv[6], const Material* m, int id)
+        : Triangle(a, b, c, m, id) {
+        n0 = normalize(na);
+        n1 = normalize(nb);
+        n2 = normalize(nc);
+        float du1x = uv[2] - uv[0], du1y = uv[3] - uv[1];
+        float du2x = uv[4] - uv[0], du2y = uv[5] - uv[1];
+        float det = du1x * du2y - du2x * du1y;
+        bool done = false;
+        if (std::fabs(det) > 1e-8f) {
+            float inv = 1.0f / det;
+            V3 t = inv * (du2y * e1 - du1y * e2);
+            V3 bb = inv * (-du2x * e1 + du1x * e2);
+            float tl = length(t), bl = length(bb);
+            if (tl > 1e-8f && bl > 1e-8f) {
+                tan = t / tl;
+                bitan = bb / bl;
+                done = true;
+            }
+        }
+        if (!done) {
+            tan = normalize(e1);
+            bitan = normalize(e2);
+        }
+    }
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override {
+        if (!Triangle::hit(r, tmin, tmax, rec)) return false;
+        float w0 = 1.0f - rec.u - rec.v;
+        V3 sn = normalize(w0 * n0 + rec.u * n1 + rec.v * n2);
+        rec.normal = rec.front_face ? sn : -sn;
+        return true;
+    }
+};
+
 static void sphere_uv(V3 p, float& u, float& v) {
     float theta = std::acos(-p.y);
     float phi = std::atan2(-p.z, p.x) + kPi;
@@ -2928,7 +2969,21 @@ static Scene* build_scene(const rt_scene_desc* d) {
         const float* v = p.v;
         Hittable* h = nullptr;
         switch (p.kind) {
-            case RT_PRIM_TRIANGLE: h = new Triangle(V3(v[0], v[1], v[2]), V3(v[3], v[4], v[5]), V3(v[6], v[7], v[8]), m, (int)i); break;
+            case RT_PRIM_TRIANGLE: {
+                const float* nv = d->normals ? d->normals + 9 * i : nullptr;
+                bool smooth = false;
+                for (int k = 0; nv && k < 9; k++) smooth = smooth || nv[k] != 0.0f;
+                if (smooth) {
+                    static const float kBary[6] = {0, 0, 1, 0, 0, 1};
+                    const float* uv = d->texcoords ? d->texcoords + 6 * i : kBary;
+                    h = new SmoothTriangle(V3(v[0], v[1], v[2]), V3(v[3], v[4], v[5]), V3(v[6], v[7], v[8]),
+                                           V3(nv[0], nv[1], nv[2]), V3(nv[3], nv[4], nv[5]), V3(nv[6], nv[7], nv[8]),
+                                           uv, m, (int)i);
+                } else {
+                    h = new Triangle(V3(v[0], v[1], v[2]), V3(v[3], v[4], v[5]), V3(v[6], v[7], v[8]), m, (int)i);
+                }
+                break;
+            }
             case RT_PRIM_SPHERE: h = new Sphere(V3(v[0], v[1], v[2]), v[3], m, (int)i); break;
             case RT_PRIM_QUAD:
                 h = new Quad(V3(v[0], v[1], v[2]), V3(v[3], v[4], v[5]), V3(v[6], v[7], v[8]), m, (int)i,
@@ -3018,6 +3073,7 @@ static Scene* build_scene(const rt_scene_desc* d) {
                 const Material* gm = sc->materials[d->primitives[l.primitive].material].get();
                 const Emissive* em = dynamic_cast<const Emissive*>(gm);
                 if (!em || d->primitives[l.primitive].kind == RT_PRIM_PLANE) { g_err = "geometry light needs an emissive tri/quad/sphere"; return nullptr; }
+                if (dynamic_cast<const SmoothTriangle*>(g)) { g_err = "smooth triangle geometry light"; return nullptr; }
                 L = new GeometryLight(g, em, l.shadow_samples, l.soft_radius);
                 L->proxy = nullptr;
                 break;
