@@ -10,8 +10,10 @@
                    an unknown type is a grey-default Lambertian, as in the reference
   entities         sphere, quad, triangle / smooth_triangle (n0-n2 and uv0-uv2 make it
                    a SmoothTriangle), plane / infinite_plane (CreateEntity :2282-2325,
-                   CreateTriangleEntity :3430-3461) with scale / rotate / translate
-                   (ComputeTransformMatrix :3788-3822) and visible_to_camera
+                   CreateTriangleEntity :3430-3461), mesh / obj (CreateMeshEntity
+                   :2828-2935 through raytracer/obj_loader.py, no subdivision) with
+                   scale / rotate / translate (ComputeTransformMatrix :3788-3822) and
+                   visible_to_camera
   lights           sphere, area, directional / sun (CreateLight :3463-3636, with the
                    emissive proxies it appends); unknown types are skipped
   camera/cameras   ResolveCamera :731-781, focal_pos via ComputeFocusDistance :3744-3786
@@ -23,17 +25,18 @@ then the light proxies — which is the primitive order the renderer's IDs
 follow.  Transforms are applied to the primitive's defining points and
 vectors (the reference wraps the primitive in a ray-transforming
 Transform): the same surfaces up to float rounding.  Everything else in the
-schema (imports, templates, instances, groups, CSG, meshes, textures and
+schema (imports, templates, instances, groups, CSG, mesh subdivision, textures and
 maps, mix materials, metal, bound media, non-isotropic phases, point / spot
 / portal lights, motion, sky models other than flat, ground shapes other
 than the plane) raises SceneLoadError naming the feature, so a scene is
 either loaded as the reference would assemble it or refused.
 """
 import math
+import os
 
 import numpy as np
 
-from . import abi, scenes
+from . import abi, obj_loader, scenes
 
 DEFAULT_SKY = (0.5, 0.7, 1.0)           # SceneLoader.DefaultSkyColor
 GEOMETRY_SHADOW_SAMPLES = 4             # GeometryLight.DefaultShadowSamples
@@ -139,7 +142,52 @@ def _vec(M, v):
 _UNSUPPORTED_ENTITY_KEYS = ("motion", "interior_medium", "exterior_medium", "displacement", "template")
 
 
-def _entity(b, e, mat):
+def _mesh_entity(b, e, mat, flags, M, uniform, ident, base_dir, warnings):
+    """CreateMeshEntity (SceneLoader.cs:2828-2935) without subdivision or
+    displacement: the OBJ at `path` (relative to the scene file) as one
+    Mesh; a missing path or file, or a file without triangles, is a warning
+    and the entity is skipped, as in the reference."""
+    name = e.get("name") or "(unnamed)"
+    for k in ("subdivision_iterations", "subdivision_pixel_error"):
+        if float(e.get(k) or 0) > 0:
+            raise SceneLoadError(f"mesh '{name}': '{k}' (mesh subdivision) is not supported")
+    if str(e.get("subdivision_scheme") or "").strip():
+        raise SceneLoadError(f"mesh '{name}': 'subdivision_scheme' (mesh subdivision) is not supported")
+    path = e.get("path")
+    if path is None or not str(path).strip():
+        warnings.append(f"Mesh entity '{name}' requires a 'path' to an OBJ file. Skipping.")
+        return None
+    path = str(path)
+    full = path if os.path.isabs(path) else os.path.join(base_dir or ".", path)
+    if not os.path.isfile(full):
+        warnings.append(f"Mesh entity '{name}': OBJ file not found at '{full}'. Skipping.")
+        return None
+    w = []
+    try:
+        mesh = obj_loader.load(full, w)
+    except obj_loader.ObjLoadError as err:
+        raise SceneLoadError(f"mesh '{name}': {err}") from None
+    warnings.extend(f"Mesh '{name}': {x}" for x in w)
+    if mesh is None:
+        warnings.append(f"Mesh entity '{name}': failed to load '{full}'. Skipping.")
+        return None
+    tris, normals = mesh.tris, mesh.normals
+    if not ident:
+        P = tris.reshape(-1, 3).astype(np.float64)
+        tris = (P @ M[:3, :3].T + M[:3, 3]).astype(np.float32).reshape(-1, 9)
+        if normals is not None:
+            # the Transform wrapper maps the interpolated normal by the inverse
+            # transpose, which commutes with the interpolation only for a
+            # uniform scale
+            if uniform is None:
+                raise SceneLoadError(f"mesh '{name}': non-uniform scale of a mesh with vertex normals "
+                                     "is not supported")
+            Nm = np.linalg.inv(M[:3, :3]).T
+            normals = (normals.reshape(-1, 3).astype(np.float64) @ Nm.T).astype(np.float32).reshape(-1, 9)
+    return b.mesh_triangles(tris, mat, normals, mesh.uvs, flags)
+
+
+def _entity(b, e, mat, base_dir=None, warnings=None):
     typ = (e.get("type") or "").lower()
     for k in _UNSUPPORTED_ENTITY_KEYS:
         if e.get(k) is not None:
@@ -147,6 +195,8 @@ def _entity(b, e, mat):
     flags = 0 if e.get("visible_to_camera", True) else abi.RT_PRIMF_CAMERA_INVISIBLE
     M, uniform = _transform(e)
     ident = np.array_equal(M, np.eye(4))
+    if typ in ("mesh", "obj"):
+        return _mesh_entity(b, e, mat, flags, M, uniform, ident, base_dir, [] if warnings is None else warnings)
     if typ == "sphere":
         c = _v3(e.get("center"), (0.0, 0.0, 0.0))
         r = _f(e, "radius", 1.0)
@@ -234,8 +284,10 @@ def _camera(data, selector, width, height):
                               focus_dist=focus)
 
 
-def load_data(data, width, height, shadow_samples_override=None, camera=None):
-    """SceneLoader.Load on an already parsed YAML document (a dict)."""
+def load_data(data, width, height, shadow_samples_override=None, camera=None, base_dir=None, warnings=None):
+    """SceneLoader.Load on an already parsed YAML document (a dict); mesh
+    paths resolve against base_dir, and the loader's warnings (skipped
+    meshes, malformed OBJ lines) are appended to `warnings`."""
     if not isinstance(data, dict):
         raise SceneLoadError("not a scene document")
     for k in ("imports", "templates", "mediums"):
@@ -305,7 +357,7 @@ def load_data(data, width, height, shadow_samples_override=None, camera=None):
         b.plane(anchor, tuple(float(x) for x in n), gm)
 
     for e in data.get("entities") or []:
-        _entity(b, e, get_material(e.get("material")))
+        _entity(b, e, get_material(e.get("material")), base_dir, warnings)
 
     # lights (proxies appended after the entities)
     lights_key = "lights" in data
@@ -337,10 +389,11 @@ def load_data(data, width, height, shadow_samples_override=None, camera=None):
     return b.build(cam, sky=sky, medium=medium)
 
 
-def load(path, width, height, shadow_samples_override=None, camera=None):
+def load(path, width, height, shadow_samples_override=None, camera=None, warnings=None):
     """SceneLoader.Load(yamlPath, imageWidth, imageHeight, shadowSamplesOverride,
     cameraSelector): a scenes.SceneDesc for the renderer."""
     import yaml
     with open(path, "r", encoding="utf-8") as f:
         data = yaml.safe_load(f)
-    return load_data(data, width, height, shadow_samples_override, camera)
+    return load_data(data, width, height, shadow_samples_override, camera,
+                     base_dir=os.path.dirname(os.path.abspath(path)), warnings=warnings)

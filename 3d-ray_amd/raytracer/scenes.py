@@ -209,18 +209,31 @@ class SceneBuilder:
         vertices = np.asarray(vertices, np.float32)
         faces = np.asarray(faces, np.int64)
         tris = vertices[faces].reshape(-1, 9)
+        fn = fu = None
         if normals is not None:
             nf = faces if normal_faces is None else np.asarray(normal_faces, np.int64)
-            self._normals.append((self._count, np.asarray(normals, np.float32)[nf].reshape(-1, 9)))
+            fn = np.asarray(normals, np.float32)[nf].reshape(-1, 9)
         if uvs is not None:
             tf = faces if uv_faces is None else np.asarray(uv_faces, np.int64)
-            self._uvs.append((self._count, np.asarray(uvs, np.float32)[tf].reshape(-1, 6)))
+            fu = np.asarray(uvs, np.float32)[tf].reshape(-1, 6)
+        return self.mesh_triangles(tris, mats, fn, fu)
+
+    def mesh_triangles(self, tris, mats, normals=None, uvs=None, flags=0):
+        """A Mesh object from its triangle list (n, 9) in Mesh order, with
+        per-triangle vertex normals (n, 9; zero rows stay flat) and vertex
+        UVs (n, 6); flags apply to the whole mesh."""
+        tris = np.asarray(tris, np.float32).reshape(-1, 9)
+        if normals is not None:
+            self._normals.append((self._count, np.asarray(normals, np.float32).reshape(-1, 9)))
+        if uvs is not None:
+            self._uvs.append((self._count, np.asarray(uvs, np.float32).reshape(-1, 6)))
         mesh_id = self._next_mesh
         self._next_mesh += 1
         first = self._count
         p = np.zeros(len(tris), PRIM_DTYPE)
         p["kind"] = abi.RT_PRIM_TRIANGLE
         p["material"] = mats
+        p["flags"] = flags
         p["mesh"] = mesh_id
         p["v"][:, :9] = tris
         self._prims.append(p)
