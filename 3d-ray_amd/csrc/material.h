@@ -88,6 +88,7 @@ RT_HD GMat disney_gmat(const rt_material& m, f3 c, float metallic_in, float roug
     if (!thin && m.spec_trans >= 0.5f && m.roughness <= 0.05f) flags |= kMatSpecTransmissive;
     if (!thin && m.spec_trans > 0.0f) flags |= kMatDielectricIor;
     if (m.flags & RT_MATF_TEXTURED) flags |= kMatTextured;
+    if (m.flags & RT_MATF_IMAGE) flags |= kMatImage;
     g.m1 = make_float4(c.x, c.y, c.z, metallic);
     g.m2 = make_float4(f0.x, f0.y, f0.z, roughness);
     g.m3 = make_float4(alpha, ax, ay, ior);

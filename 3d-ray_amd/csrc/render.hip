@@ -64,6 +64,10 @@ struct DevScene {
     const float4* smooth;       // 4 per primitive or null: (N0,Tx) (N1,Ty) (N2,Tz) (B, smooth)
     const float* tex;           // planar [kTexChannels][tex_h][tex_w]
     int tex_w, tex_h;
+    // image textures (kMatImage)
+    const GImage* images;
+    const GLevel* img_levels;
+    const float4* img_texels;
 };
 
 struct RParams {
@@ -76,6 +80,8 @@ struct RParams {
     int aovs;
     int maxs;                   // shadow slots per path
     int base_sample;            // first sample pass of the window
+    int footprints;             // camera-ray differentials feed image-texture footprints
+    int diff_rays;              // camera rays as GetRayWithDifferentials builds them (filtering on)
 };
 
 // A window is a run of whole sample passes of one or more jobs; part k
@@ -216,6 +222,7 @@ struct PathBufs {
     // radiance sum a0 + ai), 2 = replay (texture gradients); 0 = plain render
     int adj_pass;
     float adj_inv_spp;
+    float4* fp;                 // per slot: image-texture footprint of the camera hit (dudx, dvdx, dudy, dvdy)
     float4* adj_q;              // 5 per slot: pending bounce gradient (15 coefficients), (u, v, flag, -)
     float4* adj_g;              // 2 per shadow record: NEE gradient (5), (u, v, flag)
     const float* adj_db;        // dLoss/dBeauty, planar [3][npix]
@@ -354,6 +361,117 @@ __device__ __forceinline__ GMat textured_gmat(const DevScene& S, const GMat& g, 
     GMat t = disney_gmat(S.raw[mat], mk(tp[0], tp[1], tp[2]), tp[4], tp[3], S.lut_e + 32 * 32);
     t.m0.z = g.m0.z;
     return t;
+}
+
+// ── ImageTexture (ImageTexture.cs:148-289) ─────────────────────────────
+__device__ __forceinline__ int img_wrap(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+// SampleBilinear: texel centres at (i + 0.5) / w, repeat addressing.
+__device__ __forceinline__ f3 img_bilinear(const DevScene& S, const GLevel L, float u, float v) {
+    const float px = u * (float)L.w - 0.5f, py = v * (float)L.h - 0.5f;
+    const int x0 = (int)floorf(px), y0 = (int)floorf(py);
+    const float fx = px - (float)x0, fy = py - (float)y0;
+    const int x0w = img_wrap(x0, L.w), y0w = img_wrap(y0, L.h);
+    const int x1w = img_wrap(x0 + 1, L.w), y1w = img_wrap(y0 + 1, L.h);
+    const float4* T = S.img_texels + L.off;
+    const f3 c00 = f3of(T[y0w * L.w + x0w]), c10 = f3of(T[y0w * L.w + x1w]);
+    const f3 c01 = f3of(T[y1w * L.w + x0w]), c11 = f3of(T[y1w * L.w + x1w]);
+    return lerp3(lerp3(c00, c10, fx), lerp3(c01, c11, fx), fy);
+}
+
+// SampleTrilinear
+__device__ __forceinline__ f3 img_trilinear(const DevScene& S, const GImage& im, float u, float v, float lod) {
+    const int lo = (int)floorf(lod);
+    const int hi = min(lo + 1, im.mips - 1);
+    const float f = lod - (float)lo;
+    const f3 a = img_bilinear(S, S.img_levels[im.level0 + lo], u, v);
+    if (f <= 0.0f) return a;
+    return lerp3(a, img_bilinear(S, S.img_levels[im.level0 + hi], u, v), f);
+}
+
+// ImageTexture.Value: point-sampled level 0 without a footprint; with one
+// (fp = dudx, dvdx, dudy, dvdy; all zero = none) ratio-trilinear filtering:
+// LOD from the minor axis (anisotropy capped at 8), up to 8 taps along the
+// major axis.
+__device__ __forceinline__ f3 img_value(const DevScene& S, int img, float u, float v, float4 fp) {
+    const GImage im = S.images[img];
+    float dudx = fp.x * im.scale_u, dvdx = fp.y * im.scale_v;
+    float dudy = fp.z * im.scale_u, dvdy = fp.w * im.scale_v;
+    float lx = sqrtf(dudx * dudx + dvdx * dvdx), ly = sqrtf(dudy * dudy + dvdy * dvdy);
+    float minor = lx <= ly ? lx : ly, major = lx >= ly ? lx : ly;
+    if (im.mips <= 1 || !(major > 0.0f)) {
+        float uu = u * im.scale_u, vv = v * im.scale_v;
+        uu = uu - floorf(uu);
+        vv = vv - floorf(vv);
+        return img_bilinear(S, S.img_levels[im.level0], uu, 1.0f - vv);
+    }
+    if (minor * 8.0f < major) minor = major / 8.0f;
+    const GLevel L0 = S.img_levels[im.level0];
+    const float maxdim = fmaxf((float)L0.w, (float)L0.h);
+    float lod = log2f(fmaxf(minor * maxdim, 1.0f));
+    lod = fminf(fmaxf(lod, 0.0f), (float)(im.mips - 1));
+    int n = (int)ceilf(major / minor);
+    n = n < 1 ? 1 : (n > 8 ? 8 : n);
+    const float mu = lx >= ly ? dudx : dudy, mv = lx >= ly ? dvdx : dvdy;
+    const float uc = u * im.scale_u, vc = v * im.scale_v;
+    const float inv = 1.0f / (float)n;
+    f3 acc = splat(0.0f);
+    for (int i = 0; i < n; i++) {
+        const float t = ((float)i + 0.5f) * inv - 0.5f;
+        float uu = uc + t * mu, vv = vc + t * mv;
+        uu = uu - floorf(uu);
+        vv = vv - floorf(vv);
+        acc = acc + img_trilinear(S, im, uu, 1.0f - vv, lod);
+    }
+    return acc * inv;
+}
+
+// Material record of an image-textured hit: Lambertian albedo or Disney
+// baseColor from the texture (SceneLoader.cs:1027-1031).
+__device__ __forceinline__ GMat image_gmat(const DevScene& S, const GMat& g, int mat, float2 uv, float4 fp) {
+    const rt_material& m = S.raw[mat];
+    const f3 c = img_value(S, m.image, uv.x, uv.y, fp);
+    if (g.m0.x == RT_MAT_LAMBERTIAN) {
+        GMat t = g;
+        t.m1 = make_float4(c.x, c.y, c.z, g.m1.w);
+        return t;
+    }
+    GMat t = disney_gmat(m, c, m.metallic, m.roughness, S.lut_e + 32 * 32);
+    t.m0.z = g.m0.z;
+    return t;
+}
+
+// ── ray-differential footprint (FootprintMath.cs:38-119) ────────────────
+__device__ __forceinline__ bool tangent_plane_hit(f3 o, f3 d, f3 P, f3 N, f3& hit) {
+    const float den = dot(d, N);
+    if (fabsf(den) < 1e-12f) return false;
+    const float t = dot(P - o, N) / den;
+    hit = o + t * d;
+    return isfinite(t);
+}
+
+// UV partials (dudx, dvdx, dudy, dvdy) of the camera differentials
+// (ox, dx), (oy, dy) at hit point P with normal N; zero when a ray is
+// parallel to the tangent plane or the dpdu/dpdv basis is singular.
+__device__ __forceinline__ float4 uv_footprint(f3 ox, f3 dx, f3 dy, f3 P, f3 N, f3 dpdu, f3 dpdv) {
+    f3 px, py;
+    if (!tangent_plane_hit(ox, dx, P, N, px) || !tangent_plane_hit(ox, dy, P, N, py)) return make_float4(0, 0, 0, 0);
+    const f3 ddx = px - P, ddy = py - P;
+    const float ax = fabsf(N.x), ay = fabsf(N.y), az = fabsf(N.z);
+    int d0, d1;
+    if (ax > ay && ax > az) { d0 = 1; d1 = 2; }
+    else if (ay > az) { d0 = 0; d1 = 2; }
+    else { d0 = 0; d1 = 1; }
+    const float a00 = get(dpdu, d0), a01 = get(dpdv, d0), a10 = get(dpdu, d1), a11 = get(dpdv, d1);
+    const float det = a00 * a11 - a01 * a10;
+    if (fabsf(det) < 1e-20f) return make_float4(0, 0, 0, 0);
+    const float inv = 1.0f / det;
+    const float bx0 = get(ddx, d0), bx1 = get(ddx, d1), by0 = get(ddy, d0), by1 = get(ddy, d1);
+    return make_float4((a11 * bx0 - a01 * bx1) * inv, (-a10 * bx0 + a00 * bx1) * inv,
+                       (a11 * by0 - a01 * by1) * inv, (-a10 * by0 + a00 * by1) * inv);
 }
 
 // ── lights (SphereLight/AreaLight/DirectionalLight/GeometryLight) ───────
@@ -699,7 +817,9 @@ __device__ __forceinline__ void camera_ray(const DevScene& S, const RParams& R, 
     rd = S.lens_radius * rd;
     f3 off = S.cam_u * rd.x + S.cam_vv * rd.y;
     o = S.cam_o + off;
-    d = target - o;
+    // GetRayWithDifferentials aims from the lens point; GetRay (texture
+    // filtering off) subtracts the origin and the offset in turn
+    d = R.diff_rays ? target - o : (target - S.cam_o) - off;
 }
 
 // Window sample `sid` → (part, job, pixel, pass) through the part table.
@@ -838,6 +958,94 @@ __global__ void __launch_bounds__(kBlock) k_trace(JOBS jobs, PathBufs P) {
         P.hit_tuv[p] = make_float4(h.t, h.u, h.v, 0.0f);
     });
     if (COUNT) flush_counts(P.tcount, tc);
+}
+
+// HitRecord.DpDu/DpDv an entry's hit writes (Triangle.cs:83-84, Quad.cs:79-80,
+// Sphere.cs:65-70); false for SmoothTriangle and InfinitePlane.
+__device__ __forceinline__ bool entry_dpdu(const DevScene& S, int32_t e, f3 o, f3 d, float t, f3& du, f3& dv) {
+    const GEntry en = S.entries[e];
+    const int kind = tag_of(en.a) & 0xff;
+    if (kind == kTri) {
+        if (S.smooth && S.smooth[4 * (size_t)__float_as_int(en.b.w) + 3].w != 0.0f) return false;
+        du = f3of(en.b);
+        dv = f3of(en.c);
+        return true;
+    }
+    if (kind == kQuad) {
+        const int ai = __float_as_int(en.d.y);
+        du = f3of(S.aux[4 * ai]);
+        dv = f3of(S.aux[4 * ai + 1]);
+        return true;
+    }
+    if (kind == kSphere) {
+        const Surf s = make_surf(S, e, o, d, t, 0.0f, 0.0f);
+        const f3 outward = (s.p - f3of(en.a)) / en.b.x;
+        const float st = sqrtf(fmaxf(0.0f, 1.0f - outward.y * outward.y));
+        du = s.tan * ((2.0f * kPi) * en.b.x * st);
+        dv = s.bitan * (kPi * en.b.x);
+        return true;
+    }
+    return false;
+}
+
+// Footprint of every camera-ray hit on an image-textured material
+// (Renderer.cs:1665-1677): the differentials of GetRayWithDifferentials
+// (Camera.cs:127-153, pixel deltas shrunk by ceil(sqrt(spp)),
+// Renderer.cs:616-624) projected on the hit's tangent plane and solved for
+// the UV partials against DpDu/DpDv, or the hit's tangent frame when the
+// record holds none.  A SmoothTriangle or plane hit keeps the DpDu/DpDv an
+// earlier accepted hit of the same traversal left on the reference's
+// record, so those hits re-trace the segment with trace_closest_dp.
+template <class JOBS, int SD, bool COUNT, bool PRIV>
+__global__ void __launch_bounds__(kBlock) k_fp(JOBS jobs, PathBufs P) {
+    __shared__ uint2 lds_stk[PRIV ? 1 : SD * kBlock];
+    uint2 priv_stk[PRIV ? SD : 1];
+    uint2* stk = PRIV ? priv_stk : &lds_stk[threadIdx.x];
+    constexpr int STRIDE = PRIV ? 1 : kBlock;
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    if (p >= P.n || P.state[p] != kSlotActive) return;
+    jobs.run(true, jobs.job_of(P.job, p), [&](const DevScene& S, const RParams& R) {
+        float4 fp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const int he = P.hit_e[p];
+        const float4 d4 = P.d[p];
+        const bool camera = pf_cat(__float_as_int(d4.w)) == 0;
+        if (R.footprints && he >= 0 && camera && __float_as_int(P.thr[p].w) == R.max_depth &&
+            (S.mats[__float_as_int(S.entries[he].c.w)].m0.y & kMatImage)) {
+            const f3 o = f3of(P.o[p]), d = f3of(d4);
+            const float4 h = P.hit_tuv[p];
+            const Surf s = make_surf(S, he, o, d, h.x, h.y, h.z);
+            f3 du = splat(0.0f), dv = splat(0.0f);
+            if (!entry_dpdu(S, he, o, d, h.x, du, dv)) {
+                int32_t dp;
+                float dpt;
+                trace_closest_dp<SD, STRIDE>(S.nodes, S.entries, S.aux, S.smooth, S.top_start, S.top_count, o, d,
+                                             kEps, kInfinity, stk, (uint32_t*)&P.counters[kCntOverflow], dp, dpt);
+                if (dp < 0 || !entry_dpdu(S, dp, o, d, dpt, du, dv)) du = dv = splat(0.0f);
+            }
+            const f3 dpdu = dot(du, du) > 0.0f ? du : s.tan;
+            const f3 dpdv = dot(dv, dv) > 0.0f ? dv : s.bitan;
+            // GetRayWithDifferentials: the primary's jitter (and lens) sample
+            const int pix = P.pix[p], i = pix % R.W, j = pix / R.W;
+            Rng rng{(uint32_t)P.pass[p], pixel_seed(i, j, R.seed_xor), 0u};
+            const float ju = rng.next(), jv = rng.next();
+            const float su = ((float)i + ju) / (float)R.W;
+            const float tv = ((float)(R.H - j - 1) + jv) / (float)R.H;
+            const int sq = (int)ceilf(sqrtf((float)R.spp));
+            const float shrink = 1.0f / (float)sq;
+            const float dsdx = shrink / (float)R.W, dtdy = shrink / (float)R.H;
+            f3 org = S.cam_o;
+            if (S.lens_radius != 0.0f) {
+                f3 lo, ld;
+                Rng r2{(uint32_t)P.pass[p], pixel_seed(i, j, R.seed_xor), 0u};
+                camera_ray(S, R, i, j, r2, lo, ld);
+                org = lo;
+            }
+            const f3 tx = (S.cam_llc + (su + dsdx) * S.cam_h) + tv * S.cam_v;
+            const f3 ty = (S.cam_llc + su * S.cam_h) + (tv - dtdy) * S.cam_v;
+            fp = uv_footprint(org, tx - org, ty - org, s.p, s.n, dpdu, dpdv);
+        }
+        P.fp[p] = fp;
+    });
 }
 
 __device__ __forceinline__ void add_target(PathBufs& P, int p, bool ind, f3 c) {
@@ -1084,6 +1292,10 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             if (textured) {
                 tuv = tex_coord(S, he, o, d, htuv);
                 g = textured_gmat(S, g, s.mat, tuv, tp);
+            } else if (g.m0.y & kMatImage) {
+                // camera hits carry the footprint k_fp left; later hits are point-sampled
+                const float4 fpv = P.fp ? P.fp[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                g = image_gmat(S, g, s.mat, tex_coord(S, he, o, d, htuv), fpv);
             }
             int kind = g.m0.x;
             int mflags = g.m0.y;
@@ -2011,6 +2223,9 @@ struct rt_scene {
     float4* d_tuv = nullptr;
     float4* d_smooth = nullptr;
     float* d_tex = nullptr;
+    GImage* d_images = nullptr;
+    GLevel* d_levels = nullptr;
+    float4* d_imtex = nullptr;
     int stack_depth = 0;
     int64_t device_bytes = 0;
 };
@@ -2094,6 +2309,9 @@ static DevScene dev_view(const rt_scene* sc) {
     S.raw = sc->d_raw;
     S.tuv = h.tex_uv.empty() ? nullptr : sc->d_tuv;
     S.smooth = h.smooth.empty() ? nullptr : sc->d_smooth;
+    S.images = h.images.empty() ? nullptr : sc->d_images;
+    S.img_levels = sc->d_levels;
+    S.img_texels = sc->d_imtex;
     S.tex = h.textured ? sc->d_tex : nullptr;
     S.tex_w = h.tex_w;
     S.tex_h = h.tex_h;
@@ -2180,6 +2398,7 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
             if (e) HIP_TRY(hipEventRecord(e[0], ctx->stream));
             DISPATCH_SD_C(JOBS, stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, jobs, Pw);
             if (e) HIP_TRY(hipEventRecord(e[1], ctx->stream));
+            if (Pw.fp) DISPATCH_SD_C(JOBS, stack_depth, false, ctx->priv_stack, k_fp, grid, ctx->stream, jobs, Pw);
             bool adj_replay = false;
             if constexpr (std::is_same<JOBS, OneJob>::value) adj_replay = Pw.adj_pass == 2;
             if (adj_replay) {
@@ -2455,6 +2674,8 @@ static int check_settings(rt_ctx* ctx, const rt_settings* s) {
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "negative photon budget or gather radius");
     if (s->light_sampling < RT_LIGHTS_ALL || s->light_sampling > RT_LIGHTS_UNIFORM)
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bad light_sampling");
+    if (s->texture_filtering < 0 || s->texture_filtering > 2)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bad texture_filtering");
     return RT_OK;
 }
 
@@ -2598,6 +2819,9 @@ int rt_scene_create(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
     if ((rc = upload(ctx, &sc->d_raw, h.raw_mats.data(), h.raw_mats.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_tuv, h.tex_uv.data(), h.tex_uv.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_smooth, h.smooth.data(), h.smooth.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_images, h.images.data(), h.images.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_levels, h.img_levels.data(), h.img_levels.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_imtex, h.img_texels.data(), h.img_texels.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_tex, h.texels.data(), h.texels.size(), b))) return cleanup(rc);
     std::vector<float> ld(2 * std::max<size_t>(h.lights.size(), 1), 0.0f);
     if ((rc = upload(ctx, &sc->d_ldist, ld.data(), ld.size(), b))) return cleanup(rc);
@@ -2620,6 +2844,9 @@ void rt_scene_destroy(rt_scene* sc) {
     hipFree(sc->d_tuv);
     hipFree(sc->d_smooth);
     hipFree(sc->d_tex);
+    hipFree(sc->d_images);
+    hipFree(sc->d_levels);
+    hipFree(sc->d_imtex);
     delete sc;
 }
 
@@ -2680,6 +2907,7 @@ int rt_trace_primary(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, int3
     R.npix = (int)npix;
     R.base_sample = sample_begin;
     R.seed_xor = s->seed_xor;
+    R.diff_rays = s->texture_filtering != 2 ? 1 : 0;
     DevScene S = dev_view(sc);
     char* mem = nullptr;
     size_t bytes = (size_t)n * 16 + 64;
@@ -2820,9 +3048,9 @@ int rt_photon_map(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, int64_t
 // ── render set-up helpers shared by rt_render and rt_render_batch ─────────
 // Path pool of N slots (and per-sample results of a window of WS samples)
 // in the context's grow-only buffer.
-static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, bool gather, PathBufs& P) {
+static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, bool gather, bool fp, PathBufs& P) {
     size_t per_path = 8 * 16 + 8 * 4 + 4 + 16 + (aovs ? 5 * 16 : 0) + 4 * 16 * (size_t)maxs + 8 * 4 + 4 * (size_t)maxs +
-                      (gather ? 64 : 0);
+                      (gather ? 64 : 0) + (fp ? 16 : 0);
     size_t per_sample = aovs ? 48 : 16;
     size_t need = (size_t)N * per_path + (size_t)WS * per_sample + 40 * 256 + 16384 + 8 * N +
                   4 * (size_t)((N + kBlock - 1) / kBlock + 3) * (kWcSeg + kMaxSeg);
@@ -2887,6 +3115,7 @@ static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, 
         P.res_a = (float4*)take(16 * WS);
         P.res_n = (float4*)take(16 * WS);
     }
+    if (fp) P.fp = (float4*)take(16 * N);
     if ((size_t)(m - (char*)ctx->path_mem) > ctx->path_bytes) return fail(ctx, RT_ERR_INTERNAL, "path buffer layout");
     return RT_OK;
 }
@@ -2997,6 +3226,8 @@ static RParams make_params(const rt_settings* s, const HostScene& h, int maxs) {
     R.seed_xor = s->seed_xor;
     R.aovs = s->capture_aovs ? 1 : 0;
     R.maxs = maxs;
+    R.diff_rays = s->texture_filtering != 2 ? 1 : 0;
+    R.footprints = R.diff_rays && !h.images.empty() ? 1 : 0;
     return R;
 }
 
@@ -3067,11 +3298,12 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     if ((int64_t)maxs * N > (1ll << 31) - 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "too many shadow slots");
     if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
     const bool want_caustics = s->caustics && s->caustic_photons > 0 && !h.lights.empty();
-    if (want_caustics && h.textured)
+    if (want_caustics && (h.textured || !h.images.empty()))
         return fail(ctx, RT_ERR_UNSUPPORTED, "photon caustics with textured materials are not supported");
 
     PathBufs P;
-    rc = alloc_paths(ctx, N, WS, maxs, s->capture_aovs != 0, want_caustics, P);
+    rc = alloc_paths(ctx, N, WS, maxs, s->capture_aovs != 0, want_caustics,
+                     !h.images.empty() && s->texture_filtering != 2, P);
     if (rc) return rc;
     FilmBufs F;
     size_t acc_bytes = 0;
@@ -3172,7 +3404,7 @@ int rt_adjoint(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const floa
     if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
 
     PathBufs P;
-    rc = alloc_paths(ctx, N, WS, maxs, false, false, P);
+    rc = alloc_paths(ctx, N, WS, maxs, false, false, !h.images.empty() && s->texture_filtering != 2, P);
     if (rc) return rc;
     // adjoint buffers: pending bounce terms, NEE terms, dLoss/dBeauty, gradient
     const size_t b_q = 16 * 5 * (size_t)N, b_g = 16 * 2 * (size_t)N * std::max(maxs, 1);
@@ -3331,8 +3563,11 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
     const int64_t N = std::min<int64_t>(N0, WS);
     if (N > (1ll << 30) || (int64_t)maxs * N > (1ll << 31) - 1)
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "path pool too large");
+    bool any_fp = false;
+    for (int j = 0; j < n_jobs; j++)
+        any_fp |= !jobs[j].scene->hs.images.empty() && jobs[j].settings->texture_filtering != 2;
     PathBufs P;
-    rc = alloc_paths(ctx, N, WS, maxs, aovs, false, P);
+    rc = alloc_paths(ctx, N, WS, maxs, aovs, false, any_fp, P);
     if (rc) return rc;
     // per-job films, one buffer
     std::vector<FilmBufs> F(n_jobs);

@@ -64,6 +64,18 @@ enum MatFlags : int32_t {
     kMatDielectricIor = 8,      // TryGetDielectricIor returns true
     kMatEmissiveProxy = 16,
     kMatTextured = 32,          // baseColor/roughness/metallic from the scene's texture set
+    kMatImage = 64,             // albedo / baseColor from an ImageTexture (rt_material.image)
+};
+
+// ImageTexture on the device (ImageTexture.cs:42-289): one header per image
+// and a table of mip levels; texels are linear RGB (w unused), level by level.
+struct GImage {
+    int32_t level0;             // first entry of the level table
+    int32_t mips;               // level count, 1 + floor(log2(max(w, h)))
+    float scale_u, scale_v;     // uv_scale
+};
+struct GLevel {
+    int32_t off, w, h, pad;     // first texel, size
 };
 
 // Device material record (8 × 16 B).  Disney ShadingParams and the

@@ -58,6 +58,10 @@ struct HostScene {
     // SmoothTriangle data, 4 per primitive: (N0, T.x) (N1, T.y) (N2, T.z)
     // (B, 1); a zero record is a flat triangle; empty = no smooth triangles
     std::vector<float4> smooth;
+    // image textures (ImageTexture): headers, mip levels, linear RGB texels
+    std::vector<GImage> images;
+    std::vector<GLevel> img_levels;
+    std::vector<float4> img_texels;
 };
 
 // Builds the host scene; returns RT_OK or an error with `err` set.

@@ -36,7 +36,8 @@ static_assert(sizeof(rt_material) == 128, "rt_material layout");
 static_assert(sizeof(rt_light) == 80, "rt_light layout");
 static_assert(sizeof(rt_camera) == 56, "rt_camera layout");
 static_assert(sizeof(rt_medium) == 32, "rt_medium layout");
-static_assert(sizeof(rt_scene_desc) == 184, "rt_scene_desc layout");
+static_assert(sizeof(rt_scene_desc) == 200, "rt_scene_desc layout");
+static_assert(sizeof(rt_image) == 24, "rt_image layout");
 static_assert(sizeof(rt_settings) == 96, "rt_settings layout");
 static_assert(sizeof(rt_film) == 88, "rt_film layout");
 static_assert(sizeof(rt_render_stats) == 152, "rt_render_stats layout");
@@ -68,6 +69,82 @@ static void uv_frame(f3 e1, f3 e2, const float* uv, f3& T, f3& B) {
     T = normalize(e1);
     B = normalize(e2);
 }
+// ImageTexture construction (ImageTexture.cs:60-138): 8-bit sRGB decoded
+// through the gamma-2.2 table, level 0 in linear RGB, then 2x2 box levels
+// (floor halving, the last row/column reused at odd edges, sum in texel
+// order times 0.25) down to 1 + floor(log2(max(w, h))) levels.
+static int build_images(const rt_scene_desc* d, HostScene& hs, std::string& err) {
+    const int64_t ni = d->num_images;
+    if (ni < 0 || (ni > 0 && !d->images)) { err = "bad image table"; return RT_ERR_INVALID_ARGUMENT; }
+    for (int64_t i = 0; i < d->num_materials; i++) {
+        const rt_material& m = d->materials[i];
+        if (!(m.flags & RT_MATF_IMAGE)) continue;
+        if (m.kind != RT_MAT_LAMBERTIAN && m.kind != RT_MAT_DISNEY) {
+            err = "image textures are supported as Lambertian albedo and Disney baseColor only";
+            return RT_ERR_UNSUPPORTED;
+        }
+        if (m.kind == RT_MAT_DISNEY && (m.spec_trans != 0.0f || (m.flags & RT_MATF_TEXTURED))) {
+            err = "an image-textured Disney material needs spec_trans = 0 and no texture set";
+            return RT_ERR_UNSUPPORTED;
+        }
+        if (m.image < 0 || m.image >= ni) { err = "material image index out of range"; return RT_ERR_INVALID_ARGUMENT; }
+    }
+    static float lut[256];
+    static bool lut_ready = false;
+    if (!lut_ready) {
+        const float inv255 = 1.0f / 255.0f;
+        for (int i = 0; i < 256; i++) lut[i] = powf((float)i * inv255, 2.2f);
+        lut_ready = true;
+    }
+    int64_t total = 0;
+    for (int64_t k = 0; k < ni; k++) {
+        const rt_image& im = d->images[k];
+        if (!im.rgba8 || im.width <= 0 || im.height <= 0 || (int64_t)im.width * im.height > (1ll << 26)) {
+            err = "bad image (null pixels, empty, or more than 2^26 texels)";
+            return RT_ERR_INVALID_ARGUMENT;
+        }
+        int w = im.width, h = im.height, big = std::max(w, h), mips = 1;
+        while ((big >>= 1) > 0) mips++;
+        GImage g{(int32_t)hs.img_levels.size(), mips, im.scale_u, im.scale_v};
+        hs.images.push_back(g);
+        int64_t off = (int64_t)hs.img_texels.size();
+        for (int m = 0; m < mips; m++) {
+            total += (int64_t)w * h;
+            if (total > (1ll << 30)) { err = "image textures exceed 2^30 texels"; return RT_ERR_UNSUPPORTED; }
+            hs.img_levels.push_back(GLevel{(int32_t)off, w, h, 0});
+            off += (int64_t)w * h;
+            w = w > 1 ? w / 2 : 1;
+            h = h > 1 ? h / 2 : 1;
+        }
+        hs.img_texels.resize((size_t)off);
+        const GLevel* L = &hs.img_levels[g.level0];
+        float4* t0 = &hs.img_texels[L[0].off];
+        for (int64_t p = 0; p < (int64_t)L[0].w * L[0].h; p++) {
+            const uint8_t* px = im.rgba8 + 4 * p;
+            t0[p] = make_float4(lut[px[0]], lut[px[1]], lut[px[2]], 0.0f);
+        }
+        for (int m = 1; m < mips; m++) {
+            const GLevel pv = L[m - 1], cu = L[m];
+            const float4* prev = &hs.img_texels[pv.off];
+            float4* cur = &hs.img_texels[cu.off];
+            for (int y = 0; y < cu.h; y++)
+                for (int x = 0; x < cu.w; x++) {
+                    int x0 = 2 * x, y0 = 2 * y;
+                    int x1 = std::min(x0 + 1, pv.w - 1), y1 = std::min(y0 + 1, pv.h - 1);
+                    float r = 0.0f, gg = 0.0f, b = 0.0f;
+                    const int idx[4] = {y0 * pv.w + x0, y0 * pv.w + x1, y1 * pv.w + x0, y1 * pv.w + x1};
+                    for (int q = 0; q < 4; q++) {
+                        r += prev[idx[q]].x;
+                        gg += prev[idx[q]].y;
+                        b += prev[idx[q]].z;
+                    }
+                    cur[y * cu.w + x] = make_float4(r * 0.25f, gg * 0.25f, b * 0.25f, 0.0f);
+                }
+        }
+    }
+    return RT_OK;
+}
+
 static int32_t fbits(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
 static float ibits(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
 
@@ -358,6 +435,7 @@ static GMat make_gmat(const rt_material& m, const float* Eavg) {
     switch (m.kind) {
         case RT_MAT_LAMBERTIAN:
             g.m1 = make_float4(c.x, c.y, c.z, 0.0f);
+            if (m.flags & RT_MATF_IMAGE) flags |= kMatImage;
             break;
         case RT_MAT_EMISSIVE: {
             float inten = mx(m.intensity, 0.0f);
@@ -527,6 +605,18 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
             hs.opaque_shadows = false;
     }
     hs.raw_mats.assign(d->materials, d->materials + d->num_materials);
+    int rc_img = build_images(d, hs, err);
+    if (rc_img != RT_OK) return rc_img;
+    bool any_image = false;
+    for (int64_t i = 0; i < d->num_materials; i++) any_image |= (d->materials[i].flags & RT_MATF_IMAGE) != 0;
+    if (any_image && d->texcoords && !hs.textured) {
+        hs.tex_uv.resize(2 * (size_t)np);
+        for (int64_t i = 0; i < np; i++) {
+            const float* t = d->texcoords + 6 * i;
+            hs.tex_uv[2 * i] = make_float4(t[0], t[1], t[2], t[3]);
+            hs.tex_uv[2 * i + 1] = make_float4(t[4], t[5], 0.0f, 0.0f);
+        }
+    }
     if (hs.textured) {
         hs.tex_w = d->tex_width;
         hs.tex_h = d->tex_height;

@@ -260,6 +260,147 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
     return h;
 }
 
+// Closest-hit traversal that also reports which accepted hit last wrote
+// HitRecord.DpDu/DpDv on the reference's shared record.  BvhNode.Hit and
+// Mesh.Hit pass one record down the whole chain and every accepted primitive
+// hit overwrites the fields it sets: Triangle (edges), Quad (U, V) and
+// Sphere (scaled tangent frame) set DpDu/DpDv, SmoothTriangle and
+// InfinitePlane do not, so after a traversal whose closest hit is one of the
+// latter the record still holds the last earlier writer's values.  A
+// HittableList (the world list, a Mesh of <= 4 faces) collects its members
+// in a fresh record and copies it out only when a member hit (Triangle.cs:83,
+// Quad.cs:79, Sphere.cs:68-70, HittableList.cs:19-38, BvhNode.cs:157-206).
+// Returns the closest hit as trace_closest does, with h_dp = the entry of the
+// last writer (-1: none) and dp_t its hit distance.  `smooth` marks smooth
+// triangles (4 float4 per primitive, w of the fourth != 0).
+template <int SD, int STRIDE>
+__device__ __forceinline__ HitOut trace_closest_dp(const GNode* __restrict__ nodes, const GEntry* __restrict__ entries,
+                                                   const float4* __restrict__ aux, const float4* __restrict__ smooth,
+                                                   uint32_t top_start, uint32_t top_count, f3 o, f3 d, float tmin,
+                                                   float tmax, uint2* stk, uint32_t* overflow, int32_t& dp,
+                                                   float& dp_t) {
+    HitOut h;
+    h.entry = -1;
+    h.t = 0.0f;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    dp = -1;
+    dp_t = 0.0f;
+    TravCount tc{0u, 0u, 0u};
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float closest = tmax;
+    auto writer = [&](uint32_t idx, int32_t kind, float4 b) {
+        if (kind == kQuad || kind == kSphere) return true;
+        if (kind != kTri) return false;
+        return !(smooth && smooth[4 * (size_t)__float_as_int(b.w) + 3].w != 0.0f);
+    };
+    for (uint32_t ti = 0; ti < top_count; ti++) {
+        uint32_t eidx = top_start + ti;
+        float4 ea = entries[eidx].a, eb = entries[eidx].b, ec = entries[eidx].c;
+        int32_t tag = tag_of(ea);
+        int32_t kind = tag & 0xff, flags = tag >> 8;
+        if (kind != kBvh) {
+            if (test_prim<false>(entries, ea, eb, ec, eidx, kind, flags, o, d, tmin, aux, closest, h, tc) &&
+                writer(eidx, kind, eb)) {
+                dp = (int32_t)eidx;
+                dp_t = h.t;
+            }
+            continue;
+        }
+        float te;
+        if (!(flags & kEntryNoBox) && !slab(f3of(ea), f3of(eb), o, inv, tmin, closest, te)) continue;
+        uint32_t cur = (uint32_t)__float_as_int(eb.w);
+        bool list_leaf = (flags & kEntryNoBox) != 0;
+        int sp = 0;
+        for (;;) {
+            if (ref_is_leaf(cur)) {
+                uint32_t start = leaf_start(cur), cnt = leaf_count(cur);
+                bool descended = false;
+                // a <= 4-face Mesh is a HittableList: its own record
+                int32_t outer_dp = dp;
+                float outer_t = dp_t;
+                bool any = false;
+                if (list_leaf) dp = -1;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    uint32_t idx = start + k;
+                    float4 la = entries[idx].a, lb = entries[idx].b, lc = entries[idx].c;
+                    int32_t ltag = tag_of(la);
+                    int32_t lk = ltag & 0xff, lf = ltag >> 8;
+                    if (lk != kBvh) {
+                        if (test_prim<false>(entries, la, lb, lc, idx, lk, lf, o, d, tmin, aux, closest, h, tc)) {
+                            any = true;
+                            if (writer(idx, lk, lb)) {
+                                dp = (int32_t)idx;
+                                dp_t = h.t;
+                            }
+                        }
+                        continue;
+                    }
+                    float lte;
+                    if (!(lf & kEntryNoBox) && !slab(f3of(la), f3of(lb), o, inv, tmin, closest, lte)) continue;
+                    if (k + 1 < cnt) {
+                        if (sp >= SD) { *overflow = 1u; return h; }
+                        stk[sp * STRIDE] = make_uint2(leaf_ref(idx + 1, cnt - k - 1), __float_as_uint(-FLT_MAX));
+                        sp++;
+                    }
+                    cur = (uint32_t)__float_as_int(lb.w);
+                    list_leaf = (lf & kEntryNoBox) != 0;
+                    descended = true;
+                    break;
+                }
+                if (list_leaf && !descended) {
+                    if (!any) {
+                        dp = outer_dp;
+                        dp_t = outer_t;
+                    }
+                    list_leaf = false;
+                }
+                if (descended) continue;
+            } else {
+                const GNode nd = nodes[cur];
+                f3 c0mn = mk(nd.a.x, nd.a.y, nd.a.z), c0mx = mk(nd.a.w, nd.b.x, nd.b.y);
+                f3 c1mn = mk(nd.b.z, nd.b.w, nd.c.x), c1mx = mk(nd.c.y, nd.c.z, nd.c.w);
+                float te0, te1;
+                bool h0 = slab(c0mn, c0mx, o, inv, tmin, closest, te0);
+                bool h1 = slab(c1mn, c1mx, o, inv, tmin, closest, te1);
+                int axis = nd.d.z;
+                float da = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+                bool left_first = da >= 0.0f;
+                uint32_t rn = (uint32_t)(left_first ? nd.d.x : nd.d.y);
+                uint32_t rf = (uint32_t)(left_first ? nd.d.y : nd.d.x);
+                bool hn = left_first ? h0 : h1, hf = left_first ? h1 : h0;
+                float tf = left_first ? te1 : te0;
+                if (hn) {
+                    if (hf) {
+                        if (sp >= SD) { *overflow = 1u; return h; }
+                        stk[sp * STRIDE] = make_uint2(rf, __float_as_uint(tf));
+                        sp++;
+                    }
+                    cur = rn;
+                    continue;
+                }
+                if (hf) {
+                    cur = rf;
+                    continue;
+                }
+            }
+            // pop
+            bool found = false;
+            while (sp > 0) {
+                sp--;
+                uint2 s = stk[sp * STRIDE];
+                if (__uint_as_float(s.y) <= closest) {
+                    cur = s.x;
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) break;
+        }
+    }
+    return h;
+}
+
 // Box test of the any-hit traversal: the same slab distances as slab(),
 // without the NaN and empty-box guards.  A NaN slab distance only arises for
 // an axis-parallel ray whose origin lies exactly on a box plane (the

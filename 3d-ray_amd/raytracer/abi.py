@@ -41,7 +41,7 @@ class rt_material(C.Structure):
                 ("anisotropic_rotation", C.c_float), ("diff_trans", C.c_float), ("coat_ior", C.c_float),
                 ("coat_roughness", C.c_float), ("thin_film_thickness", C.c_float),
                 ("thin_film_ior", C.c_float), ("transmission_depth", C.c_float),
-                ("transmission_color", F3), ("reserved", C.c_int32 * 4)]
+                ("transmission_color", F3), ("image", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class rt_light(C.Structure):
@@ -68,7 +68,16 @@ class rt_scene_desc(C.Structure):
                 ("camera", rt_camera), ("sky_color", F3), ("reserved0", C.c_int32),
                 ("global_medium", rt_medium), ("texcoords", C.POINTER(C.c_float)),
                 ("texels", C.POINTER(C.c_float)), ("tex_width", C.c_int32), ("tex_height", C.c_int32),
-                ("normals", C.POINTER(C.c_float))]
+                ("normals", C.POINTER(C.c_float)), ("images", C.c_void_p), ("num_images", C.c_int64)]
+
+
+class rt_image(C.Structure):
+    _fields_ = [("rgba8", C.POINTER(C.c_uint8)), ("width", C.c_int32), ("height", C.c_int32),
+                ("scale_u", C.c_float), ("scale_v", C.c_float)]
+
+
+RT_MATF_IMAGE = 32
+TEXTURE_FILTERING = {"auto": 0, "on": 1, "off": 2}
 
 
 class rt_settings(C.Structure):
@@ -79,7 +88,8 @@ class rt_settings(C.Structure):
                 ("exposure_ev", C.c_float), ("rr_min_bounces", C.c_int32),
                 ("rr_min_survival", C.c_float), ("seed_xor", C.c_uint32), ("caustics", C.c_int32),
                 ("caustic_photons", C.c_int32), ("caustic_gather_radius", C.c_float),
-                ("paths_in_flight", C.c_int32), ("count_traversal", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("paths_in_flight", C.c_int32), ("count_traversal", C.c_int32),
+                ("texture_filtering", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 FP = C.POINTER(C.c_float)

@@ -59,12 +59,53 @@ def _f(d, key, default):
     return default if v is None else float(np.float32(v))
 
 
-_UNSUPPORTED_MATERIAL_KEYS = ("texture", "normal_map", "bump_map", "coat_normal_map", "subsurface_radius",
+_UNSUPPORTED_MATERIAL_KEYS = ("normal_map", "bump_map", "coat_normal_map", "subsurface_radius",
                               "displacement", "emission_texture")
+MISSING_TEXTURE = (1.0, 0.0, 1.0)        # CreateImageTexture's magenta fallback
 
 
-def _material(b, m):
+def read_image(path):
+    """8-bit RGBA pixels (h, w, 4) of an image file, top row first (the
+    Rgba32 rows ImageSharp hands ImageTexture)."""
+    from PIL import Image
+    with Image.open(path) as im:
+        if im.mode in ("I;16", "I;16B", "I", "F"):
+            raise SceneLoadError(f"image texture '{path}': {im.mode} images are not supported (8-bit only)")
+        return np.asarray(im.convert("RGBA"), np.uint8).copy()
+
+
+def _image_texture(b, t, base_dir, warnings, mid):
+    """CreateImageTexture (SceneLoader.cs:1879-1910): the image at `path`
+    (relative to the scene file) with uv_scale; a missing path or file is
+    the magenta SolidColor, with a warning.  Returns an image index or None."""
+    for k in t:
+        if k not in ("type", "path", "uv_scale") and t[k] is not None:
+            raise SceneLoadError(f"material '{mid}': image texture key '{k}' is not supported")
+    path = t.get("path")
+    if path is None or not str(path).strip():
+        warnings.append("Image texture requires a 'path' field. Using fallback magenta.")
+        return None
+    path = str(path)
+    full = path if os.path.isabs(path) else os.path.join(base_dir or ".", path)
+    if not os.path.isfile(full):
+        warnings.append(f"Image texture file not found: {full}. Using fallback magenta.")
+        return None
+    try:
+        px = read_image(full)
+    except SceneLoadError:
+        raise
+    except Exception as err:          # the reference warns and falls back
+        warnings.append(f"Failed to load image texture '{full}': {err}. Using fallback magenta.")
+        return None
+    sc = t.get("uv_scale")
+    su = float(np.float32(sc[0])) if sc else 1.0
+    sv = float(np.float32(sc[1])) if sc and len(sc) > 1 else su
+    return b.image(px, (su, sv))
+
+
+def _material(b, m, base_dir=None, warnings=None):
     """CreateMaterial for the supported kinds; returns the table index."""
+    warnings = [] if warnings is None else warnings
     typ = (m.get("type") or "").lower()
     if typ in ("mix", "blend"):
         raise SceneLoadError(f"material '{m.get('id')}': mix materials are not supported")
@@ -74,8 +115,19 @@ def _material(b, m):
         if k in _UNSUPPORTED_MATERIAL_KEYS or (k.endswith("_texture") and m[k] is not None):
             raise SceneLoadError(f"material '{m.get('id')}': '{k}' (textures and maps) is not supported")
     color = _v3(m.get("color"), (0.5, 0.5, 0.5))
+    img = None
+    if m.get("texture") is not None:
+        t = m["texture"]
+        tt = (t.get("type") or "").lower() if isinstance(t, dict) else ""
+        if tt != "image":
+            raise SceneLoadError(f"material '{m.get('id')}': texture type '{tt}' is not supported (image only)")
+        if typ not in ("lambertian", "disney", "disney_bsdf", "pbr", ""):
+            raise SceneLoadError(f"material '{m.get('id')}': image textures on '{typ}' are not supported")
+        img = _image_texture(b, t, base_dir, warnings, m.get("id"))
+        if img is None:
+            color = MISSING_TEXTURE
     if typ == "lambertian":
-        return b.lambertian(color)
+        return b.lambertian(color, image=img)
     if typ == "dielectric":
         return b.dielectric(_f(m, "refraction_index", 1.5), color)
     if typ == "emissive":
@@ -98,8 +150,12 @@ def _material(b, m):
         cr = _f(m, "coat_roughness", -1.0)
         if cr >= 0.0:
             kw["coat_roughness"] = max(cr, 0.0)
+        if img is not None:
+            if kw["spec_trans"] != 0.0:
+                raise SceneLoadError(f"material '{m.get('id')}': an image-textured Disney needs spec_trans = 0")
+            kw["image"] = img
         return b.disney(color, **kw)
-    return b.lambertian(color)
+    return b.lambertian(color, image=img)
 
 
 def _transform(e):
@@ -290,6 +346,7 @@ def load_data(data, width, height, shadow_samples_override=None, camera=None, ba
     meshes, malformed OBJ lines) are appended to `warnings`."""
     if not isinstance(data, dict):
         raise SceneLoadError("not a scene document")
+    warnings = [] if warnings is None else warnings
     for k in ("imports", "templates", "mediums"):
         if data.get(k):
             raise SceneLoadError(f"'{k}' is not supported")
@@ -302,7 +359,7 @@ def load_data(data, width, height, shadow_samples_override=None, camera=None, ba
     for m in data.get("materials") or []:
         if m.get("id") is None:
             continue
-        materials[str(m["id"])] = _material(b, m)
+        materials[str(m["id"])] = _material(b, m, base_dir, warnings)
 
     def get_material(mid):
         if mid is not None and str(mid) in materials:

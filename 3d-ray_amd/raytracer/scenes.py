@@ -29,7 +29,7 @@ MAT_DTYPE = np.dtype([("kind", "<i4"), ("flags", "<i4"), ("color", "<f4", (3,)),
                       ("clearcoat_gloss", "<f4"), ("spec_trans", "<f4"), ("ior", "<f4"), ("anisotropic", "<f4"),
                       ("anisotropic_rotation", "<f4"), ("diff_trans", "<f4"), ("coat_ior", "<f4"),
                       ("coat_roughness", "<f4"), ("thin_film_thickness", "<f4"), ("thin_film_ior", "<f4"),
-                      ("transmission_depth", "<f4"), ("transmission_color", "<f4", (3,)), ("reserved", "<i4", (4,))])
+                      ("transmission_depth", "<f4"), ("transmission_color", "<f4", (3,)), ("image", "<i4"), ("reserved", "<i4", (3,))])
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("shadow_samples", "<i4"), ("primitive", "<i4"), ("material", "<i4"),
                         ("color", "<f4", (3,)), ("intensity", "<f4"), ("position", "<f4", (3,)), ("radius", "<f4"),
                         ("u", "<f4", (3,)), ("soft_radius", "<f4"), ("v", "<f4", (3,)), ("reserved", "<f4")])
@@ -56,7 +56,7 @@ class SceneDesc:
     """Owns the numpy tables behind one rt_scene_desc (keeps them alive)."""
 
     def __init__(self, prims, mats, lights, camera, sky=(0.0, 0.0, 0.0), medium=None, texcoords=None,
-                 texture=None, normals=None):
+                 texture=None, normals=None, images=None):
         self.prims = np.ascontiguousarray(prims, PRIM_DTYPE)
         self.mats = np.ascontiguousarray(mats, MAT_DTYPE)
         self.lights = np.ascontiguousarray(lights, LIGHT_DTYPE)
@@ -91,6 +91,19 @@ class SceneDesc:
         if self.normals is not None:
             assert len(self.normals) == len(self.prims)
             d.normals = C.cast(self.normals.ctypes.data, C.POINTER(C.c_float))
+        # image textures: [(rgba8 (h, w, 4) uint8, (scale_u, scale_v))]
+        self.images = [(np.ascontiguousarray(px, np.uint8), tuple(float(np.float32(x)) for x in sc))
+                       for px, sc in (images or [])]
+        if self.images:
+            self._image_recs = (abi.rt_image * len(self.images))()
+            for k, (px, sc) in enumerate(self.images):
+                assert px.ndim == 3 and px.shape[2] == 4
+                r = self._image_recs[k]
+                r.rgba8 = C.cast(px.ctypes.data, C.POINTER(C.c_uint8))
+                r.height, r.width = px.shape[0], px.shape[1]
+                r.scale_u, r.scale_v = sc
+            d.images = C.cast(self._image_recs, C.c_void_p)
+            d.num_images = len(self.images)
         self.desc = d
 
     @property
@@ -117,6 +130,7 @@ class SceneBuilder:
         self._next_mesh = 0
         self._normals = []        # (first primitive, (n, 9) vertex normals) of smooth triangles
         self._uvs = []            # (first primitive, (n, 6) vertex UVs)
+        self._images = []         # image textures: (rgba8, (scale_u, scale_v))
         self.geometry_shadow_samples = 4
         self.shadow_samples_override = None
 
@@ -146,12 +160,23 @@ class SceneBuilder:
                 m["coat_roughness"] = v
             elif k == "textured":
                 flags |= abi.RT_MATF_TEXTURED if v else 0
+            elif k == "image":
+                flags |= abi.RT_MATF_IMAGE
+                m["image"] = v
             else:
                 m[k] = v
         m["flags"] = flags
         return self.add_materials(m)
 
-    def lambertian(self, albedo):
+    def image(self, rgba8, scale=(1.0, 1.0)):
+        """An ImageTexture from 8-bit RGBA pixels (h, w, 4), top row first,
+        with the YAML uv_scale; returns its index for image=."""
+        self._images.append((np.ascontiguousarray(rgba8, np.uint8), tuple(scale)))
+        return len(self._images) - 1
+
+    def lambertian(self, albedo, image=None):
+        if image is not None:
+            return self.material(abi.RT_MAT_LAMBERTIAN, albedo, image=image)
         return self.material(abi.RT_MAT_LAMBERTIAN, albedo)
 
     def emissive(self, color, intensity=1.0, light_proxy=False):
@@ -315,7 +340,8 @@ class SceneBuilder:
             g["material"] = prims["material"][emissive]
             lights.append(g)
         lights = np.concatenate(lights) if lights else np.zeros(0, LIGHT_DTYPE)
-        return SceneDesc(prims, mats, lights, camera, sky, medium, texcoords=texcoords, normals=normals)
+        return SceneDesc(prims, mats, lights, camera, sky, medium, texcoords=texcoords, normals=normals,
+                         images=self._images)
 
 
 # ═══════════════════════════════════════════════════════════════════════

@@ -108,11 +108,16 @@ typedef enum rt_material_flags {
     RT_MATF_THIN_WALLED = 2,          /* DisneyBsdf.ThinWalled */
     RT_MATF_TRANSMISSION_COLOR = 4,   /* DisneyBsdf.TransmissionColor != null */
     RT_MATF_COAT_ROUGHNESS = 8,       /* DisneyBsdf.CoatRoughness != null */
-    RT_MATF_TEXTURED = 16             /* DISNEY only: baseColor, roughness and metallic
+    RT_MATF_TEXTURED = 16,            /* DISNEY only: baseColor, roughness and metallic
                                          come from the scene's texture set at the
                                          hit's texture coordinate (no reference
                                          counterpart; spec_trans and diff_trans
                                          must be 0) */
+    RT_MATF_IMAGE = 32                /* LAMBERTIAN albedo / DISNEY baseColor is the
+                                         ImageTexture rt_material.image of the scene's
+                                         image table (SceneLoader.cs:1027-1031,
+                                         ImageTexture.cs:42-289); DISNEY needs
+                                         spec_trans = 0 */
 } rt_material_flags;
 
 typedef struct rt_material {
@@ -126,7 +131,8 @@ typedef struct rt_material {
     float anisotropic_rotation, diff_trans, coat_ior, coat_roughness;
     float thin_film_thickness, thin_film_ior, transmission_depth;
     float transmission_color[3];
-    int32_t reserved[4];
+    int32_t image;      /* RT_MATF_IMAGE: index into rt_scene_desc.images */
+    int32_t reserved[3];
 } rt_material;          /* 128 bytes */
 
 /* ── Lights (final ILight list order of SceneLoader.Load) ─────────────────
@@ -215,7 +221,24 @@ typedef struct rt_scene_desc {
      * tangent frame from the UV gradients of its texcoords (barycentric UVs
      * without them).  Such a triangle may not be a GEOMETRY light. */
     const float* normals;
+    /* Image textures of RT_MATF_IMAGE materials (ImageTexture.cs:42-289). */
+    const struct rt_image* images;
+    int64_t num_images;
 } rt_scene_desc;
+
+/* One ImageTexture: the decoded 8-bit image (ImageSharp's Rgba32 rows, top
+ * row first; alpha ignored) and the YAML uv_scale.  The library decodes
+ * sRGB with ImageTexture's gamma-2.2 table, builds the 2x2 box mip chain
+ * and filters as ImageTexture.Value does (bilinear at level 0 without a
+ * footprint; ratio-trilinear with at most 8 taps from the camera ray's
+ * differentials, FootprintMath.cs:38-119). */
+typedef struct rt_image {
+    const uint8_t* rgba8;   /* width * height * 4 bytes */
+    int32_t width;
+    int32_t height;
+    float scale_u;
+    float scale_v;
+} rt_image;             /* 24 bytes */
 
 /* ── Render settings (Renderer ctor arguments + Render(width, height)) ───── */
 typedef enum rt_sampler_kind { RT_SAMPLER_SOBOL = 0, RT_SAMPLER_PRNG = 1 } rt_sampler_kind;
@@ -245,7 +268,10 @@ typedef struct rt_settings {
     int32_t paths_in_flight;        /* path slots in flight; 0 = library default (2^24) */
     int32_t count_traversal;        /* 1: count BVH node/primitive tests (slower
                                        instrumented kernels; fills rt_render_stats) */
-    int32_t reserved[5];
+    int32_t texture_filtering;      /* --texture-filtering: 0 auto (= on), 1 on, 2 off:
+                                       camera-ray differentials for image textures
+                                       (Renderer.cs:344-348) */
+    int32_t reserved[4];
 } rt_settings;          /* 96 bytes */
 
 /* ── Film (RenderBuffers.cs:52-169; FrameBuffer planar layout

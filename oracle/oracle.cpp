@@ -92,6 +92,8 @@ struct Material;
 // HitRecord.cs:9-163 (the fields the path reads)
 struct HitRecord {
     V3 point, normal, shading_ng, tangent, bitangent;
+    V3 dpdu, dpdv;                 // HitRecord.DpDu/DpDv: left as they are by primitives that do not set them
+    float fp[4] = {0, 0, 0, 0};    // FilterFootprint UV partials (dudx, dvdx, dudy, dvdy); zero = none
     float t = 0, u = 0, v = 0;
     bool front_face = false;
     const Material* mat = nullptr;
@@ -151,6 +153,8 @@ struct Triangle : Hittable {
         rec.v = v;
         rec.tangent = tan;
         rec.bitangent = bitan;
+        rec.dpdu = e1;                  // Triangle.cs:83-84
+        rec.dpdv = e2;
         rec.mat = mat;
         rec.prim = prim;
         rec.camera_invisible = false;
@@ -209,7 +213,10 @@ struct SmoothTriangle : Triangle {
         }
     }
     bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override {
+        V3 du = rec.dpdu, dv = rec.dpdv;   // SmoothTriangle.Hit does not set DpDu/DpDv
         if (!Triangle::hit(r, tmin, tmax, rec)) return false;
+        rec.dpdu = du;
+        rec.dpdv = dv;
         float w0 = 1.0f - rec.u - rec.v;
         V3 sn = normalize(w0 * n0 + rec.u * n1 + rec.v * n2);
         rec.normal = rec.front_face ? sn : -sn;
@@ -269,6 +276,10 @@ struct Sphere : Hittable {
         if (length_sq(td) < 1e-4f) td = V3(1, 0, 0);
         rec.tangent = normalize(td);
         rec.bitangent = normalize(cross(outward, rec.tangent));
+        // Sphere.cs:65-70: dP/du and dP/dv of phi = 2 pi u, theta = pi v
+        float st = std::sqrt(std::fmax(0.0f, 1.0f - outward.y * outward.y));
+        rec.dpdu = rec.tangent * (2.0f * kPi * r * st);
+        rec.dpdv = rec.bitangent * (kPi * r);
         rec.mat = mat;
         rec.prim = prim;
         rec.camera_invisible = false;
@@ -367,6 +378,8 @@ struct Quad : Hittable {
         rec.set_face_normal(r, n);
         rec.tangent = tan;
         rec.bitangent = bitan;
+        rec.dpdu = u;                   // Quad.cs:79-80
+        rec.dpdv = v;
         rec.mat = mat;
         rec.prim = prim;
         rec.camera_invisible = false;
@@ -792,6 +805,10 @@ struct Material {
     // textured Disney: the material record whose baseColor / roughness /
     // metallic the texture set replaces per hit (null: not textured)
     virtual const rt_material* texture_raw() const { return nullptr; }
+    // ImageTexture albedo / baseColor: the image index and the material
+    // record whose colour it replaces per hit (-1: none)
+    int image = -1;
+    rt_material image_raw{};
 };
 
 // Lambertian.cs:37-96
@@ -1506,6 +1523,137 @@ static Material* make_material(const rt_material& m) {
 }
 
 // ═════════════════════════════════════════════════════════════════════════
+// ImageTexture (ImageTexture.cs:42-289) and FootprintMath (FootprintMath.cs:38-119)
+// ═════════════════════════════════════════════════════════════════════════
+struct ImageTex {
+    std::vector<std::vector<float>> mip;   // linear RGB, row-major
+    std::vector<int> mw, mh;
+    float su = 1.0f, sv = 1.0f;
+    explicit ImageTex(const rt_image& im) : su(im.scale_u), sv(im.scale_v) {
+        // gamma-2.2 decode table of the byte values
+        float table[256];
+        const float inv255 = 1.0f / 255.0f;
+        for (int i = 0; i < 256; i++) table[i] = std::pow(i * inv255, 2.2f);
+        int w = im.width, h = im.height;
+        std::vector<float> l0((size_t)w * h * 3);
+        for (size_t p = 0; p < (size_t)w * h; p++)
+            for (int c = 0; c < 3; c++) l0[3 * p + c] = table[im.rgba8[4 * p + c]];
+        int levels = 1 + (int)std::floor(std::log2((double)std::max(w, h)));
+        mip.push_back(std::move(l0));
+        mw.push_back(w);
+        mh.push_back(h);
+        for (int m = 1; m < levels; m++) {
+            int pw = mw[m - 1], ph = mh[m - 1];
+            int cw = pw > 1 ? pw / 2 : 1, ch = ph > 1 ? ph / 2 : 1;
+            const std::vector<float>& prev = mip[m - 1];
+            std::vector<float> cur((size_t)cw * ch * 3);
+            for (int y = 0; y < ch; y++)
+                for (int x = 0; x < cw; x++) {
+                    int x0 = x * 2, y0 = y * 2, x1 = std::min(x0 + 1, pw - 1), y1 = std::min(y0 + 1, ph - 1);
+                    for (int c = 0; c < 3; c++) {
+                        float s = 0.0f;
+                        s += prev[3 * ((size_t)y0 * pw + x0) + c];
+                        s += prev[3 * ((size_t)y0 * pw + x1) + c];
+                        s += prev[3 * ((size_t)y1 * pw + x0) + c];
+                        s += prev[3 * ((size_t)y1 * pw + x1) + c];
+                        cur[3 * ((size_t)y * cw + x) + c] = s * 0.25f;
+                    }
+                }
+            mip.push_back(std::move(cur));
+            mw.push_back(cw);
+            mh.push_back(ch);
+        }
+    }
+    int levels() const { return (int)mip.size(); }
+    static int wrap(int i, int n) { i %= n; return i < 0 ? i + n : i; }
+    V3 texel(int m, int x, int y) const {
+        const float* p = &mip[m][3 * ((size_t)y * mw[m] + x)];
+        return V3(p[0], p[1], p[2]);
+    }
+    V3 bilinear(float u, float v, int m) const {
+        int w = mw[m], h = mh[m];
+        float px = u * w - 0.5f, py = v * h - 0.5f;
+        int x0 = (int)std::floor(px), y0 = (int)std::floor(py);
+        float fx = px - x0, fy = py - y0;
+        V3 top = lerp(texel(m, wrap(x0, w), wrap(y0, h)), texel(m, wrap(x0 + 1, w), wrap(y0, h)), fx);
+        V3 bot = lerp(texel(m, wrap(x0, w), wrap(y0 + 1, h)), texel(m, wrap(x0 + 1, w), wrap(y0 + 1, h)), fx);
+        return lerp(top, bot, fy);
+    }
+    V3 trilinear(float u, float v, float lod) const {
+        int lo = (int)std::floor(lod);
+        int hi = std::min(lo + 1, levels() - 1);
+        float f = lod - lo;
+        V3 a = bilinear(u, v, lo);
+        if (f <= 0.0f) return a;
+        return lerp(a, bilinear(u, v, hi), f);
+    }
+    V3 value(float u, float v) const {
+        u *= su;
+        v *= sv;
+        u = u - std::floor(u);
+        v = v - std::floor(v);
+        return bilinear(u, 1.0f - v, 0);
+    }
+    // the footprint-aware Value (fp all zero: no footprint)
+    V3 value(float u, float v, const float fp[4]) const {
+        bool has = fp[0] != 0.0f || fp[1] != 0.0f || fp[2] != 0.0f || fp[3] != 0.0f;
+        if (!has || levels() <= 1) return value(u, v);
+        float dudx = fp[0] * su, dvdx = fp[1] * sv, dudy = fp[2] * su, dvdy = fp[3] * sv;
+        float lx = std::sqrt(dudx * dudx + dvdx * dvdx), ly = std::sqrt(dudy * dudy + dvdy * dvdy);
+        float minor = std::min(lx, ly), major = std::max(lx, ly);
+        if (major <= 0.0f) return value(u, v);
+        if (minor * 8 < major) minor = major / 8;
+        float maxd = (float)std::max(mw[0], mh[0]);
+        float lod = std::log2(std::max(minor * maxd, 1.0f));
+        lod = std::min(std::max(lod, 0.0f), (float)(levels() - 1));
+        int n = (int)std::ceil(major / minor);
+        n = std::min(std::max(n, 1), 8);
+        float mu = lx >= ly ? dudx : dudy, mv = lx >= ly ? dvdx : dvdy;
+        float uc = u * su, vc = v * sv;
+        V3 acc(0.0f);
+        float inv = 1.0f / n;
+        for (int i = 0; i < n; i++) {
+            float t = (i + 0.5f) * inv - 0.5f;
+            float uu = uc + t * mu, vv = vc + t * mv;
+            uu = uu - std::floor(uu);
+            vv = vv - std::floor(vv);
+            acc += trilinear(uu, 1.0f - vv, lod);
+        }
+        return acc * inv;
+    }
+};
+
+static bool plane_hit_point(V3 o, V3 d, V3 P, V3 N, V3& hit) {
+    float den = dot(d, N);
+    if (std::fabs(den) < 1e-12f) return false;
+    float t = dot(P - o, N) / den;
+    hit = o + t * d;
+    return std::isfinite(t);
+}
+
+// FootprintMath.Compute: the UV partials of ray r's differentials at the
+// hit (point P, normal N) against dpdu/dpdv; zero when none.
+static void footprint(const Ray& r, V3 P, V3 N, V3 dpdu, V3 dpdv, float out[4]) {
+    out[0] = out[1] = out[2] = out[3] = 0.0f;
+    V3 px, py;
+    if (!plane_hit_point(r.dox, r.ddx, P, N, px) || !plane_hit_point(r.dox, r.ddy, P, N, py)) return;
+    V3 dx = px - P, dy = py - P;
+    float ax = std::fabs(N.x), ay = std::fabs(N.y), az = std::fabs(N.z);
+    int i0, i1;
+    if (ax > ay && ax > az) { i0 = 1; i1 = 2; }
+    else if (ay > az) { i0 = 0; i1 = 2; }
+    else { i0 = 0; i1 = 1; }
+    float a00 = dpdu[i0], a01 = dpdv[i0], a10 = dpdu[i1], a11 = dpdv[i1];
+    float det = a00 * a11 - a01 * a10;
+    if (std::fabs(det) < 1e-20f) return;
+    float inv = 1.0f / det;
+    out[0] = (a11 * dx[i0] - a01 * dx[i1]) * inv;
+    out[1] = (-a10 * dx[i0] + a00 * dx[i1]) * inv;
+    out[2] = (a11 * dy[i0] - a01 * dy[i1]) * inv;
+    out[3] = (-a10 * dy[i0] + a00 * dy[i1]) * inv;
+}
+
+// ═════════════════════════════════════════════════════════════════════════
 // Transparent shadow rays (ShadowRay.cs:59-158)
 // ═════════════════════════════════════════════════════════════════════════
 static thread_local bool t_block_spec_trans = false;
@@ -2000,15 +2148,28 @@ struct Camera {
         llc = origin - horiz / 2.0f - vert / 2.0f - c.focus_dist * w;
         lens_radius = c.aperture / 2.0f;
     }
-    // GetRayWithDifferentials without motion (Camera.cs:127-153); the
-    // differentials only feed texture filtering and are not carried.
-    Ray get_ray(float s, float t) const {
+    // GetRayWithDifferentials without motion (Camera.cs:127-153) with
+    // pixel deltas dsdx, dtdy; GetRay (Camera.cs:79-105) when texture
+    // filtering is off (diff = false): no differentials, and the lens ray
+    // subtracts the origin and the lens offset in turn.
+    Ray get_ray(float s, float t, float dsdx = 0.0f, float dtdy = 0.0f, bool diff = true) const {
         V3 target = llc + s * horiz + t * vert;
-        if (lens_radius == 0.0f) return Ray(origin, target - origin);
-        V3 rd = lens_radius * random_in_unit_disk();
-        V3 off = u * rd.x + v * rd.y;
-        V3 lo = origin + off;
-        return Ray(lo, target - lo);
+        Ray r;
+        V3 lo = origin;
+        if (lens_radius == 0.0f) r = Ray(origin, target - origin);
+        else {
+            V3 rd = lens_radius * random_in_unit_disk();
+            V3 off = u * rd.x + v * rd.y;
+            lo = origin + off;
+            r = diff ? Ray(lo, target - lo) : Ray(lo, target - origin - off);
+        }
+        if (diff) {
+            r.has_diff = true;
+            r.dox = lo;
+            r.ddx = llc + (s + dsdx) * horiz + t * vert - lo;
+            r.ddy = llc + s * horiz + (t - dtdy) * vert - lo;
+        }
+        return r;
     }
 };
 
@@ -2062,6 +2223,7 @@ struct Scene {
         return ((w[0] * pl[idx[0]] + w[1] * pl[idx[1]]) + w[2] * pl[idx[2]]) + w[3] * pl[idx[3]];
     }
     std::vector<uint8_t> tri_flag;   // per primitive: TRIANGLE
+    std::vector<std::unique_ptr<ImageTex>> images;
     bool textured = false;
     // Texture lookup of a textured hit: filtered parameters and footprint.
     struct TexHit {
@@ -2698,6 +2860,20 @@ struct Renderer {
         std::unique_ptr<Disney> tex_mat;
         Scene::TexHit th;
         const Scene::TexHit* thp = nullptr;
+        std::unique_ptr<Material> img_mat;
+        if (rec.mat && rec.mat->image >= 0) {
+            // ITexture.Value(in HitRecord): HitRecord U/V and the footprint
+            float u, v;
+            bool tri = rec.prim >= 0 && rec.prim < (int)sc.tri_flag.size() && sc.tri_flag[rec.prim];
+            sc.tex_coord(rec, tri, u, v);
+            V3 c = sc.images[rec.mat->image]->value(u, v, rec.fp);
+            rt_material m = rec.mat->image_raw;
+            m.color[0] = c.x;
+            m.color[1] = c.y;
+            m.color[2] = c.z;
+            img_mat.reset(make_material(m));
+            rec.mat = img_mat.get();
+        }
         if (rec.mat && rec.mat->texture_raw()) {
             th = sc.tex_hit(*rec.mat->texture_raw(), rec);
             thp = &th;
@@ -2879,12 +3055,23 @@ struct Renderer {
             rec = HitRecord();
             hit = sc.world->hit(cur, kEpsilon, kInfinity, rec);
             if (!hit || cat != kCatCamera || !rec.camera_invisible) break;
-            cur = Ray(offset_origin(rec.point, cur.d), cur.d);
+            Ray next(offset_origin(rec.point, cur.d), cur.d);
+            next.has_diff = cur.has_diff;
+            next.dox = cur.dox;
+            next.ddx = cur.ddx;
+            next.ddy = cur.ddy;
+            cur = next;
             if (++skips >= 8) break;
         }
         if (capture_aovs && hit && cat == kCatCamera && probe && probe->pending && !probe->have_depth) {
             probe->depth = rec.t * length(cur.d);
             probe->have_depth = true;
+        }
+        // analytic filter footprint of a ray with differentials (Renderer.cs:1665-1677)
+        if (hit && cur.has_diff) {
+            V3 du = length_sq(rec.dpdu) > 0.0f ? rec.dpdu : rec.tangent;
+            V3 dv = length_sq(rec.dpdv) > 0.0f ? rec.dpdv : rec.bitangent;
+            footprint(cur, rec.point, rec.normal, du, dv, rec.fp);
         }
         if (!sc.medium) {
             V3 r = !hit ? sample_sky() : shade_surface(cur, rec, depth, prev_pdf, prev_delta, absorption, beta, iors, cst);
@@ -2953,9 +3140,28 @@ static Scene* build_scene(const rt_scene_desc* d) {
         return nullptr;
     }
     auto sc = std::make_unique<Scene>();
+    if (d->num_images < 0 || (d->num_images > 0 && !d->images)) { g_err = "bad image table"; return nullptr; }
+    for (int64_t k = 0; k < d->num_images; k++) {
+        const rt_image& im = d->images[k];
+        if (!im.rgba8 || im.width <= 0 || im.height <= 0) { g_err = "bad image"; return nullptr; }
+        sc->images.emplace_back(new ImageTex(im));
+    }
+    bool any_image = false;
     for (int64_t i = 0; i < d->num_materials; i++) {
-        Material* m = make_material(d->materials[i]);
+        const rt_material& rm = d->materials[i];
+        Material* m = make_material(rm);
         if (!m) { g_err = "bad material kind"; return nullptr; }
+        if (rm.flags & RT_MATF_IMAGE) {
+            if ((rm.kind != RT_MAT_LAMBERTIAN && rm.kind != RT_MAT_DISNEY) || rm.image < 0 || rm.image >= d->num_images ||
+                (rm.kind == RT_MAT_DISNEY && (rm.spec_trans != 0.0f || (rm.flags & RT_MATF_TEXTURED)))) {
+                delete m;
+                g_err = "unsupported image-textured material";
+                return nullptr;
+            }
+            m->image = rm.image;
+            m->image_raw = rm;
+            any_image = true;
+        }
         sc->materials.emplace_back(m);
     }
     int64_t np = d->num_primitives;
@@ -3029,6 +3235,7 @@ static Scene* build_scene(const rt_scene_desc* d) {
         sc->tex.assign(d->texels, d->texels + (size_t)5 * d->tex_width * d->tex_height);
         if (d->texcoords) sc->tuv.assign(d->texcoords, d->texcoords + 6 * (size_t)np);
     }
+    if (any_image && d->texcoords && sc->tuv.empty()) sc->tuv.assign(d->texcoords, d->texcoords + 6 * (size_t)np);
     for (Hittable* o : sc->objects)
         if (Mesh* me = dynamic_cast<Mesh*>(o)) me->finish();
     // World assembly (SceneLoader.cs:460-487)
@@ -3137,6 +3344,32 @@ void orc_sobol_batch(const uint32_t* index, const uint32_t* dim, const uint32_t*
 }
 
 void* orc_scene_create(const rt_scene_desc* d) { return build_scene(d); }
+
+// ImageTexture probes: the mip chain (levels concatenated, RGB; returns the
+// level count, sizes in wh[2 * level]) and n lookups at (u, v) with
+// footprints fp (4 per lookup, zero = none).
+int orc_image_levels(const rt_image* im, float* out, int64_t cap, int32_t* wh) {
+    ImageTex t(*im);
+    int64_t off = 0;
+    for (int m = 0; m < t.levels(); m++) {
+        wh[2 * m] = t.mw[m];
+        wh[2 * m + 1] = t.mh[m];
+        for (float x : t.mip[m]) {
+            if (off < cap) out[off] = x;
+            off++;
+        }
+    }
+    return t.levels();
+}
+void orc_image_sample(const rt_image* im, int64_t n, const float* uv, const float* fp, float* out) {
+    ImageTex t(*im);
+    for (int64_t i = 0; i < n; i++) {
+        V3 c = t.value(uv[2 * i], uv[2 * i + 1], fp + 4 * i);
+        out[3 * i] = c.x;
+        out[3 * i + 1] = c.y;
+        out[3 * i + 2] = c.z;
+    }
+}
 void orc_scene_destroy(void* s) { delete (Scene*)s; }
 
 int orc_scene_info(void* sp, rt_scene_info* info) {
@@ -3274,7 +3507,7 @@ int orc_trace_primary(void* sp, const rt_settings* st, int32_t sbegin, int32_t s
                 float jv = random_float();
                 float uu = ((float)i + ju) / (float)W;
                 float vv = ((float)(H - j - 1) + jv) / (float)H;
-                Ray ray = s->camera.get_ray(uu, vv);
+                Ray ray = s->camera.get_ray(uu, vv, 0.0f, 0.0f, st->texture_filtering != 2);
                 HitRecord rec;
                 bool hit;
                 int skips = 0;
@@ -3347,6 +3580,9 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
             if (tile >= total) break;
             int x0 = (tile % tx) * T, y0 = (tile / tx) * T;
             int x1 = std::min(x0 + T, W), y1 = std::min(y0 + T, H);
+            // pixel deltas of the differential rays, shrunk by ceil(sqrt(spp)) (Renderer.cs:556,616-624)
+            const float pd_shrink = 1.0f / (float)(int)std::ceil(std::sqrt((float)spp));
+            const float pd_s = pd_shrink / (float)W, pd_t = pd_shrink / (float)H;
             for (int j = y0; j < y1; j++) {
                 if (j % row_step != 0) continue;
                 for (int i = x0; i < x1; i++) {
@@ -3361,7 +3597,7 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
                         float jv = random_float();
                         float uu = ((float)i + ju) / (float)W;
                         float vv = ((float)(H - j - 1) + jv) / (float)H;
-                        Ray ray = s->camera.get_ray(uu, vv);
+                        Ray ray = s->camera.get_ray(uu, vv, pd_s, pd_t, st->texture_filtering != 2);
                         IorStack iors;
                         probe = AovProbe();
                         V3 L = RR.trace_ray(ray, RR.max_depth, 0.0f, true, iors, V3(0.0f), V3(1.0f), kCatCamera);
@@ -3492,6 +3728,8 @@ int orc_adjoint(void* sp, const rt_settings* st, const float* d_beauty, double* 
         A.grad = grad.data();
         A.plane = plane;
         Renderer RR(*s, *st);
+        const float pd_shrink = 1.0f / (float)(int)std::ceil(std::sqrt((float)spp));
+        const float pd_s = pd_shrink / (float)W, pd_t = pd_shrink / (float)H;
         for (;;) {
             int j = next.fetch_add(1);
             if (j >= H) break;
@@ -3507,7 +3745,7 @@ int orc_adjoint(void* sp, const rt_settings* st, const float* d_beauty, double* 
                     float jv = random_float();
                     float uu = ((float)i + ju) / (float)W;
                     float vv = ((float)(H - j - 1) + jv) / (float)H;
-                    Ray ray = s->camera.get_ray(uu, vv);
+                    Ray ray = s->camera.get_ray(uu, vv, pd_s, pd_t, st->texture_filtering != 2);
                     IorStack iors;
                     for (int k = 0; k < 3; k++) A.aw[k] = db[k] / (double)spp;
                     t_adj = &A;

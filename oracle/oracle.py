@@ -357,6 +357,47 @@ def caustic_estimate(pmap, points, normals, views, albedo, max_r):
 
 # ── denoiser (denoise.cpp) ────────────────────────────────────────────────
 
+def _image_rec(rgba8, scale=(1.0, 1.0)):
+    px = np.ascontiguousarray(rgba8, np.uint8)
+    r = abi.rt_image()
+    r.rgba8 = C.cast(px.ctypes.data, C.POINTER(C.c_uint8))
+    r.height, r.width = px.shape[0], px.shape[1]
+    r.scale_u, r.scale_v = scale
+    return r, px
+
+
+def image_levels(rgba8):
+    """ImageTexture's mip chain: a list of (h, w, 3) float32 levels."""
+    r, px = _image_rec(rgba8)
+    h, w = px.shape[:2]
+    total = int(sum(max(w >> m, 1) * max(h >> m, 1) for m in range(32))) * 3
+    out = np.zeros(total, np.float32)
+    wh = np.zeros(64, np.int32)
+    L = lib()
+    L.orc_image_levels.restype = C.c_int
+    L.orc_image_levels.argtypes = [C.POINTER(abi.rt_image), C.c_void_p, C.c_int64, C.c_void_p]
+    n = L.orc_image_levels(C.byref(r), out.ctypes.data, out.size, wh.ctypes.data)
+    levels, off = [], 0
+    for m in range(n):
+        lw, lh = int(wh[2 * m]), int(wh[2 * m + 1])
+        levels.append(out[off:off + lw * lh * 3].reshape(lh, lw, 3))
+        off += lw * lh * 3
+    return levels
+
+
+def image_sample(rgba8, uv, fp=None, scale=(1.0, 1.0)):
+    """ImageTexture.Value at uv (n, 2) with footprints fp (n, 4) (None: no footprint)."""
+    r, px = _image_rec(rgba8, scale)
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    fp = np.zeros((len(uv), 4), np.float32) if fp is None else np.ascontiguousarray(fp, np.float32).reshape(-1, 4)
+    out = np.zeros((len(uv), 3), np.float32)
+    L = lib()
+    L.orc_image_sample.restype = None
+    L.orc_image_sample.argtypes = [C.POINTER(abi.rt_image), C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_image_sample(C.byref(r), len(uv), uv.ctypes.data, fp.ctypes.data, out.ctypes.data)
+    return out
+
+
 def _dn_lib():
     L = lib()
     if not getattr(L, "_dn_typed", False):

@@ -119,6 +119,10 @@ inline float shadow_terminator(V3 ng, V3 ns, V3 w) {
 // ── Ray (Ray.cs:39-57) ───────────────────────────────────────────────────
 struct Ray {
     V3 o, d, inv;
+    // RayDifferential of a camera ray (RayDifferential.cs:28-43): both
+    // auxiliary rays start at dox
+    bool has_diff = false;
+    V3 dox, ddx, ddy;
     Ray() {}
     Ray(V3 origin, V3 dir) : o(origin), d(dir), inv(V3(1.0f) / dir) {}
     V3 at(float t) const { return o + t * d; }
