@@ -352,17 +352,6 @@ __device__ __forceinline__ float2 tex_coord(const DevScene& S, int32_t he, f3 o,
     return make_float2(u, v);
 }
 
-// Material record of a textured Disney hit: the material's knobs with base
-// colour, roughness and metallic from the texture set.
-__device__ __forceinline__ GMat textured_gmat(const DevScene& S, const GMat& g, int mat, float2 uv, float tp[kTexChannels]) {
-    const TexFoot foot = tex_footprint(S.tex_w, S.tex_h, uv.x, uv.y);
-#pragma unroll
-    for (int c = 0; c < kTexChannels; c++) tp[c] = tex_value(S.tex, S.tex_w, S.tex_h, foot, c);
-    GMat t = disney_gmat(S.raw[mat], mk(tp[0], tp[1], tp[2]), tp[4], tp[3], S.lut_e + 32 * 32);
-    t.m0.z = g.m0.z;
-    return t;
-}
-
 // ── ImageTexture (ImageTexture.cs:148-289) ─────────────────────────────
 __device__ __forceinline__ int img_wrap(int i, int n) {
     i %= n;
@@ -429,17 +418,32 @@ __device__ __forceinline__ f3 img_value(const DevScene& S, int img, float u, flo
     return acc * inv;
 }
 
-// Material record of an image-textured hit: Lambertian albedo or Disney
-// baseColor from the texture (SceneLoader.cs:1027-1031).
-__device__ __forceinline__ GMat image_gmat(const DevScene& S, const GMat& g, int mat, float2 uv, float4 fp) {
+
+// Material record of a textured hit, one disney_gmat site for both
+// sources: the texture set (Disney base colour, roughness and metallic) or
+// an ImageTexture (Lambertian albedo or Disney baseColor,
+// SceneLoader.cs:1027-1031; fp = the hit's footprint).
+__device__ __forceinline__ GMat textured_gmat(const DevScene& S, const GMat& g, int mat, float2 uv,
+                                              float tp[kTexChannels], bool image, float4 fp) {
     const rt_material& m = S.raw[mat];
-    const f3 c = img_value(S, m.image, uv.x, uv.y, fp);
-    if (g.m0.x == RT_MAT_LAMBERTIAN) {
-        GMat t = g;
-        t.m1 = make_float4(c.x, c.y, c.z, g.m1.w);
-        return t;
+    f3 c;
+    float metallic = m.metallic, roughness = m.roughness;
+    if (image) {
+        c = img_value(S, m.image, uv.x, uv.y, fp);
+        if (g.m0.x == RT_MAT_LAMBERTIAN) {
+            GMat t = g;
+            t.m1 = make_float4(c.x, c.y, c.z, g.m1.w);
+            return t;
+        }
+    } else {
+        const TexFoot foot = tex_footprint(S.tex_w, S.tex_h, uv.x, uv.y);
+#pragma unroll
+        for (int k = 0; k < kTexChannels; k++) tp[k] = tex_value(S.tex, S.tex_w, S.tex_h, foot, k);
+        c = mk(tp[0], tp[1], tp[2]);
+        metallic = tp[4];
+        roughness = tp[3];
     }
-    GMat t = disney_gmat(m, c, m.metallic, m.roughness, S.lut_e + 32 * 32);
+    GMat t = disney_gmat(m, c, metallic, roughness, S.lut_e + 32 * 32);
     t.m0.z = g.m0.z;
     return t;
 }
@@ -1289,13 +1293,12 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             float tp[kTexChannels];
             float2 tuv = make_float2(0.0f, 0.0f);
             const bool textured = (g.m0.y & kMatTextured) != 0;
-            if (textured) {
+            const bool image = (g.m0.y & kMatImage) != 0;
+            if (textured || image) {
                 tuv = tex_coord(S, he, o, d, htuv);
-                g = textured_gmat(S, g, s.mat, tuv, tp);
-            } else if (g.m0.y & kMatImage) {
                 // camera hits carry the footprint k_fp left; later hits are point-sampled
-                const float4 fpv = P.fp ? P.fp[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                g = image_gmat(S, g, s.mat, tex_coord(S, he, o, d, htuv), fpv);
+                const float4 fpv = image && P.fp ? P.fp[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                g = textured_gmat(S, g, s.mat, tuv, tp, image, fpv);
             }
             int kind = g.m0.x;
             int mflags = g.m0.y;
