@@ -1,8 +1,9 @@
 """Scenes loaded from YAML (raytracer/scene_loader.py) through the GPU
 renderer: primary hits bit-exact against the oracle on the reference's own
-caustic-lens test scene, the plain-method Cornell config and the OBJ mesh
-scene (the reference's model files, smooth and flat), and the radiance
-against the oracle on the same sample stream."""
+caustic-lens test scene, the plain-method Cornell config, the OBJ mesh
+scene (the reference's model files, smooth and flat) and the image-texture
+scene (the reference's texture files, footprint-filtered camera hits), and
+the radiance against the oracle on the same sample stream."""
 import os
 
 import numpy as np
@@ -15,7 +16,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
 
 
-@pytest.mark.parametrize("name", ["caustic-lens.yaml", "cornell-c1.yaml", "mesh-trio.yaml"])
+@pytest.mark.parametrize("name", ["caustic-lens.yaml", "cornell-c1.yaml", "mesh-trio.yaml", "textured.yaml"])
 def test_yaml_scene_primary_hits_and_radiance(gpu_ctx, oracle, name):
     W, H, spp = 96, 72, 8
     desc = scene_loader.load(os.path.join(HERE, name), W, H)

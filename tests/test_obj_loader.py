@@ -170,6 +170,15 @@ def test_mesh_entities_in_the_scene_front_end():
     assert np.allclose(nn, tor.normals.reshape(-1, 3).astype(np.float64) @ R.T, atol=1e-6)
 
 
+def test_textured_scene_loads():
+    w = []
+    d = scene_loader.load(os.path.join(HERE, "scenes", "textured.yaml"), 64, 48, warnings=w)
+    assert w == [] and len(d.images) == 3 and d.images[0][1] == (4.0, 4.0) and d.images[2][1] == (2.0, 1.0)
+    kinds = d.mats["kind"][d.prims["material"]]
+    assert (d.mats["flags"][d.prims["material"]][:962] & abi.RT_MATF_IMAGE).all()
+    assert kinds[1] == abi.RT_MAT_DISNEY and d.normals is not None and d.texcoords is not None
+
+
 def test_mesh_entity_skips_and_refusals(tmp_path):
     cube = os.path.join(MODELS, "test-cube.obj")
     base = {"lights": [], "world": {"sky": {"color": [0, 0, 0]}},
