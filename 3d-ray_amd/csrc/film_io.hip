@@ -453,7 +453,11 @@ int rt_png_encode(rt_ctx* ctx, const float* display, int32_t width, int32_t heig
     constexpr int64_t kSeg = 1 << 20;
     const int64_t nseg = (raw_n + kSeg - 1) / kSeg;
     const int64_t zbound = 2 + 4 + nseg * ((int64_t)compressBound((uLong)kSeg) + 16);
-    const int64_t bound = 8 + 25 + 12 + zbound + 12;
+    // the zlib stream goes out in IDAT chunks of at most kIdat bytes (a PNG
+    // chunk length is below 2^31)
+    constexpr int64_t kIdat = 1 << 30;
+    const int64_t nidat = (zbound + kIdat - 1) / kIdat;
+    const int64_t bound = 8 + 25 + 12 * nidat + zbound + 12;
     if (!out) {
         *size = bound;
         return RT_OK;
@@ -512,18 +516,18 @@ int rt_png_encode(rt_ctx* ctx, const float* display, int32_t width, int32_t heig
     idat.u8(0x78);   // zlib header: deflate, 32K window, default level
     idat.u8(0x9C);
     for (auto& s : seg) idat.raw(s.data(), s.size());
-    idat.be32((uint32_t)adler32(adler32(0L, Z_NULL, 0), raw.data(), (uInt)raw_n));
+    idat.be32((uint32_t)adler32_z(adler32(0L, Z_NULL, 0), raw.data(), (z_size_t)raw_n));
 
     Bytes f;
     const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
     f.raw(sig, 8);
-    auto chunk = [&](const char* type, const std::vector<uint8_t>& data) {
-        f.be32((uint32_t)data.size());
+    auto chunk = [&](const char* type, const uint8_t* data, size_t n) {
+        f.be32((uint32_t)n);
         const size_t at = f.v.size();
         f.raw(type, 4);
-        f.raw(data.data(), data.size());
+        if (n) f.raw(data, n);
         uLong crc = crc32(0L, Z_NULL, 0);
-        crc = crc32(crc, f.v.data() + at, (uInt)(4 + data.size()));
+        crc = crc32_z(crc, f.v.data() + at, (z_size_t)(4 + n));
         f.be32((uint32_t)crc);
     };
     Bytes ihdr;
@@ -534,9 +538,10 @@ int rt_png_encode(rt_ctx* ctx, const float* display, int32_t width, int32_t heig
     ihdr.u8(0);
     ihdr.u8(0);
     ihdr.u8(0);
-    chunk("IHDR", ihdr.v);
-    chunk("IDAT", idat.v);
-    chunk("IEND", {});
+    chunk("IHDR", ihdr.v.data(), ihdr.v.size());
+    for (size_t at = 0; at < idat.v.size(); at += (size_t)kIdat)
+        chunk("IDAT", idat.v.data() + at, std::min<size_t>((size_t)kIdat, idat.v.size() - at));
+    chunk("IEND", nullptr, 0);
     if ((int64_t)f.v.size() > capacity) return ctx_fail(ctx, RT_ERR_INTERNAL, "PNG larger than its bound");
     memcpy(out, f.v.data(), f.v.size());
     *size = (int64_t)f.v.size();

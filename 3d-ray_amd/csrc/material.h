@@ -98,6 +98,7 @@ RT_HD GMat disney_gmat(const rt_material& m, f3 c, float metallic_in, float roug
     g.m7 = make_float4(clearcoat, cc_alpha, cc_f0, rot);
     g.m8 = make_float4(ttint.x, ttint.y, ttint.z, alpha_iso);
     g.m9 = make_float4(sigma.x, sigma.y, sigma.z, eavg);
+    g.m10 = make_float4(mx(m.thin_film_thickness, 0.0f), mx(m.thin_film_ior, 1.0001f), 0.0f, 0.0f);
     // raw spec_trans for ShadowTransmittance (DisneyBsdf.cs:570-584)
     g.m0 = make_int4(RT_MAT_DISNEY, flags, -1, __builtin_bit_cast(int32_t, m.spec_trans));
     return g;

@@ -53,6 +53,8 @@ struct DevScene {
     int num_lights;
     int medium;
     int opaque_shadows;
+    int env_light;              // first EnvironmentLight of the list, -1 without one
+    f3 sky;                     // flat sky radiance (FlatSky.EvaluateRadiance)
     int caustics;               // photon map active: gather at diffuse/glossy hits, suppress L S+ re-hits
     int block_spec;             // ShadowRay.BlockSpecularTransmission
     f3 cam_o, cam_llc, cam_h, cam_v, cam_u, cam_vv;
@@ -185,7 +187,15 @@ constexpr int kCntSidBase = 44;   // first window sample of this iteration's res
 // kCntPend..+2 (written by k_batch_end for the host): paths pending in the next
 // iteration, the stack-overflow flag, and whether the window's samples are all issued
 constexpr int kIterBatch = 4;   // bounce iterations enqueued between host checks
-enum : int32_t { kSlotIdle = 0, kSlotActive = 1, kSlotRetired = 2 };
+// kSlotBounce: shaded by the NEE phase of a split shade, waiting for its
+// BSDF bounce (k_shade<..., kPhBounce>) in the same iteration
+enum : int32_t { kSlotIdle = 0, kSlotActive = 1, kSlotRetired = 2, kSlotBounce = 3 };
+// Shading phases: the whole vertex in one kernel (kept for the adjoint's
+// replay pass), or split in two launches — medium, emission, NEE and the
+// caustic gather request first, then the BSDF sample, roulette and next ray
+// of surface vertices — so that neither kernel holds both the light-sample
+// and the BSDF-sample state.
+enum : int { kPhAll = 0, kPhNee = 1, kPhBounce = 2 };
 
 // Shadow queue view: segment j holds the shadow rays of light sample j
 // (in light-list order) at q_sh[j*stride ...]; pre[] is the prefix of the
@@ -511,10 +521,14 @@ __device__ __forceinline__ float sphere_cone_pdf(f3 c, float r, f3 hp, f3 wi) {
     return sa > kEps ? 1.0f / sa : 0.0f;
 }
 
+// FlatSky.Pdf: uniform over the sphere (FlatSky.cs:31-35)
+constexpr float kUniformSpherePdf = 1.0f / (4.0f * kPi);
+
 // ILight.PdfSolidAngle for every supported light
 __device__ float light_pdf(const GLight& L, f3 hp, f3 wi) {
     int kind = L.l0.x;
     if (kind == RT_LIGHT_SPHERE) return sphere_cone_pdf(f3of(L.l2), L.l2.w, hp, wi);
+    if (kind == RT_LIGHT_ENVIRONMENT) return L.l0.w ? kUniformSpherePdf : 0.0f;
     if (kind == RT_LIGHT_AREA) {
         f3 n = f3of(L.l5), corner = f3of(L.l2), u = f3of(L.l3), v = f3of(L.l4);
         float denom = dot(n, wi);
@@ -585,6 +599,22 @@ __device__ LSample light_sample(const GLight& L, f3 hp, f3 sn, int si, Rng& rng)
     o.tmax = 0.0f;
     o.dist = 0.0f;
     int kind = L.l0.x, ss = L.l0.y, sq = L.l0.z;
+    if (kind == RT_LIGHT_ENVIRONMENT) {
+        // EnvironmentLight.IlluminateAndTest (EnvironmentLight.cs:66-104) over a
+        // flat sky: a uniform direction (FlatSky.ImportanceSample), below the
+        // horizon of a surface point discarded, an unbounded shadow ray, the
+        // colour L / (pdf N); a black sky cannot sample and draws nothing
+        if (L.l0.w == 0) return o;
+        const f3 dir = random_unit_vector(rng);
+        if (length_sq(sn) > 0.0f && dot(sn, dir) <= 0.0f) return o;
+        o.so = offset_origin(hp, sn);
+        o.tmax = kInfinity;
+        o.color = f3of(L.l1) / (kUniformSpherePdf * (float)ss);
+        o.dir = dir;
+        o.dist = kInfinity;
+        o.skip = false;
+        return o;
+    }
     if (kind == RT_LIGHT_SPHERE) {
         f3 c = f3of(L.l2);
         float r = L.l2.w;
@@ -1083,21 +1113,31 @@ __device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, i
 // dF/dp / F (per channel) pending until the next shade, where the radiance
 // gathered after this vertex, D = (primal a0 + ai) - (a0 + ai so far), is
 // known (path replay backpropagation, Vicini, Speierer and Jakob 2021).
-template <class JOBS, int WPE, bool ADJ = false>
+//
+// PH = kPhNee leaves surface vertices of scattering materials in state
+// kSlotBounce with the vertex's throughput (medium weight applied), radiance
+// weight (segment weight applied), sampler dimension and pending-shadow
+// flags stored; PH = kPhBounce picks them up.  Sampler dimensions are drawn
+// in the same order either way.
+template <class JOBS, int WPE, bool ADJ = false, int PH = kPhAll>
 __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, int maxs) {
+    static_assert(!ADJ || PH == kPhAll, "the adjoint replay shades whole vertices");
+    constexpr bool kFront = PH != kPhBounce;    // medium, emission, NEE, gather
+    constexpr bool kBack = PH != kPhNee;        // BSDF bounce
     const int p = xcd_block() * kBlock + threadIdx.x;
     const int st_in = p < P.n ? P.state[p] : kSlotIdle;
-    const bool active = st_in == kSlotActive;
+    const bool active = st_in == (PH == kPhBounce ? kSlotBounce : kSlotActive);
     const int job = active ? jobs.job_of(P.job, p) : 0;
     int n_shadow = 0;
     uint32_t smask = 0;   // bit j: light sample j produced a shadow ray
     bool cont = false;
+    bool bounce = false;  // kPhNee: the vertex waits for its BSDF bounce
     bool gather = false;  // a caustic gather request was queued
     // per-lane shadow records are written straight into the path's slots
     jobs.run_lane(active, job, [&](const DevScene& S, const RParams& R) {
         float4 o4 = P.o[p], d4 = P.d[p], thr4 = P.thr[p], wt4 = P.wt[p];
         int32_t flags = __float_as_int(d4.w);
-        flags = fold_shadows(P, R, p, flags);
+        if constexpr (kFront) flags = fold_shadows(P, R, p, flags);
         bool adj_pend = false;   // this vertex leaves a bounce gradient pending
         float adj_qv[16];
         float2 adj_uv = make_float2(0.0f, 0.0f);
@@ -1147,7 +1187,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         f3 mb = splat(1.0f);
         bool scat = false;
         float tmed = 0.0f;
-        if (S.medium) {
+        if (kFront && S.medium) {
             float tmax = hit ? htuv.x : 1e30f;
             int ch = (int)(rng.next() * 3.0f);
             if (ch > 2) ch = 2;
@@ -1166,7 +1206,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         }
         // ── Beer-Lambert on the segment (ApplyBeerLambert) ──────────────
         f3 bl = splat(1.0f);
-        if (sigma_abs.x > 0.0f || sigma_abs.y > 0.0f || sigma_abs.z > 0.0f) {
+        if (kFront && (sigma_abs.x > 0.0f || sigma_abs.y > 0.0f || sigma_abs.z > 0.0f)) {
             if (!scat && !hit) bl = splat(0.0f);
             else {
                 float tseg = scat ? tmed : htuv.x;
@@ -1177,8 +1217,10 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             }
         }
         f3 seg = bl * mb;
-        if (primary) P.w0[p] = make_float4(seg.x, seg.y, seg.z, 0.0f);
-        else wt = wt * seg;
+        if constexpr (kFront) {
+            if (primary) P.w0[p] = make_float4(seg.x, seg.y, seg.z, 0.0f);
+            else wt = wt * seg;
+        }
 
         int aovf = 0;
         float4 av0, av1, av2;
@@ -1207,15 +1249,23 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         int next_cat = 0;
         f3 thr_next = thr, wt_next = wt;
         bool next_tgt = tgt;
+        if constexpr (PH == kPhBounce) {
+            // the light samples the NEE phase left for the BSDF evaluation
+            const int32_t m = P.smask[p];
+            smask = grouped ? (uint32_t)m : 0u;
+            n_shadow = grouped ? __popc((uint32_t)m) : m;
+        }
 
-        if (scat) {
-            // ── medium scattering event (Renderer.cs:1703-1770) ─────────
-            f3 ps3 = o + d * tmed;
-            if (R.aovs && (aovf & kAovPending)) {
-                if (!(aovf & kAovHaveDepth)) { av1.w = tmed * length(d); aovf |= kAovHaveDepth; }
-                aov_commit(splat(1.0f), splat(0.0f));
-            }
-            f3 seg_w = primary ? splat(1.0f) : wt;
+        // ── NEE light sampling (ComputeDirectLighting / ComputeDirectLightingMedium,
+        // Renderer.cs:1894-2036, 2099-2206): the light pick (one dimension when a
+        // strategy picks a light), then every shadow sample of the chosen
+        // lights in list order with each light's own draws.  A medium vertex
+        // weighs each sample with the phase function at once; a surface vertex
+        // (records = true) leaves a candidate record per sample for the BSDF
+        // evaluation: direction and light-pdf term (-1: delta light), light
+        // colour and pick probability (-1: all lights), and the medium
+        // transmittance in sh_r.
+        auto sample_lights = [&](f3 pt, f3 nrm, bool medium_vtx, f3 w_vtx, bool records) {
             int first_l = 0, last_l = S.num_lights;
             float ppick = -1.0f;
             if (R.picked && S.num_lights > 0) {
@@ -1235,27 +1285,44 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             for (int li = first_l; li < last_l; li++) {
                 GLight L = S.lights[li];
                 for (int si = 0; si < L.l0.y; si++, jn++) {
-                    LSample ls = light_sample(L, ps3, splat(0.0f), si, rng);
-                    if (ls.skip) continue;
-                    f3 tr = medium_tr(S, ls.dist);
-                    float wn = 1.0f;
+                    LSample ls = light_sample(L, pt, nrm, si, rng);
+                    if (ls.skip || !records) continue;
+                    const f3 tr = S.medium ? medium_tr(S, ls.dist) : splat(1.0f);
+                    float pn = -1.0f;
                     if (!ls.delta) {
-                        float pl = light_pdf(L, ps3, ls.dir);
-                        float pn = ppick > 0.0f ? ppick * pl : pl;
-                        if (pn + kInv4Pi > 0.0f) wn = mis_w(pn, kInv4Pi, R.power_mis);
+                        const float pl = light_pdf(L, pt, ls.dir);
+                        pn = ppick > 0.0f ? ppick * pl : pl;
                     }
-                    f3 c = wn * ls.color * kInv4Pi * tr;
-                    if (ppick > 0.0f) c = c / ppick;
-                    c = seg_w * c;
-                    if (is_zero(c)) continue;
-                    int64_t sl = (int64_t)(grouped ? jn : n_shadow) * P.n + p;
+                    const int64_t sl = (int64_t)(grouped ? jn : n_shadow) * P.n + p;
+                    if (medium_vtx) {
+                        float wn = 1.0f;
+                        if (!ls.delta && pn + kInv4Pi > 0.0f) wn = mis_w(pn, kInv4Pi, R.power_mis);
+                        f3 c = wn * ls.color * kInv4Pi * tr;
+                        if (ppick > 0.0f) c = c / ppick;
+                        c = w_vtx * c;
+                        if (is_zero(c)) continue;
+                        P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
+                        P.sh_c[sl] = make_float4(c.x, c.y, c.z, 0.0f);
+                    } else {
+                        P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, pn);
+                        P.sh_c[sl] = make_float4(ls.color.x, ls.color.y, ls.color.z, ppick);
+                        if (S.medium) P.sh_r[sl] = make_float4(tr.x, tr.y, tr.z, 0.0f);
+                    }
                     P.sh_o[sl] = make_float4(ls.so.x, ls.so.y, ls.so.z, ls.tmax);
-                    P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
-                    P.sh_c[sl] = make_float4(c.x, c.y, c.z, 0.0f);
                     if (jn < kMaxSeg) smask |= 1u << jn;
                     n_shadow++;
                 }
             }
+        };
+
+        if (kFront && scat) {
+            // ── medium scattering event (Renderer.cs:1703-1770) ─────────
+            f3 ps3 = o + d * tmed;
+            if (R.aovs && (aovf & kAovPending)) {
+                if (!(aovf & kAovHaveDepth)) { av1.w = tmed * length(d); aovf |= kAovHaveDepth; }
+                aov_commit(splat(1.0f), splat(0.0f));
+            }
+            sample_lights(ps3, splat(0.0f), true, primary ? splat(1.0f) : wt, true);
             f3 mt = thr * mb;
             float iscale = 1.0f;
             bool kill = false;
@@ -1283,9 +1350,20 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                 next_cs = kCstDiffuse;
                 cont = true;
             }
-        } else if (!hit) {
-            // ── miss: flat black sky (SampleSky) ───────────────────────
-            if (R.aovs && (aovf & kAovPending)) aov_commit(clamp01(splat(0.0f)), splat(0.0f));
+        } else if (kFront && !hit) {
+            // ── miss: the flat sky (SampleSky, Renderer.cs:1820-1861), weighed
+            // against the environment light's NEE by MIS after a non-delta
+            // bounce when the light list holds one
+            const f3 sky = S.sky;
+            if (R.aovs && (aovf & kAovPending)) aov_commit(clamp01(sky), splat(0.0f));
+            if (sky.x > 0.0f || sky.y > 0.0f || sky.z > 0.0f) {
+                f3 sw = sky;
+                if (!prev_delta && S.env_light >= 0 && S.lights[S.env_light].l0.w) {
+                    const float pl = kUniformSpherePdf;
+                    sw = prev_pdf + pl <= 1e-20f ? splat(0.0f) : sky * mis_w(prev_pdf, pl, R.power_mis);
+                }
+                add_target(P, p, tgt, sanitize((primary ? splat(1.0f) : wt) * sw));
+            }
         } else {
             // ── surface (ShadeSurface, Renderer.cs:1008-1228) ──────────
             Surf s = make_surf(S, he, o, d, htuv.x, htuv.y, htuv.z);
@@ -1304,13 +1382,13 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             int mflags = g.m0.y;
             f3 thr_s = thr * mb;
             f3 vw = primary ? splat(1.0f) : wt;   // weight of this vertex's own terms
-            if (R.aovs && (aovf & kAovPending)) {
+            if (kFront && R.aovs && (aovf & kAovPending)) {
                 P.av2[p] = make_float4(s.n.x, s.n.y, s.n.z, 0.0f);
                 f3 ta = clamp01(f3of(g.m1));
                 av1 = make_float4(ta.x, ta.y, ta.z, av1.w);
             }
             // emission with MIS (WeightEmission, Renderer.cs:1549-1565)
-            if (kind == RT_MAT_EMISSIVE && s.front && !(S.caustics && cst == kCstSuppress)) {
+            if (kFront && kind == RT_MAT_EMISSIVE && s.front && !(S.caustics && cst == kCstSuppress)) {
                 f3 raw = f3of(g.m1);
                 if (raw.x > 0.0f || raw.y > 0.0f || raw.z > 0.0f) {
                     f3 em = raw;
@@ -1324,164 +1402,221 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                 }
             }
             f3 view = normalize(-d);
-            if (mflags & kMatDielectricIor) {
+            if ((kBack || S.caustics) && (mflags & kMatDielectricIor)) {
                 float top = ior_n > 0 ? P.ior[(int64_t)(ior_n - 1) * P.n + p] : 1.0f;
                 float enc = ior_n >= 2 ? P.ior[(int64_t)(ior_n - 2) * P.n + p] : 1.0f;
                 float mi = g.m3.w;
                 if (mi > 0.0f) s.rel_eta = s.front ? (top / mi) : (mi / enc);
             }
-            DisneyP DP;
-            if (kind == RT_MAT_DISNEY) DP = load_disney(g);
-            // adjoint weight of this vertex's terms: dLoss/dPixel * w0 / spp
-            DisneyD DD;
-            f3 adj_w = splat(0.0f);
-            const bool adj_on = ADJ && P.adj_pass == 2 && textured && kind == RT_MAT_DISNEY;
-            if constexpr (ADJ) {
-                if (adj_on) {
-                    DD = disney_d(S.raw[s.mat], tp, S.lut_e + 32 * 32);
-                    const int pix = P.pix[p];
-                    const f3 w0v = primary ? seg : f3of(P.w0[p]);
-                    adj_w = mk(P.adj_db[pix], P.adj_db[R.npix + pix], P.adj_db[2 * R.npix + pix]) * w0v * P.adj_inv_spp;
-                    adj_uv = tuv;
+            const bool has_brdf = kind == RT_MAT_LAMBERTIAN || kind == RT_MAT_DISNEY;
+            if constexpr (kFront) {
+                // ── NEE light samples; caustic gather request (Renderer.cs:1108-1109)
+                if (kind != RT_MAT_EMISSIVE) sample_lights(s.p, s.n, false, vw, has_brdf);
+                if (S.caustics && has_brdf) {
+                    gather = true;
+                    P.gq0[p] = make_float4(o.x, o.y, o.z, htuv.x);
+                    P.gq1[p] = make_float4(d.x, d.y, d.z, __int_as_float(he));
+                    P.gq2[p] = make_float4(htuv.y, htuv.z, s.rel_eta, __int_as_float(p));
+                    P.gq3[p] = make_float4(vw.x, vw.y, vw.z, __int_as_float(tgt ? 1 : 0));
                 }
+                // the NEE phase hands scattering vertices to the bounce phase
+                if (PH == kPhNee) bounce = has_brdf || kind == RT_MAT_DIELECTRIC;
             }
-            // ── NEE (ComputeDirectLighting, Renderer.cs:1894-2036) ──────
-            if (kind != RT_MAT_EMISSIVE) {
-                float view_term = shadow_terminator(s.n, s.n, view);
-                bool has_brdf = kind == RT_MAT_LAMBERTIAN || kind == RT_MAT_DISNEY;
-                int first_l = 0, last_l = S.num_lights;
-                float ppick = -1.0f;
-                if (R.picked && S.num_lights > 0) {
-                    float xi = rng.next();
-                    int lo = 0, hi = S.num_lights - 1;
-                    if (S.num_lights > 1)
-                        while (lo < hi) {
-                            int mid = (lo + hi) / 2;
-                            if (S.light_cdf[mid] < xi) lo = mid + 1;
-                            else hi = mid;
-                        }
-                    first_l = lo;
-                    last_l = lo + 1;
-                    ppick = S.light_pdf[lo];
+            if constexpr (kBack) {
+                DisneyP DP;
+                if (kind == RT_MAT_DISNEY) DP = load_disney(g);
+                // adjoint weight of this vertex's terms: dLoss/dPixel * w0 / spp
+                DisneyD DD;
+                f3 adj_w = splat(0.0f);
+                const bool adj_on = ADJ && P.adj_pass == 2 && textured && kind == RT_MAT_DISNEY;
+                if constexpr (ADJ) {
+                    if (adj_on) {
+                        DD = disney_d(S.raw[s.mat], tp, S.lut_e + 32 * 32);
+                        const int pix = P.pix[p];
+                        const f3 w0v = primary ? seg : f3of(P.w0[p]);
+                        adj_w = mk(P.adj_db[pix], P.adj_db[R.npix + pix], P.adj_db[2 * R.npix + pix]) * w0v * P.adj_inv_spp;
+                        adj_uv = tuv;
+                    }
                 }
-                int jn = 0;
-                for (int li = first_l; li < last_l; li++) {
-                    GLight L = S.lights[li];
-                    for (int si = 0; si < L.l0.y; si++, jn++) {
-                        LSample ls = light_sample(L, s.p, s.n, si, rng);
-                        if (ls.skip || !has_brdf) continue;
+                // ── BSDF evaluation of the light samples (EvaluateDirect, Pdf,
+                // MisWeight): c = w_nee Le brdf Tr / pPick times the vertex weight;
+                // a zero contribution drops the sample's shadow ray (the
+                // ungrouped queue keeps it with c = 0)
+                if (has_brdf && n_shadow > 0) {
+                    const float view_term = shadow_terminator(s.n, s.n, view);
+                    uint32_t left = smask;
+                    for (int k = 0;; k++) {
+                        int j;
+                        if (grouped) {
+                            if (left == 0u) break;
+                            j = __ffs(left) - 1;
+                            left &= left - 1u;
+                        } else {
+                            if (k >= n_shadow) break;
+                            j = k;
+                        }
+                        const int64_t sl = (int64_t)j * P.n + p;
+                        const float4 rd = P.sh_d[sl], rc = P.sh_c[sl];
+                        const f3 dir = f3of(rd), col = f3of(rc);
+                        const float pn = rd.w, ppick = rc.w;
                         f3 brdf;
                         if (kind == RT_MAT_LAMBERTIAN) {
-                            float ndl = fmaxf(dot(s.n, ls.dir), 0.0f);
+                            float ndl = fmaxf(dot(s.n, dir), 0.0f);
                             brdf = ndl <= 0.0f ? splat(0.0f) : f3of(g.m1) * (ndl / kPi);
                         } else {
-                            brdf = disney_eval_direct(DP, S.lut_e, ls.dir, view, s);
+                            brdf = disney_eval_direct(DP, S.lut_e, dir, view, s);
                         }
-                        brdf *= view_term * shadow_terminator(s.n, s.n, ls.dir);
-                        if (is_zero(brdf)) continue;
-                        f3 tr = S.medium ? medium_tr(S, ls.dist) : splat(1.0f);
+                        brdf *= view_term * shadow_terminator(s.n, s.n, dir);
+                        f3 c = splat(0.0f), tr = splat(1.0f);
                         float wn = 1.0f;
-                        if (!ls.delta) {
-                            float pb;
-                            if (kind == RT_MAT_LAMBERTIAN) {
-                                float c = dot(s.n, ls.dir);
-                                pb = c > 0.0f ? c * (1.0f / kPi) : 0.0f;
-                            } else pb = disney_pdf(DP, view, ls.dir, s);
-                            if (pb > 0.0f) {
-                                float pl = light_pdf(L, s.p, ls.dir);
-                                float pn = ppick > 0.0f ? ppick * pl : pl;
-                                wn = (pn + pb > 0.0f) ? mis_w(pn, pb, R.power_mis) : 1.0f;
+                        if (!is_zero(brdf)) {
+                            if (S.medium) tr = f3of(P.sh_r[sl]);
+                            if (pn >= 0.0f) {   // not a delta light
+                                float pb;
+                                if (kind == RT_MAT_LAMBERTIAN) {
+                                    float cs = dot(s.n, dir);
+                                    pb = cs > 0.0f ? cs * (1.0f / kPi) : 0.0f;
+                                } else pb = disney_pdf(DP, view, dir, s);
+                                if (pb > 0.0f) wn = (pn + pb > 0.0f) ? mis_w(pn, pb, R.power_mis) : 1.0f;
                             }
+                            c = wn * col * brdf * tr;
+                            if (ppick > 0.0f) c = c / ppick;
+                            c = vw * c;
                         }
-                        f3 c = wn * ls.color * brdf * tr;
-                        if (ppick > 0.0f) c = c / ppick;
-                        c = vw * c;
-                        if (is_zero(c)) continue;
-                        int64_t sl = (int64_t)(grouped ? jn : n_shadow) * P.n + p;
-                        P.sh_o[sl] = make_float4(ls.so.x, ls.so.y, ls.so.z, ls.tmax);
-                        P.sh_d[sl] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
+                        const bool drop = is_zero(c);
+                        if (drop && grouped) {
+                            smask &= ~(1u << j);
+                            n_shadow--;
+                            continue;
+                        }
                         P.sh_c[sl] = make_float4(c.x, c.y, c.z, 0.0f);
                         if constexpr (ADJ) {
                             float4 g0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g1 = g0;
-                            if (adj_on) {
+                            if (adj_on && !drop) {
                                 // c = vw * (wn Le tr / ppick) * st * brdf(p): dc/dp = k * dbrdf/dp
-                                const D3 db = disney_eval_direct_d(DP, DD, S.lut_e, ls.dir, view, s);
-                                const float st = view_term * shadow_terminator(s.n, s.n, ls.dir);
-                                f3 k = wn * ls.color * tr;
+                                const D3 db = disney_eval_direct_d(DP, DD, S.lut_e, dir, view, s);
+                                const float st = view_term * shadow_terminator(s.n, s.n, dir);
+                                f3 k = wn * col * tr;
                                 if (ppick > 0.0f) k = k / ppick;
                                 k = vw * k * adj_w * st;
                                 float gj[kNd];
-#pragma unroll
-                                for (int j = 0; j < kNd; j++) gj[j] = (k.x * db.x.d[j] + k.y * db.y.d[j]) + k.z * db.z.d[j];
+    #pragma unroll
+                                for (int q = 0; q < kNd; q++) gj[q] = (k.x * db.x.d[q] + k.y * db.y.d[q]) + k.z * db.z.d[q];
                                 g0 = make_float4(gj[0], gj[1], gj[2], gj[3]);
                                 g1 = make_float4(gj[4], tuv.x, tuv.y, 1.0f);
                             }
                             P.adj_g[2 * sl] = g0;
                             P.adj_g[2 * sl + 1] = g1;
                         }
-                        if (jn < kMaxSeg) smask |= 1u << jn;
-                        n_shadow++;
                     }
                 }
-            }
-            // ── caustic gather request (Renderer.cs:1108-1109) ────────────
-            if (S.caustics && (kind == RT_MAT_LAMBERTIAN || kind == RT_MAT_DISNEY)) {
-                gather = true;
-                P.gq0[p] = make_float4(o.x, o.y, o.z, htuv.x);
-                P.gq1[p] = make_float4(d.x, d.y, d.z, __int_as_float(he));
-                P.gq2[p] = make_float4(htuv.y, htuv.z, s.rel_eta, __int_as_float(p));
-                P.gq3[p] = make_float4(vw.x, vw.y, vw.z, __int_as_float(tgt ? 1 : 0));
-            }
-            // ── indirect bounce ─────────────────────────────────────────
-            bool have = false, legacy = false, is_delta = false;
-            int legacy_lobe = -1;   // Disney lobe of a fallback (Scatter) bounce
-            f3 att = splat(0.0f), wo = splat(0.0f), origin = s.p;
-            float spdf = 0.0f;
-            BsdfS bs;
-            bs.has_next_abs = false;
-            bs.transition = 0;
-            if (kind == RT_MAT_LAMBERTIAN) {
-                f3 alb = f3of(g.m1);
-                f3 dir = s.n + random_unit_vector(rng);
-                if (near_zero(dir)) dir = s.n;
-                f3 w = normalize(dir);
-                float ndw = dot(s.n, w);
-                if (ndw > 0.0f) {
-                    bs.wo = w;
-                    bs.f = alb * (1.0f / kPi);
-                    bs.pdf = ndw * (1.0f / kPi);
-                    bs.delta = false;
-                    have = true;
-                } else {
-                    f3 d2 = s.n + random_unit_vector(rng);
-                    if (near_zero(d2)) d2 = s.n;
-                    wo = d2;
-                    att = alb;
+                // ── indirect bounce ─────────────────────────────────────────
+                bool have = false, legacy = false, is_delta = false;
+                int legacy_lobe = -1;   // Disney lobe of a fallback (Scatter) bounce
+                f3 att = splat(0.0f), wo = splat(0.0f), origin = s.p;
+                float spdf = 0.0f;
+                BsdfS bs;
+                bs.has_next_abs = false;
+                bs.transition = 0;
+                if (kind == RT_MAT_LAMBERTIAN) {
+                    f3 alb = f3of(g.m1);
+                    f3 dir = s.n + random_unit_vector(rng);
+                    if (near_zero(dir)) dir = s.n;
+                    f3 w = normalize(dir);
+                    float ndw = dot(s.n, w);
+                    if (ndw > 0.0f) {
+                        bs.wo = w;
+                        bs.f = alb * (1.0f / kPi);
+                        bs.pdf = ndw * (1.0f / kPi);
+                        bs.delta = false;
+                        have = true;
+                    } else {
+                        f3 d2 = s.n + random_unit_vector(rng);
+                        if (near_zero(d2)) d2 = s.n;
+                        wo = d2;
+                        att = alb;
+                        legacy = true;
+                    }
+                } else if (kind == RT_MAT_DISNEY) {
+                    if (disney_sample(DP, S.lut_e, view, s, rng, bs)) have = true;
+                    else {
+                        if (disney_scatter(DP, S.lut_e, s, d, rng, att, wo, legacy_lobe)) legacy = true;
+                    }
+                } else if (kind == RT_MAT_DIELECTRIC) {
+                    dielectric_scatter(g, s, d, rng, att, wo, origin);
                     legacy = true;
+                    is_delta = true;
                 }
-            } else if (kind == RT_MAT_DISNEY) {
-                if (disney_sample(DP, S.lut_e, view, s, rng, bs)) have = true;
-                else {
-                    if (disney_scatter(DP, S.lut_e, s, d, rng, att, wo, legacy_lobe)) legacy = true;
-                }
-            } else if (kind == RT_MAT_DIELECTRIC) {
-                dielectric_scatter(g, s, d, rng, att, wo, origin);
-                legacy = true;
-                is_delta = true;
-            }
-            if (have) {
-                // ShadeSampleBounce (Renderer.cs:1235-1383)
-                bool ok = true;
-                if (bs.delta) att = bs.f;
-                else {
-                    float a = fabsf(dot(s.n, bs.wo));
-                    if (a <= 0.0f || bs.pdf <= 0.0f) ok = false;
-                    else att = bs.f * a / bs.pdf;
-                }
-                if (ok) {
+                if (have) {
+                    // ShadeSampleBounce (Renderer.cs:1235-1383)
+                    bool ok = true;
+                    if (bs.delta) att = bs.f;
+                    else {
+                        float a = fabsf(dot(s.n, bs.wo));
+                        if (a <= 0.0f || bs.pdf <= 0.0f) ok = false;
+                        else att = bs.f * a / bs.pdf;
+                    }
+                    if (ok) {
+                        if (R.aovs && (aovf & kAovPending)) {
+                            if (bs.delta) {
+                                f3 aw = f3of(av0) * clamp01(bs.f);
+                                av0 = make_float4(aw.x, aw.y, aw.z, av0.w);
+                            } else aov_commit(f3of(av1), s.n);
+                        }
+                        f3 nt = thr_s * att;
+                        bool killed = false;
+                        if (used >= R.rr_min_b) {
+                            float sp = fmax3(nt);
+                            sp = fmaxf(sp, R.rr_min_s);
+                            sp = fminf(sp, 0.95f);
+                            if (rng.next() > sp) killed = true;
+                            else {
+                                float inv = 1.0f / sp;
+                                att *= inv;
+                                nt *= inv;
+                            }
+                        }
+                        if (!killed) {
+                            f3 od = dot(bs.wo, s.n) >= 0.0f ? s.n : -s.n;
+                            next_o = offset_origin(s.p, od);
+                            next_d = bs.wo;
+                            next_pdf = bs.delta ? 0.0f : bs.pdf;
+                            next_delta = bs.delta;
+                            next_cat = dot(s.n, bs.wo) < 0.0f ? 3 : (bs.delta ? 2 : 1);
+                            if (bs.has_next_abs) sigma_abs = bs.next_abs;
+                            if (bs.transition == 1) {
+                                float v = (mflags & kMatDielectricIor) ? g.m3.w
+                                                                       : (ior_n > 0 ? P.ior[(int64_t)(ior_n - 1) * P.n + p] : 1.0f);
+                                if (ior_n >= 8) {
+                                    for (int k = 0; k < 7; k++) P.ior[(int64_t)k * P.n + p] = P.ior[(int64_t)(k + 1) * P.n + p];
+                                    P.ior[(int64_t)7 * P.n + p] = v;
+                                } else {
+                                    P.ior[(int64_t)ior_n * P.n + p] = v;
+                                    ior_n++;
+                                }
+                            } else if (bs.transition == 2 && ior_n > 0) ior_n--;
+                            thr_next = nt;
+                            wt_next = primary ? att : wt * att;
+                            if (primary) next_tgt = true;
+                            next_cs = next_cst(cst, bs.delta);
+                            cont = true;
+                            if constexpr (ADJ) {
+                                if (adj_on) {
+                                    // non-delta: att = F(wo) |cos| / pdf / q, so d(att)/dp / att =
+                                    // dF/dp / F; delta: the lobe weight over its fixed probability
+                                    const D3 fw = bs.delta ? lobe_att_d(bs.lobe, DP, DD, S.lut_e, view, bs.wo, s)
+                                                           : disney_eval_d(DP, DD, S.lut_e, view, bs.wo, s);
+                                    log_deriv(fw, adj_w, adj_qv);
+                                    adj_pend = true;
+                                }
+                            }
+                        }
+                    }
+                } else if (legacy) {
+                    // legacy Scatter path (Renderer.cs:1127-1224)
                     if (R.aovs && (aovf & kAovPending)) {
-                        if (bs.delta) {
-                            f3 aw = f3of(av0) * clamp01(bs.f);
+                        if (is_delta) {
+                            f3 aw = f3of(av0) * clamp01(att);
                             av0 = make_float4(aw.x, aw.y, aw.z, av0.w);
                         } else aov_commit(f3of(av1), s.n);
                     }
@@ -1499,90 +1634,34 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                         }
                     }
                     if (!killed) {
-                        f3 od = dot(bs.wo, s.n) >= 0.0f ? s.n : -s.n;
-                        next_o = offset_origin(s.p, od);
-                        next_d = bs.wo;
-                        next_pdf = bs.delta ? 0.0f : bs.pdf;
-                        next_delta = bs.delta;
-                        next_cat = dot(s.n, bs.wo) < 0.0f ? 3 : (bs.delta ? 2 : 1);
-                        if (bs.has_next_abs) sigma_abs = bs.next_abs;
-                        if (bs.transition == 1) {
-                            float v = (mflags & kMatDielectricIor) ? g.m3.w
-                                                                   : (ior_n > 0 ? P.ior[(int64_t)(ior_n - 1) * P.n + p] : 1.0f);
-                            if (ior_n >= 8) {
-                                for (int k = 0; k < 7; k++) P.ior[(int64_t)k * P.n + p] = P.ior[(int64_t)(k + 1) * P.n + p];
-                                P.ior[(int64_t)7 * P.n + p] = v;
-                            } else {
-                                P.ior[(int64_t)ior_n * P.n + p] = v;
-                                ior_n++;
-                            }
-                        } else if (bs.transition == 2 && ior_n > 0) ior_n--;
+                        next_o = origin;
+                        next_d = wo;
+                        next_pdf = 0.0f;
+                        next_delta = is_delta;
+                        next_cat = dot(s.n, wo) < 0.0f ? 3 : (is_delta ? 2 : 1);
+                        if ((mflags & kMatDielectricIor) && dot(wo, s.n) < 0.0f) {
+                            if (s.front) {
+                                float v = g.m3.w;
+                                if (ior_n >= 8) {
+                                    for (int k = 0; k < 7; k++) P.ior[(int64_t)k * P.n + p] = P.ior[(int64_t)(k + 1) * P.n + p];
+                                    P.ior[(int64_t)7 * P.n + p] = v;
+                                } else {
+                                    P.ior[(int64_t)ior_n * P.n + p] = v;
+                                    ior_n++;
+                                }
+                            } else if (ior_n > 0) ior_n--;
+                        }
                         thr_next = nt;
                         wt_next = primary ? att : wt * att;
                         if (primary) next_tgt = true;
-                        next_cs = next_cst(cst, bs.delta);
+                        next_cs = next_cst(cst, is_delta);
                         cont = true;
                         if constexpr (ADJ) {
-                            if (adj_on) {
-                                // non-delta: att = F(wo) |cos| / pdf / q, so d(att)/dp / att =
-                                // dF/dp / F; delta: the lobe weight over its fixed probability
-                                const D3 fw = bs.delta ? lobe_att_d(bs.lobe, DP, DD, S.lut_e, view, bs.wo, s)
-                                                       : disney_eval_d(DP, DD, S.lut_e, view, bs.wo, s);
-                                log_deriv(fw, adj_w, adj_qv);
+                            if (adj_on && legacy_lobe >= 0) {
+                                const f3 V = normalize(-d);
+                                log_deriv(lobe_att_d(legacy_lobe, DP, DD, S.lut_e, V, wo, s), adj_w, adj_qv);
                                 adj_pend = true;
                             }
-                        }
-                    }
-                }
-            } else if (legacy) {
-                // legacy Scatter path (Renderer.cs:1127-1224)
-                if (R.aovs && (aovf & kAovPending)) {
-                    if (is_delta) {
-                        f3 aw = f3of(av0) * clamp01(att);
-                        av0 = make_float4(aw.x, aw.y, aw.z, av0.w);
-                    } else aov_commit(f3of(av1), s.n);
-                }
-                f3 nt = thr_s * att;
-                bool killed = false;
-                if (used >= R.rr_min_b) {
-                    float sp = fmax3(nt);
-                    sp = fmaxf(sp, R.rr_min_s);
-                    sp = fminf(sp, 0.95f);
-                    if (rng.next() > sp) killed = true;
-                    else {
-                        float inv = 1.0f / sp;
-                        att *= inv;
-                        nt *= inv;
-                    }
-                }
-                if (!killed) {
-                    next_o = origin;
-                    next_d = wo;
-                    next_pdf = 0.0f;
-                    next_delta = is_delta;
-                    next_cat = dot(s.n, wo) < 0.0f ? 3 : (is_delta ? 2 : 1);
-                    if ((mflags & kMatDielectricIor) && dot(wo, s.n) < 0.0f) {
-                        if (s.front) {
-                            float v = g.m3.w;
-                            if (ior_n >= 8) {
-                                for (int k = 0; k < 7; k++) P.ior[(int64_t)k * P.n + p] = P.ior[(int64_t)(k + 1) * P.n + p];
-                                P.ior[(int64_t)7 * P.n + p] = v;
-                            } else {
-                                P.ior[(int64_t)ior_n * P.n + p] = v;
-                                ior_n++;
-                            }
-                        } else if (ior_n > 0) ior_n--;
-                    }
-                    thr_next = nt;
-                    wt_next = primary ? att : wt * att;
-                    if (primary) next_tgt = true;
-                    next_cs = next_cst(cst, is_delta);
-                    cont = true;
-                    if constexpr (ADJ) {
-                        if (adj_on && legacy_lobe >= 0) {
-                            const f3 V = normalize(-d);
-                            log_deriv(lobe_att_d(legacy_lobe, DP, DD, S.lut_e, V, wo, s), adj_w, adj_qv);
-                            adj_pend = true;
                         }
                     }
                 }
@@ -1596,7 +1675,15 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         if (cont) {
             if (depth - 1 <= 0 || fmax3(thr_next) < 1e-4f) cont = false;
         }
-        if (cont) {
+        if (bounce) {
+            // the bounce phase continues from here: roulette throughput with
+            // the medium weight, radiance weight with the segment weight, the
+            // next sampler dimension
+            const f3 ts = thr * mb;
+            P.d[p] = make_float4(d.x, d.y, d.z, __int_as_float(flags));
+            P.thr[p] = make_float4(ts.x, ts.y, ts.z, __int_as_float(depth));
+            P.wt[p] = make_float4(wt.x, wt.y, wt.z, __uint_as_float(rng.dim));
+        } else if (cont) {
             flags &= ~(kPfPrevDelta | (3 << kPfCatShift) | kPfTgtInd | (3 << kPfCstShift));
             flags |= (next_delta ? kPfPrevDelta : 0) | (next_cat << kPfCatShift) | (next_tgt ? kPfTgtInd : 0) |
                      (next_cs << kPfCstShift);
@@ -1626,30 +1713,54 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
     });
     // slot state, per-slot outputs and this wave's counts (no atomics: the
     // queues are laid out by k_wave_scan + k_compact from the counts)
-    bool retire = active && !cont;
+    const bool retire = active && !cont && !bounce;
     if (p < P.n) {
-        if (retire) P.state[p] = kSlotRetired;
-        P.smask[p] = maxs <= kMaxSeg ? (int32_t)smask : n_shadow;
-        P.gflag[p] = gather ? 1 : 0;
+        if (PH == kPhBounce) {
+            if (active) {
+                P.state[p] = cont ? kSlotActive : kSlotRetired;
+                P.smask[p] = maxs <= kMaxSeg ? (int32_t)smask : n_shadow;
+            }
+        } else {
+            if (retire) P.state[p] = kSlotRetired;
+            else if (bounce) P.state[p] = kSlotBounce;
+            P.smask[p] = maxs <= kMaxSeg ? (int32_t)smask : n_shadow;
+            P.gflag[p] = gather ? 1 : 0;
+        }
     }
     const int w = p >> 6;
-    // retired this iteration, plus slots restarted as already-retired
-    // (samples of depth-0 jobs), which k_retire finishes as well
-    const int n_cont = __popcll(__ballot(cont)), n_ret = __popcll(__ballot(retire || st_in == kSlotRetired));
-    const int n_g = __popcll(__ballot(gather));
-    int n_sh = n_shadow;
-    for (int off = 32; off > 0; off >>= 1) n_sh += __shfl_xor(n_sh, off, 64);
-    if (threadIdx.x == 0) {
-        P.wc[kWcNext * P.nw + w] = n_cont;
-        P.wc[kWcRetire * P.nw + w] = n_ret;
-        P.wc[kWcGather * P.nw + w] = n_g;
-        P.wc[kWcShadow * P.nw + w] = maxs <= kMaxSeg ? 0 : n_sh;
-    }
-    if (maxs <= kMaxSeg)
-        for (int j = 0; j < maxs; j++) {
-            int c = __popcll(__ballot((smask >> j) & 1u));
-            if (threadIdx.x == 0) P.wc[(kWcSeg + j) * P.nw + w] = c;
+    if (PH != kPhNee) {
+        // continuing and retired paths: retired this iteration, plus slots
+        // restarted as already-retired (samples of depth-0 jobs), which
+        // k_retire finishes as well; the bounce phase also counts the
+        // vertices the NEE phase ended or continued (medium scattering)
+        const bool is_cont = PH == kPhBounce ? (active ? cont : st_in == kSlotActive) : cont;
+        const bool is_ret = PH == kPhBounce ? (active ? !cont : st_in == kSlotRetired) : (retire || st_in == kSlotRetired);
+        const int n_cont = __popcll(__ballot(is_cont)), n_ret = __popcll(__ballot(is_ret));
+        // shadow rays of every slot: the bounce phase's own vertices after the
+        // BSDF evaluation, the others as the NEE phase left them
+        uint32_t sm = smask;
+        int nsh = n_shadow;
+        if (PH == kPhBounce && !active) {
+            const int32_t m = p < P.n ? P.smask[p] : 0;
+            sm = maxs <= kMaxSeg ? (uint32_t)m : 0u;
+            nsh = maxs <= kMaxSeg ? 0 : m;
         }
+        for (int off = 32; off > 0; off >>= 1) nsh += __shfl_xor(nsh, off, 64);
+        if (threadIdx.x == 0) {
+            P.wc[kWcNext * P.nw + w] = n_cont;
+            P.wc[kWcRetire * P.nw + w] = n_ret;
+            P.wc[kWcShadow * P.nw + w] = maxs <= kMaxSeg ? 0 : nsh;
+        }
+        if (maxs <= kMaxSeg)
+            for (int j = 0; j < maxs; j++) {
+                int c = __popcll(__ballot((sm >> j) & 1u));
+                if (threadIdx.x == 0) P.wc[(kWcSeg + j) * P.nw + w] = c;
+            }
+    }
+    if (PH != kPhBounce) {
+        const int n_g = __popcll(__ballot(gather));
+        if (threadIdx.x == 0) P.wc[kWcGather * P.nw + w] = n_g;
+    }
 }
 
 // Per-wave counts when no path is traced (depth-0 renders): every pending
@@ -2192,7 +2303,8 @@ struct rt_ctx {
     size_t film_bytes = 0;
     hipEvent_t ev[8] = {};
     bool priv_stack = true;    // traversal stack in private scratch (false: LDS)
-    int shade_wpe = 2;         // minimum waves per SIMD the shading kernel is compiled for
+    int shade_wpe = 3;         // minimum waves per SIMD the NEE shading kernel is compiled for
+    int bounce_wpe = 2;        // ... and the bounce shading kernel
     void* ph_mem = nullptr;    // photon pass buffers (grow-only)
     size_t ph_bytes = 0;
     hipEvent_t iev[2][kIterBatch][5] = {};   // per-iteration kernel timing, two batches
@@ -2297,6 +2409,8 @@ static DevScene dev_view(const rt_scene* sc) {
     S.top_count = h.top_count;
     S.num_lights = (int)h.lights.size();
     S.medium = h.medium ? 1 : 0;
+    S.env_light = h.env_light;
+    S.sky = h.sky;
     S.opaque_shadows = h.opaque_shadows ? 1 : 0;
     S.cam_o = h.cam_origin;
     S.cam_llc = h.cam_llc;
@@ -2407,11 +2521,20 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
             if (adj_replay) {
                 if constexpr (std::is_same<JOBS, OneJob>::value)
                     hipLaunchKernelGGL((k_shade<JOBS, 1, true>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
-            } else switch (ctx->shade_wpe) {
-                case 2: hipLaunchKernelGGL((k_shade<JOBS, 2>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
-                case 3: hipLaunchKernelGGL((k_shade<JOBS, 3>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
-                case 4: hipLaunchKernelGGL((k_shade<JOBS, 4>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
-                default: hipLaunchKernelGGL((k_shade<JOBS, 1>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+            } else {
+                // split shade: NEE phase, then the bounce phase
+                switch (ctx->shade_wpe) {
+                    case 1: hipLaunchKernelGGL((k_shade<JOBS, 1, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
+                    case 2: hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
+                    case 4: hipLaunchKernelGGL((k_shade<JOBS, 4, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
+                    default: hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                }
+                switch (ctx->bounce_wpe) {
+                    case 1: hipLaunchKernelGGL((k_shade<JOBS, 1, false, kPhBounce>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
+                    case 2: hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
+                    case 4: hipLaunchKernelGGL((k_shade<JOBS, 4, false, kPhBounce>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
+                    default: hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhBounce>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                }
             }
             if (e) HIP_TRY(hipEventRecord(e[2], ctx->stream));
             hipLaunchKernelGGL(k_wave_scan, dim3(kWcSeg + (maxs <= kMaxSeg ? maxs : 0)), dim3(1024), 0, ctx->stream, Pw);
@@ -2547,6 +2670,8 @@ static int photon_pass(rt_ctx* ctx, const rt_scene* sc, const DevScene& S, int b
     std::vector<float> lum;
     float total = 0.0f;
     for (size_t i = 0; i < h.lights.size(); i++) {
+        // CausticPhotonTracer.CanEmit: an environment light emits no photons
+        if (h.lights[i].l0.x == RT_LIGHT_ENVIRONMENT) continue;
         float p = std::max(0.0f, h.light_power[i]);
         if (p <= 0.0f) continue;
         src.push_back(PhotonSrc{(int)i, 0, 0, 0});
@@ -2735,6 +2860,8 @@ int rt_create(int32_t device, rt_ctx** out) {
     ctx->use_graph = gr && std::string(gr) == "1";
     const char* wpe = getenv("RT3D_SHADE_WPE");
     if (wpe) ctx->shade_wpe = std::max(1, std::min(4, atoi(wpe)));
+    const char* bwpe = getenv("RT3D_BOUNCE_WPE");
+    if (bwpe) ctx->bounce_wpe = std::max(1, std::min(4, atoi(bwpe)));
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RT_ERR_DEVICE;

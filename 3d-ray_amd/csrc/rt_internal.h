@@ -103,6 +103,9 @@ struct alignas(16) GMat {
     float4 m8;
     // m9: sigma_a.xyz, e_avg(alpha_iso)
     float4 m9;
+    // m10: thin film thickness (nm, 0 = none), film ior (ShadingParams,
+    // DisneyBsdf.cs:287-291)
+    float4 m10;
 };
 
 // Light record (4 × 16 B)

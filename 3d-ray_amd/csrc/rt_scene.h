@@ -39,6 +39,7 @@ struct HostScene {
     f3 cam_origin, cam_llc, cam_horiz, cam_vert, cam_u, cam_v;
     float lens_radius = 0.0f;
     f3 sky;
+    int env_light = -1;                // first EnvironmentLight of the list (Renderer.cs:426)
     bool medium = false;
     bool opaque_shadows = true;        // no material transmits shadow rays
     f3 sig_a, sig_s, sig_t;

@@ -10,6 +10,7 @@
 // threads like the reference's Parallel.Invoke (BvhNode.cs:132-142); the
 // ranges are disjoint so the result does not depend on scheduling.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -89,13 +90,13 @@ static int build_images(const rt_scene_desc* d, HostScene& hs, std::string& err)
         }
         if (m.image < 0 || m.image >= ni) { err = "material image index out of range"; return RT_ERR_INVALID_ARGUMENT; }
     }
-    static float lut[256];
-    static bool lut_ready = false;
-    if (!lut_ready) {
+    // ImageTexture's sRGB decode table, built once (thread-safe static init)
+    static const std::array<float, 256> lut = [] {
+        std::array<float, 256> t{};
         const float inv255 = 1.0f / 255.0f;
-        for (int i = 0; i < 256; i++) lut[i] = powf((float)i * inv255, 2.2f);
-        lut_ready = true;
-    }
+        for (int i = 0; i < 256; i++) t[i] = powf((float)i * inv255, 2.2f);
+        return t;
+    }();
     int64_t total = 0;
     for (int64_t k = 0; k < ni; k++) {
         const rt_image& im = d->images[k];
@@ -563,12 +564,6 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
         err = "invalid table pointer or count";
         return RT_ERR_INVALID_ARGUMENT;
     }
-    if (d->sky_color[0] != 0.0f || d->sky_color[1] != 0.0f || d->sky_color[2] != 0.0f) {
-        // A non-black flat sky becomes an EnvironmentLight in the reference
-        // (FlatSky.HasImportanceSampling, SceneLoader.cs:528-531).
-        err = "non-black sky (environment light) is not supported";
-        return RT_ERR_UNSUPPORTED;
-    }
     const int64_t np = d->num_primitives;
     if (np >= (int64_t)kLeafStartMask) { err = "too many primitives"; return RT_ERR_UNSUPPORTED; }
     hs = HostScene();
@@ -580,8 +575,10 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
     for (int64_t i = 0; i < d->num_materials; i++) {
         int k = d->materials[i].kind;
         if (k < RT_MAT_LAMBERTIAN || k > RT_MAT_DISNEY) { err = "bad material kind"; return RT_ERR_INVALID_ARGUMENT; }
-        if (k == RT_MAT_DISNEY && d->materials[i].thin_film_thickness > 0.0f) {
-            err = "thin-film Disney materials are not supported";
+        if (k == RT_MAT_DISNEY && d->materials[i].thin_film_thickness > 0.0f &&
+            (d->materials[i].flags & RT_MATF_TEXTURED)) {
+            // the texture adjoint's derivatives cover the Schlick Fresnel only
+            err = "a thin-film Disney material cannot take the texture set";
             return RT_ERR_UNSUPPORTED;
         }
         if (d->materials[i].flags & RT_MATF_TEXTURED) {
@@ -905,6 +902,24 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
                 }
                 power = kPi * area * luminance(em);
                 hs.light_prim[i] = l.primitive;
+                break;
+            }
+            case RT_LIGHT_ENVIRONMENT: {
+                // EnvironmentLight over the flat sky: it samples only when the
+                // sky can (FlatSky.HasImportanceSampling: luminance > 1e-6);
+                // power pi * mean luminance * the bounds' mean cross-section
+                // (EnvironmentLight.ApproximatePower, EnvironmentLight.cs:49-64)
+                const f3 sky = hs.sky;
+                const bool sampleable = luminance(sky) > 1e-6f;
+                ss = std::max(1, l.shadow_samples);
+                g.l1 = make_float4(sky.x, sky.y, sky.z, 1.0f);
+                g.l0.w = sampleable ? 1 : 0;
+                if (sampleable) {
+                    const f3 e = hs.bounds.mx - hs.bounds.mn;
+                    const float cross_section = 0.5f * (e.x * e.y + e.y * e.z + e.x * e.z);
+                    power = (kPi * luminance(sky)) * cross_section;
+                }
+                if (hs.env_light < 0) hs.env_light = (int)i;
                 break;
             }
             default:

@@ -189,6 +189,7 @@ struct DisneyP {
     float pD, pS, pT, pSh, pMs, pDt, pC, spec_trans, spec_trans_raw;
     float sheen, sheen_tint, sheen_rough, diff_trans;
     float clearcoat, cc_alpha, cc_f0, rot, alpha_iso, eavg;
+    float tf_thick, tf_ior;   // thin film (thickness 0: plain Schlick Fresnel)
     int32_t flags;
 };
 __device__ __forceinline__ DisneyP load_disney(const GMat& g) {
@@ -204,6 +205,7 @@ __device__ __forceinline__ DisneyP load_disney(const GMat& g) {
     p.clearcoat = g.m7.x; p.cc_alpha = g.m7.y; p.cc_f0 = g.m7.z; p.rot = g.m7.w;
     p.ttint = f3of(g.m8); p.alpha_iso = g.m8.w;
     p.sigma = f3of(g.m9); p.eavg = g.m9.w;
+    p.tf_thick = g.m10.x; p.tf_ior = g.m10.y;
     return p;
 }
 
@@ -250,7 +252,75 @@ __device__ __forceinline__ f3 ms_weight(const DisneyP& P, const float* E, float 
     f3 fms = favg * favg * ea / den;
     return fms * ((1.0f - eo) * (1.0f - ei) / om);
 }
-__device__ __forceinline__ f3 eval_fresnel(float c, f3 f0) {
+// ThinFilm.Evaluate (ThinFilm.cs:40-136): Belcour-Barla 2017 iridescent
+// Fresnel of a film (ior eta, thickness d nm) over a substrate of normal
+// reflectance f0, seen from air at cos(theta1) = c1: Airy series of the two
+// interfaces truncated after two harmonics, each projected to linear
+// Rec.709 through the Gaussian fit of the CIE sensitivities (EvalSensitivity,
+// ThinFilm.cs:150-186).
+__device__ __forceinline__ float tf_schlick(float f0, float c) {
+    float x = clampf(1.0f - c, 0.0f, 1.0f);
+    float x5 = x * x;
+    x5 *= x5 * x;
+    return f0 + (1.0f - f0) * x5;
+}
+__device__ __forceinline__ f3 tf_sensitivity(float opd, f3 shift) {
+    const float phase = 2.0f * kPi * opd * 1e-9f;
+    const f3 val = mk(5.4856e-13f, 4.4201e-13f, 5.2481e-13f);
+    const f3 pos = mk(1.6810e+06f, 1.7953e+06f, 2.2084e+06f);
+    const f3 var = mk(4.3278e+09f, 9.3046e+09f, 6.6121e+09f);
+    const f3 s2v = mk(sqrtf(2.0f * kPi * var.x), sqrtf(2.0f * kPi * var.y), sqrtf(2.0f * kPi * var.z));
+    const f3 ct = mk(cosf(pos.x * phase + shift.x), cosf(pos.y * phase + shift.y), cosf(pos.z * phase + shift.z));
+    const float ph2 = phase * phase;
+    const f3 et = mk(expf(-ph2 * var.x), expf(-ph2 * var.y), expf(-ph2 * var.z));
+    f3 xyz = val * s2v * ct * et;
+    xyz.x += 9.7470e-14f * sqrtf(2.0f * kPi * 4.5282e+09f) * cosf(2.2399e+06f * phase + shift.x) * expf(-ph2 * 4.5282e+09f);
+    xyz = xyz / 1.0685e-7f;
+    return mk(3.2404542f * xyz.x - 1.5371385f * xyz.y - 0.4985314f * xyz.z,
+              -0.9692660f * xyz.x + 1.8760108f * xyz.y + 0.0415560f * xyz.z,
+              0.0556434f * xyz.x - 0.2040259f * xyz.y + 1.0572252f * xyz.z);
+}
+__device__ f3 thin_film(float cos1, float film_ior, float thick_nm, f3 f0) {
+    const float outside = 1.0f;
+    const float thickness = fmaxf(thick_nm, 0.0f);
+    const float tt = clampf((thickness - 0.0f) / (30.0f - 0.0f), 0.0f, 1.0f);
+    const float sm = tt * tt * (3.0f - 2.0f * tt);
+    const float eta2 = fmaxf(outside + (film_ior - outside) * sm, 1.0001f);
+    const float c1 = clampf(cos1, 0.0f, 1.0f);
+    const float er = outside / eta2;
+    const float s2sq = er * er * (1.0f - c1 * c1);
+    if (s2sq >= 1.0f) return splat(1.0f);
+    const float cos2 = sqrtf(1.0f - s2sq);
+    const float q12 = (eta2 - outside) / (eta2 + outside);
+    const float r12 = tf_schlick(q12 * q12, c1);
+    const float t121 = 1.0f - r12;
+    const float phi12 = eta2 < outside ? kPi : 0.0f;
+    const float phi21 = kPi - phi12;
+    const f3 bf = mk(clampf(f0.x, 0.0f, 0.9999f), clampf(f0.y, 0.0f, 0.9999f), clampf(f0.z, 0.0f, 0.9999f));
+    const f3 sq = mk(sqrtf(bf.x), sqrtf(bf.y), sqrtf(bf.z));
+    const f3 bior = (splat(1.0f) + sq) / (splat(1.0f) - sq);
+    const f3 q23 = (bior - splat(eta2)) / (bior + splat(eta2));
+    const f3 r023 = q23 * q23;
+    const f3 r23 = mk(tf_schlick(r023.x, cos2), tf_schlick(r023.y, cos2), tf_schlick(r023.z, cos2));
+    const f3 phi23 = mk(bior.x < eta2 ? kPi : 0.0f, bior.y < eta2 ? kPi : 0.0f, bior.z < eta2 ? kPi : 0.0f);
+    const float opd = 2.0f * eta2 * thickness * cos2;
+    const f3 phi = splat(phi21) + phi23;
+    const f3 R = splat(r12) * r23;
+    const f3 R123 = mk(clampf(R.x, 1e-5f, 0.9999f), clampf(R.y, 1e-5f, 0.9999f), clampf(R.z, 1e-5f, 0.9999f));
+    const f3 r123 = mk(sqrtf(R123.x), sqrtf(R123.y), sqrtf(R123.z));
+    const f3 Rs = (t121 * t121) * r23 / (splat(1.0f) - R123);
+    f3 I = splat(r12) + Rs;
+    f3 Cm = Rs - splat(t121);
+    for (int m = 1; m <= 2; m++) {
+        Cm *= r123;
+        const f3 Sm = 2.0f * tf_sensitivity((float)m * opd, (float)m * phi);
+        I += Cm * Sm;
+    }
+    return mk(fmaxf(I.x, 0.0f), fmaxf(I.y, 0.0f), fmaxf(I.z, 0.0f));
+}
+// EvalFresnel (DisneyBsdf.cs:1984-1989): Schlick, or the thin film
+__device__ __forceinline__ f3 eval_fresnel(float c, f3 f0, const DisneyP& P) {
+    if (P.tf_thick > 0.0f) return thin_film(c, P.tf_ior, P.tf_thick, f0);
     float w = schlick_weight(c);
     return f0 + (splat(1.0f) - f0) * w;
 }
@@ -282,7 +352,7 @@ __device__ f3 disney_eval_direct(const DisneyP& P, const float* E, f3 L, f3 V, c
         f3 vl = fr.to_local(V), ll = fr.to_local(L), hl = fr.to_local(H);
         float D = d_ggx(hl, P.ax, P.ay);
         float G = g1_ggx(vl, P.ax, P.ay) * g1_ggx(ll, P.ax, P.ay);
-        f3 F = eval_fresnel(vdh, P.f0);
+        f3 F = eval_fresnel(vdh, P.f0, P);
         spec = D * G * F / fmaxf(4.0f * ndv, 1e-6f);
     }
     f3 cc = splat(0.0f);
@@ -334,7 +404,7 @@ __device__ f3 disney_eval(const DisneyP& P, const float* E, f3 V, f3 L, const Su
     f3 vl = fr.to_local(V), ll = fr.to_local(L), hlc = fr.to_local(H);
     float D = d_ggx(hlc, P.ax, P.ay);
     float G = g1_ggx(vl, P.ax, P.ay) * g1_ggx(ll, P.ax, P.ay);
-    f3 F = eval_fresnel(vdh, P.f0);
+    f3 F = eval_fresnel(vdh, P.f0, P);
     f3 spec = D * G / fmaxf(4.0f * ndv * ndl, 1e-7f) * F;
     f3 cc = splat(0.0f);
     if (P.clearcoat > 0.0f) {
@@ -427,7 +497,7 @@ __device__ bool disney_scatter(const DisneyP& P, const float* E, const Surf& s, 
         } else {
             dir = L;
             float vdh = fmaxf(dot(V, H), 1e-4f);
-            f3 F = eval_fresnel(vdh, P.f0);
+            f3 F = eval_fresnel(vdh, P.f0, P);
             att = F * g1_ggx(fr.to_local(L), P.ax, P.ay);
             att = att / fmaxf(P.pS, 0.1f);
         }

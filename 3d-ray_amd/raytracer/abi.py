@@ -19,7 +19,7 @@ RT_MAT_LAMBERTIAN, RT_MAT_EMISSIVE, RT_MAT_DIELECTRIC, RT_MAT_DISNEY = 0, 1, 2, 
 RT_MATF_LIGHT_PROXY, RT_MATF_THIN_WALLED, RT_MATF_TRANSMISSION_COLOR, RT_MATF_COAT_ROUGHNESS = 1, 2, 4, 8
 RT_MATF_TEXTURED = 16
 TEX_CHANNELS = 5   # texture set planes: base colour r, g, b, roughness, metallic
-RT_LIGHT_SPHERE, RT_LIGHT_AREA, RT_LIGHT_DIRECTIONAL, RT_LIGHT_GEOMETRY = 0, 1, 2, 3
+RT_LIGHT_SPHERE, RT_LIGHT_AREA, RT_LIGHT_DIRECTIONAL, RT_LIGHT_GEOMETRY, RT_LIGHT_ENVIRONMENT = 0, 1, 2, 3, 4
 RT_SAMPLER_SOBOL, RT_SAMPLER_PRNG = 0, 1
 RT_MIS_BALANCE, RT_MIS_POWER = 0, 1
 RT_LIGHTS_ALL, RT_LIGHTS_POWER, RT_LIGHTS_UNIFORM = 0, 1, 2

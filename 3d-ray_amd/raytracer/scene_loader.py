@@ -187,6 +187,21 @@ def _transform(e):
     return M, uniform
 
 
+def _rotated(e):
+    """True when the entity's transform rotates (a non-zero rotate angle)."""
+    r = e.get("rotate")
+    return r is not None and any(float(x) % 360.0 != 0.0 for x in list(r)[:3])
+
+
+def _reads_surface_frame(b, mat):
+    """True when the material's shading reads the hit's UV or tangent frame:
+    an image texture, or a Disney material with anisotropy."""
+    rec = b.material_record(mat)
+    if int(rec["flags"]) & abi.RT_MATF_IMAGE:
+        return True
+    return int(rec["kind"]) == abi.RT_MAT_DISNEY and float(rec["anisotropic"]) != 0.0
+
+
 def _pt(M, p):
     return tuple((M @ np.array([*p, 1.0]))[:3])
 
@@ -256,6 +271,12 @@ def _entity(b, e, mat, base_dir=None, warnings=None):
     if typ == "sphere":
         c = _v3(e.get("center"), (0.0, 0.0, 0.0))
         r = _f(e, "radius", 1.0)
+        if _rotated(e) and _reads_surface_frame(b, mat):
+            # the reference's Transform wrapper hits the sphere in object space,
+            # so its UV and tangent (Sphere.cs:51-58) turn with the entity; the
+            # sphere here is baked into world space and would keep them fixed
+            raise SceneLoadError(f"entity '{e.get('name')}': a rotated sphere with an image-textured or "
+                                 "anisotropic material is not supported")
         if not ident:
             if uniform is None:
                 raise SceneLoadError(f"entity '{e.get('name')}': non-uniform scale of a sphere is not supported")
@@ -292,6 +313,12 @@ def _entity(b, e, mat, base_dir=None, warnings=None):
     if typ in ("plane", "infinite_plane"):
         if flags:
             raise SceneLoadError(f"entity '{e.get('name')}': camera-invisible planes are not supported")
+        if _rotated(e) and _reads_surface_frame(b, mat):
+            # InfinitePlane derives its UV axes from the normal it holds: the
+            # reference's rotated object-space axes are not those of the baked
+            # world-space normal
+            raise SceneLoadError(f"entity '{e.get('name')}': a rotated plane with an image-textured or "
+                                 "anisotropic material is not supported")
         p = _v3(e.get("point"), (0.0, 0.0, 0.0))
         n = _v3(e.get("normal"), (0.0, 1.0, 0.0))
         if not ident:
@@ -397,6 +424,17 @@ def load_data(data, width, height, shadow_samples_override=None, camera=None, ba
             raise SceneLoadError(f"ground type '{gt}' is not supported (infinite plane only)")
         if gd.get("visibility") is not None:
             raise SceneLoadError("ground visibility masks are not supported")
+        # UvTransformedHittable (SceneLoader.cs:1360-1367) is not built: a
+        # ground UV transform other than the identity is refused
+        for k, ident_v in (("uv_scale", (1.0, 1.0)), ("uv_offset", (0.0, 0.0)), ("uv_rotation", 0.0)):
+            val = gd.get(k)
+            if val is None:
+                continue
+            vals = [float(x) for x in val] if isinstance(val, (list, tuple)) else [float(val)] * (
+                2 if isinstance(ident_v, tuple) else 1)
+            want = list(ident_v) if isinstance(ident_v, tuple) else [ident_v]
+            if vals[:len(want)] != want:
+                raise SceneLoadError(f"ground '{k}' (a UV transform of the ground) is not supported")
         if gd.get("material"):
             gm = get_material(gd["material"])
         elif gd.get("color") is not None or gd.get("roughness") is not None or gd.get("metallic") is not None:

@@ -168,6 +168,15 @@ class SceneBuilder:
         m["flags"] = flags
         return self.add_materials(m)
 
+    def material_record(self, index):
+        """The MAT_DTYPE record of material `index`."""
+        k = int(index)
+        for chunk in self._mats:
+            if k < len(chunk):
+                return chunk[k]
+            k -= len(chunk)
+        raise IndexError(index)
+
     def image(self, rgba8, scale=(1.0, 1.0)):
         """An ImageTexture from 8-bit RGBA pixels (h, w, 4), top row first,
         with the YAML uv_scale; returns its index for image=."""
@@ -339,6 +348,20 @@ class SceneBuilder:
             g["primitive"] = emissive
             g["material"] = prims["material"][emissive]
             lights.append(g)
+        # AddEnvironmentLight: a flat sky that can be sampled (luminance above
+        # 1e-6, FlatSky.HasImportanceSampling) becomes the last light
+        # (SceneLoader.cs:524-528)
+        sk = np.asarray(sky, np.float32)
+        lum = (np.float32(0.2126) * sk[0] + np.float32(0.7152) * sk[1]) + np.float32(0.0722) * sk[2]
+        if lum > np.float32(1e-6):
+            env = np.zeros(1, LIGHT_DTYPE)
+            env["kind"] = abi.RT_LIGHT_ENVIRONMENT
+            env["shadow_samples"] = self.shadow_samples_override or 1
+            env["primitive"] = -1
+            env["material"] = -1
+            env["color"] = sk
+            env["intensity"] = 1.0
+            lights.append(env)
         lights = np.concatenate(lights) if lights else np.zeros(0, LIGHT_DTYPE)
         return SceneDesc(prims, mats, lights, camera, sky, medium, texcoords=texcoords, normals=normals,
                          images=self._images)

@@ -55,12 +55,54 @@ def traversal_bytes(node, tri, sph, rays):
     return 64.0 * node + 64.0 * tri + 16.0 * sph + 48.0 * rays
 
 
+def host_cpu():
+    """(model name, nproc, cores this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = nproc
+    # the box's per-GPU CPU share, which its OMP_NUM_THREADS states
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        usable = min(usable, int(share))
+    return model, nproc, max(1, usable)
+
+
+def baseline_oracle():
+    """Builds the oracle sources with -O3 -march=native for this host into a
+    temporary directory and points the oracle loader at it; falls back to
+    the prebuilt -O2 library.  Returns the build flags used."""
+    import subprocess
+    import tempfile
+    out = tempfile.mkdtemp(prefix="rt3d_baseline_")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "baseline", f"BASELINE_DIR={out}"],
+                       check=True, capture_output=True, timeout=600)
+        os.environ["ORACLE_LIB"] = os.path.join(out, "liboracle_baseline.so")
+        return "-O3 -march=native -ffp-contract=off"
+    except Exception as e:  # noqa: BLE001 - a missing compiler must not hide the GPU result
+        log(f"native baseline build failed ({e}); using the prebuilt -O2 oracle")
+        return "-O2 -ffp-contract=off (prebuilt)"
+
+
 def cpu_baseline(desc, width, height, spp, depth, rows_target_s, caustics=None):
-    """Times the CPU restatement of the reference (oracle/) on every k-th
-    scanline of the same frame at full spp, on the host's cores."""
+    """Times the CPU restatement of the reference (oracle/, built for this
+    host) on every k-th scanline of the same frame at full spp, with a
+    thread per usable host core and the reference's 16x16 tile queue."""
+    flags = baseline_oracle()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    model, nproc, threads = host_cpu()
     st = scenes_settings(width, height, spp, depth)
     if caustics:
         st.caustics, st.caustic_photons, st.caustic_gather_radius = 1, caustics[0], caustics[1]
@@ -80,8 +122,9 @@ def cpu_baseline(desc, width, height, spp, depth, rows_target_s, caustics=None):
     el = time.perf_counter() - t0
     samples = r["stats"].camera_samples
     return {"value": samples / el / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "host": {"cpu_model": model, "nproc": nproc, "threads": threads, "build": flags},
             "sample": f"every {step}th scanline of the {width}x{height} frame at {spp} spp "
-                      f"({samples} camera samples, {el:.1f} s, {threads} threads)"}
+                      f"({samples} camera samples, {el:.1f} s, {threads} threads = the usable cores of {nproc})"}
 
 
 def scenes_settings(width, height, spp, depth):

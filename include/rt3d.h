@@ -145,12 +145,21 @@ typedef struct rt_material {
  *                                                   (DirectionalLight.cs:102-181)
  *   GEOMETRY     primitive = emissive TRIANGLE/QUAD/SPHERE, material = its
  *                Emissive, shadow_samples, soft_radius (GeometryLight.cs:133-279)
+ *   ENVIRONMENT  the EnvironmentLight over the scene's flat sky
+ *                (rt_scene_desc.sky_color; the record's colour is not read),
+ *                shadow_samples (EnvironmentLight.cs:20-124).  SceneLoader
+ *                appends it after the geometry lights when the sky can be
+ *                sampled (SceneLoader.cs:524-528); the first one in the list
+ *                also weighs BSDF-sampled sky hits by MIS (Renderer.cs:426,
+ *                1820-1861).  Without one the sky is seen at full weight and
+ *                never sampled by NEE, as the reference's Renderer does.
  */
 typedef enum rt_light_kind {
     RT_LIGHT_SPHERE = 0,
     RT_LIGHT_AREA = 1,
     RT_LIGHT_DIRECTIONAL = 2,
-    RT_LIGHT_GEOMETRY = 3
+    RT_LIGHT_GEOMETRY = 3,
+    RT_LIGHT_ENVIRONMENT = 4
 } rt_light_kind;
 
 typedef struct rt_light {
@@ -197,7 +206,9 @@ typedef struct rt_scene_desc {
     const rt_light* lights;
     int64_t num_lights;
     rt_camera camera;
-    float sky_color[3];   /* flat sky (FlatSky.cs); must be black (see rt_scene_create) */
+    float sky_color[3];   /* flat sky radiance (FlatSky.cs): seen by escaping rays
+                             (SampleSky, Renderer.cs:1820-1861), sampled by an
+                             RT_LIGHT_ENVIRONMENT light when the list holds one */
     int32_t reserved0;
     rt_medium global_medium;
     /* Texture set of RT_MATF_TEXTURED materials (no reference counterpart:

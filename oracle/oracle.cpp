@@ -902,6 +902,74 @@ struct Dielectric : Material {
 };
 
 // DisneyBsdf.cs
+// ThinFilm.Evaluate (ThinFilm.cs:40-136) and its helpers (:150-217):
+// Belcour-Barla 2017 iridescent Fresnel, RGB through the Gaussian fit of the
+// CIE sensitivities.
+static float tf_schlick(float f0, float cos_t) {
+    float x = std::fmin(std::fmax(1.0f - cos_t, 0.0f), 1.0f);
+    float x5 = x * x;
+    x5 *= x5 * x;
+    return f0 + (1.0f - f0) * x5;
+}
+static V3 tf_sensitivity(float opd_nm, V3 shift) {
+    float phase = 2.0f * kPi * opd_nm * 1e-9f;
+    V3 val(5.4856e-13f, 4.4201e-13f, 5.2481e-13f);
+    V3 pos(1.6810e+06f, 1.7953e+06f, 2.2084e+06f);
+    V3 var(4.3278e+09f, 9.3046e+09f, 6.6121e+09f);
+    V3 s2v(std::sqrt(2.0f * kPi * var.x), std::sqrt(2.0f * kPi * var.y), std::sqrt(2.0f * kPi * var.z));
+    V3 cs(std::cos(pos.x * phase + shift.x), std::cos(pos.y * phase + shift.y), std::cos(pos.z * phase + shift.z));
+    float ph2 = phase * phase;
+    V3 ex(std::exp(-ph2 * var.x), std::exp(-ph2 * var.y), std::exp(-ph2 * var.z));
+    V3 xyz = val * s2v * cs * ex;
+    xyz.x += 9.7470e-14f * std::sqrt(2.0f * kPi * 4.5282e+09f) * std::cos(2.2399e+06f * phase + shift.x) *
+             std::exp(-ph2 * 4.5282e+09f);
+    xyz = xyz / 1.0685e-7f;
+    float r = 3.2404542f * xyz.x - 1.5371385f * xyz.y - 0.4985314f * xyz.z;
+    float g = -0.9692660f * xyz.x + 1.8760108f * xyz.y + 0.0415560f * xyz.z;
+    float b = 0.0556434f * xyz.x - 0.2040259f * xyz.y + 1.0572252f * xyz.z;
+    return V3(r, g, b);
+}
+static V3 thin_film_fresnel(float cos1, float film_ior, float thick_nm, V3 base_f0) {
+    const float outside = 1.0f;
+    float thickness = std::fmax(thick_nm, 0.0f);
+    float t = std::fmin(std::fmax((thickness - 0.0f) / (30.0f - 0.0f), 0.0f), 1.0f);
+    float sm = t * t * (3.0f - 2.0f * t);
+    float eta2 = std::fmax(outside + (film_ior - outside) * sm, 1.0001f);
+    float c1 = std::fmin(std::fmax(cos1, 0.0f), 1.0f);
+    float er = outside / eta2;
+    float s2sq = er * er * (1.0f - c1 * c1);
+    if (s2sq >= 1.0f) return V3(1.0f);
+    float cos2 = std::sqrt(1.0f - s2sq);
+    float q12 = (eta2 - outside) / (eta2 + outside);
+    float r12 = tf_schlick(q12 * q12, c1);
+    float t121 = 1.0f - r12;
+    float phi12 = (eta2 < outside) ? kPi : 0.0f;
+    float phi21 = kPi - phi12;
+    V3 bf(std::fmin(std::fmax(base_f0.x, 0.0f), 0.9999f), std::fmin(std::fmax(base_f0.y, 0.0f), 0.9999f),
+          std::fmin(std::fmax(base_f0.z, 0.0f), 0.9999f));
+    V3 sq(std::sqrt(bf.x), std::sqrt(bf.y), std::sqrt(bf.z));
+    V3 bior = (V3(1.0f) + sq) / (V3(1.0f) - sq);
+    V3 q23 = (bior - V3(eta2)) / (bior + V3(eta2));
+    V3 r023 = q23 * q23;
+    V3 r23(tf_schlick(r023.x, cos2), tf_schlick(r023.y, cos2), tf_schlick(r023.z, cos2));
+    V3 phi23(bior.x < eta2 ? kPi : 0.0f, bior.y < eta2 ? kPi : 0.0f, bior.z < eta2 ? kPi : 0.0f);
+    float opd = 2.0f * eta2 * thickness * cos2;
+    V3 phi = V3(phi21) + phi23;
+    V3 R = V3(r12) * r23;
+    V3 R123(std::fmin(std::fmax(R.x, 1e-5f), 0.9999f), std::fmin(std::fmax(R.y, 1e-5f), 0.9999f),
+            std::fmin(std::fmax(R.z, 1e-5f), 0.9999f));
+    V3 r123(std::sqrt(R123.x), std::sqrt(R123.y), std::sqrt(R123.z));
+    V3 Rs = (t121 * t121) * r23 / (V3(1.0f) - R123);
+    V3 I = V3(r12) + Rs;
+    V3 Cm = Rs - V3(t121);
+    for (int mm = 1; mm <= 2; mm++) {
+        Cm = Cm * r123;
+        V3 Sm = 2.0f * tf_sensitivity((float)mm * opd, (float)mm * phi);
+        I = I + Cm * Sm;
+    }
+    return vmax(I, V3(0.0f));
+}
+
 struct Disney : Material {
     V3 base;
     float metallic_, roughness_, specular_, specular_tint_, sheen_, sheen_tint_, sheen_roughness_;
@@ -975,7 +1043,9 @@ struct Disney : Material {
         V3 diel = lerp(V3(scaled), scaled * tint, sp.specular_tint);
         return lerp(diel, base, sp.metallic);
     }
+    // EvalFresnel (DisneyBsdf.cs:1984-1989)
     V3 eval_fresnel(float c, V3 f0) const {
+        if (sp.tf_thick > 0.0f) return thin_film_fresnel(c, sp.tf_ior, sp.tf_thick, f0);
         float w = schlick_weight(c);
         return f0 + (V3(1.0f) - f0) * w;
     }
@@ -1739,6 +1809,8 @@ struct Light {
     // CausticPhotonTracer.TryEmit (CausticPhotonTracer.cs:279-345): one photon
     // ray and the flux of the whole light
     virtual bool emit_photon(const AABB& b, Ray& ray, V3& flux) const = 0;
+    // CausticPhotonTracer.CanEmit (CausticPhotonTracer.cs:270-275)
+    virtual bool can_emit() const { return true; }
 };
 
 // SphereLight.cs:112-271
@@ -1975,6 +2047,41 @@ struct DirectionalLight : Light {
     }
 };
 
+// EnvironmentLight.cs:20-124 over a FlatSky (FlatSky.cs:10-40,
+// SkySettings.SampleDirectly / PdfSolidAngle, SkySettings.cs:315-337)
+struct EnvironmentLight : Light {
+    V3 sky;
+    bool sampleable;   // FlatSky.HasImportanceSampling
+    EnvironmentLight(V3 s, int ss) {
+        sky = s;
+        shadow_samples = std::max(1, ss);
+        sampleable = luminance(s) > 1e-6f;
+    }
+    bool is_delta() const override { return false; }
+    bool can_emit() const override { return false; }
+    float approximate_power(const AABB& b) const override {
+        if (!sampleable) return 0.0f;
+        V3 e = b.mx - b.mn;
+        float cross_section = 0.5f * (e.x * e.y + e.y * e.z + e.x * e.z);
+        float irradiance = kPi * luminance(sky);
+        return irradiance * cross_section;
+    }
+    bool emit_photon(const AABB&, Ray&, V3&) const override { return false; }
+    LightSample illuminate(V3 hp, V3 sn, const Hittable* world, int) const override {
+        if (!sampleable) return {true, V3(0.0f), V3(0, 1, 0), 0.0f};
+        V3 dir = random_unit_vector();
+        const float pdf = 1.0f / (4.0f * kPi);
+        float dist = kInfinity;
+        if (length_sq(sn) > 0.0f && dot(sn, dir) <= 0.0f) return {true, V3(0.0f), dir, dist};
+        V3 so = offset_origin(hp, sn);
+        Ray sr(so, dir);
+        V3 tr = shadow_transmittance(world, sr, kEpsilon, kInfinity);
+        if (near_zero(tr)) return {true, V3(0.0f), dir, dist};
+        return {false, (sky * tr) / (pdf * (float)shadow_samples), dir, dist};
+    }
+    float pdf_solid_angle(V3, V3) const override { return sampleable ? 1.0f / (4.0f * kPi) : 0.0f; }
+};
+
 // GeometryLight.cs:60-279 for TRIANGLE / QUAD (area sampling) and SPHERE
 // (solid-angle sampling)
 struct GeometryLight : Light {
@@ -2184,6 +2291,7 @@ struct Scene {
     std::vector<Light*> light_ptrs;
     Camera camera;
     V3 sky;
+    const struct EnvironmentLight* env_light = nullptr;
     bool medium = false;
     V3 sig_a, sig_s, sig_t;
     AABB bounds;
@@ -2475,6 +2583,7 @@ static bool build_photon_map(const Scene& sc, int budget, float cell_size, int n
     std::vector<float> lum;
     float total = 0.0f;
     for (const Light* l : sc.light_ptrs) {
+        if (!l->can_emit()) continue;
         float p = std::fmax(0.0f, l->approximate_power(sc.bounds));
         if (p <= 0.0f) continue;
         src.push_back(l);
@@ -2737,11 +2846,16 @@ struct Renderer {
         return scat;
     }
 
-    // SampleSky (Renderer.cs:1820-1861), flat sky, no environment light
-    V3 sample_sky() const {
+    // SampleSky (Renderer.cs:1820-1861): the flat sky, MIS-weighed against
+    // the first environment light's NEE after a non-delta bounce
+    V3 sample_sky(const Ray& ray, float prev_pdf, bool prev_delta) const {
         V3 sky = sc.sky;
         if (capture_aovs && probe && probe->pending) probe->commit_as(clamp01(sky), V3(0.0f));
-        return sky;
+        const EnvironmentLight* env = sc.env_light;
+        if (prev_delta || !env || !env->sampleable) return sky;
+        float pl = env->pdf_solid_angle(ray.o, ray.d);
+        if (prev_pdf + pl <= 1e-20f) return V3(0.0f);
+        return sky * mis_weight(prev_pdf, pl);
     }
 
     // ComputeDirectLighting (Renderer.cs:1894-2036)
@@ -3074,7 +3188,8 @@ struct Renderer {
             footprint(cur, rec.point, rec.normal, du, dv, rec.fp);
         }
         if (!sc.medium) {
-            V3 r = !hit ? sample_sky() : shade_surface(cur, rec, depth, prev_pdf, prev_delta, absorption, beta, iors, cst);
+            V3 r = !hit ? sample_sky(cur, prev_pdf, prev_delta)
+                        : shade_surface(cur, rec, depth, prev_pdf, prev_delta, absorption, beta, iors, cst);
             return beer_lambert(r, absorption, hit ? rec.t : std::numeric_limits<float>::infinity());
         }
         float tmax = hit ? rec.t : 1e30f;
@@ -3114,7 +3229,7 @@ struct Renderer {
             }
             return beer_lambert(mb * (lnee + lind), absorption, tmed);
         }
-        V3 r = !hit ? mb * sample_sky()
+        V3 r = !hit ? mb * sample_sky(cur, prev_pdf, prev_delta)
                     : mb * shade_surface(cur, rec, depth, prev_pdf, prev_delta, absorption, beta * mb, iors, cst);
         return beer_lambert(r, absorption, hit ? rec.t : std::numeric_limits<float>::infinity());
     }
@@ -3274,6 +3389,12 @@ static Scene* build_scene(const rt_scene_desc* d) {
                                   l.soft_radius, px);
                 break;
             case RT_LIGHT_DIRECTIONAL: L = new DirectionalLight(pos, col, l.intensity, l.radius, l.shadow_samples); break;
+            case RT_LIGHT_ENVIRONMENT: {
+                auto* E = new EnvironmentLight(sc->sky, l.shadow_samples);
+                if (!sc->env_light) sc->env_light = E;   // lights.OfType<EnvironmentLight>().FirstOrDefault()
+                L = E;
+                break;
+            }
             case RT_LIGHT_GEOMETRY: {
                 if (l.primitive < 0 || l.primitive >= np) { g_err = "geometry light primitive"; return nullptr; }
                 const Hittable* g = sc->prim_objs[l.primitive];
@@ -3864,6 +3985,16 @@ static HitRecord probe_rec(const float* frame) {
 }
 
 // out: f[3] = Evaluate(V,L), pdf = Pdf(V,L), fd[3] = EvaluateDirect(L,V,N)
+// ThinFilm.Evaluate at n (cos, film ior, thickness nm, f0 rgb) tuples -> rgb.
+int orc_thin_film(const float* args, int64_t n, float* out) {
+    for (int64_t k = 0; k < n; k++) {
+        const float* a = args + 6 * k;
+        V3 r = thin_film_fresnel(a[0], a[1], a[2], V3(a[3], a[4], a[5]));
+        out[3 * k] = r.x; out[3 * k + 1] = r.y; out[3 * k + 2] = r.z;
+    }
+    return 0;
+}
+
 int orc_bsdf_eval(const rt_material* m, const float* frame, const float* V, const float* L, float* f, float* pdf,
                   float* fd) {
     std::unique_ptr<Material> mat(make_material(*m));

@@ -23,12 +23,15 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # ORACLE_LIB: the benchmark's CPU baseline loads the same sources
+        # built for its host (make baseline); everything else the default
+        path = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
         L = C.CDLL(path)
         vp, fp, ip, i64p = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
         L.orc_last_error.restype = C.c_char_p
+        L.orc_thin_film.argtypes = [fp, C.c_int64, fp]
         L.orc_sobol.restype = C.c_float
         L.orc_sobol.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_sobol_batch.argtypes = [C.POINTER(C.c_uint32)] * 3 + [C.c_int64, fp]
@@ -217,6 +220,14 @@ class OracleScene:
         p = np.ascontiguousarray(p, np.float32)
         wi = np.ascontiguousarray(wi, np.float32)
         return lib().orc_light_pdf(self.h, li, _f(p), _f(wi))
+
+
+def thin_film(args):
+    """ThinFilm.Evaluate over rows (cos, film ior, thickness nm, f0 r, g, b)."""
+    a = np.ascontiguousarray(args, np.float32).reshape(-1, 6)
+    out = np.zeros((len(a), 3), np.float32)
+    lib().orc_thin_film(_f(a), len(a), _f(out))
+    return out
 
 
 def _frame(n, t=None, b=None, front=True, eta=0.0):

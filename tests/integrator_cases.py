@@ -403,3 +403,42 @@ def check_estimator_invariance(means):
     (mb, sb), (mp, sp) = means["balance"], means["power"]
     print(f"Cornell image mean: balance {mb:.5f} +- {sb:.5f}, power {mp:.5f} +- {sp:.5f}", flush=True)
     assert abs(mb - mp) <= 3.5 * np.hypot(sb, sp), (mb, mp, sb, sp)
+
+
+SKY = (0.4, 0.6, 0.9)
+
+
+def sky_scene(W, H, sky=SKY, extra_light=False):
+    """An infinite Lambertian ground (albedo RHO) under a flat sky, which the
+    scene builder turns into an EnvironmentLight; optionally a sphere light."""
+    b = scenes.SceneBuilder()
+    b.plane((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), b.lambertian((RHO, RHO, RHO)))
+    if extra_light:
+        b.sphere_light((0.5, 1.5, -0.5), 0.2, (1.0, 1.0, 1.0), 20.0, shadow_samples=2)
+    cam = scenes.make_camera((0.0, 1.5, 3.0), (0.0, 0.3, 0.0), 60.0, W / H)
+    return b.build(cam, sky=sky)
+
+
+def check_sky_furnace(render, primary, mis, pick, W=32, H=32, spp=16, runs=6, depth=3):
+    """Uniform sky radiance L over an unoccluded ground: the irradiance is
+    pi L, so every ground sample's expected value is rho L per channel (NEE
+    towards the environment light with w_nee, escaping BSDF bounces with the
+    complementary weight; a ground bounce can only escape); sky pixels are L
+    exactly.  A missing MIS complement, a wrong sphere pdf or sample weight,
+    or a horizon test on the wrong side moves the ground by percent."""
+    desc = sky_scene(W, H)
+    L = np.array(SKY)
+    sums = []
+    for k in range(runs):
+        st = unclamped(W, H, spp, depth, mis, pick, seed_xor=(0x27D4EB2F * (k + 1)) & 0xFFFFFFFF)
+        prim = primary(desc, st)[0].reshape(spp, H, W)
+        ground = np.all(prim == 0, 0)
+        sky_px = np.all(prim < 0, 0)
+        img = render(desc, st).astype(np.float64)
+        assert ground.sum() > 100 and sky_px.sum() > 50
+        assert np.allclose(img[:, sky_px].T, L.astype(np.float32), rtol=1e-6), "sky pixels must be the sky"
+        sums.append(img[:, ground].sum(1) / (RHO * L * ground.sum()))
+    sums = np.array(sums)            # (runs, 3): ground mean / (rho L)
+    m, se = sums.mean(0), sums.std(0, ddof=1) / np.sqrt(runs)
+    print(f"sky furnace mis={mis} pick={pick}: ground / (rho L) = {m} +- {se}", flush=True)
+    assert np.all(np.abs(m - 1.0) <= 3.0 * se + 2e-4), (m, se)

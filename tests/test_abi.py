@@ -73,8 +73,14 @@ def test_build_rejects_bad_tables(native):
     bad2 = scenes.SceneDesc(d.prims.copy(), d.mats, d.lights, d.camera)
     bad2.prims["kind"][0] = 17
     assert L.rt_debug_bvh_dump(bad2.ptr, None, 0) < 0
+    # a flat sky is accepted (with or without its environment light)
     sky = scenes.SceneDesc(d.prims, d.mats, d.lights, d.camera, sky=(0.5, 0.5, 0.5))
-    assert L.rt_debug_bvh_dump(sky.ptr, None, 0) < 0
+    assert L.rt_debug_bvh_dump(sky.ptr, None, 0) > 0
+    badl = np.zeros(1, scenes.LIGHT_DTYPE)
+    badl["kind"] = 9
+    badl["shadow_samples"] = 1
+    bad3 = scenes.SceneDesc(d.prims, d.mats, badl, d.camera)
+    assert L.rt_debug_bvh_dump(bad3.ptr, None, 0) < 0
     assert L.rt_debug_bvh_dump(None, None, 0) < 0
 
 

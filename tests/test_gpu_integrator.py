@@ -85,3 +85,44 @@ def test_reference_caustic_tests_on_the_device(native, gpu_ctx):
     assert 0.85 <= ratio <= 1.25
     assert (lon > 0.99).sum() <= 250
     assert bon[:top].mean() > 1.05 * boff[:top].mean()
+
+
+@pytest.mark.parametrize("mis,pick", [("balance", "all"), ("power", "all"), ("balance", "uniform")])
+def test_sky_furnace(native, gpu_ctx, mis, pick):
+    render, primary = _gpu_fns(native, gpu_ctx)
+    IC.check_sky_furnace(render, primary, mis, pick, spp=32)
+
+
+def test_environment_light_same_streams(native, oracle, gpu_ctx):
+    """A flat sky with its EnvironmentLight over occluding geometry (Cornell
+    walls without a ceiling, both spheres, a sphere light): the device follows
+    the oracle's paths (same sampler streams), so pixels agree to fp32
+    re-association."""
+    import numpy as np
+    from raytracer import abi, scenes
+    W = H = 64
+    base = scenes.cornell(W, H)
+    keep = [i for i in range(len(base.prims)) if not (2 <= i <= 3)]   # drop the ceiling triangles
+    b = scenes.SceneBuilder()
+    b.add_materials(base.mats)
+    for i in keep:
+        p = base.prims[i]
+        if p["kind"] == abi.RT_PRIM_TRIANGLE:
+            v = p["v"][:9].reshape(3, 3)
+            b.triangle(v[0], v[1], v[2], int(p["material"]))
+        elif p["kind"] == abi.RT_PRIM_SPHERE:
+            b.sphere(tuple(p["v"][:3]), float(p["v"][3]), int(p["material"]))
+    b.sphere_light((0.3, 1.6, 0.3), 0.1, (1.0, 0.9, 0.8), 25.0, shadow_samples=2)
+    desc = b.build(base.camera, sky=(0.5, 0.7, 1.0))
+    assert desc.lights["kind"][-1] == abi.RT_LIGHT_ENVIRONMENT
+    st = abi.settings_defaults(W, H, 16, 6)
+    render, primary = _gpu_fns(native, gpu_ctx)
+    g = render(desc, st).astype(np.float64)
+    c = oracle.OracleScene(desc).render(st)["beauty"].astype(np.float64)
+    lum = lambda x: 0.2126 * x[0] + 0.7152 * x[1] + 0.0722 * x[2]
+    lg, lc = lum(g), lum(c)
+    rel = np.abs(lg - lc) / np.maximum(np.maximum(lg, lc), 1e-3)
+    frac = (rel <= 1e-3).mean()
+    mrel = abs(lg.mean() - lc.mean()) / lc.mean()
+    print(f"environment light same streams: {frac:.4f} of pixels within 1e-3, mean rel {mrel:.2e}", flush=True)
+    assert frac >= 0.97 and mrel <= 2e-3

@@ -4,7 +4,11 @@ recursive restatement of Renderer.TraceRay.
 Two kinds of checks:
   * same sampler streams (seed_xor = 0 on both sides): the two integrators
     follow the same paths, so pixels agree up to fp32 re-association
-    (tolerances below, per pixel);
+    (tolerances below, per pixel).  The few pixels outside the per-pixel
+    tolerance (at most 0.4 % on the Disney scenes, none on the Cornell
+    boxes) hold a path whose sampled direction differs in its last bits —
+    the device's cosf / sinf / expf / powf are not glibc's — and then
+    diverges; the image mean still agrees to 1e-4;
   * independent streams (different seed_xor): per-pixel |mu_gpu - mu_cpu| <=
     3 sigma for >= 99% of pixels and none beyond 6 sigma, sigma from the run
     to run spread of several independent renders on each side.
@@ -17,7 +21,7 @@ from raytracer import abi, scenes
 pytestmark = pytest.mark.gpu
 
 SAME_STREAM_REL = 1e-3     # per-pixel relative tolerance (same sampler streams)
-SAME_STREAM_FRAC = 0.97    # fraction of pixels that must meet it
+SAME_STREAM_FRAC = 0.99    # fraction of pixels that must meet it (see the module docstring)
 MEAN_REL = 2e-3            # image-mean relative tolerance
 
 

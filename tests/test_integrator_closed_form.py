@@ -55,3 +55,19 @@ def test_planar_emitters_avoid_the_self_shadow_rule():
 def test_estimator_invariance_cornell(oracle):
     render, _ = _oracle_fns(oracle)
     IC.check_estimator_invariance(IC.estimator_means(render))
+
+
+@pytest.mark.parametrize("mis,pick", [("balance", "all"), ("power", "all"), ("balance", "uniform")])
+def test_sky_furnace(oracle, mis, pick):
+    render, primary = _oracle_fns(oracle)
+    IC.check_sky_furnace(render, primary, mis, pick)
+
+
+def test_sky_environment_light_record(oracle):
+    """SceneLoader adds the EnvironmentLight last, with the override's shadow
+    samples or one; a black sky adds none."""
+    from raytracer import abi
+    d = IC.sky_scene(8, 8, extra_light=True)
+    assert list(d.lights["kind"]) == [abi.RT_LIGHT_SPHERE, abi.RT_LIGHT_ENVIRONMENT]
+    assert d.lights[1]["shadow_samples"] == 1
+    assert len(IC.sky_scene(8, 8, sky=(0.0, 0.0, 0.0)).lights) == 0

@@ -125,7 +125,9 @@ def test_disney_fields_and_ground_material():
     assert m["sheen_tint"] == np.float32(0.5) and m["clearcoat_gloss"] == 1.0   # MaterialData defaults
     assert d.medium == ((float(np.float32(0.1)), 0.0, float(np.float32(0.2))), (0.5, 0.5, 0.5))
     kinds = list(d.lights["kind"])
-    assert kinds == [abi.RT_LIGHT_DIRECTIONAL, abi.RT_LIGHT_SPHERE]        # the unknown light is skipped
+    # the unknown light is skipped; the default sky (no world.sky) is sampled
+    # by the EnvironmentLight SceneLoader appends last
+    assert kinds == [abi.RT_LIGHT_DIRECTIONAL, abi.RT_LIGHT_SPHERE, abi.RT_LIGHT_ENVIRONMENT]
     assert d.lights[0]["shadow_samples"] == 0                               # directional: auto
     assert d.lights[1]["radius"] == np.float32(0.5)                          # radius <= 0: 0.5
     assert d.prims["flags"][d.lights[1]["primitive"]] == abi.RT_PRIMF_CAMERA_INVISIBLE
@@ -145,3 +147,34 @@ def test_disney_fields_and_ground_material():
 def test_unsupported_features_are_refused(doc, what):
     with pytest.raises(scene_loader.SceneLoadError, match=what):
         scene_loader.load_data(doc, 8, 8)
+
+
+def test_rotated_surface_frames_are_refused_where_they_matter():
+    """A sphere or plane baked into world space keeps the UV / tangent frame
+    of its world-space shape, while the reference's Transform wrapper turns
+    it with the entity: rotating one whose material reads that frame (image
+    texture, anisotropic Disney) is refused; a plain material loads."""
+    mats = [{"id": "plain", "type": "lambertian", "color": [0.5, 0.5, 0.5]},
+            {"id": "aniso", "type": "disney", "color": [0.5, 0.5, 0.5], "anisotropic": 0.6}]
+
+    def doc(typ, mat, rotate):
+        e = {"type": typ, "material": mat, "rotate": rotate}
+        return {"lights": [], "materials": mats, "entities": [e]}
+
+    for typ in ("sphere", "plane"):
+        scene_loader.load_data(doc(typ, "plain", [0, 30, 0]), 8, 8)
+        scene_loader.load_data(doc(typ, "aniso", [0, 360, 0]), 8, 8)
+        with pytest.raises(scene_loader.SceneLoadError, match=f"rotated {typ}"):
+            scene_loader.load_data(doc(typ, "aniso", [0, 30, 0]), 8, 8)
+
+
+@pytest.mark.parametrize("key,value,ok", [("uv_scale", [2, 2], False), ("uv_scale", [1, 1], True),
+                                          ("uv_offset", [0.5, 0], False), ("uv_rotation", 15, False),
+                                          ("uv_rotation", 0, True)])
+def test_ground_uv_transform_is_refused(key, value, ok):
+    doc = {"lights": [], "world": {"ground": {"y": 0, key: value}}}
+    if ok:
+        scene_loader.load_data(doc, 8, 8)
+    else:
+        with pytest.raises(scene_loader.SceneLoadError, match=key):
+            scene_loader.load_data(doc, 8, 8)
