@@ -2285,6 +2285,21 @@ __global__ void __launch_bounds__(kBlock) k_occluded(DevScene S, const float* __
     occ[p] = b ? 1 : 0;
 }
 
+// Staging of rt_occluded_noalloc: the segments (32 B each) at offset 0, the
+// results (4 B each) at kNoAllocMaxRays * 32, then the traversal-overflow
+// flag after the last result.  One copy per GPU, alive until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for the device (rt_ctx::scratch).
+constexpr int64_t kNoAllocMaxRays = 1 << 20;
+constexpr size_t kNoAllocResults = (size_t)kNoAllocMaxRays * 32;
+constexpr size_t kNoAllocBytes = kNoAllocResults + 4 * ((size_t)kNoAllocMaxRays + 64);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+__global__ void __launch_bounds__(kBlock) k_zero_i32(int32_t* x, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] = 0;
+}
+
 }  // namespace rt
 
 // ═════════════════════════════════════════════════════════════════════════
@@ -2321,6 +2336,7 @@ struct rt_ctx {
     size_t io_bytes = 0;
     void* dn_mem = nullptr;                   // denoiser buffers (grow-only)
     size_t dn_bytes = 0;
+    char* scratch = nullptr;                  // this device's copy of rt::scratch (hipGetSymbolAddress)
 };
 
 struct rt_scene {
@@ -2866,6 +2882,15 @@ int rt_create(int32_t device, rt_ctx** out) {
         delete ctx;
         return RT_ERR_DEVICE;
     }
+    // the device address of this GPU's static staging array (the array's
+    // name in host code is not a device address)
+    void* scr = nullptr;
+    if (hipGetSymbolAddress(&scr, HIP_SYMBOL(rt::scratch)) != hipSuccess || !scr) {
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return RT_ERR_DEVICE;
+    }
+    ctx->scratch = (char*)scr;
     bool ok = true;
     for (auto& e : ctx->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
     for (auto& set : ctx->iev)
@@ -3087,6 +3112,37 @@ int rt_occluded(rt_ctx* ctx, const rt_scene* sc, const float* rays, int64_t num_
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(mem);
     if (e != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string("occluded: ") + hipGetErrorString(e));
+    if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
+    return RT_OK;
+}
+
+int rt_occluded_noalloc(rt_ctx* ctx, const rt_scene* sc, const float* rays, int64_t num_rays, int32_t* occluded) {
+    if (!ctx || !sc || num_rays < 0 || (num_rays > 0 && (!rays || !occluded))) return RT_ERR_INVALID_ARGUMENT;
+    if (num_rays == 0) return RT_OK;
+    if (num_rays > kNoAllocMaxRays)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT,
+                    "rt_occluded_noalloc: at most " + std::to_string(kNoAllocMaxRays) + " segments per call");
+    if (!ctx->scratch) return fail(ctx, RT_ERR_DEVICE, "rt_occluded_noalloc: no staging on this device");
+    hipSetDevice(ctx->device);
+    const int n = (int)num_rays;
+    DevScene S = dev_view(sc);
+    float* d_rays = (float*)ctx->scratch;
+    int32_t* d_occ = (int32_t*)(ctx->scratch + kNoAllocResults);
+    // the overflow flag after the results starts at zero (a fill kernel: the
+    // call may sit in captured work); every result is written by the kernel
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(kBlock), 0, ctx->stream, d_occ + n, 4);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(d_rays, rays, (size_t)n * 32, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+        DISPATCH_SD(sc->stack_depth, k_occluded, grid, ctx->stream, S, d_rays, n, d_occ);
+        e = hipGetLastError();
+    }
+    int32_t ovf = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&ovf, d_occ + n, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(occluded, d_occ, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string("occluded_noalloc: ") + hipGetErrorString(e));
     if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
     return RT_OK;
 }

@@ -63,6 +63,7 @@ def load():
     L.rt_trace_primary.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.c_int32, C.c_int32, ip, fp, fp, fp]
     L.rt_trace_rays.argtypes = [vp, vp, fp, C.c_int64, ip, fp, fp, fp]
     L.rt_occluded.argtypes = [vp, vp, fp, C.c_int64, ip]
+    L.rt_occluded_noalloc.argtypes = [vp, vp, fp, C.c_int64, ip]
     i64p = C.POINTER(C.c_int64)
     L.rt_photon_map.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.c_int64, C.c_int64, i64p, i64p, i64p, fp, fp, fp,
                                 ip, ip, ip, fp]
@@ -86,7 +87,8 @@ EXPORTED_SYMBOLS = ("rt_get_version", "rt_material_defaults", "rt_settings_defau
                     "rt_last_error", "rt_scene_create", "rt_scene_destroy", "rt_scene_get_info",
                     "rt_scene_set_textures", "rt_render",
                     "rt_trace_primary", "rt_trace_rays", "rt_photon_map", "rt_render_batch", "rt_context_tune",
-                    "rt_adjoint", "rt_occluded", "rt_exr_encode", "rt_pfm_encode", "rt_png_encode",
+                    "rt_adjoint", "rt_occluded", "rt_occluded_noalloc", "rt_exr_encode", "rt_pfm_encode",
+                    "rt_png_encode",
                     "rt_denoise",
                     "rt_debug_bvh_dump")
 

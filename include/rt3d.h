@@ -448,6 +448,16 @@ int rt_trace_primary(rt_ctx* ctx, const rt_scene* scene, const rt_settings* sett
  * (src/RayTracer/Rendering/ShadowRay.cs:59-158, BvhNode.Hit with the segment). */
 int rt_occluded(rt_ctx* ctx, const rt_scene* scene, const float* rays, int64_t num_rays, int32_t* occluded);
 
+/* rt_occluded without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and answers as rt_occluded; the segments and results are
+ * staged in the library's static per-GPU array (about 36 MiB, held until the
+ * process ends), so at most 1,048,576 segments per call (more is
+ * RT_ERR_INVALID_ARGUMENT).  Calls that overlap on one GPU share that array
+ * and race: serialise them. */
+int rt_occluded_noalloc(rt_ctx* ctx, const rt_scene* scene, const float* rays, int64_t num_rays, int32_t* occluded);
+
 /* Closest hit of arbitrary rays (origin xyz, direction xyz per ray, 6 floats)
  * against the scene, tMin = 1e-4, tMax = float.MaxValue, exactly as
  * world.Hit(ray, MathUtils.Epsilon, MathUtils.Infinity) (Renderer.cs:1629).
