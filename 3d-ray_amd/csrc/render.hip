@@ -54,6 +54,7 @@ struct DevScene {
     int medium;
     int opaque_shadows;
     int env_light;              // first EnvironmentLight of the list, -1 without one
+    int thin_film;              // some Disney material has a thin film
     f3 sky;                     // flat sky radiance (FlatSky.EvaluateRadiance)
     int caustics;               // photon map active: gather at diffuse/glossy hits, suppress L S+ re-hits
     int block_spec;             // ShadowRay.BlockSpecularTransmission
@@ -137,6 +138,7 @@ struct OneJob {
 };
 struct ManyJobs {
     const JobDev* J;
+    int thin_film;              // some job's scene has a thin-film material (host-side launch choice)
     __device__ __forceinline__ int job_of(const int32_t* job, int p) const { return job[p]; }
     __device__ __forceinline__ const DevScene& scene(int j) const { return J[j].S; }
     __device__ __forceinline__ const RParams& params(int j) const { return J[j].R; }
@@ -1119,7 +1121,7 @@ __device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, i
 // weight (segment weight applied), sampler dimension and pending-shadow
 // flags stored; PH = kPhBounce picks them up.  Sampler dimensions are drawn
 // in the same order either way.
-template <class JOBS, int WPE, bool ADJ = false, int PH = kPhAll>
+template <class JOBS, int WPE, bool ADJ = false, int PH = kPhAll, bool TF = true>
 __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, int maxs) {
     static_assert(!ADJ || PH == kPhAll, "the adjoint replay shades whole vertices");
     constexpr bool kFront = PH != kPhBounce;    // medium, emission, NEE, gather
@@ -1315,14 +1317,93 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             }
         };
 
+        // ── the vertex before its light samples ─────────────────────────
+        f3 ps3 = o;                 // medium scattering point
+        Surf s{};
+        GMat g{};
+        float tp[kTexChannels];
+        float2 tuv = make_float2(0.0f, 0.0f);
+        bool textured = false, image = false, scatters = false, has_brdf = false;
+        int kind = -1, mflags = 0;
+        const f3 thr_s = thr * mb;
+        const f3 vw = primary ? splat(1.0f) : wt;   // weight of this vertex's own terms
+        f3 view = splat(0.0f);
+        bool nee = false;           // the vertex samples the lights
         if (kFront && scat) {
             // ── medium scattering event (Renderer.cs:1703-1770) ─────────
-            f3 ps3 = o + d * tmed;
+            ps3 = o + d * tmed;
             if (R.aovs && (aovf & kAovPending)) {
                 if (!(aovf & kAovHaveDepth)) { av1.w = tmed * length(d); aovf |= kAovHaveDepth; }
                 aov_commit(splat(1.0f), splat(0.0f));
             }
-            sample_lights(ps3, splat(0.0f), true, primary ? splat(1.0f) : wt, true);
+            nee = true;
+        } else if (kFront && !hit) {
+            // ── miss: the flat sky (SampleSky, Renderer.cs:1820-1861), weighed
+            // against the environment light's NEE by MIS after a non-delta
+            // bounce when the light list holds one
+            const f3 sky = S.sky;
+            if (R.aovs && (aovf & kAovPending)) aov_commit(clamp01(sky), splat(0.0f));
+            if (sky.x > 0.0f || sky.y > 0.0f || sky.z > 0.0f) {
+                f3 sw = sky;
+                if (!prev_delta && S.env_light >= 0 && S.lights[S.env_light].l0.w) {
+                    const float pl = kUniformSpherePdf;
+                    sw = prev_pdf + pl <= 1e-20f ? splat(0.0f) : sky * mis_w(prev_pdf, pl, R.power_mis);
+                }
+                add_target(P, p, tgt, sanitize((primary ? splat(1.0f) : wt) * sw));
+            }
+        } else {
+            // ── surface (ShadeSurface, Renderer.cs:1008-1228) ──────────
+            s = make_surf(S, he, o, d, htuv.x, htuv.y, htuv.z);
+            g = S.mats[s.mat];
+            textured = (g.m0.y & kMatTextured) != 0;
+            image = (g.m0.y & kMatImage) != 0;
+            // textures and images feed the BSDF and the albedo AOV of a
+            // scattering vertex: the bounce phase's work
+            if (kBack && (textured || image)) {
+                tuv = tex_coord(S, he, o, d, htuv);
+                // camera hits carry the footprint k_fp left; later hits are point-sampled
+                const float4 fpv = image && P.fp ? P.fp[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                g = textured_gmat(S, g, s.mat, tuv, tp, image, fpv);
+            }
+            kind = g.m0.x;
+            mflags = g.m0.y;
+            scatters = kind == RT_MAT_LAMBERTIAN || kind == RT_MAT_DISNEY || kind == RT_MAT_DIELECTRIC;
+            // tentative AOVs (AovProbe, Renderer.cs:777-815): in the phase
+            // that holds the vertex's textured material
+            const bool aov_here = PH == kPhAll || (PH == kPhNee ? !scatters : true);
+            if (aov_here && R.aovs && (aovf & kAovPending)) {
+                P.av2[p] = make_float4(s.n.x, s.n.y, s.n.z, 0.0f);
+                f3 ta = clamp01(f3of(g.m1));
+                av1 = make_float4(ta.x, ta.y, ta.z, av1.w);
+            }
+            // emission with MIS (WeightEmission, Renderer.cs:1549-1565)
+            if (kFront && kind == RT_MAT_EMISSIVE && s.front && !(S.caustics && cst == kCstSuppress)) {
+                f3 raw = f3of(g.m1);
+                if (raw.x > 0.0f || raw.y > 0.0f || raw.z > 0.0f) {
+                    f3 em = raw;
+                    int lix = g.m0.z;
+                    if (!prev_delta && lix >= 0) {
+                        float pl = light_pdf(S.lights[lix], o, d);
+                        if (prev_pdf + pl <= 1e-20f) em = splat(0.0f);
+                        else em = raw * mis_w(prev_pdf, pl, R.power_mis);
+                    }
+                    add_target(P, p, tgt, sanitize(vw * em));
+                }
+            }
+            view = normalize(-d);
+            if ((kBack || S.caustics) && (mflags & kMatDielectricIor)) {
+                float top = ior_n > 0 ? P.ior[(int64_t)(ior_n - 1) * P.n + p] : 1.0f;
+                float enc = ior_n >= 2 ? P.ior[(int64_t)(ior_n - 2) * P.n + p] : 1.0f;
+                float mi = g.m3.w;
+                if (mi > 0.0f) s.rel_eta = s.front ? (top / mi) : (mi / enc);
+            }
+            has_brdf = kind == RT_MAT_LAMBERTIAN || kind == RT_MAT_DISNEY;
+            nee = kFront && kind != RT_MAT_EMISSIVE;
+        }
+        // NEE light samples (one site for medium and surface vertices)
+        if (kFront && nee) sample_lights(scat ? ps3 : s.p, scat ? splat(0.0f) : s.n, scat, vw, scat || has_brdf);
+        if (kFront && scat) {
+            // ── roulette and phase sampling of a medium vertex
             f3 mt = thr * mb;
             float iscale = 1.0f;
             bool kill = false;
@@ -1350,68 +1431,10 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                 next_cs = kCstDiffuse;
                 cont = true;
             }
-        } else if (kFront && !hit) {
-            // ── miss: the flat sky (SampleSky, Renderer.cs:1820-1861), weighed
-            // against the environment light's NEE by MIS after a non-delta
-            // bounce when the light list holds one
-            const f3 sky = S.sky;
-            if (R.aovs && (aovf & kAovPending)) aov_commit(clamp01(sky), splat(0.0f));
-            if (sky.x > 0.0f || sky.y > 0.0f || sky.z > 0.0f) {
-                f3 sw = sky;
-                if (!prev_delta && S.env_light >= 0 && S.lights[S.env_light].l0.w) {
-                    const float pl = kUniformSpherePdf;
-                    sw = prev_pdf + pl <= 1e-20f ? splat(0.0f) : sky * mis_w(prev_pdf, pl, R.power_mis);
-                }
-                add_target(P, p, tgt, sanitize((primary ? splat(1.0f) : wt) * sw));
-            }
-        } else {
-            // ── surface (ShadeSurface, Renderer.cs:1008-1228) ──────────
-            Surf s = make_surf(S, he, o, d, htuv.x, htuv.y, htuv.z);
-            GMat g = S.mats[s.mat];
-            float tp[kTexChannels];
-            float2 tuv = make_float2(0.0f, 0.0f);
-            const bool textured = (g.m0.y & kMatTextured) != 0;
-            const bool image = (g.m0.y & kMatImage) != 0;
-            if (textured || image) {
-                tuv = tex_coord(S, he, o, d, htuv);
-                // camera hits carry the footprint k_fp left; later hits are point-sampled
-                const float4 fpv = image && P.fp ? P.fp[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                g = textured_gmat(S, g, s.mat, tuv, tp, image, fpv);
-            }
-            int kind = g.m0.x;
-            int mflags = g.m0.y;
-            f3 thr_s = thr * mb;
-            f3 vw = primary ? splat(1.0f) : wt;   // weight of this vertex's own terms
-            if (kFront && R.aovs && (aovf & kAovPending)) {
-                P.av2[p] = make_float4(s.n.x, s.n.y, s.n.z, 0.0f);
-                f3 ta = clamp01(f3of(g.m1));
-                av1 = make_float4(ta.x, ta.y, ta.z, av1.w);
-            }
-            // emission with MIS (WeightEmission, Renderer.cs:1549-1565)
-            if (kFront && kind == RT_MAT_EMISSIVE && s.front && !(S.caustics && cst == kCstSuppress)) {
-                f3 raw = f3of(g.m1);
-                if (raw.x > 0.0f || raw.y > 0.0f || raw.z > 0.0f) {
-                    f3 em = raw;
-                    int lix = g.m0.z;
-                    if (!prev_delta && lix >= 0) {
-                        float pl = light_pdf(S.lights[lix], o, d);
-                        if (prev_pdf + pl <= 1e-20f) em = splat(0.0f);
-                        else em = raw * mis_w(prev_pdf, pl, R.power_mis);
-                    }
-                    add_target(P, p, tgt, sanitize(vw * em));
-                }
-            }
-            f3 view = normalize(-d);
-            if ((kBack || S.caustics) && (mflags & kMatDielectricIor)) {
-                float top = ior_n > 0 ? P.ior[(int64_t)(ior_n - 1) * P.n + p] : 1.0f;
-                float enc = ior_n >= 2 ? P.ior[(int64_t)(ior_n - 2) * P.n + p] : 1.0f;
-                float mi = g.m3.w;
-                if (mi > 0.0f) s.rel_eta = s.front ? (top / mi) : (mi / enc);
-            }
-            const bool has_brdf = kind == RT_MAT_LAMBERTIAN || kind == RT_MAT_DISNEY;
+        } else if (hit) {
+            // ── the surface vertex after its light samples
             if constexpr (kFront) {
-                // ── NEE light samples; caustic gather request (Renderer.cs:1108-1109)
-                if (kind != RT_MAT_EMISSIVE) sample_lights(s.p, s.n, false, vw, has_brdf);
+                // caustic gather request (Renderer.cs:1108-1109)
                 if (S.caustics && has_brdf) {
                     gather = true;
                     P.gq0[p] = make_float4(o.x, o.y, o.z, htuv.x);
@@ -1420,11 +1443,16 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                     P.gq3[p] = make_float4(vw.x, vw.y, vw.z, __int_as_float(tgt ? 1 : 0));
                 }
                 // the NEE phase hands scattering vertices to the bounce phase
-                if (PH == kPhNee) bounce = has_brdf || kind == RT_MAT_DIELECTRIC;
+                if (PH == kPhNee) bounce = scatters;
             }
             if constexpr (kBack) {
                 DisneyP DP;
-                if (kind == RT_MAT_DISNEY) DP = load_disney(g);
+                if (kind == RT_MAT_DISNEY) {
+                    DP = load_disney(g);
+                    // kernels for scenes without thin films: no film (the
+                    // iridescent Fresnel compiles out)
+                    if constexpr (!TF) DP.tf_thick = 0.0f;
+                }
                 // adjoint weight of this vertex's terms: dLoss/dPixel * w0 / spp
                 DisneyD DD;
                 f3 adj_w = splat(0.0f);
@@ -2426,6 +2454,7 @@ static DevScene dev_view(const rt_scene* sc) {
     S.num_lights = (int)h.lights.size();
     S.medium = h.medium ? 1 : 0;
     S.env_light = h.env_light;
+    S.thin_film = h.thin_film ? 1 : 0;
     S.sky = h.sky;
     S.opaque_shadows = h.opaque_shadows ? 1 : 0;
     S.cam_o = h.cam_origin;
@@ -2511,6 +2540,9 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
                        int stack_depth, const DevScene* gather_scene, const DevPhotons* M, const WindowPlan& plan,
                        ACCUM accum, LoopTiming& tm) {
     const int64_t N = P.n;
+    bool thin_film;
+    if constexpr (std::is_same<JOBS, OneJob>::value) thin_film = jobs.S.thin_film != 0;
+    else thin_film = jobs.thin_film != 0;
     const int pool_blocks = (int)((N + kBlock - 1) / kBlock);
     const int shadow_blocks = (int)(((int64_t)N * maxs + kBlock - 1) / kBlock);
     size_t pbytes = sizeof(PartDev) * std::max<size_t>(plan.parts.size(), 1);
@@ -2538,18 +2570,22 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
                 if constexpr (std::is_same<JOBS, OneJob>::value)
                     hipLaunchKernelGGL((k_shade<JOBS, 1, true>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
             } else {
-                // split shade: NEE phase, then the bounce phase
-                switch (ctx->shade_wpe) {
-                    case 1: hipLaunchKernelGGL((k_shade<JOBS, 1, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
-                    case 2: hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
-                    case 4: hipLaunchKernelGGL((k_shade<JOBS, 4, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
-                    default: hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
-                }
-                switch (ctx->bounce_wpe) {
-                    case 1: hipLaunchKernelGGL((k_shade<JOBS, 1, false, kPhBounce>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
-                    case 2: hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
-                    case 4: hipLaunchKernelGGL((k_shade<JOBS, 4, false, kPhBounce>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs); break;
-                    default: hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhBounce>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                // split shade: NEE phase, then the bounce phase (with the
+                // thin-film Fresnel only when a scene of the launch has one)
+                if (ctx->shade_wpe <= 2)
+                    hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                else
+                    hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                if (thin_film) {
+                    if (ctx->bounce_wpe <= 2)
+                        hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce, true>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                    else
+                        hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhBounce, true>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                } else {
+                    if (ctx->bounce_wpe <= 2)
+                        hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce, false>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                    else
+                        hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhBounce, false>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
                 }
             }
             if (e) HIP_TRY(hipEventRecord(e[2], ctx->stream));
@@ -3789,7 +3825,8 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
     hipEvent_t* ev = ctx->ev;
     HIP_TRY(hipEventRecord(ev[6], ctx->stream));
     LoopTiming tm;
-    ManyJobs many{(const JobDev*)ctx->jobs_mem};
+    ManyJobs many{(const JobDev*)ctx->jobs_mem, 0};
+    for (int j = 0; j < n_jobs; j++) many.thin_film |= jd[j].S.thin_film;
     auto accum = [&](const PartDev& pt) {
         const int64_t npix = pt.npix;
         hipLaunchKernelGGL(k_accum, dim3((unsigned)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,

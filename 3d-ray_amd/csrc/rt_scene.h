@@ -40,6 +40,7 @@ struct HostScene {
     float lens_radius = 0.0f;
     f3 sky;
     int env_light = -1;                // first EnvironmentLight of the list (Renderer.cs:426)
+    bool thin_film = false;            // some Disney material has a thin film
     bool medium = false;
     bool opaque_shadows = true;        // no material transmits shadow rays
     f3 sig_a, sig_s, sig_t;

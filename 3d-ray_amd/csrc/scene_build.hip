@@ -37,7 +37,8 @@ static_assert(sizeof(rt_material) == 128, "rt_material layout");
 static_assert(sizeof(rt_light) == 80, "rt_light layout");
 static_assert(sizeof(rt_camera) == 56, "rt_camera layout");
 static_assert(sizeof(rt_medium) == 32, "rt_medium layout");
-static_assert(sizeof(rt_scene_desc) == 200, "rt_scene_desc layout");
+static_assert(sizeof(rt_scene_desc) == 248, "rt_scene_desc layout");
+static_assert(sizeof(rt_node) == 160, "rt_node layout");
 static_assert(sizeof(rt_image) == 24, "rt_image layout");
 static_assert(sizeof(rt_settings) == 96, "rt_settings layout");
 static_assert(sizeof(rt_film) == 88, "rt_film layout");
@@ -566,6 +567,7 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
     }
     const int64_t np = d->num_primitives;
     if (np >= (int64_t)kLeafStartMask) { err = "too many primitives"; return RT_ERR_UNSUPPORTED; }
+    if (d->num_nodes != 0) { err = "object hierarchies are not built yet"; return RT_ERR_UNSUPPORTED; }
     hs = HostScene();
     hs.num_prims = np;
     build_energy_lut(hs.lut_e, hs.lut_eavg);
@@ -575,6 +577,7 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
     for (int64_t i = 0; i < d->num_materials; i++) {
         int k = d->materials[i].kind;
         if (k < RT_MAT_LAMBERTIAN || k > RT_MAT_DISNEY) { err = "bad material kind"; return RT_ERR_INVALID_ARGUMENT; }
+        if (k == RT_MAT_DISNEY && d->materials[i].thin_film_thickness > 0.0f) hs.thin_film = true;
         if (k == RT_MAT_DISNEY && d->materials[i].thin_film_thickness > 0.0f &&
             (d->materials[i].flags & RT_MATF_TEXTURED)) {
             // the texture adjoint's derivatives cover the Schlick Fresnel only

@@ -68,7 +68,19 @@ class rt_scene_desc(C.Structure):
                 ("camera", rt_camera), ("sky_color", F3), ("reserved0", C.c_int32),
                 ("global_medium", rt_medium), ("texcoords", C.POINTER(C.c_float)),
                 ("texels", C.POINTER(C.c_float)), ("tex_width", C.c_int32), ("tex_height", C.c_int32),
-                ("normals", C.POINTER(C.c_float)), ("images", C.c_void_p), ("num_images", C.c_int64)]
+                ("normals", C.POINTER(C.c_float)), ("images", C.c_void_p), ("num_images", C.c_int64),
+                ("nodes", C.c_void_p), ("num_nodes", C.c_int64), ("node_children", C.POINTER(C.c_int32)),
+                ("num_node_children", C.c_int64), ("roots", C.POINTER(C.c_int32)), ("num_roots", C.c_int64)]
+
+
+RT_NODE_PRIMITIVES, RT_NODE_GROUP, RT_NODE_TRANSFORM, RT_NODE_INSTANCE = 0, 1, 2, 3
+RT_NODEF_CAMERA_INVISIBLE = 1
+
+
+class rt_node(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("flags", C.c_int32), ("first", C.c_int32), ("count", C.c_int32),
+                ("child", C.c_int32), ("material", C.c_int32), ("reserved", C.c_int32 * 2),
+                ("matrix", C.c_float * 16), ("inverse", C.c_float * 16)]
 
 
 class rt_image(C.Structure):
@@ -141,7 +153,7 @@ class rt_denoise_options(C.Structure):
 
 
 for _t, _n in ((rt_primitive, 64), (rt_material, 128), (rt_light, 80), (rt_camera, 56),
-               (rt_medium, 32), (rt_settings, 96)):
+               (rt_medium, 32), (rt_settings, 96), (rt_node, 160)):
     assert C.sizeof(_t) == _n, (_t.__name__, C.sizeof(_t))
 
 

@@ -33,7 +33,11 @@ MAT_DTYPE = np.dtype([("kind", "<i4"), ("flags", "<i4"), ("color", "<f4", (3,)),
 LIGHT_DTYPE = np.dtype([("kind", "<i4"), ("shadow_samples", "<i4"), ("primitive", "<i4"), ("material", "<i4"),
                         ("color", "<f4", (3,)), ("intensity", "<f4"), ("position", "<f4", (3,)), ("radius", "<f4"),
                         ("u", "<f4", (3,)), ("soft_radius", "<f4"), ("v", "<f4", (3,)), ("reserved", "<f4")])
+NODE_DTYPE = np.dtype([("kind", "<i4"), ("flags", "<i4"), ("first", "<i4"), ("count", "<i4"), ("child", "<i4"),
+                       ("material", "<i4"), ("reserved", "<i4", (2,)), ("matrix", "<f4", (16,)),
+                       ("inverse", "<f4", (16,))])
 assert PRIM_DTYPE.itemsize == 64 and MAT_DTYPE.itemsize == 128 and LIGHT_DTYPE.itemsize == 80
+assert NODE_DTYPE.itemsize == 160
 
 _DISNEY_DEFAULTS = dict(metallic=0.0, roughness=0.5, specular=0.5, specular_tint=0.0, sheen=0.0, sheen_tint=0.5,
                         sheen_roughness=0.3, clearcoat=0.0, clearcoat_gloss=1.0, spec_trans=0.0, ior=1.5,
@@ -56,7 +60,7 @@ class SceneDesc:
     """Owns the numpy tables behind one rt_scene_desc (keeps them alive)."""
 
     def __init__(self, prims, mats, lights, camera, sky=(0.0, 0.0, 0.0), medium=None, texcoords=None,
-                 texture=None, normals=None, images=None):
+                 texture=None, normals=None, images=None, hierarchy=None):
         self.prims = np.ascontiguousarray(prims, PRIM_DTYPE)
         self.mats = np.ascontiguousarray(mats, MAT_DTYPE)
         self.lights = np.ascontiguousarray(lights, LIGHT_DTYPE)
@@ -104,6 +108,19 @@ class SceneDesc:
                 r.scale_u, r.scale_v = sc
             d.images = C.cast(self._image_recs, C.c_void_p)
             d.num_images = len(self.images)
+        # object hierarchy: (nodes NODE_DTYPE, group child lists int32, roots int32)
+        self.hierarchy = None
+        if hierarchy is not None:
+            nodes, children, roots = hierarchy
+            self.hierarchy = (np.ascontiguousarray(nodes, NODE_DTYPE), np.ascontiguousarray(children, np.int32),
+                              np.ascontiguousarray(roots, np.int32))
+            n, ch, r = self.hierarchy
+            d.nodes = n.ctypes.data if len(n) else None
+            d.num_nodes = len(n)
+            d.node_children = C.cast(ch.ctypes.data, C.POINTER(C.c_int32)) if len(ch) else None
+            d.num_node_children = len(ch)
+            d.roots = C.cast(r.ctypes.data, C.POINTER(C.c_int32)) if len(r) else None
+            d.num_roots = len(r)
         self.desc = d
 
     @property
@@ -133,6 +150,12 @@ class SceneBuilder:
         self._images = []         # image textures: (rgba8, (scale_u, scale_v))
         self.geometry_shadow_samples = 4
         self.shadow_samples_override = None
+        # object hierarchy (rt_node): top-level items in object order, each a
+        # flat object [first, count] or a root node
+        self._objects = []        # [first, count, claimed]
+        self._top = []            # ("obj", object index) | ("node", node index)
+        self._nodes = []
+        self._children = []
 
     # ── materials ───────────────────────────────────────────────────────
     def add_materials(self, recs):
@@ -209,7 +232,69 @@ class SceneBuilder:
         first = self._count
         self._prims.append(p)
         self._count += 1
+        self._object(first, 1)
         return first
+
+    def _object(self, first, count):
+        self._top.append(("obj", len(self._objects)))
+        self._objects.append([first, count, False])
+
+    # ── object hierarchy (rt_node: Group / Transform / Instance) ────────
+    def _node(self, **kw):
+        n = np.zeros(1, NODE_DTYPE)
+        n["child"] = -1
+        n["material"] = -1
+        for k, v in kw.items():
+            n[k] = v
+        self._nodes.append(n)
+        return len(self._nodes) - 1
+
+    def node_prims(self, first):
+        """The object starting at primitive `first` (a lone primitive or a
+        mesh) as a PRIMITIVES node; it leaves the top-level object list."""
+        for o in self._objects:
+            if o[0] == first:
+                if o[2]:
+                    raise ValueError("object already wrapped")
+                o[2] = True
+                return self._node(kind=abi.RT_NODE_PRIMITIVES, first=first, count=o[1])
+        raise ValueError(f"no object starts at primitive {first}")
+
+    def node_group(self, children, camera_invisible=False):
+        first = len(self._children)
+        self._children.extend(int(c) for c in children)
+        return self._node(kind=abi.RT_NODE_GROUP, first=first, count=len(children),
+                          flags=abi.RT_NODEF_CAMERA_INVISIBLE if camera_invisible else 0)
+
+    def node_transform(self, child, matrix, inverse=None, camera_invisible=False):
+        """Transform(child, matrix): matrix is the 4x4 object-to-world matrix
+        in System.Numerics layout (row vectors, translation in row 4); the
+        inverse defaults to the float64 inverse rounded to float32."""
+        m = np.asarray(matrix, np.float32).reshape(4, 4)
+        inv = (np.linalg.inv(m.astype(np.float64)).astype(np.float32) if inverse is None
+               else np.asarray(inverse, np.float32).reshape(4, 4))
+        return self._node(kind=abi.RT_NODE_TRANSFORM, child=child, matrix=m.ravel(), inverse=inv.ravel(),
+                          flags=abi.RT_NODEF_CAMERA_INVISIBLE if camera_invisible else 0)
+
+    def node_instance(self, child, material=-1):
+        return self._node(kind=abi.RT_NODE_INSTANCE, child=child, material=material)
+
+    def add_root(self, node):
+        """Appends `node` to the world's object list at this point of the
+        object order."""
+        self._top.append(("node", int(node)))
+
+    def _hierarchy(self):
+        if not self._nodes:
+            return None
+        roots = []
+        for kind, i in self._top:
+            if kind == "node":
+                roots.append(i)
+            elif not self._objects[i][2]:
+                first, count, _ = self._objects[i]
+                roots.append(self._node(kind=abi.RT_NODE_PRIMITIVES, first=first, count=count))
+        return (np.concatenate(self._nodes), np.asarray(self._children, np.int32), np.asarray(roots, np.int32))
 
     def triangle(self, v0, v1, v2, mat, flags=0, normals=None, uvs=None):
         """A Triangle, or a SmoothTriangle when vertex normals (n0, n1, n2)
@@ -233,6 +318,8 @@ class SceneBuilder:
         p["v"][:, :9] = tris
         self._prims.append(p)
         self._count += len(tris)
+        for k in range(len(tris)):
+            self._object(first + k, 1)
         return first
 
     def mesh(self, vertices, faces, mats, normals=None, normal_faces=None, uvs=None, uv_faces=None):
@@ -272,6 +359,7 @@ class SceneBuilder:
         p["v"][:, :9] = tris
         self._prims.append(p)
         self._count += len(tris)
+        self._object(first, len(tris))
         return first
 
     def sphere(self, center, radius, mat, flags=0):
@@ -364,7 +452,7 @@ class SceneBuilder:
             lights.append(env)
         lights = np.concatenate(lights) if lights else np.zeros(0, LIGHT_DTYPE)
         return SceneDesc(prims, mats, lights, camera, sky, medium, texcoords=texcoords, normals=normals,
-                         images=self._images)
+                         images=self._images, hierarchy=self._hierarchy())
 
 
 # ═══════════════════════════════════════════════════════════════════════

@@ -132,17 +132,18 @@ def scenes_settings(width, height, spp, depth):
     return abi.settings_defaults(width, height, spp, depth)
 
 
-def run_batch(args):
-    """Batched workload: every scene rendered once per scene (plain: one
-    rt_render call each) and once all together (rt_render_batch, one path
-    pool); both at the scenes' full spp, so the films are identical and the
-    RMSE equal by construction.  value = plain time / batched time."""
+def batch_compare(count, steps, warmup, tune, ctx=None):
+    """Plain (one rt_render call per scene) against batched (rt_render_batch,
+    one path pool) over the first `count` scenes of the seeded batch
+    workload, both at the scenes' full spp, so the films are identical and
+    the RMSE equal by construction.  Returns the timings and the batched
+    run's per-stage kernel times."""
     import ctypes as C
     from raytracer import abi, autotune
     L = native.load()
-    log(f"generating {args.batch_scenes} scenes")
-    work = scenes.batch_workload(count=args.batch_scenes)
-    ctx = native.Context(0)
+    log(f"generating {count} scenes")
+    work = scenes.batch_workload(count=count)
+    ctx = ctx or native.Context(0)
     t0 = time.perf_counter()
     items = []
     for j in work:
@@ -153,7 +154,7 @@ def run_batch(args):
     log(f"{len(items)} scenes resident ({ntri} triangles), build+upload {time.perf_counter() - t0:.1f} s; "
         f"{samples / 1e6:.0f}M camera samples")
     tuned = None
-    if not args.no_tune:
+    if tune:
         log("autotuning the batched form")
         tuned = autotune.tune(items, batch=True, log=log)
         log(f"autotune: {tuned}")
@@ -171,12 +172,12 @@ def run_batch(args):
         return total
 
     log("warmup")
-    for k in range(args.warmup):
+    for k in range(warmup):
         plain()
         batched()
     tp = tb = 0.0
     total = None
-    for k in range(args.steps):
+    for k in range(steps):
         t0 = time.perf_counter()
         plain()
         tp += time.perf_counter() - t0
@@ -184,9 +185,20 @@ def run_batch(args):
         total = batched()
         tb += time.perf_counter() - t0
         log(f"step {k}: plain {tp / (k + 1) * 1e3:.0f} ms, batched {tb / (k + 1) * 1e3:.0f} ms")
-    tp /= args.steps
-    tb /= args.steps
+    tp /= steps
+    tb /= steps
     per = {"trace": total.trace_ms, "shadow": total.shadow_ms, "shade": total.shade_ms}
+    return {"scenes": len(items), "triangles": ntri, "camera_samples": samples, "autotune": tuned,
+            "plain_s": tp, "batched_s": tb, "kernel_ms_batched": per}
+
+
+def run_batch(args):
+    """Batched workload: value = plain time / batched time over the batch
+    workload's scenes (films identical by construction)."""
+    b = batch_compare(args.batch_scenes, args.steps, args.warmup, not args.no_tune)
+    tp, tb, samples, ntri, per = b["plain_s"], b["batched_s"], b["camera_samples"], b["triangles"], \
+        b["kernel_ms_batched"]
+    tuned = b["autotune"]
     out = {
         "metric": "batched/plain speedup at equal RMSE",
         "value": round(tp / tb, 3),
@@ -200,14 +212,14 @@ def run_batch(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded procedural scenes)",
-        "config": {"workload": f"{len(items)} scenes, {ntri} triangles, log-uniform 10k-2M per scene, resolutions "
+        "config": {"workload": f"{b['scenes']} scenes, {ntri} triangles, log-uniform 10k-2M per scene, resolutions "
                                f"256^2..1920x1080, spp 4/16/64, depth 4/8/12, 1-8 sphere lights",
-                   "scenes": len(items), "camera_samples": samples, "triangles": ntri, "autotune": tuned},
+                   "scenes": b["scenes"], "camera_samples": samples, "triangles": ntri, "autotune": tuned},
         "plain_msamples_per_s": round(samples / tp / 1e6, 2),
         "batched_msamples_per_s": round(samples / tb / 1e6, 2),
         "plain_ms": round(tp * 1e3, 1),
         "batched_ms": round(tb * 1e3, 1),
-        "films": "identical per scene (bit-exact, tests/test_gpu_batch.py)",
+        "films": "identical per scene (bit-exact, tests/test_gpu_batch.py, tests/test_gpu_fullsize.py)",
         "kernel_ms_batched": {k: round(v, 1) for k, v in per.items()},
         "roofline": None,
         "cpu_baseline": None,
@@ -452,6 +464,9 @@ def main():
                          "1080p multilayer EXR and PNG encoding; denoise: NFOR / NL-means on a 1080p film")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
+    ap.add_argument("--batch-check", type=int, default=16, metavar="SCENES",
+                    help="icosphere: also time the first SCENES scenes of the batch workload plain vs batched "
+                         "(one step) and report the ratio beside the headline; 0 skips it")
     ap.add_argument("--photons", type=int, default=20_000_000)
     ap.add_argument("--gather-radius", type=float, default=0.01)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
@@ -602,6 +617,22 @@ def main():
         except Exception as e:  # the baseline must not hide the GPU result
             log(f"cpu baseline failed: {e}")
 
+    batch = None
+    if args.batch_check and not caus:
+        log(f"batched vs plain over {args.batch_check} scenes of the batch workload")
+        try:
+            b = batch_compare(args.batch_check, 1, 1, False, ctx)
+            batch = {"metric": "batched/plain speedup at equal RMSE", "value": round(b["plain_s"] / b["batched_s"], 3),
+                     "unit": "x", "scenes": b["scenes"], "triangles": b["triangles"],
+                     "camera_samples": b["camera_samples"], "plain_ms": round(b["plain_s"] * 1e3, 1),
+                     "batched_ms": round(b["batched_s"] * 1e3, 1),
+                     "plain_msamples_per_s": round(b["camera_samples"] / b["plain_s"] / 1e6, 2),
+                     "batched_msamples_per_s": round(b["camera_samples"] / b["batched_s"] / 1e6, 2),
+                     "note": "first scenes of the seeded 64-scene workload, one timed step after one warmup, "
+                             "library-default pool; films identical per scene"}
+        except Exception as e:  # the side measurement must not hide the headline
+            log(f"batch check failed: {e}")
+
     out = {
         "metric": "Msamples/s caustics+volumes 1080p vs CPU" if caus else "Msamples/s @1M-tri 1080p vs CPU",
         "value": round(value, 2),
@@ -626,6 +657,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if batch:
+        out["batched_vs_plain"] = batch
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(out), flush=True)

@@ -235,7 +235,68 @@ typedef struct rt_scene_desc {
     /* Image textures of RT_MATF_IMAGE materials (ImageTexture.cs:42-289). */
     const struct rt_image* images;
     int64_t num_images;
+    /* Object hierarchy (optional; rt_node below).  num_nodes == 0: the world
+     * list is the flat object list (each primitive, or each run of
+     * consecutive primitives sharing a mesh id, is one object, in order).
+     * num_nodes > 0: the world list is nodes[roots[0 .. num_roots)], in that
+     * order (SceneLoader.cs:460-487 assembles it into the world BVH plus the
+     * infinite planes); node_children holds the child lists of GROUP nodes. */
+    const struct rt_node* nodes;
+    int64_t num_nodes;
+    const int32_t* node_children;
+    int64_t num_node_children;
+    const int32_t* roots;
+    int64_t num_roots;
 } rt_scene_desc;
+
+/* ── Object hierarchy: the reference's IHittable wrappers ─────────────────
+ * A node is one IHittable of the loaded scene.  Nodes may be shared: an
+ * INSTANCE's template (SceneLoader.cs:3259-3337 builds a template once and
+ * wraps it in one Instance per use) is one node referenced by every
+ * instance, so its BVHs and primitives are resident once.
+ *   PRIMITIVES  primitive `first` when count == 1 and its mesh id < 0, else
+ *               the `count` faces [first, first + count) of one mesh (a Mesh:
+ *               nested BVH over > 4 faces, a list otherwise, Mesh.cs:75-116);
+ *   GROUP       Group.cs:20-60: children node_children[first .. first+count);
+ *               a BVH over the finite children when more than 4, the
+ *               infinite planes after it;
+ *   TRANSFORM   Transform.cs:74-129: `child` hit in object space — the ray
+ *               mapped by `inverse`, the hit point by `matrix`, normals by
+ *               the transpose of `inverse` (normalised), tangents and DpDu /
+ *               DpDv by `matrix` (MapHitToWorld, :162-207); its box is the
+ *               child's box with its 8 corners mapped by `matrix`;
+ *   INSTANCE    Instance.cs:54-82: `child` unchanged, with every hit's
+ *               material replaced by `material` when it is >= 0.
+ * Matrices use System.Numerics.Matrix4x4's field order M11, M12, ... M44
+ * and its row-vector convention: a point maps to
+ * (x M11 + y M21 + z M31 + M41, ...), evaluated as Vector3.Transform does
+ * (fma(x, row1, fma(y, row2, fma(z, row3, row4)))); a direction drops row4.
+ * `inverse` is the host's Matrix4x4.Invert of `matrix` (passed, not
+ * recomputed, so the ray transform uses the host's inverse bit for bit).
+ * RT_NODEF_CAMERA_INVISIBLE wraps the node in CameraInvisibleHittable.
+ * Limits: wrapper nesting (TRANSFORM / INSTANCE / camera-invisible nodes on
+ * one root-to-primitive path) at most 4; a GROUP's list part at most 15
+ * members; no GEOMETRY light, caustics, rt_adjoint or image texture in a
+ * scene with wrappers (RT_ERR_UNSUPPORTED). */
+typedef enum rt_node_kind {
+    RT_NODE_PRIMITIVES = 0,
+    RT_NODE_GROUP = 1,
+    RT_NODE_TRANSFORM = 2,
+    RT_NODE_INSTANCE = 3
+} rt_node_kind;
+#define RT_NODEF_CAMERA_INVISIBLE 1
+
+typedef struct rt_node {
+    int32_t kind;           /* rt_node_kind */
+    int32_t flags;          /* RT_NODEF_* */
+    int32_t first;          /* PRIMITIVES: first primitive; GROUP: first index into node_children */
+    int32_t count;          /* PRIMITIVES: primitive count; GROUP: child count */
+    int32_t child;          /* TRANSFORM / INSTANCE: the wrapped node */
+    int32_t material;       /* INSTANCE: override material, -1 = the template's own materials */
+    int32_t reserved[2];
+    float matrix[16];       /* TRANSFORM: object -> world, M11 .. M44 */
+    float inverse[16];      /* TRANSFORM: Matrix4x4.Invert(matrix) as the host computed it */
+} rt_node;              /* 160 bytes */
 
 /* One ImageTexture: the decoded 8-bit image (ImageSharp's Rgba32 rows, top
  * row first; alpha ignored) and the YAML uv_scale.  The library decodes
@@ -402,8 +463,8 @@ int rt_adjoint(rt_ctx* ctx, const rt_scene* scene, const rt_settings* settings, 
                float* d_texels, rt_render_stats* stats);
 
 /* Kernel configuration of a context (no reference counterpart; set by an
- * autotuner): shade_waves_per_simd = minimum waves per SIMD the shading
- * kernel is compiled for (1-4), traversal_stack = 1 private scratch, 2 LDS,
+ * autotuner): shade_waves_per_simd = minimum waves per SIMD the NEE shading
+ * kernel is compiled for (2 or 3; 1 runs as 2 and 4 as 3), traversal_stack = 1 private scratch, 2 LDS,
  * bounce_graph = 1 direct launches, 2 replay of hipGraph-captured bounce
  * iterations (no per-kernel timings then); 0 keeps a setting.  Results do
  * not depend on any of them. */

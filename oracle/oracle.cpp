@@ -113,7 +113,13 @@ struct Hittable {
     virtual ~Hittable() {}
     virtual bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const = 0;
     virtual AABB box() const = 0;
+    // SceneLoader.IsInfinitePlane (SceneLoader.cs:3640-3650): the world
+    // list's test, which sees through Transform, CameraInvisible and a Group
+    // of planes
     virtual bool is_plane() const { return false; }
+    // Group.IsInfinitePlane (Group.cs:88-94): only InfinitePlane and a
+    // Transform of one
+    virtual bool is_plane_in_group() const { return false; }
 };
 
 // Triangle.cs:35-89,144-149
@@ -435,6 +441,7 @@ struct Plane : Hittable {
     }
     AABB box() const override { return AABB(V3(-1e6f), V3(1e6f)); }
     bool is_plane() const override { return true; }
+    bool is_plane_in_group() const override { return true; }
 };
 
 // CameraInvisibleHittable: stamps the camera bit of the visibility mask.
@@ -618,6 +625,107 @@ struct Mesh : Hittable {
     ~Mesh() override { delete accel; }
     bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override { return accel->hit(r, tmin, tmax, rec); }
     AABB box() const override { return accel->box(); }
+};
+
+// ── Wrappers of the object hierarchy (rt_node) ──────────────────────────
+// System.Numerics.Vector3.Transform / TransformNormal with Matrix4x4 in its
+// field order M11..M44 (row vectors): the third-party runtime (.NET 10,
+// RayTracer.csproj TargetFramework net10.0) evaluates them in
+// DirectXMath's XMVector3Transform order with multiply-adds,
+// fma(x, row1, fma(y, row2, fma(z, row3, row4))); TransformNormal drops
+// row4 and starts from z * row3.
+static V3 xf_point(V3 p, const float* M) {
+    return V3(std::fmaf(p.x, M[0], std::fmaf(p.y, M[4], std::fmaf(p.z, M[8], M[12]))),
+              std::fmaf(p.x, M[1], std::fmaf(p.y, M[5], std::fmaf(p.z, M[9], M[13]))),
+              std::fmaf(p.x, M[2], std::fmaf(p.y, M[6], std::fmaf(p.z, M[10], M[14]))));
+}
+static V3 xf_vec(V3 v, const float* M) {
+    return V3(std::fmaf(v.x, M[0], std::fmaf(v.y, M[4], v.z * M[8])),
+              std::fmaf(v.x, M[1], std::fmaf(v.y, M[5], v.z * M[9])),
+              std::fmaf(v.x, M[2], std::fmaf(v.y, M[6], v.z * M[10])));
+}
+
+// Transform.cs:74-240: the child hit in object space (ToLocalRay), the
+// record mapped back (MapHitToWorld); the world box is the child box's 8
+// corners mapped by the matrix (ComputeWorldBox).
+struct TransformH : Hittable {
+    Hittable* inner;
+    float M[16], I[16], N[16];          // matrix, inverse, normal matrix (transposed inverse)
+    AABB wbox;
+    TransformH(Hittable* h, const float* m, const float* inv) : inner(h) {
+        for (int k = 0; k < 16; k++) { M[k] = m[k]; I[k] = inv[k]; }
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++) N[4 * r + c] = inv[4 * c + r];
+        AABB b = inner->box();
+        V3 mn(FLT_MAX), mx(-FLT_MAX);
+        for (int k = 0; k < 8; k++) {
+            V3 c((k & 4) ? b.mx.x : b.mn.x, (k & 2) ? b.mx.y : b.mn.y, (k & 1) ? b.mx.z : b.mn.z);
+            V3 t = xf_point(c, M);
+            mn = V3(std::fmin(mn.x, t.x), std::fmin(mn.y, t.y), std::fmin(mn.z, t.z));
+            mx = V3(std::fmax(mx.x, t.x), std::fmax(mx.y, t.y), std::fmax(mx.z, t.z));
+        }
+        wbox = AABB(mn, mx);
+    }
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override {
+        Ray local(xf_point(r.o, I), xf_vec(r.d, I));
+        if (!inner->hit(local, tmin, tmax, rec)) return false;
+        rec.point = xf_point(rec.point, M);
+        rec.normal = normalize(xf_vec(rec.normal, N));
+        rec.tangent = normalize(xf_vec(rec.tangent, M));
+        rec.bitangent = normalize(xf_vec(rec.bitangent, M));
+        if (dot(rec.dpdu, rec.dpdu) > 0.0f) rec.dpdu = xf_vec(rec.dpdu, M);
+        if (dot(rec.dpdv, rec.dpdv) > 0.0f) rec.dpdv = xf_vec(rec.dpdv, M);
+        return true;
+    }
+    AABB box() const override { return wbox; }
+    bool is_plane() const override { return inner->is_plane(); }
+    bool is_plane_in_group() const override { return inner->is_plane_in_group(); }
+};
+
+// Instance.cs:54-84: the shared template's hit with the material override.
+struct InstanceH : Hittable {
+    Hittable* inner;
+    const Material* override_mat;
+    InstanceH(Hittable* h, const Material* m) : inner(h), override_mat(m) {}
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override {
+        if (!inner->hit(r, tmin, tmax, rec)) return false;
+        if (override_mat) rec.mat = override_mat;
+        return true;
+    }
+    AABB box() const override { return inner->box(); }
+};
+
+// Group.cs:20-80: a BVH over the finite children when more than 4, the
+// infinite planes after it; one member is hit directly.
+struct GroupH : Hittable {
+    std::vector<Hittable*> children;
+    Hittable* root = nullptr;
+    HittableList* list = nullptr;
+    Hittable* bvh = nullptr;
+    explicit GroupH(std::vector<Hittable*> ch) : children(std::move(ch)) {
+        std::vector<Hittable*> fin, inf, roots;
+        for (Hittable* c : children) (c->is_plane_in_group() ? inf : fin).push_back(c);
+        if (fin.size() > 4) {
+            bvh = build_bvh(fin);
+            roots.push_back(bvh);
+        } else roots.insert(roots.end(), fin.begin(), fin.end());
+        roots.insert(roots.end(), inf.begin(), inf.end());
+        if (roots.size() == 1) root = roots[0];
+        else {
+            list = new HittableList();
+            list->objs = roots;
+            root = list;
+        }
+    }
+    ~GroupH() override { delete bvh; delete list; }
+    bool hit(const Ray& r, float tmin, float tmax, HitRecord& rec) const override { return root->hit(r, tmin, tmax, rec); }
+    AABB box() const override { return root->box(); }
+    bool is_plane() const override {
+        if (children.empty()) return false;
+        for (Hittable* c : children)
+            if (!c->is_plane()) return false;
+        return true;
+    }
 };
 
 // ═════════════════════════════════════════════════════════════════════════
@@ -3254,6 +3362,7 @@ static Scene* build_scene(const rt_scene_desc* d) {
         g_err = "null table";
         return nullptr;
     }
+    if (d->num_nodes != 0) { g_err = "object hierarchies are not built yet"; return nullptr; }
     auto sc = std::make_unique<Scene>();
     if (d->num_images < 0 || (d->num_images > 0 && !d->images)) { g_err = "bad image table"; return nullptr; }
     for (int64_t k = 0; k < d->num_images; k++) {

@@ -38,7 +38,7 @@ def test_struct_sizes():
     assert C.sizeof(abi.rt_light) == 80
     assert C.sizeof(abi.rt_settings) == 96
     # the same sizes are static_asserts in the library (scene_build.hip)
-    sizes = {"rt_camera": 56, "rt_medium": 32, "rt_scene_desc": 200, "rt_image": 24, "rt_film": 88, "rt_render_stats": 152,
+    sizes = {"rt_camera": 56, "rt_medium": 32, "rt_scene_desc": 248, "rt_node": 160, "rt_image": 24, "rt_film": 88, "rt_render_stats": 152,
              "rt_scene_info": 96, "rt_batch_job": 32, "rt_exr_channel": 24, "rt_denoise_options": 32}
     for name, size in sizes.items():
         assert C.sizeof(getattr(abi, name)) == size, name
