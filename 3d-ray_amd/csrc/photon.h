@@ -416,16 +416,22 @@ struct DevPhotons {
     int n;
 };
 
+// Max-heap of the k nearest photons (PhotonMap's insertion order and sift
+// rules).  Entry j of the heap lives at hi[j * S], hd[j * S]: the gather
+// keeps one heap per lane in LDS, lane-interleaved (S = the block size).
+template <int S>
 __device__ __forceinline__ void heap_insert(int* hi, float* hd, int& count, int pi, float d2) {
     if (count < kGatherK) {
         int i = count++;
-        hi[i] = pi;
-        hd[i] = d2;
+        hi[i * S] = pi;
+        hd[i * S] = d2;
         while (i > 0) {
             int parent = (i - 1) >> 1;
-            if (hd[parent] >= hd[i]) break;
-            float tf = hd[parent]; hd[parent] = hd[i]; hd[i] = tf;
-            int ti = hi[parent]; hi[parent] = hi[i]; hi[i] = ti;
+            const float hp = hd[parent * S], hc = hd[i * S];
+            if (hp >= hc) break;
+            hd[parent * S] = hc;
+            hd[i * S] = hp;
+            int ti = hi[parent * S]; hi[parent * S] = hi[i * S]; hi[i * S] = ti;
             i = parent;
         }
     } else if (d2 < hd[0]) {
@@ -434,11 +440,13 @@ __device__ __forceinline__ void heap_insert(int* hi, float* hd, int& count, int 
         int i = 0;
         for (;;) {
             int l = 2 * i + 1, r = 2 * i + 2, lg = i;
-            if (l < count && hd[l] > hd[lg]) lg = l;
-            if (r < count && hd[r] > hd[lg]) lg = r;
+            float hlg = hd[i * S];
+            if (l < count && hd[l * S] > hlg) { lg = l; hlg = hd[l * S]; }
+            if (r < count && hd[r * S] > hlg) { lg = r; hlg = hd[r * S]; }
             if (lg == i) break;
-            float tf = hd[lg]; hd[lg] = hd[i]; hd[i] = tf;
-            int ti = hi[lg]; hi[lg] = hi[i]; hi[i] = ti;
+            hd[lg * S] = hd[i * S];
+            hd[i * S] = hlg;
+            int ti = hi[lg * S]; hi[lg * S] = hi[i * S]; hi[i * S] = ti;
             i = lg;
         }
     }
@@ -446,6 +454,7 @@ __device__ __forceinline__ void heap_insert(int* hi, float* hd, int& count, int 
 
 // k nearest photons within max_r of p; returns the count and the kernel
 // radius (distance of the farthest gathered photon).
+template <int S>
 __device__ int photon_knn(const DevPhotons& M, f3 p, int* hi, float* hd, float& radius) {
     radius = 0.0f;
     if (M.n <= 0) return 0;
@@ -478,7 +487,7 @@ __device__ int photon_knn(const DevPhotons& M, f3 p, int* hi, float* hd, float& 
                         if (px != x || py != y || pz != z) continue;
                         float d2 = length_sq(pos - p);
                         if (d2 > max_r2) continue;
-                        heap_insert(hi, hd, count, pi, d2);
+                        heap_insert<S>(hi, hd, count, pi, d2);
                     }
                 }
         float reach = (float)ring * M.cell;

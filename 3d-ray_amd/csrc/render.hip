@@ -71,6 +71,9 @@ struct DevScene {
     const GImage* images;
     const GLevel* img_levels;
     const float4* img_texels;
+    // object hierarchy: wrapper transforms (8 float4 each); null = the scene
+    // has no wrapper entries
+    const float4* xf;
 };
 
 struct RParams {
@@ -139,6 +142,7 @@ struct OneJob {
 struct ManyJobs {
     const JobDev* J;
     int thin_film;              // some job's scene has a thin-film material (host-side launch choice)
+    int hier;                   // some job's scene has an object hierarchy (host-side launch choice)
     __device__ __forceinline__ int job_of(const int32_t* job, int p) const { return job[p]; }
     __device__ __forceinline__ const DevScene& scene(int j) const { return J[j].S; }
     __device__ __forceinline__ const RParams& params(int j) const { return J[j].R; }
@@ -212,6 +216,7 @@ struct PathBufs {
     float* ior;                 // 8 × N (slot k of path p at k*N + p)
     int32_t* hit_e;
     float4* hit_tuv;
+    int4* hit_xf;               // wrapper chain of the hit (scenes with an object hierarchy)
     float4 *av0, *av1, *av2, *av3, *av4;   // AOV probe
     float4 *sh_o, *sh_d, *sh_c, *sh_r;     // shadow records, record k of slot p at k*N + p (N × maxs)
     int32_t* state;             // per slot: kSlotIdle, kSlotActive (to trace), kSlotRetired
@@ -346,6 +351,48 @@ __device__ __forceinline__ Surf make_surf(const DevScene& S, int32_t eidx, f3 o,
         s.bitan = cross(n, ua);
     }
     return s;
+}
+
+// Surface of a hit found under wrapper entries (rt_node): the primitive's
+// Hit in the frame of the innermost transform, then MapHitToWorld through
+// each transform outward (point by the matrix, normal by the transposed
+// inverse, tangent frame by the matrix, each normalised; Transform.cs:162-207)
+// and the outermost material override last (Instance.cs:71-82).  lo / ld
+// receive the ray of the hit's frame (texture coordinates are read there).
+__device__ __forceinline__ Surf make_surf_w(const DevScene& S, int32_t eidx, int4 chain, f3 o, f3 d, float t,
+                                            float u, float v, f3& lo, f3& ld) {
+    lo = o;
+    ld = d;
+    chain_ray(S.entries, S.xf, chain, lo, ld);
+    Surf s = make_surf(S, eidx, lo, ld, t, u, v);
+    const int32_t c[4] = {chain.x, chain.y, chain.z, chain.w};
+#pragma unroll
+    for (int k = kMaxWrap - 1; k >= 0; k--) {
+        if (c[k] < 0) continue;
+        const float4 ec = S.entries[c[k]].c;
+        const int32_t xi = __float_as_int(ec.x), om = __float_as_int(ec.y);
+        if (xi >= 0) {
+            const float4* M = S.xf + 8 * xi;
+            s.p = xf_point(s.p, M);
+            s.n = normalize(xf_normal(s.n, M + 4));
+            s.tan = normalize(xf_vec(s.tan, M));
+            s.bitan = normalize(xf_vec(s.bitan, M));
+        }
+        if (om >= 0) s.mat = om;
+    }
+    return s;
+}
+
+// A hit's camera visibility: its primitive's flag or a camera-invisible
+// wrapper above it.
+__device__ __forceinline__ bool hit_cam_invisible(const DevScene& S, int32_t e, int4 chain) {
+    if ((tag_of(S.entries[e].a) >> 8) & kEntryCamInvisible) return true;
+    if (!S.xf) return false;
+    const int32_t c[4] = {chain.x, chain.y, chain.z, chain.w};
+#pragma unroll
+    for (int k = 0; k < kMaxWrap; k++)
+        if (c[k] >= 0 && ((tag_of(S.entries[c[k]].a) >> 8) & kEntryCamInvisible)) return true;
+    return false;
 }
 
 // Texture coordinate of a hit: the primitive's interpolated vertex UVs when
@@ -961,7 +1008,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_end(PathBufs P) {
 
 // Traversal stack placement: LDS (lane-interleaved) or the wave's private
 // scratch (PRIV); chosen per context.
-template <class JOBS, int SD, bool COUNT, bool PRIV>
+template <class JOBS, int SD, bool COUNT, bool PRIV, bool XF = false>
 __global__ void __launch_bounds__(kBlock) k_trace(JOBS jobs, PathBufs P) {
     __shared__ uint2 lds_stk[PRIV ? 1 : SD * kBlock];
     uint2 priv_stk[PRIV ? SD : 1];
@@ -980,18 +1027,26 @@ __global__ void __launch_bounds__(kBlock) k_trace(JOBS jobs, PathBufs P) {
         HitOut h;
         bool moved = false;
         for (int skip = 0;;) {
-            h = trace_closest<SD, STRIDE, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
-                                                 kInfinity, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
+            h = trace_closest<SD, STRIDE, COUNT, false, XF>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d,
+                                                            kEps, kInfinity, stk, (uint32_t*)&P.counters[kCntOverflow],
+                                                            tc, S.xf);
             if (h.entry < 0 || !camera) break;
-            int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
-            if (!(fl & kEntryCamInvisible)) break;
-            o = offset_origin(o + h.t * d, d);
+            if (XF) {
+                if (!hit_cam_invisible(S, h.entry, h.chain)) break;
+                f3 lo, ld;
+                o = offset_origin(make_surf_w(S, h.entry, h.chain, o, d, h.t, h.u, h.v, lo, ld).p, d);
+            } else {
+                int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
+                if (!(fl & kEntryCamInvisible)) break;
+                o = offset_origin(o + h.t * d, d);
+            }
             moved = true;
             if (++skip >= 8) break;
         }
         if (moved) P.o[p] = make_float4(o.x, o.y, o.z, o4.w);
         P.hit_e[p] = h.entry;
         P.hit_tuv[p] = make_float4(h.t, h.u, h.v, 0.0f);
+        if (XF) P.hit_xf[p] = h.chain;
     });
     if (COUNT) flush_counts(P.tcount, tc);
 }
@@ -1353,14 +1408,16 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             }
         } else {
             // ── surface (ShadeSurface, Renderer.cs:1008-1228) ──────────
-            s = make_surf(S, he, o, d, htuv.x, htuv.y, htuv.z);
+            f3 lo = o, ld = d;      // the ray of the hit's frame
+            if (S.xf) s = make_surf_w(S, he, P.hit_xf[p], o, d, htuv.x, htuv.y, htuv.z, lo, ld);
+            else s = make_surf(S, he, o, d, htuv.x, htuv.y, htuv.z);
             g = S.mats[s.mat];
             textured = (g.m0.y & kMatTextured) != 0;
             image = (g.m0.y & kMatImage) != 0;
             // textures and images feed the BSDF and the albedo AOV of a
             // scattering vertex: the bounce phase's work
             if (kBack && (textured || image)) {
-                tuv = tex_coord(S, he, o, d, htuv);
+                tuv = tex_coord(S, he, lo, ld, htuv);
                 // camera hits carry the footprint k_fp left; later hits are point-sampled
                 const float4 fpv = image && P.fp ? P.fp[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 g = textured_gmat(S, g, s.mat, tuv, tp, image, fpv);
@@ -1936,10 +1993,13 @@ __global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPh
     Surf s = make_surf(S, __float_as_int(b.w), o, d, a.w, c.x, c.y);
     s.rel_eta = c.z;
     const int p = __float_as_int(c.w);
-    int hi[kGatherK];
-    float hd[kGatherK];
+    // the lane's heap of the k nearest photons, lane-interleaved in LDS
+    __shared__ int lds_hi[kGatherK * kBlock];
+    __shared__ float lds_hd[kGatherK * kBlock];
+    int* hi = &lds_hi[threadIdx.x];
+    float* hd = &lds_hd[threadIdx.x];
     float radius;
-    int n = photon_knn(M, s.p, hi, hd, radius);
+    int n = photon_knn<kBlock>(M, s.p, hi, hd, radius);
     if (n == 0) return;
     bool under = n < kGatherK;
     if (under) radius = M.max_r;
@@ -1952,7 +2012,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPh
     float inv_r = 1.0f / radius;
     f3 sum = splat(0.0f);
     for (int k = 0; k < n; k++) {
-        int pi = hi[k];
+        int pi = hi[k * kBlock];
         f3 inc = f3of(M.inc[pi]);
         if (dot(inc, s.n) <= 0.0f) continue;
         float wgt = 1.0f - length(f3of(M.pos[pi]) - s.p) * inv_r / 1.0f;
@@ -1972,8 +2032,8 @@ __global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPh
 }
 
 // Transparent shadow walk (ShadowRay.Transmittance, ShadowRay.cs:59-158)
-template <class JOBS, int SD, bool COUNT, bool PRIV>
-__global__ void __launch_bounds__(kBlock, 7) k_shadow(JOBS jobs, PathBufs P, int maxs) {
+template <class JOBS, int SD, bool COUNT, bool PRIV, bool XF = false>
+__global__ void __launch_bounds__(kBlock, XF ? 4 : 7) k_shadow(JOBS jobs, PathBufs P, int maxs) {
     __shared__ uint2 lds_stk[PRIV ? 1 : SD * kBlock];
     uint2 priv_stk[PRIV ? SD : 1];
     uint2* stk = PRIV ? priv_stk : &lds_stk[threadIdx.x];
@@ -2032,9 +2092,9 @@ __global__ void __launch_bounds__(kBlock, 7) k_shadow(JOBS jobs, PathBufs P, int
                     atomicAdd(&P.tcount[kTcAnyOther], (unsigned long long)t4.other);
                 }
             } else
-                blocked = trace_closest<SD, STRIDE, COUNT, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count,
-                                                                 org, dir, kEps, rem, stk,
-                                                                 (uint32_t*)&P.counters[kCntOverflow], tc)
+                blocked = trace_closest<SD, STRIDE, COUNT, true, XF>(S.nodes, S.entries, S.aux, S.top_start,
+                                                                     S.top_count, org, dir, kEps, rem, stk,
+                                                                     (uint32_t*)&P.counters[kCntOverflow], tc, S.xf)
                               .entry >= 0;
             f3 c = blocked ? splat(0.0f) : f3of(P.sh_c[sl]);
             P.sh_r[sl] = make_float4(c.x, c.y, c.z, 0.0f);
@@ -2044,10 +2104,13 @@ __global__ void __launch_bounds__(kBlock, 7) k_shadow(JOBS jobs, PathBufs P, int
         bool in_med = false;
         bool done = false;
         for (int it = 0; it < 8 && !done; it++) {
-            HitOut h = trace_closest<SD, STRIDE, COUNT>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, org, dir,
-                                                        kEps, rem, stk, (uint32_t*)&P.counters[kCntOverflow], tc);
+            HitOut h = trace_closest<SD, STRIDE, COUNT, false, XF>(S.nodes, S.entries, S.aux, S.top_start,
+                                                                   S.top_count, org, dir, kEps, rem, stk,
+                                                                   (uint32_t*)&P.counters[kCntOverflow], tc, S.xf);
             if (h.entry < 0) { done = true; break; }
-            Surf s = make_surf(S, h.entry, org, dir, h.t, h.u, h.v);
+            f3 lo, ld;
+            Surf s = XF ? make_surf_w(S, h.entry, h.chain, org, dir, h.t, h.u, h.v, lo, ld)
+                        : make_surf(S, h.entry, org, dir, h.t, h.u, h.v);
             const GMat g = S.mats[s.mat];
             if (S.block_spec && (g.m0.y & kMatSpecTransmissive)) { thr = splat(0.0f); done = true; break; }
             if (in_med) {
@@ -2255,15 +2318,23 @@ __global__ void __launch_bounds__(kBlock) k_trace_primary(DevScene S, RParams R,
     HitOut h;
     TravCount tc{0u, 0u, 0u};
     for (int skip = 0;;) {
-        h = trace_closest<SD, kBlock, false>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps, kInfinity,
-                                             &stk[threadIdx.x], (uint32_t*)(prim + n), tc);
+        if (S.xf)
+            h = trace_closest<SD, kBlock, false, false, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o,
+                                                              d, kEps, kInfinity, &stk[threadIdx.x],
+                                                              (uint32_t*)(prim + n), tc, S.xf);
+        else
+            h = trace_closest<SD, kBlock, false>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
+                                                 kInfinity, &stk[threadIdx.x], (uint32_t*)(prim + n), tc);
         if (h.entry < 0) break;
-        int32_t fl = tag_of(S.entries[h.entry].a) >> 8;
-        if (!(fl & kEntryCamInvisible)) break;
-        o = offset_origin(o + h.t * d, d);
+        if (!hit_cam_invisible(S, h.entry, h.chain)) break;
+        if (S.xf) {
+            f3 lo, ld;
+            o = offset_origin(make_surf_w(S, h.entry, h.chain, o, d, h.t, h.u, h.v, lo, ld).p, d);
+        } else o = offset_origin(o + h.t * d, d);
         if (++skip >= 8) break;
     }
     float hu, hv;
+    if (S.xf && h.entry >= 0) chain_ray(S.entries, S.xf, h.chain, o, d);
     report_uv(S.entries, h, o, d, hu, hv);
     prim[p] = h.entry < 0 ? -1 : __float_as_int(S.entries[h.entry].b.w);
     to[p] = h.entry < 0 ? 0.0f : h.t;
@@ -2280,8 +2351,15 @@ __global__ void __launch_bounds__(kBlock) k_trace_rays(DevScene S, const float* 
     const float* r = rays + 6 * (int64_t)p;
     TravCount tc{0u, 0u, 0u};
     f3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
-    HitOut h = trace_closest<SD, kBlock, false>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
-                                                kInfinity, &stk[threadIdx.x], (uint32_t*)(prim + n), tc);
+    HitOut h;
+    if (S.xf) {
+        h = trace_closest<SD, kBlock, false, false, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d,
+                                                          kEps, kInfinity, &stk[threadIdx.x], (uint32_t*)(prim + n), tc,
+                                                          S.xf);
+        if (h.entry >= 0) chain_ray(S.entries, S.xf, h.chain, o, d);
+    } else
+        h = trace_closest<SD, kBlock, false>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, kEps,
+                                             kInfinity, &stk[threadIdx.x], (uint32_t*)(prim + n), tc);
     float hu, hv;
     report_uv(S.entries, h, o, d, hu, hv);
     prim[p] = h.entry < 0 ? -1 : __float_as_int(S.entries[h.entry].b.w);
@@ -2304,6 +2382,11 @@ __global__ void __launch_bounds__(kBlock) k_occluded(DevScene S, const float* __
         TravCount tc{0u, 0u, 0u};
         b = trace_any4<SD, kBlock>(S.nodes4, S.entries, S.aux, S.top_start, S.top_count, o, d, r[6], r[7],
                                    &stk[threadIdx.x], (uint32_t*)(occ + n), tc);
+    } else if (S.xf) {
+        TravCount tc{0u, 0u, 0u};
+        b = trace_closest<SD, kBlock, false, true, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d,
+                                                         r[6], r[7], &stk[threadIdx.x], (uint32_t*)(occ + n), tc, S.xf)
+                .entry >= 0;
     } else {
         TravCount tc{0u, 0u, 0u};
         b = trace_closest<SD, kBlock, false, true>(S.nodes, S.entries, S.aux, S.top_start, S.top_count, o, d, r[6], r[7],
@@ -2385,6 +2468,7 @@ struct rt_scene {
     GImage* d_images = nullptr;
     GLevel* d_levels = nullptr;
     float4* d_imtex = nullptr;
+    float4* d_xf = nullptr;
     int stack_depth = 0;
     int64_t device_bytes = 0;
 };
@@ -2477,6 +2561,9 @@ static DevScene dev_view(const rt_scene* sc) {
     S.tex = h.textured ? sc->d_tex : nullptr;
     S.tex_w = h.tex_w;
     S.tex_h = h.tex_h;
+    // non-null for every scene with wrapper entries (the table is allocated
+    // even when no wrapper transforms: Instance / camera-invisible only)
+    S.xf = h.hier ? sc->d_xf : nullptr;
     return S;
 }
 
@@ -2503,6 +2590,18 @@ static DevScene dev_view(const rt_scene* sc) {
         if ((sd) <= 24) hipLaunchKernelGGL((KERNEL<J, 24, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);       \
         else if ((sd) <= 32) hipLaunchKernelGGL((KERNEL<J, 32, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);  \
         else hipLaunchKernelGGL((KERNEL<J, 64, C_, P_>), grid, dim3(kBlock), 0, __VA_ARGS__);                  \
+    } while (0)
+// scenes with wrapper entries: the XF traversal, private-scratch stacks
+#define DISPATCH_SD_X(J, sd, cnt, KERNEL, grid, ...)                                   \
+    do {                                                                                \
+        if (cnt) DISPATCH_SD_TX(J, sd, KERNEL, true, grid, __VA_ARGS__);                \
+        else DISPATCH_SD_TX(J, sd, KERNEL, false, grid, __VA_ARGS__);                   \
+    } while (0)
+#define DISPATCH_SD_TX(J, sd, KERNEL, C_, grid, ...)                                                                 \
+    do {                                                                                                             \
+        if ((sd) <= 24) hipLaunchKernelGGL((KERNEL<J, 24, C_, true, true>), grid, dim3(kBlock), 0, __VA_ARGS__);      \
+        else if ((sd) <= 32) hipLaunchKernelGGL((KERNEL<J, 32, C_, true, true>), grid, dim3(kBlock), 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERNEL<J, 64, C_, true, true>), grid, dim3(kBlock), 0, __VA_ARGS__);                 \
     } while (0)
 #define DISPATCH_SD(sd, KERNEL, grid, ...)                                                       \
     do {                                                                                         \
@@ -2540,9 +2639,14 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
                        int stack_depth, const DevScene* gather_scene, const DevPhotons* M, const WindowPlan& plan,
                        ACCUM accum, LoopTiming& tm) {
     const int64_t N = P.n;
-    bool thin_film;
-    if constexpr (std::is_same<JOBS, OneJob>::value) thin_film = jobs.S.thin_film != 0;
-    else thin_film = jobs.thin_film != 0;
+    bool thin_film, hier;
+    if constexpr (std::is_same<JOBS, OneJob>::value) {
+        thin_film = jobs.S.thin_film != 0;
+        hier = jobs.S.xf != nullptr;
+    } else {
+        thin_film = jobs.thin_film != 0;
+        hier = jobs.hier != 0;
+    }
     const int pool_blocks = (int)((N + kBlock - 1) / kBlock);
     const int shadow_blocks = (int)(((int64_t)N * maxs + kBlock - 1) / kBlock);
     size_t pbytes = sizeof(PartDev) * std::max<size_t>(plan.parts.size(), 1);
@@ -2561,7 +2665,8 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
         if (traced) {
             dim3 grid(pool_blocks);
             if (e) HIP_TRY(hipEventRecord(e[0], ctx->stream));
-            DISPATCH_SD_C(JOBS, stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, jobs, Pw);
+            if (hier) DISPATCH_SD_X(JOBS, stack_depth, counting, k_trace, grid, ctx->stream, jobs, Pw);
+            else DISPATCH_SD_C(JOBS, stack_depth, counting, ctx->priv_stack, k_trace, grid, ctx->stream, jobs, Pw);
             if (e) HIP_TRY(hipEventRecord(e[1], ctx->stream));
             if (Pw.fp) DISPATCH_SD_C(JOBS, stack_depth, false, ctx->priv_stack, k_fp, grid, ctx->stream, jobs, Pw);
             bool adj_replay = false;
@@ -2593,8 +2698,11 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
             hipLaunchKernelGGL(k_compact, grid, dim3(kBlock), 0, ctx->stream, Pw, maxs);
             if (gather_scene) hipLaunchKernelGGL(k_gather, grid, dim3(kBlock), 0, ctx->stream, *gather_scene, Pw, *M);
             if (e) HIP_TRY(hipEventRecord(e[3], ctx->stream));
-            DISPATCH_SD_C(JOBS, stack_depth, counting, ctx->priv_stack, k_shadow, dim3(shadow_blocks), ctx->stream, jobs,
-                          Pw, maxs);
+            if (hier)
+                DISPATCH_SD_X(JOBS, stack_depth, counting, k_shadow, dim3(shadow_blocks), ctx->stream, jobs, Pw, maxs);
+            else
+                DISPATCH_SD_C(JOBS, stack_depth, counting, ctx->priv_stack, k_shadow, dim3(shadow_blocks), ctx->stream,
+                              jobs, Pw, maxs);
             if (e) HIP_TRY(hipEventRecord(e[4], ctx->stream));
             if constexpr (std::is_same<JOBS, OneJob>::value)
                 if (adj_replay)
@@ -2985,6 +3093,9 @@ int rt_scene_create(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
     // Stack bound: deepest world path + a leaf continuation + the deepest
     // nested mesh path (build_host_scene records the maximum over all BVHs).
     sc->stack_depth = h.world_depth + (h.mesh_depth >= 0 ? 1 + h.mesh_depth : 0) + 1;
+    // an object hierarchy: nested BVHs, leaf continuations and one frame
+    // marker per wrapper along the deepest path
+    if (h.hier) sc->stack_depth = std::max(sc->stack_depth, h.hier_stack + 1);
     // the any-hit BVH needs up to 3 pushes per level; use it only when its
     // bound fits the stacks the binary traversal's depth class provides
     if (h.any_stack > 64) h.nodes4.clear();
@@ -3014,6 +3125,7 @@ int rt_scene_create(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
     if ((rc = upload(ctx, &sc->d_levels, h.img_levels.data(), h.img_levels.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_imtex, h.img_texels.data(), h.img_texels.size(), b))) return cleanup(rc);
     if ((rc = upload(ctx, &sc->d_tex, h.texels.data(), h.texels.size(), b))) return cleanup(rc);
+    if ((rc = upload(ctx, &sc->d_xf, h.xf.data(), h.xf.size(), b))) return cleanup(rc);
     std::vector<float> ld(2 * std::max<size_t>(h.lights.size(), 1), 0.0f);
     if ((rc = upload(ctx, &sc->d_ldist, ld.data(), ld.size(), b))) return cleanup(rc);
     *out = sc;
@@ -3038,6 +3150,7 @@ void rt_scene_destroy(rt_scene* sc) {
     hipFree(sc->d_images);
     hipFree(sc->d_levels);
     hipFree(sc->d_imtex);
+    hipFree(sc->d_xf);
     delete sc;
 }
 
@@ -3227,6 +3340,7 @@ int rt_photon_map(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, int64_t
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "negative photon budget or gather radius");
     hipSetDevice(ctx->device);
     const HostScene& h = sc->hs;
+    if (h.hier) return fail(ctx, RT_ERR_UNSUPPORTED, "photon maps of a scene with an object hierarchy are not supported");
     float gr = s->caustic_gather_radius > 0.0f ? s->caustic_gather_radius : std::max(h.scene_radius * 0.06f, 1e-4f);
     DevScene S = dev_view(sc);
     PhotonPass PP;
@@ -3270,9 +3384,10 @@ int rt_photon_map(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, int64_t
 // ── render set-up helpers shared by rt_render and rt_render_batch ─────────
 // Path pool of N slots (and per-sample results of a window of WS samples)
 // in the context's grow-only buffer.
-static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, bool gather, bool fp, PathBufs& P) {
+static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, bool gather, bool fp, bool xf,
+                       PathBufs& P) {
     size_t per_path = 8 * 16 + 8 * 4 + 4 + 16 + (aovs ? 5 * 16 : 0) + 4 * 16 * (size_t)maxs + 8 * 4 + 4 * (size_t)maxs +
-                      (gather ? 64 : 0) + (fp ? 16 : 0);
+                      (gather ? 64 : 0) + (fp ? 16 : 0) + (xf ? 16 : 0);
     size_t per_sample = aovs ? 48 : 16;
     size_t need = (size_t)N * per_path + (size_t)WS * per_sample + 40 * 256 + 16384 + 8 * N +
                   4 * (size_t)((N + kBlock - 1) / kBlock + 3) * (kWcSeg + kMaxSeg);
@@ -3304,6 +3419,7 @@ static int alloc_paths(rt_ctx* ctx, int64_t N, int64_t WS, int maxs, bool aovs, 
     P.ior = (float*)take(4 * 8 * N);
     P.hit_e = (int32_t*)take(4 * N);
     P.hit_tuv = (float4*)take(16 * N);
+    if (xf) P.hit_xf = (int4*)take(16 * N);
     if (aovs) {
         P.av0 = (float4*)take(16 * N);
         P.av1 = (float4*)take(16 * N);
@@ -3522,10 +3638,12 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     const bool want_caustics = s->caustics && s->caustic_photons > 0 && !h.lights.empty();
     if (want_caustics && (h.textured || !h.images.empty()))
         return fail(ctx, RT_ERR_UNSUPPORTED, "photon caustics with textured materials are not supported");
+    if (want_caustics && h.hier)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "photon caustics in a scene with an object hierarchy are not supported");
 
     PathBufs P;
     rc = alloc_paths(ctx, N, WS, maxs, s->capture_aovs != 0, want_caustics,
-                     !h.images.empty() && s->texture_filtering != 2, P);
+                     !h.images.empty() && s->texture_filtering != 2, h.hier, P);
     if (rc) return rc;
     FilmBufs F;
     size_t acc_bytes = 0;
@@ -3609,6 +3727,7 @@ int rt_adjoint(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const floa
     if (h.medium) return fail(ctx, RT_ERR_UNSUPPORTED, "adjoint: participating media are not supported");
     if (!h.opaque_shadows) return fail(ctx, RT_ERR_UNSUPPORTED, "adjoint: transmissive materials are not supported");
     if (s->caustics) return fail(ctx, RT_ERR_UNSUPPORTED, "adjoint: photon caustics are not supported");
+    if (h.hier) return fail(ctx, RT_ERR_UNSUPPORTED, "adjoint: scenes with an object hierarchy are not supported");
     if ((double)s->max_sample_radiance * (double)s->indirect_clamp_factor < 1e30)
         return fail(ctx, RT_ERR_UNSUPPORTED, "adjoint: radiance clamps must be inactive "
                                              "(max_sample_radiance * indirect_clamp_factor >= 1e30)");
@@ -3626,7 +3745,7 @@ int rt_adjoint(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const floa
     if (maxs > 255) return fail(ctx, RT_ERR_UNSUPPORTED, "more than 255 shadow samples per shading point");
 
     PathBufs P;
-    rc = alloc_paths(ctx, N, WS, maxs, false, false, !h.images.empty() && s->texture_filtering != 2, P);
+    rc = alloc_paths(ctx, N, WS, maxs, false, false, !h.images.empty() && s->texture_filtering != 2, false, P);
     if (rc) return rc;
     // adjoint buffers: pending bounce terms, NEE terms, dLoss/dBeauty, gradient
     const size_t b_q = 16 * 5 * (size_t)N, b_g = 16 * 2 * (size_t)N * std::max(maxs, 1);
@@ -3785,11 +3904,13 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
     const int64_t N = std::min<int64_t>(N0, WS);
     if (N > (1ll << 30) || (int64_t)maxs * N > (1ll << 31) - 1)
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "path pool too large");
-    bool any_fp = false;
-    for (int j = 0; j < n_jobs; j++)
+    bool any_fp = false, any_hier = false;
+    for (int j = 0; j < n_jobs; j++) {
         any_fp |= !jobs[j].scene->hs.images.empty() && jobs[j].settings->texture_filtering != 2;
+        any_hier |= jobs[j].scene->hs.hier;
+    }
     PathBufs P;
-    rc = alloc_paths(ctx, N, WS, maxs, aovs, false, any_fp, P);
+    rc = alloc_paths(ctx, N, WS, maxs, aovs, false, any_fp, any_hier, P);
     if (rc) return rc;
     // per-job films, one buffer
     std::vector<FilmBufs> F(n_jobs);
@@ -3826,7 +3947,10 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
     HIP_TRY(hipEventRecord(ev[6], ctx->stream));
     LoopTiming tm;
     ManyJobs many{(const JobDev*)ctx->jobs_mem, 0};
-    for (int j = 0; j < n_jobs; j++) many.thin_film |= jd[j].S.thin_film;
+    for (int j = 0; j < n_jobs; j++) {
+        many.thin_film |= jd[j].S.thin_film;
+        many.hier |= jd[j].S.xf != nullptr ? 1 : 0;
+    }
     auto accum = [&](const PartDev& pt) {
         const int64_t npix = pt.npix;
         hipLaunchKernelGGL(k_accum, dim3((unsigned)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,

@@ -20,13 +20,25 @@
 //   BVH    a = (box.min, tag) b = (box.max, child ref)              — a nested
 //          BVH (the world BVH root or a Mesh's own BVH, Mesh.cs:93-96);
 //          flag kEntryNoBox = a Mesh whose faces are a plain list (<= 4).
+//   WRAP   a = (box.min, tag) b = (box.max, child entry) c = (transform,
+//          override material, -, -) — Transform / Instance /
+//          CameraInvisibleHittable around the entry at index `child`
+//          (transform -1: none, material -1: none); no box test of its own
+//          (Transform.Hit and Instance.Hit call the child directly); the box
+//          is the builder's.
+// Transforms: 8 float4 each, rows 1-4 of the object-to-world matrix then
+// rows 1-4 of its inverse (System.Numerics layout, row vectors; w unused).
 #pragma once
 #include <stdint.h>
 #include "rt_common.h"
 
 namespace rt {
 
-enum EntryKind : int32_t { kTri = 0, kSphere = 1, kQuad = 2, kPlane = 3, kBvh = 4 };
+enum EntryKind : int32_t { kTri = 0, kSphere = 1, kQuad = 2, kPlane = 3, kBvh = 4, kWrap = 5 };
+
+// Most wrapper entries on one root-to-primitive path, and the stack entry
+// that marks a wrapper's frame (a leaf reference of count 0).
+constexpr int kMaxWrap = 4;
 enum EntryFlags : int32_t { kEntryCulled = 1, kEntryCamInvisible = 2, kEntryNoBox = 4 };
 
 constexpr uint32_t kLeafBit = 0x80000000u;
@@ -36,6 +48,28 @@ RT_HD uint32_t leaf_ref(uint32_t start, uint32_t count) { return kLeafBit | (cou
 RT_HD bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0u; }
 RT_HD uint32_t leaf_start(uint32_t r) { return r & kLeafStartMask; }
 RT_HD uint32_t leaf_count(uint32_t r) { return (r >> kLeafCountShift) & 15u; }
+constexpr uint32_t kFrameRef = kLeafBit;
+
+// System.Numerics Vector3.Transform / TransformNormal with a Matrix4x4 given
+// as its rows R[0..3] (row vectors: p' = p M), in the runtime's multiply-add
+// order fma(x, row1, fma(y, row2, fma(z, row3, row4))); a direction drops
+// row4.  xf_normal multiplies by the transposed inverse given the inverse's
+// rows (Transform.cs:83, :188): component j is the dot with inverse row j.
+RT_HD f3 xf_point(f3 p, const float4* R) {
+    return mk(fmaf(p.x, R[0].x, fmaf(p.y, R[1].x, fmaf(p.z, R[2].x, R[3].x))),
+              fmaf(p.x, R[0].y, fmaf(p.y, R[1].y, fmaf(p.z, R[2].y, R[3].y))),
+              fmaf(p.x, R[0].z, fmaf(p.y, R[1].z, fmaf(p.z, R[2].z, R[3].z))));
+}
+RT_HD f3 xf_vec(f3 v, const float4* R) {
+    return mk(fmaf(v.x, R[0].x, fmaf(v.y, R[1].x, v.z * R[2].x)),
+              fmaf(v.x, R[0].y, fmaf(v.y, R[1].y, v.z * R[2].y)),
+              fmaf(v.x, R[0].z, fmaf(v.y, R[1].z, v.z * R[2].z)));
+}
+RT_HD f3 xf_normal(f3 v, const float4* I) {
+    return mk(fmaf(v.x, I[0].x, fmaf(v.y, I[0].y, v.z * I[0].z)),
+              fmaf(v.x, I[1].x, fmaf(v.y, I[1].y, v.z * I[1].z)),
+              fmaf(v.x, I[2].x, fmaf(v.y, I[2].y, v.z * I[2].z)));
+}
 
 struct alignas(64) GNode {
     float4 a, b, c;

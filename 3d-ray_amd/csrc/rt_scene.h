@@ -64,6 +64,12 @@ struct HostScene {
     std::vector<GImage> images;
     std::vector<GLevel> img_levels;
     std::vector<float4> img_texels;
+    // object hierarchy (rt_node): wrapper transforms, 8 per TRANSFORM node
+    // (matrix rows 1-4, inverse rows 1-4); `hier` = the world was assembled
+    // from nodes (wrapper entries may occur)
+    std::vector<float4> xf;
+    bool hier = false;
+    int hier_stack = 0;                // traversal stack bound of the hierarchy
 };
 
 // Builds the host scene; returns RT_OK or an error with `err` set.

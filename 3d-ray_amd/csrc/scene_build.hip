@@ -557,6 +557,235 @@ static void build_any_bvh(HostScene& hs) {
     hs.any_stack = need + 1;
 }
 
+// Flat objects of the primitive list (each primitive, or a mesh's faces).
+struct Flat {
+    const std::vector<int32_t>& first;
+    const std::vector<int32_t>& count;
+    const std::vector<Box>& box;
+    const std::vector<char>& plane;
+    const std::vector<int>& need;
+};
+
+// Object hierarchy (rt_node; Group.cs:20-80, Transform.cs:74-240,
+// Instance.cs:54-84, CameraInvisibleHittable): one entry per node, built
+// once however many parents share it.  Outputs the roots' entries, boxes,
+// world-list plane flags and stack needs.
+static int build_hierarchy(const rt_scene_desc* d, HostScene& hs, Flattener& F, const Flat& flat,
+                           std::vector<GEntry>& r_entry, std::vector<Box>& r_box, std::vector<char>& r_plane,
+                           std::vector<int>& r_need, std::string& err) {
+    const int64_t nn = d->num_nodes, np = d->num_primitives;
+    if (nn <= 0 || !d->nodes || d->num_roots <= 0 || !d->roots || d->num_node_children < 0 ||
+        (d->num_node_children > 0 && !d->node_children)) {
+        err = "bad node tables";
+        return RT_ERR_INVALID_ARGUMENT;
+    }
+    hs.hier = true;
+    std::vector<int32_t> obj_of(np, -1);
+    for (size_t o = 0; o < flat.first.size(); o++) obj_of[flat.first[o]] = (int32_t)o;
+    struct Out {
+        GEntry e;
+        Box box;
+        bool plane_world = false, plane_group = false;
+        int wdepth = 0, need = 0;
+        int32_t standalone = -1;     // index of a copy of `e` that a wrapper can name
+    };
+    std::vector<Out> out(nn);
+    std::vector<int> state(nn, 0);
+    auto standalone = [&](int32_t n) {
+        if (out[n].standalone < 0) {
+            out[n].standalone = (int32_t)hs.entries.size();
+            hs.entries.push_back(out[n].e);
+        }
+        return out[n].standalone;
+    };
+    auto wrap_entry = [&](int32_t child_entry, const Box& b, int32_t xf, int32_t mat, int32_t flags) {
+        GEntry e{};
+        e.a = make_float4(b.mn.x, b.mn.y, b.mn.z, ibits(kWrap | (flags << 8)));
+        e.b = make_float4(b.mx.x, b.mx.y, b.mx.z, ibits(child_entry));
+        e.c = make_float4(ibits(xf), ibits(mat), 0.0f, 0.0f);
+        return e;
+    };
+    std::function<int(int32_t)> make = [&](int32_t n) -> int {
+        if (n < 0 || n >= nn) { err = "node index out of range"; return RT_ERR_INVALID_ARGUMENT; }
+        if (state[n] == 2) return RT_OK;
+        if (state[n] == 1) { err = "node cycle"; return RT_ERR_INVALID_ARGUMENT; }
+        state[n] = 1;
+        const rt_node& nd = d->nodes[n];
+        Out& o = out[n];
+        const bool cam_inv = (nd.flags & RT_NODEF_CAMERA_INVISIBLE) != 0;
+        switch (nd.kind) {
+            case RT_NODE_PRIMITIVES: {
+                int32_t ob = (nd.first >= 0 && nd.first < np) ? obj_of[nd.first] : -1;
+                if (ob < 0 || flat.count[ob] != nd.count) {
+                    err = "a PRIMITIVES node must name one whole object (a primitive or a mesh)";
+                    return RT_ERR_INVALID_ARGUMENT;
+                }
+                o.e = F.obj_entry[ob];
+                o.box = flat.box[ob];
+                o.plane_world = o.plane_group = flat.plane[ob] != 0;
+                o.need = flat.need[ob];
+                break;
+            }
+            case RT_NODE_GROUP: {
+                if (nd.count <= 0 || nd.first < 0 || (int64_t)nd.first + nd.count > d->num_node_children) {
+                    err = "bad group child range";
+                    return RT_ERR_INVALID_ARGUMENT;
+                }
+                std::vector<int32_t> fin, inf;
+                bool all_planes = true;
+                for (int32_t k = 0; k < nd.count; k++) {
+                    const int32_t c = d->node_children[nd.first + k];
+                    int rc = make(c);
+                    if (rc != RT_OK) return rc;
+                    (out[c].plane_group ? inf : fin).push_back(c);
+                    all_planes = all_planes && out[c].plane_world;
+                    o.wdepth = std::max(o.wdepth, out[c].wdepth);
+                }
+                std::vector<GEntry> re;
+                std::vector<Box> rb;
+                std::vector<int> rn;
+                if (fin.size() > 4) {
+                    std::vector<GEntry> ce(fin.size());
+                    std::vector<int32_t> items(fin.size());
+                    std::vector<Box> bx(fin.size());
+                    int cneed = 0;
+                    for (size_t k = 0; k < fin.size(); k++) {
+                        ce[k] = out[fin[k]].e;
+                        items[k] = (int32_t)k;
+                        bx[k] = out[fin[k]].box;
+                        cneed = std::max(cneed, out[fin[k]].need);
+                    }
+                    auto root = build_node(items, bx, 0, (int)items.size(), 0);
+                    std::vector<GEntry> saved;
+                    saved.swap(F.obj_entry);
+                    F.obj_entry = ce;
+                    F.deepest = 0;
+                    uint32_t ref = F.flatten(root.get(), items, false, 0);
+                    F.obj_entry.swap(saved);
+                    re.push_back(bvh_entry(root->box, ref, 0));
+                    rb.push_back(root->box);
+                    rn.push_back(F.deepest + 1 + cneed);
+                } else {
+                    for (int32_t c : fin) { re.push_back(out[c].e); rb.push_back(out[c].box); rn.push_back(out[c].need); }
+                }
+                for (int32_t c : inf) { re.push_back(out[c].e); rb.push_back(out[c].box); rn.push_back(out[c].need); }
+                if (re.size() == 1) {
+                    o.e = re[0];
+                    o.box = rb[0];
+                    o.need = rn[0];
+                } else {
+                    if (re.size() > 15) { err = "a group's list part holds more than 15 members"; return RT_ERR_UNSUPPORTED; }
+                    uint32_t start = (uint32_t)hs.entries.size();
+                    Box b = rb[0];
+                    int mneed = 0;
+                    for (size_t k = 0; k < re.size(); k++) {
+                        hs.entries.push_back(re[k]);
+                        if (k) b = box_union(b, rb[k]);
+                        mneed = std::max(mneed, rn[k]);
+                    }
+                    o.e = bvh_entry(b, leaf_ref(start, (uint32_t)re.size()), kEntryNoBox);
+                    o.box = b;
+                    o.need = 1 + mneed;
+                }
+                o.plane_world = all_planes;
+                o.plane_group = false;
+                break;
+            }
+            case RT_NODE_TRANSFORM:
+            case RT_NODE_INSTANCE: {
+                int rc = make(nd.child);
+                if (rc != RT_OK) return rc;
+                const Out& c = out[nd.child];
+                int32_t xi = -1, mat = -1;
+                if (nd.kind == RT_NODE_TRANSFORM) {
+                    xi = (int32_t)(hs.xf.size() / 8);
+                    for (int r = 0; r < 4; r++)
+                        hs.xf.push_back(make_float4(nd.matrix[4 * r], nd.matrix[4 * r + 1], nd.matrix[4 * r + 2], 0.0f));
+                    for (int r = 0; r < 4; r++)
+                        hs.xf.push_back(make_float4(nd.inverse[4 * r], nd.inverse[4 * r + 1], nd.inverse[4 * r + 2], 0.0f));
+                    for (int k = 0; k < 16; k++)
+                        if (!std::isfinite(nd.matrix[k]) || !std::isfinite(nd.inverse[k])) {
+                            err = "non-finite transform";
+                            return RT_ERR_INVALID_ARGUMENT;
+                        }
+                    // ComputeWorldBox (Transform.cs:211-240)
+                    const float4* M = &hs.xf[8 * xi];
+                    f3 mn = splat(FLT_MAX), mxv = splat(-FLT_MAX);
+                    for (int k = 0; k < 8; k++) {
+                        f3 cr = mk((k & 4) ? c.box.mx.x : c.box.mn.x, (k & 2) ? c.box.mx.y : c.box.mn.y,
+                                   (k & 1) ? c.box.mx.z : c.box.mn.z);
+                        f3 t = xf_point(cr, M);
+                        mn = mk(fminf(mn.x, t.x), fminf(mn.y, t.y), fminf(mn.z, t.z));
+                        mxv = mk(fmaxf(mxv.x, t.x), fmaxf(mxv.y, t.y), fmaxf(mxv.z, t.z));
+                    }
+                    o.box = Box{mn, mxv};
+                    o.plane_world = c.plane_world;
+                    o.plane_group = c.plane_group && !cam_inv;
+                } else {
+                    if (nd.material >= d->num_materials) { err = "bad override material"; return RT_ERR_INVALID_ARGUMENT; }
+                    mat = nd.material < 0 ? -1 : nd.material;
+                    o.box = c.box;
+                }
+                o.e = wrap_entry(standalone(nd.child), o.box, xi, mat, cam_inv ? kEntryCamInvisible : 0);
+                o.need = 1 + c.need;
+                o.wdepth = 1 + c.wdepth;
+                break;
+            }
+            default: err = "bad node kind"; return RT_ERR_INVALID_ARGUMENT;
+        }
+        if (cam_inv && nd.kind != RT_NODE_TRANSFORM && nd.kind != RT_NODE_INSTANCE) {
+            // CameraInvisibleHittable around a primitive object or a group
+            const bool pw = o.plane_world;
+            const int32_t inner = (int32_t)hs.entries.size();
+            hs.entries.push_back(o.e);
+            o.e = wrap_entry(inner, o.box, -1, -1, kEntryCamInvisible);
+            o.need += 1;
+            o.wdepth += 1;
+            o.plane_world = pw;
+            o.plane_group = false;
+        }
+        if (o.wdepth > kMaxWrap) { err = "wrapper nesting deeper than 4"; return RT_ERR_UNSUPPORTED; }
+        state[n] = 2;
+        return RT_OK;
+    };
+    for (int64_t r = 0; r < d->num_roots; r++) {
+        int rc = make(d->roots[r]);
+        if (rc != RT_OK) return rc;
+        const Out& o = out[d->roots[r]];
+        r_entry.push_back(o.e);
+        r_box.push_back(o.box);
+        r_plane.push_back(o.plane_world ? 1 : 0);
+        r_need.push_back(o.need);
+    }
+    // geometry lights sample world-space primitives: none may sit under a transform
+    std::vector<char> seen(2 * nn, 0), under(np, 0);
+    std::function<void(int32_t, bool)> mark = [&](int32_t n, bool x) {
+        if (seen[2 * n + x]) return;
+        seen[2 * n + x] = 1;
+        const rt_node& nd = d->nodes[n];
+        if (nd.kind == RT_NODE_PRIMITIVES) {
+            if (x)
+                for (int32_t k = 0; k < nd.count; k++) under[nd.first + k] = 1;
+        } else if (nd.kind == RT_NODE_GROUP) {
+            for (int32_t k = 0; k < nd.count; k++) mark(d->node_children[nd.first + k], x);
+        } else mark(nd.child, x || nd.kind == RT_NODE_TRANSFORM);
+    };
+    for (int64_t r = 0; r < d->num_roots; r++) mark(d->roots[r], false);
+    for (int64_t i = 0; i < d->num_lights; i++) {
+        const rt_light& l = d->lights[i];
+        if (l.kind == RT_LIGHT_GEOMETRY && l.primitive >= 0 && l.primitive < np && under[l.primitive]) {
+            err = "a geometry light on a transformed primitive";
+            return RT_ERR_UNSUPPORTED;
+        }
+    }
+    for (int64_t i = 0; i < d->num_materials; i++)
+        if (d->materials[i].flags & RT_MATF_IMAGE) {
+            err = "image textures in a scene with an object hierarchy";
+            return RT_ERR_UNSUPPORTED;
+        }
+    return RT_OK;
+}
+
 int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
     if (!d) { err = "desc is null"; return RT_ERR_INVALID_ARGUMENT; }
     if (d->num_primitives < 0 || d->num_materials < 0 || d->num_lights < 0 ||
@@ -567,7 +796,6 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
     }
     const int64_t np = d->num_primitives;
     if (np >= (int64_t)kLeafStartMask) { err = "too many primitives"; return RT_ERR_UNSUPPORTED; }
-    if (d->num_nodes != 0) { err = "object hierarchies are not built yet"; return RT_ERR_UNSUPPORTED; }
     hs = HostScene();
     hs.num_prims = np;
     build_energy_lut(hs.lut_e, hs.lut_eavg);
@@ -699,7 +927,8 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
     const int64_t nobj = (int64_t)obj_first.size();
     hs.num_objects = nobj;
     std::vector<Box> obox(nobj);
-    std::vector<bool> is_plane(nobj, false);
+    std::vector<char> is_plane(nobj, 0);
+    std::vector<int> obj_need(nobj, 0);        // traversal stack entries below the object's entry
     F.obj_entry.resize(nobj);
     for (int64_t o = 0; o < nobj; o++) {
         int32_t f = obj_first[o];
@@ -719,6 +948,7 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
             F.deepest = 0;
             uint32_t ref = F.flatten(root.get(), items, true, 0);
             hs.mesh_depth = std::max(hs.mesh_depth, F.deepest);
+            obj_need[o] = F.deepest;
             hs.mesh_refs.push_back(ref);
             obox[o] = root->box;
             F.obj_entry[o] = bvh_entry(root->box, ref, 0);
@@ -733,16 +963,42 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
         }
     }
 
+    // World object list: the flat objects, or the roots of the hierarchy
+    std::vector<GEntry> wl_entry = F.obj_entry;
+    std::vector<Box> wl_box = obox;
+    std::vector<char> wl_plane = is_plane;
+    std::vector<int> wl_need = obj_need;
+    if (d->num_nodes != 0 || d->num_roots != 0) {
+        Flat flat{obj_first, obj_count, obox, is_plane, obj_need};
+        wl_entry.clear();
+        wl_box.clear();
+        wl_plane.clear();
+        wl_need.clear();
+        int rc = build_hierarchy(d, hs, F, flat, wl_entry, wl_box, wl_plane, wl_need, err);
+        if (rc != RT_OK) return rc;
+        F.obj_entry = wl_entry;
+        hs.num_objects = (int64_t)wl_entry.size();
+    }
+    const int64_t nw = (int64_t)wl_entry.size();
+
     // World (SceneLoader.cs:460-487)
     std::vector<GEntry> top;
     std::vector<Box> top_boxes;
-    if (nobj > 4) {
+    int world_need = 0;
+    if (nw > 4) {
         std::vector<int32_t> fin;
         std::vector<Box> fbx;
         std::vector<int32_t> inf;
-        for (int64_t o = 0; o < nobj; o++) {
-            if (is_plane[o]) inf.push_back((int32_t)o);
-            else { fin.push_back((int32_t)o); fbx.push_back(obox[o]); }
+        int fin_need = 0;
+        for (int64_t o = 0; o < nw; o++) {
+            if (wl_plane[o]) {
+                inf.push_back((int32_t)o);
+                world_need = std::max(world_need, wl_need[o]);
+            } else {
+                fin.push_back((int32_t)o);
+                fbx.push_back(wl_box[o]);
+                fin_need = std::max(fin_need, wl_need[o]);
+            }
         }
         if (!fin.empty()) {
             auto root = build_node(fin, fbx, 0, (int)fin.size(), 0);
@@ -754,16 +1010,22 @@ int build_host_scene(const rt_scene_desc* d, HostScene& hs, std::string& err) {
             top.push_back(bvh_entry(root->box, ref, 0));
             top_boxes.push_back(root->box);
             hs.has_bvh = true;
+            world_need = std::max(world_need, F.deepest + 1 + fin_need);
         }
-        for (int32_t o : inf) { top.push_back(F.obj_entry[o]); top_boxes.push_back(obox[o]); }
+        for (int32_t o : inf) { top.push_back(wl_entry[o]); top_boxes.push_back(wl_box[o]); }
     } else {
-        for (int64_t o = 0; o < nobj; o++) { top.push_back(F.obj_entry[o]); top_boxes.push_back(obox[o]); }
+        for (int64_t o = 0; o < nw; o++) {
+            top.push_back(wl_entry[o]);
+            top_boxes.push_back(wl_box[o]);
+            world_need = std::max(world_need, wl_need[o]);
+        }
     }
     hs.top_start = (uint32_t)hs.entries.size();
     hs.top_count = (uint32_t)top.size();
     for (auto& e : top) hs.entries.push_back(e);
     if (hs.entries.size() >= kLeafStartMask) { err = "scene too large"; return RT_ERR_UNSUPPORTED; }
-    build_any_bvh(hs);
+    hs.hier_stack = world_need;
+    if (!hs.hier) build_any_bvh(hs);
 
     // Camera basis (Camera.cs:26-51)
     {

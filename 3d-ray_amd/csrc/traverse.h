@@ -114,7 +114,26 @@ __device__ __forceinline__ bool slab(f3 mn_, f3 mx_, f3 o, f3 inv, float tmin, f
 struct HitOut {
     int32_t entry;   // entry index, -1 = miss
     float t, u, v;
+    int4 chain;      // wrapper entries above the hit, outermost first (-1: none)
 };
+
+// The ray of the frame a hit was found in: the world ray taken through the
+// transforms of the wrapper chain, outermost first (ToLocalRay,
+// Transform.cs:140-155).
+__device__ __forceinline__ void chain_ray(const GEntry* __restrict__ entries, const float4* __restrict__ xf,
+                                          int4 chain, f3& o, f3& d) {
+    const int32_t c[4] = {chain.x, chain.y, chain.z, chain.w};
+#pragma unroll
+    for (int k = 0; k < kMaxWrap; k++) {
+        if (c[k] < 0) break;
+        const int32_t xi = __float_as_int(entries[c[k]].c.x);
+        if (xi >= 0) {
+            const float4* I = xf + 8 * xi + 4;
+            o = xf_point(o, I);
+            d = xf_vec(d, I);
+        }
+    }
+}
 
 // Reference-traversal work counters (instrumented kernels only):
 // node = BvhNode.Hit calls (box tests); tri = Hit calls on 64-B primitive
@@ -160,33 +179,87 @@ __device__ __forceinline__ bool test_prim(const GEntry* __restrict__ entries, fl
 // answer to "is there any hit" equals the closest-hit answer's existence,
 // because until the first hit both traversals visit the same nodes with
 // the same tMax.
-template <int SD, int STRIDE, bool COUNT, bool ANY = false>
+// XF: the scene has wrapper entries (Transform / Instance /
+// CameraInvisibleHittable, rt_node).  Entering one pushes a frame marker,
+// records the entry in the chain and maps the ray into the child's frame
+// (a wrapper has no box test of its own: Transform.Hit and Instance.Hit call
+// the child directly); popping the marker returns to the parent frame by
+// re-deriving its ray from the world ray through the remaining chain, so the
+// ray of every frame is the same float sequence the reference's nested Hit
+// calls compute.  A hit carries the chain it was found under.
+template <int SD, int STRIDE, bool COUNT, bool ANY = false, bool XF = false>
 __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes, const GEntry* __restrict__ entries,
                                                 const float4* __restrict__ aux, uint32_t top_start, uint32_t top_count,
                                                 f3 o, f3 d, float tmin, float tmax, uint2* stk,
-                                                uint32_t* overflow, TravCount& tc) {
+                                                uint32_t* overflow, TravCount& tc,
+                                                const float4* __restrict__ xf = nullptr) {
     HitOut h;
     h.entry = -1;
     h.t = 0.0f;
     h.u = 0.0f;
     h.v = 0.0f;
+    h.chain = make_int4(-1, -1, -1, -1);
+    const f3 ow = o, dw = d;
+    int4 chain = make_int4(-1, -1, -1, -1);
+    int depth = 0;
     f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float closest = tmax;
+    // enters wrapper entry `e` (its c.x = transform); returns the child's leaf reference
+    auto enter = [&](uint32_t e, float4 eb, float4 ec, int& sp) -> bool {
+        if (depth >= kMaxWrap || sp >= SD) {
+            *overflow = 1u;
+            return false;
+        }
+        stk[sp * STRIDE] = make_uint2(kFrameRef, 0u);
+        sp++;
+        if (depth == 0) chain.x = (int32_t)e;
+        else if (depth == 1) chain.y = (int32_t)e;
+        else if (depth == 2) chain.z = (int32_t)e;
+        else chain.w = (int32_t)e;
+        depth++;
+        const int32_t xi = __float_as_int(ec.x);
+        if (xi >= 0) {
+            const float4* I = xf + 8 * xi + 4;
+            o = xf_point(o, I);
+            d = xf_vec(d, I);
+            inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        }
+        return true;
+    };
+    auto leave = [&]() {
+        depth--;
+        if (depth == 0) chain.x = -1;
+        else if (depth == 1) chain.y = -1;
+        else if (depth == 2) chain.z = -1;
+        else chain.w = -1;
+        o = ow;
+        d = dw;
+        chain_ray(entries, xf, chain, o, d);
+        inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    };
     for (uint32_t ti = 0; ti < top_count; ti++) {
         uint32_t eidx = top_start + ti;
         float4 ea = entries[eidx].a, eb = entries[eidx].b, ec = entries[eidx].c;
         int32_t tag = tag_of(ea);
         int32_t kind = tag & 0xff, flags = tag >> 8;
-        if (kind != kBvh) {
-            if (test_prim<COUNT>(entries, ea, eb, ec, eidx, kind, flags, o, d, tmin, aux, closest, h, tc) && ANY)
-                return h;
+        if (kind != kBvh && !(XF && kind == kWrap)) {
+            if (test_prim<COUNT>(entries, ea, eb, ec, eidx, kind, flags, o, d, tmin, aux, closest, h, tc)) {
+                if (XF) h.chain = chain;
+                if (ANY) return h;
+            }
             continue;
         }
-        float te;
-        if (COUNT && !(flags & kEntryNoBox)) tc.node++;
-        if (!(flags & kEntryNoBox) && !slab(f3of(ea), f3of(eb), o, inv, tmin, closest, te)) continue;
-        uint32_t cur = (uint32_t)__float_as_int(eb.w);
         int sp = 0;
+        uint32_t cur;
+        if (XF && kind == kWrap) {
+            if (!enter(eidx, eb, ec, sp)) return h;
+            cur = leaf_ref((uint32_t)__float_as_int(eb.w), 1u);
+        } else {
+            float te;
+            if (COUNT && !(flags & kEntryNoBox)) tc.node++;
+            if (!(flags & kEntryNoBox) && !slab(f3of(ea), f3of(eb), o, inv, tmin, closest, te)) continue;
+            cur = (uint32_t)__float_as_int(eb.w);
+        }
         for (;;) {
             if (ref_is_leaf(cur)) {
                 uint32_t start = leaf_start(cur), cnt = leaf_count(cur);
@@ -196,20 +269,27 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
                     float4 la = entries[idx].a, lb = entries[idx].b, lc = entries[idx].c;
                     int32_t ltag = tag_of(la);
                     int32_t lk = ltag & 0xff, lf = ltag >> 8;
-                    if (lk != kBvh) {
-                        if (test_prim<COUNT>(entries, la, lb, lc, idx, lk, lf, o, d, tmin, aux, closest, h, tc) && ANY)
-                            return h;
+                    if (lk != kBvh && !(XF && lk == kWrap)) {
+                        if (test_prim<COUNT>(entries, la, lb, lc, idx, lk, lf, o, d, tmin, aux, closest, h, tc)) {
+                            if (XF) h.chain = chain;
+                            if (ANY) return h;
+                        }
                         continue;
                     }
-                    float lte;
-                    if (COUNT && !(lf & kEntryNoBox)) tc.node++;
-                    if (!(lf & kEntryNoBox) && !slab(f3of(la), f3of(lb), o, inv, tmin, closest, lte)) continue;
+                    if (!(XF && lk == kWrap)) {
+                        float lte;
+                        if (COUNT && !(lf & kEntryNoBox)) tc.node++;
+                        if (!(lf & kEntryNoBox) && !slab(f3of(la), f3of(lb), o, inv, tmin, closest, lte)) continue;
+                    }
                     if (k + 1 < cnt) {
                         if (sp >= SD) { *overflow = 1u; return h; }
                         stk[sp * STRIDE] = make_uint2(leaf_ref(idx + 1, cnt - k - 1), __float_as_uint(-FLT_MAX));
                         sp++;
                     }
-                    cur = (uint32_t)__float_as_int(lb.w);
+                    if (XF && lk == kWrap) {
+                        if (!enter(idx, lb, lc, sp)) return h;
+                        cur = leaf_ref((uint32_t)__float_as_int(lb.w), 1u);
+                    } else cur = (uint32_t)__float_as_int(lb.w);
                     descended = true;
                     break;
                 }
@@ -248,6 +328,10 @@ __device__ __forceinline__ HitOut trace_closest(const GNode* __restrict__ nodes,
             while (sp > 0) {
                 sp--;
                 uint2 s = stk[sp * STRIDE];
+                if (XF && s.x == kFrameRef) {
+                    leave();
+                    continue;
+                }
                 if (__uint_as_float(s.y) <= closest) {
                     cur = s.x;
                     found = true;

@@ -11,9 +11,15 @@
   entities         sphere, quad, triangle / smooth_triangle (n0-n2 and uv0-uv2 make it
                    a SmoothTriangle), plane / infinite_plane (CreateEntity :2282-2325,
                    CreateTriangleEntity :3430-3461), mesh / obj (CreateMeshEntity
-                   :2828-2935 through raytracer/obj_loader.py, no subdivision) with
-                   scale / rotate / translate (ComputeTransformMatrix :3788-3822) and
-                   visible_to_camera
+                   :2828-2935 through raytracer/obj_loader.py, no subdivision), group
+                   (CreateGroupEntity :3226-3244, BuildChildList :3345-3418) and
+                   instance of a named template (CreateInstanceEntity :3259-3337),
+                   with scale / rotate / translate (ComputeTransformMatrix :3788-3822)
+                   and visible_to_camera
+  templates        named groups with a default pose, built once and shared by their
+                   instances (:294-321)
+  imports          other scene files' materials, entities, lights, templates and
+                   mediums, placed before the local ones (ProcessImports :563-675)
   lights           sphere, area, directional / sun (CreateLight :3463-3636, with the
                    emissive proxies it appends); unknown types are skipped
   camera/cameras   ResolveCamera :731-781, focal_pos via ComputeFocusDistance :3744-3786
@@ -22,14 +28,20 @@
 
 The assembled tables keep SceneLoader's object order — ground, entities,
 then the light proxies — which is the primitive order the renderer's IDs
-follow.  Transforms are applied to the primitive's defining points and
-vectors (the reference wraps the primitive in a ray-transforming
-Transform): the same surfaces up to float rounding.  Everything else in the
-schema (imports, templates, instances, groups, CSG, mesh subdivision, textures and
-maps, mix materials, metal, bound media, non-isotropic phases, point / spot
-/ portal lights, motion, sky models other than flat, ground shapes other
-than the plane) raises SceneLoadError naming the feature, so a scene is
-either loaded as the reference would assemble it or refused.
+follow.  A transformed entity, a group and an instance become object
+hierarchy nodes (rt_node: Transform, Group, Instance and
+CameraInvisibleHittable wrappers, the ray mapped into object space as
+Transform.Hit does); two cases keep the older form, the transform applied
+to the primitive's defining points and vectors (the same surfaces up to
+float rounding): emissive entities, whose geometry lights sample
+world-space primitives, and every entity of a scene with image textures,
+which the library filters only without wrappers (groups and instances are
+then refused).  Everything else in the schema (CSG, mesh subdivision,
+procedural textures and maps, mix materials, metal, bound media,
+non-isotropic phases, point / spot / portal lights, motion, sky models
+other than flat, ground shapes other than the plane, emissive objects
+inside groups or templates) raises SceneLoadError naming the feature, so a
+scene is either loaded as the reference would assemble it or refused.
 """
 import math
 import os
@@ -210,7 +222,32 @@ def _vec(M, v):
     return tuple(M[:3, :3] @ np.array(v))
 
 
-_UNSUPPORTED_ENTITY_KEYS = ("motion", "interior_medium", "exterior_medium", "displacement", "template")
+_UNSUPPORTED_ENTITY_KEYS = ("motion", "interior_medium", "exterior_medium", "displacement")
+
+
+class _Ctx:
+    """Per-load state of the object hierarchy: whether wrappers may be used
+    (no image texture anywhere: the library filters image textures only in
+    scenes without wrappers), the named templates and their built nodes
+    (SceneLoader.cs:294-321: built once, on first use)."""
+
+    def __init__(self, hier_ok, templates, get_material, base_dir, warnings):
+        self.hier_ok = hier_ok
+        self.templates = templates
+        self.cache = {}
+        self.get_material = get_material
+        self.base_dir = base_dir
+        self.warnings = warnings
+
+
+def _row_matrix(M):
+    """The column-vector 4x4 of _transform as System.Numerics lays it out
+    (row vectors, translation in row 4)."""
+    return np.asarray(M, np.float64).T.astype(np.float32)
+
+
+def _emissive(b, mat):
+    return int(b.material_record(mat)["kind"]) == abi.RT_MAT_EMISSIVE
 
 
 def _mesh_entity(b, e, mat, flags, M, uniform, ident, base_dir, warnings):
@@ -258,16 +295,53 @@ def _mesh_entity(b, e, mat, flags, M, uniform, ident, base_dir, warnings):
     return b.mesh_triangles(tris, mat, normals, mesh.uvs, flags)
 
 
-def _entity(b, e, mat, base_dir=None, warnings=None):
+def _entity(b, e, mat, base_dir=None, warnings=None, ctx=None):
+    """One top-level entity (the entity loop of SceneLoader.Load, :333-447):
+    its object, then Transform and CameraInvisibleHittable around it.  With
+    wrappers allowed a transformed entity keeps its object-space primitive
+    under a TRANSFORM node (Transform.Hit maps the ray); otherwise, and for
+    emissive entities (their geometry lights sample world-space
+    primitives), the transform is applied to the primitive's points."""
     typ = (e.get("type") or "").lower()
     for k in _UNSUPPORTED_ENTITY_KEYS:
         if e.get(k) is not None:
             raise SceneLoadError(f"entity '{e.get('name')}': '{k}' is not supported")
-    flags = 0 if e.get("visible_to_camera", True) else abi.RT_PRIMF_CAMERA_INVISIBLE
+    warnings = [] if warnings is None else warnings
+    visible = bool(e.get("visible_to_camera", True))
     M, uniform = _transform(e)
     ident = np.array_equal(M, np.eye(4))
+    if typ in ("group", "instance"):
+        if ctx is None or not ctx.hier_ok:
+            raise SceneLoadError(f"entity '{e.get('name')}': groups and instances in a scene with image "
+                                 "textures are not supported")
+        node = _group_node(b, e, mat, ctx) if typ == "group" else _instance_node(b, e, ctx)
+        if node is None:
+            return None
+        if not ident:
+            node = b.node_transform(node, _row_matrix(M), camera_invisible=not visible)
+        elif not visible:
+            node = b.node_group([node], camera_invisible=True)
+        b.add_root(node)
+        return node
+    if not ident and ctx is not None and ctx.hier_ok and not _emissive(b, mat):
+        if typ in ("plane", "infinite_plane") and not visible:
+            raise SceneLoadError(f"entity '{e.get('name')}': camera-invisible planes are not supported")
+        first = _object_entity(b, e, typ, mat, 0, np.eye(4), 1.0, True, base_dir, warnings)
+        if first is None:
+            return None
+        node = b.node_transform(b.node_prims(first), _row_matrix(M), camera_invisible=not visible)
+        b.add_root(node)
+        return node
+    flags = 0 if visible else abi.RT_PRIMF_CAMERA_INVISIBLE
+    return _object_entity(b, e, typ, mat, flags, M, uniform, ident, base_dir, warnings)
+
+
+def _object_entity(b, e, typ, mat, flags, M, uniform, ident, base_dir, warnings):
+    """CreateEntity / CreateMeshEntity: one primitive object (a mesh is one
+    object), with M applied to its points when not the identity; returns
+    its first primitive (None when skipped)."""
     if typ in ("mesh", "obj"):
-        return _mesh_entity(b, e, mat, flags, M, uniform, ident, base_dir, [] if warnings is None else warnings)
+        return _mesh_entity(b, e, mat, flags, M, uniform, ident, base_dir, warnings)
     if typ == "sphere":
         c = _v3(e.get("center"), (0.0, 0.0, 0.0))
         r = _f(e, "radius", 1.0)
@@ -328,6 +402,119 @@ def _entity(b, e, mat, base_dir=None, warnings=None):
     raise SceneLoadError(f"entity '{e.get('name')}': type '{typ}' is not supported")
 
 
+def _child_node(b, c, fallback, ctx):
+    """One child of a group or template (BuildChildList, SceneLoader.cs:3345-3418):
+    its own material or the parent's, its object, then its Transform and
+    CameraInvisibleHittable; None when the child yields no object."""
+    for k in _UNSUPPORTED_ENTITY_KEYS:
+        if c.get(k) is not None:
+            raise SceneLoadError(f"entity '{c.get('name')}': '{k}' is not supported")
+    typ = (c.get("type") or "").lower()
+    if c.get("motion"):
+        ctx.warnings.append(f"Child entity '{c.get('name') or typ}': 'motion' is only supported on top-level "
+                            "entities — ignored.")
+    mat = ctx.get_material(c["material"]) if c.get("material") is not None else fallback
+    if typ == "group":
+        node = _group_node(b, c, mat, ctx)
+    elif typ == "instance":
+        node = None                        # CreateEntity returns null for an instance child
+    elif typ in ("csg",) or typ.startswith("heightfield") or typ == "terrain":
+        raise SceneLoadError(f"entity '{c.get('name')}': type '{typ}' is not supported")
+    else:
+        if _emissive(b, mat):
+            raise SceneLoadError(f"entity '{c.get('name')}': emissive objects inside groups or templates "
+                                 "are not supported")
+        first = _object_entity(b, c, typ, mat, 0, np.eye(4), 1.0, True, ctx.base_dir, ctx.warnings)
+        node = None if first is None else b.node_prims(first)
+    if node is None:
+        return None
+    M, _ = _transform(c)
+    visible = bool(c.get("visible_to_camera", True))
+    if not np.array_equal(M, np.eye(4)):
+        node = b.node_transform(node, _row_matrix(M), camera_invisible=not visible)
+    elif not visible:
+        node = b.node_group([node], camera_invisible=True)
+    return node
+
+
+def _group_node(b, e, mat, ctx):
+    """CreateGroupEntity (SceneLoader.cs:3226-3244): a Group of the children."""
+    children = e.get("children") or []
+    if not children:
+        ctx.warnings.append(f"Group '{e.get('name') or '(unnamed)'}' has no children. Skipping.")
+        return None
+    nodes = [n for n in (_child_node(b, c, mat, ctx) for c in children) if n is not None]
+    if not nodes:
+        ctx.warnings.append(f"Group '{e.get('name') or '(unnamed)'}': all children failed to load. Skipping.")
+        return None
+    return b.node_group(nodes)
+
+
+def _instance_node(b, e, ctx):
+    """CreateInstanceEntity + BuildTemplateGeometry (SceneLoader.cs:3259-3337):
+    the named template's Group (with its default-pose Transform) built once
+    and shared; an Instance around it with the entity's material as override
+    when the entity names one."""
+    name = e.get("name") or "(unnamed)"
+    tname = e.get("template")
+    if tname is None or not str(tname).strip():
+        ctx.warnings.append(f"Instance '{name}' requires a 'template' name. Skipping.")
+        return None
+    key = str(tname).lower()
+    tdef = ctx.templates.get(key)
+    if tdef is None:
+        ctx.warnings.append(f"Instance '{name}': template '{tname}' not found. Skipping.")
+        return None
+    if key not in ctx.cache:
+        fallback = ctx.get_material(tdef.get("material"))
+        nodes = [n for n in (_child_node(b, c, fallback, ctx) for c in tdef.get("children") or []) if n is not None]
+        if not nodes:
+            ctx.warnings.append(f"Instance '{name}' of '{tname}': all template children failed to load. Skipping.")
+            return None
+        shared = b.node_group(nodes)
+        M, _ = _transform(tdef)
+        if not np.array_equal(M, np.eye(4)):
+            shared = b.node_transform(shared, _row_matrix(M))
+        ctx.cache[key] = shared
+    over = ctx.get_material(e["material"]) if e.get("material") is not None else -1
+    return b.node_instance(ctx.cache[key], material=over)
+
+
+def _merge_imports(data, base_dir, visited, warnings):
+    """ProcessImports (SceneLoader.cs:563-675): imported materials, entities,
+    lights, templates and mediums go before the local ones (a local id or
+    name wins); nested imports first; a repeated file is skipped."""
+    imports = data.get("imports") or []
+    if not imports:
+        return
+    import yaml
+    got = {k: [] for k in ("materials", "entities", "lights", "templates", "mediums")}
+    for imp in imports:
+        path = imp.get("path") if isinstance(imp, dict) else imp
+        if path is None or not str(path).strip():
+            warnings.append("Import entry has empty 'path'. Skipping.")
+            continue
+        full = os.path.abspath(path if os.path.isabs(str(path)) else os.path.join(base_dir, str(path)))
+        if full in visited:
+            warnings.append(f"Circular import detected: '{path}'. Skipping.")
+            continue
+        visited.add(full)
+        if not os.path.isfile(full):
+            warnings.append(f"Import file not found: '{path}' (resolved to '{full}'). Skipping.")
+            continue
+        with open(full, "r", encoding="utf-8") as f:
+            sub = yaml.safe_load(f)
+        if not isinstance(sub, dict):
+            warnings.append(f"Failed to parse import file: '{path}'. Skipping.")
+            continue
+        _merge_imports(sub, os.path.dirname(full), visited, warnings)
+        for k in got:
+            got[k].extend(sub.get(k) or [])
+    for k, v in got.items():
+        if v:
+            data[k] = v + list(data.get(k) or [])
+
+
 def _camera(data, selector, width, height):
     """ResolveCamera, then the Camera constructor's inputs."""
     cams = data.get("cameras") or []
@@ -374,9 +561,11 @@ def load_data(data, width, height, shadow_samples_override=None, camera=None, ba
     if not isinstance(data, dict):
         raise SceneLoadError("not a scene document")
     warnings = [] if warnings is None else warnings
-    for k in ("imports", "templates", "mediums"):
-        if data.get(k):
-            raise SceneLoadError(f"'{k}' is not supported")
+    if data.get("imports"):
+        data = dict(data)
+        _merge_imports(data, base_dir or ".", set(), warnings)
+    # named media (SceneLoader.cs:235-256) only act through an entity's
+    # interior / exterior binding, which is refused per entity
     b = scenes.SceneBuilder()
     b.shadow_samples_override = shadow_samples_override
     b.geometry_shadow_samples = GEOMETRY_SHADOW_SAMPLES
@@ -451,8 +640,19 @@ def load_data(data, width, height, shadow_samples_override=None, camera=None, ba
         n = n / np.float32(np.sqrt(np.float32(n @ n)))
         b.plane(anchor, tuple(float(x) for x in n), gm)
 
+    templates = {}
+    for t in data.get("templates") or []:
+        if not t.get("name") or not str(t["name"]).strip():
+            warnings.append("Template without a 'name' field. Skipping.")
+            continue
+        if not t.get("children"):
+            warnings.append(f"Template '{t['name']}' has no 'children'. Skipping.")
+            continue
+        templates[str(t["name"]).lower()] = t          # last one wins
+    has_images = any(int(r["flags"]) & abi.RT_MATF_IMAGE for r in (b.material_record(m) for m in range(b.material_count)))
+    ctx = _Ctx(not has_images, templates, get_material, base_dir, warnings)
     for e in data.get("entities") or []:
-        _entity(b, e, get_material(e.get("material")), base_dir, warnings)
+        _entity(b, e, get_material(e.get("material")), base_dir, warnings, ctx)
 
     # lights (proxies appended after the entities)
     lights_key = "lights" in data

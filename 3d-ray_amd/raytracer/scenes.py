@@ -191,6 +191,10 @@ class SceneBuilder:
         m["flags"] = flags
         return self.add_materials(m)
 
+    @property
+    def material_count(self):
+        return self._mat_count
+
     def material_record(self, index):
         """The MAT_DTYPE record of material `index`."""
         k = int(index)

@@ -19,6 +19,7 @@
 #include <vector>
 #include <algorithm>
 #include <functional>
+#include <unordered_map>
 #include <string>
 #include <cmath>
 
@@ -3356,13 +3357,124 @@ struct Renderer {
 // ── Scene construction ───────────────────────────────────────────────────
 static std::string g_err;
 
+// Object hierarchy (rt_node): the world list becomes the roots' hittables;
+// Group / Transform / Instance / CameraInvisible wrappers per node, shared
+// nodes built once (SceneLoader.cs:3259-3337 caches a template and wraps
+// it in one Instance per use).  Limits as the device's: wrapper nesting at
+// most 4, a Group's list part at most 15 members.
+static bool build_hierarchy(const rt_scene_desc* d, Scene& sc, const std::vector<int32_t>& obj_first,
+                            std::vector<char>& under_xf) {
+    const int64_t nn = d->num_nodes, np = d->num_primitives;
+    if (nn <= 0 || !d->nodes || d->num_roots <= 0 || !d->roots || d->num_node_children < 0 ||
+        (d->num_node_children > 0 && !d->node_children)) {
+        g_err = "bad node tables";
+        return false;
+    }
+    // flat object starting at each primitive (a lone primitive or a mesh)
+    std::vector<int32_t> obj_of(np, -1), obj_size(obj_first.size(), 0);
+    for (size_t o = 0; o < obj_first.size(); o++) {
+        obj_of[obj_first[o]] = (int32_t)o;
+        int32_t end = o + 1 < obj_first.size() ? obj_first[o + 1] : (int32_t)np;
+        obj_size[o] = end - obj_first[o];
+    }
+    std::vector<Hittable*> built(nn, nullptr);
+    std::vector<int> state(nn, 0), wdepth(nn, 0);
+    std::function<Hittable*(int32_t)> make = [&](int32_t n) -> Hittable* {
+        if (n < 0 || n >= nn) { g_err = "node index out of range"; return nullptr; }
+        if (state[n] == 2) return built[n];
+        if (state[n] == 1) { g_err = "node cycle"; return nullptr; }
+        state[n] = 1;
+        const rt_node& nd = d->nodes[n];
+        Hittable* h = nullptr;
+        int depth = 0;
+        bool wrapper = (nd.flags & RT_NODEF_CAMERA_INVISIBLE) != 0;
+        switch (nd.kind) {
+            case RT_NODE_PRIMITIVES:
+                if (nd.first < 0 || nd.first >= np || obj_of[nd.first] < 0 || obj_size[obj_of[nd.first]] != nd.count) {
+                    g_err = "a PRIMITIVES node must name one whole object (a primitive or a mesh)";
+                    return nullptr;
+                }
+                h = sc.objects[obj_of[nd.first]];
+                break;
+            case RT_NODE_GROUP: {
+                if (nd.count <= 0 || nd.first < 0 || (int64_t)nd.first + nd.count > d->num_node_children) {
+                    g_err = "bad group child range";
+                    return nullptr;
+                }
+                std::vector<Hittable*> ch;
+                for (int32_t k = 0; k < nd.count; k++) {
+                    int32_t c = d->node_children[nd.first + k];
+                    Hittable* x = make(c);
+                    if (!x) return nullptr;
+                    depth = std::max(depth, wdepth[c]);
+                    ch.push_back(x);
+                }
+                auto* g = new GroupH(ch);
+                sc.owned.push_back(g);
+                if (g->list && g->list->objs.size() > 15) { g_err = "group list part over 15 members"; return nullptr; }
+                h = g;
+                break;
+            }
+            case RT_NODE_TRANSFORM:
+            case RT_NODE_INSTANCE: {
+                Hittable* x = make(nd.child);
+                if (!x) return nullptr;
+                depth = wdepth[nd.child];
+                wrapper = true;
+                if (nd.kind == RT_NODE_TRANSFORM) h = new TransformH(x, nd.matrix, nd.inverse);
+                else {
+                    const Material* om = nullptr;
+                    if (nd.material >= 0) {
+                        if (nd.material >= (int32_t)sc.materials.size()) { g_err = "bad override material"; return nullptr; }
+                        om = sc.materials[nd.material].get();
+                    }
+                    h = new InstanceH(x, om);
+                }
+                sc.owned.push_back(h);
+                break;
+            }
+            default: g_err = "bad node kind"; return nullptr;
+        }
+        if (nd.flags & RT_NODEF_CAMERA_INVISIBLE) {
+            h = new CameraInvisible(h);
+            sc.owned.push_back(h);
+        }
+        wdepth[n] = depth + (wrapper ? 1 : 0);
+        if (wdepth[n] > 4) { g_err = "wrapper nesting over 4"; return nullptr; }
+        state[n] = 2;
+        built[n] = h;
+        return h;
+    };
+    std::vector<Hittable*> roots;
+    for (int64_t r = 0; r < d->num_roots; r++) {
+        Hittable* h = make(d->roots[r]);
+        if (!h) return false;
+        roots.push_back(h);
+    }
+    // primitives reachable through a TRANSFORM
+    std::vector<char> seen(2 * nn, 0);
+    std::function<void(int32_t, bool)> mark = [&](int32_t n, bool xf) {
+        if (seen[2 * n + xf]) return;
+        seen[2 * n + xf] = 1;
+        const rt_node& nd = d->nodes[n];
+        if (nd.kind == RT_NODE_PRIMITIVES) {
+            if (xf)
+                for (int32_t k = 0; k < nd.count; k++) under_xf[nd.first + k] = 1;
+        } else if (nd.kind == RT_NODE_GROUP) {
+            for (int32_t k = 0; k < nd.count; k++) mark(d->node_children[nd.first + k], xf);
+        } else mark(nd.child, xf || nd.kind == RT_NODE_TRANSFORM);
+    };
+    for (int64_t r = 0; r < d->num_roots; r++) mark(d->roots[r], false);
+    sc.objects = roots;
+    return true;
+}
+
 static Scene* build_scene(const rt_scene_desc* d) {
     if (!d || (d->num_primitives > 0 && !d->primitives) || (d->num_materials > 0 && !d->materials) ||
         (d->num_lights > 0 && !d->lights)) {
         g_err = "null table";
         return nullptr;
     }
-    if (d->num_nodes != 0) { g_err = "object hierarchies are not built yet"; return nullptr; }
     auto sc = std::make_unique<Scene>();
     if (d->num_images < 0 || (d->num_images > 0 && !d->images)) { g_err = "bad image table"; return nullptr; }
     for (int64_t k = 0; k < d->num_images; k++) {
@@ -3390,6 +3502,7 @@ static Scene* build_scene(const rt_scene_desc* d) {
     }
     int64_t np = d->num_primitives;
     sc->prim_objs.assign(np, nullptr);
+    std::vector<int32_t> obj_first;      // first primitive of each flat object
     Mesh* cur_mesh = nullptr;
     int cur_mesh_id = -1;
     for (int64_t i = 0; i < np; i++) {
@@ -3433,12 +3546,14 @@ static Scene* build_scene(const rt_scene_desc* d) {
                 cur_mesh_id = p.mesh;
                 sc->owned.push_back(cur_mesh);
                 sc->objects.push_back(cur_mesh);
+                obj_first.push_back((int32_t)i);
             }
             cur_mesh->tris.push_back(obj);
         } else {
             cur_mesh = nullptr;
             cur_mesh_id = -1;
             sc->objects.push_back(obj);
+            obj_first.push_back((int32_t)i);
         }
     }
     sc->tri_flag.assign(np, 0);
@@ -3462,6 +3577,17 @@ static Scene* build_scene(const rt_scene_desc* d) {
     if (any_image && d->texcoords && sc->tuv.empty()) sc->tuv.assign(d->texcoords, d->texcoords + 6 * (size_t)np);
     for (Hittable* o : sc->objects)
         if (Mesh* me = dynamic_cast<Mesh*>(o)) me->finish();
+    if (d->num_nodes > 0 || d->num_roots > 0) {
+        std::vector<char> under_xf(np, 0);
+        if (!build_hierarchy(d, *sc, obj_first, under_xf)) return nullptr;
+        if (any_image) { g_err = "image textures in a scene with an object hierarchy"; return nullptr; }
+        for (int64_t i = 0; i < d->num_lights; i++)
+            if (d->lights[i].kind == RT_LIGHT_GEOMETRY && d->lights[i].primitive >= 0 &&
+                d->lights[i].primitive < np && under_xf[d->lights[i].primitive]) {
+                g_err = "geometry light on a transformed primitive";
+                return nullptr;
+            }
+    }
     // World assembly (SceneLoader.cs:460-487)
     sc->world = new HittableList();
     if (sc->objects.size() > 4) {
@@ -3713,6 +3839,35 @@ int orc_trace_rays(void* sp, const float* rays, int64_t n, int32_t* prim, float*
         if (v) v[i] = h ? rec.v : 0.0f;
         if (node_tests) node_tests[i] = t_node_tests;
         if (prim_tests) prim_tests[i] = t_prim_tests;
+    }
+    return 0;
+}
+
+// Closest-hit records of world-space rays (the fields MapHitToWorld maps):
+// f[8 i ..] = t, point, normal, front; m[2 i ..] = primitive, material index
+// (-1, -1 on a miss).
+int orc_trace_records(void* sp, const float* rays, int64_t n, float* f, int32_t* m) {
+    Scene* s = (Scene*)sp;
+    std::unordered_map<const Material*, int32_t> mat_index;
+    for (size_t k = 0; k < s->materials.size(); k++) mat_index[s->materials[k].get()] = (int32_t)k;
+    for (int64_t i = 0; i < n; i++) {
+        const float* r = rays + 6 * i;
+        Ray ray(V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5]));
+        HitRecord rec;
+        bool h = s->world->hit(ray, kEpsilon, kInfinity, rec);
+        float* o = f + 8 * i;
+        if (!h) {
+            for (int k = 0; k < 8; k++) o[k] = 0.0f;
+            m[2 * i] = m[2 * i + 1] = -1;
+            continue;
+        }
+        o[0] = rec.t;
+        o[1] = rec.point.x; o[2] = rec.point.y; o[3] = rec.point.z;
+        o[4] = rec.normal.x; o[5] = rec.normal.y; o[6] = rec.normal.z;
+        o[7] = rec.front_face ? 1.0f : 0.0f;
+        m[2 * i] = rec.prim;
+        auto it = mat_index.find(rec.mat);
+        m[2 * i + 1] = it == mat_index.end() ? -1 : it->second;
     }
     return 0;
 }

@@ -42,6 +42,7 @@ def lib():
         L.orc_bvh_dump.restype = C.c_int64
         L.orc_bvh_dump.argtypes = [vp, ip, C.c_int64]
         L.orc_trace_rays.argtypes = [vp, fp, C.c_int64, ip, fp, fp, fp, i64p, i64p]
+        L.orc_trace_records.argtypes = [vp, fp, C.c_int64, fp, ip]
         L.orc_trace_primary.argtypes = [vp, C.POINTER(abi.rt_settings), C.c_int32, C.c_int32, ip, fp, fp, fp, i64p, i64p]
         L.orc_render.argtypes = [vp, C.POINTER(abi.rt_settings), C.POINTER(abi.rt_film), C.c_int32, C.c_int32,
                                  C.POINTER(abi.rt_render_stats)]
@@ -143,6 +144,17 @@ class OracleScene:
         pt = np.empty(n, np.int64) if counters else None
         lib().orc_trace_rays(self.h, _f(rays), n, _i(prim), _f(t), _f(u), _f(v), _i64(nt), _i64(pt))
         return (prim, t, u, v, nt, pt) if counters else (prim, t, u, v)
+
+    def trace_records(self, rays):
+        """Closest-hit records of world rays: dict of t (n,), point (n, 3),
+        normal (n, 3), front (n,) bool, prim (n,), material (n,)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = len(rays)
+        f = np.zeros((n, 8), np.float32)
+        m = np.zeros((n, 2), np.int32)
+        lib().orc_trace_records(self.h, _f(rays), n, _f(f), _i(m))
+        return {"t": f[:, 0], "point": f[:, 1:4], "normal": f[:, 4:7], "front": f[:, 7] != 0, "prim": m[:, 0],
+                "material": m[:, 1]}
 
     def texture_sample(self, uv):
         """Bilinear texture-set values (n, 5) at texture coordinates uv (n, 2)."""

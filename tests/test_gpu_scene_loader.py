@@ -16,7 +16,8 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
 
 
-@pytest.mark.parametrize("name", ["caustic-lens.yaml", "cornell-c1.yaml", "mesh-trio.yaml", "textured.yaml"])
+@pytest.mark.parametrize("name", ["caustic-lens.yaml", "cornell-c1.yaml", "mesh-trio.yaml", "textured.yaml",
+                                  "templates.yaml"])
 def test_yaml_scene_primary_hits_and_radiance(gpu_ctx, oracle, name):
     W, H, spp = 96, 72, 8
     desc = scene_loader.load(os.path.join(HERE, name), W, H)

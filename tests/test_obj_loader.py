@@ -152,20 +152,38 @@ def test_mesh_entities_in_the_scene_front_end():
     assert len(d.prims) == 1 + sum(counts) + 1 and kinds[0] == abi.RT_PRIM_PLANE and kinds[-1] == abi.RT_PRIM_QUAD
     ids = mesh[1:-1]
     assert list(np.unique(ids)) == list(range(6)) and list(np.bincount(ids)) == counts
+    # each mesh keeps its object-space faces under a TRANSFORM node
+    nodes, _, roots = d.hierarchy
+
+    def matrix_of(first):
+        for r in roots:
+            if nodes["kind"][r] == abi.RT_NODE_TRANSFORM and nodes["first"][nodes["child"][r]] == first:
+                return nodes["matrix"][r].reshape(4, 4).astype(np.float64)
+        raise AssertionError(first)
+
+    def world(first, pts):
+        M = matrix_of(first)
+        return np.c_[pts.reshape(-1, 3).astype(np.float64), np.ones(pts.size // 3)] @ M
+
     sph = obj_loader.load(os.path.join(MODELS, "test-sphere.obj"))
     s = d.prims["v"][1:961, :9].reshape(-1, 3, 3)
-    assert np.allclose(s, tris_of(sph) + np.float32([-3, 1, 0]), atol=1e-6)
+    assert np.array_equal(s, tris_of(sph))
+    assert np.allclose(world(1, s)[:, :3], (tris_of(sph) + np.float32([-3, 1, 0])).reshape(-1, 3), atol=1e-6)
     assert np.array_equal(d.normals[1:961], sph.normals) and np.array_equal(d.texcoords[1:961], sph.uvs)
-    cube = d.prims["v"][961:973, :9].reshape(-1, 3, 3)
-    assert np.allclose(cube.min((0, 1)), [0.75, 0.0, -0.75]) and np.allclose(cube.max((0, 1)), [2.25, 1.5, 0.75])
+    cube = world(961, d.prims["v"][961:973, :9])[:, :3]
+    assert np.allclose(cube.min(0), [0.75, 0.0, -0.75]) and np.allclose(cube.max(0), [2.25, 1.5, 0.75])
     assert not d.normals[961:973].any()
-    # the steel torus: rotate X 90 turns the ring's +y axis to +z; normals turn with it
-    st = slice(1 + 960 + 12 + 2 * 1536, 1 + 960 + 12 + 3 * 1536)
+    # the steel torus: rotate X 90 turns the ring's +y axis to +z (its normals
+    # turn with it when the renderer maps them by the transposed inverse)
+    first = 1 + 960 + 12 + 2 * 1536
+    st = slice(first, first + 1536)
     tor = obj_loader.load(os.path.join(MODELS, "test-torus-mesh.obj"))
     R = np.array([[1, 0, 0], [0, 0, -1], [0, 1, 0]], np.float64)
     want = tor.tris.reshape(-1, 3).astype(np.float64) @ R.T * 0.6 + [0, 1.5, 3]
-    assert np.allclose(d.prims["v"][st, :9].reshape(-1, 3), want, atol=1e-5)
-    nn = d.normals[st].reshape(-1, 3).astype(np.float64)
+    assert np.array_equal(d.prims["v"][st, :9], tor.tris.reshape(-1, 9))
+    assert np.allclose(world(first, d.prims["v"][st, :9])[:, :3], want, atol=1e-5)
+    M = matrix_of(first)
+    nn = d.normals[st].reshape(-1, 3).astype(np.float64) @ np.linalg.inv(M)[:3, :3].T
     nn /= np.linalg.norm(nn, axis=1, keepdims=True)
     assert np.allclose(nn, tor.normals.reshape(-1, 3).astype(np.float64) @ R.T, atol=1e-6)
 
@@ -197,11 +215,18 @@ def test_mesh_entity_skips_and_refusals(tmp_path):
     with pytest.raises(scene_loader.SceneLoadError, match="subdivision"):
         scene_loader.load_data(dict(base, entities=[{"type": "mesh", "path": cube, "subdivision_iterations": 2}]),
                                8, 8)
+    # any scale of a smooth mesh: its object-space faces and normals under a
+    # TRANSFORM node (the renderer maps the interpolated normal by the
+    # transposed inverse)
     sph = os.path.join(MODELS, "test-sphere.obj")
-    with pytest.raises(scene_loader.SceneLoadError, match="non-uniform"):
-        scene_loader.load_data(dict(base, entities=[{"type": "mesh", "path": sph, "scale": [1, 2, 1]}]), 8, 8)
+    d = scene_loader.load_data(dict(base, entities=[{"type": "mesh", "path": sph, "scale": [1, 2, 1]}]), 8, 8)
+    nodes, _, roots = d.hierarchy
+    assert nodes["kind"][roots[0]] == abi.RT_NODE_TRANSFORM
+    assert np.allclose(np.diag(nodes["matrix"][roots[0]].reshape(4, 4)), [1, 2, 1, 1])
+    assert np.array_equal(d.normals, obj_loader.load(sph).normals)
     # a flat mesh takes any scale; an emissive mesh cannot be a geometry light here
     d = scene_loader.load_data(dict(base, entities=[{"type": "mesh", "path": cube, "scale": [1, 2, 3]}]), 8, 8)
-    assert np.allclose(d.prims["v"][:, :9].reshape(-1, 3).max(0), [0.5, 1.0, 1.5])
+    assert np.allclose(d.prims["v"][:, :9].reshape(-1, 3).max(0), [0.5, 0.5, 0.5])
+    assert np.allclose(np.diag(d.hierarchy[0]["matrix"][d.hierarchy[2][0]].reshape(4, 4)), [1, 2, 3, 1])
     with pytest.raises(ValueError, match="emissive meshes"):
         scene_loader.load_data(dict(base, entities=[{"type": "mesh", "path": cube, "material": "e"}]), 8, 8)

@@ -75,14 +75,42 @@ def test_transforms_are_applied_in_reference_order():
         x, y = x * math.cos(math.pi / 2) - y * math.sin(math.pi / 2), x * math.sin(math.pi / 2) + y * math.cos(math.pi / 2)
         return [x + 1, y + 2, z + 3]
 
+    # each entity keeps its object-space primitive under a TRANSFORM node
+    nodes, _, roots = d.hierarchy
+    assert len(roots) == 3 and np.all(nodes["kind"][roots] == abi.RT_NODE_TRANSFORM)
+
+    def world(k, p):      # Vector3.Transform with the node's row-vector matrix
+        M = nodes["matrix"][roots[k]].reshape(4, 4).astype(np.float64)
+        return np.array([*p, 1.0]) @ M
+
+    def world_vec(k, v):
+        return np.array(v, np.float64) @ nodes["matrix"][roots[k]].reshape(4, 4)[:3, :3]
+
+    tri = d.prims["v"][0][:9].reshape(3, 3)
+    assert np.allclose(tri, np.eye(3))
+    for k, p in enumerate(([1, 0, 0], [0, 1, 0], [0, 0, 1])):
+        assert np.allclose(world(0, p)[:3], ref(p), atol=1e-6)
+    assert np.allclose(d.prims["v"][1][:4], [1, 0, 0, 0.5])
+    assert np.allclose(world(1, [1, 0, 0])[:3], [3, 0, -1]) and np.allclose(world_vec(1, [0.5, 0, 0]), [1.5, 0, 0])
+    assert np.allclose(world_vec(2, [1, 0, 0]), [0, 0, -1], atol=1e-6)   # rotate Y 90: u = +x -> -z
+    for k in range(3):
+        M = nodes["matrix"][roots[k]].reshape(4, 4).astype(np.float64)
+        assert np.allclose(nodes["inverse"][roots[k]].reshape(4, 4) @ M, np.eye(4), atol=1e-6)
+    # an emissive entity keeps the transform applied to its points (its geometry
+    # light samples world-space primitives); so does every entity of a scene with
+    # image textures, where a non-uniformly scaled sphere is refused
+    doc["materials"].append({"id": "e", "type": "emissive", "color": [1, 1, 1], "intensity": 2})
+    doc["entities"][0]["material"] = "e"
+    d = scene_loader.load_data(doc, 64, 64)
     tri = d.prims["v"][0][:9].reshape(3, 3)
     for k, p in enumerate(([1, 0, 0], [0, 1, 0], [0, 0, 1])):
         assert np.allclose(tri[k], ref(p), atol=1e-6)
-    assert np.allclose(d.prims["v"][1][:4], [3, 0, -1, 1.5])
-    q = d.prims["v"][2][:9].reshape(3, 3)       # rotate Y 90: u = +x -> -z
-    assert np.allclose(q, [[0, 0, 0], [0, 0, -1], [0, 1, 0]], atol=1e-6)
+    assert len(d.hierarchy[2]) == 3        # the emissive triangle and the two transforms
+    img = os.path.join(HERE, "..", "textures", "checkerboard.png")
     with pytest.raises(scene_loader.SceneLoadError, match="non-uniform"):
-        scene_loader.load_data({"lights": [], "entities": [{"type": "sphere", "scale": [1, 2, 1]}]}, 8, 8)
+        scene_loader.load_data({"lights": [], "materials": [
+            {"id": "t", "type": "lambertian", "texture": {"type": "image", "path": img}}],
+            "entities": [{"type": "sphere", "scale": [1, 2, 1]}]}, 8, 8)
 
 
 def test_defaults_camera_selection_and_focus():
@@ -142,7 +170,9 @@ def test_disney_fields_and_ground_material():
     ({"lights": [], "world": {"sky": {"type": "hdri"}}}, "sky type"),
     ({"lights": [], "world": {"ground": {"type": "quad"}}}, "ground type"),
     ({"lights": [], "entities": [{"type": "sphere", "motion": [{"t": 1}]}]}, "motion"),
-    ({"lights": [], "imports": ["x.yaml"]}, "imports"),
+    ({"lights": [], "entities": [{"type": "group", "children": [{"type": "csg"}]}]}, "type 'csg'"),
+    ({"lights": [], "materials": [{"id": "e", "type": "emissive"}],
+      "entities": [{"type": "group", "children": [{"type": "sphere", "material": "e"}]}]}, "emissive objects"),
 ])
 def test_unsupported_features_are_refused(doc, what):
     with pytest.raises(scene_loader.SceneLoadError, match=what):
@@ -178,3 +208,80 @@ def test_ground_uv_transform_is_refused(key, value, ok):
     else:
         with pytest.raises(scene_loader.SceneLoadError, match=key):
             scene_loader.load_data(doc, 8, 8)
+
+
+def _tree_doc():
+    return {
+        "lights": [{"type": "sphere", "position": [0, 5, 0], "radius": 0.2, "intensity": 20}],
+        "materials": [{"id": "bark", "type": "lambertian", "color": [0.4, 0.3, 0.2]},
+                      {"id": "leaf", "type": "disney", "color": [0.1, 0.6, 0.1], "roughness": 0.5},
+                      {"id": "gold", "type": "lambertian", "color": [0.9, 0.7, 0.1]}],
+        "templates": [{"name": "tree", "material": "bark", "translate": [0, 0.5, 0],
+                       "children": [{"type": "sphere", "radius": 0.5, "material": "leaf", "translate": [0, 1, 0]},
+                                    {"type": "quad", "q": [-0.1, 0, 0], "u": [0.2, 0, 0], "v": [0, 1, 0]}]}],
+        "entities": [{"type": "instance", "template": "tree", "translate": [-2, 0, 0]},
+                     {"type": "instance", "template": "Tree", "material": "gold", "rotate": [0, 45, 0],
+                      "scale": [1, 2, 1]},
+                     {"type": "instance", "template": "missing"},
+                     {"type": "group", "translate": [2, 0, 0], "material": "leaf",
+                      "children": [{"type": "sphere", "radius": 0.2, "center": [0, 0, k * 0.5]} for k in range(6)]},
+                     {"type": "sphere", "radius": 0.3, "material": "bark"}],
+    }
+
+
+def test_templates_instances_and_groups():
+    warnings = []
+    d = scene_loader.load_data(_tree_doc(), 64, 48, warnings=warnings)
+    nodes, children, roots = d.hierarchy
+    kind = nodes["kind"]
+    assert any("template 'missing' not found" in w for w in warnings)
+    # roots: two instances, the group, the sphere, the light's proxy (object order)
+    assert len(roots) == 5
+    t0, t1, g, sph, proxy = roots
+    assert kind[t0] == kind[t1] == kind[g] == abi.RT_NODE_TRANSFORM
+    i0, i1 = nodes["child"][t0], nodes["child"][t1]
+    assert kind[i0] == kind[i1] == abi.RT_NODE_INSTANCE
+    # one template build shared by both instances (case-insensitive name); the
+    # second instance overrides the material
+    assert nodes["child"][i0] == nodes["child"][i1]
+    assert nodes["material"][i0] == -1 and nodes["material"][i1] == 2
+    shared = nodes["child"][i0]
+    assert kind[shared] == abi.RT_NODE_TRANSFORM                 # the template's default pose
+    assert np.allclose(nodes["matrix"][shared].reshape(4, 4)[3, :3], [0, 0.5, 0])
+    grp = nodes["child"][shared]
+    assert kind[grp] == abi.RT_NODE_GROUP and nodes["count"][grp] == 2
+    kids = children[nodes["first"][grp]:nodes["first"][grp] + 2]
+    assert kind[kids[0]] == abi.RT_NODE_TRANSFORM and kind[kids[1]] == abi.RT_NODE_PRIMITIVES
+    leaf_prim = nodes["first"][nodes["child"][kids[0]]]
+    quad_prim = nodes["first"][kids[1]]
+    assert d.prims["material"][leaf_prim] == 1 and d.prims["material"][quad_prim] == 0   # own / fallback
+    # the group: 6 children (a BVH in the renderer), the group's material as fallback
+    gg = nodes["child"][g]
+    assert kind[gg] == abi.RT_NODE_GROUP and nodes["count"][gg] == 6
+    assert np.all(d.prims["material"][nodes["first"][children[nodes["first"][gg]:nodes["first"][gg] + 6]]] == 1)
+    assert kind[sph] == abi.RT_NODE_PRIMITIVES and kind[proxy] == abi.RT_NODE_PRIMITIVES
+    # the instanced template's primitives exist once
+    assert len(d.prims) == 2 + 6 + 1 + 1
+
+
+def test_imports_merge_before_local(tmp_path):
+    lib = tmp_path / "lib.yaml"
+    lib.write_text("materials:\n  - {id: red, type: lambertian, color: [1, 0, 0]}\n"
+                   "  - {id: blue, type: lambertian, color: [0, 0, 1]}\n"
+                   "entities:\n  - {type: sphere, material: red, radius: 0.25}\n"
+                   "templates:\n  - name: pebble\n    children: [{type: sphere, radius: 0.1, material: blue}]\n")
+    main = tmp_path / "main.yaml"
+    main.write_text("imports: [{path: lib.yaml}, {path: lib.yaml}, {path: nowhere.yaml}]\n"
+                    "lights: []\n"
+                    "materials:\n  - {id: red, type: lambertian, color: [0, 1, 0]}\n"
+                    "entities:\n  - {type: sphere, material: red, radius: 0.5, center: [1, 0, 0]}\n"
+                    "  - {type: instance, template: pebble, translate: [0, 1, 0]}\n")
+    warnings = []
+    d = scene_loader.load(str(main), 32, 32, warnings=warnings)
+    assert any("Circular import" in w for w in warnings) and any("not found" in w for w in warnings)
+    # imported entity first, then the local ones; the local 'red' wins for both
+    assert np.allclose(d.prims["v"][:2, 3], [0.25, 0.5])
+    reds = d.mats["color"][d.prims["material"][:2]]
+    assert np.allclose(reds, [[0, 1, 0], [0, 1, 0]])
+    nodes, _, roots = d.hierarchy
+    assert len(roots) == 3 and nodes["kind"][roots[2]] == abi.RT_NODE_TRANSFORM

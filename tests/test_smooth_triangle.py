@@ -148,12 +148,22 @@ def test_builder_and_yaml_tables():
     d = scene_loader.load_data(doc, 16, 16)
     n = d.normals
     assert n.shape == (4, 9) and not n[0].any() and not n[3].any()
-    # rotate Z 90 maps the normals (0,1,1) -> (-1,0,1), (1,0,1) -> (0,1,1); a uniform scale does not turn them
-    assert np.allclose(n[1].reshape(3, 3), [[0, 0, 0.5], [-0.5, 0, 0.5], [0, 0.5, 0.5]], atol=1e-6)
+    # the transformed smooth triangle keeps its object-space normals; its
+    # TRANSFORM node's transposed inverse maps (0,1,1) -> (-1,0,1) and
+    # (1,0,1) -> (0,1,1) (rotate Z 90; the uniform scale does not turn them)
+    assert np.allclose(n[1].reshape(3, 3), [[0, 0, 1], [0, 1, 1], [1, 0, 1]], atol=1e-6)
+    nodes, _, roots = d.hierarchy
+    M = nodes["matrix"][roots[1]].reshape(4, 4).astype(np.float64)
+    assert nodes["kind"][roots[1]] == abi.RT_NODE_TRANSFORM and nodes["first"][nodes["child"][roots[1]]] == 1
+    w = n[1].reshape(3, 3).astype(np.float64) @ np.linalg.inv(M)[:3, :3].T
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    want = np.array([[0, 0, 1], [-1, 0, 1], [0, 1, 1]], np.float64)
+    assert np.allclose(w, want / np.linalg.norm(want, axis=1, keepdims=True), atol=1e-6)
     assert np.allclose(n[2], [0, 0, 1] * 3)
     assert np.allclose(d.texcoords[1], [0, 0, 2, 0, 0, 2]) and np.allclose(d.texcoords[2], [0, 0, 1, 0, 0, 1])
-    with pytest.raises(scene_loader.SceneLoadError, match="non-uniform"):
-        scene_loader.load_data({"lights": [], "entities": [dict(doc["entities"][1], scale=[1, 2, 1])]}, 8, 8)
+    # any scale now: a non-uniform one stays on the TRANSFORM node
+    d = scene_loader.load_data({"lights": [], "entities": [dict(doc["entities"][1], scale=[1, 2, 1])]}, 8, 8)
+    assert np.allclose(np.diag(d.hierarchy[0]["matrix"][d.hierarchy[2][0]].reshape(4, 4))[:3] @ [0, 0, 1], 1.0)
 
 
 def test_refusals(oracle):
