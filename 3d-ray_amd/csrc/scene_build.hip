@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <functional>
 #include <future>
@@ -275,6 +276,17 @@ static std::unique_ptr<BNode> build_node(std::vector<int32_t>& items, std::vecto
     return n;
 }
 
+// Device copy of a box: an empty box (min > max on an axis, AABB.Empty) as
+// NaN bounds, which the traversal's NaN-propagating slab test rejects as
+// AABB.Hit rejects an empty box (AABB.cs:36).
+static Box dev_box(const Box& b) {
+    if (b.mn.x > b.mx.x || b.mn.y > b.mx.y || b.mn.z > b.mx.z) {
+        const float q = std::numeric_limits<float>::quiet_NaN();
+        return Box{mk(q, q, q), mk(q, q, q)};
+    }
+    return b;
+}
+
 struct Flattener {
     HostScene& hs;
     const rt_scene_desc* d;
@@ -349,8 +361,8 @@ struct Flattener {
         hs.nodes.push_back(GNode{});
         uint32_t rl = flatten(n->l.get(), items, prim_items, depth + 1);
         uint32_t rr = flatten(n->r.get(), items, prim_items, depth + 1);
-        const Box& a = n->l->box;
-        const Box& b = n->r->box;
+        const Box a = dev_box(n->l->box);
+        const Box b = dev_box(n->r->box);
         GNode& g = hs.nodes[idx];
         g.a = make_float4(a.mn.x, a.mn.y, a.mn.z, a.mx.x);
         g.b = make_float4(a.mx.y, a.mx.z, b.mn.x, b.mn.y);
@@ -360,7 +372,8 @@ struct Flattener {
     }
 };
 
-static GEntry bvh_entry(const Box& b, uint32_t ref, int32_t flags) {
+static GEntry bvh_entry(const Box& box, uint32_t ref, int32_t flags) {
+    const Box b = dev_box(box);
     GEntry e{};
     e.a = make_float4(b.mn.x, b.mn.y, b.mn.z, ibits(kBvh | (flags << 8)));
     e.b = make_float4(b.mx.x, b.mx.y, b.mx.z, ibits((int32_t)ref));

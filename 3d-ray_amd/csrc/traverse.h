@@ -95,20 +95,24 @@ __device__ __forceinline__ bool hit_plane(f3 o, f3 d, f3 p, f3 n, float tmin, fl
     return true;
 }
 
-// AABB.Hit (AABB.cs:28-52).  min/max of the two slab distances use the
-// hardware maxNum/minNum; a NaN anywhere in t0/t1 (origin on a slab plane of
-// an axis-parallel ray) makes the reference's NaN-propagating reductions fail
-// the `tExit >= tEnter` test, which the explicit NaN check reproduces.
+// AABB.Hit (AABB.cs:28-52).  The reference's Vector3.Min/Max and
+// MathF.Max/Min propagate NaN (a NaN slab distance comes from an
+// axis-parallel ray whose origin lies on a slab plane: 0 * inf), so such a
+// box fails `tExit >= tEnter`; gfx950's v_minimum3 / v_maximum3 are those
+// IEEE 754-2019 operations.  The reference's explicit empty-box rejection
+// (Min > Max on an axis) is folded into the data: the host stores an empty
+// box as NaN bounds, which fail the same comparison.
+__device__ __forceinline__ float min_nan(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+__device__ __forceinline__ float max_nan(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+
 __device__ __forceinline__ bool slab(f3 mn_, f3 mx_, f3 o, f3 inv, float tmin, float tmax, float& te) {
     float t0x = (mn_.x - o.x) * inv.x, t0y = (mn_.y - o.y) * inv.y, t0z = (mn_.z - o.z) * inv.z;
     float t1x = (mx_.x - o.x) * inv.x, t1y = (mx_.y - o.y) * inv.y, t1z = (mx_.z - o.z) * inv.z;
-    float tsx = fminf(t0x, t1x), tsy = fminf(t0y, t1y), tsz = fminf(t0z, t1z);
-    float tlx = fmaxf(t0x, t1x), tly = fmaxf(t0y, t1y), tlz = fmaxf(t0z, t1z);
-    te = fmaxf(tmin, fmaxf(tsx, fmaxf(tsy, tsz)));
-    float tx = fminf(tmax, fminf(tlx, fminf(tly, tlz)));
-    bool nan = (t0x != t0x) | (t0y != t0y) | (t0z != t0z) | (t1x != t1x) | (t1y != t1y) | (t1z != t1z);
-    bool empty = (mn_.x > mx_.x) | (mn_.y > mx_.y) | (mn_.z > mx_.z);
-    return (tx >= te) & !nan & !empty;
+    float tsx = min_nan(t0x, t1x), tsy = min_nan(t0y, t1y), tsz = min_nan(t0z, t1z);
+    float tlx = max_nan(t0x, t1x), tly = max_nan(t0y, t1y), tlz = max_nan(t0z, t1z);
+    te = max_nan(tmin, max_nan(tsx, max_nan(tsy, tsz)));
+    float tx = min_nan(tmax, min_nan(tlx, min_nan(tly, tlz)));
+    return tx >= te;
 }
 
 struct HitOut {
@@ -485,17 +489,14 @@ __device__ __forceinline__ HitOut trace_closest_dp(const GNode* __restrict__ nod
     return h;
 }
 
-// Box test of the any-hit traversal: the same slab distances as slab(),
-// without the NaN and empty-box guards.  A NaN slab distance only arises for
-// an axis-parallel ray whose origin lies exactly on a box plane (the
-// hardware min/max then drop it and the box is entered, where the reference
-// would skip it); unused child slots hold a box far outside the scene whose
-// reference is an empty leaf.
+// Box test of the any-hit traversal: the same test as slab() without the
+// entry distance.  Unused child slots hold a point box far outside the scene
+// whose reference is an empty leaf.
 __device__ __forceinline__ bool slab_any(f3 mn_, f3 mx_, f3 o, f3 inv, float tmin, float tmax) {
     float t0x = (mn_.x - o.x) * inv.x, t0y = (mn_.y - o.y) * inv.y, t0z = (mn_.z - o.z) * inv.z;
     float t1x = (mx_.x - o.x) * inv.x, t1y = (mx_.y - o.y) * inv.y, t1z = (mx_.z - o.z) * inv.z;
-    float te = fmaxf(tmin, fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z))));
-    float tx = fminf(tmax, fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z))));
+    float te = max_nan(tmin, max_nan(min_nan(t0x, t1x), max_nan(min_nan(t0y, t1y), min_nan(t0z, t1z))));
+    float tx = min_nan(tmax, min_nan(max_nan(t0x, t1x), min_nan(max_nan(t0y, t1y), max_nan(t0z, t1z))));
     return tx >= te;
 }
 
