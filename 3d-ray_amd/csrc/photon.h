@@ -405,9 +405,12 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
 
 // ── gather (PhotonMap.GatherKNearest + Renderer.GatherCaustics) ────────────
 struct DevPhotons {
-    const float4* pos;
+    const float4* pos;      // photon order (the deposited set as traced)
     const float4* inc;
     const float4* pw;
+    const float4* spos;     // the same records in bucket order: entry kk is
+    const float4* sinc;     // photon idx[kk], so a cell's photons are one
+    const float4* spw;      // contiguous run of start[b] .. start[b+1]
     const int32_t* start;
     const int32_t* idx;
     const float4* origin;   // device scalar (x, y, z)
@@ -416,46 +419,70 @@ struct DevPhotons {
     int n;
 };
 
-// Max-heap of the k nearest photons (PhotonMap's insertion order and sift
-// rules).  Entry j of the heap lives at hi[j * S], hd[j * S]: the gather
-// keeps one heap per lane in LDS, lane-interleaved (S = the block size).
+// Photon records in bucket order (a gather of the three planes by idx).
+__global__ void __launch_bounds__(kBlock) k_photon_order(int n, const int32_t* __restrict__ idx,
+                                                         const float4* __restrict__ ip, const float4* __restrict__ ii,
+                                                         const float4* __restrict__ iw, float4* __restrict__ op,
+                                                         float4* __restrict__ oi, float4* __restrict__ ow) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    int pi = idx[k];
+    op[k] = ip[pi];
+    oi[k] = ii[pi];
+    ow[k] = iw[pi];
+}
+
+// Max-heap of the k nearest photons with PhotonMap's insertion order and
+// sift rules (PhotonMap.cs:296-317).  Entry j is (squared distance, record
+// index) at h[j * S]: the gather keeps one heap per lane in LDS, lane-
+// interleaved (S = the block size), one 8-byte access per entry.  The sifts
+// move a hole instead of swapping pairs; every comparison is the one the
+// swapping form makes (the moving entry keeps its distance), so the layout
+// after each insertion is the same.
 template <int S>
-__device__ __forceinline__ void heap_insert(int* hi, float* hd, int& count, int pi, float d2) {
+__device__ __forceinline__ void heap_insert(float2* h, int& count, int id, float d2) {
+    const float2 e = make_float2(d2, __int_as_float(id));
     if (count < kGatherK) {
         int i = count++;
-        hi[i * S] = pi;
-        hd[i * S] = d2;
         while (i > 0) {
-            int parent = (i - 1) >> 1;
-            const float hp = hd[parent * S], hc = hd[i * S];
-            if (hp >= hc) break;
-            hd[parent * S] = hc;
-            hd[i * S] = hp;
-            int ti = hi[parent * S]; hi[parent * S] = hi[i * S]; hi[i * S] = ti;
+            const int parent = (i - 1) >> 1;
+            const float2 hp = h[parent * S];
+            if (hp.x >= d2) break;
+            h[i * S] = hp;
             i = parent;
         }
-    } else if (d2 < hd[0]) {
-        hi[0] = pi;
-        hd[0] = d2;
+        h[i * S] = e;
+    } else if (d2 < h[0].x) {
         int i = 0;
         for (;;) {
-            int l = 2 * i + 1, r = 2 * i + 2, lg = i;
-            float hlg = hd[i * S];
-            if (l < count && hd[l * S] > hlg) { lg = l; hlg = hd[l * S]; }
-            if (r < count && hd[r * S] > hlg) { lg = r; hlg = hd[r * S]; }
+            const int l = 2 * i + 1, r = 2 * i + 2;
+            int lg = i;
+            float hlg = d2;
+            float2 elg = e;
+            if (l < count) {
+                const float2 hl = h[l * S];
+                if (hl.x > hlg) { lg = l; hlg = hl.x; elg = hl; }
+            }
+            if (r < count) {
+                const float2 hr = h[r * S];
+                if (hr.x > hlg) { lg = r; hlg = hr.x; elg = hr; }
+            }
             if (lg == i) break;
-            hd[lg * S] = hd[i * S];
-            hd[i * S] = hlg;
-            int ti = hi[lg * S]; hi[lg * S] = hi[i * S]; hi[i * S] = ti;
+            h[i * S] = elg;
             i = lg;
         }
+        h[i * S] = e;
     }
 }
 
-// k nearest photons within max_r of p; returns the count and the kernel
-// radius (distance of the farthest gathered photon).
+// k nearest photons within max_r of p (PhotonMap.GatherKNearest,
+// PhotonMap.cs:209-294): rings of cells by Chebyshev distance, cells beyond
+// the radius cap or the current k-th distance skipped, stop once the k-th
+// distance is inside the scanned reach.  Returns the count and the kernel
+// radius (distance of the farthest gathered photon); the heap holds
+// bucket-order record indices.
 template <int S>
-__device__ int photon_knn(const DevPhotons& M, f3 p, int* hi, float* hd, float& radius) {
+__device__ int photon_knn(const DevPhotons& M, f3 p, float2* h, float& radius) {
     radius = 0.0f;
     if (M.n <= 0) return 0;
     f3 org = f3of(M.origin[0]);
@@ -466,36 +493,62 @@ __device__ int photon_knn(const DevPhotons& M, f3 p, int* hi, float* hd, float& 
     const int max_rings = (int)ceilf(M.max_r * M.inv_cell) + 1;
     for (int ring = 0; ring <= max_rings; ring++) {
         for (int z = cz - ring; z <= cz + ring; z++)
-            for (int y = cy - ring; y <= cy + ring; y++)
-                for (int x = cx - ring; x <= cx + ring; x++) {
-                    int cheb = max(abs(x - cx), max(abs(y - cy), abs(z - cz)));
-                    if (cheb != ring) continue;
+            for (int y = cy - ring; y <= cy + ring; y++) {
+                // rows inside the shell hold only its two end cells
+                const bool face = ring == 0 || abs(z - cz) == ring || abs(y - cy) == ring;
+                const int xstep = face ? 1 : 2 * ring;
+                for (int x = cx - ring; x <= cx + ring; x += xstep) {
                     float bx = org.x + (float)x * M.cell, by = org.y + (float)y * M.cell, bz = org.z + (float)z * M.cell;
                     float dx = fmaxf(fmaxf(bx - p.x, p.x - (bx + M.cell)), 0.0f);
                     float dy = fmaxf(fmaxf(by - p.y, p.y - (by + M.cell)), 0.0f);
                     float dz = fmaxf(fmaxf(bz - p.z, p.z - (bz + M.cell)), 0.0f);
                     float md2 = (dx * dx + dy * dy) + dz * dz;
                     if (md2 > max_r2) continue;
-                    if (count >= kGatherK && md2 >= hd[0]) continue;
+                    if (count >= kGatherK && md2 >= h[0].x) continue;
                     uint32_t b = photon_bucket(x, y, z, M.mask);
-                    int k1 = M.start[b + 1];
-                    for (int kk = M.start[b]; kk < k1; kk++) {
-                        int pi = M.idx[kk];
-                        f3 pos = f3of(M.pos[pi]);
+                    const int k0 = M.start[b], k1 = M.start[b + 1];
+                    if (k0 >= k1) continue;
+                    // the next record is in flight while this one is tested
+                    float4 nxt = M.spos[k0];
+                    for (int kk = k0; kk < k1; kk++) {
+                        const f3 pos = f3of(nxt);
+                        if (kk + 1 < k1) nxt = M.spos[kk + 1];
                         int px, py, pz;
                         photon_cell(pos, org, M.inv_cell, px, py, pz);
                         if (px != x || py != y || pz != z) continue;
                         float d2 = length_sq(pos - p);
                         if (d2 > max_r2) continue;
-                        heap_insert<S>(hi, hd, count, pi, d2);
+                        heap_insert<S>(h, count, kk, d2);
                     }
                 }
+            }
         float reach = (float)ring * M.cell;
-        if (count >= kGatherK && hd[0] <= reach * reach) break;
+        if (count >= kGatherK && h[0].x <= reach * reach) break;
         if (reach > M.max_r) break;
     }
-    radius = count > 0 ? sqrtf(hd[0]) : 0.0f;
+    radius = count > 0 ? sqrtf(h[0].x) : 0.0f;
     return count;
+}
+
+// One k-nearest search per query point, results in heap order as photon-
+// order indices (the record's idx entry), -1 past the count.
+__global__ void __launch_bounds__(kBlock) k_photon_knn_query(DevPhotons M, const float* __restrict__ pts, int n,
+                                                             int32_t* __restrict__ cnt, float* __restrict__ rad,
+                                                             int32_t* __restrict__ out) {
+    __shared__ float2 lds_h[kGatherK * kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    f3 p;
+    p.x = pts[3 * i];
+    p.y = pts[3 * i + 1];
+    p.z = pts[3 * i + 2];
+    float2* h = &lds_h[threadIdx.x];
+    float r;
+    const int c = photon_knn<kBlock>(M, p, h, r);
+    cnt[i] = c;
+    rad[i] = r;
+    for (int j = 0; j < kGatherK; j++)
+        out[(int64_t)i * kGatherK + j] = j < c ? M.idx[__float_as_int(h[j * kBlock].y)] : -1;
 }
 
 }  // namespace rt

@@ -1994,12 +1994,10 @@ __global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPh
     s.rel_eta = c.z;
     const int p = __float_as_int(c.w);
     // the lane's heap of the k nearest photons, lane-interleaved in LDS
-    __shared__ int lds_hi[kGatherK * kBlock];
-    __shared__ float lds_hd[kGatherK * kBlock];
-    int* hi = &lds_hi[threadIdx.x];
-    float* hd = &lds_hd[threadIdx.x];
+    __shared__ float2 lds_h[kGatherK * kBlock];
+    float2* h = &lds_h[threadIdx.x];
     float radius;
-    int n = photon_knn<kBlock>(M, s.p, hi, hd, radius);
+    int n = photon_knn<kBlock>(M, s.p, h, radius);
     if (n == 0) return;
     bool under = n < kGatherK;
     if (under) radius = M.max_r;
@@ -2011,16 +2009,17 @@ __global__ void __launch_bounds__(kBlock) k_gather(DevScene S, PathBufs P, DevPh
     f3 view = normalize(-d);
     float inv_r = 1.0f / radius;
     f3 sum = splat(0.0f);
+    // heap order, as PhotonMap hands the set back
     for (int k = 0; k < n; k++) {
-        int pi = hi[k * kBlock];
-        f3 inc = f3of(M.inc[pi]);
+        const int kk = __float_as_int(h[k * kBlock].y);
+        f3 inc = f3of(M.sinc[kk]);
         if (dot(inc, s.n) <= 0.0f) continue;
-        float wgt = 1.0f - length(f3of(M.pos[pi]) - s.p) * inv_r / 1.0f;
+        float wgt = 1.0f - length(f3of(M.spos[kk]) - s.p) * inv_r / 1.0f;
         if (wgt <= 0.0f) continue;
         f3 f;
         if (disney) f = disney_eval(DP, S.lut_e, view, inc, s);
         else f = dot(s.n, inc) <= 0.0f ? splat(0.0f) : f3of(g.m1) * (1.0f / kPi);
-        sum += f * (f3of(M.pw[pi]) * wgt);
+        sum += f * (f3of(M.spw[kk]) * wgt);
     }
     const float norm = 1.0f - 2.0f / (3.0f * 1.0f);
     f3 est = sum * (1.0f / (norm * kPi * radius * radius));
@@ -2806,7 +2805,6 @@ struct PhotonPass {
     DevPhotons M{};
     int64_t emitted = 0;
     int n = 0, nb = 0;
-    const float4* slot_pos = nullptr;
     const uint32_t* key0 = nullptr;   // bucket of each photon, photon order
 };
 
@@ -2902,7 +2900,6 @@ static int photon_pass(rt_ctx* ctx, const rt_scene* sc, const DevScene& S, int b
     HIP_TRY(hipStreamSynchronize(st));
     if (hv[1]) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow in the photon pass");
     const int n = hv[0];
-    out.slot_pos = sp;
     if (n == 0) return RT_OK;
     hipLaunchKernelGGL(k_photon_compact, g, dim3(kBlock), 0, st, ntot, flag, posof, sp, si, sw, cp, ci, cw);
     // grid: origin = min position, cell, power-of-two bucket table
@@ -2930,6 +2927,9 @@ static int photon_pass(rt_ctx* ctx, const rt_scene* sc, const DevScene& S, int b
         kin = kout;
         vin = vout;
     }
+    // photon records in bucket order, into the (now free) per-slot planes
+    hipLaunchKernelGGL(k_photon_order, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n, vin, cp, ci, cw, sp, si,
+                       sw);
     // CSR bucket starts
     dev_scan(st, cnt, start, nb, bsum);
     HIP_TRY(hipMemcpyAsync(start + nb, &hv[0], 4, hipMemcpyHostToDevice, st));
@@ -2942,6 +2942,9 @@ static int photon_pass(rt_ctx* ctx, const rt_scene* sc, const DevScene& S, int b
     M.pos = cp;
     M.inc = ci;
     M.pw = cw;
+    M.spos = sp;
+    M.sinc = si;
+    M.spw = sw;
     M.start = start;
     M.idx = vin;
     M.origin = org;
@@ -3376,6 +3379,56 @@ int rt_photon_map(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, int64_t
         grid[2] = o.z;
         grid[3] = PP.M.cell;
     }
+    return RT_OK;
+}
+
+int rt_photon_gather(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const float* points, int64_t n_points,
+                     int32_t* count, float* radius, int32_t* photon_idx) {
+    if (!ctx || !sc || !s) return RT_ERR_INVALID_ARGUMENT;
+    if (n_points < 0 || (n_points > 0 && (!points || !count || !radius || !photon_idx)))
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "points and all three outputs are required");
+    if (n_points > (1ll << 26)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "more than 2^26 query points");
+    if (s->caustic_photons < 0 || s->caustic_gather_radius < 0.0f)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "negative photon budget or gather radius");
+    hipSetDevice(ctx->device);
+    const HostScene& h = sc->hs;
+    if (h.hier) return fail(ctx, RT_ERR_UNSUPPORTED, "photon maps of a scene with an object hierarchy are not supported");
+    const float gr = s->caustic_gather_radius > 0.0f ? s->caustic_gather_radius
+                                                     : std::max(h.scene_radius * 0.06f, 1e-4f);
+    DevScene S = dev_view(sc);
+    PhotonPass PP;
+    int rc = photon_pass(ctx, sc, S, s->caustic_photons, gr * 0.5f, PP);
+    if (rc) return rc;
+    const int n = (int)n_points;
+    if (n == 0) return RT_OK;
+    if (PP.n == 0) {
+        for (int i = 0; i < n; i++) {
+            count[i] = 0;
+            radius[i] = 0.0f;
+            for (int j = 0; j < kGatherK; j++) photon_idx[(int64_t)i * kGatherK + j] = -1;
+        }
+        return RT_OK;
+    }
+    PP.M.max_r = gr;
+    const size_t b_pts = 12ull * n, b_cnt = 4ull * n, b_idx = 4ull * kGatherK * n;
+    char* mem = nullptr;
+    HIP_TRY(hipMalloc((void**)&mem, b_pts + 2 * b_cnt + b_idx + 64));
+    float* d_pts = (float*)mem;
+    int32_t* d_cnt = (int32_t*)(mem + ((b_pts + 15) & ~15ull));
+    float* d_rad = (float*)((char*)d_cnt + b_cnt);
+    int32_t* d_idx = (int32_t*)((char*)d_rad + b_cnt);
+    hipError_t e = hipMemcpyAsync(d_pts, points, b_pts, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_photon_knn_query, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, PP.M,
+                           d_pts, n, d_cnt, d_rad, d_idx);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(count, d_cnt, b_cnt, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(radius, d_rad, b_cnt, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(photon_idx, d_idx, b_idx, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(mem);
+    if (e != hipSuccess) return fail(ctx, RT_ERR_DEVICE, std::string("photon_gather: ") + hipGetErrorString(e));
     return RT_OK;
 }
 

@@ -67,6 +67,7 @@ def load():
     i64p = C.POINTER(C.c_int64)
     L.rt_photon_map.argtypes = [vp, vp, C.POINTER(abi.rt_settings), C.c_int64, C.c_int64, i64p, i64p, i64p, fp, fp, fp,
                                 ip, ip, ip, fp]
+    L.rt_photon_gather.argtypes = [vp, vp, C.POINTER(abi.rt_settings), fp, C.c_int64, ip, fp, ip]
     L.rt_render_batch.argtypes = [vp, C.POINTER(abi.rt_batch_job), C.c_int32, C.c_int32, C.POINTER(abi.rt_render_stats)]
     L.rt_context_tune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
     L.rt_adjoint.argtypes = [vp, vp, C.POINTER(abi.rt_settings), fp, fp, C.POINTER(abi.rt_render_stats)]
@@ -83,10 +84,14 @@ def load():
     return L
 
 
+GATHER_K = 40   # Renderer.CausticGatherK
+
+
 EXPORTED_SYMBOLS = ("rt_get_version", "rt_material_defaults", "rt_settings_defaults", "rt_create", "rt_destroy",
                     "rt_last_error", "rt_scene_create", "rt_scene_destroy", "rt_scene_get_info",
                     "rt_scene_set_textures", "rt_render",
-                    "rt_trace_primary", "rt_trace_rays", "rt_photon_map", "rt_render_batch", "rt_context_tune",
+                    "rt_trace_primary", "rt_trace_rays", "rt_photon_map", "rt_photon_gather", "rt_render_batch",
+                    "rt_context_tune",
                     "rt_adjoint", "rt_occluded", "rt_occluded_noalloc", "rt_exr_encode", "rt_pfm_encode",
                     "rt_png_encode",
                     "rt_denoise",
@@ -207,6 +212,18 @@ class Scene:
                                        C.byref(c), _f(pos), _f(inc), _f(pw), _i(key), _i(idx), _i(start), _f(grid)))
         out.update(pos=pos, inc=inc, power=pw, key=key, idx=idx, start=start, grid=grid)
         return out
+
+    def photon_gather(self, settings, points):
+        """k-nearest photon search of the render on the device
+        (rt_photon_gather): count (n,), radius (n,), idx (n, 40) photon-order
+        indices in heap order, -1 past the count."""
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        n = len(pts)
+        cnt, rad = np.zeros(n, np.int32), np.zeros(n, np.float32)
+        idx = np.full((n, GATHER_K), -1, np.int32)
+        self.ctx.check(load().rt_photon_gather(self.ctx.h, self.h, C.byref(settings), _f(pts), n, _i(cnt), _f(rad),
+                                               _i(idx)))
+        return cnt, rad, idx
 
     def close(self):
         if getattr(self, "h", None):

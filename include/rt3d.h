@@ -541,6 +541,19 @@ int rt_photon_map(rt_ctx* ctx, const rt_scene* scene, const rt_settings* setting
                   int64_t cap_buckets, int64_t* n_photons, int64_t* n_emitted, int64_t* n_buckets, float* pos,
                   float* inc, float* power, int32_t* key, int32_t* idx, int32_t* start, float* grid);
 
+/* Parity hook: builds the photon map rt_photon_map builds for `settings`
+ * and runs the render's k-nearest search on the device at each of n_points
+ * points (3 floats each), with k = 40 (Renderer.CausticGatherK) and the
+ * gather radius as the radius cap.  count[i] = photons found, radius[i] =
+ * distance of the farthest (0 when none), photon_idx[40 * i + j] = photon j
+ * of point i in the order the search's max-heap holds them (the order the
+ * render sums them in), as a photon-order index (rt_photon_map's order);
+ * -1 past count[i].
+ * Replaces: PhotonMap.GatherKNearest's result list and radius
+ * (PhotonMap.cs:209-240, heap rules :296-317). */
+int rt_photon_gather(rt_ctx* ctx, const rt_scene* scene, const rt_settings* settings, const float* points,
+                     int64_t n_points, int32_t* count, float* radius, int32_t* photon_idx);
+
 /* ── Film files.  Each encoder writes a whole file image into `out` (the
  * caller writes it to disk); with out = NULL it only stores in *size an upper
  * bound of the bytes needed, and a capacity below that bound is

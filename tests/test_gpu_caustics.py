@@ -7,6 +7,9 @@
     streams; transcendental last-ulp differences can flip a rare Fresnel
     coin, so counts and powers agree to tolerance, and a coin-free mirror
     scene agrees photon by photon);
+  * the device k-nearest search (k = 40) returns the oracle's photons in
+    the oracle's max-heap order, and the same radius, bit for bit
+    (PhotonMap.GatherKNearest, PhotonMap.cs:209-317);
   * rendering with caustics matches the oracle's recursive integrator with
     the same photon budget.
 """
@@ -29,6 +32,59 @@ def _settings(W, H, spp, depth, photons, radius):
 
 def _small_caustics(W=64, H=48):
     return scenes.caustics_scene(W, H, water_cells=40)
+
+
+def knn_points(pos, cell, radius, n, seed):
+    """Query points for the k-nearest search: near photons (full heaps,
+    ring stops), photon positions themselves (zero distances), cell
+    corners (the floor at cell boundaries), and uniform points over the
+    photons' bounds and beyond them (sparse, empty and radius-capped)."""
+    rng = np.random.default_rng(seed)
+    pos = np.asarray(pos, np.float32)
+    q = n // 5
+    near = pos[rng.integers(0, len(pos), q)] + rng.normal(0, radius * 0.5, (q, 3)).astype(np.float32)
+    at = pos[rng.integers(0, len(pos), q)]
+    org = pos.min(0)
+    corner = org + np.floor((pos[rng.integers(0, len(pos), q)] - org) / cell).astype(np.float32) * np.float32(cell)
+    lo, hi = pos.min(0), pos.max(0)
+    span = np.maximum(hi - lo, radius)
+    uni = (lo - 0.1 * span + rng.random((n - 3 * q, 3)) * 1.2 * span).astype(np.float32)
+    return np.ascontiguousarray(np.concatenate([near, at, corner, uni]).astype(np.float32))
+
+
+def check_knn(sc, st, m, oracle, radius, n, seed):
+    h = oracle.PhotonMap.from_positions(m["pos"], float(m["grid"][3]))
+    pts = knn_points(m["pos"], float(m["grid"][3]), radius, n, seed)
+    cnt, rad, idx = sc.photon_gather(st, pts)
+    ocnt, orad, oidx = h.gather(pts, 40, radius)
+    full = (cnt == 40).mean()
+    print(f"{len(pts)} queries: full heaps {full:.3f}, empty {(cnt == 0).mean():.3f}; "
+          f"count mismatches {(cnt != ocnt).sum()}, radius mismatches {(rad.view(np.int32) != orad.view(np.int32)).sum()}, "
+          f"order mismatches {(idx != oidx).any(1).sum()}", flush=True)
+    assert np.array_equal(cnt, ocnt)
+    assert np.array_equal(rad.view(np.int32), orad.view(np.int32))
+    assert np.array_equal(idx, oidx)
+    return full
+
+
+@pytest.mark.parametrize("photons,radius", [(3000, 0.05), (250000, 0.05), (250000, 0.2)])
+def test_device_knn_is_the_oracle_heap(native, oracle, gpu_ctx, photons, radius):
+    desc = _small_caustics()
+    sc = native.Scene(gpu_ctx, desc)
+    st = _settings(64, 48, 1, 8, photons, radius)
+    m = sc.photon_map(st)
+    assert m["n"] > 0
+    full = check_knn(sc, st, m, oracle, radius, 50000, photons)
+    if radius == 0.2:
+        # about half the searches fill the heap and stop on the k-th distance
+        assert full > 0.3
+
+
+def test_device_knn_without_photons(native, gpu_ctx):
+    desc = _small_caustics()
+    sc = native.Scene(gpu_ctx, desc)
+    cnt, rad, idx = sc.photon_gather(_settings(64, 48, 1, 8, 0, 0.05), np.zeros((7, 3), np.float32))
+    assert (cnt == 0).all() and (rad == 0).all() and (idx == -1).all()
 
 
 @pytest.mark.parametrize("photons", [1, 3000, 250000])

@@ -115,6 +115,15 @@ def test_20M_photon_grid_is_the_host_sort(oracle, caustics):
     assert np.array_equal(m["start"], h.start)
 
 
+def test_20M_photon_knn_is_the_oracle_heap(oracle, caustics):
+    from test_gpu_caustics import check_knn
+    desc, sc, _ = caustics
+    st = _caustic_settings(1920, 1080, 1, 12, 20_000_000, 0.01)
+    m = sc.photon_map(st)
+    full = check_knn(sc, st, m, oracle, 0.01, 200_000, 7)
+    assert full > 0.05
+
+
 def test_20M_photon_set_matches_oracle(caustics):
     desc, sc, osc = caustics
     budget, radius = 20_000_000, 0.01
