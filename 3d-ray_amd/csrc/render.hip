@@ -86,6 +86,8 @@ struct RParams {
     int aovs;
     int maxs;                   // shadow slots per path
     int base_sample;            // first sample pass of the window
+    int soff;                   // rt_settings.sample_offset: global index of pass 0
+    int spp_fp;                 // spp of the whole render (footprint shrink)
     int footprints;             // camera-ray differentials feed image-texture footprints
     int diff_rays;              // camera rays as GetRayWithDifferentials builds them (filtering on)
 };
@@ -925,6 +927,7 @@ __device__ __forceinline__ void resolve_sample(const PathBufs& P, int sid, int& 
 __device__ __forceinline__ void init_path(const DevScene& S, const RParams& R, PathBufs& P, int p, int sid, int job,
                                           int pix, int pass) {
     int i = pix % R.W, j = pix / R.W;
+    pass += R.soff;   // the sample's global index (Sobol index, half parity)
     Rng rng{(uint32_t)pass, pixel_seed(i, j, R.seed_xor), 0u};
     f3 o, d;
     camera_ray(S, R, i, j, rng, o, d);
@@ -1121,7 +1124,7 @@ __global__ void __launch_bounds__(kBlock) k_fp(JOBS jobs, PathBufs P) {
             const float ju = rng.next(), jv = rng.next();
             const float su = ((float)i + ju) / (float)R.W;
             const float tv = ((float)(R.H - j - 1) + jv) / (float)R.H;
-            const int sq = (int)ceilf(sqrtf((float)R.spp));
+            const int sq = (int)ceilf(sqrtf((float)R.spp_fp));
             const float shrink = 1.0f / (float)sq;
             const float dsdx = shrink / (float)R.W, dtdy = shrink / (float)R.H;
             f3 org = S.cam_o;
@@ -2245,7 +2248,7 @@ __global__ void __launch_bounds__(kBlock) k_accum(RParams R, PathBufs P, FilmBuf
     f3 sa = mk(F.sa[pix], F.sa[np + pix], F.sa[2 * np + pix]);
     f3 sb = mk(F.sb[pix], F.sb[np + pix], F.sb[2 * np + pix]);
     for (int sl = 0; sl < passes; sl++) {
-        int s = pass0 + sl;
+        int s = R.soff + pass0 + sl;
         int64_t k = (int64_t)off + (int64_t)sl * np + pix;
         float4 l = P.res_l[k];
         f3 L = f3of(l);
@@ -2274,12 +2277,14 @@ __global__ void k_resolve(RParams R, FilmBufs F, float exposure) {
     int pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= R.npix) return;
     const int np = R.npix;
-    const int nA = (R.spp + 1) / 2, nB = R.spp / 2;
+    // even global indices form half A (RenderBuffers.cs:43-46)
+    const int nA = (R.soff & 1) ? R.spp / 2 : (R.spp + 1) / 2, nB = R.spp - nA;
     for (int c = 0; c < 3; c++) {
         int k = c * np + pix;
         float lin = F.sum[k] / (float)R.spp;
         F.out_beauty[k] = lin;
-        float mA = F.sa[k] / (float)nA;
+        // a half without samples repeats the other (one sample per pixel)
+        float mA = nA > 0 ? F.sa[k] / (float)nA : F.sb[k] / (float)nB;
         F.out_a[k] = mA;
         F.out_b[k] = nB > 0 ? F.sb[k] / (float)nB : mA;
         float x = fmaxf(lin * exposure, 0.0f);
@@ -2287,7 +2292,8 @@ __global__ void k_resolve(RParams R, FilmBufs F, float exposure) {
         float b = x * (2.43f * x + 0.59f) + 0.14f;
         F.out_disp[k] = powf(clampf(a / b, 0.0f, 1.0f), 1.0f / 2.2f);
         if (R.aovs) {
-            float aa = F.alba[k] / (float)nA, na = F.nrma[k] / (float)nA;
+            float aa = nA > 0 ? F.alba[k] / (float)nA : F.albb[k] / (float)nB;
+            float na = nA > 0 ? F.nrma[k] / (float)nA : F.nrmb[k] / (float)nB;
             F.out_alba[k] = aa;
             F.out_albb[k] = nB > 0 ? F.albb[k] / (float)nB : aa;
             F.out_nrma[k] = na;
@@ -2297,6 +2303,7 @@ __global__ void k_resolve(RParams R, FilmBufs F, float exposure) {
     if (R.aovs) {
         float dA = F.dca[pix] > 0 ? F.depa[pix] / (float)F.dca[pix] : -1.0f;
         float dB = nB > 0 ? (F.dcb[pix] > 0 ? F.depb[pix] / (float)F.dcb[pix] : -1.0f) : dA;
+        if (nA == 0) dA = dB;
         F.out_depa[pix] = dA;
         F.out_depb[pix] = dB;
     }
@@ -2967,6 +2974,10 @@ static int check_settings(rt_ctx* ctx, const rt_settings* s) {
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bad light_sampling");
     if (s->texture_filtering < 0 || s->texture_filtering > 2)
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bad texture_filtering");
+    if (s->sample_offset < 0 || (int64_t)s->sample_offset + s->samples_per_pixel > INT32_MAX)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "sample_offset must be >= 0 with sample_offset + spp < 2^31");
+    if (s->sample_total < 0 || (s->sample_total > 0 && s->sample_total < s->sample_offset + s->samples_per_pixel))
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "sample_total must be 0 or cover sample_offset + spp");
     return RT_OK;
 }
 
@@ -3615,6 +3626,8 @@ static RParams make_params(const rt_settings* s, const HostScene& h, int maxs) {
     R.rr_min_b = s->rr_min_bounces >= 0 ? s->rr_min_bounces : h.rr_min_bounces;
     R.rr_min_s = s->rr_min_survival >= 0.0f ? s->rr_min_survival : h.rr_min_survival;
     R.seed_xor = s->seed_xor;
+    R.soff = s->sample_offset;
+    R.spp_fp = s->sample_total > 0 ? s->sample_total : s->samples_per_pixel;
     R.aovs = s->capture_aovs ? 1 : 0;
     R.maxs = maxs;
     R.diff_rays = s->texture_filtering != 2 ? 1 : 0;

@@ -101,7 +101,8 @@ class rt_settings(C.Structure):
                 ("rr_min_survival", C.c_float), ("seed_xor", C.c_uint32), ("caustics", C.c_int32),
                 ("caustic_photons", C.c_int32), ("caustic_gather_radius", C.c_float),
                 ("paths_in_flight", C.c_int32), ("count_traversal", C.c_int32),
-                ("texture_filtering", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("texture_filtering", C.c_int32), ("sample_offset", C.c_int32),
+                ("sample_total", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 FP = C.POINTER(C.c_float)

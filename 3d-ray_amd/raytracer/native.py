@@ -237,10 +237,11 @@ class Scene:
 class RenderBuffers:
     """Linear-HDR film (RenderBuffers.cs:52-169), planes shaped (C, H, W)."""
 
-    def __init__(self, planes, spp):
+    def __init__(self, planes, spp, first=0):
         self.__dict__.update(planes)
-        self.samples_a = (spp + 1) // 2
-        self.samples_b = spp // 2
+        # even global sample indices in [first, first + spp) form half A
+        self.samples_a = spp // 2 if first % 2 else (spp + 1) // 2
+        self.samples_b = spp - self.samples_a
 
 
 class RenderResult:
@@ -260,7 +261,7 @@ class Renderer:
     def __init__(self, scene, samples_per_pixel, max_depth, max_sample_radiance=10.0, mis_heuristic="balance",
                  light_sampling_strategy="all", indirect_clamp_factor=0.25, exposure_ev=0.0, rr_min_bounces=-1,
                  rr_min_survival=-1.0, seed_xor=0, paths_in_flight=0, enable_caustics=False, caustic_photons=0,
-                 caustic_gather_radius=0.0):
+                 caustic_gather_radius=0.0, sample_offset=0, sample_total=0):
         self.scene = scene
         s = abi.rt_settings()
         load().rt_settings_defaults(C.byref(s))
@@ -278,6 +279,8 @@ class Renderer:
         s.caustics = 1 if enable_caustics else 0
         s.caustic_photons = caustic_photons
         s.caustic_gather_radius = caustic_gather_radius
+        s.sample_offset = sample_offset     # resume / chunking: samples [offset, offset + spp)
+        s.sample_total = sample_total
         self.settings = s
 
     def _film(self, width, height, capture_aovs):
@@ -301,7 +304,8 @@ class Renderer:
                   for k, a in planes.items()}
         shaped["sample_count"] = count.reshape(height, width)
         pixels = np.transpose(shaped["display"], (1, 2, 0))
-        return RenderResult(pixels, RenderBuffers(shaped, self.settings.samples_per_pixel), stats)
+        return RenderResult(pixels, RenderBuffers(shaped, self.settings.samples_per_pixel, self.settings.sample_offset),
+                            stats)
 
     def render(self, width, height, capture_aovs=False):
         s = self.settings

@@ -343,7 +343,18 @@ typedef struct rt_settings {
     int32_t texture_filtering;      /* --texture-filtering: 0 auto (= on), 1 on, 2 off:
                                        camera-ray differentials for image textures
                                        (Renderer.cs:344-348) */
-    int32_t reserved[4];
+    int32_t sample_offset;          /* first sample index: the call renders samples
+                                       [sample_offset, sample_offset + spp) of every
+                                       pixel (Sobol index, A/B half by the index's
+                                       parity); 0 = the reference's render.  Chunks
+                                       [0, n1) and [n1, n1 + n2) merge into the
+                                       [0, n1 + n2) render by sample-weighted means
+                                       (resume / chunking; no reference counterpart) */
+    int32_t sample_total;           /* the whole render's spp when this call is one
+                                       chunk of it (0 = samples_per_pixel): sets the
+                                       ray-differential pixel footprint, which shrinks
+                                       by ceil(sqrt(spp)) of the render (Renderer.cs:556) */
+    int32_t reserved[2];
 } rt_settings;          /* 96 bytes */
 
 /* ── Film (RenderBuffers.cs:52-169; FrameBuffer planar layout
@@ -494,8 +505,9 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
  * pixel, the camera ray of Renderer.cs:630-653 (+ the camera-visibility skip
  * of TraceRay, Renderer.cs:1609-1644) is traced and its closest hit written as
  * prim_id (-1 = miss), t, u, v.  Output index = s_local*W*H + y*W + x.
- * NULL outputs are skipped.  No reference counterpart (HitRecord carries no
- * primitive index); this is the bit-exact primary-hit gate. */
+ * NULL outputs are skipped; settings.sample_offset is not applied (the
+ * sample indices are sample_begin's).  No reference counterpart (HitRecord
+ * carries no primitive index); this is the bit-exact primary-hit gate. */
 int rt_trace_primary(rt_ctx* ctx, const rt_scene* scene, const rt_settings* settings,
                      int32_t sample_begin, int32_t sample_count,
                      int32_t* prim_id, float* t, float* u, float* v);

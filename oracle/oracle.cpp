@@ -3950,7 +3950,10 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
     int total = tx * ty;
     std::atomic<int> next(0);
     std::atomic<int64_t> seg_closest(0), seg_shadow(0);
-    const int nA = (spp + 1) / 2, nB = spp / 2;
+    // chunk of a longer render: global sample indices [soff, soff + spp), the
+    // footprint of the whole render's spp; even global indices form half A
+    const int soff = st->sample_offset, spp_fp = st->sample_total > 0 ? st->sample_total : spp;
+    const int nA = (soff & 1) ? spp / 2 : (spp + 1) / 2, nB = spp - nA;
     auto worker = [&]() {
         AovProbe probe;
         Renderer RR(*s, *st);
@@ -3966,7 +3969,7 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
             int x0 = (tile % tx) * T, y0 = (tile / tx) * T;
             int x1 = std::min(x0 + T, W), y1 = std::min(y0 + T, H);
             // pixel deltas of the differential rays, shrunk by ceil(sqrt(spp)) (Renderer.cs:556,616-624)
-            const float pd_shrink = 1.0f / (float)(int)std::ceil(std::sqrt((float)spp));
+            const float pd_shrink = 1.0f / (float)(int)std::ceil(std::sqrt((float)spp_fp));
             const float pd_s = pd_shrink / (float)W, pd_t = pd_shrink / (float)H;
             for (int j = y0; j < y1; j++) {
                 if (j % row_step != 0) continue;
@@ -3977,7 +3980,7 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
                     uint32_t ps = pixel_seed(i, j, st->seed_xor);
                     for (int smp = 0; smp < spp; smp++) {
                         SamplerCtx& c = t_smp;
-                        c.active = true; c.seed = ps; c.index = (uint32_t)smp; c.dim = 0; c.fixed = nullptr;
+                        c.active = true; c.seed = ps; c.index = (uint32_t)(soff + smp); c.dim = 0; c.fixed = nullptr;
                         float ju = random_float();
                         float jv = random_float();
                         float uu = ((float)i + ju) / (float)W;
@@ -3989,7 +3992,7 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
                         c.active = false;
                         L = RR.clamp_radiance(L);
                         cum += L;
-                        bool isA = (smp & 1) == 0;
+                        bool isA = ((soff + smp) & 1) == 0;
                         if (isA) sa += L; else sb += L;
                         if (RR.capture_aovs) {
                             if (probe.pending) probe.commit();
@@ -4002,17 +4005,19 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
                     auto put3 = [&](float* p, V3 c) { if (p) { p[px] = c.x; p[np + px] = c.y; p[2 * np + px] = c.z; } };
                     if (film) {
                         put3(film->beauty, lin);
-                        V3 mA = sa / (float)nA;
+                        // a half without samples repeats the other
+                        V3 mA = nA > 0 ? sa / (float)nA : sb / (float)nB;
                         V3 mB = nB > 0 ? sb / (float)nB : mA;
                         put3(film->beauty_a, mA);
                         put3(film->beauty_b, mB);
                         if (film->sample_count) film->sample_count[px] = spp;
                         put3(film->display, Renderer::aces(lin * RR.exposure));
                         if (RR.capture_aovs) {
-                            V3 alA = aa / (float)nA, alB = nB > 0 ? ab / (float)nB : alA;
-                            V3 nrA = na / (float)nA, nrB = nB > 0 ? nb / (float)nB : nrA;
+                            V3 alA = nA > 0 ? aa / (float)nA : ab / (float)nB, alB = nB > 0 ? ab / (float)nB : alA;
+                            V3 nrA = nA > 0 ? na / (float)nA : nb / (float)nB, nrB = nB > 0 ? nb / (float)nB : nrA;
                             float dA = dca > 0 ? da / (float)dca : -1.0f;
                             float dB = nB > 0 ? (dcb > 0 ? db / (float)dcb : -1.0f) : dA;
+                            if (nA == 0) dA = dB;
                             put3(film->albedo_a, alA);
                             put3(film->albedo_b, alB);
                             put3(film->normal_a, nrA);
