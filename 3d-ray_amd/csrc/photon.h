@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(kBlock) k_photon_trace(DevScene S, const Photo
         else hi = mid - 1;
     }
     const PhotonSrc ps = src[lo];
-    Rng rng{(uint32_t)(g - ps.offset), ((uint32_t)lo * 0x9E3779B1u) ^ 0xC2B2AE35u, 0u};
+    Rng rng{(uint32_t)(g - ps.offset), ((uint32_t)lo * 0x9E3779B1u) ^ 0xC2B2AE35u, 0u, false};
     const GLight L = S.lights[ps.light];
     f3 o, d, power;
     bool dep = false;

@@ -88,6 +88,9 @@ struct RParams {
     int base_sample;            // first sample pass of the window
     int soff;                   // rt_settings.sample_offset: global index of pass 0
     int spp_fp;                 // spp of the whole render (footprint shrink)
+    int prng;                   // PRNG sampler: stratified camera jitter, hashed draws
+    int sq;                     // strata per axis, ceil(sqrt(spp)) (PRNG sampler)
+    float inv_sq;
     int footprints;             // camera-ray differentials feed image-texture footprints
     int diff_rays;              // camera rays as GetRayWithDifferentials builds them (filtering on)
 };
@@ -880,8 +883,16 @@ __device__ __forceinline__ void sample_of(const RParams& R, int sid, int& i, int
 
 // Camera ray (Renderer.cs:630-653, Camera.cs:127-153)
 __device__ __forceinline__ void camera_ray(const DevScene& S, const RParams& R, int i, int j, Rng& rng, f3& o, f3& d) {
-    float ju = rng.next();
-    float jv = rng.next();
+    float ju, jv;
+    if (R.prng) {
+        // stratum (sx, sy) = (s mod sq, s div sq) of the sq x sq grid (Renderer.cs:677-682)
+        const int s = (int)rng.index;
+        ju = ((float)(s % R.sq) + rng.next()) * R.inv_sq;
+        jv = ((float)(s / R.sq) + rng.next()) * R.inv_sq;
+    } else {
+        ju = rng.next();
+        jv = rng.next();
+    }
     float u = ((float)i + ju) / (float)R.W;
     float v = ((float)(R.H - j - 1) + jv) / (float)R.H;
     f3 target = (S.cam_llc + u * S.cam_h) + v * S.cam_v;
@@ -928,7 +939,7 @@ __device__ __forceinline__ void init_path(const DevScene& S, const RParams& R, P
                                           int pix, int pass) {
     int i = pix % R.W, j = pix / R.W;
     pass += R.soff;   // the sample's global index (Sobol index, half parity)
-    Rng rng{(uint32_t)pass, pixel_seed(i, j, R.seed_xor), 0u};
+    Rng rng{(uint32_t)pass, pixel_seed(i, j, R.seed_xor), 0u, R.prng != 0};
     f3 o, d;
     camera_ray(S, R, i, j, rng, o, d);
     P.o[p] = make_float4(o.x, o.y, o.z, 0.0f);
@@ -1120,7 +1131,7 @@ __global__ void __launch_bounds__(kBlock) k_fp(JOBS jobs, PathBufs P) {
             const f3 dpdv = dot(dv, dv) > 0.0f ? dv : s.bitan;
             // GetRayWithDifferentials: the primary's jitter (and lens) sample
             const int pix = P.pix[p], i = pix % R.W, j = pix / R.W;
-            Rng rng{(uint32_t)P.pass[p], pixel_seed(i, j, R.seed_xor), 0u};
+            Rng rng{(uint32_t)P.pass[p], pixel_seed(i, j, R.seed_xor), 0u, R.prng != 0};
             const float ju = rng.next(), jv = rng.next();
             const float su = ((float)i + ju) / (float)R.W;
             const float tv = ((float)(R.H - j - 1) + jv) / (float)R.H;
@@ -1130,7 +1141,7 @@ __global__ void __launch_bounds__(kBlock) k_fp(JOBS jobs, PathBufs P) {
             f3 org = S.cam_o;
             if (S.lens_radius != 0.0f) {
                 f3 lo, ld;
-                Rng r2{(uint32_t)P.pass[p], pixel_seed(i, j, R.seed_xor), 0u};
+                Rng r2{(uint32_t)P.pass[p], pixel_seed(i, j, R.seed_xor), 0u, R.prng != 0};
                 camera_ray(S, R, i, j, r2, lo, ld);
                 org = lo;
             }
@@ -1225,7 +1236,8 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         }
         int depth = __float_as_int(thr4.w);
         const int pix = P.pix[p];
-        Rng rng{(uint32_t)P.pass[p], pixel_seed(pix % R.W, pix / R.W, R.seed_xor), __float_as_uint(wt4.w)};
+        Rng rng{(uint32_t)P.pass[p], pixel_seed(pix % R.W, pix / R.W, R.seed_xor), __float_as_uint(wt4.w),
+                R.prng != 0};
         f3 o = f3of(o4), d = f3of(d4);
         float prev_pdf = o4.w;
         bool prev_delta = (flags & kPfPrevDelta) != 0;
@@ -2318,7 +2330,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_primary(DevScene S, RParams R,
     if (p >= n) return;
     int i, j, s;
     sample_of(R, p, i, j, s);
-    Rng rng{(uint32_t)s, pixel_seed(i, j, R.seed_xor), 0u};
+    Rng rng{(uint32_t)s, pixel_seed(i, j, R.seed_xor), 0u, false};
     f3 o, d;
     camera_ray(S, R, i, j, rng, o, d);
     HitOut h;
@@ -2962,12 +2974,23 @@ static int photon_pass(rt_ctx* ctx, const rt_scene* sc, const DevScene& S, int b
     return RT_OK;
 }
 
+// Camera samples per pixel the render takes: spp (Sobol) or the
+// ceil(sqrt(spp))^2 strata of the PRNG sampler (Renderer.cs:556-558).
+static int render_spp(const rt_settings* s) {
+    if (s->sampler != RT_SAMPLER_PRNG) return s->samples_per_pixel;
+    const int q = (int)ceilf(sqrtf((float)s->samples_per_pixel));
+    return q * q;
+}
+
 static int check_settings(rt_ctx* ctx, const rt_settings* s) {
     if (!s) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "settings is null");
     if (s->width <= 0 || s->height <= 0 || s->samples_per_pixel <= 0 || s->max_depth < 0)
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "width, height and samples_per_pixel must be positive, max_depth >= 0");
     if ((int64_t)s->width * s->height > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
-    if (s->sampler != RT_SAMPLER_SOBOL) return fail(ctx, RT_ERR_UNSUPPORTED, "only the Sobol sampler is implemented");
+    if (s->sampler != RT_SAMPLER_SOBOL && s->sampler != RT_SAMPLER_PRNG)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bad sampler");
+    if (s->sampler == RT_SAMPLER_PRNG && (s->sample_offset != 0 || s->sample_total != 0))
+        return fail(ctx, RT_ERR_UNSUPPORTED, "the PRNG sampler renders whole strata grids (no sample_offset)");
     if (s->caustics && (s->caustic_photons < 0 || s->caustic_gather_radius < 0.0f))
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "negative photon budget or gather radius");
     if (s->light_sampling < RT_LIGHTS_ALL || s->light_sampling > RT_LIGHTS_UNIFORM)
@@ -3215,6 +3238,7 @@ int rt_trace_primary(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, int3
     int rc = check_settings(ctx, s);
     if (rc) return rc;
     if (sample_begin < 0 || sample_count <= 0) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bad sample range");
+    if (s->sampler != RT_SAMPLER_SOBOL) return fail(ctx, RT_ERR_UNSUPPORTED, "primary hits are traced with the Sobol sampler");
     hipSetDevice(ctx->device);
     int64_t npix = (int64_t)s->width * s->height;
     int64_t n = npix * sample_count;
@@ -3616,7 +3640,10 @@ static RParams make_params(const rt_settings* s, const HostScene& h, int maxs) {
     RParams R{};
     R.W = s->width;
     R.H = s->height;
-    R.spp = s->samples_per_pixel;
+    R.spp = render_spp(s);
+    R.prng = s->sampler == RT_SAMPLER_PRNG ? 1 : 0;
+    R.sq = (int)ceilf(sqrtf((float)s->samples_per_pixel));
+    R.inv_sq = 1.0f / (float)R.sq;
     R.max_depth = s->max_depth;
     R.npix = s->width * s->height;
     R.max_rad = s->max_sample_radiance;
@@ -3627,7 +3654,7 @@ static RParams make_params(const rt_settings* s, const HostScene& h, int maxs) {
     R.rr_min_s = s->rr_min_survival >= 0.0f ? s->rr_min_survival : h.rr_min_survival;
     R.seed_xor = s->seed_xor;
     R.soff = s->sample_offset;
-    R.spp_fp = s->sample_total > 0 ? s->sample_total : s->samples_per_pixel;
+    R.spp_fp = s->sample_total > 0 ? s->sample_total : R.spp;
     R.aovs = s->capture_aovs ? 1 : 0;
     R.maxs = maxs;
     R.diff_rays = s->texture_filtering != 2 ? 1 : 0;
@@ -3655,7 +3682,7 @@ static int finish_film(rt_ctx* ctx, const RParams& R, const FilmBufs& F, const r
         if (c.dst) HIP_TRY(hipMemcpyAsync(c.dst, c.src, c.n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (film->sample_count)
-        for (int64_t k = 0; k < npix; k++) film->sample_count[k] = s->samples_per_pixel;
+        for (int64_t k = 0; k < npix; k++) film->sample_count[k] = R.spp;
     return RT_OK;
 }
 
@@ -3686,7 +3713,7 @@ int rt_render(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const rt_fi
     if (rc) return rc;
     hipSetDevice(ctx->device);
     const HostScene& h = sc->hs;
-    const int spp = s->samples_per_pixel;
+    const int spp = render_spp(s);
     const int npix = s->width * s->height;
     // Path pool of N slots; samples are taken in windows of K whole passes
     // over the image whose per-sample results are summed into the film in
@@ -3787,6 +3814,7 @@ int rt_adjoint(rt_ctx* ctx, const rt_scene* sc, const rt_settings* s, const floa
     if (!ctx || !sc || !d_beauty || !d_texels) return RT_ERR_INVALID_ARGUMENT;
     int rc = check_settings(ctx, s);
     if (rc) return rc;
+    if (s->sampler != RT_SAMPLER_SOBOL) return fail(ctx, RT_ERR_UNSUPPORTED, "adjoint: the Sobol sampler only");
     hipSetDevice(ctx->device);
     const HostScene& h = sc->hs;
     if (!h.textured) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "scene has no textured material");
@@ -3933,7 +3961,7 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
         for (int j = 0; j < n_jobs; j++) {
             const rt_settings* st = jobs[j].settings;
             const int64_t npix = (int64_t)st->width * st->height;
-            const int spp = st->samples_per_pixel;
+            const int spp = render_spp(st);
             if (npix * spp >= 2 * N0) {
                 flush_packed();
                 Segment sg{j, WindowPlan()};
@@ -4045,7 +4073,7 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
     if (ovf) return fail(ctx, RT_ERR_INTERNAL, "traversal stack overflow");
     int64_t samples = 0;
     for (int j = 0; j < n_jobs; j++) {
-        int64_t sj = (int64_t)jobs[j].settings->width * jobs[j].settings->height * jobs[j].settings->samples_per_pixel;
+        int64_t sj = (int64_t)jobs[j].settings->width * jobs[j].settings->height * render_spp(jobs[j].settings);
         samples += sj;
         if (jobs[j].stats) {
             std::memset(jobs[j].stats, 0, sizeof(rt_render_stats));

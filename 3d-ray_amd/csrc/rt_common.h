@@ -107,6 +107,22 @@ RT_HD float sobol(uint32_t index, uint32_t dim, uint32_t pixel_seed) {
     v = owen(v, sobol_hash(pixel_seed ^ sobol_hash(dim * 0xa511e9b3u + 0x736f6c34u)));
     return (float)(v >> 8) * (1.0f / 16777216.0f);
 }
+// PRNG sampler mode (SamplerKind.Prng, Sampler.cs:9-16).  The reference
+// draws from a thread-local System.Random whose stream depends on thread
+// scheduling, so no value of it is reproducible; here each draw is a
+// counter-based hash of (sample index, dimension, pixel seed): the
+// PCG-RXS-M-XS 32-bit permutation used as a hash (Jarzynski and Olano,
+// "Hash Functions for GPU Rendering", JCGT 9(3), 2020), chained over the
+// three words, then the same 24-bit float conversion as the Sobol path.
+RT_HD uint32_t pcg_hash(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+RT_HD float prng_sample(uint32_t index, uint32_t dim, uint32_t pixel_seed) {
+    const uint32_t h = pcg_hash(pixel_seed ^ pcg_hash(index ^ pcg_hash(dim + 0x9e3779b9u)));
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
 RT_HD uint32_t pixel_seed(int i, int j, uint32_t x) {
     return (((uint32_t)i * 73856093u) ^ ((uint32_t)j * 19349663u)) ^ x;
 }

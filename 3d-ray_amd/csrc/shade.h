@@ -10,9 +10,15 @@
 namespace rt {
 
 // Per-path sampler state: one Sobol dimension per draw.
+// Sampler draw stream of one camera sample: (sample index, pixel seed,
+// next dimension); prng (uniform over a render) selects the PRNG mode.
 struct Rng {
     uint32_t index, seed, dim;
-    __device__ __forceinline__ float next() { return sobol(index, dim++, seed); }
+    bool prng;
+    __device__ __forceinline__ float next() {
+        const uint32_t d = dim++;
+        return prng ? prng_sample(index, d, seed) : sobol(index, d, seed);
+    }
 };
 
 // Surface record reconstructed from the hit entry (HitRecord.cs:9-163).

@@ -255,13 +255,25 @@ _MIS = {"balance": abi.RT_MIS_BALANCE, "power": abi.RT_MIS_POWER}
 _LIGHTS = {"all": abi.RT_LIGHTS_ALL, "power": abi.RT_LIGHTS_POWER, "uniform": abi.RT_LIGHTS_UNIFORM}
 
 
+_SAMPLERS = {"sobol": abi.RT_SAMPLER_SOBOL, "prng": abi.RT_SAMPLER_PRNG}
+
+
+def render_spp(settings):
+    """Camera samples per pixel a render takes: spp with the Sobol sampler,
+    ceil(sqrt(spp))^2 strata with the PRNG sampler (Renderer.cs:556-558)."""
+    if settings.sampler == abi.RT_SAMPLER_PRNG:
+        q = int(np.ceil(np.sqrt(np.float32(settings.samples_per_pixel))))
+        return q * q
+    return settings.samples_per_pixel
+
+
 class Renderer:
     """Mirror of Renderer(...) + Render(width, height, options)."""
 
     def __init__(self, scene, samples_per_pixel, max_depth, max_sample_radiance=10.0, mis_heuristic="balance",
                  light_sampling_strategy="all", indirect_clamp_factor=0.25, exposure_ev=0.0, rr_min_bounces=-1,
                  rr_min_survival=-1.0, seed_xor=0, paths_in_flight=0, enable_caustics=False, caustic_photons=0,
-                 caustic_gather_radius=0.0, sample_offset=0, sample_total=0):
+                 caustic_gather_radius=0.0, sample_offset=0, sample_total=0, sampler="sobol"):
         self.scene = scene
         s = abi.rt_settings()
         load().rt_settings_defaults(C.byref(s))
@@ -279,6 +291,7 @@ class Renderer:
         s.caustics = 1 if enable_caustics else 0
         s.caustic_photons = caustic_photons
         s.caustic_gather_radius = caustic_gather_radius
+        s.sampler = _SAMPLERS[sampler]
         s.sample_offset = sample_offset     # resume / chunking: samples [offset, offset + spp)
         s.sample_total = sample_total
         self.settings = s
@@ -304,8 +317,7 @@ class Renderer:
                   for k, a in planes.items()}
         shaped["sample_count"] = count.reshape(height, width)
         pixels = np.transpose(shaped["display"], (1, 2, 0))
-        return RenderResult(pixels, RenderBuffers(shaped, self.settings.samples_per_pixel, self.settings.sample_offset),
-                            stats)
+        return RenderResult(pixels, RenderBuffers(shaped, render_spp(self.settings), self.settings.sample_offset), stats)
 
     def render(self, width, height, capture_aovs=False):
         s = self.settings

@@ -322,7 +322,9 @@ typedef struct rt_settings {
     int32_t height;
     int32_t samples_per_pixel;
     int32_t max_depth;
-    int32_t sampler;                /* rt_sampler_kind (only SOBOL is implemented) */
+    int32_t sampler;                /* rt_sampler_kind; PRNG: ceil(sqrt(spp))^2 stratified
+                                       samples with hashed draws (Renderer.cs:556-558,
+                                       677-682; not with sample_offset / sample_total) */
     int32_t mis;                    /* rt_mis_kind */
     int32_t light_sampling;         /* rt_light_sampling */
     int32_t capture_aovs;           /* RenderCaptureOptions.CaptureAovs */

@@ -35,6 +35,7 @@ namespace orc {
 // ═════════════════════════════════════════════════════════════════════════
 struct SamplerCtx {
     bool active = false;
+    bool prng = false;          // PRNG sampler mode (prng_sample) instead of Sobol
     uint32_t seed = 0, index = 0, dim = 0;
     const float* fixed = nullptr;
     int nfixed = 0, fixed_pos = 0;
@@ -60,7 +61,7 @@ static float random_float() {
     }
     if (!c.active) return rng_next(c.rng);
     uint32_t d = c.dim++;
-    return sobol_sample(c.index, d, c.seed);
+    return c.prng ? prng_sample(c.index, d, c.seed) : sobol_sample(c.index, d, c.seed);
 }
 
 // MathUtils.RandomUnitVector (MathUtils.cs:93-101)
@@ -3699,6 +3700,10 @@ void orc_sobol_batch(const uint32_t* index, const uint32_t* dim, const uint32_t*
     for (int64_t i = 0; i < n; i++) out[i] = sobol_sample(index[i], dim[i], seed[i]);
 }
 
+void orc_prng_batch(const uint32_t* index, const uint32_t* dim, const uint32_t* seed, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; i++) out[i] = prng_sample(index[i], dim[i], seed[i]);
+}
+
 void* orc_scene_create(const rt_scene_desc* d) { return build_scene(d); }
 
 // ImageTexture probes: the mip chain (levels concatenated, RGB; returns the
@@ -3803,12 +3808,19 @@ int64_t orc_bvh_dump(void* sp, int32_t* out, int64_t cap) {
 
 namespace orc {
 
-static int check_settings(const rt_settings* st) {
+static int check_settings(const rt_settings* st, bool prng_ok = false) {
     if (!st || st->width <= 0 || st->height <= 0 || st->samples_per_pixel <= 0 || st->max_depth < 0) {
         g_err = "invalid settings";
         return RT_ERR_INVALID_ARGUMENT;
     }
-    if (st->sampler != RT_SAMPLER_SOBOL) { g_err = "only the Sobol sampler is implemented"; return RT_ERR_UNSUPPORTED; }
+    if (st->sampler != RT_SAMPLER_SOBOL && !(prng_ok && st->sampler == RT_SAMPLER_PRNG)) {
+        g_err = "sampler not supported here";
+        return RT_ERR_UNSUPPORTED;
+    }
+    if (st->sampler == RT_SAMPLER_PRNG && (st->sample_offset != 0 || st->sample_total != 0)) {
+        g_err = "the PRNG sampler renders whole strata grids (no sample_offset)";
+        return RT_ERR_UNSUPPORTED;
+    }
     if (st->caustics && st->caustic_photons < 0) { g_err = "negative photon budget"; return RT_ERR_INVALID_ARGUMENT; }
     return 0;
 }
@@ -3926,7 +3938,7 @@ int orc_trace_primary(void* sp, const rt_settings* st, int32_t sbegin, int32_t s
 int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nthreads, int32_t row_step,
                rt_render_stats* stats) {
     Scene* s = (Scene*)sp;
-    int rc = check_settings(st);
+    int rc = check_settings(st, true);
     if (rc) return rc;
     if (row_step < 1) row_step = 1;
     // photon pre-pass (Renderer.cs:385-406): gather-radius cap 6 % of the
@@ -3943,7 +3955,11 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
         caus = build_photon_map(*s, st->caustic_photons, gr * 0.5f, std::max(1, (int)nthreads), pm);
     }
     Renderer R(*s, *st);
-    const int W = st->width, H = st->height, spp = st->samples_per_pixel;
+    // PRNG mode: ceil(sqrt(spp))^2 stratified samples (Renderer.cs:556-558)
+    const bool prng = st->sampler == RT_SAMPLER_PRNG;
+    const int sq = (int)std::ceil(std::sqrt((float)st->samples_per_pixel));
+    const int W = st->width, H = st->height, spp = prng ? sq * sq : st->samples_per_pixel;
+    const float inv_sq = 1.0f / (float)sq;
     const int64_t np = (int64_t)W * H;
     const int T = 16;
     int tx = (W + T - 1) / T, ty = (H + T - 1) / T;
@@ -3981,8 +3997,15 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
                     for (int smp = 0; smp < spp; smp++) {
                         SamplerCtx& c = t_smp;
                         c.active = true; c.seed = ps; c.index = (uint32_t)(soff + smp); c.dim = 0; c.fixed = nullptr;
-                        float ju = random_float();
-                        float jv = random_float();
+                        c.prng = prng;
+                        float ju, jv;
+                        if (prng) {   // stratum (sx, sy) of the sq x sq grid (Renderer.cs:677-682)
+                            ju = ((float)(smp % sq) + random_float()) * inv_sq;
+                            jv = ((float)(smp / sq) + random_float()) * inv_sq;
+                        } else {
+                            ju = random_float();
+                            jv = random_float();
+                        }
                         float uu = ((float)i + ju) / (float)W;
                         float vv = ((float)(H - j - 1) + jv) / (float)H;
                         Ray ray = s->camera.get_ray(uu, vv, pd_s, pd_t, st->texture_filtering != 2);
@@ -3990,6 +4013,7 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
                         probe = AovProbe();
                         V3 L = RR.trace_ray(ray, RR.max_depth, 0.0f, true, iors, V3(0.0f), V3(1.0f), kCatCamera);
                         c.active = false;
+                        c.prng = false;
                         L = RR.clamp_radiance(L);
                         cum += L;
                         bool isA = ((soff + smp) & 1) == 0;

@@ -35,6 +35,7 @@ def lib():
         L.orc_sobol.restype = C.c_float
         L.orc_sobol.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_sobol_batch.argtypes = [C.POINTER(C.c_uint32)] * 3 + [C.c_int64, fp]
+        L.orc_prng_batch.argtypes = [C.POINTER(C.c_uint32)] * 3 + [C.c_int64, fp]
         L.orc_scene_create.restype = vp
         L.orc_scene_create.argtypes = [C.POINTER(abi.rt_scene_desc)]
         L.orc_scene_destroy.argtypes = [vp]
@@ -96,6 +97,17 @@ def sobol(index, dim, seed):
     u32 = C.POINTER(C.c_uint32)
     lib().orc_sobol_batch(index.ctypes.data_as(u32), dim.ctypes.data_as(u32), seed.ctypes.data_as(u32),
                           index.size, _f(out))
+    return out
+
+
+def prng(index, dim, seed):
+    """The PRNG sampler mode's draw for (sample index, dimension, pixel seed)."""
+    index, dim, seed = (np.ascontiguousarray(np.broadcast_to(x, np.broadcast(index, dim, seed).shape), np.uint32)
+                        for x in (index, dim, seed))
+    out = np.empty(index.shape, np.float32)
+    u32 = C.POINTER(C.c_uint32)
+    lib().orc_prng_batch(index.ctypes.data_as(u32), dim.ctypes.data_as(u32), seed.ctypes.data_as(u32),
+                         index.size, _f(out))
     return out
 
 

@@ -191,6 +191,20 @@ inline uint32_t sobol_dim1(uint32_t index) {
     }
     return v;
 }
+// PRNG sampler mode: the reference's System.Random streams are not
+// reproducible (thread-local, seeded by thread start order), so this mode
+// draws a counter-based hash of (index, dim, seed) instead: the 32-bit
+// PCG-RXS-M-XS permutation as a hash (Jarzynski and Olano, JCGT 2020),
+// chained over the three words; value = top 24 bits * 2^-24.
+inline uint32_t pcg_hash(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+inline float prng_sample(uint32_t index, uint32_t dim, uint32_t pixel_seed) {
+    uint32_t h = pcg_hash(pixel_seed ^ pcg_hash(index ^ pcg_hash(dim + 0x9e3779b9u)));
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
 inline float sobol_sample(uint32_t index, uint32_t dim, uint32_t pixel_seed) {
     uint32_t v;
     if (dim == 0u) v = reverse_bits(index);

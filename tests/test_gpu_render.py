@@ -164,11 +164,17 @@ def test_invalid_settings_return_errors(native, gpu_ctx):
     with pytest.raises(native.RtError) as e:
         r.render(0, 16)
     assert e.value.code == abi.RT_ERR_INVALID_ARGUMENT
+    r.settings.sampler = 7
+    with pytest.raises(native.RtError) as e:
+        r.render(16, 16)
+    assert e.value.code == abi.RT_ERR_INVALID_ARGUMENT
     r.settings.sampler = abi.RT_SAMPLER_PRNG
+    r.settings.sample_offset = 1
     with pytest.raises(native.RtError) as e:
         r.render(16, 16)
     assert e.value.code == abi.RT_ERR_UNSUPPORTED
     r.settings.sampler = abi.RT_SAMPLER_SOBOL
+    r.settings.sample_offset = 0
     r.settings.caustics = 1
     r.settings.caustic_photons = -5
     with pytest.raises(native.RtError) as e:
