@@ -1190,7 +1190,9 @@ __device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, i
 // weight (segment weight applied), sampler dimension and pending-shadow
 // flags stored; PH = kPhBounce picks them up.  Sampler dimensions are drawn
 // in the same order either way.
-template <class JOBS, int WPE, bool ADJ = false, int PH = kPhAll, bool TF = true>
+// RND: the PRNG sampler mode, fixed per kernel so the Sobol kernels carry no
+// code of the other mode.
+template <class JOBS, int WPE, bool ADJ = false, int PH = kPhAll, bool TF = true, bool RND = false>
 __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, int maxs) {
     static_assert(!ADJ || PH == kPhAll, "the adjoint replay shades whole vertices");
     constexpr bool kFront = PH != kPhBounce;    // medium, emission, NEE, gather
@@ -1236,8 +1238,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         }
         int depth = __float_as_int(thr4.w);
         const int pix = P.pix[p];
-        Rng rng{(uint32_t)P.pass[p], pixel_seed(pix % R.W, pix / R.W, R.seed_xor), __float_as_uint(wt4.w),
-                R.prng != 0};
+        Rng rng{(uint32_t)P.pass[p], pixel_seed(pix % R.W, pix / R.W, R.seed_xor), __float_as_uint(wt4.w), RND};
         f3 o = f3of(o4), d = f3of(d4);
         float prev_pdf = o4.w;
         bool prev_delta = (flags & kPfPrevDelta) != 0;
@@ -1472,6 +1473,33 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             has_brdf = kind == RT_MAT_LAMBERTIAN || kind == RT_MAT_DISNEY;
             nee = kFront && kind != RT_MAT_EMISSIVE;
         }
+        if constexpr (PH == kPhNee) {
+            // what the light draws leave unchanged is stored before them, so
+            // it is not held in registers across them: the AOV state, the
+            // caustic gather request (Renderer.cs:1108-1109) and the bounce
+            // phase's throughput and weight (the sampler dimension follows
+            // the draws)
+            if (R.aovs) {
+                P.av0[p] = make_float4(av0.x, av0.y, av0.z, __int_as_float(aovf));
+                P.av1[p] = av1;
+            }
+            if (!scat && hit) {
+                if (S.caustics && has_brdf) {
+                    gather = true;
+                    P.gq0[p] = make_float4(o.x, o.y, o.z, htuv.x);
+                    P.gq1[p] = make_float4(d.x, d.y, d.z, __int_as_float(he));
+                    P.gq2[p] = make_float4(htuv.y, htuv.z, s.rel_eta, __int_as_float(p));
+                    P.gq3[p] = make_float4(vw.x, vw.y, vw.z, __int_as_float(tgt ? 1 : 0));
+                }
+                // the NEE phase hands scattering vertices to the bounce phase
+                bounce = scatters;
+                if (bounce) {
+                    const f3 ts = thr * mb;
+                    P.thr[p] = make_float4(ts.x, ts.y, ts.z, __int_as_float(depth));
+                    P.wt[p] = make_float4(wt.x, wt.y, wt.z, 0.0f);
+                }
+            }
+        }
         // NEE light samples (one site for medium and surface vertices)
         if (kFront && nee) sample_lights(scat ? ps3 : s.p, scat ? splat(0.0f) : s.n, scat, vw, scat || has_brdf);
         if (kFront && scat) {
@@ -1505,7 +1533,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             }
         } else if (hit) {
             // ── the surface vertex after its light samples
-            if constexpr (kFront) {
+            if constexpr (PH == kPhAll) {
                 // caustic gather request (Renderer.cs:1108-1109)
                 if (S.caustics && has_brdf) {
                     gather = true;
@@ -1514,8 +1542,6 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                     P.gq2[p] = make_float4(htuv.y, htuv.z, s.rel_eta, __int_as_float(p));
                     P.gq3[p] = make_float4(vw.x, vw.y, vw.z, __int_as_float(tgt ? 1 : 0));
                 }
-                // the NEE phase hands scattering vertices to the bounce phase
-                if (PH == kPhNee) bounce = scatters;
             }
             if constexpr (kBack) {
                 DisneyP DP;
@@ -1777,12 +1803,11 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         }
         if (bounce) {
             // the bounce phase continues from here: roulette throughput with
-            // the medium weight, radiance weight with the segment weight, the
-            // next sampler dimension
-            const f3 ts = thr * mb;
-            P.d[p] = make_float4(d.x, d.y, d.z, __int_as_float(flags));
-            P.thr[p] = make_float4(ts.x, ts.y, ts.z, __int_as_float(depth));
-            P.wt[p] = make_float4(wt.x, wt.y, wt.z, __uint_as_float(rng.dim));
+            // the medium weight and radiance weight with the segment weight
+            // (stored before the light draws), the path flags (direction
+            // unchanged) and the next sampler dimension
+            reinterpret_cast<float*>(P.d + p)[3] = __int_as_float(flags);
+            reinterpret_cast<float*>(P.wt + p)[3] = __uint_as_float(rng.dim);
         } else if (cont) {
             flags &= ~(kPfPrevDelta | (3 << kPfCatShift) | kPfTgtInd | (3 << kPfCstShift));
             flags |= (next_delta ? kPfPrevDelta : 0) | (next_cat << kPfCatShift) | (next_tgt ? kPfTgtInd : 0) |
@@ -1795,7 +1820,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         } else {
             P.d[p] = make_float4(d.x, d.y, d.z, __int_as_float(flags));
         }
-        if (R.aovs) {
+        if (PH != kPhNee && R.aovs) {
             P.av0[p] = make_float4(av0.x, av0.y, av0.z, __int_as_float(aovf));
             P.av1[p] = av1;
         }
@@ -2657,8 +2682,9 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
                        int stack_depth, const DevScene* gather_scene, const DevPhotons* M, const WindowPlan& plan,
                        ACCUM accum, LoopTiming& tm) {
     const int64_t N = P.n;
-    bool thin_film, hier;
+    bool thin_film, hier, rnd = false;
     if constexpr (std::is_same<JOBS, OneJob>::value) {
+        rnd = jobs.R.prng != 0;
         thin_film = jobs.S.thin_film != 0;
         hier = jobs.S.xf != nullptr;
     } else {
@@ -2694,21 +2720,41 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
                     hipLaunchKernelGGL((k_shade<JOBS, 1, true>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
             } else {
                 // split shade: NEE phase, then the bounce phase (with the
-                // thin-film Fresnel only when a scene of the launch has one)
-                if (ctx->shade_wpe <= 2)
-                    hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
-                else
-                    hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
-                if (thin_film) {
-                    if (ctx->bounce_wpe <= 2)
-                        hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce, true>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
-                    else
-                        hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhBounce, true>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                // thin-film Fresnel only when a scene of the launch has one);
+                // single renders with the PRNG sampler use their own pair
+                if (rnd) {
+                    if constexpr (std::is_same<JOBS, OneJob>::value) {
+                        hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee, true, true>), grid, dim3(kBlock), 0,
+                                           ctx->stream, jobs, Pw, maxs);
+                        if (thin_film)
+                            hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce, true, true>), grid, dim3(kBlock), 0,
+                                               ctx->stream, jobs, Pw, maxs);
+                        else
+                            hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce, false, true>), grid, dim3(kBlock),
+                                               0, ctx->stream, jobs, Pw, maxs);
+                    }
                 } else {
-                    if (ctx->bounce_wpe <= 2)
-                        hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce, false>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                    if (ctx->shade_wpe <= 2)
+                        hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw,
+                                           maxs);
                     else
-                        hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhBounce, false>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw, maxs);
+                        hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw,
+                                           maxs);
+                    if (thin_film) {
+                        if (ctx->bounce_wpe <= 2)
+                            hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce, true>), grid, dim3(kBlock), 0,
+                                               ctx->stream, jobs, Pw, maxs);
+                        else
+                            hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhBounce, true>), grid, dim3(kBlock), 0,
+                                               ctx->stream, jobs, Pw, maxs);
+                    } else {
+                        if (ctx->bounce_wpe <= 2)
+                            hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce, false>), grid, dim3(kBlock), 0,
+                                               ctx->stream, jobs, Pw, maxs);
+                        else
+                            hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhBounce, false>), grid, dim3(kBlock), 0,
+                                               ctx->stream, jobs, Pw, maxs);
+                    }
                 }
             }
             if (e) HIP_TRY(hipEventRecord(e[2], ctx->stream));
@@ -3925,6 +3971,8 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
         rc = check_settings(ctx, b.settings);
         if (rc) return rc;
         if (b.settings->caustics) return fail(ctx, RT_ERR_UNSUPPORTED, "photon caustics are not supported in a batch");
+        if (b.settings->sampler != RT_SAMPLER_SOBOL)
+            return fail(ctx, RT_ERR_UNSUPPORTED, "a batch renders with the Sobol sampler");
         for (auto& p : strategy)
             if (p.first == b.scene && p.second != b.settings->light_sampling)
                 return fail(ctx, RT_ERR_UNSUPPORTED, "one scene with two light-sampling strategies in a batch");
