@@ -148,6 +148,7 @@ struct ManyJobs {
     const JobDev* J;
     int thin_film;              // some job's scene has a thin-film material (host-side launch choice)
     int hier;                   // some job's scene has an object hierarchy (host-side launch choice)
+    int all_lights;             // every job samples all lights (host-side launch choice)
     __device__ __forceinline__ int job_of(const int32_t* job, int p) const { return job[p]; }
     __device__ __forceinline__ const DevScene& scene(int j) const { return J[j].S; }
     __device__ __forceinline__ const RParams& params(int j) const { return J[j].R; }
@@ -1191,8 +1192,9 @@ __device__ __forceinline__ int32_t fold_shadows(PathBufs& P, const RParams& R, i
 // flags stored; PH = kPhBounce picks them up.  Sampler dimensions are drawn
 // in the same order either way.
 // RND: the PRNG sampler mode, fixed per kernel so the Sobol kernels carry no
-// code of the other mode.
-template <class JOBS, int WPE, bool ADJ = false, int PH = kPhAll, bool TF = true, bool RND = false>
+// code of the other mode.  UL: every job of the launch samples all lights
+// (no light pick), so the NEE light loop runs over a wave-uniform index.
+template <class JOBS, int WPE, bool ADJ = false, int PH = kPhAll, bool TF = true, bool RND = false, bool UL = false>
 __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, int maxs) {
     static_assert(!ADJ || PH == kPhAll, "the adjoint replay shades whole vertices");
     constexpr bool kFront = PH != kPhBounce;    // medium, emission, NEE, gather
@@ -1296,17 +1298,28 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         }
 
         int aovf = 0;
-        float4 av0, av1, av2;
-        if (R.aovs) {
+        float4 av0, av1;
+        bool aov_dirty = false;   // kPhBounce: the AOV state was fetched (and is stored back)
+        // the bounce phase fetches the AOV state where it uses it (after the
+        // BSDF sample), so it is not held in registers through the light
+        // samples and the sampling; the NEE phase already set the depth
+        if (PH != kPhBounce && R.aovs) {
             av0 = P.av0[p];
             av1 = P.av1[p];
-            av2 = P.av2[p];
             aovf = __float_as_int(av0.w);
             if (hit && cat == 0 && (aovf & kAovPending) && !(aovf & kAovHaveDepth)) {
                 av1.w = htuv.x * length(d);
                 aovf |= kAovHaveDepth;
             }
         }
+        auto aov_fetch = [&]() {
+            if constexpr (PH == kPhBounce) {
+                av0 = P.av0[p];
+                av1 = P.av1[p];
+                aovf = __float_as_int(av0.w);
+                aov_dirty = true;
+            }
+        };
         auto aov_commit = [&](f3 alb, f3 nrm) {
             f3 aw = f3of(av0);
             f3 a = aw * alb;
@@ -1355,7 +1368,12 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                 ppick = S.light_pdf[lo];
             }
             int jn = 0;
-            for (int li = first_l; li < last_l; li++) {
+            // UL (Strategy All): every lane walks the whole list, so the
+            // light index is wave-uniform and the light record is read with
+            // scalar loads into scalar registers; otherwise each lane walks
+            // its own range (its picked light)
+            const int l_begin = UL ? 0 : first_l, l_end = UL ? S.num_lights : last_l;
+            for (int li = l_begin; li < l_end; li++) {
                 GLight L = S.lights[li];
                 for (int si = 0; si < L.l0.y; si++, jn++) {
                     LSample ls = light_sample(L, pt, nrm, si, rng);
@@ -1444,7 +1462,16 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             // tentative AOVs (AovProbe, Renderer.cs:777-815): in the phase
             // that holds the vertex's textured material
             const bool aov_here = PH == kPhAll || (PH == kPhNee ? !scatters : true);
-            if (aov_here && R.aovs && (aovf & kAovPending)) {
+            if constexpr (PH == kPhBounce) {
+                if (R.aovs && (__float_as_int(P.av0[p].w) & kAovPending)) {
+                    P.av2[p] = make_float4(s.n.x, s.n.y, s.n.z, 0.0f);
+                    const f3 ta = clamp01(f3of(g.m1));
+                    float* a1 = reinterpret_cast<float*>(P.av1 + p);
+                    a1[0] = ta.x;
+                    a1[1] = ta.y;
+                    a1[2] = ta.z;
+                }
+            } else if (aov_here && R.aovs && (aovf & kAovPending)) {
                 P.av2[p] = make_float4(s.n.x, s.n.y, s.n.z, 0.0f);
                 f3 ta = clamp01(f3of(g.m1));
                 av1 = make_float4(ta.x, ta.y, ta.z, av1.w);
@@ -1683,6 +1710,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                         else att = bs.f * a / bs.pdf;
                     }
                     if (ok) {
+                        if (R.aovs) aov_fetch();
                         if (R.aovs && (aovf & kAovPending)) {
                             if (bs.delta) {
                                 f3 aw = f3of(av0) * clamp01(bs.f);
@@ -1740,6 +1768,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                     }
                 } else if (legacy) {
                     // legacy Scatter path (Renderer.cs:1127-1224)
+                    if (R.aovs) aov_fetch();
                     if (R.aovs && (aovf & kAovPending)) {
                         if (is_delta) {
                             f3 aw = f3of(av0) * clamp01(att);
@@ -1820,7 +1849,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         } else {
             P.d[p] = make_float4(d.x, d.y, d.z, __int_as_float(flags));
         }
-        if (PH != kPhNee && R.aovs) {
+        if (PH == kPhAll ? R.aovs != 0 : (PH == kPhBounce && aov_dirty)) {
             P.av0[p] = make_float4(av0.x, av0.y, av0.z, __int_as_float(aovf));
             P.av1[p] = av1;
         }
@@ -2473,7 +2502,7 @@ struct rt_ctx {
     hipEvent_t ev[8] = {};
     bool priv_stack = true;    // traversal stack in private scratch (false: LDS)
     int shade_wpe = 3;         // minimum waves per SIMD the NEE shading kernel is compiled for
-    int bounce_wpe = 2;        // ... and the bounce shading kernel
+    int bounce_wpe = 3;        // ... and the bounce shading kernel
     void* ph_mem = nullptr;    // photon pass buffers (grow-only)
     size_t ph_bytes = 0;
     hipEvent_t iev[2][kIterBatch][5] = {};   // per-iteration kernel timing, two batches
@@ -2682,14 +2711,16 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
                        int stack_depth, const DevScene* gather_scene, const DevPhotons* M, const WindowPlan& plan,
                        ACCUM accum, LoopTiming& tm) {
     const int64_t N = P.n;
-    bool thin_film, hier, rnd = false;
+    bool thin_film, hier, rnd = false, all_lights;
     if constexpr (std::is_same<JOBS, OneJob>::value) {
         rnd = jobs.R.prng != 0;
+        all_lights = jobs.R.picked == 0;
         thin_film = jobs.S.thin_film != 0;
         hier = jobs.S.xf != nullptr;
     } else {
         thin_film = jobs.thin_film != 0;
         hier = jobs.hier != 0;
+        all_lights = jobs.all_lights != 0;
     }
     const int pool_blocks = (int)((N + kBlock - 1) / kBlock);
     const int shadow_blocks = (int)(((int64_t)N * maxs + kBlock - 1) / kBlock);
@@ -2724,8 +2755,12 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
                 // single renders with the PRNG sampler use their own pair
                 if (rnd) {
                     if constexpr (std::is_same<JOBS, OneJob>::value) {
-                        hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee, true, true>), grid, dim3(kBlock), 0,
-                                           ctx->stream, jobs, Pw, maxs);
+                        if (all_lights)
+                            hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee, true, true, true>), grid, dim3(kBlock), 0,
+                                               ctx->stream, jobs, Pw, maxs);
+                        else
+                            hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee, true, true, false>), grid, dim3(kBlock),
+                                               0, ctx->stream, jobs, Pw, maxs);
                         if (thin_film)
                             hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce, true, true>), grid, dim3(kBlock), 0,
                                                ctx->stream, jobs, Pw, maxs);
@@ -2734,12 +2769,21 @@ static int run_windows(rt_ctx* ctx, const JOBS& jobs, PathBufs P, int maxs, bool
                                                0, ctx->stream, jobs, Pw, maxs);
                     }
                 } else {
-                    if (ctx->shade_wpe <= 2)
-                        hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw,
-                                           maxs);
-                    else
-                        hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream, jobs, Pw,
-                                           maxs);
+                    if (ctx->shade_wpe <= 2) {
+                        if (all_lights)
+                            hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee, true, false, true>), grid, dim3(kBlock),
+                                               0, ctx->stream, jobs, Pw, maxs);
+                        else
+                            hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream,
+                                               jobs, Pw, maxs);
+                    } else {
+                        if (all_lights)
+                            hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhNee, true, false, true>), grid, dim3(kBlock),
+                                               0, ctx->stream, jobs, Pw, maxs);
+                        else
+                            hipLaunchKernelGGL((k_shade<JOBS, 3, false, kPhNee>), grid, dim3(kBlock), 0, ctx->stream,
+                                               jobs, Pw, maxs);
+                    }
                     if (thin_film) {
                         if (ctx->bounce_wpe <= 2)
                             hipLaunchKernelGGL((k_shade<JOBS, 2, false, kPhBounce, true>), grid, dim3(kBlock), 0,
@@ -4088,10 +4132,11 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
     hipEvent_t* ev = ctx->ev;
     HIP_TRY(hipEventRecord(ev[6], ctx->stream));
     LoopTiming tm;
-    ManyJobs many{(const JobDev*)ctx->jobs_mem, 0};
+    ManyJobs many{(const JobDev*)ctx->jobs_mem, 0, 0, 1};
     for (int j = 0; j < n_jobs; j++) {
         many.thin_film |= jd[j].S.thin_film;
         many.hier |= jd[j].S.xf != nullptr ? 1 : 0;
+        if (R[j].picked) many.all_lights = 0;
     }
     auto accum = [&](const PartDev& pt) {
         const int64_t npix = pt.npix;

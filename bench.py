@@ -467,6 +467,9 @@ def main():
     ap.add_argument("--batch-check", type=int, default=16, metavar="SCENES",
                     help="icosphere: also time the first SCENES scenes of the batch workload plain vs batched "
                          "(one step) and report the ratio beside the headline; 0 skips it")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="icosphere: skip the side lines of the caustics frame (one step) and the adjoint/forward "
+                         "cost (two steps), run as child processes and reported beside the headline")
     ap.add_argument("--photons", type=int, default=20_000_000)
     ap.add_argument("--gather-radius", type=float, default=0.01)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
@@ -661,7 +664,39 @@ def main():
         out["batched_vs_plain"] = batch
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    if not caus and not args.no_extras:
+        extras = {}
+        for name, extra in (("caustics", ["--workload", "caustics", "--steps", "1", "--warmup", "0", "--no-count"]),
+                            ("adjoint", ["--workload", "adjoint", "--steps", "2", "--warmup", "1"])):
+            log(f"side line: {name}")
+            line = side_line(extra)
+            if line:
+                keep = ("metric", "value", "unit", "ms_per_step", "higher_is_better", "config", "kernel_ms_per_step",
+                        "forward_device_ms", "adjoint_device_ms")
+                extras[name] = {k: line[k] for k in keep if k in line}
+        if extras:
+            out["side_lines"] = extras
     print(json.dumps(out), flush=True)
+
+
+def side_line(extra):
+    """Runs another workload of this script in a child process (its own GPU
+    context, the same device) and returns its JSON line, or None."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--no-cpu-baseline", "--batch-check", "0", "--no-extras"]
+    try:
+        r = subprocess.run(cmd + extra, capture_output=True, text=True, timeout=600)
+    except Exception as e:  # a side line must not hide the headline
+        log(f"side line failed: {e}")
+        return None
+    for ln in reversed(r.stdout.strip().splitlines()):
+        if ln.startswith("{"):
+            try:
+                return json.loads(ln)
+            except ValueError:
+                break
+    log(f"side line failed (exit {r.returncode}): {r.stderr.strip()[-400:]}")
+    return None
 
 
 if __name__ == "__main__":

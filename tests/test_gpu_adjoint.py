@@ -222,6 +222,6 @@ def test_torch_autograd_wrapper(native, gpu_ctx):
         opt.step()
         with torch.no_grad():
             theta.clamp_(0.0, 1.0)
-        losses.append(float(l))
+        losses.append(float(l.detach()))
     print("losses", ["%.4f" % x for x in losses], flush=True)
     assert losses[-1] < 0.8 * losses[0]
