@@ -1035,7 +1035,7 @@ __global__ void __launch_bounds__(kBlock) k_trace(JOBS jobs, PathBufs P) {
         if (COUNT) flush_counts(P.tcount, tc);
         return;
     }
-    jobs.run(true, jobs.job_of(P.job, p), [&](const DevScene& S, const RParams&) {
+    jobs.run_lane(true, jobs.job_of(P.job, p), [&](const DevScene& S, const RParams&) {
         float4 o4 = P.o[p], d4 = P.d[p];
         f3 o = f3of(o4), d = f3of(d4);
         bool camera = pf_cat(__float_as_int(d4.w)) == 0;
@@ -2137,7 +2137,7 @@ __global__ void __launch_bounds__(kBlock, XF ? 4 : 7) k_shadow(JOBS jobs, PathBu
     int64_t sl = P.q_sh[qi];
     // slot of the record: segment j holds light sample j's records at j*N + slot
     const int slot = grouped ? (int)(sl - (int64_t)seg * P.n) : (int)(sl % P.n);
-    jobs.run(true, jobs.job_of(P.job, slot), [&](const DevScene& S, const RParams&) {
+    jobs.run_lane(true, jobs.job_of(P.job, slot), [&](const DevScene& S, const RParams&) {
         float4 o4 = P.sh_o[sl], d4 = P.sh_d[sl];
         f3 org = f3of(o4), dir = f3of(d4);
         float rem = o4.w;
