@@ -1214,7 +1214,12 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
     bool gather = false;  // a caustic gather request was queued
     // per-lane shadow records are written straight into the path's slots
     jobs.run_lane(active, job, [&](const DevScene& S, const RParams& R) {
-        float4 o4 = P.o[p], d4 = P.d[p], thr4 = P.thr[p], wt4 = P.wt[p];
+        float4 o4 = P.o[p], d4 = P.d[p], wt4 = P.wt[p];
+        // the bounce phase reads the throughput and the medium absorption where
+        // it uses them (after its BSDF sample), not at entry
+        float4 thr4;
+        if constexpr (PH == kPhBounce) thr4 = make_float4(0.0f, 0.0f, 0.0f, reinterpret_cast<const float*>(P.thr + p)[3]);
+        else thr4 = P.thr[p];
         int32_t flags = __float_as_int(d4.w);
         if constexpr (kFront) flags = fold_shadows(P, R, p, flags);
         bool adj_pend = false;   // this vertex leaves a bounce gradient pending
@@ -1253,8 +1258,11 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         const int cst = (flags >> kPfCstShift) & 3;
         int next_cs = cst;
         f3 thr = f3of(thr4), wt = f3of(wt4);
-        float4 ab4 = P.ab[p];
+        float4 ab4;
+        if constexpr (PH == kPhBounce) ab4 = make_float4(0.0f, 0.0f, 0.0f, reinterpret_cast<const float*>(P.ab + p)[3]);
+        else ab4 = P.ab[p];
         f3 sigma_abs = f3of(ab4);
+        bool abs_set = PH != kPhBounce;   // sigma_abs holds the path's value
         int ior_n = __float_as_int(ab4.w);
         int32_t he = P.hit_e[p];
         float4 htuv = P.hit_tuv[p];
@@ -1418,7 +1426,11 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
         float2 tuv = make_float2(0.0f, 0.0f);
         bool textured = false, image = false, scatters = false, has_brdf = false;
         int kind = -1, mflags = 0;
-        const f3 thr_s = thr * mb;
+        // roulette throughput with the medium weight (the bounce phase fetches it)
+        auto thr_at = [&]() -> f3 {
+            if constexpr (PH == kPhBounce) return f3of(P.thr[p]);
+            else return thr * mb;
+        };
         const f3 vw = primary ? splat(1.0f) : wt;   // weight of this vertex's own terms
         f3 view = splat(0.0f);
         bool nee = false;           // the vertex samples the lights
@@ -1721,7 +1733,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                                 av0 = make_float4(aw.x, aw.y, aw.z, av0.w);
                             } else aov_commit(f3of(av1), s.n);
                         }
-                        f3 nt = thr_s * att;
+                        f3 nt = thr_at() * att;
                         bool killed = false;
                         if (used >= R.rr_min_b) {
                             float sp = fmax3(nt);
@@ -1741,7 +1753,10 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                             next_pdf = bs.delta ? 0.0f : bs.pdf;
                             next_delta = bs.delta;
                             next_cat = dot(s.n, bs.wo) < 0.0f ? 3 : (bs.delta ? 2 : 1);
-                            if (bs.has_next_abs) sigma_abs = bs.next_abs;
+                            if (bs.has_next_abs) {
+                                sigma_abs = bs.next_abs;
+                                abs_set = true;
+                            }
                             if (bs.transition == 1) {
                                 float v = (mflags & kMatDielectricIor) ? g.m3.w
                                                                        : (ior_n > 0 ? P.ior[(int64_t)(ior_n - 1) * P.n + p] : 1.0f);
@@ -1779,7 +1794,7 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
                             av0 = make_float4(aw.x, aw.y, aw.z, av0.w);
                         } else aov_commit(f3of(av1), s.n);
                     }
-                    f3 nt = thr_s * att;
+                    f3 nt = thr_at() * att;
                     bool killed = false;
                     if (used >= R.rr_min_b) {
                         float sp = fmax3(nt);
@@ -1849,7 +1864,8 @@ __global__ void __launch_bounds__(kBlock, WPE) k_shade(JOBS jobs, PathBufs P, in
             P.d[p] = make_float4(next_d.x, next_d.y, next_d.z, __int_as_float(flags));
             P.thr[p] = make_float4(thr_next.x, thr_next.y, thr_next.z, __int_as_float(depth - 1));
             P.wt[p] = make_float4(wt_next.x, wt_next.y, wt_next.z, __uint_as_float(rng.dim));
-            P.ab[p] = make_float4(sigma_abs.x, sigma_abs.y, sigma_abs.z, __int_as_float(ior_n));
+            const f3 sab = abs_set ? sigma_abs : f3of(P.ab[p]);
+            P.ab[p] = make_float4(sab.x, sab.y, sab.z, __int_as_float(ior_n));
         } else {
             P.d[p] = make_float4(d.x, d.y, d.z, __int_as_float(flags));
         }
