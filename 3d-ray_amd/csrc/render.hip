@@ -3732,13 +3732,13 @@ static int shadow_slots(const rt_settings* s, const HostScene& h) {
 }
 
 // Renderer constructor arguments (Renderer.cs:300-378) as device parameters.
-// Default path pool: 16 M slots (~9 GB of path state with 5 shadow records
-// per path).  Larger pools mean fewer bounce iterations per frame (each
-// with fixed scan/compaction/launch costs) and shorter launch tails; the
-// 1080p x 64 spp frame of the 1.31 M-triangle scene renders at 287 / 329 /
-// 344 / 349 Msamples/s with 2 / 8 / 16 / 32 M slots, and slower again at
-// 64 M, where the end-of-window drain dominates.
-constexpr int64_t kDefaultPool = 1ll << 24;
+// Default path pool: 32 M slots (~19 GB of path state with 5 shadow records
+// per path, 6.5 % of HBM).  Larger pools mean fewer bounce iterations per
+// frame (each with fixed scan/compaction/launch costs) and more rays per
+// launch over the same tree; the 1080p x 64 spp frame of the 1.31 M-triangle
+// scene renders at 397 / 397 / 412 / 381 / 388 Msamples/s with 16 / 28 / 32
+// / 36 / 40 M slots (the 32 M figure repeated on two boxes within 0.4 %).
+constexpr int64_t kDefaultPool = 1ll << 25;
 
 // Samples per window for a pool of `pool` slots.  A window's per-sample
 // results stay resident until it completes (16 B per sample, 48 B with

@@ -454,7 +454,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--subdiv", type=int, default=8)
-    ap.add_argument("--pool", type=int, default=0, help="path slots in flight (0 = library default, 2^24)")
+    ap.add_argument("--pool", type=int, default=0, help="path slots in flight (0 = library default, 2^25)")
     ap.add_argument("--workload", choices=("icosphere", "caustics", "batch", "adjoint", "film", "denoise"),
                     default="icosphere",
                     help="icosphere: the 1.31M-triangle Disney scene (headline); caustics: glass spheres, "

@@ -339,7 +339,7 @@ typedef struct rt_settings {
     int32_t caustics;               /* --caustics: photon-map caustics (Renderer.cs:380-406) */
     int32_t caustic_photons;        /* --caustic-photons: emitted photon budget */
     float caustic_gather_radius;    /* gather-radius cap; 0 = 6 % of the scene radius */
-    int32_t paths_in_flight;        /* path slots in flight; 0 = library default (2^24) */
+    int32_t paths_in_flight;        /* path slots in flight; 0 = library default (2^25) */
     int32_t count_traversal;        /* 1: count BVH node/primitive tests (slower
                                        instrumented kernels; fills rt_render_stats) */
     int32_t texture_filtering;      /* --texture-filtering: 0 auto (= on), 1 on, 2 off:
