@@ -3739,6 +3739,11 @@ static int shadow_slots(const rt_settings* s, const HostScene& h) {
 // scene renders at 397 / 397 / 412 / 381 / 388 Msamples/s with 16 / 28 / 32
 // / 36 / 40 M slots (the 32 M figure repeated on two boxes within 0.4 %).
 constexpr int64_t kDefaultPool = 1ll << 25;
+// rt_render_batch's default pool: 16 M slots.  Jobs of at least two pools of
+// samples run alone; with the 32 M pool more of a batch's jobs fall below
+// that and share windows, where their waves mix scenes (16-scene check:
+// batched/plain 0.978 at 16 M, 0.909 at 32 M).
+constexpr int64_t kDefaultBatchPool = 1ll << 24;
 
 // Samples per window for a pool of `pool` slots.  A window's per-sample
 // results stay resident until it completes (16 B per sample, 48 B with
@@ -4052,7 +4057,7 @@ int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32
     // samples runs alone (single-job kernels, its windows like rt_render's);
     // smaller jobs are packed job-major into shared windows (batch kernels),
     // cut at pass boundaries, so they fill the pool together.
-    const int64_t N0 = pool_paths > 0 ? pool_paths : kDefaultPool;
+    const int64_t N0 = pool_paths > 0 ? pool_paths : kDefaultBatchPool;
     const int64_t cap = window_cap(N0);
     struct Segment {
         int job;   // >= 0: that job alone; -1: packed jobs

@@ -498,7 +498,7 @@ typedef struct rt_batch_job {
  * so they fill the GPU together; a larger job runs alone in the pool with
  * the single-scene kernels, in job order, without a host round trip.  Each job's film is identical to rt_render of that job (same
  * per-sample values, same summation order).  pool_paths = path slots of the
- * pool (0 = library default).  total (optional) receives summed counts and
+ * pool (0 = the batch default, 2^24).  total (optional) receives summed counts and
  * the batch's timings. */
 int rt_render_batch(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, int32_t pool_paths,
                     rt_render_stats* total);
