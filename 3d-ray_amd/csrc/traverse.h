@@ -504,8 +504,9 @@ __device__ __forceinline__ bool slab_any(f3 mn_, f3 mx_, f3 o, f3 inv, float tmi
 // t in [tmin, tmax]?  The answer equals the closest-hit traversal's
 // existence of a hit (every primitive whose boxes the segment enters is
 // tested; the visit order does not matter for a yes/no answer), with a
-// quarter of the dependent node fetches.  Hit children: the first is
-// entered, the others pushed.
+// quarter of the dependent node fetches.  Hit children are visited nearest
+// first (a 5-exchange sorting network on their entry distances), which
+// finds occluders near the shading point sooner.
 template <int SD, int STRIDE, bool COUNT = false>
 __device__ __forceinline__ bool trace_any4(const GNode4* __restrict__ nodes4, const GEntry* __restrict__ entries,
                                            const float4* __restrict__ aux, uint32_t top_start, uint32_t top_count,
@@ -566,23 +567,40 @@ __device__ __forceinline__ bool trace_any4(const GNode4* __restrict__ nodes4, co
                 const float by1[4] = {nd.mxy.x, nd.mxy.y, nd.mxy.z, nd.mxy.w};
                 const float bz1[4] = {nd.mxz.x, nd.mxz.y, nd.mxz.z, nd.mxz.w};
                 const uint32_t cref[4] = {(uint32_t)nd.ch.x, (uint32_t)nd.ch.y, (uint32_t)nd.ch.z, (uint32_t)nd.ch.w};
-                bool have = false;
-                uint32_t next = 0;
+                // hit children nearest first: entered in order of entry
+                // distance (a miss sorts last), the others pushed farthest first
+                float key[4];
+                uint32_t ref[4];
+                int nh = 0;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    if (!slab_any(mk(bx0[k], by0[k], bz0[k]), mk(bx1[k], by1[k], bz1[k]), o, inv, tmin, closest))
-                        continue;
-                    if (!have) {
-                        next = cref[k];
-                        have = true;
-                    } else {
-                        if (sp >= SD) { *overflow = 1u; return false; }
-                        stk[sp * STRIDE] = make_uint2(cref[k], 0u);
-                        sp++;
-                    }
+                    float te;
+                    const bool hk = slab(mk(bx0[k], by0[k], bz0[k]), mk(bx1[k], by1[k], bz1[k]), o, inv, tmin, closest, te);
+                    key[k] = hk ? te : __int_as_float(0x7f800000);
+                    ref[k] = cref[k];
+                    nh += hk ? 1 : 0;
                 }
-                if (have) {
-                    cur = next;
+                auto cswap = [&](int a, int b) {
+                    const bool sw = key[b] < key[a];
+                    const float ka = key[a], kb = key[b];
+                    const uint32_t ra = ref[a], rb = ref[b];
+                    key[a] = sw ? kb : ka;
+                    key[b] = sw ? ka : kb;
+                    ref[a] = sw ? rb : ra;
+                    ref[b] = sw ? ra : rb;
+                };
+                cswap(0, 1);
+                cswap(2, 3);
+                cswap(0, 2);
+                cswap(1, 3);
+                cswap(1, 2);
+                if (nh > 0) {
+                    // the pushes overflow exactly when the one-by-one pushes would
+                    if (sp + nh - 1 > SD) { *overflow = 1u; return false; }
+                    if (nh > 3) { stk[sp * STRIDE] = make_uint2(ref[3], 0u); sp++; }
+                    if (nh > 2) { stk[sp * STRIDE] = make_uint2(ref[2], 0u); sp++; }
+                    if (nh > 1) { stk[sp * STRIDE] = make_uint2(ref[1], 0u); sp++; }
+                    cur = ref[0];
                     continue;
                 }
             }
