@@ -22,6 +22,7 @@
 // (the reference's pathThroughput, which excludes Beer-Lambert) and a
 // separate radiance weight (which includes it).
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -3084,12 +3085,29 @@ static int photon_pass(rt_ctx* ctx, const rt_scene* sc, const DevScene& S, int b
     return RT_OK;
 }
 
+static bool chunked(const rt_settings* s) { return s->sample_offset != 0 || s->sample_total != 0; }
+
+// q with q * q == n, or 0 when n is not a positive square.
+static int exact_isqrt(int n) {
+    if (n <= 0) return 0;
+    int q = (int)std::lround(std::sqrt((double)n));
+    return (int64_t)q * q == n ? q : 0;
+}
+
 // Camera samples per pixel the render takes: spp (Sobol) or the
-// ceil(sqrt(spp))^2 strata of the PRNG sampler (Renderer.cs:556-558).
+// ceil(sqrt(spp))^2 strata of the PRNG sampler (Renderer.cs:556-558).  A
+// chunk of a PRNG render takes exactly its spp samples of the whole
+// render's sample_total = q^2 strata.
 static int render_spp(const rt_settings* s) {
-    if (s->sampler != RT_SAMPLER_PRNG) return s->samples_per_pixel;
+    if (s->sampler != RT_SAMPLER_PRNG || chunked(s)) return s->samples_per_pixel;
     const int q = (int)ceilf(sqrtf((float)s->samples_per_pixel));
     return q * q;
+}
+
+// Strata per axis of the PRNG sampler's grid.
+static int strata_per_axis(const rt_settings* s) {
+    if (s->sampler == RT_SAMPLER_PRNG && chunked(s)) return exact_isqrt(s->sample_total);
+    return (int)ceilf(sqrtf((float)s->samples_per_pixel));
 }
 
 static int check_settings(rt_ctx* ctx, const rt_settings* s) {
@@ -3099,8 +3117,9 @@ static int check_settings(rt_ctx* ctx, const rt_settings* s) {
     if ((int64_t)s->width * s->height > (1ll << 30)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "image too large");
     if (s->sampler != RT_SAMPLER_SOBOL && s->sampler != RT_SAMPLER_PRNG)
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bad sampler");
-    if (s->sampler == RT_SAMPLER_PRNG && (s->sample_offset != 0 || s->sample_total != 0))
-        return fail(ctx, RT_ERR_UNSUPPORTED, "the PRNG sampler renders whole strata grids (no sample_offset)");
+    if (s->sampler == RT_SAMPLER_PRNG && chunked(s) && exact_isqrt(s->sample_total) == 0)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT,
+                    "a chunk of a PRNG render needs sample_total = the whole render's q x q strata");
     if (s->caustics && (s->caustic_photons < 0 || s->caustic_gather_radius < 0.0f))
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "negative photon budget or gather radius");
     if (s->light_sampling < RT_LIGHTS_ALL || s->light_sampling > RT_LIGHTS_UNIFORM)
@@ -3757,7 +3776,7 @@ static RParams make_params(const rt_settings* s, const HostScene& h, int maxs) {
     R.H = s->height;
     R.spp = render_spp(s);
     R.prng = s->sampler == RT_SAMPLER_PRNG ? 1 : 0;
-    R.sq = (int)ceilf(sqrtf((float)s->samples_per_pixel));
+    R.sq = strata_per_axis(s);
     R.inv_sq = 1.0f / (float)R.sq;
     R.max_depth = s->max_depth;
     R.npix = s->width * s->height;

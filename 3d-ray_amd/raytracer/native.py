@@ -260,8 +260,11 @@ _SAMPLERS = {"sobol": abi.RT_SAMPLER_SOBOL, "prng": abi.RT_SAMPLER_PRNG}
 
 def render_spp(settings):
     """Camera samples per pixel a render takes: spp with the Sobol sampler,
-    ceil(sqrt(spp))^2 strata with the PRNG sampler (Renderer.cs:556-558)."""
-    if settings.sampler == abi.RT_SAMPLER_PRNG:
+    ceil(sqrt(spp))^2 strata with the PRNG sampler (Renderer.cs:556-558),
+    except in a chunk (sample_offset / sample_total set), which takes exactly
+    its spp samples of the whole render's grid."""
+    chunk = settings.sample_offset != 0 or settings.sample_total != 0
+    if settings.sampler == abi.RT_SAMPLER_PRNG and not chunk:
         q = int(np.ceil(np.sqrt(np.float32(settings.samples_per_pixel))))
         return q * q
     return settings.samples_per_pixel

@@ -324,7 +324,10 @@ typedef struct rt_settings {
     int32_t max_depth;
     int32_t sampler;                /* rt_sampler_kind; PRNG: ceil(sqrt(spp))^2 stratified
                                        samples with hashed draws (Renderer.cs:556-558,
-                                       677-682; not with sample_offset / sample_total) */
+                                       677-682); a chunk of a PRNG render takes exactly
+                                       spp samples, strata of the whole render's
+                                       sample_total = q * q grid (sample s in
+                                       stratum (s mod q, s div q)) */
     int32_t mis;                    /* rt_mis_kind */
     int32_t light_sampling;         /* rt_light_sampling */
     int32_t capture_aovs;           /* RenderCaptureOptions.CaptureAovs */
@@ -355,7 +358,8 @@ typedef struct rt_settings {
     int32_t sample_total;           /* the whole render's spp when this call is one
                                        chunk of it (0 = samples_per_pixel): sets the
                                        ray-differential pixel footprint, which shrinks
-                                       by ceil(sqrt(spp)) of the render (Renderer.cs:556) */
+                                       by ceil(sqrt(spp)) of the render (Renderer.cs:556);
+                                       with the PRNG sampler a chunk needs it, a square */
     int32_t reserved[2];
 } rt_settings;          /* 96 bytes */
 

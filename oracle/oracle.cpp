@@ -3808,6 +3808,13 @@ int64_t orc_bvh_dump(void* sp, int32_t* out, int64_t cap) {
 
 namespace orc {
 
+// q with q * q == n, or 0 when n is not a positive square.
+static int exact_isqrt(int n) {
+    if (n <= 0) return 0;
+    int q = (int)std::lround(std::sqrt((double)n));
+    return (int64_t)q * q == n ? q : 0;
+}
+
 static int check_settings(const rt_settings* st, bool prng_ok = false) {
     if (!st || st->width <= 0 || st->height <= 0 || st->samples_per_pixel <= 0 || st->max_depth < 0) {
         g_err = "invalid settings";
@@ -3817,9 +3824,10 @@ static int check_settings(const rt_settings* st, bool prng_ok = false) {
         g_err = "sampler not supported here";
         return RT_ERR_UNSUPPORTED;
     }
-    if (st->sampler == RT_SAMPLER_PRNG && (st->sample_offset != 0 || st->sample_total != 0)) {
-        g_err = "the PRNG sampler renders whole strata grids (no sample_offset)";
-        return RT_ERR_UNSUPPORTED;
+    if (st->sampler == RT_SAMPLER_PRNG && (st->sample_offset != 0 || st->sample_total != 0) &&
+        exact_isqrt(st->sample_total) == 0) {
+        g_err = "a chunk of a PRNG render needs sample_total = the whole render's q x q strata";
+        return RT_ERR_INVALID_ARGUMENT;
     }
     if (st->caustics && st->caustic_photons < 0) { g_err = "negative photon budget"; return RT_ERR_INVALID_ARGUMENT; }
     return 0;
@@ -3955,10 +3963,12 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
         caus = build_photon_map(*s, st->caustic_photons, gr * 0.5f, std::max(1, (int)nthreads), pm);
     }
     Renderer R(*s, *st);
-    // PRNG mode: ceil(sqrt(spp))^2 stratified samples (Renderer.cs:556-558)
+    // PRNG mode: ceil(sqrt(spp))^2 stratified samples (Renderer.cs:556-558);
+    // a chunk takes exactly its spp samples of the sample_total = q^2 grid
     const bool prng = st->sampler == RT_SAMPLER_PRNG;
-    const int sq = (int)std::ceil(std::sqrt((float)st->samples_per_pixel));
-    const int W = st->width, H = st->height, spp = prng ? sq * sq : st->samples_per_pixel;
+    const bool chunk = st->sample_offset != 0 || st->sample_total != 0;
+    const int sq = prng && chunk ? exact_isqrt(st->sample_total) : (int)std::ceil(std::sqrt((float)st->samples_per_pixel));
+    const int W = st->width, H = st->height, spp = prng && !chunk ? sq * sq : st->samples_per_pixel;
     const float inv_sq = 1.0f / (float)sq;
     const int64_t np = (int64_t)W * H;
     const int T = 16;
@@ -4000,8 +4010,9 @@ int orc_render(void* sp, const rt_settings* st, const rt_film* film, int32_t nth
                         c.prng = prng;
                         float ju, jv;
                         if (prng) {   // stratum (sx, sy) of the sq x sq grid (Renderer.cs:677-682)
-                            ju = ((float)(smp % sq) + random_float()) * inv_sq;
-                            jv = ((float)(smp / sq) + random_float()) * inv_sq;
+                            const int gs = soff + smp;   // global sample index
+                            ju = ((float)(gs % sq) + random_float()) * inv_sq;
+                            jv = ((float)(gs / sq) + random_float()) * inv_sq;
                         } else {
                             ju = random_float();
                             jv = random_float();

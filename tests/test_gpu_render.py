@@ -169,10 +169,10 @@ def test_invalid_settings_return_errors(native, gpu_ctx):
         r.render(16, 16)
     assert e.value.code == abi.RT_ERR_INVALID_ARGUMENT
     r.settings.sampler = abi.RT_SAMPLER_PRNG
-    r.settings.sample_offset = 1
+    r.settings.sample_offset = 1   # a PRNG chunk without its grid's sample_total
     with pytest.raises(native.RtError) as e:
         r.render(16, 16)
-    assert e.value.code == abi.RT_ERR_UNSUPPORTED
+    assert e.value.code == abi.RT_ERR_INVALID_ARGUMENT
     r.settings.sampler = abi.RT_SAMPLER_SOBOL
     r.settings.sample_offset = 0
     r.settings.caustics = 1

@@ -11,7 +11,9 @@ reference fixes, and what is checked here:
     (s mod q, s div q) of the q x q grid, q = ceil(sqrt(spp)), and the
     render takes q^2 samples (Renderer.cs:556-558, 677-682);
   * the PRNG render estimates the same image as the Sobol render
-    (per-pixel agreement within 3 sigma over independent runs).
+    (per-pixel agreement within 3 sigma over independent runs);
+  * chunks of a PRNG render (sample_offset / sample_total, no reference
+    counterpart) keep the whole render's strata and merge back into it.
 """
 import numpy as np
 import pytest
@@ -53,10 +55,41 @@ def test_prng_render_takes_the_strata_grid(oracle):
     assert (o["sample_count"] == 16).all()          # ceil(sqrt(10))^2
     o = _render(oracle, desc, W, H, 9, abi.RT_SAMPLER_PRNG, 0)
     assert (o["sample_count"] == 9).all()
-    st = abi.settings_defaults(W, H, 9, 5)
-    st.sampler, st.sample_offset = abi.RT_SAMPLER_PRNG, 3
-    with pytest.raises(RuntimeError):
-        oracle.OracleScene(desc).render(st)
+    # a chunk takes exactly its spp samples of the sample_total = q^2 grid;
+    # without a square sample_total it is refused
+    st = abi.settings_defaults(W, H, 6, 5)
+    st.sampler, st.sample_offset, st.sample_total = abi.RT_SAMPLER_PRNG, 3, 16
+    assert (oracle.OracleScene(desc).render(st)["sample_count"] == 6).all()
+    for off, total in ((3, 0), (3, 10)):
+        st.sample_offset, st.sample_total = off, total
+        with pytest.raises(RuntimeError):
+            oracle.OracleScene(desc).render(st)
+
+
+@pytest.mark.parametrize("split", [(7, 9), (5, 6, 5), (1, 15)])
+def test_prng_chunks_merge_to_the_whole_render(oracle, split):
+    # spp 10 renders the 4 x 4 grid; chunks of it, each drawing sample s in
+    # stratum (s mod 4, s div 4) of the whole grid, merge back by
+    # sample-weighted means (a chunk that restarted the strata at its own
+    # first sample would move its camera rays and miss by far more)
+    from test_chunking import halves, merge
+    W = H = 16
+    desc = scenes.cornell(W, H)
+    full = _render(oracle, desc, W, H, 10, abi.RT_SAMPLER_PRNG, 0)
+    parts, offs = [], []
+    off = 0
+    for n in split:
+        st = abi.settings_defaults(W, H, n, 5)
+        st.sampler, st.sample_offset, st.sample_total = abi.RT_SAMPLER_PRNG, off, 16
+        parts.append(oracle.OracleScene(desc).render(st))
+        offs.append(off)
+        off += n
+    tol = dict(rtol=2e-5, atol=1e-6)
+    assert np.allclose(merge(parts, "beauty", list(split)), full["beauty"], **tol)
+    for key, h in (("beauty_a", 0), ("beauty_b", 1)):
+        w = [halves(o, n)[h] for o, n in zip(offs, split)]
+        keep = [i for i in range(len(parts)) if w[i] > 0]
+        assert np.allclose(merge([parts[i] for i in keep], key, [w[i] for i in keep]), full[key], **tol)
 
 
 def test_prng_render_estimates_the_sobol_image(oracle):
