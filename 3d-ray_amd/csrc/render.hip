@@ -883,11 +883,11 @@ __device__ __forceinline__ void sample_of(const RParams& R, int sid, int& i, int
     j = pix / R.W;
 }
 
-// Camera ray (Renderer.cs:630-653, Camera.cs:127-153)
-__device__ __forceinline__ void camera_ray(const DevScene& S, const RParams& R, int i, int j, Rng& rng, f3& o, f3& d) {
-    float ju, jv;
+// The sample's jitter inside its pixel: two Sobol draws, or with the PRNG
+// sampler two draws inside stratum (sx, sy) = (s mod sq, s div sq) of the
+// sq x sq grid (Renderer.cs:677-682).
+__device__ __forceinline__ void pixel_jitter(const RParams& R, Rng& rng, float& ju, float& jv) {
     if (R.prng) {
-        // stratum (sx, sy) = (s mod sq, s div sq) of the sq x sq grid (Renderer.cs:677-682)
         const int s = (int)rng.index;
         ju = ((float)(s % R.sq) + rng.next()) * R.inv_sq;
         jv = ((float)(s / R.sq) + rng.next()) * R.inv_sq;
@@ -895,6 +895,12 @@ __device__ __forceinline__ void camera_ray(const DevScene& S, const RParams& R, 
         ju = rng.next();
         jv = rng.next();
     }
+}
+
+// Camera ray (Renderer.cs:630-653, Camera.cs:127-153)
+__device__ __forceinline__ void camera_ray(const DevScene& S, const RParams& R, int i, int j, Rng& rng, f3& o, f3& d) {
+    float ju, jv;
+    pixel_jitter(R, rng, ju, jv);
     float u = ((float)i + ju) / (float)R.W;
     float v = ((float)(R.H - j - 1) + jv) / (float)R.H;
     f3 target = (S.cam_llc + u * S.cam_h) + v * S.cam_v;
@@ -1138,7 +1144,8 @@ __global__ void __launch_bounds__(kBlock) k_fp(JOBS jobs, PathBufs P) {
             // GetRayWithDifferentials: the primary's jitter (and lens) sample
             const int pix = P.pix[p], i = pix % R.W, j = pix / R.W;
             Rng rng{(uint32_t)P.pass[p], pixel_seed(i, j, R.seed_xor), 0u, R.prng != 0};
-            const float ju = rng.next(), jv = rng.next();
+            float ju, jv;
+            pixel_jitter(R, rng, ju, jv);
             const float su = ((float)i + ju) / (float)R.W;
             const float tv = ((float)(R.H - j - 1) + jv) / (float)R.H;
             const int sq = (int)ceilf(sqrtf((float)R.spp_fp));

@@ -16,21 +16,26 @@ from test_image_texture import HERE, rand_image, textured_quad
 pytestmark = pytest.mark.gpu
 
 
-def _compare(name, desc, gpu_ctx, oracle, W, H, spp, depth, filt=0, alb_tol=2e-4, alb_frac=1.0):
+def _compare(name, desc, gpu_ctx, oracle, W, H, spp, depth, filt=0, alb_tol=2e-4, alb_frac=1.0, sampler="sobol"):
     st = abi.settings_defaults(W, H, spp, depth)
     st.capture_aovs = 1
     st.texture_filtering = filt
+    prng = sampler == "prng"
     sc = native.Scene(gpu_ctx, desc)
     try:
-        p, t, _, _ = sc.trace_primary(st, 0, 2)
-        r = native.Renderer(sc, spp, depth)
+        if not prng:   # the primary-hit hook traces Sobol samples
+            p, t, _, _ = sc.trace_primary(st, 0, 2)
+        r = native.Renderer(sc, spp, depth, sampler=sampler)
         r.settings.texture_filtering = filt
         buf = r.render(W, H, capture_aovs=True).buffers
     finally:
         sc.close()
     osc = oracle.OracleScene(desc)
-    op, ot, _, _, _, _ = osc.trace_primary(st, 0, 2)
-    assert np.array_equal(p, op) and np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+    if not prng:
+        op, ot, _, _, _, _ = osc.trace_primary(st, 0, 2)
+        assert np.array_equal(p, op) and np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+    else:
+        st.sampler = abi.RT_SAMPLER_PRNG
     o = osc.render(st, aovs=True)
     diff = np.abs(buf.albedo_a - o["albedo_a"])
     within = float((diff < alb_tol).mean())
@@ -48,6 +53,13 @@ def _compare(name, desc, gpu_ctx, oracle, W, H, spp, depth, filt=0, alb_tol=2e-4
 def test_textured_quad(gpu_ctx, oracle, filt, spp):
     img = rand_image(256, 256, 6)
     _compare(f"quad spp={spp} filtering={filt}", textured_quad(64, 64, img), gpu_ctx, oracle, 64, 64, spp, 2, filt)
+
+
+def test_textured_quad_prng_sampler(gpu_ctx, oracle):
+    # the footprint's ray differentials start from the stratified jitter of
+    # the camera ray (Renderer.cs:677-697)
+    img = rand_image(256, 256, 6)
+    _compare("quad prng spp=6", textured_quad(64, 64, img), gpu_ctx, oracle, 64, 64, 6, 2, 0, sampler="prng")
 
 
 @pytest.mark.parametrize("filt", [0, 2])
